@@ -1,0 +1,185 @@
+"""Benchmark of the per-step particle pipeline on one MI355X.
+
+Workload: the dam break with mDBC walls (3.22 x 1.8 m tank, 1.2 x 0.6 m column,
+dx = 1 mm, 740,478 particles, fluid positions jittered by +-0.01 dx from a fixed
+seed), one symplectic step = mDBC wall density, EOS, predictor, cell list, EOS,
+pair loop, corrector, removal, cell list, EOS, pair loop (the stage order of
+cases/damBreakLong_mDBC/SPH_simulationSymplectic.h:59-116).
+
+Measured in one run:
+  * mixed path (FP32 pair arithmetic, FP64 integration)  -> `value`
+  * plain FP64 path                                        -> `fp64` block
+  * the mixed path's density error against the FP64 path after the timed steps
+  * the oracle (the C++ restatement of the reference OpenMP path) on the host
+    cores, on a bounded sample of the same workload       -> `cpu_baseline`
+
+`value` = ordered pair interactions (|r_ij| < 2h, i active) per second over the
+timed steps, inputs resident in HBM.  Pair-kernel time comes from HIP events
+around every pair launch on the library's stream.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import tinysph_old_amd as T  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+# algorithmic bytes (per pair: neighbour record; per active particle: own record + outputs)
+B_PAIR = {T.PREC_FP64: 50, T.PREC_MIXED: 26}
+B_PART = {T.PREC_FP64: 74, T.PREC_MIXED: 50}  # mixed keeps FP64 outputs (a 16 B + dRho 8 B)
+SEED = 20241202
+
+
+def run_path(case, precision: int, steps: int, warmup: int):
+    ps = T.system_for_case(case, precision=precision)
+    ps.particle_to_cell()
+    ps.density_to_pressure()
+    ps.compute_forces(T.forces_variant(case.wall))
+    # warmup (graph capture + replay)
+    ps.step(case.wall, case.dt, warmup)
+    ps.synchronize()
+    # timed region A: graph replay, whole-step throughput
+    ps.reset_counters()
+    ps.synchronize()
+    t0 = time.perf_counter()
+    ps.step(case.wall, case.dt, steps)
+    ps.synchronize()
+    t_graph = time.perf_counter() - t0
+    cnt = ps.counters()
+    # timed region B: same steps launched eagerly with HIP events on every pair kernel
+    ps.pair_timing(2 * steps + 8)
+    ps.synchronize()
+    t0 = time.perf_counter()
+    ps.step(case.wall, case.dt, steps)
+    ps.synchronize()
+    t_eager = time.perf_counter() - t0
+    pair_ms, launches = ps.pair_timing_read()
+    ps.pair_timing(0)
+    cnt_b = ps.counters()
+    state = ps.download(("rho", "r", "v", "p"))
+    ps.close()
+    pairs_a = cnt["pairs"]
+    pairs_b = cnt_b["pairs"] - cnt["pairs"]
+    act_b = cnt_b["active"] - cnt["active"]
+    return dict(t_graph=t_graph, t_eager=t_eager, pairs=pairs_a, cand=cnt["candidates"],
+                passes=cnt["pair_passes"], pair_ms=pair_ms, launches=launches,
+                pairs_b=pairs_b, active_b=act_b, state=state, counters=cnt_b)
+
+
+def cpu_baseline(case, budget_s: float = 20.0):
+    """Oracle (C++ restatement of the reference path, OpenMP) on the host cores."""
+    from oracle import pyoracle as po
+    from tests.helpers import oracle_init, oracle_step
+
+    threads = min(16, os.cpu_count() or 1)
+    O = po.oracle_case(case.oracle_params(), case.arrays, omp_threads=threads)
+    oracle_init(O)
+    O.call("density_to_pressure")
+    O.call("compute_forces", 0)
+    # pair count of one pass is taken from the GPU counters of the same case
+    t0 = time.perf_counter()
+    n = 0
+    while True:
+        oracle_step(O, case.wall, case.dt)
+        n += 1
+        if time.perf_counter() - t0 > budget_s or n >= 50:
+            break
+    return n, time.perf_counter() - t0, threads
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--dx", type=float, default=0.001)
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: every GPU box of this build has one MI355X, and the "
+              "pipeline is single-GPU (multi-GPU sharding is design-only)", file=sys.stderr)
+        return 2
+
+    case = T.build_case(T.CASE_DAM_BREAK_MDBC, dx=args.dx, jitter=0.01, seed=SEED)
+    print(f"[bench] case dam_break_mdbc np={case.np} fluid={case.n_fluid} wall={case.n_wall} "
+          f"dt={case.dt:.4g}", file=sys.stderr, flush=True)
+    fp64 = run_path(case, T.PREC_FP64, args.steps, args.warmup)
+    print(f"[bench] fp64 done {fp64['t_graph']:.3f}s", file=sys.stderr, flush=True)
+    mixed = run_path(case, T.PREC_MIXED, args.steps, args.warmup)
+    print(f"[bench] mixed done {mixed['t_graph']:.3f}s", file=sys.stderr, flush=True)
+
+    def rate(r):
+        return r["pairs"] / r["t_graph"]
+
+    k = args.steps
+    # density error of the mixed path vs the FP64 path after warmup + 2*steps steps
+    rho_m, rho_d = mixed["state"]["rho"], fp64["state"]["rho"]
+    rho_err = float(np.linalg.norm(rho_m - rho_d) / np.linalg.norm(rho_d))
+    r_err = float(np.linalg.norm(mixed["state"]["r"] - fp64["state"]["r"]) /
+                  np.linalg.norm(fp64["state"]["r"]))
+
+    def roof(r, prec):
+        avg_ms = r["pair_ms"] / max(r["launches"], 1)
+        pairs_per_launch = r["pairs_b"] / max(r["launches"], 1)
+        act_per_launch = r["active_b"] / max(r["launches"], 1)
+        bytes_per_launch = pairs_per_launch * B_PAIR[prec] + act_per_launch * B_PART[prec]
+        achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+        return avg_ms, achieved, bytes_per_launch, pairs_per_launch
+
+    m_avg, m_ach, m_bytes, m_ppl = roof(mixed, T.PREC_MIXED)
+    d_avg, d_ach, d_bytes, d_ppl = roof(fp64, T.PREC_FP64)
+
+    line = {
+        "metric": "pair-interactions/s & steps/s, mixed vs FP64 GPU, at stated error",
+        "value": rate(mixed),
+        "unit": "pair-interactions/s",
+        "n_gpus": 1,
+        "steps": k,
+        "warmup": args.warmup,
+        "ms_per_step": mixed["t_graph"] / k * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 pairs / f64 integration",
+        "data": "synthetic (dam-break lattice, seeded jitter)",
+        "config": {"workload": "dam break mDBC 3.22x1.8 m tank, 1.2x0.6 m column, dx=1 mm",
+                   "particles": case.np, "fluid": case.n_fluid, "wall": case.n_wall,
+                   "dt": case.dt, "pair_passes_per_step": 2, "path": "mixed, hipGraph step"},
+        "steps_per_s": k / mixed["t_graph"],
+        "pairs_per_step": mixed["pairs"] / k,
+        "fp64": {"pair_interactions_per_s": rate(fp64), "steps_per_s": k / fp64["t_graph"],
+                 "ms_per_step": fp64["t_graph"] / k * 1e3, "pair_kernel_avg_ms": d_avg,
+                 "roofline_frac": d_ach / HBM_PEAK_GBS},
+        "mixed_over_fp64": rate(mixed) / rate(fp64),
+        "error_vs_fp64": {"rho_rel_l2": rho_err, "r_rel_l2": r_err,
+                          "after_steps": args.warmup + 2 * k},
+        "roofline": {"bound": "hbm", "achieved": m_ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": m_ach / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_forces32<0>", "avg_launch_ms": m_avg,
+                     "algorithmic_bytes_per_launch": m_bytes, "pairs_per_launch": m_ppl},
+    }
+    if not args.no_cpu:
+        n, t, threads = cpu_baseline(case, args.cpu_budget)
+        pairs_per_step = fp64["pairs"] / k
+        line["cpu_baseline"] = {"value": n * pairs_per_step / t, "unit": "pair-interactions/s",
+                                "cores": threads, "kind": "port",
+                                "sample": f"{n} full symplectic steps of the same 740k-particle "
+                                          f"dam break (oracle, OpenMP, {threads} threads), "
+                                          f"{t:.1f} s",
+                                "steps_per_s": n / t}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,194 @@
+/*
+ * tinysph.h -- C ABI of libtinysph_hip.so, the MI355X (gfx950) implementation of
+ * the tinySPH per-step particle pipeline: cell-list build, pair loops and time
+ * integration, run as hand-written HIP kernels on device-resident SoA data.
+ *
+ * Each entry point replaces one reference stage function that a case driver's
+ * SPH_simulation::RUN() calls on a Particle_system (citations are to the
+ * reference tree tomashalada/tinySPH_old).  Conventions:
+ *   - every function returns 0 on success or a negative TSPH_E* code; the text
+ *     of the last error is available from tsph_last_error();
+ *   - one context owns one particle system on one GPU and one HIP stream; calls
+ *     on a context are stream-ordered and not thread safe;
+ *   - host arrays are in reference particle order (the order of Add_particle
+ *     calls, src/SPH_particle_system.h:25-50), vectors interleaved as {x, y}
+ *     like the reference's tdouble2 (src/SPH_defs.h:72-93).
+ *   - stage calls are asynchronous; tsph_download / tsph_get_* synchronise.
+ */
+#ifndef TINYSPH_H
+#define TINYSPH_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TSPH_ABI_VERSION 1
+
+/* error codes */
+#define TSPH_OK 0
+#define TSPH_EINVAL -1   /* bad argument */
+#define TSPH_EHIP -2     /* HIP runtime error */
+#define TSPH_ESTATE -3   /* call out of order (e.g. forces before a cell list) */
+#define TSPH_ECAPACITY -4/* particle count beyond the context capacity */
+#define TSPH_EGRID -5    /* a particle lies outside the cell grid */
+
+/* particle types, src/SPH_defs.h:44-65 */
+#define TSPH_FLUID 0
+#define TSPH_WALL 1
+#define TSPH_INLET 6
+#define TSPH_OUTLET 11
+
+/* arithmetic of the pair loops */
+#define TSPH_PREC_FP64 0  /* IEEE double with the reference's float-rounded kernel */
+#define TSPH_PREC_MIXED 1 /* FP32 pair arithmetic on cell-relative FP32 positions,
+                             FP64 integration of position and density */
+
+/* pair-loop variants (which reference Compute_Forces is replaced) */
+#define TSPH_FORCES_WCSPH 0 /* src/SPH_interaction.h:8-149 (fluid only; mDBC cases) */
+#define TSPH_FORCES_DBC 1   /* src/SPH_interactionDBC.h:8-155 */
+#define TSPH_FORCES_BT 2    /* src/SPH_interactionBT.h:8-192 */
+
+/* wall treatment used by tsph_step_symplectic */
+#define TSPH_WALL_DBC 0
+#define TSPH_WALL_MDBC 1
+#define TSPH_WALL_BT 2
+
+/* SPH_constants + Simulation_data (src/SPH_data.h:30-46, 111-125).  The domain
+ * is the padded one the drivers' PREP_SIMULATION_DATA produces
+ * (e.g. cases/damBreakLong_mDBC/SPH_simulationSymplectic.h:206-221).  Mass and
+ * gravity are derived as in the Particle_system constructor
+ * (src/SPH_particle_system.h:191-211). */
+typedef struct tsph_params {
+  double h, kap, cs, rho0, avisc, delta, dp;
+  double x_0, x_m, y_0, y_m;
+  int precision;   /* TSPH_PREC_* */
+  int capacity;    /* maximum particle count held on the device */
+} tsph_params;
+
+/* Host view of a particle system in reference order.  Pointers the caller does
+ * not need may be NULL (on upload: fields start as Add_particle sets them). */
+typedef struct tsph_particles {
+  int np;
+  int32_t *type;        /* part_type */
+  int32_t *index;       /* data.index (download only) */
+  double *p, *rho, *drho;
+  double *r, *v, *a;    /* 2*np each, {x,y} interleaved */
+  double *r_o, *v_o, *rho_o;
+  double *gnr;          /* mDBC ghost node per particle (2*np) */
+  double *n;            /* BT wall normal per particle (2*np) */
+} tsph_particles;
+
+/* counters accumulated since the last tsph_reset_counters */
+typedef struct tsph_counters {
+  uint64_t pairs;        /* ordered (i,j), i active, |r_i-r_j| < 2h, over all pair passes */
+  uint64_t candidates;   /* ordered (i,j), j != i in i's 9-cell stencil */
+  uint64_t active;       /* active particles summed over pair passes */
+  uint64_t pair_passes;  /* number of Compute_Forces calls */
+  uint64_t density_out_of_range; /* rho > 1500 or < 0 seen by the EOS (src/SPH_pressure.h:76-80) */
+  uint64_t outside_grid; /* particles that fell outside the cell grid */
+  uint64_t removed;      /* particles removed out of the domain (np_out) */
+} tsph_counters;
+
+typedef struct tsph_ctx tsph_ctx;
+
+const char *tsph_last_error(void);
+int tsph_abi_version(void);
+
+/* Particle_system constructor + Divide_To_Cells, src/SPH_particle_system.h:191-211,
+ * src/SPH_find_pairs.h:5-42 */
+int tsph_create(const tsph_params *prm, tsph_ctx **out);
+int tsph_destroy(tsph_ctx *ctx);
+
+/* grid size chosen by Divide_To_Cells (src/SPH_find_pairs.h:19-20) */
+int tsph_grid(tsph_ctx *ctx, int *ncx, int *ncy);
+int tsph_mass(tsph_ctx *ctx, double *m);
+
+/* Replaces the host std::vector fields of Particle_system (a whole upload
+ * stands for the sequence of Add_particle calls that built them). */
+int tsph_upload(tsph_ctx *ctx, const tsph_particles *host);
+/* Copies the current state back in reference order; host->np is set. */
+int tsph_download(tsph_ctx *ctx, tsph_particles *host);
+int tsph_get_np(tsph_ctx *ctx, int *np);
+
+/* Clear_cells, src/SPH_find_pairs.h:74-92 */
+int tsph_clear_cells(tsph_ctx *ctx);
+/* Particle_To_Cell, src/SPH_find_pairs.h:45-72: cell hash, counting sort by
+ * cell with reference in-cell order, cell start offsets, SoA reorder. */
+int tsph_particle_to_cell(tsph_ctx *ctx);
+/* Cell lists as CSR in reference particle indices: start[ncx*ncy+1], items[np]. */
+int tsph_get_cell_lists(tsph_ctx *ctx, int32_t *start, int32_t *items);
+
+/* Density_to_pressure, src/SPH_pressure.h:65-89 */
+int tsph_density_to_pressure(tsph_ctx *ctx);
+/* Compute_Forces, variant TSPH_FORCES_* */
+int tsph_compute_forces(tsph_ctx *ctx, int variant);
+/* mDBC_compute_density_mdbcGeo, src/SPH_mDBC.h:9-50 */
+int tsph_mdbc_density(tsph_ctx *ctx);
+/* BT_compute_density_BTGeo, src/SPH_BT.h:9-50 */
+int tsph_bt_density(tsph_ctx *ctx);
+/* Integrate_SymplecticPredictor / Corrector, src/SPH_integration.h:266-309 */
+int tsph_symplectic_predictor(tsph_ctx *ctx, double dt);
+int tsph_symplectic_corrector(tsph_ctx *ctx, double dt);
+/* RemoveParticlesOutOfDomain, src/SPH_btdebug.h:4-31 (takes effect at the
+ * next tsph_particle_to_cell, which the drivers always call next) */
+int tsph_remove_out_of_domain(tsph_ctx *ctx);
+
+/* One or more whole symplectic steps in the drivers' stage order
+ * (cases/stillWater_DBC/SPH_simulationSymplectic.h:141-198,
+ *  cases/damBreakLong_mDBC/SPH_simulationSymplectic.h:59-116,
+ *  cases/damBreakLong_BT/SPH_simulationSymplectic.h:66-122), replayed from a
+ * captured HIP graph.  wall = TSPH_WALL_*. */
+int tsph_step_symplectic(tsph_ctx *ctx, int wall, double dt, int nsteps);
+
+/* Water_Elevation / Water_Elevation_inverse, src/SPH_water_level.h:87-150
+ * (NaN where the reference leaves the level unset) */
+int tsph_water_elevation(tsph_ctx *ctx, double x, double y0, int inverse, double *level);
+/* Pressure_in_point -> Kernel_velocity_approximation_TEST, src/SPH_measure_pressure.h:6-15,
+ * src/testing.h:6-148: out4 = {rho, p, vx, vy} */
+int tsph_interpolate_point(tsph_ctx *ctx, double x, double y, double *out4);
+
+int tsph_get_counters(tsph_ctx *ctx, tsph_counters *out);
+int tsph_reset_counters(tsph_ctx *ctx);
+/* pair-kernel launch shape: threads per block (64..256, multiple of 64) */
+int tsph_set_block_size(tsph_ctx *ctx, int threads);
+/* wait for all work queued on the context */
+int tsph_synchronize(tsph_ctx *ctx);
+
+/* Pair-kernel timing with HIP events on the context stream.  While enabled
+ * (max_launches > 0) every pair-loop launch, up to max_launches, is bracketed
+ * by its own event pair and tsph_step_symplectic launches eagerly instead of
+ * replaying its graph.  tsph_pair_timing_read synchronises, returns the summed
+ * kernel time and the number of timed launches, and restarts the record. */
+int tsph_pair_timing(tsph_ctx *ctx, int max_launches);
+int tsph_pair_timing_read(tsph_ctx *ctx, double *total_ms, int *launches);
+
+/* ---- case setup (host only; no GPU needed) ----
+ * Presets built with the reference's geometry routines (src/SPH_create.h,
+ * src/SPH_create_bt.h) in the order of the reference case geometry files. */
+#define TSPH_CASE_STILL_WATER_DBC 1 /* cases/stillWater_DBC/geometry.h, 1.0 m tank, 0.5 m water */
+#define TSPH_CASE_DAM_BREAK_MDBC 2  /* cases/damBreakLong_mDBC/geometry.h, 3.22x1.8 m tank,
+                                       1.2x0.6 m column, fluid positions jittered */
+#define TSPH_CASE_DAM_BREAK_BT 3    /* cases/damBreakLong_BT/geometry.h, 1.61x1.0 m tank,
+                                       0.6x0.3 m column */
+
+typedef struct tsph_case {
+  tsph_params prm;           /* constants and padded domain (capacity = np) */
+  double dt;                 /* time step of the preset */
+  int wall;                  /* TSPH_WALL_* */
+  int steps;                 /* step count of the preset */
+  int n_fluid, n_wall;
+  tsph_particles particles;  /* arrays owned by the case */
+} tsph_case;
+
+/* dx <= 0 selects the preset's spacing; jitter is the uniform position
+ * perturbation of fluid particles in units of dx (negative: preset default);
+ * seed drives the jitter (SplitMix64). */
+int tsph_case_build(int preset, double dx, double jitter, uint64_t seed, tsph_case **out);
+int tsph_case_free(tsph_case *c);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TINYSPH_H */

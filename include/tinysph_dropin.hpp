@@ -1,0 +1,483 @@
+// tinysph_dropin.hpp -- host-side drop-in for the reference's stage library.
+//
+// A case driver written against the reference (struct SPH_simulation with
+// INIT/RUN calling free functions on a Particle_system, e.g.
+// cases/damBreakLong_mDBC/SPH_simulationSymplectic.h:24-203) compiles against
+// this header instead of src/*.h and runs its per-step pipeline on the GPU
+// through libtinysph_hip.so:
+//   * Particle_system / SPH_data / SPH_constants / SPH_special / Simulation_data
+//     keep the reference's field names (src/SPH_data.h:5-125,
+//     src/SPH_particle_system.h:7-213) as host mirrors;
+//   * the stage functions keep the reference signatures and call the C ABI;
+//   * the geometry routines (src/SPH_create.h, src/SPH_create_bt.h) build the
+//     host mirror exactly as the reference does.
+// The pair-loop variant is chosen at compile time like the reference chooses it
+// by #include: define TSPH_DROPIN_FORCES to TSPH_FORCES_WCSPH (default,
+// src/SPH_interaction.h), TSPH_FORCES_DBC (src/SPH_interactionDBC.h) or
+// TSPH_FORCES_BT (src/SPH_interactionBT.h) before including this header.
+//
+// Data stays on the device between stages.  The host vectors are refreshed by
+// tsph_sync_host(particles); host edits after the first stage call must be
+// followed by tsph_mark_host_dirty(particles).  Errors throw tsph_error.
+#pragma once
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "tinysph.h"
+
+#ifndef TSPH_DROPIN_FORCES
+#define TSPH_DROPIN_FORCES TSPH_FORCES_WCSPH
+#endif
+
+// ---- src/SPH_defs.h:31-93 ----
+#define SQ(x) ((x) * (x))
+#define NORM(x, y) sqrt(SQ(x) + SQ(y))
+#define LEN(x0, y0, x1, y1) sqrt(SQ(x1 - x0) + SQ(y1 - y0))
+#define POS(x, y, X, Y) (y + (x) * (Y))
+
+enum {
+  fluid = 0, wall = 1, virt = 2, mov_a = 3, mov_b = 4, mov_c = 5, inlet = 6, inlet1 = 7,
+  inlet2 = 8, inlet3 = 9, inlet5 = 10, outlet = 11, outlet1 = 12, outlet2 = 13, outlet3 = 14,
+  outlet4 = 15, outlet5 = 16, outletf = 17
+};
+
+typedef struct { double x, y; } tdouble2;
+inline tdouble2 TDouble2(double x, double y) { tdouble2 p = {x, y}; return p; }
+inline tdouble2 operator+(const tdouble2 &a, const tdouble2 &b) { return TDouble2(a.x + b.x, a.y + b.y); }
+inline tdouble2 operator-(const tdouble2 &a, const tdouble2 &b) { return TDouble2(a.x - b.x, a.y - b.y); }
+inline tdouble2 operator*(const tdouble2 &a, const double &b) { return TDouble2(a.x * b, a.y * b); }
+inline tdouble2 operator/(const tdouble2 &a, const double &b) { return TDouble2(a.x / b, a.y / b); }
+using realvec = tdouble2;
+using real = double;
+using idx = int;
+
+struct tsph_error : std::runtime_error {
+  explicit tsph_error(const std::string &m) : std::runtime_error(m) {}
+};
+
+inline void tsph_check(int rc, const char *what) {
+  if (rc != TSPH_OK) throw tsph_error(std::string(what) + ": " + tsph_last_error());
+}
+
+// ---- src/SPH_data.h:5-125 ----
+struct SPH_data {
+  std::vector<idx> index;
+  std::vector<unsigned short int> part_type;
+  std::vector<real> p, rho, drho;
+  std::vector<realvec> r, v, a;
+  std::vector<realvec> r_o, v_o, v_oo;
+  std::vector<real> rho_o, rho_oo;
+};
+
+struct SPH_constants {
+  real h, kap, dp, cs, rho0;
+  realvec graviy;
+  real avisc, delta, m;
+};
+
+struct SPH_special {
+  std::vector<realvec> gnr;  // mDBC ghost nodes (walls only, in creation order)
+  std::vector<realvec> n;    // BT normals (walls only, in creation order)
+};
+
+struct Interaction_data {
+  unsigned int ncx, ncy;
+};
+
+struct Simulation_data {
+  real x_0, x_m, y_0, y_m;
+  real hh, kh;
+  int nvl;
+  Simulation_data(real _x_0, real _x_m, real _y_0, real _y_m)
+      : x_0(_x_0), x_m(_x_m), y_0(_y_0), y_m(_y_m), hh(0), kh(0), nvl(0) {}
+};
+
+// ---- src/SPH_particle_system.h:7-213 ----
+struct Particle_system {
+  unsigned int np;
+  SPH_data data;
+  SPH_constants data_const;
+  SPH_special special;
+  Interaction_data pairs;
+  unsigned int np_out;
+
+  // device side
+  tsph_ctx *dev = nullptr;
+  int precision = TSPH_PREC_FP64;
+  int device_capacity = 0;  // 0: the particle count at Divide_To_Cells
+  bool host_dirty = true;   // host vectors newer than the device
+  bool dev_newer = false;   // device newer than the host vectors
+
+  void Add_particle(short unsigned int _type, real _p, real _rho, realvec _r, realvec _v, realvec _a) {
+    np++;
+    data.part_type.push_back(_type);
+    data.index.push_back(np - 1);
+    data.p.push_back(_p);
+    data.rho.push_back(_rho);
+    data.drho.push_back(0.);
+    data.r.push_back(_r);
+    data.v.push_back(_v);
+    data.a.push_back(_a);
+    data.r_o.push_back(_r);
+    data.rho_o.push_back(_rho);
+    data.rho_oo.push_back(_rho);
+    data.v_o.push_back(_v);
+    data.v_oo.push_back(_v);
+    host_dirty = true;
+  }
+
+  Particle_system(double _h, double _kap, double _cs, double _rho0, double _visco, double _delta,
+                  double _dp) {
+    np = 0;
+    np_out = 0;
+    pairs.ncx = pairs.ncy = 0;
+    data_const.h = _h;
+    data_const.kap = _kap;
+    data_const.cs = _cs;
+    data_const.rho0 = _rho0;
+    data_const.dp = _dp;
+    data_const.graviy = {0., 9.81};
+    data_const.m = _rho0 / (1 / _dp * 1 / _dp);
+    data_const.avisc = _visco;
+    data_const.delta = _delta;
+  }
+  Particle_system(const Particle_system &) = delete;
+  Particle_system &operator=(const Particle_system &) = delete;
+  ~Particle_system() {
+    if (dev) tsph_destroy(dev);
+  }
+};
+
+// ---------------------------------------------------------------------------
+// host <-> device
+// ---------------------------------------------------------------------------
+inline void tsph_mark_host_dirty(Particle_system &ps) { ps.host_dirty = true; }
+
+// per-particle copy of a wall-only extra (gnr / n): walls come first in every
+// reference case, so entry k of the extra belongs to particle k
+inline std::vector<double> tsph_expand_extra(const Particle_system &ps, const std::vector<realvec> &e) {
+  std::vector<double> out(2 * (size_t)ps.np, 0.0);
+  for (size_t k = 0; k < e.size() && k < ps.np; ++k) {
+    out[2 * k] = e[k].x;
+    out[2 * k + 1] = e[k].y;
+  }
+  return out;
+}
+
+inline void tsph_push(Particle_system &ps) {
+  if (!ps.dev) throw tsph_error("Divide_To_Cells must be called before the first stage");
+  if (!ps.host_dirty) return;
+  const int n = (int)ps.np;
+  std::vector<int32_t> type(n), index(n);
+  for (int i = 0; i < n; ++i) {
+    type[i] = ps.data.part_type[i];
+    index[i] = ps.data.index[i];
+  }
+  auto flat = [n](const std::vector<realvec> &v) {
+    std::vector<double> o(2 * (size_t)n);
+    for (int i = 0; i < n; ++i) { o[2 * i] = v[i].x; o[2 * i + 1] = v[i].y; }
+    return o;
+  };
+  std::vector<double> r = flat(ps.data.r), v = flat(ps.data.v), a = flat(ps.data.a),
+                      ro = flat(ps.data.r_o), vo = flat(ps.data.v_o);
+  std::vector<double> gnr, nn;
+  tsph_particles h{};
+  h.np = n;
+  h.type = type.data();
+  h.index = index.data();
+  h.p = ps.data.p.data();
+  h.rho = ps.data.rho.data();
+  h.drho = ps.data.drho.data();
+  h.r = r.data(); h.v = v.data(); h.a = a.data(); h.r_o = ro.data(); h.v_o = vo.data();
+  h.rho_o = ps.data.rho_o.data();
+  if (!ps.special.gnr.empty()) { gnr = tsph_expand_extra(ps, ps.special.gnr); h.gnr = gnr.data(); }
+  if (!ps.special.n.empty()) { nn = tsph_expand_extra(ps, ps.special.n); h.n = nn.data(); }
+  tsph_check(tsph_upload(ps.dev, &h), "tsph_upload");
+  ps.host_dirty = false;
+}
+
+inline void tsph_sync_host(Particle_system &ps) {
+  if (!ps.dev || !ps.dev_newer) return;
+  int n = 0;
+  tsph_check(tsph_get_np(ps.dev, &n), "tsph_get_np");
+  std::vector<int32_t> type(n), index(n);
+  std::vector<double> p(n), rho(n), drho(n), rho_o(n), r(2 * (size_t)n), v(2 * (size_t)n),
+      a(2 * (size_t)n), ro(2 * (size_t)n), vo(2 * (size_t)n);
+  tsph_particles h{};
+  h.type = type.data(); h.index = index.data(); h.p = p.data(); h.rho = rho.data();
+  h.drho = drho.data(); h.rho_o = rho_o.data(); h.r = r.data(); h.v = v.data(); h.a = a.data();
+  h.r_o = ro.data(); h.v_o = vo.data();
+  tsph_check(tsph_download(ps.dev, &h), "tsph_download");
+  const int m = h.np;
+  auto vec = [m](const std::vector<double> &f) {
+    std::vector<realvec> o(m);
+    for (int i = 0; i < m; ++i) o[i] = {f[2 * i], f[2 * i + 1]};
+    return o;
+  };
+  ps.np_out += ps.np - (unsigned)m;
+  ps.np = (unsigned)m;
+  ps.data.part_type.assign(type.begin(), type.begin() + m);
+  ps.data.index.assign(index.begin(), index.begin() + m);
+  ps.data.p.assign(p.begin(), p.begin() + m);
+  ps.data.rho.assign(rho.begin(), rho.begin() + m);
+  ps.data.drho.assign(drho.begin(), drho.begin() + m);
+  ps.data.rho_o.assign(rho_o.begin(), rho_o.begin() + m);
+  ps.data.r = vec(r); ps.data.v = vec(v); ps.data.a = vec(a); ps.data.r_o = vec(ro); ps.data.v_o = vec(vo);
+  ps.data.v_oo.resize(m);
+  ps.data.rho_oo.resize(m);
+  ps.dev_newer = false;
+}
+
+inline void tsph_ran(Particle_system &ps) { ps.dev_newer = true; }
+
+// ---------------------------------------------------------------------------
+// stage functions (reference signatures)
+// ---------------------------------------------------------------------------
+
+// src/SPH_find_pairs.h:5-42: sizes the grid and creates the device context
+inline void Divide_To_Cells(Particle_system &ps, Simulation_data sd) {
+  tsph_params prm{};
+  prm.h = ps.data_const.h; prm.kap = ps.data_const.kap; prm.cs = ps.data_const.cs;
+  prm.rho0 = ps.data_const.rho0; prm.avisc = ps.data_const.avisc; prm.delta = ps.data_const.delta;
+  prm.dp = ps.data_const.dp;
+  prm.x_0 = sd.x_0; prm.x_m = sd.x_m; prm.y_0 = sd.y_0; prm.y_m = sd.y_m;
+  prm.precision = ps.precision;
+  prm.capacity = ps.device_capacity > 0 ? ps.device_capacity : (int)(ps.np > 0 ? ps.np : 1);
+  if (ps.dev) tsph_destroy(ps.dev);
+  ps.dev = nullptr;
+  tsph_check(tsph_create(&prm, &ps.dev), "tsph_create");
+  int ncx = 0, ncy = 0;
+  tsph_check(tsph_grid(ps.dev, &ncx, &ncy), "tsph_grid");
+  ps.pairs.ncx = (unsigned)ncx;
+  ps.pairs.ncy = (unsigned)ncy;
+  ps.host_dirty = true;
+}
+
+// src/SPH_find_pairs.h:74-92
+inline void Clear_cells(Particle_system &ps) {
+  if (ps.dev) tsph_check(tsph_clear_cells(ps.dev), "Clear_cells");
+}
+
+// src/SPH_find_pairs.h:45-72
+inline void Particle_To_Cell(Particle_system &ps, Simulation_data) {
+  tsph_push(ps);
+  tsph_check(tsph_particle_to_cell(ps.dev), "Particle_To_Cell");
+  tsph_ran(ps);
+}
+
+// src/SPH_pressure.h:65-89
+inline void Density_to_pressure(Particle_system &ps) {
+  tsph_push(ps);
+  tsph_check(tsph_density_to_pressure(ps.dev), "Density_to_pressure");
+  tsph_ran(ps);
+}
+
+// src/SPH_interaction.h:8 / src/SPH_interactionDBC.h:8 / src/SPH_interactionBT.h:8
+inline void Compute_Forces(Particle_system &ps) {
+  tsph_push(ps);
+  tsph_check(tsph_compute_forces(ps.dev, TSPH_DROPIN_FORCES), "Compute_Forces");
+  tsph_ran(ps);
+}
+
+// src/SPH_mDBC.h:9-50
+inline void mDBC_compute_density_mdbcGeo(Particle_system &ps, Simulation_data) {
+  tsph_push(ps);
+  tsph_check(tsph_mdbc_density(ps.dev), "mDBC_compute_density_mdbcGeo");
+  tsph_ran(ps);
+}
+
+// src/SPH_BT.h:9-50
+inline void BT_compute_density_BTGeo(Particle_system &ps, Simulation_data) {
+  tsph_push(ps);
+  tsph_check(tsph_bt_density(ps.dev), "BT_compute_density_BTGeo");
+  tsph_ran(ps);
+}
+
+// src/SPH_integration.h:266-309
+inline void Integrate_SymplecticPredictor(Particle_system &ps, double dt) {
+  tsph_push(ps);
+  tsph_check(tsph_symplectic_predictor(ps.dev, dt), "Integrate_SymplecticPredictor");
+  tsph_ran(ps);
+}
+inline void Integrate_SymplecticCorrector(Particle_system &ps, double dt) {
+  tsph_push(ps);
+  tsph_check(tsph_symplectic_corrector(ps.dev, dt), "Integrate_SymplecticCorrector");
+  tsph_ran(ps);
+}
+
+// src/SPH_btdebug.h:4-31
+inline void RemoveParticlesOutOfDomain(Particle_system &ps, Simulation_data) {
+  tsph_push(ps);
+  tsph_check(tsph_remove_out_of_domain(ps.dev), "RemoveParticlesOutOfDomain");
+  tsph_ran(ps);
+}
+
+// src/SPH_water_level.h:87-150
+inline real Water_Elevation(Particle_system &ps, Simulation_data, real x, real y0) {
+  tsph_push(ps);
+  double wl = 0;
+  tsph_check(tsph_water_elevation(ps.dev, x, y0, 0, &wl), "Water_Elevation");
+  return wl;
+}
+inline real Water_Elevation_inverse(Particle_system &ps, Simulation_data, real x, real y0) {
+  tsph_push(ps);
+  double wl = 0;
+  tsph_check(tsph_water_elevation(ps.dev, x, y0, 1, &wl), "Water_Elevation_inverse");
+  return wl;
+}
+// src/SPH_measure_pressure.h:6-15
+inline real Pressure_in_point(Particle_system &ps, Simulation_data, realvec ar) {
+  tsph_push(ps);
+  double out[4];
+  tsph_check(tsph_interpolate_point(ps.dev, ar.x, ar.y, out), "Pressure_in_point");
+  return out[1];
+}
+
+// ---------------------------------------------------------------------------
+// geometry, src/SPH_create.h and src/SPH_create_bt.h (host side, same arithmetic)
+// ---------------------------------------------------------------------------
+
+// src/SPH_pressure.h:49-63
+inline real Pressure_to_density(real p, real rho0, real c0) {
+  const real b_const = c0 * c0 * rho0 / 7.;
+  return (pow(p / b_const + 1, 1. / 7)) * rho0;
+}
+
+// src/SPH_create.h:546-581
+inline void Create_Wall_x_mdbc(Particle_system &ps, double x_0, double x_1, double y, int orientation,
+                               double y_w) {
+  const real dp = ps.data_const.dp, kh = ps.data_const.h * ps.data_const.kap;
+  const real len = LEN(x_0, y, x_1, y);
+  const int npart = (int)(len / dp);
+  const int nvl = std::ceil(kh / dp);
+  realvec r, v = {0, 0}, a = {0, 0}, gn;
+  for (int l = 0; l < nvl; l++) {
+    r.y = y + dp * l * orientation;
+    for (int i = 0; i < npart + 1; i++) {
+      r.x = x_0 + i * dp;
+      ps.Add_particle(wall, 0, ps.data_const.rho0, r, v, a);
+      gn.x = r.x;
+      gn.y = y_w + fabs(r.y - y_w);
+      ps.special.gnr.push_back(gn);
+    }
+  }
+}
+
+// src/SPH_create.h:583-619
+inline void Create_Wall_y_mdbc(Particle_system &ps, double x, double y_0, double y_1, int orientation,
+                               double x_w) {
+  const real dp = ps.data_const.dp, kh = ps.data_const.h * ps.data_const.kap;
+  const real len = LEN(x, y_0, x, y_1);
+  const int npart = (int)(len / dp);
+  const int nvl = std::ceil(kh / dp);
+  realvec r, v = {0, 0}, a = {0, 0}, gn;
+  for (int l = 0; l < nvl; l++) {
+    r.x = x + dp * l * orientation;
+    for (int i = 0; i < npart + 1; i++) {
+      r.y = y_0 + i * dp;
+      ps.Add_particle(wall, 0, ps.data_const.rho0, r, v, a);
+      gn.x = x_w + fabs(r.x - x_w) * (-1) * (orientation);
+      gn.y = r.y;
+      ps.special.gnr.push_back(gn);
+    }
+  }
+}
+
+// src/SPH_create.h:621-652
+inline void Create_corner_mdbc(Particle_system &ps, int orientationX, int orientationY, realvec xyv) {
+  const real dp = ps.data_const.dp, kh = ps.data_const.h * ps.data_const.kap;
+  const int nvl = std::ceil(kh / dp);
+  realvec r, v = {0, 0}, a = {0, 0}, gn;
+  for (int l = 0; l < nvl; l++) {
+    r.y = xyv.y + dp * l * orientationY + dp / 2 * orientationY;
+    for (int i = 0; i < nvl; i++) {
+      r.x = xyv.x + dp * i * orientationX + dp / 2 * orientationX;
+      ps.Add_particle(wall, 0, ps.data_const.rho0, r, v, a);
+      gn.x = xyv.x + fabs(r.x - xyv.x) * (-1) * orientationX;
+      gn.y = xyv.y + fabs(r.y - xyv.y) * (-1) * orientationY;
+      ps.special.gnr.push_back(gn);
+    }
+  }
+}
+
+// src/SPH_create.h:105-139
+inline void Create_Block_of_Fluid(Particle_system &ps, double x_0, double y_0, double x_1, double y_1) {
+  const real dp = ps.data_const.dp;
+  const real len_x = LEN(x_0, y_0, x_1, y_0), len_y = LEN(x_0, y_0, x_0, y_1);
+  const int npart_x = (int)(len_x / dp), npart_y = (int)(len_y / dp);
+  realvec r, v = {0, 0}, a = {0, 0};
+  for (int i = 0; i < npart_x + 1; i++) {
+    r.x = x_0 + i * dp;
+    for (int j = 0; j < npart_y + 1; j++) {
+      r.y = y_0 + j * dp;
+      ps.Add_particle(fluid, 0, ps.data_const.rho0, r, v, a);
+    }
+  }
+}
+
+// src/SPH_create.h:689-727 (hydrostatic pressure and density)
+inline void Create_Block_of_FluidWP(Particle_system &ps, double x_0, double y_0, double x_1, double y_1) {
+  const real dp = ps.data_const.dp;
+  const real len_x = LEN(x_0, y_0, x_1, y_0), len_y = LEN(x_0, y_0, x_0, y_1);
+  const int npart_x = (int)(len_x / dp), npart_y = (int)(len_y / dp);
+  realvec r, v = {0, 0}, a = {0, 0};
+  for (int i = 0; i < npart_x + 1; i++) {
+    r.x = x_0 + i * dp;
+    for (int j = 0; j < npart_y + 1; j++) {
+      r.y = y_0 + j * dp;
+      const real rhoin = Pressure_to_density(9.81 * (y_1 - r.y) * 1000, 1000., ps.data_const.cs);
+      ps.Add_particle(fluid, 9.81 * (y_1 - r.y) * 1000, rhoin, r, v, a);
+    }
+  }
+}
+
+// src/SPH_create_bt.h:49-82
+inline void Create_Wall_x_bt(Particle_system &ps, double x_0, double x_1, double y, int orientation, realvec n) {
+  const real dp = ps.data_const.dp;
+  const real len = LEN(x_0, y, x_1, y);
+  const int npart = (int)(len / dp);
+  realvec r, v = {0, 0}, a = {0, 0};
+  for (int l = 0; l < 1; l++) {
+    r.y = y + dp * l * orientation;
+    for (int i = 0; i < npart + 1; i++) {
+      r.x = x_0 + i * dp;
+      ps.Add_particle(wall, 0, ps.data_const.rho0, r, v, a);
+      ps.special.n.push_back(n);
+    }
+  }
+}
+
+// src/SPH_create_bt.h:84-118
+inline void Create_Wall_y_bt(Particle_system &ps, double x, double y_0, double y_1, int orientation, realvec n) {
+  const real dp = ps.data_const.dp;
+  const real len = LEN(x, y_0, x, y_1);
+  const int npart = (int)(len / dp);
+  realvec r, v = {0, 0}, a = {0, 0};
+  for (int l = 0; l < 1; l++) {
+    r.x = x + dp * l * orientation;
+    for (int i = 0; i < npart + 1; i++) {
+      r.y = y_0 + i * dp;
+      ps.Add_particle(wall, 0, ps.data_const.rho0, r, v, a);
+      ps.special.n.push_back(n);
+    }
+  }
+}
+
+// src/SPH_create_bt.h:120-149
+inline void Create_corner_bt(Particle_system &ps, int orientationX, int orientationY, realvec xyv, realvec n) {
+  const real dp = ps.data_const.dp;
+  realvec r, v = {0, 0}, a = {0, 0};
+  for (int l = 0; l < 1; l++) {
+    r.y = xyv.y + dp * l * orientationY;
+    for (int i = 0; i < 1; i++) {
+      r.x = xyv.x + dp * i * orientationX;
+      ps.Add_particle(wall, 0, ps.data_const.rho0, r, v, a);
+      ps.special.n.push_back(n);
+    }
+  }
+}

@@ -1,0 +1,236 @@
+// Test infrastructure only: a C-ABI harness around the reference's own stage
+// functions, compiled from the reference headers where they lie (see
+// oracle/Makefile).  It is used by tests/ to pin the oracle restatement
+// (oracle/tsph_oracle.cpp) and may serve as the "reference" CPU baseline.
+// Nothing in the product links or loads it.
+//
+// One shared object is built per pair-loop variant because every variant
+// header defines a function named Compute_Forces:
+//   REF_WCSPH  src/SPH_interaction.h     + src/SPH_mDBC.h  (mDBC cases)
+//   REF_DBC    src/SPH_interactionDBC.h                   (DBC cases)
+//   REF_BT     src/SPH_interactionBT.h   + src/SPH_BT.h    (BT cases)
+// It is built without -fopenmp: the reference kernel returns a pointer to a
+// static buffer (src/SPH_kernel.h:54), which is only well defined serially.
+
+#include "SPH_defs.h"
+#include "SPH_particle_system.h"
+#include "SPH_find_pairs.h"
+#include "SPH_pressure.h"
+#if defined(REF_DBC)
+#include "SPH_interactionDBC.h"
+#elif defined(REF_BT)
+#include "SPH_interactionBT.h"
+#include "SPH_BT.h"
+#else
+#include "SPH_interaction.h"
+#include "SPH_mDBC.h"
+#endif
+#include "SPH_integration.h"
+#include "SPH_btdebug.h"
+#include "SPH_water_level.h"
+#include "SPH_measure_pressure.h"
+#if defined(REF_BT)
+#include "SPH_create_bt.h"
+#else
+#include "SPH_create.h"
+#endif
+#if defined(REF_WCSPH)
+#include "SPH_buffer_zones.h"
+#endif
+
+namespace {
+
+struct RefCase {
+  Particle_system ps;
+  Simulation_data sd;
+  RefCase(const double* q)
+      : ps(q[0], q[1], q[2], q[3], q[4], q[5], q[6]), sd(q[7], q[8], q[9], q[10]) {
+    sd.hh = q[0];
+    sd.kh = q[0] * q[1];
+    sd.nvl = 0;
+  }
+};
+
+std::vector<realvec>* vec2_field(RefCase* c, int f) {
+  switch (f) {
+    case 0: return &c->ps.data.r;
+    case 1: return &c->ps.data.v;
+    case 2: return &c->ps.data.a;
+    case 6: return &c->ps.data.r_o;
+    case 7: return &c->ps.data.v_o;
+    case 9: return &c->ps.special.gnr;
+    case 10: return &c->ps.special.n;
+    default: return nullptr;
+  }
+}
+
+std::vector<real>* scal_field(RefCase* c, int f) {
+  switch (f) {
+    case 3: return &c->ps.data.p;
+    case 4: return &c->ps.data.rho;
+    case 5: return &c->ps.data.drho;
+    case 8: return &c->ps.data.rho_o;
+    default: return nullptr;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+// q = {h, kap, cs, rho0, avisc, delta, dp, x_0, x_m, y_0, y_m} (padded domain)
+void* ref_new(const double* q) { return new RefCase(q); }
+void ref_free(void* c) { delete static_cast<RefCase*>(c); }
+
+void ref_add(void* c, int type, double p, double rho, double rx, double ry,
+             double vx, double vy) {
+  static_cast<RefCase*>(c)->ps.Add_particle((unsigned short)type, p, rho, {rx, ry},
+                                           {vx, vy}, {0., 0.});
+}
+void ref_push_gnr(void* c, double x, double y) {
+  static_cast<RefCase*>(c)->ps.special.gnr.push_back({x, y});
+}
+void ref_push_n(void* c, double x, double y) {
+  static_cast<RefCase*>(c)->ps.special.n.push_back({x, y});
+}
+
+int ref_np(void* c) { return (int)static_cast<RefCase*>(c)->ps.np; }
+int ref_np_out(void* c) { return (int)static_cast<RefCase*>(c)->ps.np_out; }
+double ref_mass(void* c) { return static_cast<RefCase*>(c)->ps.data_const.m; }
+
+// Field ids: 0 r, 1 v, 2 a, 3 p, 4 rho, 5 drho, 6 r_o, 7 v_o, 8 rho_o, 9 gnr, 10 n.
+// Returns the number of doubles written/read (0 for an unknown field).
+int ref_field_len(void* vc, int f) {
+  RefCase* c = static_cast<RefCase*>(vc);
+  if (auto* v = vec2_field(c, f)) return 2 * (int)v->size();
+  if (auto* s = scal_field(c, f)) return (int)s->size();
+  return 0;
+}
+int ref_get(void* vc, int f, double* out) {
+  RefCase* c = static_cast<RefCase*>(vc);
+  if (auto* v = vec2_field(c, f)) {
+    for (size_t i = 0; i < v->size(); ++i) { out[2 * i] = (*v)[i].x; out[2 * i + 1] = (*v)[i].y; }
+    return 2 * (int)v->size();
+  }
+  if (auto* s = scal_field(c, f)) {
+    for (size_t i = 0; i < s->size(); ++i) out[i] = (*s)[i];
+    return (int)s->size();
+  }
+  return 0;
+}
+int ref_set(void* vc, int f, const double* in) {
+  RefCase* c = static_cast<RefCase*>(vc);
+  if (auto* v = vec2_field(c, f)) {
+    for (size_t i = 0; i < v->size(); ++i) { (*v)[i].x = in[2 * i]; (*v)[i].y = in[2 * i + 1]; }
+    return 2 * (int)v->size();
+  }
+  if (auto* s = scal_field(c, f)) {
+    for (size_t i = 0; i < s->size(); ++i) (*s)[i] = in[i];
+    return (int)s->size();
+  }
+  return 0;
+}
+void ref_get_type(void* vc, int* out) {
+  RefCase* c = static_cast<RefCase*>(vc);
+  for (size_t i = 0; i < c->ps.data.part_type.size(); ++i) out[i] = c->ps.data.part_type[i];
+}
+void ref_get_index(void* vc, int* out) {
+  RefCase* c = static_cast<RefCase*>(vc);
+  for (size_t i = 0; i < c->ps.data.index.size(); ++i) out[i] = c->ps.data.index[i];
+}
+
+// ---- cell list ----
+void ref_divide_to_cells(void* vc) {
+  RefCase* c = static_cast<RefCase*>(vc);
+  Divide_To_Cells(c->ps, c->sd);
+}
+void ref_clear_cells(void* vc) { Clear_cells(static_cast<RefCase*>(vc)->ps); }
+void ref_particle_to_cell(void* vc) {
+  RefCase* c = static_cast<RefCase*>(vc);
+  Particle_To_Cell(c->ps, c->sd);
+}
+void ref_grid(void* vc, int* ncx, int* ncy) {
+  RefCase* c = static_cast<RefCase*>(vc);
+  *ncx = (int)c->ps.pairs.ncx;
+  *ncy = (int)c->ps.pairs.ncy;
+}
+// CSR export of the cell lists: start[ncells+1], items[np]
+void ref_cell_csr(void* vc, int* start, int* items) {
+  RefCase* c = static_cast<RefCase*>(vc);
+  int k = 0;
+  for (size_t cell = 0; cell < c->ps.cells.size(); ++cell) {
+    start[cell] = k;
+    for (int p : c->ps.cells[cell].cp) items[k++] = p;
+  }
+  start[c->ps.cells.size()] = k;
+}
+
+// ---- stages ----
+void ref_density_to_pressure(void* vc) { Density_to_pressure(static_cast<RefCase*>(vc)->ps); }
+void ref_compute_forces(void* vc) { Compute_Forces(static_cast<RefCase*>(vc)->ps); }
+void ref_wall_density(void* vc) {
+  RefCase* c = static_cast<RefCase*>(vc);
+#if defined(REF_BT)
+  BT_compute_density_BTGeo(c->ps, c->sd);
+#elif defined(REF_WCSPH)
+  mDBC_compute_density_mdbcGeo(c->ps, c->sd);
+#else
+  (void)c;
+#endif
+}
+void ref_predictor(void* vc, double dt) { Integrate_SymplecticPredictor(static_cast<RefCase*>(vc)->ps, dt); }
+void ref_corrector(void* vc, double dt) { Integrate_SymplecticCorrector(static_cast<RefCase*>(vc)->ps, dt); }
+void ref_remove_out_of_domain(void* vc) {
+  RefCase* c = static_cast<RefCase*>(vc);
+  RemoveParticlesOutOfDomain(c->ps, c->sd);
+}
+
+// ---- measurements ----
+double ref_water_elevation(void* vc, double x, double y0) {
+  RefCase* c = static_cast<RefCase*>(vc);
+  return Water_Elevation(c->ps, c->sd, x, y0);
+}
+double ref_water_elevation_inverse(void* vc, double x, double y0) {
+  RefCase* c = static_cast<RefCase*>(vc);
+  return Water_Elevation_inverse(c->ps, c->sd, x, y0);
+}
+double ref_sum_kernel_function(void* vc, double x, double y) {
+  RefCase* c = static_cast<RefCase*>(vc);
+  return Sum_kernel_function(c->ps, c->sd, {x, y});
+}
+void ref_interpolate_point(void* vc, double x, double y, double* out4) {
+  RefCase* c = static_cast<RefCase*>(vc);
+  std::array<real, 4> f = Kernel_velocity_approximation_TEST(c->ps, c->sd, {x, y});
+  for (int i = 0; i < 4; ++i) out4[i] = f[i];
+}
+
+// ---- case geometry (the reference's own creation routines) ----
+#if defined(REF_BT)
+void ref_create_wall_x_bt(void* vc, double x0, double x1, double y, int o, double nx, double ny) {
+  Create_Wall_x_bt(static_cast<RefCase*>(vc)->ps, x0, x1, y, o, {nx, ny});
+}
+void ref_create_wall_y_bt(void* vc, double x, double y0, double y1, int o, double nx, double ny) {
+  Create_Wall_y_bt(static_cast<RefCase*>(vc)->ps, x, y0, y1, o, {nx, ny});
+}
+void ref_create_corner_bt(void* vc, int ox, int oy, double cx, double cy, double nx, double ny) {
+  Create_corner_bt(static_cast<RefCase*>(vc)->ps, ox, oy, {cx, cy}, {nx, ny});
+}
+#else
+void ref_create_wall_x_mdbc(void* vc, double x0, double x1, double y, int o, double yw) {
+  Create_Wall_x_mdbc(static_cast<RefCase*>(vc)->ps, x0, x1, y, o, yw);
+}
+void ref_create_wall_y_mdbc(void* vc, double x, double y0, double y1, int o, double xw) {
+  Create_Wall_y_mdbc(static_cast<RefCase*>(vc)->ps, x, y0, y1, o, xw);
+}
+void ref_create_corner_mdbc(void* vc, int ox, int oy, double cx, double cy) {
+  Create_corner_mdbc(static_cast<RefCase*>(vc)->ps, ox, oy, {cx, cy});
+}
+void ref_create_block_of_fluid(void* vc, double x0, double y0, double x1, double y1) {
+  Create_Block_of_Fluid(static_cast<RefCase*>(vc)->ps, x0, y0, x1, y1);
+}
+#endif
+void ref_create_block_of_fluid_wp(void* vc, double x0, double y0, double x1, double y1) {
+  Create_Block_of_FluidWP(static_cast<RefCase*>(vc)->ps, x0, y0, x1, y1);
+}
+
+}  // extern "C"

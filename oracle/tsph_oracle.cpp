@@ -1,0 +1,638 @@
+// tsph_oracle.cpp -- the ORACLE: a CPU restatement of the reference tinySPH
+// per-step particle pipeline, used only as the checker by tests/, by
+// __graft_entry__.smoke() and by bench.py's CPU-baseline leg.  It is test
+// infrastructure, never part of the shipped product.
+//
+// Every stage follows the reference operation by operation (same operand
+// order, same float-rounding points, same neighbour-cell order, same
+// ascending in-cell order), cited per function.  Two deliberate differences,
+// neither of which changes a single bit of the results:
+//   * kernel values are returned by value instead of through a static buffer
+//     (src/SPH_kernel.h:54), so the pair loops may run under OpenMP;
+//   * cell lists are stored as one CSR array instead of a std::vector per cell
+//     (src/SPH_data.h:95-108); the in-cell order is still ascending particle
+//     index (src/SPH_find_pairs.h:61-71).
+// Pinning: tests/test_oracle_ref.py compares every stage here bit-for-bit with
+// the reference's own functions compiled in place (oracle/_ref); the
+// closed-form and invariant tests in tests/test_oracle_math.py cover the rest.
+//
+// Build: oracle/Makefile (-O3 -funroll-loops -ffp-contract=off; OpenMP variant
+// liboracle_omp.so for the multi-core baseline).
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+#include <algorithm>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+struct V2 { double x, y; };
+inline V2 operator+(V2 a, V2 b) { return {a.x + b.x, a.y + b.y}; }
+inline V2 operator-(V2 a, V2 b) { return {a.x - b.x, a.y - b.y}; }
+inline V2 operator*(V2 a, double b) { return {a.x * b, a.y * b}; }
+inline V2 operator/(V2 a, double b) { return {a.x / b, a.y / b}; }
+
+// Particle type ids, src/SPH_defs.h:44-65
+enum { T_FLUID = 0, T_WALL = 1, T_INLET = 6, T_OUTLET = 11 };
+const double EPS = 0.01;  // src/SPH_defs.h:69
+
+struct Params {  // SPH_constants + Simulation_data, src/SPH_data.h:30-46,111-125
+  double h, kap, cs, rho0, avisc, delta, dp;
+  double x0, xm, y0, ym;
+  double m, gx, gy;
+};
+
+struct Case {
+  Params q;
+  int np = 0, np_out = 0;
+  std::vector<int> type, index;
+  std::vector<double> p, rho, drho, rho_o;
+  std::vector<V2> r, v, a, r_o, v_o, gnr, n;
+  int ncx = 0, ncy = 0;
+  std::vector<int> cstart, citems;  // CSR cell lists
+  int nthreads = 1;
+};
+
+// src/SPH_kernel.h:52-77 (Wendland C2, 2-D) with the reference's casts:
+// awen/bwen rounded to float, wqq1/wqq rounded to float, wqq2 a float product.
+struct KW { double w, f; };
+inline KW wendland(double r, double h) {
+  const double awen = (double)(float)(0.557 / (h * h));
+  const double bwen = (double)(float)(-2.7852 / (h * h * h));
+  const double qq = r / h;
+  const float wqq1 = (float)(1.0 - 0.5 * qq);
+  const float wqq2 = wqq1 * wqq1;
+  const float wqq = (float)(qq + qq + 1.0);
+  KW k{0.0, 0.0};
+  if (qq <= 2.0) {
+    k.w = awen * (double)wqq * (double)wqq2 * (double)wqq2;
+    k.f = bwen * qq * (double)wqq2 * (double)wqq1 / r;
+  }
+  if (r == 0.0) { k.w = awen; k.f = 0.0; }
+  return k;
+}
+
+// src/SPH_viscouse_forces.h:7-25 (called with rho0 := 0.5*(rho_i+rho_j))
+inline double artificial_viscosity(double h, double drs, double drdv, double rhobar,
+                                   double c0, double alpha) {
+  double av = 0.0;
+  const double visco = drdv > alpha ? drdv : alpha;
+  if (drdv < 0) {
+    const double mu = h * drdv / (drs * drs + EPS * h * h);
+    av = -visco * c0 * mu / rhobar;
+  }
+  return av;
+}
+
+// src/SPH_viscouse_forces.h:45-61
+inline double diffusion_fourtakas(double h, double drs, double dr_z, double drdW,
+                                  double arho, double nrho, double rho0, double c0,
+                                  double delta, double m, double gravity) {
+  const double cb = c0 * c0 * rho0 / 7.;
+  const double dpH = 1. + dr_z * rho0 * gravity / cb;
+  const double drhoH = rho0 * std::pow(dpH, 1. / 7) - rho0;
+  const double psi = 2. * c0 * h * delta * ((nrho - arho) - drhoH) / (drs * drs + EPS * h * h);
+  return psi * drdW * m / nrho;
+}
+
+// src/SPH_pressure.h:7-19
+inline double tait_pressure(double rho, double rho0, double c0) {
+  const double b = c0 * c0 * rho0 / 7.;
+  const double rr = rho / rho0;
+  return b * (std::pow(rr, 7.) - 1);
+}
+
+// src/SPH_pressure.h:49-63
+inline double pressure_to_density(double p, double rho0, double c0) {
+  const double b = c0 * c0 * rho0 / 7.;
+  return std::pow(p / b + 1, 1. / 7) * rho0;
+}
+
+inline double kh_of(const Params& q) { return q.kap * q.h; }
+inline double of_of(const Params& q) { return q.h * 0.273; }
+
+// cell of a point, src/SPH_find_pairs.h:64-65 (truncation toward zero)
+inline void cell_xy(const Params& q, V2 r, int& cx, int& cy) {
+  const double kh = kh_of(q), of = of_of(q);
+  cx = (int)((r.x - q.x0 + of) / kh);
+  cy = (int)((r.y - q.y0 + of) / kh);
+}
+
+// 9-cell stencil in the reference order, src/SPH_interaction.h:26-36
+inline void stencil(int c, int ncy, int ac[9]) {
+  ac[0] = c; ac[1] = c + 1; ac[2] = c + 1 + ncy; ac[3] = c + ncy; ac[4] = c - 1 + ncy;
+  ac[5] = c - 1; ac[6] = c - 1 - ncy; ac[7] = c - ncy; ac[8] = c + 1 - ncy;
+}
+
+inline int ncells(const Case& s) { return s.ncx * s.ncy; }
+
+// ---------------------------------------------------------------------------
+// Pair loops.  Accumulation order per active particle: the 9 cells in stencil
+// order, then ascending particle index inside each cell; candidates with
+// cl < 0 are skipped and no upper bound is checked (src/SPH_interaction.h:71-80).
+// ---------------------------------------------------------------------------
+
+// variant 0: fluid-only WCSPH, src/SPH_interaction.h:8-149
+// variant 1: DBC, src/SPH_interactionDBC.h:8-155
+// variant 2: BT boundary integral, src/SPH_interactionBT.h:8-192
+void compute_forces(Case& s, int variant) {
+  const Params& q = s.q;
+  const int nc = ncells(s);
+  const double h = q.h, m = q.m, c0 = q.cs, delta = q.delta, rho0 = q.rho0, dp = q.dp;
+  const V2 g{q.gx, q.gy};
+#pragma omp parallel for schedule(dynamic, 64) num_threads(s.nthreads)
+  for (int c = 0; c < nc; ++c) {
+    int ac[9];
+    stencil(c, s.ncy, ac);
+    for (int ii = s.cstart[c]; ii < s.cstart[c + 1]; ++ii) {
+      const int ai = s.citems[ii];
+      const int at = s.type[ai];
+      if (variant == 1) {
+        if (at == T_INLET || at == T_OUTLET) continue;
+      } else if (at != T_FLUID) {
+        continue;
+      }
+      const V2 ar = s.r[ai], av = s.v[ai];
+      const double arho = s.rho[ai], ap = s.p[ai];
+      V2 acc{0., 0.};
+      double drho_sum = 0, diff_sum = 0, gamma = 0;
+      for (int k = 0; k < 9; ++k) {
+        const int cl = ac[k];
+        if (cl < 0) continue;
+        for (int jj = s.cstart[cl]; jj < s.cstart[cl + 1]; ++jj) {
+          const int ni = s.citems[jj];
+          if (ai == ni) continue;
+          const int nt = s.type[ni];
+          if (variant == 1 && at == T_WALL && nt == T_WALL) continue;
+          const V2 nr = s.r[ni], nv = s.v[ni];
+          const double nrho = s.rho[ni], np = s.p[ni];
+          const V2 dr = ar - nr, dv = av - nv;
+          const double drs = std::sqrt((dr.x * dr.x) + (dr.y * dr.y));
+          const double drdv = dr.x * dv.x + dr.y * dv.y;
+          const KW k2 = wendland(drs, h);
+          const V2 dW = dr * k2.f;
+          const double dvdW = dv.x * dW.x + dv.y * dW.y;
+          const double drdW = dr.x * dW.x + dr.y * dW.y;
+          const double p_temp = (ap + np) / (arho * nrho);
+          const double visco = artificial_viscosity(h, drs, drdv, 0.5 * (arho + nrho), c0, q.avisc);
+          const double diff = diffusion_fourtakas(h, drs, dr.y, drdW, arho, nrho, rho0, c0, delta, m, g.y);
+          if (variant == 2) {
+            // src/SPH_interactionBT.h:120-159
+            if (nt == T_WALL) {
+              const V2 nb = s.n[ni];
+              acc.x += k2.w * (p_temp + visco) * dp * nb.x * nrho;
+              acc.y += k2.w * (p_temp + visco) * dp * nb.y * nrho;
+              const double dvnb = dv.x * nb.x + dv.y * nb.y;
+              drho_sum -= dvnb * k2.w * nrho * dp;
+            } else {
+              acc.x -= dW.x * (p_temp + visco) * m;
+              acc.y -= dW.y * (p_temp + visco) * m;
+              drho_sum += dvdW * m;
+              diff_sum -= diff;
+            }
+            if (nt == T_FLUID) gamma += k2.w * m / nrho;
+          } else {
+            acc.x -= dW.x * (p_temp + visco) * m;
+            acc.y -= dW.y * (p_temp + visco) * m;
+            drho_sum += dvdW * m;
+            diff_sum -= diff;
+          }
+        }
+      }
+      if (variant == 2) {
+        // src/SPH_interactionBT.h:166-178
+        if (gamma > EPS) {
+          s.a[ai] = acc / gamma - g;
+          s.drho[ai] = drho_sum / gamma + diff_sum / gamma;
+        } else {
+          s.a[ai] = V2{0., 0.} - g;
+          s.drho[ai] = 0;
+        }
+      } else if (variant == 1 && at == T_WALL) {
+        s.a[ai] = V2{0., 0.};  // src/SPH_interactionDBC.h:137-139
+        s.drho[ai] = drho_sum + diff_sum;
+      } else {
+        s.a[ai] = acc - g;
+        s.drho[ai] = drho_sum + diff_sum;
+      }
+    }
+  }
+}
+
+// determinant, src/SPH_kernel_approx.h:663-675
+inline double det3(double a11, double a12, double a13, double a21, double a22, double a23,
+                   double a31, double a32, double a33) {
+  return (a11 * a22 * a33 + a12 * a23 * a31 + a13 * a21 * a32 - a31 * a22 * a13 -
+          a32 * a23 * a11 - a33 * a21 * a12);
+}
+
+// src/SPH_kernel_approx.h:1141-1342 (fluid neighbours of the ghost node `ar`)
+double mdbc_density_at(const Case& s, V2 ar, V2 gnd) {
+  const Params& q = s.q;
+  const double h = q.h, m = q.m;
+  double a11 = 0, a12 = 0, a13 = 0, a21 = 0, a22 = 0, a23 = 0, a31 = 0, a32 = 0, a33 = 0;
+  double drhodx = 0, drhody = 0, wsum = 0, rhowsum = 0;
+  int cx, cy;
+  cell_xy(q, ar, cx, cy);
+  int ac[9];
+  stencil(cy + cx * s.ncy, s.ncy, ac);
+  for (int k = 0; k < 9; ++k) {
+    const int cl = ac[k];
+    if (cl < 0) continue;
+    for (int jj = s.cstart[cl]; jj < s.cstart[cl + 1]; ++jj) {
+      const int ni = s.citems[jj];
+      if (s.type[ni] != T_FLUID) continue;
+      const V2 nr = s.r[ni];
+      const double nrho = s.rho[ni];
+      const V2 dr = ar - nr;
+      const double drs = std::sqrt(dr.x * dr.x + dr.y * dr.y);
+      const KW kw = wendland(drs, h);
+      wsum += kw.w;
+      rhowsum += m * kw.w;
+      drhodx += dr.x * kw.f * m;
+      drhody += dr.y * kw.f * m;
+      a11 += kw.w * (m / nrho);
+      a12 += dr.x * kw.w * (m / nrho);
+      a13 += dr.y * kw.w * (m / nrho);
+      a21 += dr.x * kw.f * (m / nrho);
+      a22 += dr.x * dr.x * kw.f * (m / nrho);
+      a23 += dr.x * dr.y * kw.f * (m / nrho);
+      a31 += dr.y * kw.f * (m / nrho);
+      a32 += dr.y * dr.x * kw.f * (m / nrho);
+      a33 += dr.y * dr.y * kw.f * (m / nrho);
+    }
+  }
+  (void)wsum;
+  const double det = det3(a11, a12, a13, a21, a22, a23, a31, a32, a33);
+  double dens;
+  if (det > 0.001) {
+    const double i11 = (a22 * a33 - a23 * a32) / det;
+    const double i12 = -(a12 * a33 - a13 * a32) / det;
+    const double i13 = (a12 * a23 - a13 * a22) / det;
+    const double i21 = -(a21 * a33 - a23 * a31) / det;
+    const double i22 = (a11 * a33 - a13 * a31) / det;
+    const double i23 = -(a11 * a23 - a13 * a21) / det;
+    const double i31 = (a21 * a32 - a22 * a31) / det;
+    const double i32 = -(a11 * a32 - a12 * a31) / det;
+    const double i33 = (a11 * a22 - a12 * a21) / det;
+    const double dtemp = rhowsum * i11 + drhodx * i12 + drhody * i13;
+    const double grx = -(rhowsum * i21 + drhodx * i22 + drhody * i23);
+    const double gry = -(rhowsum * i31 + drhodx * i32 + drhody * i33);
+    dens = (dtemp + grx * gnd.x + gry * gnd.y);
+  } else if (a11 > 0) {
+    dens = rhowsum / a11;
+  } else {
+    dens = 1000.;
+  }
+  return dens;
+}
+
+// src/SPH_mDBC.h:9-50
+void mdbc_density(Case& s) {
+#pragma omp parallel for schedule(static) num_threads(s.nthreads)
+  for (int p = 0; p < s.np; ++p) {
+    if (s.type[p] != T_WALL) continue;
+    const V2 gn = s.gnr[p];
+    const V2 gnd = s.r[p] - gn;
+    double rho = mdbc_density_at(s, gn, gnd);
+    if (rho < 1000.) rho = 1000.;
+    s.rho[p] = rho;
+  }
+}
+
+// src/SPH_kernel_approx.h:1838-1953
+double bt_density_at(const Case& s, V2 ar) {
+  const Params& q = s.q;
+  const double h = q.h, m = q.m;
+  double wsum = 0, rhowsum = 0;
+  int cx, cy;
+  cell_xy(q, ar, cx, cy);
+  int ac[9];
+  stencil(cy + cx * s.ncy, s.ncy, ac);
+  for (int k = 0; k < 9; ++k) {
+    const int cl = ac[k];
+    if (cl < 0) continue;
+    for (int jj = s.cstart[cl]; jj < s.cstart[cl + 1]; ++jj) {
+      const int ni = s.citems[jj];
+      if (s.type[ni] != T_FLUID) continue;
+      const V2 dr = ar - s.r[ni];
+      const double drs = std::sqrt(dr.x * dr.x + dr.y * dr.y);
+      const KW kw = wendland(drs, h);
+      wsum += kw.w * m / s.rho[ni];
+      rhowsum += m * kw.w;
+    }
+  }
+  return wsum > 0. ? rhowsum / wsum : 1000.;
+}
+
+// src/SPH_BT.h:9-50
+void bt_density(Case& s) {
+#pragma omp parallel for schedule(static) num_threads(s.nthreads)
+  for (int p = 0; p < s.np; ++p) {
+    if (s.type[p] != T_WALL) continue;
+    double rho = bt_density_at(s, s.r[p]);
+    if (rho < 1000.) rho = 1000.;
+    s.rho[p] = rho;
+  }
+}
+
+// Shepard sum of m W / rho over all types, src/SPH_water_level.h:6-82
+double sum_kernel_function(const Case& s, V2 ar) {
+  const Params& q = s.q;
+  const double kh = kh_of(q), of = of_of(q);
+  // the reference stores the cell coordinates in unsigned ints (:17,23-24)
+  const unsigned npx = (unsigned)(int)((ar.x - q.x0 + of) / kh);
+  const unsigned npy = (unsigned)(int)((ar.y - q.y0 + of) / kh);
+  const int c = (int)(npy + npx * (unsigned)s.ncy);
+  int ac[9];
+  stencil(c, s.ncy, ac);
+  double msum = 0;
+  for (int k = 0; k < 9; ++k) {
+    const int cl = ac[k];
+    if (cl < 0) continue;
+    for (int jj = s.cstart[cl]; jj < s.cstart[cl + 1]; ++jj) {
+      const int ni = s.citems[jj];
+      const V2 dr = ar - s.r[ni];
+      const double drs = std::sqrt(dr.x * dr.x + dr.y * dr.y);  // pow(x,2) is exact
+      const KW kw = wendland(drs, q.h);
+      msum += q.m * kw.w / s.rho[ni];
+    }
+  }
+  return msum;
+}
+
+// Shepard interpolation of rho, p, v over all types, src/testing.h:6-148
+void interpolate_point(const Case& s, V2 ar, double out[4]) {
+  const Params& q = s.q;
+  const double kh = kh_of(q), of = of_of(q);
+  const unsigned npx = (unsigned)(int)((ar.x - q.x0 + of) / kh);
+  const unsigned npy = (unsigned)(int)((ar.y - q.y0 + of) / kh);
+  const int c = (int)(npy + npx * (unsigned)s.ncy);
+  int ac[9];
+  stencil(c, s.ncy, ac);
+  V2 vw{0., 0.};
+  double wsum = 0, rhow = 0, pw = 0, mw = 0;
+  for (int k = 0; k < 9; ++k) {
+    const int cl = ac[k];
+    if (cl < 0) continue;
+    for (int jj = s.cstart[cl]; jj < s.cstart[cl + 1]; ++jj) {
+      const int ni = s.citems[jj];
+      const V2 dr = ar - s.r[ni];
+      const double drs = std::sqrt(dr.x * dr.x + dr.y * dr.y);
+      const KW kw = wendland(drs, q.h);
+      wsum += kw.w;
+      vw = vw + s.v[ni] * kw.w;
+      rhow = rhow + s.rho[ni] * kw.w;
+      pw = pw + s.p[ni] * kw.w;
+      mw += q.m * kw.w / s.rho[ni];
+    }
+  }
+  V2 vel{0., 0.};
+  double rho = 0, p = 0;
+  if (mw > 0.5) {
+    vel = V2{0., 0.} + vw / wsum;
+    rho = 0 + rhow / wsum;
+    p = 0 + pw / wsum;
+  }
+  if (wsum == 0) vel = V2{0., 0.};
+  out[0] = rho; out[1] = p; out[2] = vel.x; out[3] = vel.y;
+}
+
+}  // namespace
+
+extern "C" {
+
+// q = {h, kap, cs, rho0, avisc, delta, dp, x_0, x_m, y_0, y_m}
+void* orc_new(const double* q) {
+  Case* s = new Case();
+  Params& p = s->q;
+  p.h = q[0]; p.kap = q[1]; p.cs = q[2]; p.rho0 = q[3]; p.avisc = q[4]; p.delta = q[5];
+  p.dp = q[6]; p.x0 = q[7]; p.xm = q[8]; p.y0 = q[9]; p.ym = q[10];
+  p.m = p.rho0 / (1 / p.dp * 1 / p.dp);  // src/SPH_particle_system.h:207
+  p.gx = 0.;
+  p.gy = 9.81;  // src/SPH_particle_system.h:205
+  return s;
+}
+void orc_free(void* c) { delete static_cast<Case*>(c); }
+void orc_set_threads(void* c, int n) { static_cast<Case*>(c)->nthreads = n > 0 ? n : 1; }
+double orc_mass(void* c) { return static_cast<Case*>(c)->q.m; }
+int orc_np(void* c) { return static_cast<Case*>(c)->np; }
+int orc_np_out(void* c) { return static_cast<Case*>(c)->np_out; }
+
+// Load a whole particle set in reference order.  r, v are interleaved xy;
+// gnr / n may be NULL.  a, drho start at 0 and the "old" layers at the
+// current values, like Add_particle (src/SPH_particle_system.h:25-50).
+void orc_load(void* vc, int np, const int* type, const double* p, const double* rho,
+              const double* r, const double* v, const double* gnr, const double* n) {
+  Case& s = *static_cast<Case*>(vc);
+  s.np = np;
+  s.type.assign(type, type + np);
+  s.index.resize(np);
+  for (int i = 0; i < np; ++i) s.index[i] = i;
+  s.p.assign(p, p + np);
+  s.rho.assign(rho, rho + np);
+  s.rho_o = s.rho;
+  s.drho.assign(np, 0.);
+  s.r.resize(np); s.v.resize(np); s.a.assign(np, V2{0., 0.});
+  for (int i = 0; i < np; ++i) {
+    s.r[i] = {r[2 * i], r[2 * i + 1]};
+    s.v[i] = {v[2 * i], v[2 * i + 1]};
+  }
+  s.r_o = s.r; s.v_o = s.v;
+  s.gnr.assign(np, V2{0., 0.});
+  s.n.assign(np, V2{0., 0.});
+  if (gnr) for (int i = 0; i < np; ++i) s.gnr[i] = {gnr[2 * i], gnr[2 * i + 1]};
+  if (n) for (int i = 0; i < np; ++i) s.n[i] = {n[2 * i], n[2 * i + 1]};
+}
+
+// Field ids as the reference harness: 0 r, 1 v, 2 a, 3 p, 4 rho, 5 drho, 6 r_o,
+// 7 v_o, 8 rho_o, 9 gnr, 10 n.
+static std::vector<V2>* v2f(Case& s, int f) {
+  switch (f) { case 0: return &s.r; case 1: return &s.v; case 2: return &s.a; case 6: return &s.r_o;
+    case 7: return &s.v_o; case 9: return &s.gnr; case 10: return &s.n; default: return nullptr; }
+}
+static std::vector<double>* sf(Case& s, int f) {
+  switch (f) { case 3: return &s.p; case 4: return &s.rho; case 5: return &s.drho; case 8: return &s.rho_o;
+    default: return nullptr; }
+}
+int orc_get(void* vc, int f, double* out) {
+  Case& s = *static_cast<Case*>(vc);
+  if (auto* v = v2f(s, f)) { for (int i = 0; i < s.np; ++i) { out[2 * i] = (*v)[i].x; out[2 * i + 1] = (*v)[i].y; } return 2 * s.np; }
+  if (auto* x = sf(s, f)) { std::memcpy(out, x->data(), sizeof(double) * s.np); return s.np; }
+  return 0;
+}
+int orc_set(void* vc, int f, const double* in) {
+  Case& s = *static_cast<Case*>(vc);
+  if (auto* v = v2f(s, f)) { for (int i = 0; i < s.np; ++i) (*v)[i] = {in[2 * i], in[2 * i + 1]}; return 2 * s.np; }
+  if (auto* x = sf(s, f)) { std::memcpy(x->data(), in, sizeof(double) * s.np); return s.np; }
+  return 0;
+}
+void orc_get_type(void* vc, int* out) { Case& s = *static_cast<Case*>(vc); std::copy(s.type.begin(), s.type.end(), out); }
+void orc_get_index(void* vc, int* out) { Case& s = *static_cast<Case*>(vc); std::copy(s.index.begin(), s.index.end(), out); }
+
+// src/SPH_find_pairs.h:5-42
+void orc_divide_to_cells(void* vc) {
+  Case& s = *static_cast<Case*>(vc);
+  const double kh = kh_of(s.q), of = of_of(s.q);
+  s.ncx = (int)((s.q.xm - s.q.x0 + of) / kh) + 1;
+  s.ncy = (int)((s.q.ym - s.q.y0 + of) / kh) + 1;
+  s.cstart.assign(ncells(s) + 1, 0);
+  s.citems.clear();
+}
+void orc_grid(void* vc, int* ncx, int* ncy) {
+  Case& s = *static_cast<Case*>(vc);
+  *ncx = s.ncx; *ncy = s.ncy;
+}
+// src/SPH_find_pairs.h:74-92
+void orc_clear_cells(void* vc) {
+  Case& s = *static_cast<Case*>(vc);
+  std::fill(s.cstart.begin(), s.cstart.end(), 0);
+  s.citems.clear();
+}
+// src/SPH_find_pairs.h:45-72: serial binning in ascending particle index.  A
+// point outside the grid is undefined behaviour in the reference; here it
+// returns -1 - p (the first offending particle) instead of writing anywhere.
+int orc_particle_to_cell(void* vc) {
+  Case& s = *static_cast<Case*>(vc);
+  const int nc = ncells(s);
+  std::vector<int> cell(s.np);
+  std::vector<int> cnt(nc + 1, 0);
+  for (int p = 0; p < s.np; ++p) {
+    int cx, cy;
+    cell_xy(s.q, s.r[p], cx, cy);
+    const int c = cy + cx * s.ncy;
+    if (c < 0 || c >= nc) return -1 - p;
+    cell[p] = c;
+    cnt[c + 1]++;
+  }
+  for (int c = 0; c < nc; ++c) cnt[c + 1] += cnt[c];
+  s.cstart = cnt;
+  s.citems.assign(s.np, 0);
+  std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+  for (int p = 0; p < s.np; ++p) s.citems[fill[cell[p]]++] = p;
+  return 0;
+}
+void orc_cell_csr(void* vc, int* start, int* items) {
+  Case& s = *static_cast<Case*>(vc);
+  std::copy(s.cstart.begin(), s.cstart.end(), start);
+  std::copy(s.citems.begin(), s.citems.end(), items);
+}
+
+// src/SPH_pressure.h:65-89; returns the number of particles whose density is
+// outside [0, 1500] (the reference prints a warning for each and continues).
+int orc_density_to_pressure(void* vc) {
+  Case& s = *static_cast<Case*>(vc);
+  int bad = 0;
+#pragma omp parallel for schedule(static) num_threads(s.nthreads) reduction(+ : bad)
+  for (int p = 0; p < s.np; ++p) {
+    if (s.type[p] == T_OUTLET || s.type[p] == T_INLET) continue;
+    if (s.rho[p] > 1500. || s.rho[p] < 0.) ++bad;
+    s.p[p] = tait_pressure(s.rho[p], s.q.rho0, s.q.cs);
+  }
+  return bad;
+}
+
+void orc_compute_forces(void* vc, int variant) { compute_forces(*static_cast<Case*>(vc), variant); }
+void orc_mdbc_density(void* vc) { mdbc_density(*static_cast<Case*>(vc)); }
+void orc_bt_density(void* vc) { bt_density(*static_cast<Case*>(vc)); }
+
+// src/SPH_integration.h:266-287 (all particle types)
+void orc_predictor(void* vc, double dt) {
+  Case& s = *static_cast<Case*>(vc);
+#pragma omp parallel for schedule(static) num_threads(s.nthreads)
+  for (int p = 0; p < s.np; ++p) {
+    s.rho_o[p] = s.rho[p];
+    s.r_o[p] = s.r[p];
+    s.v_o[p] = s.v[p];
+    s.rho[p] = s.rho[p] + s.drho[p] * dt * 0.5;
+    s.r[p] = s.r[p] + s.v[p] * dt * 0.5;
+    s.v[p] = s.v[p] + s.a[p] * dt * 0.5;
+  }
+}
+
+// src/SPH_integration.h:289-309 (all particle types)
+void orc_corrector(void* vc, double dt) {
+  Case& s = *static_cast<Case*>(vc);
+#pragma omp parallel for schedule(static) num_threads(s.nthreads)
+  for (int p = 0; p < s.np; ++p) {
+    const double e = (-s.drho[p] / s.rho[p]) * dt;
+    s.rho[p] = s.rho_o[p] * ((2. - e) / (2. + e));
+    s.v[p] = s.v_o[p] + s.a[p] * dt;
+    s.r[p] = s.r_o[p] + (s.v[p] + s.v_o[p]) * dt * 0.5;
+  }
+}
+
+// src/SPH_btdebug.h:4-31 with Remove_particle, src/SPH_particle_system.h:110-146.
+// The erase happens inside the ascending loop without stepping back, so the
+// particle that slides into slot p is never examined.  gnr / n are not erased
+// by the reference; they are kept aligned with the fluid-last creation order
+// by erasing them here only where they exist for that slot (walls precede
+// fluid in every case, so the wall entries never move).
+int orc_remove_out_of_domain(void* vc) {
+  Case& s = *static_cast<Case*>(vc);
+  const double x0 = s.q.x0, xm = s.q.xm, y0 = s.q.y0, ym = s.q.ym;
+  int removed = 0;
+  for (int p = 0; p < s.np; ++p) {
+    if (s.type[p] != T_FLUID) continue;
+    const V2 r = s.r[p];
+    if ((r.y < (y0 - EPS)) || (r.y > ym + EPS) || (r.x < x0 - EPS) || (r.x > xm - EPS)) {
+      auto er = [p](auto& vec) { vec.erase(vec.begin() + p); };
+      er(s.type); er(s.index); er(s.p); er(s.rho); er(s.drho); er(s.r); er(s.v); er(s.a);
+      er(s.r_o); er(s.v_o); er(s.rho_o); er(s.gnr); er(s.n);
+      s.np--; s.np_out++; removed++;
+    }
+  }
+  return removed;
+}
+
+// src/SPH_water_level.h:87-117; returns NaN where the reference leaves WL
+// uninitialised (no sample reaches the threshold).
+double orc_water_elevation(void* vc, double x, double y0) {
+  Case& s = *static_cast<Case*>(vc);
+  const double dp_wl = s.q.dp * 0.00005;
+  V2 ar{x, y0};
+  const int nmax = int(s.q.ym / dp_wl);
+  for (int i = 0; i < nmax; ++i) {
+    ar.y += i * dp_wl;
+    if (sum_kernel_function(s, ar) <= 0.5) return ar.y - i * dp_wl;
+  }
+  return NAN;
+}
+// src/SPH_water_level.h:120-150
+double orc_water_elevation_inverse(void* vc, double x, double y0) {
+  Case& s = *static_cast<Case*>(vc);
+  const double dp_wl = s.q.dp * 0.00005;
+  V2 ar{x, y0};
+  const int nmax = int(s.q.ym / dp_wl);
+  for (int i = 0; i < nmax; ++i) {
+    ar.y -= i * dp_wl;
+    if (sum_kernel_function(s, ar) >= 0.5) return ar.y + i * dp_wl;
+  }
+  return NAN;
+}
+double orc_sum_kernel_function(void* vc, double x, double y) {
+  return sum_kernel_function(*static_cast<Case*>(vc), V2{x, y});
+}
+// src/SPH_measure_pressure.h:6-15 -> src/testing.h:6-148; out4 = {rho, p, vx, vy}
+void orc_interpolate_point(void* vc, double x, double y, double* out4) {
+  interpolate_point(*static_cast<Case*>(vc), V2{x, y}, out4);
+}
+
+// Scalar closures exported for the closed-form tests.
+double orc_wendland_w(double r, double h) { return wendland(r, h).w; }
+double orc_wendland_f(double r, double h) { return wendland(r, h).f; }
+double orc_tait(double rho, double rho0, double c0) { return tait_pressure(rho, rho0, c0); }
+double orc_pressure_to_density(double p, double rho0, double c0) { return pressure_to_density(p, rho0, c0); }
+double orc_artificial_viscosity(double h, double drs, double drdv, double rhobar, double c0, double alpha) {
+  return artificial_viscosity(h, drs, drdv, rhobar, c0, alpha);
+}
+double orc_diffusion_fourtakas(double h, double drs, double dr_z, double drdW, double arho, double nrho,
+                               double rho0, double c0, double delta, double m, double g) {
+  return diffusion_fourtakas(h, drs, dr_z, drdW, arho, nrho, rho0, c0, delta, m, g);
+}
+
+}  // extern "C"

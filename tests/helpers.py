@@ -1,0 +1,83 @@
+"""Shared test helpers: drive the oracle through the same stage sequence as the
+reference case drivers, and compare device results with it."""
+from __future__ import annotations
+
+import numpy as np
+
+from oracle import pyoracle as po
+
+WALL_DBC, WALL_MDBC, WALL_BT = 0, 1, 2
+
+
+def oracle_for(case, threads: int = 0):
+    return po.oracle_case(case.oracle_params(), case.arrays, omp_threads=threads)
+
+
+def oracle_variant(wall: int) -> int:
+    return {WALL_DBC: 1, WALL_MDBC: 0, WALL_BT: 2}[wall]
+
+
+def oracle_init(O) -> None:
+    """INIT: Divide_To_Cells + Particle_To_Cell (cases/*/SPH_simulationSymplectic.h INIT)."""
+    O.call("divide_to_cells")
+    rc = O.call("particle_to_cell")
+    assert rc == 0, "a particle lies outside the cell grid"
+
+
+def oracle_step(O, wall: int, dt: float) -> None:
+    """One symplectic step in the drivers' order
+    (cases/stillWater_DBC/SPH_simulationSymplectic.h:141-198,
+    cases/damBreakLong_mDBC/SPH_simulationSymplectic.h:59-116,
+    cases/damBreakLong_BT/SPH_simulationSymplectic.h:66-122)."""
+    var = oracle_variant(wall)
+    if wall == WALL_MDBC:
+        O.call("mdbc_density")
+        O.call("density_to_pressure")
+    elif wall == WALL_BT:
+        O.call("bt_density")
+        O.call("density_to_pressure")
+    O.call("predictor", dt)
+    O.call("clear_cells")
+    assert O.call("particle_to_cell") == 0
+    O.call("density_to_pressure")
+    O.call("compute_forces", var)
+    O.call("corrector", dt)
+    O.call("remove_out_of_domain")
+    O.call("clear_cells")
+    assert O.call("particle_to_cell") == 0
+    O.call("density_to_pressure")
+    O.call("compute_forces", var)
+
+
+def oracle_fields(O) -> dict:
+    return {
+        "type": O.types(),
+        "r": O.get(po.F_R), "v": O.get(po.F_V), "a": O.get(po.F_A),
+        "p": O.get(po.F_P), "rho": O.get(po.F_RHO), "drho": O.get(po.F_DRHO),
+    }
+
+
+def rel_err(a: np.ndarray, b: np.ndarray) -> float:
+    """max |a-b| / max |b| (scale-aware elementwise error)."""
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    den = max(float(np.max(np.abs(b))), 1e-300)
+    return float(np.max(np.abs(a - b))) / den
+
+
+def rel_l2(a: np.ndarray, b: np.ndarray) -> float:
+    a = np.asarray(a, dtype=np.float64).ravel()
+    b = np.asarray(b, dtype=np.float64).ravel()
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def brute_force_pairs(r: np.ndarray, active: np.ndarray, h: float) -> int:
+    """Ordered pairs (i active, j != i) with |r_i - r_j| < 2h, by brute force."""
+    n = len(r)
+    total = 0
+    for i in np.nonzero(active)[0]:
+        d = np.sqrt(((r - r[i]) ** 2).sum(axis=1))
+        m = d / h < 2.0
+        m[i] = False
+        total += int(m.sum())
+    return total

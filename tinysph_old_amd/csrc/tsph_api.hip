@@ -1,0 +1,503 @@
+// C ABI of libtinysph_hip.so (declared in include/tinysph.h): context
+// lifetime, host <-> device transfer in reference order, stage entry points
+// and the graph-replayed symplectic step.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "tsph_internal.h"
+
+namespace tsph {
+
+static thread_local std::string g_err;
+
+void set_error(const std::string &msg) { g_err = msg; }
+
+int hip_fail(hipError_t e, const char *what, const char *file, int line) {
+  char buf[512];
+  snprintf(buf, sizeof buf, "%s failed: %s (%s:%d)", what, hipGetErrorString(e), file, line);
+  g_err = buf;
+  return TSPH_EHIP;
+}
+
+static int fail(int code, const char *msg) {
+  g_err = msg;
+  return code;
+}
+
+template <typename T>
+static int dalloc(T **p, size_t count) {
+  *p = nullptr;
+  if (count == 0) count = 1;
+  TSPH_HIP(hipMalloc((void **)p, count * sizeof(T)));
+  return 0;
+}
+
+static int alloc_soa(Soa &s, int cap, bool gnr, bool n) {
+  int e = 0;
+  if ((e = dalloc(&s.r, cap)) || (e = dalloc(&s.v, cap)) || (e = dalloc(&s.a, cap)) ||
+      (e = dalloc(&s.r_o, cap)) || (e = dalloc(&s.v_o, cap)) || (e = dalloc(&s.rho, cap)) ||
+      (e = dalloc(&s.drho, cap)) || (e = dalloc(&s.p, cap)) || (e = dalloc(&s.rho_o, cap)) ||
+      (e = dalloc(&s.type, cap)) || (e = dalloc(&s.seq, cap)) || (e = dalloc(&s.index, cap)))
+    return e;
+  if (gnr && (e = dalloc(&s.gnr, cap))) return e;
+  if (n && (e = dalloc(&s.n, cap))) return e;
+  return 0;
+}
+
+static void free_soa(Soa &s) {
+  void *ps[] = {s.r, s.v, s.a, s.r_o, s.v_o, s.gnr, s.n, s.rho, s.drho, s.p, s.rho_o,
+                s.type, s.seq, s.index};
+  for (void *p : ps)
+    if (p) (void)hipFree(p);
+  s = Soa{};
+}
+
+static void drop_graph(tsph_ctx *c) {
+  if (c->graph.exec) (void)hipGraphExecDestroy(c->graph.exec);
+  c->graph = Graph{};
+}
+
+// Allocates the wall-extra fields (gnr / n) on first use.
+static int ensure_extras(tsph_ctx *c, bool gnr, bool n) {
+  int e = 0;
+  if (gnr && !c->has_gnr) {
+    if ((e = dalloc(&c->cur.gnr, c->capacity)) || (e = dalloc(&c->alt.gnr, c->capacity))) return e;
+    c->has_gnr = true;
+  }
+  if (n && !c->has_n) {
+    if ((e = dalloc(&c->cur.n, c->capacity)) || (e = dalloc(&c->alt.n, c->capacity))) return e;
+    if (c->prm.precision == TSPH_PREC_MIXED && (e = dalloc(&c->mx.nrm, c->capacity))) return e;
+    c->has_n = true;
+  }
+  return 0;
+}
+
+// one symplectic step in the drivers' stage order
+static int record_step(tsph_ctx *c, int wall, double dt) {
+  const int variant = wall == TSPH_WALL_DBC    ? TSPH_FORCES_DBC
+                      : wall == TSPH_WALL_MDBC ? TSPH_FORCES_WCSPH
+                                               : TSPH_FORCES_BT;
+  int e = 0;
+  // wall density + EOS (cases/damBreakLong_mDBC/SPH_simulationSymplectic.h:65-66,
+  // cases/damBreakLong_BT/SPH_simulationSymplectic.h:72-73); DBC has none
+  if (wall == TSPH_WALL_MDBC) {
+    if ((e = launch_mdbc(c)) || (e = launch_eos(c))) return e;
+  } else if (wall == TSPH_WALL_BT) {
+    if ((e = launch_bt_density(c)) || (e = launch_eos(c))) return e;
+  }
+  if ((e = launch_predictor(c, dt))) return e;
+  if ((e = launch_cell_list(c)) || (e = launch_eos(c)) || (e = launch_forces(c, variant))) return e;
+  if ((e = launch_corrector(c, dt)) || (e = launch_remove(c))) return e;
+  if ((e = launch_cell_list(c)) || (e = launch_eos(c)) || (e = launch_forces(c, variant))) return e;
+  return 0;
+}
+
+}  // namespace tsph
+
+using namespace tsph;
+
+extern "C" {
+
+const char *tsph_last_error(void) { return g_err.c_str(); }
+int tsph_abi_version(void) { return TSPH_ABI_VERSION; }
+
+int tsph_create(const tsph_params *prm, tsph_ctx **out) {
+  if (!prm || !out) return fail(TSPH_EINVAL, "null argument");
+  *out = nullptr;
+  if (!(prm->h > 0) || !(prm->kap > 0) || !(prm->dp > 0) || prm->capacity <= 0 ||
+      !(prm->x_m > prm->x_0) || !(prm->y_m > prm->y_0))
+    return fail(TSPH_EINVAL, "invalid parameters");
+  if (prm->precision != TSPH_PREC_FP64 && prm->precision != TSPH_PREC_MIXED)
+    return fail(TSPH_EINVAL, "unknown precision");
+  tsph_ctx *c = new tsph_ctx();
+  c->prm = *prm;
+  c->capacity = prm->capacity;
+  Consts &k = c->k;
+  k.h = prm->h; k.kap = prm->kap; k.cs = prm->cs; k.rho0 = prm->rho0; k.avisc = prm->avisc;
+  k.delta = prm->delta; k.dp = prm->dp;
+  k.m = prm->rho0 / (1 / prm->dp * 1 / prm->dp);  // src/SPH_particle_system.h:207
+  k.gx = 0.; k.gy = 9.81;                          // src/SPH_particle_system.h:205
+  k.x0 = prm->x_0; k.xm = prm->x_m; k.y0 = prm->y_0; k.ym = prm->y_m;
+  k.kh = prm->kap * prm->h;                        // src/SPH_find_pairs.h:11
+  k.of = prm->h * 0.273;                           // src/SPH_find_pairs.h:13
+  k.awen = (double)(float)(0.557 / (prm->h * prm->h));
+  k.bwen = (double)(float)(-2.7852 / (prm->h * prm->h * prm->h));
+  k.ncx = (int)((k.xm - k.x0 + k.of) / k.kh) + 1;  // src/SPH_find_pairs.h:19-20
+  k.ncy = (int)((k.ym - k.y0 + k.of) / k.kh) + 1;
+  const long long nc = (long long)k.ncx * k.ncy;
+  if (nc <= 0 || nc > (1ll << 30)) {
+    delete c;
+    return fail(TSPH_EINVAL, "cell grid too large");
+  }
+  k.ncells = (int)nc;
+  int e = 0;
+  auto bail = [&](int code) {
+    tsph_destroy(c);
+    return code;
+  };
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+    return bail(fail(TSPH_EHIP, "hipStreamCreate failed"));
+  const int cap = c->capacity;
+  if ((e = alloc_soa(c->cur, cap, false, false)) || (e = alloc_soa(c->alt, cap, false, false)))
+    return bail(e);
+  if ((e = dalloc(&c->scal, S_NSCAL)) || (e = dalloc(&c->counters, C_NCOUNT)) ||
+      (e = dalloc(&c->cell_count, k.ncells + 1)) || (e = dalloc(&c->cell_start, k.ncells + 2)) ||
+      (e = dalloc(&c->cellof, cap)) || (e = dalloc(&c->hrank, cap)) || (e = dalloc(&c->hcell, cap)) ||
+      (e = dalloc(&c->perm, cap)) || (e = dalloc(&c->scan_tmp, (k.ncells + 1) / 1024 + 2)) ||
+      (e = dalloc(&c->seqstate, cap)) || (e = dalloc(&c->probe_out, 4)))
+    return bail(e);
+  if (prm->precision == TSPH_PREC_MIXED) {
+    if ((e = dalloc(&c->mx.rv, cap)) || (e = dalloc(&c->mx.rp, cap))) return bail(e);
+  }
+  if (hipMemsetAsync(c->scal, 0, S_NSCAL * sizeof(int), c->stream) != hipSuccess ||
+      hipMemsetAsync(c->counters, 0, C_NCOUNT * sizeof(unsigned long long), c->stream) != hipSuccess ||
+      hipStreamSynchronize(c->stream) != hipSuccess)
+    return bail(fail(TSPH_EHIP, "initialisation failed"));
+  *out = c;
+  return 0;
+}
+
+int tsph_destroy(tsph_ctx *c) {
+  if (!c) return 0;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  drop_graph(c);
+  free_soa(c->cur);
+  free_soa(c->alt);
+  void *ps[] = {c->scal, c->counters, c->cell_count, c->cell_start, c->cellof, c->hrank,
+                c->hcell, c->perm, c->scan_tmp, c->seqstate, c->probe_out, c->mx.rv,
+                c->mx.rp, c->mx.nrm};
+  for (void *p : ps)
+    if (p) (void)hipFree(p);
+  for (hipEvent_t ev : c->tev) (void)hipEventDestroy(ev);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return 0;
+}
+
+int tsph_grid(tsph_ctx *c, int *ncx, int *ncy) {
+  if (!c || !ncx || !ncy) return fail(TSPH_EINVAL, "null argument");
+  *ncx = c->k.ncx;
+  *ncy = c->k.ncy;
+  return 0;
+}
+
+int tsph_mass(tsph_ctx *c, double *m) {
+  if (!c || !m) return fail(TSPH_EINVAL, "null argument");
+  *m = c->k.m;
+  return 0;
+}
+
+int tsph_upload(tsph_ctx *c, const tsph_particles *h) {
+  if (!c || !h) return fail(TSPH_EINVAL, "null argument");
+  const int n = h->np;
+  if (n < 0) return fail(TSPH_EINVAL, "negative particle count");
+  if (n > c->capacity) return fail(TSPH_ECAPACITY, "particle count exceeds capacity");
+  if (!h->type || !h->rho || !h->r) return fail(TSPH_EINVAL, "type, rho and r are required");
+  int e = 0;
+  if ((e = ensure_extras(c, h->gnr != nullptr, h->n != nullptr))) return e;
+  TSPH_HIP(hipStreamSynchronize(c->stream));
+  drop_graph(c);
+  std::vector<uint8_t> t(n);
+  std::vector<uint32_t> seq(n);
+  std::vector<int32_t> idx(n);
+  for (int i = 0; i < n; ++i) {
+    if (h->type[i] < 0 || h->type[i] > 254) return fail(TSPH_EINVAL, "particle type out of range");
+    t[i] = (uint8_t)h->type[i];
+    seq[i] = (uint32_t)i;
+    idx[i] = h->index ? h->index[i] : i;
+  }
+  std::vector<double> zeros(2 * (size_t)n, 0.0);
+  const size_t s1 = sizeof(double) * n, s2 = 2 * s1;
+  Soa &d = c->cur;
+  auto cp = [&](void *dst, const void *src, size_t bytes) {
+    return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
+  };
+  const double *v = h->v ? h->v : zeros.data();
+  const double *p = h->p ? h->p : zeros.data();
+  if (n > 0) {
+    TSPH_HIP(cp(d.r, h->r, s2));
+    TSPH_HIP(cp(d.v, v, s2));
+    TSPH_HIP(cp(d.a, h->a ? h->a : zeros.data(), s2));
+    TSPH_HIP(cp(d.r_o, h->r_o ? h->r_o : h->r, s2));
+    TSPH_HIP(cp(d.v_o, h->v_o ? h->v_o : v, s2));
+    TSPH_HIP(cp(d.rho, h->rho, s1));
+    TSPH_HIP(cp(d.drho, h->drho ? h->drho : zeros.data(), s1));
+    TSPH_HIP(cp(d.p, p, s1));
+    TSPH_HIP(cp(d.rho_o, h->rho_o ? h->rho_o : h->rho, s1));
+    TSPH_HIP(cp(d.type, t.data(), n));
+    TSPH_HIP(cp(d.seq, seq.data(), 4 * (size_t)n));
+    TSPH_HIP(cp(d.index, idx.data(), 4 * (size_t)n));
+    if (h->gnr) TSPH_HIP(cp(d.gnr, h->gnr, s2));
+    else if (c->has_gnr) TSPH_HIP(cp(d.gnr, zeros.data(), s2));
+    if (h->n) TSPH_HIP(cp(d.n, h->n, s2));
+    else if (c->has_n) TSPH_HIP(cp(d.n, zeros.data(), s2));
+  }
+  int sc[S_NSCAL] = {n, n, 0, 0};
+  TSPH_HIP(hipMemcpy(c->scal, sc, sizeof sc, hipMemcpyHostToDevice));
+  c->n_ub = n;
+  c->cells_valid = false;
+  return 0;
+}
+
+int tsph_get_np(tsph_ctx *c, int *np) {
+  if (!c || !np) return fail(TSPH_EINVAL, "null argument");
+  TSPH_HIP(hipStreamSynchronize(c->stream));
+  TSPH_HIP(hipMemcpy(np, &c->scal[S_NP], sizeof(int), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+// order of the live slots in reference order (by seq), skipping removed ones
+static int live_order(tsph_ctx *c, int n, std::vector<int> &order, std::vector<uint32_t> &seq,
+                      std::vector<uint8_t> &type) {
+  seq.resize(n);
+  type.resize(n);
+  if (n > 0) {
+    TSPH_HIP(hipMemcpy(seq.data(), c->cur.seq, 4 * (size_t)n, hipMemcpyDeviceToHost));
+    TSPH_HIP(hipMemcpy(type.data(), c->cur.type, n, hipMemcpyDeviceToHost));
+  }
+  order.clear();
+  for (int i = 0; i < n; ++i)
+    if (type[i] != kRemoved) order.push_back(i);
+  std::sort(order.begin(), order.end(), [&](int a, int b) { return seq[a] < seq[b]; });
+  return 0;
+}
+
+int tsph_download(tsph_ctx *c, tsph_particles *h) {
+  if (!c || !h) return fail(TSPH_EINVAL, "null argument");
+  int n = 0, e = 0;
+  if ((e = tsph_get_np(c, &n))) return e;
+  std::vector<int> order;
+  std::vector<uint32_t> seq;
+  std::vector<uint8_t> type;
+  if ((e = live_order(c, n, order, seq, type))) return e;
+  const int m = (int)order.size();
+  h->np = m;
+  std::vector<double> b1(n), b2(2 * (size_t)n);
+  auto get1 = [&](double *dst, const double *src) -> int {
+    if (!dst) return 0;
+    TSPH_HIP(hipMemcpy(b1.data(), src, sizeof(double) * n, hipMemcpyDeviceToHost));
+    for (int k = 0; k < m; ++k) dst[k] = b1[order[k]];
+    return 0;
+  };
+  auto get2 = [&](double *dst, const double2 *src) -> int {
+    if (!dst) return 0;
+    if (!src) {
+      std::fill(dst, dst + 2 * (size_t)m, 0.0);
+      return 0;
+    }
+    TSPH_HIP(hipMemcpy(b2.data(), src, 2 * sizeof(double) * n, hipMemcpyDeviceToHost));
+    for (int k = 0; k < m; ++k) {
+      dst[2 * k] = b2[2 * order[k]];
+      dst[2 * k + 1] = b2[2 * order[k] + 1];
+    }
+    return 0;
+  };
+  if (n == 0) return 0;
+  const Soa &d = c->cur;
+  if ((e = get1(h->p, d.p)) || (e = get1(h->rho, d.rho)) || (e = get1(h->drho, d.drho)) ||
+      (e = get1(h->rho_o, d.rho_o)) || (e = get2(h->r, d.r)) || (e = get2(h->v, d.v)) ||
+      (e = get2(h->a, d.a)) || (e = get2(h->r_o, d.r_o)) || (e = get2(h->v_o, d.v_o)) ||
+      (e = get2(h->gnr, d.gnr)) || (e = get2(h->n, d.n)))
+    return e;
+  if (h->type)
+    for (int k = 0; k < m; ++k) h->type[k] = type[order[k]];
+  if (h->index) {
+    std::vector<int32_t> idx(n);
+    TSPH_HIP(hipMemcpy(idx.data(), d.index, 4 * (size_t)n, hipMemcpyDeviceToHost));
+    for (int k = 0; k < m; ++k) h->index[k] = idx[order[k]];
+  }
+  return 0;
+}
+
+int tsph_clear_cells(tsph_ctx *c) {
+  if (!c) return fail(TSPH_EINVAL, "null argument");
+  c->cells_valid = false;  // the cell list is rebuilt whole by tsph_particle_to_cell
+  return 0;
+}
+
+int tsph_particle_to_cell(tsph_ctx *c) {
+  if (!c) return fail(TSPH_EINVAL, "null argument");
+  return launch_cell_list(c);
+}
+
+int tsph_get_cell_lists(tsph_ctx *c, int32_t *start, int32_t *items) {
+  if (!c || !start || !items) return fail(TSPH_EINVAL, "null argument");
+  if (!c->cells_valid) return fail(TSPH_ESTATE, "no cell list: call tsph_particle_to_cell first");
+  int n = 0, e = 0;
+  if ((e = tsph_get_np(c, &n))) return e;
+  std::vector<int> order;
+  std::vector<uint32_t> seq;
+  std::vector<uint8_t> type;
+  if ((e = live_order(c, n, order, seq, type))) return e;
+  // reference index of slot = its rank in reference order
+  std::vector<int> rank(n, -1);
+  for (size_t k = 0; k < order.size(); ++k) rank[order[k]] = (int)k;
+  TSPH_HIP(hipMemcpy(start, c->cell_start, sizeof(int32_t) * (c->k.ncells + 1), hipMemcpyDeviceToHost));
+  for (int k = 0; k < n; ++k) items[k] = rank[k];
+  return 0;
+}
+
+int tsph_density_to_pressure(tsph_ctx *c) {
+  if (!c) return fail(TSPH_EINVAL, "null argument");
+  return launch_eos(c);
+}
+
+int tsph_compute_forces(tsph_ctx *c, int variant) {
+  if (!c) return fail(TSPH_EINVAL, "null argument");
+  if (variant < TSPH_FORCES_WCSPH || variant > TSPH_FORCES_BT) return fail(TSPH_EINVAL, "unknown variant");
+  if (!c->cells_valid) return fail(TSPH_ESTATE, "no cell list: call tsph_particle_to_cell first");
+  if (variant == TSPH_FORCES_BT && !c->has_n) return fail(TSPH_ESTATE, "BT forces need wall normals (n)");
+  return launch_forces(c, variant);
+}
+
+int tsph_mdbc_density(tsph_ctx *c) {
+  if (!c) return fail(TSPH_EINVAL, "null argument");
+  if (!c->cells_valid) return fail(TSPH_ESTATE, "no cell list: call tsph_particle_to_cell first");
+  if (!c->has_gnr) return fail(TSPH_ESTATE, "mDBC needs ghost nodes (gnr)");
+  return launch_mdbc(c);
+}
+
+int tsph_bt_density(tsph_ctx *c) {
+  if (!c) return fail(TSPH_EINVAL, "null argument");
+  if (!c->cells_valid) return fail(TSPH_ESTATE, "no cell list: call tsph_particle_to_cell first");
+  return launch_bt_density(c);
+}
+
+int tsph_symplectic_predictor(tsph_ctx *c, double dt) {
+  if (!c) return fail(TSPH_EINVAL, "null argument");
+  return launch_predictor(c, dt);
+}
+
+int tsph_symplectic_corrector(tsph_ctx *c, double dt) {
+  if (!c) return fail(TSPH_EINVAL, "null argument");
+  return launch_corrector(c, dt);
+}
+
+int tsph_remove_out_of_domain(tsph_ctx *c) {
+  if (!c) return fail(TSPH_EINVAL, "null argument");
+  return launch_remove(c);
+}
+
+int tsph_step_symplectic(tsph_ctx *c, int wall, double dt, int nsteps) {
+  if (!c) return fail(TSPH_EINVAL, "null argument");
+  if (wall < TSPH_WALL_DBC || wall > TSPH_WALL_BT) return fail(TSPH_EINVAL, "unknown wall treatment");
+  if (!c->cells_valid) return fail(TSPH_ESTATE, "no cell list: call tsph_particle_to_cell first");
+  if (wall == TSPH_WALL_MDBC && !c->has_gnr) return fail(TSPH_ESTATE, "mDBC needs ghost nodes (gnr)");
+  if (wall == TSPH_WALL_BT && !c->has_n) return fail(TSPH_ESTATE, "BT needs wall normals (n)");
+  if (nsteps <= 0) return 0;
+  if (c->time_pairs) {
+    // eager launches so that every pair kernel is bracketed by its own events
+    for (int s = 0; s < nsteps; ++s)
+      if (int e = record_step(c, wall, dt)) return e;
+    return 0;
+  }
+  Graph &g = c->graph;
+  if (!g.exec || g.wall != wall || g.dt != dt || g.n_ub != c->n_ub) {
+    drop_graph(c);
+    hipGraph_t graph = nullptr;
+    TSPH_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    const int e = record_step(c, wall, dt);
+    const hipError_t ce = hipStreamEndCapture(c->stream, &graph);
+    if (e) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return e;
+    }
+    TSPH_HIP(ce);
+    const hipError_t ie = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    TSPH_HIP(ie);
+    g.wall = wall;
+    g.dt = dt;
+    g.n_ub = c->n_ub;
+  }
+  for (int s = 0; s < nsteps; ++s) TSPH_HIP(hipGraphLaunch(g.exec, c->stream));
+  return 0;
+}
+
+int tsph_water_elevation(tsph_ctx *c, double x, double y0, int inverse, double *level) {
+  if (!c || !level) return fail(TSPH_EINVAL, "null argument");
+  if (!c->cells_valid) return fail(TSPH_ESTATE, "no cell list: call tsph_particle_to_cell first");
+  return run_water_elevation(c, x, y0, inverse, level);
+}
+
+int tsph_interpolate_point(tsph_ctx *c, double x, double y, double *out4) {
+  if (!c || !out4) return fail(TSPH_EINVAL, "null argument");
+  if (!c->cells_valid) return fail(TSPH_ESTATE, "no cell list: call tsph_particle_to_cell first");
+  return run_interpolate_point(c, x, y, out4);
+}
+
+int tsph_get_counters(tsph_ctx *c, tsph_counters *out) {
+  if (!c || !out) return fail(TSPH_EINVAL, "null argument");
+  unsigned long long v[C_NCOUNT];
+  TSPH_HIP(hipStreamSynchronize(c->stream));
+  TSPH_HIP(hipMemcpy(v, c->counters, sizeof v, hipMemcpyDeviceToHost));
+  out->pairs = v[C_PAIRS];
+  out->candidates = v[C_CAND];
+  out->active = v[C_ACTIVE];
+  out->pair_passes = v[C_PASSES];
+  out->density_out_of_range = v[C_BADRHO];
+  out->outside_grid = v[C_OUTGRID];
+  out->removed = v[C_REMOVED];
+  return 0;
+}
+
+int tsph_reset_counters(tsph_ctx *c) {
+  if (!c) return fail(TSPH_EINVAL, "null argument");
+  TSPH_HIP(hipStreamSynchronize(c->stream));
+  unsigned long long z[C_NCOUNT] = {};
+  TSPH_HIP(hipMemcpy(c->counters, z, sizeof z, hipMemcpyHostToDevice));
+  return 0;
+}
+
+int tsph_set_block_size(tsph_ctx *c, int threads) {
+  if (!c) return fail(TSPH_EINVAL, "null argument");
+  if (threads < 64 || threads > 256 || threads % 64) return fail(TSPH_EINVAL, "block size must be 64..256, multiple of 64");
+  if (threads != c->block) drop_graph(c);
+  c->block = threads;
+  return 0;
+}
+
+int tsph_synchronize(tsph_ctx *c) {
+  if (!c) return fail(TSPH_EINVAL, "null argument");
+  TSPH_HIP(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int tsph_pair_timing(tsph_ctx *c, int max_launches) {
+  if (!c) return fail(TSPH_EINVAL, "null argument");
+  TSPH_HIP(hipStreamSynchronize(c->stream));
+  if (max_launches <= 0) {
+    c->time_pairs = false;
+    c->tev_used = 0;
+    return 0;
+  }
+  while ((int)c->tev.size() < 2 * max_launches) {
+    hipEvent_t ev;
+    TSPH_HIP(hipEventCreate(&ev));
+    c->tev.push_back(ev);
+  }
+  c->tev_used = 0;
+  c->time_pairs = true;
+  return 0;
+}
+
+int tsph_pair_timing_read(tsph_ctx *c, double *total_ms, int *launches) {
+  if (!c || !total_ms || !launches) return fail(TSPH_EINVAL, "null argument");
+  TSPH_HIP(hipStreamSynchronize(c->stream));
+  double tot = 0;
+  for (int k = 0; k + 1 < c->tev_used; k += 2) {
+    float ms = 0.f;
+    TSPH_HIP(hipEventElapsedTime(&ms, c->tev[k], c->tev[k + 1]));
+    tot += ms;
+  }
+  *total_ms = tot;
+  *launches = c->tev_used / 2;
+  c->tev_used = 0;
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,191 @@
+// Case presets: the reference's case setup (constants + geometry + domain
+// padding of PREP_SIMULATION_DATA) built with the drop-in geometry routines,
+// exported as flat arrays through tsph_case_build.  Host code only.
+#include <cmath>
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+
+#include "../../include/tinysph_dropin.hpp"
+
+namespace {
+
+// SplitMix64 (Steele, Lea, Flood 2014): 64-bit state, golden-ratio increment
+struct SplitMix64 {
+  uint64_t s;
+  uint64_t next() {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+  }
+  double uniform() { return (double)(next() >> 11) * (1.0 / 9007199254740992.0); }  // [0,1)
+};
+
+struct Spec {
+  double dp, hh, kap, cs, visco, delta, dt;
+  double x_0, x_1, y_0, y_1;  // unpadded domain (constants.h)
+  int steps, wall;
+};
+
+// PREP_SIMULATION_DATA: pad the domain by nvl*dp on every side
+// (cases/damBreakLong_mDBC/SPH_simulationSymplectic.h:206-221, constants.h:23)
+Simulation_data padded(const Spec &s) {
+  Simulation_data sd(s.x_0, s.x_1, s.y_0, s.y_1);
+  const int nvl = std::ceil(s.kap * s.hh / s.dp);
+  sd.hh = s.hh;
+  sd.kh = s.hh * s.kap;
+  sd.nvl = nvl;
+  sd.x_0 -= nvl * s.dp;
+  sd.x_m += nvl * s.dp;
+  sd.y_0 -= nvl * s.dp;
+  sd.y_m += nvl * s.dp;
+  return sd;
+}
+
+template <typename T>
+T *dup(const std::vector<T> &v) {
+  T *p = (T *)std::malloc(sizeof(T) * (v.empty() ? 1 : v.size()));
+  if (!p) return nullptr;
+  if (!v.empty()) std::memcpy(p, v.data(), sizeof(T) * v.size());
+  return p;
+}
+
+std::vector<double> flat(const std::vector<realvec> &v) {
+  std::vector<double> o(2 * v.size());
+  for (size_t i = 0; i < v.size(); ++i) { o[2 * i] = v[i].x; o[2 * i + 1] = v[i].y; }
+  return o;
+}
+
+int export_case(const Spec &s, Particle_system &ps, tsph_case **out) {
+  const Simulation_data sd = padded(s);
+  tsph_case *c = (tsph_case *)std::calloc(1, sizeof(tsph_case));
+  if (!c) return TSPH_EINVAL;
+  c->prm.h = s.hh; c->prm.kap = s.kap; c->prm.cs = s.cs; c->prm.rho0 = ps.data_const.rho0;
+  c->prm.avisc = s.visco; c->prm.delta = s.delta; c->prm.dp = s.dp;
+  c->prm.x_0 = sd.x_0; c->prm.x_m = sd.x_m; c->prm.y_0 = sd.y_0; c->prm.y_m = sd.y_m;
+  c->prm.precision = TSPH_PREC_FP64;
+  c->prm.capacity = (int)ps.np;
+  c->dt = s.dt;
+  c->wall = s.wall;
+  c->steps = s.steps;
+  const int n = (int)ps.np;
+  std::vector<int32_t> type(n), index(n);
+  for (int i = 0; i < n; ++i) {
+    type[i] = ps.data.part_type[i];
+    index[i] = ps.data.index[i];
+    if (type[i] == fluid) c->n_fluid++;
+    if (type[i] == wall) c->n_wall++;
+  }
+  tsph_particles &h = c->particles;
+  h.np = n;
+  h.type = dup(type);
+  h.index = dup(index);
+  h.p = dup(ps.data.p);
+  h.rho = dup(ps.data.rho);
+  h.drho = dup(ps.data.drho);
+  h.r = dup(flat(ps.data.r));
+  h.v = dup(flat(ps.data.v));
+  h.a = dup(flat(ps.data.a));
+  h.r_o = dup(flat(ps.data.r_o));
+  h.v_o = dup(flat(ps.data.v_o));
+  h.rho_o = dup(ps.data.rho_o);
+  if (!ps.special.gnr.empty()) h.gnr = dup(tsph_expand_extra(ps, ps.special.gnr));
+  if (!ps.special.n.empty()) h.n = dup(tsph_expand_extra(ps, ps.special.n));
+  *out = c;
+  return 0;
+}
+
+void jitter_fluid(Particle_system &ps, double amp, uint64_t seed) {
+  if (!(amp > 0)) return;
+  SplitMix64 g{seed};
+  for (unsigned i = 0; i < ps.np; ++i) {
+    if (ps.data.part_type[i] != fluid) continue;
+    const double jx = (2.0 * g.uniform() - 1.0) * amp;
+    const double jy = (2.0 * g.uniform() - 1.0) * amp;
+    ps.data.r[i].x += jx;
+    ps.data.r[i].y += jy;
+    ps.data.r_o[i] = ps.data.r[i];
+  }
+}
+
+}  // namespace
+
+extern "C" int tsph_case_build(int preset, double dx, double jitter, uint64_t seed, tsph_case **out) {
+  if (!out) return TSPH_EINVAL;
+  *out = nullptr;
+  const double g = 9.81;
+  Spec s{};
+  switch (preset) {
+    case TSPH_CASE_STILL_WATER_DBC: {
+      // cases/stillWater_DBC/{constants,geometry}.h with a 1.0 m tank, 0.5 m of
+      // water, h = 1.5 dx, kap = 2, c0 = 10 sqrt(g H), dt = 0.2 h / c0
+      const double W = 1.0, H = 0.5;
+      s.dp = dx > 0 ? dx : 0.01;
+      s.hh = 1.5 * s.dp; s.kap = 2.0; s.cs = 10. * std::sqrt(g * H);
+      s.visco = 0.01; s.delta = 0.1; s.dt = 0.2 * s.hh / s.cs;
+      s.x_0 = 0.; s.x_1 = W + 0.2; s.y_0 = 0.; s.y_1 = 1.;
+      s.steps = 2000; s.wall = TSPH_WALL_DBC;
+      Particle_system ps(s.hh, s.kap, s.cs, 1000., s.visco, s.delta, s.dp);
+      const double dp = s.dp;
+      Create_Wall_x_mdbc(ps, 0. + dp, W - dp, 0., -1, dp / 2);
+      Create_Wall_y_mdbc(ps, 0., 0 + dp, 1., -1, dp / 2);
+      Create_Wall_y_mdbc(ps, W, 0 + dp, 1., +1, W - dp / 2);
+      Create_corner_mdbc(ps, -1, -1, {0. + dp / 2, 0. + dp / 2});
+      Create_corner_mdbc(ps, +1, -1, {W - dp / 2, 0. + dp / 2});
+      Create_Block_of_FluidWP(ps, 0 + dp, 0 + dp, W - dp, H);
+      jitter_fluid(ps, (jitter > 0 ? jitter : 0.0) * s.dp, seed);
+      return export_case(s, ps, out);
+    }
+    case TSPH_CASE_DAM_BREAK_MDBC: {
+      // cases/damBreakLong_mDBC/{constants,geometry}.h scaled to a 3.22 x 1.8 m
+      // tank and a 1.2 x 0.6 m column; h = sqrt(2) dx, c0 = 10 sqrt(g 0.6)
+      const double L = 3.22, Ht = 1.8, CW = 1.2, CH = 0.6;
+      s.dp = dx > 0 ? dx : 0.001;
+      s.hh = 1.0 * std::sqrt(2 * s.dp * s.dp); s.kap = 2.1; s.cs = 10. * std::sqrt(g * CH);
+      s.visco = 0.02; s.delta = 0.1; s.dt = 0.2 * s.hh / s.cs;
+      s.x_0 = 0.; s.x_1 = L + 0.1; s.y_0 = 0.; s.y_1 = Ht;
+      s.steps = 1000; s.wall = TSPH_WALL_MDBC;
+      Particle_system ps(s.hh, s.kap, s.cs, 1000., s.visco, s.delta, s.dp);
+      const double dp = s.dp;
+      Create_Wall_x_mdbc(ps, 0. + dp, L, 0., -1, dp / 2);
+      Create_Wall_y_mdbc(ps, 0., 0 + dp, Ht, -1, dp / 2);
+      Create_Wall_y_mdbc(ps, L, 0 + dp, Ht, +1, L - dp / 2);
+      Create_corner_mdbc(ps, -1, -1, {0. + dp / 2, 0. + dp / 2});
+      Create_corner_mdbc(ps, +1, -1, {L - dp / 2, 0. + dp / 2});
+      Create_Block_of_Fluid(ps, 0 + dp, 0 + dp, CW, CH);
+      jitter_fluid(ps, (jitter >= 0 ? jitter : 0.01) * s.dp, seed);
+      return export_case(s, ps, out);
+    }
+    case TSPH_CASE_DAM_BREAK_BT: {
+      // cases/damBreakLong_BT/{constants,geometry}.h
+      s.dp = dx > 0 ? dx : 0.005;
+      s.hh = 1.0 * std::sqrt(2 * s.dp * s.dp); s.kap = 2.1; s.cs = 34.3;
+      s.visco = 0.02; s.delta = 0.1; s.dt = 0.00004 * (s.dp / 0.005);
+      s.x_0 = 0.; s.x_1 = 2.; s.y_0 = 0.; s.y_1 = 1.;
+      s.steps = 1000; s.wall = TSPH_WALL_BT;
+      Particle_system ps(s.hh, s.kap, s.cs, 1000., s.visco, s.delta, s.dp);
+      const double dp = s.dp;
+      Create_Wall_x_bt(ps, 0. + dp, 1.61 - dp, 0., -1, {0., 1.});
+      Create_Wall_y_bt(ps, 0., 0. + dp, 1., -1, {1., 0.});
+      Create_Wall_y_bt(ps, 1.61, 0. + dp, 1., +1, {-1., 0.});
+      Create_corner_bt(ps, -1, -1, {0., 0.}, {std::sqrt(2), std::sqrt(2)});
+      Create_corner_bt(ps, +1, -1, {1.61, 0.}, {-std::sqrt(2), std::sqrt(2)});
+      Create_Block_of_FluidWP(ps, 0.0 + dp, 0 + dp, 0.6, 0.3);
+      jitter_fluid(ps, (jitter > 0 ? jitter : 0.0) * s.dp, seed);
+      return export_case(s, ps, out);
+    }
+    default:
+      return TSPH_EINVAL;
+  }
+}
+
+extern "C" int tsph_case_free(tsph_case *c) {
+  if (!c) return 0;
+  tsph_particles &h = c->particles;
+  void *ps[] = {h.type, h.index, h.p, h.rho, h.drho, h.r, h.v, h.a, h.r_o, h.v_o, h.rho_o, h.gnr, h.n};
+  for (void *p : ps) std::free(p);
+  std::free(c);
+  return 0;
+}

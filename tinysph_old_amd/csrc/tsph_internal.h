@@ -1,0 +1,128 @@
+// Internal declarations shared by the translation units of libtinysph_hip.so.
+// Device data layout (HBM), one SoA generation per field:
+//   r, v, a, r_o, v_o, gnr, n : double2   (16 B per particle each)
+//   rho, drho, p, rho_o       : double    ( 8 B)
+//   type                      : uint8     (reference part_type; 255 = removed)
+//   seq                       : uint32    (position in reference order; gaps allowed)
+//   index                     : int32     (reference data.index)
+// After every Particle_To_Cell the arrays are permuted into (cell, seq) order,
+// so a cell's particles are contiguous and ordered exactly as the reference
+// iterates them; `cellof[k]` holds the cell of slot k and `cell_start` the CSR
+// offsets.  The particle count lives on the device (`scal[S_NP]`) so that a
+// whole step can be captured in a HIP graph; the host keeps an upper bound.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/tinysph.h"
+
+namespace tsph {
+
+constexpr uint8_t kRemoved = 255;
+
+// device scalar slots
+enum { S_NP = 0, S_MAXSEQ = 1, S_NSCAL = 4 };
+// device counter slots (uint64)
+enum {
+  C_PAIRS = 0, C_CAND = 1, C_ACTIVE = 2, C_PASSES = 3, C_BADRHO = 4, C_OUTGRID = 5,
+  C_REMOVED = 6, C_NCOUNT = 8
+};
+
+struct Soa {
+  double2 *r = nullptr, *v = nullptr, *a = nullptr, *r_o = nullptr, *v_o = nullptr;
+  double2 *gnr = nullptr, *n = nullptr;
+  double *rho = nullptr, *drho = nullptr, *p = nullptr, *rho_o = nullptr;
+  uint8_t *type = nullptr;
+  uint32_t *seq = nullptr;
+  int32_t *index = nullptr;
+};
+
+// FP32 pair records of the mixed path, rebuilt after every reorder.
+struct Mixed {
+  float4 *rv = nullptr;   // {x - cell origin x, y - cell origin y, vx, vy}
+  float2 *rp = nullptr;   // {rho, p}
+  float2 *nrm = nullptr;  // BT wall normal
+};
+
+// constants of the physics, precomputed on the host exactly as the reference does
+struct Consts {
+  double h, kap, cs, rho0, avisc, delta, dp, m, gx, gy;
+  double x0, xm, y0, ym;
+  double kh, of;
+  double awen, bwen;  // float-rounded Wendland constants, src/SPH_kernel.h:56-57
+  int ncx, ncy, ncells;
+};
+
+struct Graph {
+  hipGraphExec_t exec = nullptr;
+  int wall = -1;
+  double dt = 0;
+  int n_ub = -1;
+  int precision = -1;
+};
+
+}  // namespace tsph
+
+struct tsph_ctx {
+  tsph_params prm;
+  tsph::Consts k;
+  int capacity = 0;
+  int n_ub = 0;  // host upper bound of the device particle count
+  bool has_gnr = false, has_n = false;
+  bool cells_valid = false;
+  int block = 256;
+  hipStream_t stream = nullptr;
+  tsph::Soa cur, alt;
+  tsph::Mixed mx;
+  int *scal = nullptr;                 // S_* slots
+  unsigned long long *counters = nullptr;
+  int *cell_count = nullptr;           // ncells+1 (last = removed bucket)
+  int *cell_start = nullptr;           // ncells+2
+  int *cellof = nullptr;               // cell of each slot (sorted order)
+  int *hrank = nullptr;                // rank inside the cell from the hash pass
+  int *hcell = nullptr;                // cell from the hash pass (pre-sort order)
+  int *perm = nullptr;                 // sorted position -> old slot
+  int *scan_tmp = nullptr;             // block partials of the scan
+  uint8_t *seqstate = nullptr;         // removal: per-seq state
+  double *probe_out = nullptr;         // 4 doubles for probe results
+  // pair-kernel timing: one event pair per pair launch while enabled
+  bool time_pairs = false;
+  std::vector<hipEvent_t> tev;         // 2 per recorded launch
+  int tev_used = 0;
+  float last_pair_ms = 0.f;
+  tsph::Graph graph;
+};
+
+namespace tsph {
+
+void set_error(const std::string &msg);
+int hip_fail(hipError_t e, const char *what, const char *file, int line);
+
+#define TSPH_HIP(call)                                                  \
+  do {                                                                  \
+    hipError_t e_ = (call);                                             \
+    if (e_ != hipSuccess) return tsph::hip_fail(e_, #call, __FILE__, __LINE__); \
+  } while (0)
+
+inline int grid_for(int n, int block) { return n <= 0 ? 1 : (n + block - 1) / block; }
+
+// tsph_cells.hip
+int launch_fill_i32(int *p, int value, int count, hipStream_t s);
+int launch_cell_list(tsph_ctx *c);  // hash, count, scan, scatter, order, reorder
+// tsph_pairs.hip
+int launch_forces(tsph_ctx *c, int variant);
+int launch_mixed_records(tsph_ctx *c);
+// tsph_stages.hip
+int launch_eos(tsph_ctx *c);
+int launch_predictor(tsph_ctx *c, double dt);
+int launch_corrector(tsph_ctx *c, double dt);
+int launch_mdbc(tsph_ctx *c);
+int launch_bt_density(tsph_ctx *c);
+int launch_remove(tsph_ctx *c);
+int run_water_elevation(tsph_ctx *c, double x, double y0, int inverse, double *out);
+int run_interpolate_point(tsph_ctx *c, double x, double y, double *out4);
+
+}  // namespace tsph

@@ -1,0 +1,104 @@
+// Device closures of the pair physics.  The FP64 forms keep the reference's
+// operand order and rounding points so that, compiled with -ffp-contract=off,
+// they produce the same bits as the reference for every IEEE operation
+// (only pow() comes from the device math library).  The FP32 forms are the
+// mixed-precision path.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include "tsph_internal.h"
+
+namespace tsph {
+
+constexpr double kEps = 0.01;  // src/SPH_defs.h:69
+
+// Particle_To_Cell cell coordinates: truncation toward zero (src/SPH_find_pairs.h:64-65)
+__device__ __forceinline__ int cell_coord(double x, double lo, double of, double kh) {
+  return (int)((x - lo + of) / kh);
+}
+
+// stencil offset s of the reference neighbour order {c, c+1, c+1+ncy, c+ncy,
+// c-1+ncy, c-1, c-1-ncy, c-ncy, c+1-ncy} (src/SPH_interaction.h:26-36)
+__device__ __forceinline__ int stencil_off(int s, int ncy) {
+  switch (s) {
+    case 0: return 0;
+    case 1: return 1;
+    case 2: return 1 + ncy;
+    case 3: return ncy;
+    case 4: return -1 + ncy;
+    case 5: return -1;
+    case 6: return -1 - ncy;
+    case 7: return -ncy;
+    default: return 1 - ncy;
+  }
+}
+// the same cell expressed as (dx, dy) of cell coordinates
+__device__ __forceinline__ void stencil_dxy(int s, int &dx, int &dy) {
+  const int tx[9] = {0, 0, 1, 1, 1, 0, -1, -1, -1};
+  const int ty[9] = {0, 1, 1, 0, -1, -1, -1, 0, 1};
+  dx = tx[s];
+  dy = ty[s];
+}
+
+// Wendland C2 in 2-D, src/SPH_kernel.h:52-77.  awen/bwen are the float-rounded
+// constants; qq, W and F follow the reference's casts exactly.
+struct KW64 { double w, f; };
+__device__ __forceinline__ KW64 wendland64(double r, double h, double awen, double bwen) {
+  const double qq = r / h;
+  const float wqq1 = (float)(1.0 - 0.5 * qq);
+  const float wqq2 = wqq1 * wqq1;
+  const float wqq = (float)(qq + qq + 1.0);
+  KW64 k{0.0, 0.0};
+  if (qq <= 2.0) {
+    k.w = awen * (double)wqq * (double)wqq2 * (double)wqq2;
+    k.f = bwen * qq * (double)wqq2 * (double)wqq1 / r;
+  }
+  if (r == 0.0) { k.w = awen; k.f = 0.0; }
+  return k;
+}
+
+// Artificial_Viscosity, src/SPH_viscouse_forces.h:7-25 (rhobar = 0.5*(rho_i+rho_j))
+__device__ __forceinline__ double art_visc64(double h, double drs, double drdv, double rhobar,
+                                             double c0, double alpha) {
+  double av = 0.0;
+  const double visco = drdv > alpha ? drdv : alpha;
+  if (drdv < 0) {
+    const double mu = h * drdv / (drs * drs + kEps * h * h);
+    av = -visco * c0 * mu / rhobar;
+  }
+  return av;
+}
+
+// Density_Diffusion_term_FOURTAKAS, src/SPH_viscouse_forces.h:45-61
+__device__ __forceinline__ double diff_fourtakas64(double h, double drs, double dr_z, double drdW,
+                                                   double arho, double nrho, double rho0,
+                                                   double c0, double delta, double m,
+                                                   double gravity) {
+  const double cb = c0 * c0 * rho0 / 7.;
+  const double dpH = 1. + dr_z * rho0 * gravity / cb;
+  const double drhoH = rho0 * pow(dpH, 1. / 7) - rho0;
+  const double psi = 2. * c0 * h * delta * ((nrho - arho) - drhoH) / (drs * drs + kEps * h * h);
+  return psi * drdW * m / nrho;
+}
+
+// Compute_Pressure (Tait, gamma = 7), src/SPH_pressure.h:7-19
+__device__ __forceinline__ double tait64(double rho, double rho0, double c0) {
+  const double b = c0 * c0 * rho0 / 7.;
+  const double rr = rho / rho0;
+  return b * (pow(rr, 7.) - 1);
+}
+
+// ---------------------------------------------------------------------------
+// FP32 forms for the mixed path.  Same formulas; divisions become reciprocal
+// multiplies and the hydrostatic (1+x)^(1/7)-1 of the Fourtakas term is
+// evaluated as a short series in x (|x| < 1e-2 for any case whose kernel
+// support is below ~0.1 m of water), which avoids the cancellation of
+// computing pow(1+x, 1/7) - 1 in single precision.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float hydro_drho_f32(float x, float rho0) {
+  // (1+x)^(1/7) - 1 = x/7 - 3x^2/49 + 13x^3/343 - 65x^4/2401 + ...  (binomial series)
+  const float t = x * (1.f / 7.f);
+  return rho0 * t * (1.f + x * (-3.f / 7.f + x * (13.f / 49.f + x * (-65.f / 343.f))));
+}
+
+}  // namespace tsph

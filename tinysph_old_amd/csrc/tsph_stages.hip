@@ -1,0 +1,392 @@
+// Per-particle streaming stages and the wall / probe gathers:
+//   Density_to_pressure            src/SPH_pressure.h:65-89
+//   Integrate_SymplecticPredictor  src/SPH_integration.h:266-287
+//   Integrate_SymplecticCorrector  src/SPH_integration.h:289-309
+//   mDBC_compute_density_mdbcGeo   src/SPH_mDBC.h:9-50 with
+//                                  Kernel_density_approximation_MATRIX_mDBC,
+//                                  src/SPH_kernel_approx.h:1141-1342
+//   BT_compute_density_BTGeo       src/SPH_BT.h:9-50 with
+//                                  Kernel_density_approximation_BT_update,
+//                                  src/SPH_kernel_approx.h:1838-1953
+//   RemoveParticlesOutOfDomain     src/SPH_btdebug.h:4-31
+//   Water_Elevation(_inverse)      src/SPH_water_level.h:6-150
+//   Kernel_velocity_approximation_TEST  src/testing.h:6-148
+// The streaming stages are HBM-bound (a few flops per 16-100 bytes).
+#include <cmath>
+#include <vector>
+
+#include "tsph_internal.h"
+#include "tsph_physics.h"
+
+namespace tsph {
+
+namespace {
+
+__global__ void k_eos(Soa d, const int *__restrict__ scal, Consts k,
+                      unsigned long long *__restrict__ counters) {
+  const int n = scal[S_NP];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned bad = 0;
+  if (i < n) {
+    const int t = d.type[i];
+    if (t != TSPH_OUTLET && t != TSPH_INLET) {
+      const double rho = d.rho[i];
+      if (rho > 1500. || rho < 0.) bad = 1;
+      d.p[i] = tait64(rho, k.rho0, k.cs);
+    }
+  }
+  const unsigned long long w = __ballot(bad);
+  if (w && (threadIdx.x & 63) == 0) atomicAdd(&counters[C_BADRHO], (unsigned long long)__popcll(w));
+}
+
+__global__ void k_predictor(Soa d, const int *__restrict__ scal, double dt) {
+  const int n = scal[S_NP];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double rho = d.rho[i];
+  const double2 r = d.r[i], v = d.v[i], a = d.a[i];
+  d.rho_o[i] = rho;
+  d.r_o[i] = r;
+  d.v_o[i] = v;
+  d.rho[i] = rho + d.drho[i] * dt * 0.5;
+  d.r[i] = make_double2(r.x + v.x * dt * 0.5, r.y + v.y * dt * 0.5);
+  d.v[i] = make_double2(v.x + a.x * dt * 0.5, v.y + a.y * dt * 0.5);
+}
+
+__global__ void k_corrector(Soa d, const int *__restrict__ scal, double dt) {
+  const int n = scal[S_NP];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double e = (-d.drho[i] / d.rho[i]) * dt;
+  d.rho[i] = d.rho_o[i] * ((2. - e) / (2. + e));
+  const double2 vo = d.v_o[i], a = d.a[i], ro = d.r_o[i];
+  const double2 v = make_double2(vo.x + a.x * dt, vo.y + a.y * dt);
+  d.v[i] = v;
+  d.r[i] = make_double2(ro.x + (v.x + vo.x) * dt * 0.5, ro.y + (v.y + vo.y) * dt * 0.5);
+}
+
+// cell of a sampling point, computed like Particle_To_Cell
+__device__ __forceinline__ int point_cell(const Consts &k, double x, double y) {
+  return cell_coord(y, k.y0, k.of, k.kh) + cell_coord(x, k.x0, k.of, k.kh) * k.ncy;
+}
+
+__device__ double mdbc_density_at(const Soa &d, const int *__restrict__ cell_start, const Consts &k,
+                                  double2 ar, double2 gnd) {
+  const double h = k.h, m = k.m;
+  double a11 = 0, a12 = 0, a13 = 0, a21 = 0, a22 = 0, a23 = 0, a31 = 0, a32 = 0, a33 = 0;
+  double drhodx = 0, drhody = 0, rhowsum = 0;
+  const double r2cut = 4.0 * h * h * (1.0 + 1e-12);
+  const int c = point_cell(k, ar.x, ar.y);
+  for (int s = 0; s < 9; ++s) {
+    const int cl = c + stencil_off(s, k.ncy);
+    if (cl < 0 || cl >= k.ncells) continue;
+    const int j1 = cell_start[cl + 1];
+    for (int j = cell_start[cl]; j < j1; ++j) {
+      if (d.type[j] != TSPH_FLUID) continue;
+      const double2 nr = d.r[j];
+      const double drx = ar.x - nr.x, dry = ar.y - nr.y;
+      const double r2 = drx * drx + dry * dry;
+      if (r2 > r2cut) continue;
+      const double drs = sqrt(r2);
+      if (drs / h > 2.0) continue;
+      const KW64 kw = wendland64(drs, h, k.awen, k.bwen);
+      const double nrho = d.rho[j];
+      rhowsum += m * kw.w;
+      drhodx += drx * kw.f * m;
+      drhody += dry * kw.f * m;
+      a11 += kw.w * (m / nrho);
+      a12 += drx * kw.w * (m / nrho);
+      a13 += dry * kw.w * (m / nrho);
+      a21 += drx * kw.f * (m / nrho);
+      a22 += drx * drx * kw.f * (m / nrho);
+      a23 += drx * dry * kw.f * (m / nrho);
+      a31 += dry * kw.f * (m / nrho);
+      a32 += dry * drx * kw.f * (m / nrho);
+      a33 += dry * dry * kw.f * (m / nrho);
+    }
+  }
+  // determinant, src/SPH_kernel_approx.h:663-675
+  const double det = (a11 * a22 * a33 + a12 * a23 * a31 + a13 * a21 * a32 - a31 * a22 * a13 -
+                      a32 * a23 * a11 - a33 * a21 * a12);
+  if (det > 0.001) {
+    const double i11 = (a22 * a33 - a23 * a32) / det;
+    const double i12 = -(a12 * a33 - a13 * a32) / det;
+    const double i13 = (a12 * a23 - a13 * a22) / det;
+    const double i21 = -(a21 * a33 - a23 * a31) / det;
+    const double i22 = (a11 * a33 - a13 * a31) / det;
+    const double i23 = -(a11 * a23 - a13 * a21) / det;
+    const double i31 = (a21 * a32 - a22 * a31) / det;
+    const double i32 = -(a11 * a32 - a12 * a31) / det;
+    const double i33 = (a11 * a22 - a12 * a21) / det;
+    const double dtemp = rhowsum * i11 + drhodx * i12 + drhody * i13;
+    const double grx = -(rhowsum * i21 + drhodx * i22 + drhody * i23);
+    const double gry = -(rhowsum * i31 + drhodx * i32 + drhody * i33);
+    return (dtemp + grx * gnd.x + gry * gnd.y);
+  } else if (a11 > 0) {
+    return rhowsum / a11;
+  }
+  return 1000.;
+}
+
+__global__ void k_mdbc(Soa d, const int *__restrict__ cell_start, const int *__restrict__ scal,
+                       Consts k) {
+  const int n = scal[S_NP];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || d.type[i] != TSPH_WALL) return;
+  const double2 gn = d.gnr[i], r = d.r[i];
+  const double2 gnd = make_double2(r.x - gn.x, r.y - gn.y);
+  double rho = mdbc_density_at(d, cell_start, k, gn, gnd);
+  if (rho < 1000.) rho = 1000.;
+  d.rho[i] = rho;
+}
+
+__global__ void k_bt_density(Soa d, const int *__restrict__ cell_start, const int *__restrict__ scal,
+                             Consts k) {
+  const int n = scal[S_NP];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || d.type[i] != TSPH_WALL) return;
+  const double h = k.h, m = k.m;
+  const double r2cut = 4.0 * h * h * (1.0 + 1e-12);
+  const double2 ar = d.r[i];
+  double wsum = 0, rhowsum = 0;
+  const int c = point_cell(k, ar.x, ar.y);
+  for (int s = 0; s < 9; ++s) {
+    const int cl = c + stencil_off(s, k.ncy);
+    if (cl < 0 || cl >= k.ncells) continue;
+    const int j1 = cell_start[cl + 1];
+    for (int j = cell_start[cl]; j < j1; ++j) {
+      if (d.type[j] != TSPH_FLUID) continue;
+      const double2 nr = d.r[j];
+      const double drx = ar.x - nr.x, dry = ar.y - nr.y;
+      const double r2 = drx * drx + dry * dry;
+      if (r2 > r2cut) continue;
+      const double drs = sqrt(r2);
+      if (drs / h > 2.0) continue;
+      const KW64 kw = wendland64(drs, h, k.awen, k.bwen);
+      wsum += kw.w * m / d.rho[j];
+      rhowsum += m * kw.w;
+    }
+  }
+  double rho = wsum > 0. ? rhowsum / wsum : 1000.;
+  if (rho < 1000.) rho = 1000.;
+  d.rho[i] = rho;
+}
+
+// Removal.  The reference erases inside an ascending loop without stepping
+// back, so the particle that follows an erased one in reference order is never
+// examined: in a run of consecutive candidates every other one (the 1st, 3rd,
+// ...) is removed.  State per reference position (seq): 0 absent, 1 kept,
+// 2 candidate.
+__global__ void k_rm_mark(Soa d, const int *__restrict__ scal, Consts k,
+                          uint8_t *__restrict__ state) {
+  const int n = scal[S_NP];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double2 r = d.r[i];
+  const bool out = d.type[i] == TSPH_FLUID &&
+                   ((r.y < (k.y0 - kEps)) || (r.y > k.ym + kEps) || (r.x < k.x0 - kEps) ||
+                    (r.x > k.xm - kEps));
+  state[d.seq[i]] = d.type[i] == kRemoved ? 0 : (out ? 2 : 1);
+}
+
+__global__ void k_rm_decide(Soa d, const int *__restrict__ scal, const uint8_t *__restrict__ state,
+                            unsigned long long *__restrict__ counters) {
+  const int n = scal[S_NP];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t s = d.seq[i];
+  if (state[s] != 2) return;
+  // count the candidates immediately before s in reference order (absent
+  // positions are skipped: they are particles removed earlier)
+  int run = 0;
+  for (int64_t j = (int64_t)s - 1; j >= 0; --j) {
+    const uint8_t st = state[j];
+    if (st == 0) continue;
+    if (st != 2) break;
+    ++run;
+  }
+  if ((run & 1) == 0) {
+    d.type[i] = kRemoved;
+    atomicAdd(&counters[C_REMOVED], 1ull);
+  }
+}
+
+__global__ void k_fill_u8(uint8_t *p, int count) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) p[i] = 0;
+}
+
+// Shepard volume sum, src/SPH_water_level.h:6-82 (all particle types)
+__device__ double sum_kernel_at(const Soa &d, const int *__restrict__ cell_start, const Consts &k,
+                                double x, double y) {
+  const double kh = k.kh, of = k.of;
+  const unsigned npx = (unsigned)(int)((x - k.x0 + of) / kh);
+  const unsigned npy = (unsigned)(int)((y - k.y0 + of) / kh);
+  const int c = (int)(npy + npx * (unsigned)k.ncy);
+  const double h = k.h, m = k.m;
+  double msum = 0;
+  for (int s = 0; s < 9; ++s) {
+    const int cl = c + stencil_off(s, k.ncy);
+    if (cl < 0 || cl >= k.ncells) continue;
+    const int j1 = cell_start[cl + 1];
+    for (int j = cell_start[cl]; j < j1; ++j) {
+      const double2 nr = d.r[j];
+      const double drx = x - nr.x, dry = y - nr.y;
+      const double drs = sqrt(drx * drx + dry * dry);
+      if (drs / h > 2.0) continue;
+      const KW64 kw = wendland64(drs, h, k.awen, k.bwen);
+      msum += m * kw.w / d.rho[j];
+    }
+  }
+  return msum;
+}
+
+// One block marches the sampling heights in chunks of blockDim: thread 0 forms
+// the cumulative heights of the chunk in the reference's sequential order
+// (ar.y += i*dp_wl), all threads evaluate their sample, the first sample that
+// meets the threshold ends the march.
+__global__ __launch_bounds__(256) void k_water_level(Soa d, const int *__restrict__ cell_start,
+                                                     Consts k, double x, double y0, int inverse,
+                                                     double *__restrict__ out) {
+  __shared__ double ys[256];
+  __shared__ int found;
+  const double dp_wl = k.dp * 0.00005;
+  const int nmax = int(k.ym / dp_wl);
+  double y = y0;
+  if (threadIdx.x == 0) { found = 0x7fffffff; *out = NAN; }
+  __syncthreads();
+  for (int base = 0; base < nmax; base += blockDim.x) {
+    if (threadIdx.x == 0) {
+      for (int t = 0; t < (int)blockDim.x; ++t) {
+        const int i = base + t;
+        if (inverse) y -= i * dp_wl; else y += i * dp_wl;
+        ys[t] = y;
+      }
+    }
+    __syncthreads();
+    const int i = base + threadIdx.x;
+    if (i < nmax) {
+      const double msum = sum_kernel_at(d, cell_start, k, x, ys[threadIdx.x]);
+      const bool hit = inverse ? (msum >= 0.5) : (msum <= 0.5);
+      if (hit) atomicMin(&found, i);
+    }
+    __syncthreads();
+    if (found != 0x7fffffff) {
+      if (threadIdx.x == 0) {
+        const int t = found - base;
+        *out = inverse ? ys[t] + found * dp_wl : ys[t] - found * dp_wl;
+      }
+      return;
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void k_interp_point(Soa d, const int *__restrict__ cell_start, Consts k, double x,
+                               double y, double *__restrict__ out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const double kh = k.kh, of = k.of;
+  const unsigned npx = (unsigned)(int)((x - k.x0 + of) / kh);
+  const unsigned npy = (unsigned)(int)((y - k.y0 + of) / kh);
+  const int c = (int)(npy + npx * (unsigned)k.ncy);
+  double vwx = 0, vwy = 0, wsum = 0, rhow = 0, pw = 0, mw = 0;
+  for (int s = 0; s < 9; ++s) {
+    const int cl = c + stencil_off(s, k.ncy);
+    if (cl < 0 || cl >= k.ncells) continue;
+    const int j1 = cell_start[cl + 1];
+    for (int j = cell_start[cl]; j < j1; ++j) {
+      const double2 nr = d.r[j];
+      const double drx = x - nr.x, dry = y - nr.y;
+      const double drs = sqrt(drx * drx + dry * dry);
+      if (drs / k.h > 2.0) continue;
+      const KW64 kw = wendland64(drs, k.h, k.awen, k.bwen);
+      const double2 nv = d.v[j];
+      wsum += kw.w;
+      vwx = vwx + nv.x * kw.w;
+      vwy = vwy + nv.y * kw.w;
+      rhow = rhow + d.rho[j] * kw.w;
+      pw = pw + d.p[j] * kw.w;
+      mw += k.m * kw.w / d.rho[j];
+    }
+  }
+  double rho = 0, p = 0, vx = 0, vy = 0;
+  if (mw > 0.5) {
+    vx = 0. + vwx / wsum;
+    vy = 0. + vwy / wsum;
+    rho = 0 + rhow / wsum;
+    p = 0 + pw / wsum;
+  }
+  if (wsum == 0) { vx = 0.; vy = 0.; }
+  out[0] = rho; out[1] = p; out[2] = vx; out[3] = vy;
+}
+
+}  // namespace
+
+int launch_eos(tsph_ctx *c) {
+  hipLaunchKernelGGL(k_eos, dim3(grid_for(c->n_ub, 256)), dim3(256), 0, c->stream, c->cur, c->scal,
+                     c->k, c->counters);
+  TSPH_HIP(hipGetLastError());
+  return 0;
+}
+
+int launch_predictor(tsph_ctx *c, double dt) {
+  hipLaunchKernelGGL(k_predictor, dim3(grid_for(c->n_ub, 256)), dim3(256), 0, c->stream, c->cur,
+                     c->scal, dt);
+  TSPH_HIP(hipGetLastError());
+  return 0;
+}
+
+int launch_corrector(tsph_ctx *c, double dt) {
+  hipLaunchKernelGGL(k_corrector, dim3(grid_for(c->n_ub, 256)), dim3(256), 0, c->stream, c->cur,
+                     c->scal, dt);
+  TSPH_HIP(hipGetLastError());
+  return 0;
+}
+
+int launch_mdbc(tsph_ctx *c) {
+  hipLaunchKernelGGL(k_mdbc, dim3(grid_for(c->n_ub, 256)), dim3(256), 0, c->stream, c->cur,
+                     c->cell_start, c->scal, c->k);
+  TSPH_HIP(hipGetLastError());
+  return 0;
+}
+
+int launch_bt_density(tsph_ctx *c) {
+  hipLaunchKernelGGL(k_bt_density, dim3(grid_for(c->n_ub, 256)), dim3(256), 0, c->stream, c->cur,
+                     c->cell_start, c->scal, c->k);
+  TSPH_HIP(hipGetLastError());
+  return 0;
+}
+
+int launch_remove(tsph_ctx *c) {
+  // seq values are < capacity (reference positions of the current run)
+  const int ns = c->capacity;
+  hipLaunchKernelGGL(k_fill_u8, dim3(grid_for(ns, 256)), dim3(256), 0, c->stream, c->seqstate, ns);
+  hipLaunchKernelGGL(k_rm_mark, dim3(grid_for(c->n_ub, 256)), dim3(256), 0, c->stream, c->cur,
+                     c->scal, c->k, c->seqstate);
+  hipLaunchKernelGGL(k_rm_decide, dim3(grid_for(c->n_ub, 256)), dim3(256), 0, c->stream, c->cur,
+                     c->scal, c->seqstate, c->counters);
+  TSPH_HIP(hipGetLastError());
+  return 0;
+}
+
+int run_water_elevation(tsph_ctx *c, double x, double y0, int inverse, double *out) {
+  double *dout = c->probe_out;
+  hipLaunchKernelGGL(k_water_level, dim3(1), dim3(256), 0, c->stream, c->cur, c->cell_start, c->k,
+                     x, y0, inverse, dout);
+  TSPH_HIP(hipGetLastError());
+  TSPH_HIP(hipMemcpyAsync(out, dout, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  TSPH_HIP(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int run_interpolate_point(tsph_ctx *c, double x, double y, double *out4) {
+  double *dout = c->probe_out;
+  hipLaunchKernelGGL(k_interp_point, dim3(1), dim3(64), 0, c->stream, c->cur, c->cell_start, c->k,
+                     x, y, dout);
+  TSPH_HIP(hipGetLastError());
+  TSPH_HIP(hipMemcpyAsync(out4, dout, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  TSPH_HIP(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+}  // namespace tsph

@@ -1,0 +1,258 @@
+"""Python host layer over the C ABI: case presets and a device-resident
+particle system whose methods are the reference stage functions
+(src/SPH_find_pairs.h, src/SPH_pressure.h, src/SPH_interaction*.h,
+src/SPH_mDBC.h, src/SPH_BT.h, src/SPH_integration.h, src/SPH_btdebug.h,
+src/SPH_water_level.h, src/SPH_measure_pressure.h).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib as L
+
+
+@dataclass
+class CaseData:
+    """A case preset: constants, padded domain, time step and particles in
+    reference order (vectors as (np, 2) arrays)."""
+
+    params: dict
+    dt: float
+    wall: int
+    steps: int
+    n_fluid: int
+    n_wall: int
+    arrays: dict = field(default_factory=dict)
+
+    @property
+    def np(self) -> int:
+        return len(self.arrays["type"])
+
+    def oracle_params(self) -> list:
+        p = self.params
+        return [p["h"], p["kap"], p["cs"], p["rho0"], p["avisc"], p["delta"], p["dp"],
+                p["x_0"], p["x_m"], p["y_0"], p["y_m"]]
+
+
+def build_case(preset: int, dx: float = 0.0, jitter: float = -1.0, seed: int = 12345) -> CaseData:
+    lib = L.lib()
+    out = C.POINTER(L.CaseC)()
+    L.check(lib.tsph_case_build(preset, float(dx), float(jitter), int(seed), C.byref(out)),
+            "tsph_case_build")
+    try:
+        c = out.contents
+        prm = {n: getattr(c.prm, n) for n, _ in L.Params._fields_}
+        h = c.particles
+        n = h.np
+
+        def arr(ptr, count, dtype):
+            if not ptr:
+                return None
+            return np.ctypeslib.as_array(ptr, shape=(count,)).astype(dtype, copy=True)
+
+        arrays = {
+            "type": arr(h.type, n, np.int32),
+            "index": arr(h.index, n, np.int32),
+            "p": arr(h.p, n, np.float64),
+            "rho": arr(h.rho, n, np.float64),
+            "r": arr(h.r, 2 * n, np.float64).reshape(n, 2),
+            "v": arr(h.v, 2 * n, np.float64).reshape(n, 2),
+        }
+        if h.gnr:
+            arrays["gnr"] = arr(h.gnr, 2 * n, np.float64).reshape(n, 2)
+        if h.n:
+            arrays["n"] = arr(h.n, 2 * n, np.float64).reshape(n, 2)
+        return CaseData(prm, c.dt, c.wall, c.steps, c.n_fluid, c.n_wall, arrays)
+    finally:
+        lib.tsph_case_free(out)
+
+
+VEC = ("r", "v", "a", "r_o", "v_o", "gnr", "n")
+SCAL = ("p", "rho", "drho", "rho_o")
+
+
+class ParticleSystem:
+    """Particle_system + Simulation_data on one GPU (src/SPH_particle_system.h:7-213)."""
+
+    def __init__(self, params: dict, precision: int = L.PREC_FP64, capacity: int | None = None):
+        self.lib = L.lib()
+        prm = L.Params()
+        for n, _ in L.Params._fields_:
+            if n in params and n not in ("precision", "capacity"):
+                setattr(prm, n, float(params[n]))
+        prm.precision = precision
+        prm.capacity = int(capacity if capacity is not None else params.get("capacity", 1))
+        self.capacity = prm.capacity
+        self.precision = precision
+        self.ctx = C.c_void_p()
+        L.check(self.lib.tsph_create(C.byref(prm), C.byref(self.ctx)), "tsph_create")
+
+    def close(self):
+        if self.ctx:
+            self.lib.tsph_destroy(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---- transfer ----
+    def upload(self, arrays: dict) -> None:
+        n = len(arrays["type"])
+        keep = {}
+        h = L.Particles()
+        h.np = n
+        t = np.ascontiguousarray(arrays["type"], dtype=np.int32)
+        keep["type"] = t
+        h.type = L.iptr(t)
+        if arrays.get("index") is not None:
+            ix = np.ascontiguousarray(arrays["index"], dtype=np.int32)
+            keep["index"] = ix
+            h.index = L.iptr(ix)
+        for k in VEC + SCAL:
+            a = arrays.get(k)
+            if a is None:
+                continue
+            a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+            keep[k] = a
+            setattr(h, k, L.dptr(a))
+        L.check(self.lib.tsph_upload(self.ctx, C.byref(h)), "tsph_upload")
+
+    def download(self, fields=("type", "index", "p", "rho", "drho", "r", "v", "a", "r_o", "v_o",
+                               "rho_o")) -> dict:
+        n = self.np_upper()
+        out = {}
+        h = L.Particles()
+        for k in fields:
+            if k in ("type", "index"):
+                a = np.zeros(max(n, 1), dtype=np.int32)
+                setattr(h, k, L.iptr(a))
+            elif k in VEC:
+                a = np.zeros(max(2 * n, 1), dtype=np.float64)
+                setattr(h, k, L.dptr(a))
+            else:
+                a = np.zeros(max(n, 1), dtype=np.float64)
+                setattr(h, k, L.dptr(a))
+            out[k] = a
+        L.check(self.lib.tsph_download(self.ctx, C.byref(h)), "tsph_download")
+        m = h.np
+        for k in list(out):
+            out[k] = out[k][: 2 * m].reshape(m, 2) if k in VEC else out[k][:m]
+        return out
+
+    def np_upper(self) -> int:
+        return self.np
+
+    @property
+    def np(self) -> int:
+        n = C.c_int()
+        L.check(self.lib.tsph_get_np(self.ctx, C.byref(n)), "tsph_get_np")
+        return n.value
+
+    def grid(self):
+        x, y = C.c_int(), C.c_int()
+        L.check(self.lib.tsph_grid(self.ctx, C.byref(x), C.byref(y)), "tsph_grid")
+        return x.value, y.value
+
+    @property
+    def mass(self) -> float:
+        m = C.c_double()
+        L.check(self.lib.tsph_mass(self.ctx, C.byref(m)), "tsph_mass")
+        return m.value
+
+    # ---- stages ----
+    def clear_cells(self):
+        L.check(self.lib.tsph_clear_cells(self.ctx), "Clear_cells")
+
+    def particle_to_cell(self):
+        L.check(self.lib.tsph_particle_to_cell(self.ctx), "Particle_To_Cell")
+
+    def cell_lists(self):
+        ncx, ncy = self.grid()
+        n = self.np
+        start = np.zeros(ncx * ncy + 1, dtype=np.int32)
+        items = np.zeros(max(n, 1), dtype=np.int32)
+        L.check(self.lib.tsph_get_cell_lists(self.ctx, L.iptr(start), L.iptr(items)),
+                "tsph_get_cell_lists")
+        return start, items[:n]
+
+    def density_to_pressure(self):
+        L.check(self.lib.tsph_density_to_pressure(self.ctx), "Density_to_pressure")
+
+    def compute_forces(self, variant: int):
+        L.check(self.lib.tsph_compute_forces(self.ctx, variant), "Compute_Forces")
+
+    def mdbc_density(self):
+        L.check(self.lib.tsph_mdbc_density(self.ctx), "mDBC_compute_density_mdbcGeo")
+
+    def bt_density(self):
+        L.check(self.lib.tsph_bt_density(self.ctx), "BT_compute_density_BTGeo")
+
+    def predictor(self, dt: float):
+        L.check(self.lib.tsph_symplectic_predictor(self.ctx, dt), "Integrate_SymplecticPredictor")
+
+    def corrector(self, dt: float):
+        L.check(self.lib.tsph_symplectic_corrector(self.ctx, dt), "Integrate_SymplecticCorrector")
+
+    def remove_out_of_domain(self):
+        L.check(self.lib.tsph_remove_out_of_domain(self.ctx), "RemoveParticlesOutOfDomain")
+
+    def step(self, wall: int, dt: float, nsteps: int = 1):
+        L.check(self.lib.tsph_step_symplectic(self.ctx, wall, dt, nsteps), "tsph_step_symplectic")
+
+    def synchronize(self):
+        L.check(self.lib.tsph_synchronize(self.ctx), "tsph_synchronize")
+
+    def water_elevation(self, x: float, y0: float, inverse: bool = False) -> float:
+        out = C.c_double()
+        L.check(self.lib.tsph_water_elevation(self.ctx, x, y0, int(inverse), C.byref(out)),
+                "Water_Elevation")
+        return out.value
+
+    def interpolate_point(self, x: float, y: float) -> np.ndarray:
+        out = np.zeros(4)
+        L.check(self.lib.tsph_interpolate_point(self.ctx, x, y, L.dptr(out)), "Pressure_in_point")
+        return out
+
+    def counters(self) -> dict:
+        c = L.Counters()
+        L.check(self.lib.tsph_get_counters(self.ctx, C.byref(c)), "tsph_get_counters")
+        return {n: int(getattr(c, n)) for n, _ in L.Counters._fields_}
+
+    def reset_counters(self):
+        L.check(self.lib.tsph_reset_counters(self.ctx), "tsph_reset_counters")
+
+    def set_block_size(self, threads: int):
+        L.check(self.lib.tsph_set_block_size(self.ctx, threads), "tsph_set_block_size")
+
+    def pair_timing(self, max_launches: int):
+        L.check(self.lib.tsph_pair_timing(self.ctx, max_launches), "tsph_pair_timing")
+
+    def pair_timing_read(self):
+        ms, n = C.c_double(), C.c_int()
+        L.check(self.lib.tsph_pair_timing_read(self.ctx, C.byref(ms), C.byref(n)),
+                "tsph_pair_timing_read")
+        return ms.value, n.value
+
+
+def system_for_case(case: CaseData, precision: int = L.PREC_FP64, capacity: int | None = None
+                    ) -> ParticleSystem:
+    ps = ParticleSystem(case.params, precision=precision,
+                        capacity=capacity if capacity is not None else case.np)
+    ps.upload(case.arrays)
+    return ps
+
+
+def forces_variant(wall: int) -> int:
+    return {L.WALL_DBC: L.FORCES_DBC, L.WALL_MDBC: L.FORCES_WCSPH, L.WALL_BT: L.FORCES_BT}[wall]
