@@ -151,8 +151,13 @@ int tsph_interpolate_point(tsph_ctx *ctx, double x, double y, double *out4);
 
 int tsph_get_counters(tsph_ctx *ctx, tsph_counters *out);
 int tsph_reset_counters(tsph_ctx *ctx);
-/* pair-kernel launch shape: threads per block (64..256, multiple of 64) */
-int tsph_set_block_size(tsph_ctx *ctx, int threads);
+/* Pair-loop tile: one wavefront handles `rows` consecutive cells of a grid
+ * column (1..24, default 7). */
+int tsph_set_tile_rows(tsph_ctx *ctx, int rows);
+int tsph_get_tile_rows(tsph_ctx *ctx, int *rows);
+/* Pair-loop waves kept resident per compute unit (1..32, default 8); the
+ * waves walk the list of non-empty tiles. */
+int tsph_set_waves_per_cu(tsph_ctx *ctx, int waves);
 /* wait for all work queued on the context */
 int tsph_synchronize(tsph_ctx *ctx);
 

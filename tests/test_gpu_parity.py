@@ -188,7 +188,37 @@ def test_forces_mixed(name):
         got = ps.download()
     act = active_mask(ref["type"], wall)
     assert rel_l2(got["a"][act], ref["a"][act]) <= 1e-4
-    assert rel_l2(got["drho"][act], ref["drho"][act]) <= 1e-4
+    # BT on the hydrostatic start: dRho is a small remainder of the diffusion
+    # term's (rho_j - rho_i) - drho_H cancellation (|dRho| ~ 1e-2 against terms
+    # of ~1e-1 per pair), so its FP32 relative error is ~10x the others'
+    drho_tol = 1e-3 if wall == T.WALL_BT else 1e-4
+    assert rel_l2(got["drho"][act], ref["drho"][act]) <= drho_tol
+
+
+@pytest.mark.parametrize("precision", [T.PREC_FP64, T.PREC_MIXED])
+def test_tile_size_does_not_change_results(precision):
+    """The per-particle summation order does not depend on the tile shape, so
+    every tile size (including the LDS-overflow fallback of 24-row tiles in
+    dense columns) gives identical bits."""
+    case = make("dam_break_mdbc")
+    outs = []
+    for rows in (1, 3, 7, 24):
+        with T.system_for_case(case, precision=precision) as ps:
+            ps.set_tile_rows(rows)
+            ps.particle_to_cell()
+            ps.mdbc_density()
+            ps.density_to_pressure()
+            ps.compute_forces(T.FORCES_WCSPH)
+            outs.append(ps.download(("a", "drho")))
+    for o in outs[1:]:
+        if precision == T.PREC_FP64:
+            assert np.array_equal(o["a"], outs[0]["a"])
+            assert np.array_equal(o["drho"], outs[0]["drho"])
+        else:
+            # the mixed path stages positions relative to the tile, so the
+            # FP32 rounding of dr (not the summation order) moves with it
+            assert rel_l2(o["a"], outs[0]["a"]) <= 1e-5
+            assert rel_l2(o["drho"], outs[0]["drho"]) <= 1e-5
 
 
 def test_water_level_and_probe():

@@ -73,7 +73,9 @@ EXPORTS = {
     "tsph_interpolate_point": (C.c_int, [C.c_void_p, C.c_double, C.c_double, _dp]),
     "tsph_get_counters": (C.c_int, [C.c_void_p, C.POINTER(Counters)]),
     "tsph_reset_counters": (C.c_int, [C.c_void_p]),
-    "tsph_set_block_size": (C.c_int, [C.c_void_p, C.c_int]),
+    "tsph_set_tile_rows": (C.c_int, [C.c_void_p, C.c_int]),
+    "tsph_get_tile_rows": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
+    "tsph_set_waves_per_cu": (C.c_int, [C.c_void_p, C.c_int]),
     "tsph_synchronize": (C.c_int, [C.c_void_p]),
     "tsph_pair_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "tsph_pair_timing_read": (C.c_int, [C.c_void_p, _dp, C.POINTER(C.c_int)]),

@@ -116,6 +116,7 @@ int tsph_create(const tsph_params *prm, tsph_ctx **out) {
   tsph_ctx *c = new tsph_ctx();
   c->prm = *prm;
   c->capacity = prm->capacity;
+  c->tile_rows = 7;
   Consts &k = c->k;
   k.h = prm->h; k.kap = prm->kap; k.cs = prm->cs; k.rho0 = prm->rho0; k.avisc = prm->avisc;
   k.delta = prm->delta; k.dp = prm->dp;
@@ -148,8 +149,16 @@ int tsph_create(const tsph_params *prm, tsph_ctx **out) {
       (e = dalloc(&c->cell_count, k.ncells + 1)) || (e = dalloc(&c->cell_start, k.ncells + 2)) ||
       (e = dalloc(&c->cellof, cap)) || (e = dalloc(&c->hrank, cap)) || (e = dalloc(&c->hcell, cap)) ||
       (e = dalloc(&c->perm, cap)) || (e = dalloc(&c->scan_tmp, (k.ncells + 1) / 1024 + 2)) ||
-      (e = dalloc(&c->seqstate, cap)) || (e = dalloc(&c->probe_out, 4)))
+      (e = dalloc(&c->seqstate, cap)) || (e = dalloc(&c->probe_out, 4)) ||
+      (e = dalloc(&c->tile_list, k.ncells + 8)))
     return bail(e);
+  {
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess &&
+        prop.multiProcessorCount > 0)
+      c->num_cu = prop.multiProcessorCount;
+  }
   if (prm->precision == TSPH_PREC_MIXED) {
     if ((e = dalloc(&c->mx.rv, cap)) || (e = dalloc(&c->mx.rp, cap))) return bail(e);
   }
@@ -169,7 +178,7 @@ int tsph_destroy(tsph_ctx *c) {
   free_soa(c->alt);
   void *ps[] = {c->scal, c->counters, c->cell_count, c->cell_start, c->cellof, c->hrank,
                 c->hcell, c->perm, c->scan_tmp, c->seqstate, c->probe_out, c->mx.rv,
-                c->mx.rp, c->mx.nrm};
+                c->mx.rp, c->mx.nrm, c->tile_list};
   for (void *p : ps)
     if (p) (void)hipFree(p);
   for (hipEvent_t ev : c->tev) (void)hipEventDestroy(ev);
@@ -240,6 +249,8 @@ int tsph_upload(tsph_ctx *c, const tsph_particles *h) {
   TSPH_HIP(hipMemcpy(c->scal, sc, sizeof sc, hipMemcpyHostToDevice));
   c->n_ub = n;
   c->cells_valid = false;
+  c->tiles_valid = false;
+  c->tiles_zeroed = true;  // the upload above zeroed the device tile counter
   return 0;
 }
 
@@ -453,11 +464,28 @@ int tsph_reset_counters(tsph_ctx *c) {
   return 0;
 }
 
-int tsph_set_block_size(tsph_ctx *c, int threads) {
+int tsph_set_tile_rows(tsph_ctx *c, int rows) {
   if (!c) return fail(TSPH_EINVAL, "null argument");
-  if (threads < 64 || threads > 256 || threads % 64) return fail(TSPH_EINVAL, "block size must be 64..256, multiple of 64");
-  if (threads != c->block) drop_graph(c);
-  c->block = threads;
+  if (rows < 1 || rows > 24) return fail(TSPH_EINVAL, "tile rows must be 1..24");
+  if (rows != c->tile_rows) {
+    drop_graph(c);
+    c->tiles_valid = false;
+  }
+  c->tile_rows = rows;
+  return 0;
+}
+
+int tsph_set_waves_per_cu(tsph_ctx *c, int waves) {
+  if (!c) return fail(TSPH_EINVAL, "null argument");
+  if (waves < 1 || waves > 32) return fail(TSPH_EINVAL, "waves per CU must be 1..32");
+  if (waves != c->waves_per_cu) drop_graph(c);
+  c->waves_per_cu = waves;
+  return 0;
+}
+
+int tsph_get_tile_rows(tsph_ctx *c, int *rows) {
+  if (!c || !rows) return fail(TSPH_EINVAL, "null argument");
+  *rows = c->tile_rows;
   return 0;
 }
 

@@ -186,7 +186,10 @@ __global__ void k_reorder(const int *__restrict__ cell_start, int ncells,
 }
 
 __global__ void k_set_np(const int *__restrict__ cell_start, int ncells, int *__restrict__ scal) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) scal[S_NP] = cell_start[ncells];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    scal[S_NP] = cell_start[ncells];
+    scal[S_NTILES] = 0;  // the pair-loop tile list is rebuilt for the new cell list
+  }
 }
 
 int scan_exclusive(const int *in, int len, int *out, int *partial, hipStream_t s) {
@@ -228,6 +231,8 @@ int launch_cell_list(tsph_ctx *c) {
   TSPH_HIP(hipGetLastError());
   std::swap(c->cur, c->alt);
   c->cells_valid = true;
+  c->tiles_valid = false;
+  c->tiles_zeroed = true;
   return 0;
 }
 

@@ -24,7 +24,7 @@ namespace tsph {
 constexpr uint8_t kRemoved = 255;
 
 // device scalar slots
-enum { S_NP = 0, S_MAXSEQ = 1, S_NSCAL = 4 };
+enum { S_NP = 0, S_MAXSEQ = 1, S_NTILES = 2, S_NSCAL = 4 };
 // device counter slots (uint64)
 enum {
   C_PAIRS = 0, C_CAND = 1, C_ACTIVE = 2, C_PASSES = 3, C_BADRHO = 4, C_OUTGRID = 5,
@@ -43,7 +43,7 @@ struct Soa {
 // FP32 pair records of the mixed path, rebuilt after every reorder.
 struct Mixed {
   float4 *rv = nullptr;   // {x - cell origin x, y - cell origin y, vx, vy}
-  float2 *rp = nullptr;   // {rho, p}
+  float2 *rp = nullptr;   // {rho - rho0, p}
   float2 *nrm = nullptr;  // BT wall normal
 };
 
@@ -73,7 +73,12 @@ struct tsph_ctx {
   int n_ub = 0;  // host upper bound of the device particle count
   bool has_gnr = false, has_n = false;
   bool cells_valid = false;
-  int block = 256;
+  int tile_rows = 16;  // cells per pair-loop tile (one grid-column run)
+  int waves_per_cu = 8;  // persistent pair-loop waves per compute unit
+  int num_cu = 256;      // compute units of the device
+  int *tile_list = nullptr;  // non-empty tiles of the current cell list
+  bool tiles_valid = false;  // tile_list matches the cell list and tile_rows
+  bool tiles_zeroed = false; // the device tile counter is 0 (set by the cell-list build)
   hipStream_t stream = nullptr;
   tsph::Soa cur, alt;
   tsph::Mixed mx;

@@ -5,26 +5,56 @@
 //   TSPH_FORCES_BT     src/SPH_interactionBT.h:8-192  (wall neighbours enter as a
 //                      boundary integral with normal n; Shepard gamma normalises)
 //
-// One thread per particle of the (cell, seq)-sorted arrays, so a 64-wide
-// wavefront covers a run of consecutive cells of one grid column and its 9-cell
-// stencils overlap almost completely: neighbour records are read coalesced from
-// HBM once and then served from L2/L1.  Each thread accumulates its own sums in
-// the reference order (stencil cells in the reference sequence, ascending
-// reference index inside a cell), so there are no atomics and the FP64 path is
-// deterministic.  Candidates beyond the 2h support contribute exact zeros in the
-// reference (W = F = 0); they are skipped here after the same qq <= 2 test.
+// Work decomposition.  Particles are stored in (cell, seq) order and cell ids
+// run along grid columns (POS(x,y) = y + x*ncy), so a run of consecutive cells
+// of one column holds a contiguous slice of particles, and its 9-cell
+// neighbourhood is three contiguous slices (columns cx-1, cx, cx+1, rows
+// cy0-1 .. cy1).  One wavefront takes one such tile of `tile_rows` cells:
+//   0. it loads the cell offsets of the three slices (one load per lane) and
+//      stages the slices into LDS with coalesced loads;
+//   1. each lane takes one particle of the tile and walks its candidates,
+//      keeping those inside the 2h support in a lane-private list in LDS
+//      (a cheap distance test per candidate);
+//   2. each lane then runs the pair physics over its list.
+// Splitting the loop this way keeps the lanes busy: only ~30% of 9-cell
+// candidates are real pairs, and a single loop would make every lane wait for
+// the pair arithmetic of the others at almost every candidate.  A lane whose
+// list fills up processes it and resumes its candidate walk.  Sums stay in the
+// lane's registers: no atomics, deterministic results.
+// Workgroups are dispatched round-robin over the 8 XCDs; the tile index is
+// remapped so that every XCD walks one contiguous band of grid columns and the
+// neighbour columns a tile shares with the next are found in that XCD's L2.
+// A tile whose neighbourhood exceeds the LDS budget gathers from global memory
+// instead (same arithmetic).
 //
-// Two arithmetic forms:
-//   k_forces64  IEEE double, reference operand order (with -ffp-contract=off)
-//   k_forces32  FP32 pair arithmetic on FP32 positions relative to the cell
-//               origin (float4 {x, y, vx, vy} + float2 {rho, p} records), FP32
-//               accumulation, FP64 outputs for the FP64 integrator.
+// Summation order.  FP64 (k_tile64) visits the 9 cells in the reference order
+// {c, c+1, c+1+ncy, c+ncy, c-1+ncy, c-1, c-1-ncy, c-ncy, c+1-ncy}
+// (src/SPH_interaction.h:26-36) and each cell in ascending reference index, and
+// its lists keep that order, so every per-particle sum is formed in the
+// reference order.  Cells beyond the top/bottom row of a column, which the
+// reference reaches by wrapping into the neighbouring column, hold only
+// particles far outside the 2h support and add exact zeros there; they are not
+// visited.  Candidates beyond 2h likewise add exact zeros in the reference
+// (W = F = 0); they are dropped by the same qq <= 2 test.
+// The mixed form (k_tile32) visits the three column slices in memory order.
+//
+// Arithmetic.  k_tile64: IEEE double in the reference operand order (built with
+// -ffp-contract=off).  k_tile32: FP32 pair arithmetic with hardware reciprocal
+// and square root; positions are staged relative to the tile (x from the
+// tile's column origin, y from its middle row), which keeps |x|, |y| < 6 cells
+// and their FP32 rounding below 1e-6 of the particle spacing; FP32
+// accumulation; FP64 outputs for the FP64 integrator.
 #include "tsph_internal.h"
 #include "tsph_physics.h"
 
 namespace tsph {
 
 namespace {
+
+constexpr int kWave = 64;          // one wavefront per tile
+constexpr int kMaxTileRows = 24;   // upper bound of tile_rows (LDS offset table)
+constexpr int kCellCap = 11;       // LDS records budgeted per cell of the slices
+constexpr int kList = 32;          // per-lane neighbour list entries in LDS
 
 __device__ __forceinline__ void wave_count(unsigned pairs, unsigned cand, unsigned act,
                                            unsigned long long *counters) {
@@ -43,95 +73,559 @@ __device__ __forceinline__ void wave_count(unsigned pairs, unsigned cand, unsign
 }
 
 template <int VAR>
-__global__ __launch_bounds__(256) void k_forces64(Soa d, const int *__restrict__ cell_start,
-                                                  const int *__restrict__ cellof,
-                                                  const int *__restrict__ scal, Consts k,
-                                                  unsigned long long *__restrict__ counters) {
-  const int n = scal[S_NP];
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  unsigned npair = 0, ncand = 0, nact = 0;
-  int at = -1;
-  if (i < n) at = d.type[i];
-  const bool active = (i < n) && (VAR == TSPH_FORCES_DBC ? (at != TSPH_INLET && at != TSPH_OUTLET)
-                                                        : (at == TSPH_FLUID));
-  if (active) {
-    nact = 1;
-    const double h = k.h, m = k.m, c0 = k.cs, delta = k.delta, rho0 = k.rho0, dp = k.dp;
-    const double r2cut = 4.0 * h * h * (1.0 + 1e-12);
-    const double2 ar = d.r[i], av = d.v[i];
-    const double arho = d.rho[i], ap = d.p[i];
-    const int c = cellof[i];
-    double accx = 0., accy = 0., drho_sum = 0., diff_sum = 0., gamma = 0.;
-    for (int s = 0; s < 9; ++s) {
-      const int cl = c + stencil_off(s, k.ncy);
-      if (cl < 0 || cl >= k.ncells) continue;
+__device__ __forceinline__ bool is_active(int t) {
+  return VAR == TSPH_FORCES_DBC ? (t != TSPH_INLET && t != TSPH_OUTLET) : (t == TSPH_FLUID);
+}
+
+// --------------------------------------------------------------------------
+// FP64 per-pair accumulation (reference operand order)
+// --------------------------------------------------------------------------
+struct Acc64 {
+  double accx = 0., accy = 0., drho_sum = 0., diff_sum = 0., gamma = 0.;
+};
+
+// exact reference inclusion test: qq = |dr|/h <= 2 (src/SPH_kernel.h:63)
+__device__ __forceinline__ bool in_support64(double drx, double dry, double h) {
+  const double r2 = (drx * drx) + (dry * dry);
+  if (r2 > 4.0 * h * h * (1.0 + 1e-12)) return false;
+  return sqrt(r2) / h <= 2.0;
+}
+
+template <int VAR>
+__device__ __forceinline__ void pair64(const Consts &k, Acc64 &s, double2 ar, double2 av, double arho,
+                                       double ap, double2 nr, double2 nv, double nrho, double np,
+                                       int nt, double2 nb, unsigned &npair) {
+  const double h = k.h;
+  const double drx = ar.x - nr.x, dry = ar.y - nr.y;
+  const double r2 = (drx * drx) + (dry * dry);
+  if (r2 > 4.0 * h * h * (1.0 + 1e-12)) return;
+  const double drs = sqrt(r2);
+  if (drs / h > 2.0) return;
+  if (drs / h < 2.0) ++npair;
+  const KW64 kw = wendland64(drs, h, k.awen, k.bwen);
+  const double dvx = av.x - nv.x, dvy = av.y - nv.y;
+  const double drdv = drx * dvx + dry * dvy;
+  const double dWx = drx * kw.f, dWy = dry * kw.f;
+  const double dvdW = dvx * dWx + dvy * dWy;
+  const double drdW = drx * dWx + dry * dWy;
+  const double p_temp = (ap + np) / (arho * nrho);
+  const double visco = art_visc64(h, drs, drdv, 0.5 * (arho + nrho), k.cs, k.avisc);
+  if (VAR == TSPH_FORCES_BT && nt == TSPH_WALL) {
+    s.accx += kw.w * (p_temp + visco) * k.dp * nb.x * nrho;
+    s.accy += kw.w * (p_temp + visco) * k.dp * nb.y * nrho;
+    const double dvnb = dvx * nb.x + dvy * nb.y;
+    s.drho_sum -= dvnb * kw.w * nrho * k.dp;
+  } else {
+    const double diff =
+        diff_fourtakas64(h, drs, dry, drdW, arho, nrho, k.rho0, k.cs, k.delta, k.m, k.gy);
+    s.accx -= dWx * (p_temp + visco) * k.m;
+    s.accy -= dWy * (p_temp + visco) * k.m;
+    s.drho_sum += dvdW * k.m;
+    s.diff_sum -= diff;
+  }
+  if (VAR == TSPH_FORCES_BT && nt == TSPH_FLUID) s.gamma += kw.w * k.m / nrho;
+}
+
+// final assignment, src/SPH_interaction.h:134-136, src/SPH_interactionDBC.h:137-142,
+// src/SPH_interactionBT.h:166-178
+template <int VAR>
+__device__ __forceinline__ void finish64(const Consts &k, const Acc64 &s, int at, double2 &a, double &dr) {
+  if (VAR == TSPH_FORCES_BT) {
+    if (s.gamma > kEps) {
+      a = make_double2(s.accx / s.gamma - k.gx, s.accy / s.gamma - k.gy);
+      dr = s.drho_sum / s.gamma + s.diff_sum / s.gamma;
+    } else {
+      a = make_double2(0. - k.gx, 0. - k.gy);
+      dr = 0;
+    }
+  } else if (VAR == TSPH_FORCES_DBC && at == TSPH_WALL) {
+    a = make_double2(0., 0.);
+    dr = s.drho_sum + s.diff_sum;
+  } else {
+    a = make_double2(s.accx - k.gx, s.accy - k.gy);
+    dr = s.drho_sum + s.diff_sum;
+  }
+}
+
+// --------------------------------------------------------------------------
+// FP32 per-pair accumulation
+// --------------------------------------------------------------------------
+struct Acc32 {
+  float accx = 0.f, accy = 0.f, drho_sum = 0.f, diff_sum = 0.f, gamma = 0.f;
+};
+
+struct C32 {
+  float h, ih, m, c0, rho0, dp, kh, r2cut, epsh2, awen, bwen, visc, dcoef, hx;
+};
+
+__device__ __forceinline__ C32 make_c32(const Consts &k) {
+  C32 c;
+  c.h = (float)k.h; c.ih = 1.f / c.h; c.m = (float)k.m; c.c0 = (float)k.cs; c.rho0 = (float)k.rho0;
+  c.dp = (float)k.dp; c.kh = (float)k.kh; c.r2cut = 4.f * c.h * c.h;
+  c.epsh2 = (float)kEps * c.h * c.h; c.awen = (float)k.awen; c.bwen = (float)k.bwen;
+  // Pi = -alpha c0 (h drdv / (r^2 + eps h^2)) / (0.5 (rho_i + rho_j)) for drdv < 0
+  c.visc = -2.f * (float)k.avisc * c.c0 * c.h;
+  c.dcoef = 2.f * c.c0 * c.h * (float)k.delta;
+  c.hx = (float)(k.rho0 * k.gy / (k.cs * k.cs * k.rho0 / 7.));  // rho0 g / cb
+  return c;
+}
+
+// one pair of the mixed path; dr = r_i - r_j with r2 = |dr|^2 < (2h)^2; dev is
+// rho - rho0, ir is 1/rho
+template <int VAR>
+__device__ __forceinline__ void pair32(const C32 &c, Acc32 &s, float drx, float dry, float r2,
+                                       float2 vi, float adev, float ap, float air, float2 vj,
+                                       float ndev, float np, float nir, int nt, float2 nb) {
+#pragma clang fp contract(fast)
+  const float drs = __builtin_amdgcn_sqrtf(r2);
+  const float qq = drs * c.ih;
+  const float wq1 = 1.f - 0.5f * qq;
+  const float wq2 = wq1 * wq1;
+  const float f = c.bwen * wq2 * wq1 * c.ih;  // F = bwen qq wq2 wq1 / r
+  const float dvx = vi.x - vj.x, dvy = vi.y - vj.y;
+  const float drdv = drx * dvx + dry * dvy;
+  const float p_temp = (ap + np) * (air * nir);
+  const float rinv = __builtin_amdgcn_rcpf(r2 + c.epsh2);
+  float visco = 0.f;
+  if (drdv < 0.f) visco = c.visc * drdv * rinv * __builtin_amdgcn_rcpf(2.f * c.rho0 + adev + ndev);
+  if (VAR == TSPH_FORCES_BT && nt == TSPH_WALL) {
+    const float w = c.awen * (qq + qq + 1.f) * wq2 * wq2;
+    const float nrho = c.rho0 + ndev;
+    const float t = w * (p_temp + visco) * c.dp * nrho;
+    s.accx += t * nb.x;
+    s.accy += t * nb.y;
+    s.drho_sum -= (dvx * nb.x + dvy * nb.y) * w * nrho * c.dp;
+  } else {
+    const float dWx = drx * f, dWy = dry * f;
+    const float dvdW = dvx * dWx + dvy * dWy;
+    const float drdW = drx * dWx + dry * dWy;
+    const float drhoH = hydro_drho_f32(dry * c.hx, c.rho0);
+    const float psi = c.dcoef * ((ndev - adev) - drhoH) * rinv;
+    const float t = (p_temp + visco) * c.m;
+    s.accx -= dWx * t;
+    s.accy -= dWy * t;
+    s.drho_sum += dvdW * c.m;
+    s.diff_sum -= psi * drdW * c.m * nir;
+    if (VAR == TSPH_FORCES_BT && nt == TSPH_FLUID) {
+      const float w = c.awen * (qq + qq + 1.f) * wq2 * wq2;
+      s.gamma += w * c.m * nir;
+    }
+  }
+}
+
+template <int VAR>
+__device__ __forceinline__ void finish32(const Consts &k, const Acc32 &s, int at, double2 &a, double &dr) {
+  if (VAR == TSPH_FORCES_BT) {
+    if (s.gamma > (float)kEps) {
+      a = make_double2((double)(s.accx / s.gamma) - k.gx, (double)(s.accy / s.gamma) - k.gy);
+      dr = (double)(s.drho_sum / s.gamma + s.diff_sum / s.gamma);
+    } else {
+      a = make_double2(0. - k.gx, 0. - k.gy);
+      dr = 0;
+    }
+  } else if (VAR == TSPH_FORCES_DBC && at == TSPH_WALL) {
+    a = make_double2(0., 0.);
+    dr = (double)(s.drho_sum + s.diff_sum);
+  } else {
+    a = make_double2((double)s.accx - k.gx, (double)s.accy - k.gy);
+    dr = (double)(s.drho_sum + s.diff_sum);
+  }
+}
+
+// --------------------------------------------------------------------------
+// global-memory forms (tiles whose neighbourhood overflows the LDS budget)
+// --------------------------------------------------------------------------
+template <int VAR>
+__device__ void particle64_global(const Soa &d, const int *__restrict__ cell_start,
+                                  const int *__restrict__ cellof, const Consts &k, int i,
+                                  unsigned &npair, unsigned &ncand) {
+  const int at = d.type[i];
+  const double2 ar = d.r[i], av = d.v[i];
+  const double arho = d.rho[i], ap = d.p[i];
+  const int c = cellof[i];
+  const int cx = c / k.ncy, cy = c - cx * k.ncy;
+  Acc64 s;
+  for (int q = 0; q < 9; ++q) {
+    int dx, dy;
+    stencil_dxy(q, dx, dy);
+    if (cx + dx < 0 || cx + dx >= k.ncx || cy + dy < 0 || cy + dy >= k.ncy) continue;
+    const int cl = c + dy + dx * k.ncy;
+    const int j1 = cell_start[cl + 1];
+    for (int j = cell_start[cl]; j < j1; ++j) {
+      if (j == i) continue;
+      const int nt = VAR != TSPH_FORCES_WCSPH ? d.type[j] : 0;
+      if (VAR == TSPH_FORCES_DBC && at == TSPH_WALL && nt == TSPH_WALL) continue;
+      ++ncand;
+      const double2 nb = (VAR == TSPH_FORCES_BT) ? d.n[j] : make_double2(0., 0.);
+      pair64<VAR>(k, s, ar, av, arho, ap, d.r[j], d.v[j], d.rho[j], d.p[j], nt, nb, npair);
+    }
+  }
+  double2 a;
+  double dr;
+  finish64<VAR>(k, s, at, a, dr);
+  d.a[i] = a;
+  d.drho[i] = dr;
+}
+
+template <int VAR>
+__device__ void particle32_global(const Soa &d, const Mixed &mx, const int *__restrict__ cell_start,
+                                  const int *__restrict__ cellof, const Consts &k, const C32 &cc,
+                                  int i, unsigned &npair, unsigned &ncand) {
+  const int at = d.type[i];
+  const float4 ai = mx.rv[i];
+  const float2 api = mx.rp[i];
+  const float air = __builtin_amdgcn_rcpf(cc.rho0 + api.x);
+  const int c = cellof[i];
+  const int cx = c / k.ncy, cy = c - cx * k.ncy;
+  Acc32 s;
+  for (int dx = -1; dx <= 1; ++dx) {
+    if (cx + dx < 0 || cx + dx >= k.ncx) continue;
+    for (int dy = -1; dy <= 1; ++dy) {
+      if (cy + dy < 0 || cy + dy >= k.ncy) continue;
+      const int cl = c + dy + dx * k.ncy;
+      const float bx = ai.x - (float)dx * cc.kh, by = ai.y - (float)dy * cc.kh;
       const int j1 = cell_start[cl + 1];
       for (int j = cell_start[cl]; j < j1; ++j) {
         if (j == i) continue;
-        int nt = 0;
-        if (VAR != TSPH_FORCES_WCSPH) nt = d.type[j];
+        const int nt = VAR != TSPH_FORCES_WCSPH ? d.type[j] : 0;
         if (VAR == TSPH_FORCES_DBC && at == TSPH_WALL && nt == TSPH_WALL) continue;
         ++ncand;
-        const double2 nr = d.r[j];
-        const double drx = ar.x - nr.x, dry = ar.y - nr.y;
-        const double r2 = (drx * drx) + (dry * dry);
-        if (r2 > r2cut) continue;
-        const double drs = sqrt(r2);
-        if (drs / h > 2.0) continue;
-        if (drs / h < 2.0) ++npair;
-        const KW64 kw = wendland64(drs, h, k.awen, k.bwen);
-        const double2 nv = d.v[j];
-        const double nrho = d.rho[j], np = d.p[j];
-        const double dvx = av.x - nv.x, dvy = av.y - nv.y;
-        const double drdv = drx * dvx + dry * dvy;
-        const double dWx = drx * kw.f, dWy = dry * kw.f;
-        const double dvdW = dvx * dWx + dvy * dWy;
-        const double drdW = drx * dWx + dry * dWy;
-        const double p_temp = (ap + np) / (arho * nrho);
-        const double visco = art_visc64(h, drs, drdv, 0.5 * (arho + nrho), c0, k.avisc);
-        if (VAR == TSPH_FORCES_BT && nt == TSPH_WALL) {
-          const double2 nb = d.n[j];
-          accx += kw.w * (p_temp + visco) * dp * nb.x * nrho;
-          accy += kw.w * (p_temp + visco) * dp * nb.y * nrho;
-          const double dvnb = dvx * nb.x + dvy * nb.y;
-          drho_sum -= dvnb * kw.w * nrho * dp;
-        } else {
-          const double diff =
-              diff_fourtakas64(h, drs, dry, drdW, arho, nrho, rho0, c0, delta, m, k.gy);
-          accx -= dWx * (p_temp + visco) * m;
-          accy -= dWy * (p_temp + visco) * m;
-          drho_sum += dvdW * m;
-          diff_sum -= diff;
-        }
-        if (VAR == TSPH_FORCES_BT && nt == TSPH_FLUID) gamma += kw.w * m / nrho;
+        const float4 nj = mx.rv[j];
+        const float drx = bx - nj.x, dry = by - nj.y;
+        const float r2 = drx * drx + dry * dry;
+        if (!(r2 < cc.r2cut)) continue;
+        ++npair;
+        const float2 pj = mx.rp[j];
+        const float2 nb = (VAR == TSPH_FORCES_BT) ? mx.nrm[j] : make_float2(0.f, 0.f);
+        pair32<VAR>(cc, s, drx, dry, r2, make_float2(ai.z, ai.w), api.x, api.y, air,
+                    make_float2(nj.z, nj.w), pj.x, pj.y, __builtin_amdgcn_rcpf(cc.rho0 + pj.x), nt,
+                    nb);
       }
     }
-    double2 a;
-    double dr;
-    if (VAR == TSPH_FORCES_BT) {
-      if (gamma > kEps) {
-        a = make_double2(accx / gamma - k.gx, accy / gamma - k.gy);
-        dr = drho_sum / gamma + diff_sum / gamma;
-      } else {
-        a = make_double2(0. - k.gx, 0. - k.gy);
-        dr = 0;
-      }
-    } else if (VAR == TSPH_FORCES_DBC && at == TSPH_WALL) {
-      a = make_double2(0., 0.);
-      dr = drho_sum + diff_sum;
-    } else {
-      a = make_double2(accx - k.gx, accy - k.gy);
-      dr = drho_sum + diff_sum;
-    }
-    d.a[i] = a;
-    d.drho[i] = dr;
   }
+  double2 a;
+  double dr;
+  finish32<VAR>(k, s, at, a, dr);
+  d.a[i] = a;
+  d.drho[i] = dr;
+}
+
+// --------------------------------------------------------------------------
+// tiles: one workgroup = one wavefront = one run of `rows` cells of a column
+// --------------------------------------------------------------------------
+// Dynamic LDS layout: int tab[3 * (kMaxTileRows + 3)] (cell_start of the rows
+// of the three neighbour slices), the staged neighbour records, the lane lists.
+constexpr int kTabInts = 3 * (kMaxTileRows + 3);
+constexpr int kTabBytes = ((kTabInts * 4 + 15) / 16) * 16;
+
+struct Tile {
+  int cx, cy0, cy1, row_lo, nrows, w;
+  int own0, own1;   // particle range of the tile
+  int lq[4];        // LDS start of slice q (lq[3] = records staged)
+  int base[3];      // LDS index - global index, per slice
+};
+
+// element q of a 3-array held in registers, without dynamic indexing (which
+// would place the array in scratch memory)
+__device__ __forceinline__ int sel3(const int (&a)[3], int q) {
+  return q == 0 ? a[0] : (q == 1 ? a[1] : a[2]);
+}
+
+// Persistent waves: workgroup b runs on XCD b % 8, so the list of non-empty
+// tiles (in column order) is cut into 8 contiguous bands and the waves of
+// XCD x stride through band x; neighbouring tiles, which share two of their
+// three slices, are then staged from the same L2.
+__device__ __forceinline__ void band_of_wave(int ntl, int &it, int &end, int &step) {
+  const int x = blockIdx.x & 7;
+  it = (int)(((long long)ntl * x) >> 3) + (blockIdx.x >> 3);
+  end = (int)(((long long)ntl * (x + 1)) >> 3);
+  step = gridDim.x >> 3;
+}
+
+__global__ void k_zero_tiles(int *scal) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) scal[S_NTILES] = 0;
+}
+
+// list of the non-empty tiles, in tile order within each workgroup's chunk
+__global__ __launch_bounds__(256) void k_tile_list(const int *__restrict__ cell_start, Consts k,
+                                                   int rows, int ntiles, int *__restrict__ scal,
+                                                   int *__restrict__ tile_list) {
+  __shared__ int lds[4];
+  __shared__ int base;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  int flag = 0;
+  if (t < ntiles) {
+    const int nty = (k.ncy + rows - 1) / rows;
+    const int cx = t / nty, cy0 = (t - cx * nty) * rows, cy1 = min(cy0 + rows, k.ncy);
+    flag = cell_start[cx * k.ncy + cy1] > cell_start[cx * k.ncy + cy0];
+  }
+  // block-wide exclusive scan of the flags (4 waves)
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const unsigned long long m = __ballot(flag);
+  const int before = __popcll(m & ((1ull << lane) - 1ull));
+  if (lane == 0) lds[wid] = __popcll(m);
+  __syncthreads();
+  int off = 0, tot = 0;
+  for (int w = 0; w < 4; ++w) {
+    if (w < wid) off += lds[w];
+    tot += lds[w];
+  }
+  if (threadIdx.x == 0) base = tot ? atomicAdd(&scal[S_NTILES], tot) : 0;
+  __syncthreads();
+  if (flag) tile_list[base + off + before] = t;
+}
+
+// Loads the cell offsets of the three slices (one global load per lane, all in
+// flight together) and derives the tile ranges.  False for an empty tile.
+__device__ __forceinline__ bool tile_begin(const Consts &k, const int *__restrict__ cell_start,
+                                           int tile, int rows, int *tab, Tile &t) {
+  const int nty = (k.ncy + rows - 1) / rows;
+  t.cx = tile / nty;
+  t.cy0 = (tile - t.cx * nty) * rows;
+  t.cy1 = min(t.cy0 + rows, k.ncy);
+  t.row_lo = max(t.cy0 - 1, 0);
+  t.nrows = min(t.cy1 + 1, k.ncy) - t.row_lo;
+  t.w = t.nrows + 1;
+  for (int e = threadIdx.x; e < 3 * t.w; e += blockDim.x) {
+    const int q = e / t.w, r = e - q * t.w;
+    const int col = t.cx + q - 1;
+    tab[e] = (col >= 0 && col < k.ncx) ? cell_start[col * k.ncy + t.row_lo + r] : 0;
+  }
+  __syncthreads();
+  t.own0 = tab[t.w + (t.cy0 - t.row_lo)];
+  t.own1 = tab[t.w + (t.cy1 - t.row_lo)];
+  int l = 0;
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const int g0 = tab[q * t.w], g1 = tab[q * t.w + t.nrows];
+    t.lq[q] = l;
+    t.base[q] = l - g0;
+    l += g1 - g0;
+  }
+  t.lq[3] = l;
+  return t.own1 > t.own0;
+}
+
+__device__ __forceinline__ int slice_of(const Tile &t, int l) {
+  return l >= t.lq[2] ? 2 : (l >= t.lq[1] ? 1 : 0);
+}
+
+// --------------------------------------------------------------------------
+// FP64 tile kernel
+// --------------------------------------------------------------------------
+template <int VAR>
+__global__ __launch_bounds__(kWave) void k_tile64(Soa d, const int *__restrict__ cell_start,
+                                                  const int *__restrict__ cellof, Consts k, int rows,
+                                                  const int *__restrict__ scal,
+                                                  const int *__restrict__ tile_list, int cap,
+                                                  unsigned long long *__restrict__ counters) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  int *tab = reinterpret_cast<int *>(smem);
+  double2 *s_r = reinterpret_cast<double2 *>(smem + kTabBytes);
+  double2 *s_v = s_r + cap;
+  double2 *s_rp = s_v + cap;  // {rho, p}
+  double2 *s_n = s_rp + cap;  // BT only
+  uint16_t *s_list = reinterpret_cast<uint16_t *>(s_n + (VAR == TSPH_FORCES_BT ? cap : 0));
+  uint8_t *s_t = reinterpret_cast<uint8_t *>(s_list + kList * kWave);
+
+  if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&counters[C_PASSES], 1ull);
+  unsigned npair = 0, ncand = 0, nact = 0;
+  int it, it_end, it_step;
+  band_of_wave(scal[S_NTILES], it, it_end, it_step);
+  for (; it < it_end; it += it_step) {
+  __syncthreads();  // the previous tile's LDS is no longer read
+  Tile t;
+  tile_begin(k, cell_start, tile_list[it], rows, tab, t);
+  if (t.lq[3] > cap) {
+    for (int i = t.own0 + threadIdx.x; i < t.own1; i += blockDim.x) {
+      if (!is_active<VAR>(d.type[i])) continue;
+      ++nact;
+      particle64_global<VAR>(d, cell_start, cellof, k, i, npair, ncand);
+    }
+    continue;
+  }
+  for (int l = threadIdx.x; l < t.lq[3]; l += blockDim.x) {
+    const int j = l - sel3(t.base, slice_of(t, l));
+    s_r[l] = d.r[j];
+    s_v[l] = d.v[j];
+    s_rp[l] = make_double2(d.rho[j], d.p[j]);
+    s_t[l] = d.type[j];
+    if (VAR == TSPH_FORCES_BT) s_n[l] = d.n[j];
+  }
+  __syncthreads();
+
+  const int lane = threadIdx.x;
+  uint16_t *my_list = s_list + lane;  // entry e at my_list[e * kWave]
+  for (int i0 = t.own0; i0 < t.own1; i0 += kWave) {
+    const int i = i0 + lane;
+    int li = 0, at = 0, cy = 0;
+    bool act = false;
+    if (i < t.own1) {
+      li = i + t.base[1];
+      at = s_t[li];
+      act = is_active<VAR>(at);
+      cy = cellof[i] - t.cx * k.ncy;
+    }
+    nact += act;
+    const double2 ar = s_r[li], av = s_v[li], arp = s_rp[li];
+    // walk of the 9 stencil cells in reference order: cell q, LDS range [lj, lend)
+    int q = -1, lj = 0, lend = 0;
+    bool more = act;
+    Acc64 s;
+    while (__any(more)) {
+      int cnt = 0;
+      while (more && cnt < kList) {
+        if (lj >= lend) {
+          ++q;
+          if (q == 9) { more = false; break; }
+          int dx, dy;
+          stencil_dxy(q, dx, dy);
+          const int row = cy + dy;
+          if (row < 0 || row >= k.ncy) { lend = lj; continue; }  // wrapped cell: no visit
+          const int e = (dx + 1) * t.w + (row - t.row_lo);
+          const int bq = sel3(t.base, dx + 1);
+          lj = tab[e] + bq;
+          lend = tab[e + 1] + bq;
+          continue;
+        }
+        const int j = lj++;
+        if (j == li) continue;
+        if (VAR == TSPH_FORCES_DBC && at == TSPH_WALL && s_t[j] == TSPH_WALL) continue;
+        ++ncand;
+        const double2 nr = s_r[j];
+        if (in_support64(ar.x - nr.x, ar.y - nr.y, k.h)) my_list[(cnt++) * kWave] = (uint16_t)j;
+      }
+      for (int e = 0; e < cnt; ++e) {
+        const int j = my_list[e * kWave];
+        const double2 nrp = s_rp[j];
+        const int nt = VAR != TSPH_FORCES_WCSPH ? s_t[j] : 0;
+        const double2 nb = VAR == TSPH_FORCES_BT ? s_n[j] : make_double2(0., 0.);
+        pair64<VAR>(k, s, ar, av, arp.x, arp.y, s_r[j], s_v[j], nrp.x, nrp.y, nt, nb, npair);
+      }
+    }
+    if (act) {
+      double2 a;
+      double dr;
+      finish64<VAR>(k, s, at, a, dr);
+      d.a[i] = a;
+      d.drho[i] = dr;
+    }
+  }
+  }  // tiles
   wave_count(npair, ncand, nact, counters);
 }
 
-// ---------------------------------------------------------------------------
-// mixed path
-// ---------------------------------------------------------------------------
+// --------------------------------------------------------------------------
+// mixed tile kernel
+// --------------------------------------------------------------------------
+template <int VAR>
+__global__ __launch_bounds__(kWave) void k_tile32(Soa d, Mixed mx, const int *__restrict__ cell_start,
+                                                  const int *__restrict__ cellof, Consts k, int rows,
+                                                  const int *__restrict__ scal,
+                                                  const int *__restrict__ tile_list, int cap,
+                                                  unsigned long long *__restrict__ counters) {
+#pragma clang fp contract(fast)
+  extern __shared__ __align__(16) unsigned char smem[];
+  int *tab = reinterpret_cast<int *>(smem);
+  float2 *s_pos = reinterpret_cast<float2 *>(smem + kTabBytes);  // tile frame
+  float2 *s_vel = s_pos + cap;
+  float2 *s_dp = s_vel + cap;   // {rho - rho0, p}
+  float2 *s_n = s_dp + cap;     // BT only
+  float *s_ir = reinterpret_cast<float *>(s_n + (VAR == TSPH_FORCES_BT ? cap : 0));  // 1/rho
+  uint16_t *s_list = reinterpret_cast<uint16_t *>(s_ir + cap);
+  uint8_t *s_t = reinterpret_cast<uint8_t *>(s_list + kList * kWave);
+
+  if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&counters[C_PASSES], 1ull);
+  const C32 cc = make_c32(k);
+  unsigned npair = 0, ncand = 0, nact = 0;
+  int it, it_end, it_step;
+  band_of_wave(scal[S_NTILES], it, it_end, it_step);
+  for (; it < it_end; it += it_step) {
+  __syncthreads();  // the previous tile's LDS is no longer read
+  Tile t;
+  tile_begin(k, cell_start, tile_list[it], rows, tab, t);
+  if (t.lq[3] > cap) {
+    for (int i = t.own0 + threadIdx.x; i < t.own1; i += blockDim.x) {
+      if (!is_active<VAR>(d.type[i])) continue;
+      ++nact;
+      particle32_global<VAR>(d, mx, cell_start, cellof, k, cc, i, npair, ncand);
+    }
+    continue;
+  }
+  const int mid = t.row_lo + (t.nrows >> 1);
+  for (int l = threadIdx.x; l < t.lq[3]; l += blockDim.x) {
+    const int q = slice_of(t, l);
+    const int j = l - sel3(t.base, q);
+    const float4 rv = mx.rv[j];
+    const float2 rp = mx.rp[j];
+    const int row = cellof[j] - (t.cx + q - 1) * k.ncy;
+    s_pos[l] = make_float2(rv.x + (float)(q - 1) * cc.kh, rv.y + (float)(row - mid) * cc.kh);
+    s_vel[l] = make_float2(rv.z, rv.w);
+    s_dp[l] = rp;
+    s_ir[l] = __builtin_amdgcn_rcpf(cc.rho0 + rp.x);
+    s_t[l] = d.type[j];
+    if (VAR == TSPH_FORCES_BT) s_n[l] = mx.nrm[j];
+  }
+  __syncthreads();
+
+  const int lane = threadIdx.x;
+  uint16_t *my_list = s_list + lane;
+  for (int i0 = t.own0; i0 < t.own1; i0 += kWave) {
+    const int i = i0 + lane;
+    int li = 0, at = 0, ri = 0;
+    bool act = false;
+    if (i < t.own1) {
+      li = i + t.base[1];
+      at = s_t[li];
+      act = is_active<VAR>(at);
+      ri = cellof[i] - t.cx * k.ncy - t.row_lo;
+    }
+    nact += act;
+    const float2 pi = s_pos[li], vi = s_vel[li], dpi = s_dp[li];
+    const float iri = s_ir[li];
+    // the three column slices, rows ri-1 .. ri+1, as one flat candidate range
+    const int rlo = max(ri - 1, 0), rhi = min(ri + 2, t.nrows);
+    const int a0 = tab[rlo] + t.base[0], n0 = tab[rhi] + t.base[0] - a0;
+    const int a1 = tab[t.w + rlo] + t.base[1], n1 = tab[t.w + rhi] + t.base[1] - a1;
+    const int a2 = tab[2 * t.w + rlo] + t.base[2], n2 = tab[2 * t.w + rhi] + t.base[2] - a2;
+    const int tot = act ? n0 + n1 + n2 : 0;
+    int cur = 0;
+    Acc32 s;
+    while (__any(cur < tot)) {
+      int cnt = 0;
+      while (cur < tot && cnt < kList) {
+        const int j = cur < n0 ? a0 + cur : (cur < n0 + n1 ? a1 + (cur - n0) : a2 + (cur - n0 - n1));
+        ++cur;
+        if (j == li) continue;
+        if (VAR == TSPH_FORCES_DBC && at == TSPH_WALL && s_t[j] == TSPH_WALL) continue;
+        ++ncand;
+        const float2 pj = s_pos[j];
+        const float drx = pi.x - pj.x, dry = pi.y - pj.y;
+        if (drx * drx + dry * dry < cc.r2cut) my_list[(cnt++) * kWave] = (uint16_t)j;
+      }
+      npair += cnt;
+      for (int e = 0; e < cnt; ++e) {
+        const int j = my_list[e * kWave];
+        const float2 pj = s_pos[j], dpj = s_dp[j];
+        const float drx = pi.x - pj.x, dry = pi.y - pj.y;
+        const int nt = VAR != TSPH_FORCES_WCSPH ? s_t[j] : 0;
+        const float2 nb = VAR == TSPH_FORCES_BT ? s_n[j] : make_float2(0.f, 0.f);
+        pair32<VAR>(cc, s, drx, dry, drx * drx + dry * dry, vi, dpi.x, dpi.y, iri, s_vel[j], dpj.x,
+                    dpj.y, s_ir[j], nt, nb);
+      }
+    }
+    if (act) {
+      double2 a;
+      double dr;
+      finish32<VAR>(k, s, at, a, dr);
+      d.a[i] = a;
+      d.drho[i] = dr;
+    }
+  }
+  }  // tiles
+  wave_count(npair, ncand, nact, counters);
+}
+
 __global__ void k_mixed_records(Soa d, const int *__restrict__ cellof, const int *__restrict__ scal,
                                 Consts k, Mixed mx) {
   const int n = scal[S_NP];
@@ -142,127 +636,11 @@ __global__ void k_mixed_records(Soa d, const int *__restrict__ cellof, const int
   const double ox = k.x0 - k.of + cx * k.kh, oy = k.y0 - k.of + cy * k.kh;
   const double2 r = d.r[i], v = d.v[i];
   mx.rv[i] = make_float4((float)(r.x - ox), (float)(r.y - oy), (float)v.x, (float)v.y);
-  mx.rp[i] = make_float2((float)d.rho[i], (float)d.p[i]);
+  // density as its deviation from rho0: FP32 resolves rho - rho0 to ~1e-7 of
+  // its size, while FP32 rho itself (ulp 6e-5 at 1000) would lose the small
+  // density differences the diffusion term is built from
+  mx.rp[i] = make_float2((float)(d.rho[i] - k.rho0), (float)d.p[i]);
   if (mx.nrm && d.n) mx.nrm[i] = make_float2((float)d.n[i].x, (float)d.n[i].y);
-}
-
-template <int VAR>
-__global__ __launch_bounds__(256) void k_forces32(Soa d, Mixed mx, const int *__restrict__ cell_start,
-                                                  const int *__restrict__ cellof,
-                                                  const int *__restrict__ scal, Consts k,
-                                                  unsigned long long *__restrict__ counters) {
-#pragma clang fp contract(fast)
-  const int n = scal[S_NP];
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  unsigned npair = 0, ncand = 0, nact = 0;
-  int at = -1;
-  if (i < n) at = d.type[i];
-  const bool active = (i < n) && (VAR == TSPH_FORCES_DBC ? (at != TSPH_INLET && at != TSPH_OUTLET)
-                                                        : (at == TSPH_FLUID));
-  if (active) {
-    nact = 1;
-    const float h = (float)k.h, m = (float)k.m, c0 = (float)k.cs, rho0 = (float)k.rho0;
-    const float dp = (float)k.dp, kh = (float)k.kh;
-    const float ih = 1.f / h;
-    const float r2cut = 4.f * h * h;
-    const float epsh2 = (float)kEps * h * h;
-    const float awen = (float)k.awen, bwen = (float)k.bwen;
-    const float avisc = (float)k.avisc;
-    const float dcoef = 2.f * c0 * h * (float)k.delta;       // 2 c0 h delta
-    const float hx = rho0 * (float)k.gy / (c0 * c0 * rho0 / 7.f);  // rho0 g / cb
-    const float4 ai = mx.rv[i];
-    const float2 api = mx.rp[i];
-    const float arho = api.x, ap = api.y;
-    const int c = cellof[i];
-    float accx = 0.f, accy = 0.f, drho_sum = 0.f, diff_sum = 0.f, gamma = 0.f;
-    for (int s = 0; s < 9; ++s) {
-      const int cl = c + stencil_off(s, k.ncy);
-      if (cl < 0 || cl >= k.ncells) continue;
-      int dcx, dcy;
-      stencil_dxy(s, dcx, dcy);
-      // position of i relative to the origin of neighbour cell cl
-      const float bx = ai.x - (float)dcx * kh, by = ai.y - (float)dcy * kh;
-      const int j1 = cell_start[cl + 1];
-      for (int j = cell_start[cl]; j < j1; ++j) {
-        if (j == i) continue;
-        int nt = 0;
-        if (VAR != TSPH_FORCES_WCSPH) nt = d.type[j];
-        if (VAR == TSPH_FORCES_DBC && at == TSPH_WALL && nt == TSPH_WALL) continue;
-        ++ncand;
-        const float4 nj = mx.rv[j];
-        const float drx = bx - nj.x, dry = by - nj.y;
-        const float r2 = drx * drx + dry * dry;
-        if (!(r2 < r2cut)) continue;
-        ++npair;
-        const float2 pj = mx.rp[j];
-        const float nrho = pj.x, np = pj.y;
-        const float drs = sqrtf(r2);
-        const float qq = drs * ih;
-        const float wq1 = 1.f - 0.5f * qq;
-        const float wq2 = wq1 * wq1;
-        // F = bwen qq wq2 wq1 / r = bwen wq2 wq1 / h
-        const float f = bwen * wq2 * wq1 * ih;
-        const float dvx = ai.z - nj.z, dvy = ai.w - nj.w;
-        const float drdv = drx * dvx + dry * dvy;
-        const float dWx = drx * f, dWy = dry * f;
-        const float inv_rr = __frcp_rn(arho * nrho);
-        const float p_temp = (ap + np) * inv_rr;
-        const float rinv = __frcp_rn(r2 + epsh2);
-        float visco = 0.f;
-        if (drdv < 0.f) {
-          const float vmax = drdv > avisc ? drdv : avisc;
-          // -visco c0 (h drdv / (r^2 + eps h^2)) / (0.5 (rho_i + rho_j))
-          visco = -vmax * c0 * (h * drdv * rinv) * (2.f * __frcp_rn(arho + nrho));
-        }
-        if (VAR == TSPH_FORCES_BT && nt == TSPH_WALL) {
-          const float w = awen * (qq + qq + 1.f) * wq2 * wq2;
-          const float2 nb = mx.nrm[j];
-          const float t = w * (p_temp + visco) * dp * nrho;
-          accx += t * nb.x;
-          accy += t * nb.y;
-          drho_sum -= (dvx * nb.x + dvy * nb.y) * w * nrho * dp;
-        } else {
-          const float dvdW = dvx * dWx + dvy * dWy;
-          const float drdW = drx * dWx + dry * dWy;
-          const float drhoH = hydro_drho_f32(dry * hx, rho0);
-          const float psi = dcoef * ((nrho - arho) - drhoH) * rinv;
-          const float t = (p_temp + visco) * m;
-          accx -= dWx * t;
-          accy -= dWy * t;
-          drho_sum += dvdW * m;
-          diff_sum -= psi * drdW * m * __frcp_rn(nrho);
-          if (VAR == TSPH_FORCES_BT && nt == TSPH_FLUID) {
-            const float w = awen * (qq + qq + 1.f) * wq2 * wq2;
-            gamma += w * m * __frcp_rn(nrho);
-          }
-        }
-      }
-    }
-    double2 a;
-    double dr;
-    if (VAR == TSPH_FORCES_BT) {
-      if (gamma > (float)kEps) {
-        a = make_double2((double)(accx / gamma) - k.gx, (double)(accy / gamma) - k.gy);
-        dr = (double)(drho_sum / gamma + diff_sum / gamma);
-      } else {
-        a = make_double2(0. - k.gx, 0. - k.gy);
-        dr = 0;
-      }
-    } else if (VAR == TSPH_FORCES_DBC && at == TSPH_WALL) {
-      a = make_double2(0., 0.);
-      dr = (double)(drho_sum + diff_sum);
-    } else {
-      a = make_double2((double)accx - k.gx, (double)accy - k.gy);
-      dr = (double)(drho_sum + diff_sum);
-    }
-    d.a[i] = a;
-    d.drho[i] = dr;
-  }
-  wave_count(npair, ncand, nact, counters);
-}
-
-__global__ void k_count_pass(unsigned long long *counters) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) counters[C_PASSES] += 1ull;
 }
 
 template <typename K>
@@ -289,48 +667,61 @@ int launch_mixed_records(tsph_ctx *c) {
 }
 
 int launch_forces(tsph_ctx *c, int variant) {
-  const int nb = c->block;
-  const dim3 g(grid_for(c->n_ub, nb)), b(nb);
+  const Consts &k = c->k;
+  const int rows = c->tile_rows;
+  const int ntiles = k.ncx * ((k.ncy + rows - 1) / rows);
+  const int cap = 3 * (rows + 2) * kCellCap;
+  const bool bt = variant == TSPH_FORCES_BT;
   hipStream_t s = c->stream;
-  int e = 0;
+  if (!c->tiles_valid) {
+    if (!c->tiles_zeroed) {
+      hipLaunchKernelGGL(k_zero_tiles, dim3(1), dim3(64), 0, s, c->scal);
+    }
+    hipLaunchKernelGGL(k_tile_list, dim3(grid_for(ntiles, 256)), dim3(256), 0, s, c->cell_start, k,
+                       rows, ntiles, c->scal, c->tile_list);
+    TSPH_HIP(hipGetLastError());
+    c->tiles_valid = true;
+    c->tiles_zeroed = false;
+  }
+  const dim3 g(8 * ((c->num_cu * c->waves_per_cu + 7) / 8)), b(kWave);
+  const size_t list = sizeof(uint16_t) * kList * kWave;
+  const int *sc = c->scal, *tl = c->tile_list;
   if (c->prm.precision == TSPH_PREC_MIXED) {
-    if ((e = launch_mixed_records(c))) return e;
-    e = launch_timed(c, [&] {
+    if (int e = launch_mixed_records(c)) return e;
+    const size_t lds = kTabBytes + list +
+                       (size_t)cap * (3 * sizeof(float2) + sizeof(float) + 1 + (bt ? sizeof(float2) : 0));
+    return launch_timed(c, [&] {
       switch (variant) {
         case TSPH_FORCES_WCSPH:
-          hipLaunchKernelGGL(k_forces32<TSPH_FORCES_WCSPH>, g, b, 0, s, c->cur, c->mx, c->cell_start,
-                             c->cellof, c->scal, c->k, c->counters);
+          hipLaunchKernelGGL(k_tile32<TSPH_FORCES_WCSPH>, g, b, lds, s, c->cur, c->mx, c->cell_start,
+                             c->cellof, k, rows, sc, tl, cap, c->counters);
           break;
         case TSPH_FORCES_DBC:
-          hipLaunchKernelGGL(k_forces32<TSPH_FORCES_DBC>, g, b, 0, s, c->cur, c->mx, c->cell_start,
-                             c->cellof, c->scal, c->k, c->counters);
+          hipLaunchKernelGGL(k_tile32<TSPH_FORCES_DBC>, g, b, lds, s, c->cur, c->mx, c->cell_start,
+                             c->cellof, k, rows, sc, tl, cap, c->counters);
           break;
         default:
-          hipLaunchKernelGGL(k_forces32<TSPH_FORCES_BT>, g, b, 0, s, c->cur, c->mx, c->cell_start,
-                             c->cellof, c->scal, c->k, c->counters);
-      }
-    });
-  } else {
-    e = launch_timed(c, [&] {
-      switch (variant) {
-        case TSPH_FORCES_WCSPH:
-          hipLaunchKernelGGL(k_forces64<TSPH_FORCES_WCSPH>, g, b, 0, s, c->cur, c->cell_start,
-                             c->cellof, c->scal, c->k, c->counters);
-          break;
-        case TSPH_FORCES_DBC:
-          hipLaunchKernelGGL(k_forces64<TSPH_FORCES_DBC>, g, b, 0, s, c->cur, c->cell_start,
-                             c->cellof, c->scal, c->k, c->counters);
-          break;
-        default:
-          hipLaunchKernelGGL(k_forces64<TSPH_FORCES_BT>, g, b, 0, s, c->cur, c->cell_start,
-                             c->cellof, c->scal, c->k, c->counters);
+          hipLaunchKernelGGL(k_tile32<TSPH_FORCES_BT>, g, b, lds, s, c->cur, c->mx, c->cell_start,
+                             c->cellof, k, rows, sc, tl, cap, c->counters);
       }
     });
   }
-  if (e) return e;
-  hipLaunchKernelGGL(k_count_pass, dim3(1), dim3(64), 0, s, c->counters);
-  TSPH_HIP(hipGetLastError());
-  return 0;
+  const size_t lds = kTabBytes + list + (size_t)cap * (3 * sizeof(double2) + 1 + (bt ? sizeof(double2) : 0));
+  return launch_timed(c, [&] {
+    switch (variant) {
+      case TSPH_FORCES_WCSPH:
+        hipLaunchKernelGGL(k_tile64<TSPH_FORCES_WCSPH>, g, b, lds, s, c->cur, c->cell_start, c->cellof,
+                           k, rows, sc, tl, cap, c->counters);
+        break;
+      case TSPH_FORCES_DBC:
+        hipLaunchKernelGGL(k_tile64<TSPH_FORCES_DBC>, g, b, lds, s, c->cur, c->cell_start, c->cellof,
+                           k, rows, sc, tl, cap, c->counters);
+        break;
+      default:
+        hipLaunchKernelGGL(k_tile64<TSPH_FORCES_BT>, g, b, lds, s, c->cur, c->cell_start, c->cellof,
+                           k, rows, sc, tl, cap, c->counters);
+    }
+  });
 }
 
 }  // namespace tsph

@@ -34,10 +34,9 @@ __device__ __forceinline__ int stencil_off(int s, int ncy) {
 }
 // the same cell expressed as (dx, dy) of cell coordinates
 __device__ __forceinline__ void stencil_dxy(int s, int &dx, int &dy) {
-  const int tx[9] = {0, 0, 1, 1, 1, 0, -1, -1, -1};
-  const int ty[9] = {0, 1, 1, 0, -1, -1, -1, 0, 1};
-  dx = tx[s];
-  dy = ty[s];
+  // dx: {0, 0, 1, 1, 1, 0, -1, -1, -1}, dy: {0, 1, 1, 0, -1, -1, -1, 0, 1}
+  dx = (s >= 2 && s <= 4) ? 1 : (s >= 6 ? -1 : 0);
+  dy = (s == 1 || s == 2 || s == 8) ? 1 : ((s >= 4 && s <= 6) ? -1 : 0);
 }
 
 // Wendland C2 in 2-D, src/SPH_kernel.h:52-77.  awen/bwen are the float-rounded

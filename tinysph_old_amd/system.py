@@ -233,8 +233,17 @@ class ParticleSystem:
     def reset_counters(self):
         L.check(self.lib.tsph_reset_counters(self.ctx), "tsph_reset_counters")
 
-    def set_block_size(self, threads: int):
-        L.check(self.lib.tsph_set_block_size(self.ctx, threads), "tsph_set_block_size")
+    def set_tile_rows(self, rows: int):
+        L.check(self.lib.tsph_set_tile_rows(self.ctx, rows), "tsph_set_tile_rows")
+
+    def set_waves_per_cu(self, waves: int):
+        L.check(self.lib.tsph_set_waves_per_cu(self.ctx, waves), "tsph_set_waves_per_cu")
+
+    @property
+    def tile_rows(self) -> int:
+        r = C.c_int()
+        L.check(self.lib.tsph_get_tile_rows(self.ctx, C.byref(r)), "tsph_get_tile_rows")
+        return r.value
 
     def pair_timing(self, max_launches: int):
         L.check(self.lib.tsph_pair_timing(self.ctx, max_launches), "tsph_pair_timing")
