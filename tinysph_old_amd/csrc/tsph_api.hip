@@ -150,8 +150,13 @@ int tsph_create(const tsph_params *prm, tsph_ctx **out) {
       (e = dalloc(&c->cellof, cap)) || (e = dalloc(&c->hrank, cap)) || (e = dalloc(&c->hcell, cap)) ||
       (e = dalloc(&c->perm, cap)) || (e = dalloc(&c->scan_tmp, (k.ncells + 1) / 1024 + 2)) ||
       (e = dalloc(&c->seqstate, cap)) || (e = dalloc(&c->probe_out, 4)) ||
-      (e = dalloc(&c->tile_list, k.ncells + 8)))
+      (e = dalloc(&c->tile_list, k.ncells + 8)) || (e = dalloc(&c->wall_list, cap)))
     return bail(e);
+  // the per-cell counters and removal states start cleared; later builds clear
+  // them as they consume them
+  if (hipMemsetAsync(c->cell_count, 0, sizeof(int) * (k.ncells + 1), c->stream) != hipSuccess ||
+      hipMemsetAsync(c->seqstate, 0, cap, c->stream) != hipSuccess)
+    return bail(fail(TSPH_EHIP, "initialisation failed"));
   {
     int dev = 0;
     hipDeviceProp_t prop;
@@ -178,7 +183,7 @@ int tsph_destroy(tsph_ctx *c) {
   free_soa(c->alt);
   void *ps[] = {c->scal, c->counters, c->cell_count, c->cell_start, c->cellof, c->hrank,
                 c->hcell, c->perm, c->scan_tmp, c->seqstate, c->probe_out, c->mx.rv,
-                c->mx.rp, c->mx.nrm, c->tile_list};
+                c->mx.rp, c->mx.nrm, c->tile_list, c->wall_list};
   for (void *p : ps)
     if (p) (void)hipFree(p);
   for (hipEvent_t ev : c->tev) (void)hipEventDestroy(ev);
@@ -248,9 +253,13 @@ int tsph_upload(tsph_ctx *c, const tsph_particles *h) {
   int sc[S_NSCAL] = {n, n, 0, 0};
   TSPH_HIP(hipMemcpy(c->scal, sc, sizeof sc, hipMemcpyHostToDevice));
   c->n_ub = n;
+  c->n_wall = 0;
+  for (int i = 0; i < n; ++i) c->n_wall += t[i] == TSPH_WALL;
   c->cells_valid = false;
   c->tiles_valid = false;
   c->tiles_zeroed = true;  // the upload above zeroed the device tile counter
+  c->records_valid = false;
+  c->marks_valid = false;
   return 0;
 }
 

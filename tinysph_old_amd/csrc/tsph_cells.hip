@@ -1,22 +1,27 @@
 // Cell-list build: the device form of Clear_cells + Particle_To_Cell
 // (src/SPH_find_pairs.h:45-92).
 //
-//   1. k_hash     cell id of every particle (reference truncation), and its
-//                 arrival rank in the cell from one atomic per particle;
-//                 removed particles go to the extra bucket `ncells`
+//   1. k_hash     cell id of every particle (reference truncation) and its
+//                 arrival rank in the cell; lanes of a wavefront that fall in
+//                 the same cell share one atomic increment (the input is in
+//                 cell order from the previous build, so a wavefront spans only
+//                 a handful of cells); removed particles go to the extra
+//                 bucket `ncells`
 //   2. scan       exclusive scan of the per-cell counts -> cell_start (CSR)
 //   3. k_scatter  sorted position = cell_start[cell] + rank
 //   4. k_order    per cell, order its slots by `seq` (reference particle
 //                 order), which makes the result identical to the reference's
 //                 serial push_back in ascending particle index and independent
 //                 of the atomic arrival order
-//   5. k_reorder  gather every SoA field into (cell, seq) order
+//   5. k_reorder  gather the SoA fields into (cell, seq) order; also lists the
+//                 wall particles for the wall-density gathers
 //
 // Steps 1-4 are a one-digit counting sort on the cell key (the key range is
 // the cell count, so a single digit covers it and its scan is the cell-start
 // table the pair loops need anyway); step 4 supplies the stability.
 // All passes read the particle count from device memory, so the sequence can
-// be captured in a HIP graph.
+// be captured in a HIP graph.  The per-cell counters are cleared by the scan
+// that consumes them, so no separate clearing pass is needed.
 #include "tsph_internal.h"
 #include "tsph_physics.h"
 
@@ -28,34 +33,57 @@ constexpr int kScanBlock = 256;
 constexpr int kScanItems = 4;
 constexpr int kScanTile = kScanBlock * kScanItems;
 
-__global__ void k_fill_i32(int *__restrict__ p, int v, int count) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < count) p[i] = v;
+__device__ __forceinline__ unsigned long long lanemask_lt() {
+  return (1ull << (threadIdx.x & 63)) - 1ull;
 }
 
 __global__ void k_hash(const double2 *__restrict__ r, const uint8_t *__restrict__ type,
-                       const int *__restrict__ scal, Consts k, int *__restrict__ cell_count,
-                       int *__restrict__ hcell, int *__restrict__ hrank,
+                       const uint32_t *__restrict__ seq, const int *__restrict__ scal, Consts k,
+                       int *__restrict__ cell_count, int *__restrict__ hcell,
+                       int *__restrict__ hrank, uint8_t *__restrict__ seqstate,
                        unsigned long long *__restrict__ counters) {
   const int n = scal[S_NP];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  int c;
-  if (type[i] == kRemoved) {
-    c = k.ncells;
-  } else {
-    const double2 ri = r[i];
-    const int cx = cell_coord(ri.x, k.x0, k.of, k.kh);
-    const int cy = cell_coord(ri.y, k.y0, k.of, k.kh);
-    c = cy + cx * k.ncy;  // POS(x,y) = y + x*Y, src/SPH_defs.h:36
-    if (c < 0 || c >= k.ncells) {
-      // undefined behaviour in the reference; counted and clamped here
-      atomicAdd(&counters[C_OUTGRID], 1ull);
-      c = c < 0 ? 0 : k.ncells - 1;
+  const bool live = i < n;
+  int c = -1;
+  if (live) {
+    if (type[i] == kRemoved) {
+      c = k.ncells;
+      seqstate[seq[i]] = 0;  // its reference position is vacated
+    } else {
+      const double2 ri = r[i];
+      const int cx = cell_coord(ri.x, k.x0, k.of, k.kh);
+      const int cy = cell_coord(ri.y, k.y0, k.of, k.kh);
+      c = cy + cx * k.ncy;  // POS(x,y) = y + x*Y, src/SPH_defs.h:36
+      if (c < 0 || c >= k.ncells) {
+        // undefined behaviour in the reference; counted and clamped here
+        atomicAdd(&counters[C_OUTGRID], 1ull);
+        c = c < 0 ? 0 : k.ncells - 1;
+      }
     }
   }
-  hcell[i] = c;
-  hrank[i] = atomicAdd(&cell_count[c], 1);
+  // group the lanes by cell; the lowest lane of each group does the atomic
+  const int lane = threadIdx.x & 63;
+  unsigned long long rem = __ballot(live);
+  int leader = 0;
+  unsigned long long grp = 0;
+  while (rem) {
+    const int l = __builtin_ctzll(rem);
+    const int key = __shfl(c, l, 64);
+    const unsigned long long m = __ballot(c == key) & rem;
+    if ((m >> lane) & 1ull) {
+      leader = l;
+      grp = m;
+    }
+    rem &= ~m;
+  }
+  int base = 0;
+  if (live && lane == leader) base = atomicAdd(&cell_count[c], __popcll(grp));
+  base = __shfl(base, leader, 64);
+  if (live) {
+    hcell[i] = c;
+    hrank[i] = base + __popcll(grp & lanemask_lt());
+  }
 }
 
 template <int NT>
@@ -94,9 +122,11 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_partials(const int *__restr
   if (threadIdx.x == 0) partial[blockIdx.x] = total;
 }
 
-// one block: exclusive scan of the block partials; writes out[len] = total
+// one block: exclusive scan of the block partials; writes out[len] = total and
+// clears the list counters that the reorder and pair passes rebuild
 __global__ __launch_bounds__(1024) void k_scan_top(int *__restrict__ partial, int nb,
-                                                   int *__restrict__ out, int len) {
+                                                   int *__restrict__ out, int len,
+                                                   int *__restrict__ scal) {
   __shared__ int lds[1024 / 64];
   int carry = 0;
   for (int base = 0; base < nb; base += 1024) {
@@ -107,10 +137,15 @@ __global__ __launch_bounds__(1024) void k_scan_top(int *__restrict__ partial, in
     if (i < nb) partial[i] = carry + ex;
     carry += total;
   }
-  if (threadIdx.x == 0) out[len] = carry;
+  if (threadIdx.x == 0) {
+    out[len] = carry;
+    scal[S_NTILES] = 0;
+    scal[S_NWALL] = 0;
+  }
 }
 
-__global__ __launch_bounds__(kScanBlock) void k_scan_final(const int *__restrict__ in, int len,
+// final pass; also clears the counts it consumed for the next build
+__global__ __launch_bounds__(kScanBlock) void k_scan_final(int *__restrict__ in, int len,
                                                            const int *__restrict__ partial,
                                                            int *__restrict__ out) {
   __shared__ int lds[kScanBlock / 64];
@@ -126,7 +161,10 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_final(const int *__restrict
   int ex = block_exclusive_scan<kScanBlock>(s, lds, total) + partial[blockIdx.x];
 #pragma unroll
   for (int j = 0; j < kScanItems; ++j) {
-    if (base + j < len) out[base + j] = ex;
+    if (base + j < len) {
+      out[base + j] = ex;
+      in[base + j] = 0;
+    }
     ex += v[j];
   }
 }
@@ -143,12 +181,47 @@ __global__ void k_scatter(const int *__restrict__ scal, const int *__restrict__ 
   cellof[pos] = c;
 }
 
-// insertion sort of each cell's slots by reference order (seq)
+// Orders each cell's slots by reference order (seq).  Cells of up to 16
+// particles are sorted in registers (all loads in flight together, then an
+// odd-even transposition network); larger cells use an insertion sort.
 __global__ void k_order(const int *__restrict__ cell_start, int ncells,
                         const uint32_t *__restrict__ seq, int *__restrict__ perm) {
+  constexpr int R = 16;
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncells) return;
   const int s = cell_start[c], e = cell_start[c + 1];
+  const int cnt = e - s;
+  if (cnt <= 1) return;
+  if (cnt <= R) {
+    int pv[R];
+    uint32_t kv[R];
+#pragma unroll
+    for (int a = 0; a < R; ++a) {
+      pv[a] = a < cnt ? perm[s + a] : 0;
+      kv[a] = 0xffffffffu;
+    }
+#pragma unroll
+    for (int a = 0; a < R; ++a)
+      if (a < cnt) kv[a] = seq[pv[a]];
+    bool sorted = true;
+#pragma unroll
+    for (int a = 1; a < R; ++a) sorted &= kv[a - 1] <= kv[a];
+    if (sorted) return;
+#pragma unroll
+    for (int pass = 0; pass < R; ++pass) {
+#pragma unroll
+      for (int a = pass & 1; a + 1 < R; a += 2) {
+        if (kv[a] > kv[a + 1]) {
+          const uint32_t tk = kv[a]; kv[a] = kv[a + 1]; kv[a + 1] = tk;
+          const int tp = pv[a]; pv[a] = pv[a + 1]; pv[a + 1] = tp;
+        }
+      }
+    }
+#pragma unroll
+    for (int a = 0; a < R; ++a)
+      if (a < cnt) perm[s + a] = pv[a];
+    return;
+  }
   for (int a = s + 1; a < e; ++a) {
     const int pa = perm[a];
     const uint32_t ka = seq[pa];
@@ -163,39 +236,48 @@ __global__ void k_order(const int *__restrict__ cell_start, int ncells,
   }
 }
 
-__global__ void k_reorder(const int *__restrict__ cell_start, int ncells,
-                          const int *__restrict__ perm, Soa src, Soa dst) {
-  const int n_new = cell_start[ncells];  // removed particles sit after this
+// Gathers every field into sorted order (the state after the build is the
+// reference's state, permuted), lists the wall particles, and sets the new
+// particle count (removed particles sit after n_new and are dropped).
+__global__ void k_reorder(const int *__restrict__ cell_start, int ncells, const int *__restrict__ perm,
+                          Soa src, Soa dst, int *__restrict__ scal, int *__restrict__ wall_list) {
+  const int n_new = cell_start[ncells];
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n_new) return;
-  const int j = perm[k];
-  dst.r[k] = src.r[j];
-  dst.v[k] = src.v[j];
-  dst.a[k] = src.a[j];
-  dst.r_o[k] = src.r_o[j];
-  dst.v_o[k] = src.v_o[j];
-  dst.rho[k] = src.rho[j];
-  dst.drho[k] = src.drho[j];
-  dst.p[k] = src.p[j];
-  dst.rho_o[k] = src.rho_o[j];
-  dst.type[k] = src.type[j];
-  dst.seq[k] = src.seq[j];
-  dst.index[k] = src.index[j];
-  if (src.gnr) dst.gnr[k] = src.gnr[j];
-  if (src.n) dst.n[k] = src.n[j];
-}
-
-__global__ void k_set_np(const int *__restrict__ cell_start, int ncells, int *__restrict__ scal) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    scal[S_NP] = cell_start[ncells];
-    scal[S_NTILES] = 0;  // the pair-loop tile list is rebuilt for the new cell list
+  if (blockIdx.x == 0 && threadIdx.x == 0) scal[S_NP] = n_new;
+  int wall = 0;
+  if (k < n_new) {
+    const int j = perm[k];
+    const uint8_t t = src.type[j];
+    wall = t == TSPH_WALL;
+    dst.r[k] = src.r[j];
+    dst.v[k] = src.v[j];
+    dst.a[k] = src.a[j];
+    dst.rho[k] = src.rho[j];
+    dst.drho[k] = src.drho[j];
+    dst.type[k] = t;
+    dst.seq[k] = src.seq[j];
+    dst.index[k] = src.index[j];
+    dst.r_o[k] = src.r_o[j];
+    dst.v_o[k] = src.v_o[j];
+    dst.rho_o[k] = src.rho_o[j];
+    dst.p[k] = src.p[j];
+    if (src.gnr) dst.gnr[k] = src.gnr[j];
+    if (src.n) dst.n[k] = src.n[j];
+  }
+  const unsigned long long m = __ballot(wall);
+  if (m) {
+    const int lane = threadIdx.x & 63;
+    int base = 0;
+    if (lane == __builtin_ctzll(m)) base = atomicAdd(&scal[S_NWALL], __popcll(m));
+    base = __shfl(base, __builtin_ctzll(m), 64);
+    if (wall) wall_list[base + __popcll(m & lanemask_lt())] = k;
   }
 }
 
-int scan_exclusive(const int *in, int len, int *out, int *partial, hipStream_t s) {
+int scan_exclusive(int *in, int len, int *out, int *partial, int *scal, hipStream_t s) {
   const int nb = (len + kScanTile - 1) / kScanTile;
   hipLaunchKernelGGL(k_scan_partials, dim3(nb), dim3(kScanBlock), 0, s, in, len, partial);
-  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, s, partial, nb, out, len);
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, s, partial, nb, out, len, scal);
   hipLaunchKernelGGL(k_scan_final, dim3(nb), dim3(kScanBlock), 0, s, in, len, partial, out);
   TSPH_HIP(hipGetLastError());
   return 0;
@@ -203,36 +285,27 @@ int scan_exclusive(const int *in, int len, int *out, int *partial, hipStream_t s
 
 }  // namespace
 
-int launch_fill_i32(int *p, int value, int count, hipStream_t s) {
-  if (count <= 0) return 0;
-  hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(count, 256)), dim3(256), 0, s, p, value, count);
-  TSPH_HIP(hipGetLastError());
-  return 0;
-}
-
-int scan_partials_needed(int len) { return (len + kScanTile - 1) / kScanTile + 1; }
-
 int launch_cell_list(tsph_ctx *c) {
   const Consts &k = c->k;
   const int nb = 256;
   const int g = grid_for(c->n_ub, nb);
   hipStream_t s = c->stream;
-  if (int e = launch_fill_i32(c->cell_count, 0, k.ncells + 1, s)) return e;
-  hipLaunchKernelGGL(k_hash, dim3(g), dim3(nb), 0, s, c->cur.r, c->cur.type, c->scal, k,
-                     c->cell_count, c->hcell, c->hrank, c->counters);
-  if (int e = scan_exclusive(c->cell_count, k.ncells + 1, c->cell_start, c->scan_tmp, s)) return e;
+  hipLaunchKernelGGL(k_hash, dim3(g), dim3(nb), 0, s, c->cur.r, c->cur.type, c->cur.seq, c->scal, k,
+                     c->cell_count, c->hcell, c->hrank, c->seqstate, c->counters);
+  if (int e = scan_exclusive(c->cell_count, k.ncells + 1, c->cell_start, c->scan_tmp, c->scal, s))
+    return e;
   hipLaunchKernelGGL(k_scatter, dim3(g), dim3(nb), 0, s, c->scal, c->hcell, c->hrank,
                      c->cell_start, c->perm, c->cellof);
   hipLaunchKernelGGL(k_order, dim3(grid_for(k.ncells, nb)), dim3(nb), 0, s, c->cell_start,
                      k.ncells, c->cur.seq, c->perm);
-  hipLaunchKernelGGL(k_reorder, dim3(g), dim3(nb), 0, s, c->cell_start, k.ncells, c->perm,
-                     c->cur, c->alt);
-  hipLaunchKernelGGL(k_set_np, dim3(1), dim3(64), 0, s, c->cell_start, k.ncells, c->scal);
+  hipLaunchKernelGGL(k_reorder, dim3(g), dim3(nb), 0, s, c->cell_start, k.ncells, c->perm, c->cur,
+                     c->alt, c->scal, c->wall_list);
   TSPH_HIP(hipGetLastError());
   std::swap(c->cur, c->alt);
   c->cells_valid = true;
   c->tiles_valid = false;
   c->tiles_zeroed = true;
+  c->records_valid = false;
   return 0;
 }
 

@@ -24,7 +24,7 @@ namespace tsph {
 constexpr uint8_t kRemoved = 255;
 
 // device scalar slots
-enum { S_NP = 0, S_MAXSEQ = 1, S_NTILES = 2, S_NSCAL = 4 };
+enum { S_NP = 0, S_MAXSEQ = 1, S_NTILES = 2, S_NWALL = 3, S_NSCAL = 4 };
 // device counter slots (uint64)
 enum {
   C_PAIRS = 0, C_CAND = 1, C_ACTIVE = 2, C_PASSES = 3, C_BADRHO = 4, C_OUTGRID = 5,
@@ -79,6 +79,10 @@ struct tsph_ctx {
   int *tile_list = nullptr;  // non-empty tiles of the current cell list
   bool tiles_valid = false;  // tile_list matches the cell list and tile_rows
   bool tiles_zeroed = false; // the device tile counter is 0 (set by the cell-list build)
+  int *wall_list = nullptr;  // slots of the wall particles of the current cell list
+  int n_wall = 0;            // wall particles (walls are never created or removed)
+  bool records_valid = false;  // mixed-path FP32 records match the FP64 state
+  bool marks_valid = false;    // removal marks match the current positions
   hipStream_t stream = nullptr;
   tsph::Soa cur, alt;
   tsph::Mixed mx;
@@ -115,7 +119,6 @@ int hip_fail(hipError_t e, const char *what, const char *file, int line);
 inline int grid_for(int n, int block) { return n <= 0 ? 1 : (n + block - 1) / block; }
 
 // tsph_cells.hip
-int launch_fill_i32(int *p, int value, int count, hipStream_t s);
 int launch_cell_list(tsph_ctx *c);  // hash, count, scan, scatter, order, reorder
 // tsph_pairs.hip
 int launch_forces(tsph_ctx *c, int variant);

@@ -631,16 +631,7 @@ __global__ void k_mixed_records(Soa d, const int *__restrict__ cellof, const int
   const int n = scal[S_NP];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const int c = cellof[i];
-  const int cx = c / k.ncy, cy = c - cx * k.ncy;
-  const double ox = k.x0 - k.of + cx * k.kh, oy = k.y0 - k.of + cy * k.kh;
-  const double2 r = d.r[i], v = d.v[i];
-  mx.rv[i] = make_float4((float)(r.x - ox), (float)(r.y - oy), (float)v.x, (float)v.y);
-  // density as its deviation from rho0: FP32 resolves rho - rho0 to ~1e-7 of
-  // its size, while FP32 rho itself (ulp 6e-5 at 1000) would lose the small
-  // density differences the diffusion term is built from
-  mx.rp[i] = make_float2((float)(d.rho[i] - k.rho0), (float)d.p[i]);
-  if (mx.nrm && d.n) mx.nrm[i] = make_float2((float)d.n[i].x, (float)d.n[i].y);
+  write_mixed_record(d, mx, cellof, k, i, d.rho[i], d.p[i]);
 }
 
 template <typename K>
@@ -687,7 +678,10 @@ int launch_forces(tsph_ctx *c, int variant) {
   const size_t list = sizeof(uint16_t) * kList * kWave;
   const int *sc = c->scal, *tl = c->tile_list;
   if (c->prm.precision == TSPH_PREC_MIXED) {
-    if (int e = launch_mixed_records(c)) return e;
+    if (!c->records_valid) {
+      if (int e = launch_mixed_records(c)) return e;
+      c->records_valid = true;
+    }
     const size_t lds = kTabBytes + list +
                        (size_t)cap * (3 * sizeof(float2) + sizeof(float) + 1 + (bt ? sizeof(float2) : 0));
     return launch_timed(c, [&] {

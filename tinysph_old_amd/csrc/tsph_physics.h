@@ -94,6 +94,23 @@ __device__ __forceinline__ double tait64(double rho, double rho0, double c0) {
 // support is below ~0.1 m of water), which avoids the cancellation of
 // computing pow(1+x, 1/7) - 1 in single precision.
 // ---------------------------------------------------------------------------
+// FP32 pair records of particle i on the mixed path: position relative to its
+// cell origin, velocity, and density as its deviation from rho0 (FP32
+// resolves rho - rho0 to ~1e-7 of its size, while FP32 rho itself, ulp 6e-5 at
+// 1000, would lose the small density differences the diffusion term is built
+// from)
+__device__ __forceinline__ void write_mixed_record(const Soa &d, const Mixed &mx,
+                                                   const int *__restrict__ cellof, const Consts &k,
+                                                   int i, double rho, double p) {
+  const int c = cellof[i];
+  const int cx = c / k.ncy, cy = c - cx * k.ncy;
+  const double ox = k.x0 - k.of + cx * k.kh, oy = k.y0 - k.of + cy * k.kh;
+  const double2 r = d.r[i], v = d.v[i];
+  mx.rv[i] = make_float4((float)(r.x - ox), (float)(r.y - oy), (float)v.x, (float)v.y);
+  mx.rp[i] = make_float2((float)(rho - k.rho0), (float)p);
+  if (mx.nrm && d.n) mx.nrm[i] = make_float2((float)d.n[i].x, (float)d.n[i].y);
+}
+
 __device__ __forceinline__ float hydro_drho_f32(float x, float rho0) {
   // (1+x)^(1/7) - 1 = x/7 - 3x^2/49 + 13x^3/343 - 65x^4/2401 + ...  (binomial series)
   const float t = x * (1.f / 7.f);

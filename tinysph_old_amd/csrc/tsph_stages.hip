@@ -22,18 +22,25 @@ namespace tsph {
 
 namespace {
 
-__global__ void k_eos(Soa d, const int *__restrict__ scal, Consts k,
-                      unsigned long long *__restrict__ counters) {
+// EOS; on the mixed path it also writes the FP32 pair records of every
+// particle (mx.rv != nullptr), which the pair loop that follows reads
+__global__ void k_eos(Soa d, Mixed mx, const int *__restrict__ cellof, const int *__restrict__ scal,
+                      Consts k, unsigned long long *__restrict__ counters) {
   const int n = scal[S_NP];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   unsigned bad = 0;
   if (i < n) {
     const int t = d.type[i];
+    const double rho = d.rho[i];
+    double p;
     if (t != TSPH_OUTLET && t != TSPH_INLET) {
-      const double rho = d.rho[i];
       if (rho > 1500. || rho < 0.) bad = 1;
-      d.p[i] = tait64(rho, k.rho0, k.cs);
+      p = tait64(rho, k.rho0, k.cs);
+      d.p[i] = p;
+    } else {
+      p = d.p[i];
     }
+    if (mx.rv) write_mixed_record(d, mx, cellof, k, i, rho, p);
   }
   const unsigned long long w = __ballot(bad);
   if (w && (threadIdx.x & 63) == 0) atomicAdd(&counters[C_BADRHO], (unsigned long long)__popcll(w));
@@ -53,7 +60,19 @@ __global__ void k_predictor(Soa d, const int *__restrict__ scal, double dt) {
   d.v[i] = make_double2(v.x + a.x * dt * 0.5, v.y + a.y * dt * 0.5);
 }
 
-__global__ void k_corrector(Soa d, const int *__restrict__ scal, double dt) {
+// removal state of a particle at reference position seq: 0 absent, 1 kept,
+// 2 candidate (fluid outside the domain by more than eps, src/SPH_btdebug.h:17-22)
+__device__ __forceinline__ uint8_t removal_state(const Consts &k, int type, double2 r) {
+  if (type == kRemoved) return 0;
+  const bool out = type == TSPH_FLUID && ((r.y < (k.y0 - kEps)) || (r.y > k.ym + kEps) ||
+                                          (r.x < k.x0 - kEps) || (r.x > k.xm - kEps));
+  return out ? 2 : 1;
+}
+
+// corrector; also records the removal state for the RemoveParticlesOutOfDomain
+// that follows it in every driver
+__global__ void k_corrector(Soa d, const int *__restrict__ scal, Consts k, double dt,
+                            uint8_t *__restrict__ state) {
   const int n = scal[S_NP];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -62,7 +81,9 @@ __global__ void k_corrector(Soa d, const int *__restrict__ scal, double dt) {
   const double2 vo = d.v_o[i], a = d.a[i], ro = d.r_o[i];
   const double2 v = make_double2(vo.x + a.x * dt, vo.y + a.y * dt);
   d.v[i] = v;
-  d.r[i] = make_double2(ro.x + (v.x + vo.x) * dt * 0.5, ro.y + (v.y + vo.y) * dt * 0.5);
+  const double2 r = make_double2(ro.x + (v.x + vo.x) * dt * 0.5, ro.y + (v.y + vo.y) * dt * 0.5);
+  d.r[i] = r;
+  state[d.seq[i]] = removal_state(k, d.type[i], r);
 }
 
 // cell of a sampling point, computed like Particle_To_Cell
@@ -128,11 +149,12 @@ __device__ double mdbc_density_at(const Soa &d, const int *__restrict__ cell_sta
   return 1000.;
 }
 
+// one thread per wall particle of the wall list built with the cell list
 __global__ void k_mdbc(Soa d, const int *__restrict__ cell_start, const int *__restrict__ scal,
-                       Consts k) {
-  const int n = scal[S_NP];
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n || d.type[i] != TSPH_WALL) return;
+                       const int *__restrict__ wall_list, Consts k) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= scal[S_NWALL]) return;
+  const int i = wall_list[w];
   const double2 gn = d.gnr[i], r = d.r[i];
   const double2 gnd = make_double2(r.x - gn.x, r.y - gn.y);
   double rho = mdbc_density_at(d, cell_start, k, gn, gnd);
@@ -141,10 +163,10 @@ __global__ void k_mdbc(Soa d, const int *__restrict__ cell_start, const int *__r
 }
 
 __global__ void k_bt_density(Soa d, const int *__restrict__ cell_start, const int *__restrict__ scal,
-                             Consts k) {
-  const int n = scal[S_NP];
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n || d.type[i] != TSPH_WALL) return;
+                             const int *__restrict__ wall_list, Consts k) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= scal[S_NWALL]) return;
+  const int i = wall_list[w];
   const double h = k.h, m = k.m;
   const double r2cut = 4.0 * h * h * (1.0 + 1e-12);
   const double2 ar = d.r[i];
@@ -177,16 +199,13 @@ __global__ void k_bt_density(Soa d, const int *__restrict__ cell_start, const in
 // examined: in a run of consecutive candidates every other one (the 1st, 3rd,
 // ...) is removed.  State per reference position (seq): 0 absent, 1 kept,
 // 2 candidate.
+// marks for a removal that does not directly follow a corrector
 __global__ void k_rm_mark(Soa d, const int *__restrict__ scal, Consts k,
                           uint8_t *__restrict__ state) {
   const int n = scal[S_NP];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const double2 r = d.r[i];
-  const bool out = d.type[i] == TSPH_FLUID &&
-                   ((r.y < (k.y0 - kEps)) || (r.y > k.ym + kEps) || (r.x < k.x0 - kEps) ||
-                    (r.x > k.xm - kEps));
-  state[d.seq[i]] = d.type[i] == kRemoved ? 0 : (out ? 2 : 1);
+  state[d.seq[i]] = removal_state(k, d.type[i], d.r[i]);
 }
 
 __global__ void k_rm_decide(Soa d, const int *__restrict__ scal, const uint8_t *__restrict__ state,
@@ -209,11 +228,6 @@ __global__ void k_rm_decide(Soa d, const int *__restrict__ scal, const uint8_t *
     d.type[i] = kRemoved;
     atomicAdd(&counters[C_REMOVED], 1ull);
   }
-}
-
-__global__ void k_fill_u8(uint8_t *p, int count) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < count) p[i] = 0;
 }
 
 // Shepard volume sum, src/SPH_water_level.h:6-82 (all particle types)
@@ -323,9 +337,12 @@ __global__ void k_interp_point(Soa d, const int *__restrict__ cell_start, Consts
 }  // namespace
 
 int launch_eos(tsph_ctx *c) {
-  hipLaunchKernelGGL(k_eos, dim3(grid_for(c->n_ub, 256)), dim3(256), 0, c->stream, c->cur, c->scal,
-                     c->k, c->counters);
+  Mixed mx = c->prm.precision == TSPH_PREC_MIXED ? c->mx : Mixed{};
+  hipLaunchKernelGGL(k_eos, dim3(grid_for(c->n_ub, 256)), dim3(256), 0, c->stream, c->cur, mx,
+                     c->cellof, c->scal, c->k, c->counters);
   TSPH_HIP(hipGetLastError());
+  // the records are only meaningful for the cell list they were built on
+  c->records_valid = c->prm.precision == TSPH_PREC_MIXED && c->cells_valid;
   return 0;
 }
 
@@ -333,39 +350,45 @@ int launch_predictor(tsph_ctx *c, double dt) {
   hipLaunchKernelGGL(k_predictor, dim3(grid_for(c->n_ub, 256)), dim3(256), 0, c->stream, c->cur,
                      c->scal, dt);
   TSPH_HIP(hipGetLastError());
+  c->records_valid = false;
+  c->marks_valid = false;
   return 0;
 }
 
 int launch_corrector(tsph_ctx *c, double dt) {
   hipLaunchKernelGGL(k_corrector, dim3(grid_for(c->n_ub, 256)), dim3(256), 0, c->stream, c->cur,
-                     c->scal, dt);
+                     c->scal, c->k, dt, c->seqstate);
   TSPH_HIP(hipGetLastError());
+  c->records_valid = false;
+  c->marks_valid = true;
   return 0;
 }
 
 int launch_mdbc(tsph_ctx *c) {
-  hipLaunchKernelGGL(k_mdbc, dim3(grid_for(c->n_ub, 256)), dim3(256), 0, c->stream, c->cur,
-                     c->cell_start, c->scal, c->k);
+  hipLaunchKernelGGL(k_mdbc, dim3(grid_for(c->n_wall, 256)), dim3(256), 0, c->stream, c->cur,
+                     c->cell_start, c->scal, c->wall_list, c->k);
   TSPH_HIP(hipGetLastError());
+  c->records_valid = false;
   return 0;
 }
 
 int launch_bt_density(tsph_ctx *c) {
-  hipLaunchKernelGGL(k_bt_density, dim3(grid_for(c->n_ub, 256)), dim3(256), 0, c->stream, c->cur,
-                     c->cell_start, c->scal, c->k);
+  hipLaunchKernelGGL(k_bt_density, dim3(grid_for(c->n_wall, 256)), dim3(256), 0, c->stream, c->cur,
+                     c->cell_start, c->scal, c->wall_list, c->k);
   TSPH_HIP(hipGetLastError());
+  c->records_valid = false;
   return 0;
 }
 
 int launch_remove(tsph_ctx *c) {
-  // seq values are < capacity (reference positions of the current run)
-  const int ns = c->capacity;
-  hipLaunchKernelGGL(k_fill_u8, dim3(grid_for(ns, 256)), dim3(256), 0, c->stream, c->seqstate, ns);
-  hipLaunchKernelGGL(k_rm_mark, dim3(grid_for(c->n_ub, 256)), dim3(256), 0, c->stream, c->cur,
-                     c->scal, c->k, c->seqstate);
+  if (!c->marks_valid) {
+    hipLaunchKernelGGL(k_rm_mark, dim3(grid_for(c->n_ub, 256)), dim3(256), 0, c->stream, c->cur,
+                       c->scal, c->k, c->seqstate);
+  }
   hipLaunchKernelGGL(k_rm_decide, dim3(grid_for(c->n_ub, 256)), dim3(256), 0, c->stream, c->cur,
                      c->scal, c->seqstate, c->counters);
   TSPH_HIP(hipGetLastError());
+  c->marks_valid = false;  // removed particles are re-marked as absent next time
   return 0;
 }
 
