@@ -1,0 +1,301 @@
+"""Closed forms and invariants that pin the oracle independently of the
+reference build: each test is chosen so that a plausible slip (dropped term,
+wrong sign or index, transposed operand) fails it."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from tests.helpers import brute_force_pairs
+
+
+@pytest.fixture(scope="module")
+def L(oracle_built):
+    return po.oracle_lib()
+
+
+def test_wendland_special_values(L):
+    h = 0.015
+    awen = float(np.float32(0.557 / (h * h)))
+    assert L.orc_wendland_w(0.0, h) == awen          # r = 0 branch
+    assert L.orc_wendland_f(0.0, h) == 0.0
+    assert L.orc_wendland_w(2 * h, h) == 0.0         # (1 - q/2) = 0 at q = 2
+    assert L.orc_wendland_w(2.0001 * h, h) == 0.0    # outside the support
+    assert L.orc_wendland_f(2.0001 * h, h) == 0.0
+    # closed form W = awen (1 - q/2)^4 (2q + 1) at an interior point, to the
+    # float rounding of the reference's intermediates
+    q = 0.7
+    w = awen * (1 - q / 2) ** 4 * (2 * q + 1)
+    assert abs(L.orc_wendland_w(q * h, h) - w) <= 1e-6 * w
+
+
+def test_wendland_normalisation(L):
+    """sum W dA over a fine lattice = 0.557 * 4 pi / 7 (the reference rounds
+    7/(4 pi) = 0.5570423 to 0.557)"""
+    h, d = 0.01, 0.0002
+    xs = np.arange(-2 * h, 2 * h + d / 2, d)
+    total = 0.0
+    for x in xs:
+        for y in xs:
+            total += L.orc_wendland_w(math.hypot(x, y), h)
+    total *= d * d
+    assert abs(total - 0.557 * 4 * math.pi / 7) <= 2e-4
+
+
+def test_wendland_gradient_matches_derivative(L):
+    """F is dW/dr / r: a transposed operand or wrong power fails this"""
+    h = 0.01
+    for q in (0.3, 0.9, 1.5, 1.9):
+        r = q * h
+        # the kernel's intermediates are float-rounded (relative noise ~6e-8),
+        # so the difference step must be much larger than that noise
+        e = 1e-3 * h
+        dwdr = (L.orc_wendland_w(r + e, h) - L.orc_wendland_w(r - e, h)) / (2 * e)
+        assert abs(L.orc_wendland_f(r, h) * r - dwdr) <= 2e-3 * abs(dwdr)
+
+
+def test_artificial_viscosity(L):
+    h, c0, alpha, rb = 0.01, 20.0, 0.02, 1000.0
+    assert L.orc_artificial_viscosity(h, 0.01, 0.3, rb, c0, alpha) == 0.0   # receding
+    assert L.orc_artificial_viscosity(h, 0.01, 0.0, rb, c0, alpha) == 0.0
+    drs, drdv = 0.012, -0.05
+    mu = h * drdv / (drs * drs + 0.01 * h * h)
+    assert L.orc_artificial_viscosity(h, drs, drdv, rb, c0, alpha) == pytest.approx(
+        -alpha * c0 * mu / rb, rel=1e-15)
+    assert L.orc_artificial_viscosity(h, drs, drdv, rb, c0, alpha) > 0
+
+
+def test_diffusion_term_vanishes_for_hydrostatic_pair(L):
+    """delta-SPH (Fourtakas): zero when rho_j - rho_i equals the hydrostatic
+    density difference of dr_z"""
+    h, c0, rho0, delta, m, g = 0.01, 20.0, 1000.0, 0.1, 0.05, 9.81
+    dz = 0.013
+    cb = c0 * c0 * rho0 / 7.0
+    drhoH = rho0 * (1 + dz * rho0 * g / cb) ** (1 / 7) - rho0
+    arho = 1002.0
+    hyd = L.orc_diffusion_fourtakas(h, 0.015, dz, 1.0, arho, arho + drhoH, rho0, c0, delta, m, g)
+    off = L.orc_diffusion_fourtakas(h, 0.015, dz, 1.0, arho, arho + drhoH + 0.1, rho0, c0, delta, m, g)
+    assert abs(hyd) <= 1e-9 * abs(off)
+    # sign: a denser neighbour above the hydrostatic profile diffuses mass in
+    assert off > 0
+
+
+def test_equation_of_state(L):
+    rho0, c0 = 1000.0, 22.0
+    assert L.orc_tait(rho0, rho0, c0) == 0.0
+    b = c0 * c0 * rho0 / 7
+    assert L.orc_tait(1010.0, rho0, c0) == pytest.approx(b * ((1.01) ** 7 - 1), rel=1e-13)
+    for p in (0.0, 500.0, 4905.0, 20000.0):
+        assert L.orc_tait(L.orc_pressure_to_density(p, rho0, c0), rho0, c0) == pytest.approx(
+            p, rel=1e-9, abs=1e-9)
+
+
+def single_particle_case(vx=0.3, vy=0.7):
+    dp = 0.01
+    h = 1.5 * dp
+    q = [h, 2.0, 20.0, 1000., 0.01, 0.1, dp, 0.0, 1.0, 0.0, 1.0]
+    parts = dict(type=np.array([0], dtype=np.int32), p=np.array([0.0]), rho=np.array([1000.0]),
+                 r=np.array([[0.5, 0.5]]), v=np.array([[vx, vy]]))
+    return po.oracle_case(q, parts)
+
+
+def test_symplectic_reproduces_free_fall_parabola():
+    """A lone particle feels only gravity (a = -g, dRho = 0); the symplectic
+    predictor/corrector pair integrates that exactly: r = r0 + v0 t - g t^2/2."""
+    O = single_particle_case()
+    O.call("divide_to_cells")
+    assert O.call("particle_to_cell") == 0
+    O.call("compute_forces", 0)
+    dt = 1e-3
+    for _ in range(50):
+        O.call("predictor", dt)
+        O.call("particle_to_cell")
+        O.call("compute_forces", 0)
+        O.call("corrector", dt)
+        O.call("particle_to_cell")
+        O.call("compute_forces", 0)
+    t = 50 * dt
+    r = O.get(po.F_R)[0]
+    assert r[0] == pytest.approx(0.5 + 0.3 * t, rel=1e-13)
+    assert r[1] == pytest.approx(0.5 + 0.7 * t - 0.5 * 9.81 * t * t, rel=1e-12)
+    assert O.get(po.F_RHO)[0] == 1000.0
+
+
+def fluid_blob(seed=3, n=12, dp=0.01):
+    rng = np.random.default_rng(seed)
+    xs, ys = np.meshgrid(np.arange(n) * dp + 0.3, np.arange(n) * dp + 0.3, indexing="ij")
+    r = np.stack([xs.ravel(), ys.ravel()], 1) + rng.uniform(-0.2 * dp, 0.2 * dp, (n * n, 2))
+    v = rng.normal(0, 0.05, (n * n, 2))
+    rho = 1000.0 + rng.uniform(-3, 3, n * n)
+    q = [1.5 * dp, 2.0, 20.0, 1000., 0.02, 0.1, dp, 0.0, 1.0, 0.0, 1.0]
+    parts = dict(type=np.zeros(n * n, dtype=np.int32), p=np.zeros(n * n), rho=rho, r=r, v=v)
+    return q, parts
+
+
+def test_momentum_conservation_of_pair_forces():
+    """Fluid-only interactions: pressure and viscosity terms are antisymmetric in
+    (i, j), so sum_i m a_i = -N m g.  A sign or index slip in either breaks it."""
+    q, parts = fluid_blob()
+    O = po.oracle_case(q, parts)
+    O.call("divide_to_cells")
+    assert O.call("particle_to_cell") == 0
+    O.call("density_to_pressure")
+    O.call("compute_forces", 0)
+    a = O.get(po.F_A)
+    n = len(a)
+    net = a.sum(axis=0) + np.array([0.0, 9.81]) * n
+    scale = np.abs(a).sum()
+    assert abs(net[0]) <= 1e-12 * scale and abs(net[1]) <= 1e-12 * scale
+
+
+def test_pair_count_equals_brute_force():
+    """Every pair within the support is found through the 9-cell lists."""
+    q, parts = fluid_blob(n=10)
+    O = po.oracle_case(q, parts)
+    O.call("divide_to_cells")
+    O.call("particle_to_cell")
+    start, items = O.cell_csr()
+    ncx, ncy = O.grid()
+    r = parts["r"]
+    h = q[0]
+    found = 0
+    cell_of = {}
+    for c in range(ncx * ncy):
+        for k in items[start[c]:start[c + 1]]:
+            cell_of[int(k)] = c
+    for i in range(len(r)):
+        c = cell_of[i]
+        for cl in (c, c + 1, c + 1 + ncy, c + ncy, c - 1 + ncy, c - 1, c - 1 - ncy, c - ncy,
+                   c + 1 - ncy):
+            if cl < 0:
+                continue
+            for j in items[start[cl]:start[cl + 1]]:
+                if j != i and math.hypot(*(r[i] - r[j])) / h < 2.0:
+                    found += 1
+    assert found == brute_force_pairs(r, np.ones(len(r), bool), h)
+
+
+def test_cell_index_closed_form():
+    """cell = trunc((y - y0 + 0.273h)/kh) + trunc((x - x0 + 0.273h)/kh) * ncy"""
+    q, parts = fluid_blob(n=8)
+    O = po.oracle_case(q, parts)
+    O.call("divide_to_cells")
+    O.call("particle_to_cell")
+    start, items = O.cell_csr()
+    ncx, ncy = O.grid()
+    h, kap = q[0], q[1]
+    kh, of = kap * h, h * 0.273
+    assert ncx == int((q[8] - q[7] + of) / kh) + 1
+    for c in range(ncx * ncy):
+        for k in items[start[c]:start[c + 1]]:
+            x, y = parts["r"][k]
+            assert c == int((y - q[9] + of) / kh) + int((x - q[7] + of) / kh) * ncy
+    # ascending particle index inside each cell
+    for c in range(ncx * ncy):
+        seg = items[start[c]:start[c + 1]]
+        assert np.all(np.diff(seg) > 0)
+
+
+def wall_over_lattice(field):
+    """A fluid lattice above one wall particle whose ghost node sits inside the
+    fluid with full kernel support; fluid density given by field(x, y)."""
+    dp = 0.01
+    h = 1.5 * dp
+    xs, ys = np.meshgrid(np.arange(21) * dp + 0.2, np.arange(12) * dp + 0.2, indexing="ij")
+    r = np.stack([xs.ravel(), ys.ravel()], 1)
+    rho = np.array([field(x, y) for x, y in r])
+    wall_r = np.array([[0.3, 0.2 - 1.5 * dp]])
+    gn = np.array([[0.3, 0.2 + 2.5 * dp]])  # mirror point inside the fluid
+    n = len(r) + 1
+    parts = dict(type=np.r_[1, np.zeros(len(r), dtype=np.int32)].astype(np.int32),
+                 p=np.zeros(n), rho=np.r_[1000.0, rho], r=np.r_[wall_r, r], v=np.zeros((n, 2)),
+                 gnr=np.r_[gn, np.zeros((len(r), 2))], n=np.r_[[[0.0, 1.0]], np.zeros((len(r), 2))])
+    q = [h, 2.0, 20.0, 1000., 0.01, 0.1, dp, 0.0, 1.0, 0.0, 1.0]
+    return q, parts
+
+
+def test_mdbc_reproduces_linear_density_field():
+    """The corrected (3x3 moment matrix) ghost-node estimate is first-order
+    consistent: a linear field is extrapolated exactly to the wall particle."""
+    f = lambda x, y: 1010.0 + 30.0 * (y - 0.2) + 5.0 * (x - 0.3)
+    q, parts = wall_over_lattice(f)
+    O = po.oracle_case(q, parts)
+    O.call("divide_to_cells")
+    assert O.call("particle_to_cell") == 0
+    O.call("mdbc_density")
+    wx, wy = parts["r"][0]
+    assert O.get(po.F_RHO)[0] == pytest.approx(f(wx, wy), rel=1e-11)
+
+
+def test_bt_wall_density_of_constant_field():
+    """BT wall density is a Shepard average: a constant field is reproduced."""
+    q, parts = wall_over_lattice(lambda x, y: 1005.0)
+    parts["r"][0] = (0.3, 0.2 - 0.5 * 0.01)  # within the support of the fluid
+    O = po.oracle_case(q, parts)
+    O.call("divide_to_cells")
+    O.call("particle_to_cell")
+    O.call("bt_density")
+    assert O.get(po.F_RHO)[0] == pytest.approx(1005.0, rel=1e-13)
+
+
+def test_removal_matches_sequential_erase_semantics():
+    """Brute-force restatement of the in-loop erase (no step back after an
+    erase) against the oracle on random run patterns."""
+    rng = np.random.default_rng(11)
+    for trial in range(20):
+        n = 40
+        out = rng.random(n) < 0.4
+        types = np.where(rng.random(n) < 0.15, 1, 0).astype(np.int32)
+        r = np.where(out[:, None], [[2.0, 0.5]], [[0.5, 0.5]])
+        # reference loop on a python list
+        alive = list(range(n))
+        p = 0
+        while p < len(alive):
+            k = alive[p]
+            if types[k] == 0 and out[k]:
+                alive.pop(p)  # erase; p is not decremented
+            p += 1
+        q = [0.015, 2.0, 20.0, 1000., 0.01, 0.1, 0.01, 0.0, 1.0, 0.0, 1.0]
+        parts = dict(type=types, p=np.zeros(n), rho=np.full(n, 1000.0), r=r.astype(float),
+                     v=np.zeros((n, 2)))
+        O = po.oracle_case(q, parts)
+        O.call("remove_out_of_domain")
+        kept = O.get(po.F_R)
+        assert O.np == len(alive)
+        assert np.array_equal(kept, r[alive])
+
+
+def test_water_level_of_still_block():
+    """The level of a uniform block lies within one spacing of its top row."""
+    dp = 0.01
+    xs, ys = np.meshgrid(np.arange(40) * dp + 0.1, np.arange(30) * dp + 0.1, indexing="ij")
+    r = np.stack([xs.ravel(), ys.ravel()], 1)
+    n = len(r)
+    q = [1.5 * dp, 2.0, 20.0, 1000., 0.01, 0.1, dp, 0.0, 1.0, 0.0, 1.0]
+    O = po.oracle_case(q, dict(type=np.zeros(n, dtype=np.int32), p=np.zeros(n),
+                                rho=np.full(n, 1000.0), r=r, v=np.zeros((n, 2))))
+    O.call("divide_to_cells")
+    O.call("particle_to_cell")
+    top = 0.1 + 29 * dp
+    wl = O.call("water_elevation", 0.3, 0.15)
+    assert top - dp <= wl <= top + dp
+    wli = O.call("water_elevation_inverse", 0.3, 0.6)
+    assert top - dp <= wli <= top + dp
+
+
+def test_hydrostatic_probe_pressure():
+    """Shepard pressure probe inside a hydrostatic column: p = rho0 g (H - y)"""
+    dp, H = 0.01, 0.3
+    xs, ys = np.meshgrid(np.arange(30) * dp + 0.1, np.arange(30) * dp + 0.01, indexing="ij")
+    r = np.stack([xs.ravel(), ys.ravel()], 1)
+    n = len(r)
+    p = 1000.0 * 9.81 * (H - r[:, 1])
+    q = [1.5 * dp, 2.0, 20.0, 1000., 0.01, 0.1, dp, 0.0, 1.0, 0.0, 1.0]
+    O = po.oracle_case(q, dict(type=np.zeros(n, dtype=np.int32), p=p, rho=np.full(n, 1000.0),
+                                r=r, v=np.zeros((n, 2))))
+    O.call("divide_to_cells")
+    O.call("particle_to_cell")
+    for y in (0.05, 0.12, 0.2):
+        got = O.interpolate_point(0.25, y)[1]
+        assert got == pytest.approx(1000.0 * 9.81 * (H - y), rel=2e-3)

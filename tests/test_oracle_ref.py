@@ -1,0 +1,162 @@
+"""Pins the oracle to the reference: the reference's own stage functions,
+compiled from its headers in place (oracle/_ref, built by `make -C oracle ref`),
+against the oracle restatement on identical inputs, bit for bit.
+
+The reference code is single-threaded here (built without -fopenmp: its kernel
+returns a static buffer, src/SPH_kernel.h:54).  Cases are built by the
+reference's own geometry routines (src/SPH_create.h, src/SPH_create_bt.h)."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from tests.helpers import oracle_step
+
+pytestmark = pytest.mark.skipif(
+    not (po.ref_available("dbc") and po.ref_available("wcsph") and po.ref_available("bt")),
+    reason="oracle/_ref not built (needs the reference tree; `make -C oracle ref`)")
+
+FIELDS = (po.F_R, po.F_V, po.F_A, po.F_P, po.F_RHO, po.F_DRHO, po.F_RO, po.F_VO, po.F_RHOO)
+
+
+def still_water(variant="dbc", dp=0.02, W=1.0):
+    """cases/stillWater_DBC/geometry.h with a W-wide tank"""
+    h, kap = 1.5 * dp, 2.0
+    cs = 10 * math.sqrt(9.81 * 0.5)
+    nvl = math.ceil(kap * h / dp)
+    q = [h, kap, cs, 1000., 0.01, 0.1, dp, -nvl * dp, W + 0.2 + nvl * dp, -nvl * dp, 1.0 + nvl * dp]
+    R = po.Case(po.ref_lib(variant), "ref_", q)
+    R.call("create_wall_x_mdbc", 0. + dp, W - dp, 0., -1, dp / 2)
+    R.call("create_wall_y_mdbc", 0., 0 + dp, 1., -1, dp / 2)
+    R.call("create_wall_y_mdbc", W, 0 + dp, 1., +1, W - dp / 2)
+    R.call("create_corner_mdbc", -1, -1, 0. + dp / 2, 0. + dp / 2)
+    R.call("create_corner_mdbc", +1, -1, W - dp / 2, 0. + dp / 2)
+    R.call("create_block_of_fluid_wp", 0 + dp, 0 + dp, W - dp, 0.5)
+    return R, q, 0.2 * h / cs
+
+
+def dam_break_mdbc(dp=0.02, L=1.61):
+    """cases/damBreakLong_mDBC/geometry.h"""
+    h, kap, cs = math.sqrt(2 * dp * dp), 2.1, 34.3
+    nvl = math.ceil(kap * h / dp)
+    q = [h, kap, cs, 1000., 0.02, 0.1, dp, -nvl * dp, 2. + nvl * dp, -nvl * dp, 1. + nvl * dp]
+    R = po.Case(po.ref_lib("wcsph"), "ref_", q)
+    R.call("create_wall_x_mdbc", 0. + dp, L, 0., -1, dp / 2)
+    R.call("create_wall_y_mdbc", 0., 0 + dp, 1., -1, dp / 2)
+    R.call("create_wall_y_mdbc", L, 0 + dp, 1., +1, L - dp / 2)
+    R.call("create_corner_mdbc", -1, -1, 0. + dp / 2, 0. + dp / 2)
+    R.call("create_corner_mdbc", +1, -1, L - dp / 2, 0. + dp / 2)
+    R.call("create_block_of_fluid", 0 + dp, 0 + dp, 0.6, 0.3)
+    return R, q, 0.00005 * dp / 0.005
+
+
+def dam_break_bt(dp=0.02):
+    """cases/damBreakLong_BT/geometry.h"""
+    h, kap, cs = math.sqrt(2 * dp * dp), 2.1, 34.3
+    nvl = math.ceil(kap * h / dp)
+    q = [h, kap, cs, 1000., 0.02, 0.1, dp, -nvl * dp, 2. + nvl * dp, -nvl * dp, 1. + nvl * dp]
+    R = po.Case(po.ref_lib("bt"), "ref_", q)
+    R.call("create_wall_x_bt", 0. + dp, 1.61 - dp, 0., -1, 0., 1.)
+    R.call("create_wall_y_bt", 0., 0. + dp, 1., -1, 1., 0.)
+    R.call("create_wall_y_bt", 1.61, 0. + dp, 1., +1, -1., 0.)
+    R.call("create_corner_bt", -1, -1, 0., 0., math.sqrt(2), math.sqrt(2))
+    R.call("create_corner_bt", +1, -1, 1.61, 0., -math.sqrt(2), math.sqrt(2))
+    R.call("create_block_of_fluid_wp", 0.0 + dp, 0 + dp, 0.6, 0.3)
+    return R, q, 0.00004 * dp / 0.005
+
+
+def oracle_from_ref(R, q, extra=None):
+    """Oracle case holding exactly the reference's particles."""
+    parts = dict(type=R.types(), p=R.get(po.F_P), rho=R.get(po.F_RHO), r=R.get(po.F_R),
+                 v=R.get(po.F_V))
+    n = len(parts["type"])
+    if extra is not None:
+        e = R.get(extra)
+        full = np.zeros((n, 2))
+        full[: len(e)] = e  # the reference stores wall extras for the walls, created first
+        parts["gnr" if extra == po.F_GNR else "n"] = full
+    return po.oracle_case(q, parts)
+
+
+def ref_step(R, wall, dt):
+    """One step in the driver order, through the reference's functions."""
+    if wall in ("mdbc", "bt"):
+        R.call("wall_density")
+        R.call("density_to_pressure")
+    R.call("predictor", dt)
+    R.call("clear_cells"); R.call("particle_to_cell"); R.call("density_to_pressure")
+    R.call("compute_forces")
+    R.call("corrector", dt)
+    R.call("remove_out_of_domain")
+    R.call("clear_cells"); R.call("particle_to_cell"); R.call("density_to_pressure")
+    R.call("compute_forces")
+
+
+def assert_same(R, O, fields=FIELDS):
+    assert R.np == O.np
+    assert np.array_equal(R.types(), O.types())
+    for f in fields:
+        a, b = R.get(f), O.get(f)
+        assert a.shape == b.shape, f
+        assert np.array_equal(a, b), (f, float(np.max(np.abs(a - b))))
+
+
+@pytest.mark.parametrize("builder,wall,extra,ovar", [
+    (still_water, "dbc", None, 1),
+    (dam_break_mdbc, "mdbc", po.F_GNR, 0),
+    (dam_break_bt, "bt", po.F_N, 2),
+])
+def test_steps_bit_exact(builder, wall, extra, ovar):
+    R, q, dt = builder()
+    O = oracle_from_ref(R, q, extra)
+    for c in (R, O):
+        c.call("divide_to_cells")
+        c.call("particle_to_cell")
+    assert R.grid() == O.grid()
+    s1, i1 = R.cell_csr()
+    s2, i2 = O.cell_csr()
+    assert np.array_equal(s1, s2) and np.array_equal(i1, i2)
+    # initial forces (as after INIT), then 15 steps
+    R.call("density_to_pressure"); R.call("compute_forces")
+    O.call("density_to_pressure"); O.call("compute_forces", ovar)
+    assert_same(R, O)
+    wmap = {"dbc": 0, "mdbc": 1, "bt": 2}
+    for _ in range(15):
+        ref_step(R, wall, dt)
+        oracle_step(O, wmap[wall], dt)
+    assert_same(R, O)
+
+
+def test_probes_bit_exact():
+    R, q, dt = dam_break_mdbc(dp=0.01)
+    O = oracle_from_ref(R, q, po.F_GNR)
+    for c in (R, O):
+        c.call("divide_to_cells"); c.call("particle_to_cell"); c.call("density_to_pressure")
+    for x in (0.1, 0.3, 0.45):
+        assert R.call("water_elevation", x, 0.0) == O.call("water_elevation", x, 0.0)
+        assert R.call("water_elevation_inverse", x, 0.5) == O.call("water_elevation_inverse", x, 0.5)
+        for y in (0.02, 0.1, 0.25):
+            assert R.call("sum_kernel_function", x, y) == O.call("sum_kernel_function", x, y)
+            assert np.array_equal(R.interpolate_point(x, y), O.interpolate_point(x, y))
+
+
+def test_removal_bit_exact():
+    """A run of consecutive out-of-domain fluid particles: the reference's
+    in-loop erase removes every other one."""
+    R, q, dt = still_water(dp=0.05)
+    r = R.get(po.F_R)
+    t = R.types()
+    fl = np.nonzero(t == 0)[0]
+    for k in (fl[3], fl[4], fl[5], fl[6], fl[9]):
+        r[k] = (q[8] + 0.1, 0.3)
+    R.set(po.F_R, r)
+    O = oracle_from_ref(R, q)
+    for c in (R, O):
+        c.call("divide_to_cells")
+    R.call("remove_out_of_domain")
+    removed = O.call("remove_out_of_domain")
+    assert removed == 3  # fl[3], fl[5], fl[9]; fl[4] and fl[6] are skipped
+    assert R.np == O.np
+    assert np.array_equal(R.get(po.F_R), O.get(po.F_R))
+    assert np.array_equal(R.types(), O.types())
