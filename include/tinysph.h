@@ -169,6 +169,30 @@ int tsph_synchronize(tsph_ctx *ctx);
 int tsph_pair_timing(tsph_ctx *ctx, int max_launches);
 int tsph_pair_timing_read(tsph_ctx *ctx, double *total_ms, int *launches);
 
+/* ---- Riemann-ALE-SPH (first order), the pipeline of the ALE dam-break driver
+ * cases/ALESPH_dambreakLong/SPH_simulationINT.h:84-132 ---- */
+
+/* ALE volume variables (SPH_special, src/SPH_data.h:54-67), reference order;
+ * vector fields interleaved {x, y} */
+typedef struct tsph_ale_fields {
+  int np;
+  double *omega, *omegarho, *omegav;
+  double *omega_o, *omegarho_o, *omegav_o;
+  double *domega, *domegarho, *domegav;
+} tsph_ale_fields;
+
+/* Allocates the ALE variables and sets them as ALESPH_Add_particle does
+ * (src/SPH_particle_system.h:51-108): omega = dp^2, omega*rho, omega*v. */
+int tsph_ale_enable(tsph_ctx *ctx);
+int tsph_ale_download(tsph_ctx *ctx, tsph_ale_fields *host);
+/* Compute_Forces of src/RALESPH_interaction.h:12-287 */
+int tsph_rale_forces(tsph_ctx *ctx);
+/* Integrate_LeapFrog_partOne / partTwo, src/RALESPH_integration.h:10-107 */
+int tsph_rale_leapfrog_one(tsph_ctx *ctx, double dt);
+int tsph_rale_leapfrog_two(tsph_ctx *ctx, double dt, int step);
+/* nsteps whole steps numbered from `step` (the driver's step counter, from 1) */
+int tsph_step_rale(tsph_ctx *ctx, double dt, int step, int nsteps);
+
 /* ---- case setup (host only; no GPU needed) ----
  * Presets built with the reference's geometry routines (src/SPH_create.h,
  * src/SPH_create_bt.h) in the order of the reference case geometry files. */
@@ -177,12 +201,20 @@ int tsph_pair_timing_read(tsph_ctx *ctx, double *total_ms, int *launches);
                                        1.2x0.6 m column, fluid positions jittered */
 #define TSPH_CASE_DAM_BREAK_BT 3    /* cases/damBreakLong_BT/geometry.h, 1.61x1.0 m tank,
                                        0.6x0.3 m column */
+#define TSPH_CASE_ALE_DAM_BREAK 4   /* cases/ALESPH_dambreakLong/geometry.h (BT walls, ALE
+                                       particles) in the 3.22x1.8 m tank with the 1.2x0.6 m
+                                       column, dx = 2 mm */
+
+/* time-stepping scheme of a preset */
+#define TSPH_SCHEME_SYMPLECTIC 0 /* WCSPH, tsph_step_symplectic */
+#define TSPH_SCHEME_RALE 1       /* Riemann-ALE-SPH, tsph_step_rale */
 
 typedef struct tsph_case {
   tsph_params prm;           /* constants and padded domain (capacity = np) */
   double dt;                 /* time step of the preset */
   int wall;                  /* TSPH_WALL_* */
   int steps;                 /* step count of the preset */
+  int scheme;                /* TSPH_SCHEME_* */
   int n_fluid, n_wall;
   tsph_particles particles;  /* arrays owned by the case */
 } tsph_case;

@@ -18,7 +18,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 REF_DIR = os.path.join(HERE, "_ref")
 
 F_R, F_V, F_A, F_P, F_RHO, F_DRHO, F_RO, F_VO, F_RHOO, F_GNR, F_N = range(11)
-VEC_FIELDS = {F_R, F_V, F_A, F_RO, F_VO, F_GNR, F_N}
+# ALE volume variables
+(F_OMEGA, F_OMEGARHO, F_OMEGAV, F_OMEGA_O, F_OMEGARHO_O, F_OMEGAV_O, F_DOMEGA, F_DOMEGARHO,
+ F_DOMEGAV) = range(11, 20)
+VEC_FIELDS = {F_R, F_V, F_A, F_RO, F_VO, F_GNR, F_N, F_OMEGAV, F_OMEGAV_O, F_DOMEGAV}
+ALE_FIELDS = tuple(range(11, 20))
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -50,7 +54,7 @@ def ref_available(variant: str) -> bool:
 
 
 def ref_lib(variant: str) -> C.CDLL:
-    """variant in {"wcsph", "dbc", "bt"}"""
+    """variant in {"wcsph", "dbc", "bt", "rale"}"""
     lib = _load(os.path.join(REF_DIR, f"ref_{variant}.so"))
     _declare(lib, "ref_")
     return lib
@@ -86,7 +90,11 @@ def _declare(lib: C.CDLL, pfx: str) -> None:
     f("water_elevation_inverse", C.c_double, C.c_void_p, C.c_double, C.c_double)
     f("sum_kernel_function", C.c_double, C.c_void_p, C.c_double, C.c_double)
     f("interpolate_point", None, C.c_void_p, C.c_double, C.c_double, _dp)
+    f("rale_leapfrog_one", None, C.c_void_p, C.c_double)
+    f("rale_leapfrog_two", None, C.c_void_p, C.c_double, C.c_int)
     if pfx == "orc_":
+        f("init_ale", None, C.c_void_p)
+        f("rale_forces", None, C.c_void_p)
         f("set_threads", None, C.c_void_p, C.c_int)
         f("load", None, C.c_void_p, C.c_int, _ip, _dp, _dp, _dp, _dp, _dp, _dp)
         f("compute_forces", None, C.c_void_p, C.c_int)
@@ -100,6 +108,7 @@ def _declare(lib: C.CDLL, pfx: str) -> None:
         f("diffusion_fourtakas", C.c_double, *([C.c_double] * 11))
     else:
         f("add", None, C.c_void_p, C.c_int, *([C.c_double] * 6))
+        f("add_ale", None, C.c_void_p, C.c_int, *([C.c_double] * 6))
         f("push_gnr", None, C.c_void_p, C.c_double, C.c_double)
         f("push_n", None, C.c_void_p, C.c_double, C.c_double)
         f("compute_forces", None, C.c_void_p)

@@ -21,16 +21,24 @@
 #elif defined(REF_BT)
 #include "SPH_interactionBT.h"
 #include "SPH_BT.h"
+#elif defined(REF_RALE)
+#include "RALESPH_interaction.h"
 #else
 #include "SPH_interaction.h"
 #include "SPH_mDBC.h"
 #endif
+#if defined(REF_RALE)
+#include "RALESPH_integration.h"
+#else
 #include "SPH_integration.h"
+#endif
 #include "SPH_btdebug.h"
 #include "SPH_water_level.h"
 #include "SPH_measure_pressure.h"
 #if defined(REF_BT)
 #include "SPH_create_bt.h"
+#elif defined(REF_RALE)
+#include "ALESPH_create.h"
 #else
 #include "SPH_create.h"
 #endif
@@ -51,6 +59,9 @@ struct RefCase {
   }
 };
 
+// Field ids: 0 r, 1 v, 2 a, 3 p, 4 rho, 5 drho, 6 r_o, 7 v_o, 8 rho_o, 9 gnr,
+// 10 n, ALE: 11 omega, 12 omegarho, 13 omegav, 14 omega_o, 15 omegarho_o,
+// 16 omegav_o, 17 domega, 18 domegarho, 19 domegav.
 std::vector<realvec>* vec2_field(RefCase* c, int f) {
   switch (f) {
     case 0: return &c->ps.data.r;
@@ -60,6 +71,9 @@ std::vector<realvec>* vec2_field(RefCase* c, int f) {
     case 7: return &c->ps.data.v_o;
     case 9: return &c->ps.special.gnr;
     case 10: return &c->ps.special.n;
+    case 13: return &c->ps.special.omegav;
+    case 16: return &c->ps.special.omegav_o;
+    case 19: return &c->ps.special.domegav;
     default: return nullptr;
   }
 }
@@ -70,6 +84,12 @@ std::vector<real>* scal_field(RefCase* c, int f) {
     case 4: return &c->ps.data.rho;
     case 5: return &c->ps.data.drho;
     case 8: return &c->ps.data.rho_o;
+    case 11: return &c->ps.special.omega;
+    case 12: return &c->ps.special.omegarho;
+    case 14: return &c->ps.special.omega_o;
+    case 15: return &c->ps.special.omegarho_o;
+    case 17: return &c->ps.special.domega;
+    case 18: return &c->ps.special.domegarho;
     default: return nullptr;
   }
 }
@@ -86,6 +106,11 @@ void ref_add(void* c, int type, double p, double rho, double rx, double ry,
              double vx, double vy) {
   static_cast<RefCase*>(c)->ps.Add_particle((unsigned short)type, p, rho, {rx, ry},
                                            {vx, vy}, {0., 0.});
+}
+void ref_add_ale(void* c, int type, double p, double rho, double rx, double ry, double vx,
+                 double vy) {
+  static_cast<RefCase*>(c)->ps.ALESPH_Add_particle((unsigned short)type, p, rho, {rx, ry},
+                                                  {vx, vy}, {0., 0.});
 }
 void ref_push_gnr(void* c, double x, double y) {
   static_cast<RefCase*>(c)->ps.special.gnr.push_back({x, y});
@@ -178,8 +203,17 @@ void ref_wall_density(void* vc) {
   (void)c;
 #endif
 }
+#if defined(REF_RALE)
+void ref_rale_leapfrog_one(void* vc, double dt) {
+  Integrate_LeapFrog_partOne(static_cast<RefCase*>(vc)->ps, dt);
+}
+void ref_rale_leapfrog_two(void* vc, double dt, int step) {
+  Integrate_LeapFrog_partTwo(static_cast<RefCase*>(vc)->ps, dt, step);
+}
+#else
 void ref_predictor(void* vc, double dt) { Integrate_SymplecticPredictor(static_cast<RefCase*>(vc)->ps, dt); }
 void ref_corrector(void* vc, double dt) { Integrate_SymplecticCorrector(static_cast<RefCase*>(vc)->ps, dt); }
+#endif
 void ref_remove_out_of_domain(void* vc) {
   RefCase* c = static_cast<RefCase*>(vc);
   RemoveParticlesOutOfDomain(c->ps, c->sd);
@@ -205,7 +239,7 @@ void ref_interpolate_point(void* vc, double x, double y, double* out4) {
 }
 
 // ---- case geometry (the reference's own creation routines) ----
-#if defined(REF_BT)
+#if defined(REF_BT) || defined(REF_RALE)
 void ref_create_wall_x_bt(void* vc, double x0, double x1, double y, int o, double nx, double ny) {
   Create_Wall_x_bt(static_cast<RefCase*>(vc)->ps, x0, x1, y, o, {nx, ny});
 }
@@ -215,6 +249,11 @@ void ref_create_wall_y_bt(void* vc, double x, double y0, double y1, int o, doubl
 void ref_create_corner_bt(void* vc, int ox, int oy, double cx, double cy, double nx, double ny) {
   Create_corner_bt(static_cast<RefCase*>(vc)->ps, ox, oy, {cx, cy}, {nx, ny});
 }
+#if defined(REF_RALE)
+void ref_create_block_of_fluid(void* vc, double x0, double y0, double x1, double y1) {
+  Create_Block_of_Fluid(static_cast<RefCase*>(vc)->ps, x0, y0, x1, y1);
+}
+#endif
 #else
 void ref_create_wall_x_mdbc(void* vc, double x0, double x1, double y, int o, double yw) {
   Create_Wall_x_mdbc(static_cast<RefCase*>(vc)->ps, x0, x1, y, o, yw);

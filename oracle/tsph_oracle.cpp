@@ -52,6 +52,9 @@ struct Case {
   std::vector<int> type, index;
   std::vector<double> p, rho, drho, rho_o;
   std::vector<V2> r, v, a, r_o, v_o, gnr, n;
+  // ALE-SPH volume variables (SPH_special, src/SPH_data.h:54-67)
+  std::vector<double> omega, omegarho, omega_o, omegarho_o, domega, domegarho;
+  std::vector<V2> omegav, omegav_o, domegav;
   int ncx = 0, ncy = 0;
   std::vector<int> cstart, citems;  // CSR cell lists
   int nthreads = 1;
@@ -402,6 +405,148 @@ void interpolate_point(const Case& s, V2 ar, double out[4]) {
   out[0] = rho; out[1] = p; out[2] = vel.x; out[3] = vel.y;
 }
 
+// ---------------------------------------------------------------------------
+// Riemann-ALE-SPH, first order (the ALESPH_dambreakLong driver run.sh selects:
+// cases/ALESPH_dambreakLong/SPH_simulationINT.h)
+// ---------------------------------------------------------------------------
+
+// velRiemannLinearized, src/ALESPH_riemann_solver.h:86-96
+inline double vel_riemann(double rhoL, double rhoR, double vL, double vR, double rhoAvg, double c) {
+  return 0.5 * (vR + vL) - 0.5 * (rhoR - rhoL) * c / rhoAvg;
+}
+// densRiemannLinearizedWithLimiter, src/ALESPH_riemann_solver.h:20-34
+inline double dens_riemann_lim(double rhoL, double rhoR, double vL, double vR, double rhoAvg, double c) {
+  const double d = vL - vR;
+  const double mx = d > 0 ? d : 0;
+  const double e = 3. * mx;
+  const double beta = e < c ? e : c;
+  return 0.5 * (rhoL + rhoR) + 0.5 * beta * (vL - vR) * rhoAvg / (c * c);
+}
+
+// RALESPH Compute_Forces, src/RALESPH_interaction.h:12-287.  Quirk kept: the
+// normalisation sum gamma_sum is declared per cell and never reset between the
+// cell's particles (:48, :257, :277-280), so a fluid particle without any
+// neighbour in its support takes the "gamma == 0" branch only if no earlier
+// fluid particle of its cell had one.
+void rale_forces(Case& s) {
+  const Params& q = s.q;
+  const int nc = ncells(s);
+  const double h = q.h, m = q.m, c0 = q.cs, rho0 = q.rho0, dp = q.dp;
+  const V2 g{q.gx, q.gy};
+#pragma omp parallel for schedule(dynamic, 64) num_threads(s.nthreads)
+  for (int c = 0; c < nc; ++c) {
+    int ac[9];
+    stencil(c, s.ncy, ac);
+    double gamma_sum = 0;
+    for (int ii = s.cstart[c]; ii < s.cstart[c + 1]; ++ii) {
+      const int ai = s.citems[ii];
+      if (s.type[ai] != T_FLUID) continue;
+      const V2 ar = s.r[ai], av = s.v[ai];
+      const double arho = s.rho[ai];
+      const double ao = s.omega[ai];
+      double domega_sum = 0, domegarho_sum = 0;
+      V2 domegav_sum{0., 0.};
+      for (int k = 0; k < 9; ++k) {
+        const int cl = ac[k];
+        if (cl < 0) continue;
+        for (int jj = s.cstart[cl]; jj < s.cstart[cl + 1]; ++jj) {
+          const int ni = s.citems[jj];
+          if (ai == ni) continue;
+          const V2 nr = s.r[ni], nv = s.v[ni];
+          const double nrho = s.rho[ni];
+          const double no = s.omega[ni];
+          const V2 dr = ar - nr, dv = av - nv;
+          const double drs = std::sqrt((dr.x * dr.x) + (dr.y * dr.y));
+          const KW kw = wendland(drs, h);
+          const double W = kw.w;
+          const V2 dW = dr * kw.f;
+          const double dvdW = dv.x * dW.x + dv.y * dW.y;
+          const V2 drn = dr * (1 / drs) * (-1);
+          const double vL = av.x * drn.x + av.y * drn.y;
+          const double vR = nv.x * drn.x + nv.y * drn.y;
+          const double avgc = c0;
+          const double avgrho = 0.5 * (arho + nrho);
+          const double vss = vel_riemann(arho, nrho, vL, vR, avgrho, avgc);
+          double rhos = dens_riemann_lim(arho, nrho, vL, vR, 0.5 * (arho + nrho), avgc);
+          double ps = tait_pressure(rhos, rho0, c0);
+          const V2 vs = drn * vss + ((av + nv) * 0.5 - drn * (vL + vR) * 0.5);
+          const V2 vz = av;
+          const double vsdW = (vs.x - vz.x) * dW.x + (vs.y - vz.y) * dW.y;
+          if (s.type[ni] == T_WALL) {
+            const V2 nb = s.n[ni] * (-1);
+            // wall state from the one-sided problem, linear EOS (:224-225)
+            rhos = arho - ((0 - av.x) * nb.x + (0 - av.y) * nb.y) * arho / c0;
+            ps = c0 * c0 * (rhos - rho0) * 1;
+            domegav_sum.x -= 2 * W * dp * (ps * nb.x);
+            domegav_sum.y -= 2 * W * dp * (ps * nb.y);
+            const double dvn = dv.x * nb.x + dv.y * nb.y;
+            domega_sum -= dvn * W * dp;
+          } else {
+            domegav_sum.x -= 2 * no * ((rhos * vs.x * (vs.x - vz.x) + ps) * dW.x +
+                                      (rhos * vs.x * (vs.y - vz.y)) * dW.y);
+            domegav_sum.y -= 2 * no * ((rhos * vs.y * (vs.x - vz.x)) * dW.x +
+                                      (rhos * vs.y * (vs.y - vz.y) + ps) * dW.y);
+            domega_sum -= dvdW * no;
+            domegarho_sum -= 2 * rhos * no * vsdW;
+          }
+          gamma_sum += W * m / nrho;
+        }
+      }
+      if (gamma_sum == 0) {
+        s.domega[ai] = 0.;
+        s.domegarho[ai] = 0.;
+        s.domegav[ai] = V2{0., 0.};
+      } else {
+        s.domega[ai] = domega_sum * ao;
+        s.domegarho[ai] = domegarho_sum * ao;
+        s.domegav[ai] = domegav_sum * ao - g * ao * arho;
+      }
+    }
+  }
+}
+
+// Integrate_LeapFrog_partOne, src/RALESPH_integration.h:10-36 (fluid only)
+void rale_leapfrog_one(Case& s, double dt) {
+#pragma omp parallel for schedule(static) num_threads(s.nthreads)
+  for (int p = 0; p < s.np; ++p) {
+    if (s.type[p] != T_FLUID) continue;
+    s.omega_o[p] = s.omega[p];
+    s.omegarho_o[p] = s.omegarho[p];
+    s.omegav_o[p] = s.omegav[p];
+    s.omega[p] = s.omega[p] + s.domega[p] * dt * 0.5;
+    s.omegarho[p] = s.omegarho[p] + s.domegarho[p] * dt * 0.5;
+    s.omegav[p] = s.omegav[p] + s.domegav[p] * dt * 0.5;
+    s.rho[p] = s.omegarho[p] / (s.omega[p]);
+    s.v[p] = s.omegav[p] / (s.rho[p] * s.omega[p]);
+  }
+}
+
+// Integrate_LeapFrog_partTwo, src/RALESPH_integration.h:38-107 (fluid only)
+void rale_leapfrog_two(Case& s, double dt, int step) {
+#pragma omp parallel for schedule(static) num_threads(s.nthreads)
+  for (int p = 0; p < s.np; ++p) {
+    if (s.type[p] != T_FLUID) continue;
+    if (step == 1) {
+      s.omega_o[p] = s.omega[p];
+      s.omegarho_o[p] = s.omegarho[p];
+      s.omegav_o[p] = s.omegav[p];
+      s.omega[p] = s.omega[p] + s.domega[p] * dt * 0.5;
+      s.omegarho[p] = s.omegarho[p] + s.domegarho[p] * dt * 0.5;
+      s.omegav[p] = s.omegav[p] + s.domegav[p] * dt * 0.5;
+      s.rho[p] = s.omegarho[p] / (s.omega[p]);
+      s.v[p] = s.omegav[p] / (s.rho[p] * s.omega[p]);
+      s.r[p] = s.r[p] + s.v[p] * dt * 0.5;
+    } else {
+      s.r[p] = s.r[p] + s.v[p] * dt;
+      s.omega[p] = s.omega_o[p] + s.domega[p] * dt;
+      s.omegarho[p] = s.omegarho_o[p] + s.domegarho[p] * dt;
+      s.omegav[p] = s.omegav_o[p] + s.domegav[p] * dt;
+      s.rho[p] = s.omegarho[p] / (s.omega[p]);
+      s.v[p] = s.omegav[p] / (s.rho[p] * s.omega[p]);
+    }
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -451,13 +596,43 @@ void orc_load(void* vc, int np, const int* type, const double* p, const double* 
 
 // Field ids as the reference harness: 0 r, 1 v, 2 a, 3 p, 4 rho, 5 drho, 6 r_o,
 // 7 v_o, 8 rho_o, 9 gnr, 10 n.
+// ALE fields: 11 omega, 12 omegarho, 13 omegav, 14 omega_o, 15 omegarho_o,
+// 16 omegav_o, 17 domega, 18 domegarho, 19 domegav
 static std::vector<V2>* v2f(Case& s, int f) {
   switch (f) { case 0: return &s.r; case 1: return &s.v; case 2: return &s.a; case 6: return &s.r_o;
-    case 7: return &s.v_o; case 9: return &s.gnr; case 10: return &s.n; default: return nullptr; }
+    case 7: return &s.v_o; case 9: return &s.gnr; case 10: return &s.n; case 13: return &s.omegav;
+    case 16: return &s.omegav_o; case 19: return &s.domegav; default: return nullptr; }
 }
 static std::vector<double>* sf(Case& s, int f) {
   switch (f) { case 3: return &s.p; case 4: return &s.rho; case 5: return &s.drho; case 8: return &s.rho_o;
+    case 11: return &s.omega; case 12: return &s.omegarho; case 14: return &s.omega_o;
+    case 15: return &s.omegarho_o; case 17: return &s.domega; case 18: return &s.domegarho;
     default: return nullptr; }
+}
+
+// ALE volume variables as ALESPH_Add_particle sets them
+// (src/SPH_particle_system.h:51-108): omega = dp^2, omega*rho, omega*v
+void orc_init_ale(void* vc) {
+  Case& s = *static_cast<Case*>(vc);
+  const double w = s.q.dp * s.q.dp;
+  s.omega.assign(s.np, w);
+  s.omegarho.resize(s.np);
+  s.omegav.resize(s.np);
+  for (int i = 0; i < s.np; ++i) {
+    s.omegarho[i] = s.rho[i] * (s.q.dp * s.q.dp);
+    s.omegav[i] = s.v[i] * (s.q.dp * s.q.dp);
+  }
+  s.omega_o = s.omega;
+  s.omegarho_o = s.omegarho;
+  s.omegav_o = s.omegav;
+  s.domega.assign(s.np, 0.);
+  s.domegarho.assign(s.np, 0.);
+  s.domegav.assign(s.np, V2{0., 0.});
+}
+void orc_rale_forces(void* vc) { rale_forces(*static_cast<Case*>(vc)); }
+void orc_rale_leapfrog_one(void* vc, double dt) { rale_leapfrog_one(*static_cast<Case*>(vc), dt); }
+void orc_rale_leapfrog_two(void* vc, double dt, int step) {
+  rale_leapfrog_two(*static_cast<Case*>(vc), dt, step);
 }
 int orc_get(void* vc, int f, double* out) {
   Case& s = *static_cast<Case*>(vc);
@@ -583,6 +758,13 @@ int orc_remove_out_of_domain(void* vc) {
       auto er = [p](auto& vec) { vec.erase(vec.begin() + p); };
       er(s.type); er(s.index); er(s.p); er(s.rho); er(s.drho); er(s.r); er(s.v); er(s.a);
       er(s.r_o); er(s.v_o); er(s.rho_o); er(s.gnr); er(s.n);
+      // ALE fields erased with the particle (ALESPH_Remove_particle semantics,
+      // src/SPH_particle_system.h:148-184; the ALE driver's plain removal
+      // leaves them unerased, which misaligns them in the reference)
+      if (!s.omega.empty()) {
+        er(s.omega); er(s.omegarho); er(s.omegav); er(s.omega_o); er(s.omegarho_o);
+        er(s.omegav_o); er(s.domega); er(s.domegarho); er(s.domegav);
+      }
       s.np--; s.np_out++; removed++;
     }
   }

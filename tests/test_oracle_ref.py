@@ -128,6 +128,88 @@ def test_steps_bit_exact(builder, wall, extra, ovar):
     assert_same(R, O)
 
 
+def dam_break_rale(dp=0.02):
+    """cases/ALESPH_dambreakLong/geometry.h with the ALE creation routines
+    (src/ALESPH_create.h: ALESPH_Add_particle sets omega = dp^2, ...)"""
+    h, kap, cs = math.sqrt(2 * dp * dp), 2.1, 34.3
+    nvl = math.ceil(kap * h / dp)
+    q = [h, kap, cs, 1000., 0.02, 0.1, dp, -nvl * dp, 2. + nvl * dp, -nvl * dp, 1. + nvl * dp]
+    R = po.Case(po.ref_lib("rale"), "ref_", q)
+    R.call("create_wall_x_bt", 0. + dp, 1.61 - dp, 0., -1, 0., 1.)
+    R.call("create_wall_y_bt", 0., 0. + dp, 1., -1, 1., 0.)
+    R.call("create_wall_y_bt", 1.61, 0. + dp, 1., +1, -1., 0.)
+    R.call("create_corner_bt", -1, -1, 0., 0., math.sqrt(2), math.sqrt(2))
+    R.call("create_corner_bt", +1, -1, 1.61, 0., -math.sqrt(2), math.sqrt(2))
+    R.call("create_block_of_fluid", 0.0 + dp, 0 + dp, 0.6, 0.3)
+    return R, q, 0.00004 * dp / 0.005
+
+
+def rale_step(C, step, dt, oracle):
+    """cases/ALESPH_dambreakLong/SPH_simulationINT.h:84-132"""
+    if step > 1:
+        C.call("rale_leapfrog_one", dt)
+    C.call("clear_cells")
+    C.call("particle_to_cell")
+    C.call("density_to_pressure")
+    C.call("rale_forces" if oracle else "compute_forces")
+    C.call("rale_leapfrog_two", dt, step)
+    C.call("remove_out_of_domain")
+
+
+@pytest.mark.skipif(not po.ref_available("rale"), reason="oracle/_ref/ref_rale.so not built")
+def test_rale_steps_bit_exact():
+    R, q, dt = dam_break_rale()
+    O = oracle_from_ref(R, q, po.F_N)
+    O.call("init_ale")
+    for f in po.ALE_FIELDS:
+        assert np.array_equal(R.get(f), O.get(f)), f
+    for c in (R, O):
+        c.call("divide_to_cells")
+        c.call("particle_to_cell")
+    for step in range(1, 16):
+        rale_step(R, step, dt, False)
+        rale_step(O, step, dt, True)
+    assert_same(R, O, (po.F_R, po.F_V, po.F_P, po.F_RHO) + po.ALE_FIELDS)
+
+
+@pytest.mark.skipif(not po.ref_available("rale"), reason="oracle/_ref/ref_rale.so not built")
+def test_rale_isolated_particle_branch():
+    """The reference never resets gamma_sum between the particles of a cell
+    (src/RALESPH_interaction.h:48,257,277-280): an isolated fluid particle gets
+    -g*omega*rho if an earlier fluid particle of its cell had neighbours, and
+    zero if it is the first of its cell.  Both situations, against the reference."""
+    dp = 0.02
+    h, kap = math.sqrt(2 * dp * dp), 2.1
+    kh = kap * h
+    q = [h, kap, 34.3, 1000., 0.02, 0.1, dp, 0.0, 1.0, 0.0, 1.0]
+    of = 0.273 * h
+    # cell (5, 5): lower-left corner x0 = 5 kh - of
+    cx0, cy0 = 5 * kh - of, 5 * kh - of
+    pts = [
+        (cx0 + 0.05 * kh, cy0 + 0.05 * kh, 0.1, 0.0),    # has a neighbour (next entry)
+        (cx0 - 0.3 * h, cy0 + 0.05 * kh, 0.0, 0.2),      # neighbour in the cell to the left
+        (cx0 + 0.97 * kh, cy0 + 0.97 * kh, 0.3, -0.1),   # isolated, later in the same cell
+        (cx0 + 3.5 * kh, cy0 + 3.5 * kh, -0.2, 0.1),     # isolated, first of its cell
+    ]
+    R = po.Case(po.ref_lib("rale"), "ref_", q)
+    for x, y, vx, vy in pts:
+        R.call("add_ale", 0, 0.0, 1001.0, x, y, vx, vy)
+    O = po.oracle_case(q, dict(type=R.types(), p=R.get(po.F_P), rho=R.get(po.F_RHO),
+                               r=R.get(po.F_R), v=R.get(po.F_V)))
+    O.call("init_ale")
+    for c in (R, O):
+        c.call("divide_to_cells")
+        c.call("particle_to_cell")
+        c.call("density_to_pressure")
+    R.call("compute_forces")
+    O.call("rale_forces")
+    dov = R.get(po.F_DOMEGAV)
+    assert np.array_equal(dov, O.get(po.F_DOMEGAV))
+    w = dp * dp
+    assert dov[2][1] == -9.81 * w * 1001.0  # isolated, after a particle with neighbours
+    assert dov[3][1] == 0.0                  # isolated, first of its cell
+
+
 def test_probes_bit_exact():
     R, q, dt = dam_break_mdbc(dp=0.01)
     O = oracle_from_ref(R, q, po.F_GNR)

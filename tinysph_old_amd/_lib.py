@@ -19,7 +19,8 @@ FLUID, WALL, INLET, OUTLET = 0, 1, 6, 11
 PREC_FP64, PREC_MIXED = 0, 1
 FORCES_WCSPH, FORCES_DBC, FORCES_BT = 0, 1, 2
 WALL_DBC, WALL_MDBC, WALL_BT = 0, 1, 2
-CASE_STILL_WATER_DBC, CASE_DAM_BREAK_MDBC, CASE_DAM_BREAK_BT = 1, 2, 3
+CASE_STILL_WATER_DBC, CASE_DAM_BREAK_MDBC, CASE_DAM_BREAK_BT, CASE_ALE_DAM_BREAK = 1, 2, 3, 4
+SCHEME_SYMPLECTIC, SCHEME_RALE = 0, 1
 
 _dp = C.POINTER(C.c_double)
 _i32p = C.POINTER(C.c_int32)
@@ -44,7 +45,17 @@ class Counters(C.Structure):
 
 class CaseC(C.Structure):
     _fields_ = [("prm", Params), ("dt", C.c_double), ("wall", C.c_int), ("steps", C.c_int),
-                ("n_fluid", C.c_int), ("n_wall", C.c_int), ("particles", Particles)]
+                ("scheme", C.c_int), ("n_fluid", C.c_int), ("n_wall", C.c_int),
+                ("particles", Particles)]
+
+
+ALE_NAMES = ("omega", "omegarho", "omegav", "omega_o", "omegarho_o", "omegav_o", "domega",
+             "domegarho", "domegav")
+ALE_VEC = ("omegav", "omegav_o", "domegav")
+
+
+class AleFields(C.Structure):
+    _fields_ = [("np", C.c_int)] + [(n, _dp) for n in ALE_NAMES]
 
 
 EXPORTS = {
@@ -79,6 +90,12 @@ EXPORTS = {
     "tsph_synchronize": (C.c_int, [C.c_void_p]),
     "tsph_pair_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "tsph_pair_timing_read": (C.c_int, [C.c_void_p, _dp, C.POINTER(C.c_int)]),
+    "tsph_ale_enable": (C.c_int, [C.c_void_p]),
+    "tsph_ale_download": (C.c_int, [C.c_void_p, C.POINTER(AleFields)]),
+    "tsph_rale_forces": (C.c_int, [C.c_void_p]),
+    "tsph_rale_leapfrog_one": (C.c_int, [C.c_void_p, C.c_double]),
+    "tsph_rale_leapfrog_two": (C.c_int, [C.c_void_p, C.c_double, C.c_int]),
+    "tsph_step_rale": (C.c_int, [C.c_void_p, C.c_double, C.c_int, C.c_int]),
     "tsph_case_build": (C.c_int, [C.c_int, C.c_double, C.c_double, C.c_uint64,
                                   C.POINTER(C.POINTER(CaseC))]),
     "tsph_case_free": (C.c_int, [C.POINTER(CaseC)]),

@@ -19,7 +19,7 @@ CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libtinysph_hip.so")
 OBJDIR = os.path.join(PKG, "_build")
 
-HIP_SOURCES = ["tsph_api.hip", "tsph_cells.hip", "tsph_pairs.hip", "tsph_stages.hip"]
+HIP_SOURCES = ["tsph_api.hip", "tsph_cells.hip", "tsph_pairs.hip", "tsph_stages.hip", "tsph_ale.hip"]
 HOST_SOURCES = ["tsph_cases.cpp"]
 
 ARCH = os.environ.get("TSPH_OFFLOAD_ARCH", "gfx950")

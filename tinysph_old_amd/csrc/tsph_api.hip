@@ -50,15 +50,84 @@ static int alloc_soa(Soa &s, int cap, bool gnr, bool n) {
 
 static void free_soa(Soa &s) {
   void *ps[] = {s.r, s.v, s.a, s.r_o, s.v_o, s.gnr, s.n, s.rho, s.drho, s.p, s.rho_o,
-                s.type, s.seq, s.index};
+                s.type, s.seq, s.index, s.omega, s.omegarho, s.omega_o, s.omegarho_o,
+                s.domega, s.domegarho, s.omegav, s.omegav_o, s.domegav};
   for (void *p : ps)
     if (p) (void)hipFree(p);
   s = Soa{};
 }
 
 static void drop_graph(tsph_ctx *c) {
-  if (c->graph.exec) (void)hipGraphExecDestroy(c->graph.exec);
-  c->graph = Graph{};
+  for (Graph *g : {&c->graph[0], &c->graph[1], &c->rale_graph[0], &c->rale_graph[1]}) {
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    *g = Graph{};
+  }
+}
+
+static HostState host_state(const tsph_ctx *c) {
+  return {c->cells_valid, c->tiles_valid, c->tiles_zeroed, c->records_valid, c->marks_valid};
+}
+
+static void set_host_state(tsph_ctx *c, const HostState &h) {
+  c->cells_valid = h.cells_valid;
+  c->tiles_valid = h.tiles_valid;
+  c->tiles_zeroed = h.tiles_zeroed;
+  c->records_valid = h.records_valid;
+  c->marks_valid = h.marks_valid;
+}
+
+// Runs nsteps steps from graphs: the graph of the current buffer parity is
+// captured from `record` (one step's launches) unless it already holds a step
+// for the same tag, dt and launch shape.  Recording mutates the host state as
+// launching would (buffer swaps, flags); a replay applies the same changes.
+template <typename R>
+static int replay_step(tsph_ctx *c, Graph *slots, int tag, double dt, int nsteps, R record) {
+  for (int s = 0; s < nsteps; ++s) {
+    Graph &g = slots[c->parity];
+    if (!g.exec || g.wall != tag || g.dt != dt || g.n_ub != c->n_ub) {
+      if (g.exec) (void)hipGraphExecDestroy(g.exec);
+      g = Graph{};
+      const Soa cur0 = c->cur, alt0 = c->alt;
+      const int parity0 = c->parity;
+      const HostState h0 = host_state(c);
+      auto undo = [&] {  // nothing was launched: restore the host view
+        c->cur = cur0;
+        c->alt = alt0;
+        c->parity = parity0;
+        set_host_state(c, h0);
+      };
+      hipGraph_t graph = nullptr;
+      TSPH_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      const int e = record();
+      const hipError_t ce = hipStreamEndCapture(c->stream, &graph);
+      if (e || ce != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        undo();
+        if (e) return e;
+        TSPH_HIP(ce);
+      }
+      const hipError_t ie = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (ie != hipSuccess) {
+        g = Graph{};
+        undo();
+        TSPH_HIP(ie);
+      }
+      g.wall = tag;
+      g.dt = dt;
+      g.n_ub = c->n_ub;
+      g.flips = c->parity != parity0;
+      g.post = host_state(c);
+    } else {
+      if (g.flips) {
+        std::swap(c->cur, c->alt);
+        c->parity ^= 1;
+      }
+      set_host_state(c, g.post);
+    }
+    TSPH_HIP(hipGraphLaunch(g.exec, c->stream));
+  }
+  return 0;
 }
 
 // Allocates the wall-extra fields (gnr / n) on first use.
@@ -73,6 +142,25 @@ static int ensure_extras(tsph_ctx *c, bool gnr, bool n) {
     if (c->prm.precision == TSPH_PREC_MIXED && (e = dalloc(&c->mx.nrm, c->capacity))) return e;
     c->has_n = true;
   }
+  return 0;
+}
+
+static int alloc_ale(Soa &s, int cap) {
+  int e = 0;
+  if ((e = dalloc(&s.omega, cap)) || (e = dalloc(&s.omegarho, cap)) || (e = dalloc(&s.omega_o, cap)) ||
+      (e = dalloc(&s.omegarho_o, cap)) || (e = dalloc(&s.domega, cap)) ||
+      (e = dalloc(&s.domegarho, cap)) || (e = dalloc(&s.omegav, cap)) ||
+      (e = dalloc(&s.omegav_o, cap)) || (e = dalloc(&s.domegav, cap)))
+    return e;
+  return 0;
+}
+
+// one Riemann-ALE step, cases/ALESPH_dambreakLong/SPH_simulationINT.h:84-132
+static int record_rale_step(tsph_ctx *c, double dt, int step) {
+  int e = 0;
+  if (step > 1 && (e = launch_rale_leapfrog_one(c, dt))) return e;
+  if ((e = launch_cell_list(c)) || (e = launch_eos(c)) || (e = launch_rale_forces(c))) return e;
+  if ((e = launch_rale_leapfrog_two(c, dt, step)) || (e = launch_remove(c))) return e;
   return 0;
 }
 
@@ -415,27 +503,91 @@ int tsph_step_symplectic(tsph_ctx *c, int wall, double dt, int nsteps) {
       if (int e = record_step(c, wall, dt)) return e;
     return 0;
   }
-  Graph &g = c->graph;
-  if (!g.exec || g.wall != wall || g.dt != dt || g.n_ub != c->n_ub) {
-    drop_graph(c);
-    hipGraph_t graph = nullptr;
-    TSPH_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    const int e = record_step(c, wall, dt);
-    const hipError_t ce = hipStreamEndCapture(c->stream, &graph);
-    if (e) {
-      if (graph) (void)hipGraphDestroy(graph);
+  return replay_step(c, c->graph, wall, dt, nsteps, [&] { return record_step(c, wall, dt); });
+}
+
+int tsph_ale_enable(tsph_ctx *c) {
+  if (!c) return fail(TSPH_EINVAL, "null argument");
+  int e = 0;
+  if (!c->has_ale) {
+    if ((e = alloc_ale(c->cur, c->capacity)) || (e = alloc_ale(c->alt, c->capacity)) ||
+        (e = dalloc(&c->ale_flag, c->capacity)))
       return e;
-    }
-    TSPH_HIP(ce);
-    const hipError_t ie = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(graph);
-    TSPH_HIP(ie);
-    g.wall = wall;
-    g.dt = dt;
-    g.n_ub = c->n_ub;
+    c->has_ale = true;
   }
-  for (int s = 0; s < nsteps; ++s) TSPH_HIP(hipGraphLaunch(g.exec, c->stream));
+  drop_graph(c);
+  return launch_ale_init(c);
+}
+
+int tsph_ale_download(tsph_ctx *c, tsph_ale_fields *h) {
+  if (!c || !h) return fail(TSPH_EINVAL, "null argument");
+  if (!c->has_ale) return fail(TSPH_ESTATE, "ALE variables not enabled (tsph_ale_enable)");
+  int n = 0, e = 0;
+  if ((e = tsph_get_np(c, &n))) return e;
+  std::vector<int> order;
+  std::vector<uint32_t> seq;
+  std::vector<uint8_t> type;
+  if ((e = live_order(c, n, order, seq, type))) return e;
+  const int m = (int)order.size();
+  std::vector<double> b(2 * (size_t)n + 2);
+  auto get1 = [&](double *dst, const double *src) -> int {
+    if (!dst) return 0;
+    if (n) TSPH_HIP(hipMemcpy(b.data(), src, sizeof(double) * n, hipMemcpyDeviceToHost));
+    for (int k = 0; k < m; ++k) dst[k] = b[order[k]];
+    return 0;
+  };
+  auto get2 = [&](double *dst, const double2 *src) -> int {
+    if (!dst) return 0;
+    if (n) TSPH_HIP(hipMemcpy(b.data(), src, 2 * sizeof(double) * n, hipMemcpyDeviceToHost));
+    for (int k = 0; k < m; ++k) {
+      dst[2 * k] = b[2 * order[k]];
+      dst[2 * k + 1] = b[2 * order[k] + 1];
+    }
+    return 0;
+  };
+  const Soa &d = c->cur;
+  if ((e = get1(h->omega, d.omega)) || (e = get1(h->omegarho, d.omegarho)) ||
+      (e = get2(h->omegav, d.omegav)) || (e = get1(h->omega_o, d.omega_o)) ||
+      (e = get1(h->omegarho_o, d.omegarho_o)) || (e = get2(h->omegav_o, d.omegav_o)) ||
+      (e = get1(h->domega, d.domega)) || (e = get1(h->domegarho, d.domegarho)) ||
+      (e = get2(h->domegav, d.domegav)))
+    return e;
+  h->np = m;
   return 0;
+}
+
+static int rale_ready(tsph_ctx *c) {
+  if (!c) return fail(TSPH_EINVAL, "null argument");
+  if (!c->has_ale) return fail(TSPH_ESTATE, "ALE variables not enabled (tsph_ale_enable)");
+  return 0;
+}
+
+int tsph_rale_forces(tsph_ctx *c) {
+  if (int e = rale_ready(c)) return e;
+  if (!c->cells_valid) return fail(TSPH_ESTATE, "no cell list: call tsph_particle_to_cell first");
+  return launch_rale_forces(c);
+}
+
+int tsph_rale_leapfrog_one(tsph_ctx *c, double dt) {
+  if (int e = rale_ready(c)) return e;
+  return launch_rale_leapfrog_one(c, dt);
+}
+
+int tsph_rale_leapfrog_two(tsph_ctx *c, double dt, int step) {
+  if (int e = rale_ready(c)) return e;
+  return launch_rale_leapfrog_two(c, dt, step);
+}
+
+int tsph_step_rale(tsph_ctx *c, double dt, int step, int nsteps) {
+  if (int e = rale_ready(c)) return e;
+  if (step < 1 || nsteps < 0) return fail(TSPH_EINVAL, "steps are numbered from 1");
+  int s = step;
+  const int last = step + nsteps;
+  // the first step differs (src/RALESPH_integration.h:42-70): launched eagerly
+  for (; s < last && (s == 1 || c->time_pairs); ++s)
+    if (int e = record_rale_step(c, dt, s)) return e;
+  if (s >= last) return 0;
+  return replay_step(c, c->rale_graph, 100, dt, last - s, [&] { return record_rale_step(c, dt, 2); });
 }
 
 int tsph_water_elevation(tsph_ctx *c, double x, double y0, int inverse, double *level) {

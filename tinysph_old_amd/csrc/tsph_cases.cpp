@@ -27,6 +27,7 @@ struct Spec {
   double dp, hh, kap, cs, visco, delta, dt;
   double x_0, x_1, y_0, y_1;  // unpadded domain (constants.h)
   int steps, wall;
+  int scheme = TSPH_SCHEME_SYMPLECTIC;
 };
 
 // PREP_SIMULATION_DATA: pad the domain by nvl*dp on every side
@@ -70,6 +71,7 @@ int export_case(const Spec &s, Particle_system &ps, tsph_case **out) {
   c->dt = s.dt;
   c->wall = s.wall;
   c->steps = s.steps;
+  c->scheme = s.scheme;
   const int n = (int)ps.np;
   std::vector<int32_t> type(n), index(n);
   for (int i = 0; i < n; ++i) {
@@ -173,6 +175,27 @@ extern "C" int tsph_case_build(int preset, double dx, double jitter, uint64_t se
       Create_corner_bt(ps, -1, -1, {0., 0.}, {std::sqrt(2), std::sqrt(2)});
       Create_corner_bt(ps, +1, -1, {1.61, 0.}, {-std::sqrt(2), std::sqrt(2)});
       Create_Block_of_FluidWP(ps, 0.0 + dp, 0 + dp, 0.6, 0.3);
+      jitter_fluid(ps, (jitter > 0 ? jitter : 0.0) * s.dp, seed);
+      return export_case(s, ps, out);
+    }
+    case TSPH_CASE_ALE_DAM_BREAK: {
+      // cases/ALESPH_dambreakLong/geometry.h (BT walls, fluid block at rho0; the
+      // ALE variables follow from ALESPH_Add_particle, see tsph_ale_enable) in
+      // the 3.22 x 1.8 m tank with the 1.2 x 0.6 m column; c0 = 10 sqrt(g 0.6)
+      const double L = 3.22, Ht = 1.8, CW = 1.2, CH = 0.6;
+      s.dp = dx > 0 ? dx : 0.002;
+      s.hh = 1.0 * std::sqrt(2 * s.dp * s.dp); s.kap = 2.1; s.cs = 10. * std::sqrt(g * CH);
+      s.visco = 0.02; s.delta = 0.1; s.dt = 0.2 * s.hh / s.cs;
+      s.x_0 = 0.; s.x_1 = L + 0.1; s.y_0 = 0.; s.y_1 = Ht;
+      s.steps = 1000; s.wall = TSPH_WALL_BT; s.scheme = TSPH_SCHEME_RALE;
+      Particle_system ps(s.hh, s.kap, s.cs, 1000., s.visco, s.delta, s.dp);
+      const double dp = s.dp;
+      Create_Wall_x_bt(ps, 0. + dp, L - dp, 0., -1, {0., 1.});
+      Create_Wall_y_bt(ps, 0., 0. + dp, Ht, -1, {1., 0.});
+      Create_Wall_y_bt(ps, L, 0. + dp, Ht, +1, {-1., 0.});
+      Create_corner_bt(ps, -1, -1, {0., 0.}, {std::sqrt(2), std::sqrt(2)});
+      Create_corner_bt(ps, +1, -1, {L, 0.}, {-std::sqrt(2), std::sqrt(2)});
+      Create_Block_of_Fluid(ps, 0.0 + dp, 0 + dp, CW, CH);
       jitter_fluid(ps, (jitter > 0 ? jitter : 0.0) * s.dp, seed);
       return export_case(s, ps, out);
     }

@@ -263,6 +263,17 @@ __global__ void k_reorder(const int *__restrict__ cell_start, int ncells, const 
     dst.p[k] = src.p[j];
     if (src.gnr) dst.gnr[k] = src.gnr[j];
     if (src.n) dst.n[k] = src.n[j];
+    if (src.omega) {
+      dst.omega[k] = src.omega[j];
+      dst.omegarho[k] = src.omegarho[j];
+      dst.omegav[k] = src.omegav[j];
+      dst.omega_o[k] = src.omega_o[j];
+      dst.omegarho_o[k] = src.omegarho_o[j];
+      dst.omegav_o[k] = src.omegav_o[j];
+      dst.domega[k] = src.domega[j];
+      dst.domegarho[k] = src.domegarho[j];
+      dst.domegav[k] = src.domegav[j];
+    }
   }
   const unsigned long long m = __ballot(wall);
   if (m) {
@@ -302,6 +313,7 @@ int launch_cell_list(tsph_ctx *c) {
                      c->alt, c->scal, c->wall_list);
   TSPH_HIP(hipGetLastError());
   std::swap(c->cur, c->alt);
+  c->parity ^= 1;
   c->cells_valid = true;
   c->tiles_valid = false;
   c->tiles_zeroed = true;

@@ -38,6 +38,10 @@ struct Soa {
   uint8_t *type = nullptr;
   uint32_t *seq = nullptr;
   int32_t *index = nullptr;
+  // ALE-SPH volume variables (allocated for ALE runs only)
+  double *omega = nullptr, *omegarho = nullptr, *omega_o = nullptr, *omegarho_o = nullptr;
+  double *domega = nullptr, *domegarho = nullptr;
+  double2 *omegav = nullptr, *omegav_o = nullptr, *domegav = nullptr;
 };
 
 // FP32 pair records of the mixed path, rebuilt after every reorder.
@@ -56,12 +60,22 @@ struct Consts {
   int ncx, ncy, ncells;
 };
 
+// host-side flags that describe what the device buffers hold
+struct HostState {
+  bool cells_valid, tiles_valid, tiles_zeroed, records_valid, marks_valid;
+};
+
+// one captured step.  A step may rebuild the cell list an odd number of times
+// (the Riemann-ALE step does it once), which swaps the cur/alt particle buffers:
+// graphs are therefore kept per buffer parity, and a replay applies the swap
+// and the host flags that the captured step left behind.
 struct Graph {
   hipGraphExec_t exec = nullptr;
   int wall = -1;
   double dt = 0;
   int n_ub = -1;
-  int precision = -1;
+  bool flips = false;  // the step swaps cur and alt an odd number of times
+  HostState post{};    // host flags after the step
 };
 
 }  // namespace tsph
@@ -83,6 +97,8 @@ struct tsph_ctx {
   int n_wall = 0;            // wall particles (walls are never created or removed)
   bool records_valid = false;  // mixed-path FP32 records match the FP64 state
   bool marks_valid = false;    // removal marks match the current positions
+  bool has_ale = false;        // ALE volume variables allocated
+  uint8_t *ale_flag = nullptr; // per particle: its own ALE gamma sum is non-zero
   hipStream_t stream = nullptr;
   tsph::Soa cur, alt;
   tsph::Mixed mx;
@@ -102,7 +118,9 @@ struct tsph_ctx {
   std::vector<hipEvent_t> tev;         // 2 per recorded launch
   int tev_used = 0;
   float last_pair_ms = 0.f;
-  tsph::Graph graph;
+  int parity = 0;             // flips whenever cur and alt are swapped
+  tsph::Graph graph[2];       // symplectic step, by parity at the step's start
+  tsph::Graph rale_graph[2];  // Riemann-ALE step, by parity at the step's start
 };
 
 namespace tsph {
@@ -121,6 +139,7 @@ inline int grid_for(int n, int block) { return n <= 0 ? 1 : (n + block - 1) / bl
 // tsph_cells.hip
 int launch_cell_list(tsph_ctx *c);  // hash, count, scan, scatter, order, reorder
 // tsph_pairs.hip
+int ensure_tile_list(tsph_ctx *c);  // list of non-empty pair-loop tiles
 int launch_forces(tsph_ctx *c, int variant);
 int launch_mixed_records(tsph_ctx *c);
 // tsph_stages.hip
@@ -131,6 +150,11 @@ int launch_mdbc(tsph_ctx *c);
 int launch_bt_density(tsph_ctx *c);
 int launch_remove(tsph_ctx *c);
 int run_water_elevation(tsph_ctx *c, double x, double y0, int inverse, double *out);
+// tsph_ale.hip
+int launch_ale_init(tsph_ctx *c);
+int launch_rale_forces(tsph_ctx *c);
+int launch_rale_leapfrog_one(tsph_ctx *c, double dt);
+int launch_rale_leapfrog_two(tsph_ctx *c, double dt, int step);
 int run_interpolate_point(tsph_ctx *c, double x, double y, double *out4);
 
 }  // namespace tsph

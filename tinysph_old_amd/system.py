@@ -26,6 +26,7 @@ class CaseData:
     n_fluid: int
     n_wall: int
     arrays: dict = field(default_factory=dict)
+    scheme: int = 0
 
     @property
     def np(self) -> int:
@@ -65,7 +66,7 @@ def build_case(preset: int, dx: float = 0.0, jitter: float = -1.0, seed: int = 1
             arrays["gnr"] = arr(h.gnr, 2 * n, np.float64).reshape(n, 2)
         if h.n:
             arrays["n"] = arr(h.n, 2 * n, np.float64).reshape(n, 2)
-        return CaseData(prm, c.dt, c.wall, c.steps, c.n_fluid, c.n_wall, arrays)
+        return CaseData(prm, c.dt, c.wall, c.steps, c.n_fluid, c.n_wall, arrays, c.scheme)
     finally:
         lib.tsph_case_free(out)
 
@@ -235,6 +236,34 @@ class ParticleSystem:
 
     def set_tile_rows(self, rows: int):
         L.check(self.lib.tsph_set_tile_rows(self.ctx, rows), "tsph_set_tile_rows")
+
+    # ---- Riemann-ALE-SPH (src/RALESPH_interaction.h, src/RALESPH_integration.h) ----
+    def ale_enable(self):
+        L.check(self.lib.tsph_ale_enable(self.ctx), "tsph_ale_enable")
+
+    def ale_download(self) -> dict:
+        n = self.np
+        h = L.AleFields()
+        out = {}
+        for k in L.ALE_NAMES:
+            a = np.zeros(max(2 * n if k in L.ALE_VEC else n, 1))
+            setattr(h, k, L.dptr(a))
+            out[k] = a
+        L.check(self.lib.tsph_ale_download(self.ctx, C.byref(h)), "tsph_ale_download")
+        m = h.np
+        return {k: (v[: 2 * m].reshape(m, 2) if k in L.ALE_VEC else v[:m]) for k, v in out.items()}
+
+    def rale_forces(self):
+        L.check(self.lib.tsph_rale_forces(self.ctx), "Compute_Forces (RALE)")
+
+    def rale_leapfrog_one(self, dt: float):
+        L.check(self.lib.tsph_rale_leapfrog_one(self.ctx, dt), "Integrate_LeapFrog_partOne")
+
+    def rale_leapfrog_two(self, dt: float, step: int):
+        L.check(self.lib.tsph_rale_leapfrog_two(self.ctx, dt, step), "Integrate_LeapFrog_partTwo")
+
+    def step_rale(self, dt: float, step: int, nsteps: int = 1):
+        L.check(self.lib.tsph_step_rale(self.ctx, dt, step, nsteps), "tsph_step_rale")
 
     def set_waves_per_cu(self, waves: int):
         L.check(self.lib.tsph_set_waves_per_cu(self.ctx, waves), "tsph_set_waves_per_cu")
