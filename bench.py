@@ -37,6 +37,41 @@ HBM_PEAK_GBS = 8000.0
 B_PAIR = {T.PREC_FP64: 50, T.PREC_MIXED: 26}
 B_PART = {T.PREC_FP64: 74, T.PREC_MIXED: 50}  # mixed keeps FP64 outputs (a 16 B + dRho 8 B)
 SEED = 20241202
+KERNEL = "k_tile32<0>"  # the mixed WCSPH pair kernel (mDBC case: fluid-only interactions)
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_dam_break_mdbc_740k.md")
+
+
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (lines `traffic_bytes_per_launch <kernel> <bytes>`), or None."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for ln in f:
+                t = ln.split()
+                if len(t) == 3 and t[0] == "traffic_bytes_per_launch" and t[1] == kernel:
+                    return float(t[2])
+    except OSError:
+        pass
+    return None
+
+
+def copy_probe(mib: int = 512, reps: int = 3):
+    """A plain device copy of known size (k_copy_probe in libtsph_copy_probe.so),
+    run so that rocprofv3's FETCH_SIZE and WRITE_SIZE per dispatch can be scaled
+    against it: each dispatch reads and writes exactly `mib` MiB."""
+    import ctypes as C
+
+    lib = C.CDLL(os.path.join(ROOT, "tinysph_old_amd", "libtsph_copy_probe.so"))
+    lib.tsph_copy_probe.restype = C.c_int
+    lib.tsph_copy_probe.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_double)]
+    ms = C.c_double()
+    rc = lib.tsph_copy_probe(mib, reps, C.byref(ms))
+    if rc != 0:
+        raise RuntimeError(f"tsph_copy_probe failed ({rc})")
+    gbs = 2 * mib * 2**20 / (ms.value * 1e-3) / 1e9
+    print(f"[bench] copy probe: {reps} x ({mib} MiB read + {mib} MiB written), "
+          f"{ms.value:.4f} ms per copy = {gbs:.0f} GB/s", file=sys.stderr, flush=True)
+    return ms.value
 
 
 def run_path(case, precision: int, steps: int, warmup: int):
@@ -104,6 +139,8 @@ def main() -> int:
     ap.add_argument("--dx", type=float, default=0.001)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run a plain 512 MiB device copy (calibration for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every GPU box of this build has one MI355X, and the "
@@ -137,6 +174,7 @@ def main() -> int:
         return avg_ms, achieved, bytes_per_launch, pairs_per_launch
 
     m_avg, m_ach, m_bytes, m_ppl = roof(mixed, T.PREC_MIXED)
+    traffic = measured_traffic(KERNEL)
     d_avg, d_ach, d_bytes, d_ppl = roof(fp64, T.PREC_FP64)
 
     line = {
@@ -164,8 +202,9 @@ def main() -> int:
         "error_vs_fp64": {"rho_rel_l2": rho_err, "r_rel_l2": r_err,
                           "after_steps": args.warmup + 2 * k},
         "roofline": {"bound": "hbm", "achieved": m_ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": m_ach / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_forces32<0>", "avg_launch_ms": m_avg,
+                     "frac": m_ach / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_gbs": traffic / (m_avg * 1e-3) / 1e9 if traffic else None,
+                     "kernel": KERNEL, "avg_launch_ms": m_avg,
                      "algorithmic_bytes_per_launch": m_bytes, "pairs_per_launch": m_ppl},
     }
     if not args.no_cpu:
@@ -177,6 +216,8 @@ def main() -> int:
                                           f"dam break (oracle, OpenMP, {threads} threads), "
                                           f"{t:.1f} s",
                                 "steps_per_s": n / t}
+    if args.copy_probe:
+        copy_probe()
     print(json.dumps(line), flush=True)
     return 0
 

@@ -184,6 +184,9 @@ typedef struct tsph_ale_fields {
 /* Allocates the ALE variables and sets them as ALESPH_Add_particle does
  * (src/SPH_particle_system.h:51-108): omega = dp^2, omega*rho, omega*v. */
 int tsph_ale_enable(tsph_ctx *ctx);
+/* tsph_ale_enable, then overwrites the fields given (non-NULL) with host values
+ * in reference order; host->np must equal the live particle count. */
+int tsph_ale_upload(tsph_ctx *ctx, const tsph_ale_fields *host);
 int tsph_ale_download(tsph_ctx *ctx, tsph_ale_fields *host);
 /* Compute_Forces of src/RALESPH_interaction.h:12-287 */
 int tsph_rale_forces(tsph_ctx *ctx);

@@ -14,7 +14,11 @@
 // The pair-loop variant is chosen at compile time like the reference chooses it
 // by #include: define TSPH_DROPIN_FORCES to TSPH_FORCES_WCSPH (default,
 // src/SPH_interaction.h), TSPH_FORCES_DBC (src/SPH_interactionDBC.h) or
-// TSPH_FORCES_BT (src/SPH_interactionBT.h) before including this header.
+// TSPH_FORCES_BT (src/SPH_interactionBT.h) before including this header.  Define
+// TSPH_DROPIN_RALE instead for the Riemann-ALE drivers
+// (cases/ALESPH_dambreakLong/SPH_simulationINT.h): Compute_Forces is then
+// src/RALESPH_interaction.h's, Integrate_LeapFrog_partOne/partTwo are provided
+// and the geometry routines create ALE particles (src/ALESPH_create.h).
 //
 // Data stays on the device between stages.  The host vectors are refreshed by
 // tsph_sync_host(particles); host edits after the first stage call must be
@@ -83,6 +87,13 @@ struct SPH_constants {
 struct SPH_special {
   std::vector<realvec> gnr;  // mDBC ghost nodes (walls only, in creation order)
   std::vector<realvec> n;    // BT normals (walls only, in creation order)
+  // ALE-SPH volume variables (src/SPH_data.h:53-66), one entry per particle
+  std::vector<real> omega, omegarho;
+  std::vector<realvec> omegav;
+  std::vector<real> omega_o, omegarho_o;
+  std::vector<realvec> omegav_o;
+  std::vector<real> domega, domegarho;
+  std::vector<realvec> domegav;
 };
 
 struct Interaction_data {
@@ -129,6 +140,23 @@ struct Particle_system {
     data.v_o.push_back(_v);
     data.v_oo.push_back(_v);
     host_dirty = true;
+  }
+
+  // src/SPH_particle_system.h:51-108 (the volume variables the Riemann-ALE
+  // path reads; the MUSCL gradients and *_oo copies are not used by it)
+  void ALESPH_Add_particle(short unsigned int _type, real _p, real _rho, realvec _r, realvec _v,
+                           realvec _a) {
+    Add_particle(_type, _p, _rho, _r, _v, _a);
+    const real w = data_const.dp * data_const.dp;
+    special.omega.push_back(w);
+    special.omegav.push_back(_v * w);
+    special.omegarho.push_back(_rho * w);
+    special.omega_o.push_back(w);
+    special.omegav_o.push_back(_v * w);
+    special.omegarho_o.push_back(_rho * w);
+    special.domega.push_back(0.);
+    special.domegarho.push_back(0.);
+    special.domegav.push_back(TDouble2(0., 0.));
   }
 
   Particle_system(double _h, double _kap, double _cs, double _rho0, double _visco, double _delta,
@@ -198,6 +226,20 @@ inline void tsph_push(Particle_system &ps) {
   if (!ps.special.gnr.empty()) { gnr = tsph_expand_extra(ps, ps.special.gnr); h.gnr = gnr.data(); }
   if (!ps.special.n.empty()) { nn = tsph_expand_extra(ps, ps.special.n); h.n = nn.data(); }
   tsph_check(tsph_upload(ps.dev, &h), "tsph_upload");
+  if (!ps.special.omega.empty()) {
+    const SPH_special &sp = ps.special;
+    std::vector<double> ov = flat(sp.omegav), ovo = flat(sp.omegav_o), dov = flat(sp.domegav);
+    tsph_ale_fields f{};
+    f.np = n;
+    f.omega = const_cast<double *>(sp.omega.data());
+    f.omegarho = const_cast<double *>(sp.omegarho.data());
+    f.omega_o = const_cast<double *>(sp.omega_o.data());
+    f.omegarho_o = const_cast<double *>(sp.omegarho_o.data());
+    f.domega = const_cast<double *>(sp.domega.data());
+    f.domegarho = const_cast<double *>(sp.domegarho.data());
+    f.omegav = ov.data(); f.omegav_o = ovo.data(); f.domegav = dov.data();
+    tsph_check(tsph_ale_upload(ps.dev, &f), "tsph_ale_upload");
+  }
   ps.host_dirty = false;
 }
 
@@ -230,6 +272,23 @@ inline void tsph_sync_host(Particle_system &ps) {
   ps.data.r = vec(r); ps.data.v = vec(v); ps.data.a = vec(a); ps.data.r_o = vec(ro); ps.data.v_o = vec(vo);
   ps.data.v_oo.resize(m);
   ps.data.rho_oo.resize(m);
+  if (!ps.special.omega.empty()) {
+    SPH_special &sp = ps.special;
+    std::vector<double> om(n), orho(n), omo(n), orhoo(n), dom(n), dorho(n), ov(2 * (size_t)n),
+        ovo(2 * (size_t)n), dov(2 * (size_t)n);
+    tsph_ale_fields f{};
+    f.omega = om.data(); f.omegarho = orho.data(); f.omega_o = omo.data();
+    f.omegarho_o = orhoo.data(); f.domega = dom.data(); f.domegarho = dorho.data();
+    f.omegav = ov.data(); f.omegav_o = ovo.data(); f.domegav = dov.data();
+    tsph_check(tsph_ale_download(ps.dev, &f), "tsph_ale_download");
+    sp.omega.assign(om.begin(), om.begin() + m);
+    sp.omegarho.assign(orho.begin(), orho.begin() + m);
+    sp.omega_o.assign(omo.begin(), omo.begin() + m);
+    sp.omegarho_o.assign(orhoo.begin(), orhoo.begin() + m);
+    sp.domega.assign(dom.begin(), dom.begin() + m);
+    sp.domegarho.assign(dorho.begin(), dorho.begin() + m);
+    sp.omegav = vec(ov); sp.omegav_o = vec(ovo); sp.domegav = vec(dov);
+  }
   ps.dev_newer = false;
 }
 
@@ -277,12 +336,33 @@ inline void Density_to_pressure(Particle_system &ps) {
   tsph_ran(ps);
 }
 
+#ifndef TSPH_DROPIN_RALE
 // src/SPH_interaction.h:8 / src/SPH_interactionDBC.h:8 / src/SPH_interactionBT.h:8
 inline void Compute_Forces(Particle_system &ps) {
   tsph_push(ps);
   tsph_check(tsph_compute_forces(ps.dev, TSPH_DROPIN_FORCES), "Compute_Forces");
   tsph_ran(ps);
 }
+#else
+// src/RALESPH_interaction.h:12-287 (first-order Riemann-ALE fluxes)
+inline void Compute_Forces(Particle_system &ps) {
+  tsph_push(ps);
+  tsph_check(tsph_rale_forces(ps.dev), "Compute_Forces");
+  tsph_ran(ps);
+}
+// src/RALESPH_integration.h:10-36
+inline void Integrate_LeapFrog_partOne(Particle_system &ps, double dt) {
+  tsph_push(ps);
+  tsph_check(tsph_rale_leapfrog_one(ps.dev, dt), "Integrate_LeapFrog_partOne");
+  tsph_ran(ps);
+}
+// src/RALESPH_integration.h:38-107
+inline void Integrate_LeapFrog_partTwo(Particle_system &ps, double dt, int step) {
+  tsph_push(ps);
+  tsph_check(tsph_rale_leapfrog_two(ps.dev, dt, step), "Integrate_LeapFrog_partTwo");
+  tsph_ran(ps);
+}
+#endif
 
 // src/SPH_mDBC.h:9-50
 inline void mDBC_compute_density_mdbcGeo(Particle_system &ps, Simulation_data) {
@@ -340,7 +420,17 @@ inline real Pressure_in_point(Particle_system &ps, Simulation_data, realvec ar) 
 
 // ---------------------------------------------------------------------------
 // geometry, src/SPH_create.h and src/SPH_create_bt.h (host side, same arithmetic)
+// With TSPH_DROPIN_RALE the routines add ALE particles, as their namesakes in
+// src/ALESPH_create.h do (same positions, ALESPH_Add_particle).
 // ---------------------------------------------------------------------------
+inline void tsph_add_particle(Particle_system &ps, short unsigned int type, real p, real rho,
+                              realvec r, realvec v, realvec a) {
+#ifdef TSPH_DROPIN_RALE
+  ps.ALESPH_Add_particle(type, p, rho, r, v, a);
+#else
+  ps.Add_particle(type, p, rho, r, v, a);
+#endif
+}
 
 // src/SPH_pressure.h:49-63
 inline real Pressure_to_density(real p, real rho0, real c0) {
@@ -360,7 +450,7 @@ inline void Create_Wall_x_mdbc(Particle_system &ps, double x_0, double x_1, doub
     r.y = y + dp * l * orientation;
     for (int i = 0; i < npart + 1; i++) {
       r.x = x_0 + i * dp;
-      ps.Add_particle(wall, 0, ps.data_const.rho0, r, v, a);
+      tsph_add_particle(ps, wall, 0, ps.data_const.rho0, r, v, a);
       gn.x = r.x;
       gn.y = y_w + fabs(r.y - y_w);
       ps.special.gnr.push_back(gn);
@@ -380,7 +470,7 @@ inline void Create_Wall_y_mdbc(Particle_system &ps, double x, double y_0, double
     r.x = x + dp * l * orientation;
     for (int i = 0; i < npart + 1; i++) {
       r.y = y_0 + i * dp;
-      ps.Add_particle(wall, 0, ps.data_const.rho0, r, v, a);
+      tsph_add_particle(ps, wall, 0, ps.data_const.rho0, r, v, a);
       gn.x = x_w + fabs(r.x - x_w) * (-1) * (orientation);
       gn.y = r.y;
       ps.special.gnr.push_back(gn);
@@ -397,7 +487,7 @@ inline void Create_corner_mdbc(Particle_system &ps, int orientationX, int orient
     r.y = xyv.y + dp * l * orientationY + dp / 2 * orientationY;
     for (int i = 0; i < nvl; i++) {
       r.x = xyv.x + dp * i * orientationX + dp / 2 * orientationX;
-      ps.Add_particle(wall, 0, ps.data_const.rho0, r, v, a);
+      tsph_add_particle(ps, wall, 0, ps.data_const.rho0, r, v, a);
       gn.x = xyv.x + fabs(r.x - xyv.x) * (-1) * orientationX;
       gn.y = xyv.y + fabs(r.y - xyv.y) * (-1) * orientationY;
       ps.special.gnr.push_back(gn);
@@ -415,7 +505,7 @@ inline void Create_Block_of_Fluid(Particle_system &ps, double x_0, double y_0, d
     r.x = x_0 + i * dp;
     for (int j = 0; j < npart_y + 1; j++) {
       r.y = y_0 + j * dp;
-      ps.Add_particle(fluid, 0, ps.data_const.rho0, r, v, a);
+      tsph_add_particle(ps, fluid, 0, ps.data_const.rho0, r, v, a);
     }
   }
 }
@@ -431,7 +521,7 @@ inline void Create_Block_of_FluidWP(Particle_system &ps, double x_0, double y_0,
     for (int j = 0; j < npart_y + 1; j++) {
       r.y = y_0 + j * dp;
       const real rhoin = Pressure_to_density(9.81 * (y_1 - r.y) * 1000, 1000., ps.data_const.cs);
-      ps.Add_particle(fluid, 9.81 * (y_1 - r.y) * 1000, rhoin, r, v, a);
+      tsph_add_particle(ps, fluid, 9.81 * (y_1 - r.y) * 1000, rhoin, r, v, a);
     }
   }
 }
@@ -446,7 +536,7 @@ inline void Create_Wall_x_bt(Particle_system &ps, double x_0, double x_1, double
     r.y = y + dp * l * orientation;
     for (int i = 0; i < npart + 1; i++) {
       r.x = x_0 + i * dp;
-      ps.Add_particle(wall, 0, ps.data_const.rho0, r, v, a);
+      tsph_add_particle(ps, wall, 0, ps.data_const.rho0, r, v, a);
       ps.special.n.push_back(n);
     }
   }
@@ -462,7 +552,7 @@ inline void Create_Wall_y_bt(Particle_system &ps, double x, double y_0, double y
     r.x = x + dp * l * orientation;
     for (int i = 0; i < npart + 1; i++) {
       r.y = y_0 + i * dp;
-      ps.Add_particle(wall, 0, ps.data_const.rho0, r, v, a);
+      tsph_add_particle(ps, wall, 0, ps.data_const.rho0, r, v, a);
       ps.special.n.push_back(n);
     }
   }
@@ -476,7 +566,7 @@ inline void Create_corner_bt(Particle_system &ps, int orientationX, int orientat
     r.y = xyv.y + dp * l * orientationY;
     for (int i = 0; i < 1; i++) {
       r.x = xyv.x + dp * i * orientationX;
-      ps.Add_particle(wall, 0, ps.data_const.rho0, r, v, a);
+      tsph_add_particle(ps, wall, 0, ps.data_const.rho0, r, v, a);
       ps.special.n.push_back(n);
     }
   }

@@ -166,3 +166,40 @@ int main() { SPH_simulation s; Create_Block_of_Fluid(s.particles, 0.02, 0.02, 0.
     res = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", f"-I{ROOT}/include", str(src)],
                          capture_output=True, text=True)
     assert res.returncode == 0, res.stderr
+
+
+def test_dropin_header_compiles_rale(tmp_path):
+    """A Riemann-ALE driver in the shape of cases/ALESPH_dambreakLong/SPH_simulationINT.h
+    compiles against the drop-in header."""
+    src = tmp_path / "driver_rale.cpp"
+    src.write_text('''
+#define TSPH_DROPIN_RALE
+#include "tinysph_dropin.hpp"
+struct SPH_simulation {
+  const double dp = 0.02, hh = 1.0 * sqrt(2 * dp * dp), kap = 2.1, dt = 1.6e-4;
+  Particle_system particles{hh, kap, 34.3, 1000., 0.02, 0.1, dp};
+  Simulation_data simulation_data{-0.1, 1.7, -0.1, 1.1};
+  void RUN(int steps) {
+    Divide_To_Cells(particles, simulation_data);
+    for (int step = 1; step < steps; step++) {
+      if (step > 1) Integrate_LeapFrog_partOne(particles, dt);
+      Clear_cells(particles);
+      Particle_To_Cell(particles, simulation_data);
+      Density_to_pressure(particles);
+      Compute_Forces(particles);
+      Integrate_LeapFrog_partTwo(particles, dt, step);
+      RemoveParticlesOutOfDomain(particles, simulation_data);
+    }
+    tsph_sync_host(particles);
+  }
+};
+int main() {
+  SPH_simulation s;
+  Create_Wall_x_bt(s.particles, 0. + 0.02, 1.61 - 0.02, 0., -1, {0., 1.});
+  Create_Block_of_Fluid(s.particles, 0.02, 0.02, 0.6, 0.3);
+  return s.particles.special.omega.size() == s.particles.np ? 0 : 1;
+}
+''')
+    res = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", f"-I{ROOT}/include", str(src)],
+                         capture_output=True, text=True)
+    assert res.returncode == 0, res.stderr

@@ -1,0 +1,153 @@
+"""The example drivers written against the reference's stage API
+(include/tinysph_dropin.hpp, tinysph_old_amd/csrc/dropin_driver.cpp): their
+particle set-up equals the reference geometry, and on the GPU their whole run
+equals the oracle driven through the same stage sequence."""
+import math
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from tests.helpers import oracle_init, oracle_step, rel_err
+
+PKG = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "tinysph_old_amd")
+DRIVERS = {"mdbc": os.path.join(PKG, "tsph_dropin_mdbc"), "rale": os.path.join(PKG, "tsph_dropin_rale")}
+DP = 0.02
+
+
+def load(path, ale):
+    raw = open(path, "rb").read()
+    np_, ne = np.frombuffer(raw[:8], dtype=np.int32)
+    off = 8
+    out = {}
+
+    def take(name, count, dtype, shape=None):
+        nonlocal off
+        size = np.dtype(dtype).itemsize * count
+        a = np.frombuffer(raw[off:off + size], dtype=dtype).copy()
+        off += size
+        out[name] = a.reshape(shape) if shape else a
+
+    take("type", np_, np.int32)
+    take("r", 2 * np_, np.float64, (np_, 2))
+    take("v", 2 * np_, np.float64, (np_, 2))
+    take("rho", np_, np.float64)
+    take("p", np_, np.float64)
+    take("extra", 2 * ne, np.float64, (ne, 2))
+    if ale:
+        take("omega", np_, np.float64)
+        take("omegarho", np_, np.float64)
+        take("omegav", 2 * np_, np.float64, (np_, 2))
+    assert off == len(raw)
+    return out
+
+
+def params(dp=DP):
+    h = math.sqrt(2 * dp * dp)
+    nvl = math.ceil(2.1 * h / dp)
+    return [h, 2.1, 34.3, 1000., 0.02, 0.1, dp, -nvl * dp, 2. + nvl * dp, -nvl * dp, 1. + nvl * dp]
+
+
+def run_driver(kind, steps, tmp_path, expect_ok=True):
+    exe = DRIVERS[kind]
+    if not os.path.exists(exe):
+        pytest.fail(f"{exe} is missing: run `python -m tinysph_old_amd.build`")
+    init, final = tmp_path / f"{kind}_init.bin", tmp_path / f"{kind}_final.bin"
+    print(f"[driver] {kind} dp={DP} steps={steps}", flush=True)
+    res = subprocess.run([exe, str(DP), str(steps), str(init), str(final)], capture_output=True,
+                         text=True, timeout=120)
+    if expect_ok:
+        assert res.returncode == 0, res.stderr
+    return res, init, final
+
+
+def with_extra(d, kind):
+    """Per-particle gnr / n from the wall-only list (walls come first)."""
+    arrays = dict(type=d["type"], p=d["p"], rho=d["rho"], r=d["r"], v=d["v"])
+    ext = np.zeros_like(d["r"])
+    ext[: len(d["extra"])] = d["extra"]
+    arrays["gnr" if kind == "mdbc" else "n"] = ext
+    return arrays
+
+
+@pytest.mark.parametrize("kind", ["mdbc", "rale"])
+@pytest.mark.skipif(not (po.ref_available("wcsph") and po.ref_available("bt")),
+                    reason="oracle/_ref not built")
+def test_driver_geometry_matches_reference(kind, tmp_path):
+    """The set-up the driver dumps before INIT equals the reference's geometry
+    routines on the same arguments (cases/damBreakLong_mDBC/geometry.h,
+    cases/ALESPH_dambreakLong/geometry.h).  Without a GPU the driver then stops
+    at Divide_To_Cells with an error."""
+    res, init, _ = run_driver(kind, 1, tmp_path, expect_ok=False)
+    d = load(init, kind == "rale")
+    dp, q = DP, params()
+    if kind == "mdbc":
+        R = po.Case(po.ref_lib("wcsph"), "ref_", q)
+        R.call("create_wall_x_mdbc", 0. + dp, 1.61, 0., -1, dp / 2)
+        R.call("create_wall_y_mdbc", 0., 0 + dp, 1., -1, dp / 2)
+        R.call("create_wall_y_mdbc", 1.61, 0 + dp, 1., +1, 1.61 - dp / 2)
+        R.call("create_corner_mdbc", -1, -1, 0. + dp / 2, 0. + dp / 2)
+        R.call("create_corner_mdbc", +1, -1, 1.61 - dp / 2, 0. + dp / 2)
+        R.call("create_block_of_fluid", 0 + dp, 0 + dp, 0.6, 0.3)
+        extra = R.get(po.F_GNR)
+    else:
+        R = po.Case(po.ref_lib("rale"), "ref_", q)
+        R.call("create_wall_x_bt", 0. + dp, 1.61 - dp, 0., -1, 0., 1.)
+        R.call("create_wall_y_bt", 0., 0. + dp, 1., -1, 1., 0.)
+        R.call("create_wall_y_bt", 1.61, 0. + dp, 1., +1, -1., 0.)
+        R.call("create_corner_bt", -1, -1, 0., 0., math.sqrt(2), math.sqrt(2))
+        R.call("create_corner_bt", +1, -1, 1.61, 0., -math.sqrt(2), math.sqrt(2))
+        R.call("create_block_of_fluid", 0.0 + dp, 0 + dp, 0.6, 0.3)
+        extra = R.get(po.F_N)
+        assert np.array_equal(d["omega"], R.get(po.F_OMEGA))
+        assert np.array_equal(d["omegarho"], R.get(po.F_OMEGARHO))
+        assert np.array_equal(d["omegav"], R.get(po.F_OMEGAV))
+    assert np.array_equal(d["type"], R.types())
+    for k, f in (("r", po.F_R), ("v", po.F_V), ("rho", po.F_RHO), ("p", po.F_P)):
+        assert np.array_equal(d[k], R.get(f)), k
+    nw = len(d["extra"])
+    assert np.array_equal(d["extra"], extra[:nw])
+
+
+@pytest.mark.gpu
+def test_driver_mdbc_matches_oracle(tmp_path):
+    """20 steps of the unchanged reference driver sequence on the GPU (FP64)
+    against the oracle on the dumped initial state."""
+    _, init, final = run_driver("mdbc", 20, tmp_path)
+    d0, d1 = load(init, False), load(final, False)
+    O = po.oracle_case(params(), with_extra(d0, "mdbc"))
+    oracle_init(O)
+    dt = 0.2 * math.sqrt(2 * DP * DP) / 34.3
+    for _ in range(20):
+        oracle_step(O, 1, dt)
+    assert np.array_equal(d1["type"], O.types())
+    assert rel_err(d1["r"], O.get(po.F_R)) <= 1e-9
+    assert rel_err(d1["v"], O.get(po.F_V)) <= 1e-9
+    assert rel_err(d1["rho"], O.get(po.F_RHO)) <= 1e-9
+    assert rel_err(d1["p"], O.get(po.F_P)) <= 1e-6
+
+
+@pytest.mark.gpu
+def test_driver_rale_matches_oracle(tmp_path):
+    _, init, final = run_driver("rale", 20, tmp_path)
+    d0, d1 = load(init, True), load(final, True)
+    O = po.oracle_case(params(), with_extra(d0, "rale"))
+    O.call("init_ale")
+    oracle_init(O)
+    dt = 0.2 * math.sqrt(2 * DP * DP) / 34.3
+    for step in range(1, 21):
+        if step > 1:
+            O.call("rale_leapfrog_one", dt)
+        O.call("clear_cells")
+        assert O.call("particle_to_cell") == 0
+        O.call("density_to_pressure")
+        O.call("rale_forces")
+        O.call("rale_leapfrog_two", dt, step)
+        O.call("remove_out_of_domain")
+    assert np.array_equal(d1["type"], O.types())
+    for k, f in (("r", po.F_R), ("v", po.F_V), ("rho", po.F_RHO), ("omega", po.F_OMEGA),
+                 ("omegarho", po.F_OMEGARHO), ("omegav", po.F_OMEGAV)):
+        assert rel_err(d1[k], O.get(f)) <= 1e-9, k
+    assert rel_err(d1["p"], O.get(po.F_P)) <= 1e-6

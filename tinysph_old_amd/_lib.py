@@ -91,6 +91,7 @@ EXPORTS = {
     "tsph_pair_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "tsph_pair_timing_read": (C.c_int, [C.c_void_p, _dp, C.POINTER(C.c_int)]),
     "tsph_ale_enable": (C.c_int, [C.c_void_p]),
+    "tsph_ale_upload": (C.c_int, [C.c_void_p, C.POINTER(AleFields)]),
     "tsph_ale_download": (C.c_int, [C.c_void_p, C.POINTER(AleFields)]),
     "tsph_rale_forces": (C.c_int, [C.c_void_p]),
     "tsph_rale_leapfrog_one": (C.c_int, [C.c_void_p, C.c_double]),

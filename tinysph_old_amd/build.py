@@ -17,6 +17,9 @@ import sys
 PKG = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libtinysph_hip.so")
+# HBM-counter calibration kernel (a plain copy of known size), its own library
+PROBE_LIB = os.path.join(PKG, "libtsph_copy_probe.so")
+PROBE_SOURCE = "tsph_copy_probe.hip"
 OBJDIR = os.path.join(PKG, "_build")
 
 HIP_SOURCES = ["tsph_api.hip", "tsph_cells.hip", "tsph_pairs.hip", "tsph_stages.hip", "tsph_ale.hip"]
@@ -51,19 +54,45 @@ def _compile(src: str) -> str:
     return obj
 
 
-def build(verbose: bool = False) -> str:
-    os.makedirs(OBJDIR, exist_ok=True)
-    srcs = HIP_SOURCES + HOST_SOURCES
-    with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
-        objs = list(ex.map(_compile, srcs))
-    tmp = LIB + ".tmp"
+# example drivers written against include/tinysph_dropin.hpp (reference stage API)
+DRIVER_SOURCE = "dropin_driver.cpp"
+DRIVERS = {
+    os.path.join(PKG, "tsph_dropin_mdbc"): [],
+    os.path.join(PKG, "tsph_dropin_rale"): ["-DTSPH_DROPIN_RALE"],
+}
+
+
+def _driver(exe: str, defs: list) -> None:
+    inc = os.path.join(os.path.dirname(PKG), "include")
+    tmp = exe + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", *defs, f"-I{inc}", os.path.join(CSRC, DRIVER_SOURCE),
+           "-o", tmp, f"-L{PKG}", "-ltinysph_hip", f"-Wl,-rpath,$ORIGIN"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"driver build failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
+    os.replace(tmp, exe)
+
+
+def _link(out: str, objs: list) -> None:
+    tmp = out + ".tmp"
     cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
-    os.replace(tmp, LIB)
+    os.replace(tmp, out)
+
+
+def build(verbose: bool = False) -> str:
+    os.makedirs(OBJDIR, exist_ok=True)
+    srcs = HIP_SOURCES + HOST_SOURCES + [PROBE_SOURCE]
+    with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
+        objs = list(ex.map(_compile, srcs))
+    _link(LIB, objs[:-1])
+    _link(PROBE_LIB, objs[-1:])
+    for exe, defs in DRIVERS.items():
+        _driver(exe, defs)
     if verbose:
-        print("built", LIB)
+        print("built", LIB, PROBE_LIB, *DRIVERS)
     return LIB
 
 

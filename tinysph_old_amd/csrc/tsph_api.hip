@@ -519,6 +519,45 @@ int tsph_ale_enable(tsph_ctx *c) {
   return launch_ale_init(c);
 }
 
+int tsph_ale_upload(tsph_ctx *c, const tsph_ale_fields *h) {
+  if (!c || !h) return fail(TSPH_EINVAL, "null argument");
+  int e = 0, n = 0;
+  if ((e = tsph_ale_enable(c)) || (e = tsph_get_np(c, &n))) return e;
+  std::vector<int> order;
+  std::vector<uint32_t> seq;
+  std::vector<uint8_t> type;
+  if ((e = live_order(c, n, order, seq, type))) return e;
+  const int m = (int)order.size();
+  if (h->np != m) return fail(TSPH_EINVAL, "ALE field count differs from the particle count");
+  std::vector<double> b(2 * (size_t)n + 2, 0.0);
+  // host entry k (reference order) goes to the device slot of the k-th live particle
+  auto put1 = [&](double *dst, const double *src) -> int {
+    if (!src) return 0;
+    if (n) TSPH_HIP(hipMemcpy(b.data(), dst, sizeof(double) * n, hipMemcpyDeviceToHost));
+    for (int k = 0; k < m; ++k) b[order[k]] = src[k];
+    if (n) TSPH_HIP(hipMemcpy(dst, b.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    return 0;
+  };
+  auto put2 = [&](double2 *dst, const double *src) -> int {
+    if (!src) return 0;
+    if (n) TSPH_HIP(hipMemcpy(b.data(), dst, 2 * sizeof(double) * n, hipMemcpyDeviceToHost));
+    for (int k = 0; k < m; ++k) {
+      b[2 * order[k]] = src[2 * k];
+      b[2 * order[k] + 1] = src[2 * k + 1];
+    }
+    if (n) TSPH_HIP(hipMemcpy(dst, b.data(), 2 * sizeof(double) * n, hipMemcpyHostToDevice));
+    return 0;
+  };
+  const Soa &d = c->cur;
+  if ((e = put1(d.omega, h->omega)) || (e = put1(d.omegarho, h->omegarho)) ||
+      (e = put2(d.omegav, h->omegav)) || (e = put1(d.omega_o, h->omega_o)) ||
+      (e = put1(d.omegarho_o, h->omegarho_o)) || (e = put2(d.omegav_o, h->omegav_o)) ||
+      (e = put1(d.domega, h->domega)) || (e = put1(d.domegarho, h->domegarho)) ||
+      (e = put2(d.domegav, h->domegav)))
+    return e;
+  return 0;
+}
+
 int tsph_ale_download(tsph_ctx *c, tsph_ale_fields *h) {
   if (!c || !h) return fail(TSPH_EINVAL, "null argument");
   if (!c->has_ale) return fail(TSPH_ESTATE, "ALE variables not enabled (tsph_ale_enable)");

@@ -241,6 +241,21 @@ class ParticleSystem:
     def ale_enable(self):
         L.check(self.lib.tsph_ale_enable(self.ctx), "tsph_ale_enable")
 
+    def ale_upload(self, fields: dict):
+        """Overwrite ALE variables (reference order; missing keys keep the
+        ALESPH_Add_particle values)."""
+        h = L.AleFields()
+        h.np = self.np
+        keep = []
+        for k in L.ALE_NAMES:
+            if fields.get(k) is None:
+                continue
+            a = np.ascontiguousarray(fields[k], dtype=np.float64).reshape(-1)
+            keep.append(a)
+            setattr(h, k, L.dptr(a))
+            h.np = len(a) // 2 if k in L.ALE_VEC else len(a)
+        L.check(self.lib.tsph_ale_upload(self.ctx, C.byref(h)), "tsph_ale_upload")
+
     def ale_download(self) -> dict:
         n = self.np
         h = L.AleFields()
