@@ -179,6 +179,7 @@ typedef struct tsph_ale_fields {
   double *omega, *omegarho, *omegav;
   double *omega_o, *omegarho_o, *omegav_o;
   double *domega, *domegarho, *domegav;
+  double *gradrho, *gradvx, *gradvy;  /* MUSCL gradients (src/SPH_data.h:68-70) */
 } tsph_ale_fields;
 
 /* Allocates the ALE variables and sets them as ALESPH_Add_particle does
@@ -195,6 +196,27 @@ int tsph_rale_leapfrog_one(tsph_ctx *ctx, double dt);
 int tsph_rale_leapfrog_two(tsph_ctx *ctx, double dt, int step);
 /* nsteps whole steps numbered from `step` (the driver's step counter, from 1) */
 int tsph_step_rale(tsph_ctx *ctx, double dt, int step, int nsteps);
+
+/* ---- ALE-SPH with MUSCL reconstruction, the pipeline of the ALE dam-break
+ * driver cases/ALESPH_dambreakLong/SPH_simulation.h:78-125 (shares the ALE
+ * variables of tsph_ale_enable) ---- */
+#define TSPH_MUSCL_NONE 0   /* first order: the gradients stay zero, as the driver runs */
+#define TSPH_MUSCL_LINEAR 1 /* Kernel_gradient_approx after Compute_Density, weight 1 */
+#define TSPH_MUSCL_MINMOD 2 /* as LINEAR, each reconstruction weighted by the minmod
+                               limiter of src/ALESPH_riemann_solver.h:180-191 */
+/* Compute_Density, src/ALESPH_density_and_omega_bt.h:9-281 (limited: minmod weights) */
+int tsph_ale_compute_density(tsph_ctx *ctx, int limited);
+/* Kernel_gradient_approx, src/ALESPH_kernel_gradient.h:10-190 */
+int tsph_ale_kernel_gradient(tsph_ctx *ctx);
+/* Integrate_density_compute_pressure, src/ALESPH_integration.h:86-135 */
+int tsph_ale_density_pressure(tsph_ctx *ctx, double dt);
+/* Compute_Acceleration_BT, src/ALESPH_acceleration_bt.h:12-295 */
+int tsph_ale_compute_acceleration(tsph_ctx *ctx, int limited);
+/* Integrate_LeapFrog_partOne_ALESPH / partTwo_ALESPH, src/ALESPH_integration.h:9-83 */
+int tsph_ale_leapfrog_one(tsph_ctx *ctx, double dt);
+int tsph_ale_leapfrog_two(tsph_ctx *ctx, double dt, int step);
+/* nsteps whole ALE-SPH steps numbered from `step`; muscl = TSPH_MUSCL_* */
+int tsph_step_ale(tsph_ctx *ctx, double dt, int step, int nsteps, int muscl);
 
 /* ---- case setup (host only; no GPU needed) ----
  * Presets built with the reference's geometry routines (src/SPH_create.h,

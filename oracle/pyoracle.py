@@ -21,7 +21,10 @@ F_R, F_V, F_A, F_P, F_RHO, F_DRHO, F_RO, F_VO, F_RHOO, F_GNR, F_N = range(11)
 # ALE volume variables
 (F_OMEGA, F_OMEGARHO, F_OMEGAV, F_OMEGA_O, F_OMEGARHO_O, F_OMEGAV_O, F_DOMEGA, F_DOMEGARHO,
  F_DOMEGAV) = range(11, 20)
-VEC_FIELDS = {F_R, F_V, F_A, F_RO, F_VO, F_GNR, F_N, F_OMEGAV, F_OMEGAV_O, F_DOMEGAV}
+F_GRADRHO, F_GRADVX, F_GRADVY = 20, 21, 22
+GRAD_FIELDS = (F_GRADRHO, F_GRADVX, F_GRADVY)
+VEC_FIELDS = {F_R, F_V, F_A, F_RO, F_VO, F_GNR, F_N, F_OMEGAV, F_OMEGAV_O, F_DOMEGAV, F_GRADRHO,
+              F_GRADVX, F_GRADVY}
 ALE_FIELDS = tuple(range(11, 20))
 
 _dp = C.POINTER(C.c_double)
@@ -92,7 +95,18 @@ def _declare(lib: C.CDLL, pfx: str) -> None:
     f("interpolate_point", None, C.c_void_p, C.c_double, C.c_double, _dp)
     f("rale_leapfrog_one", None, C.c_void_p, C.c_double)
     f("rale_leapfrog_two", None, C.c_void_p, C.c_double, C.c_int)
+    # ALE-SPH with MUSCL reconstruction
+    f("ale_compute_density", None, C.c_void_p)
+    f("ale_compute_acceleration", None, C.c_void_p)
+    f("ale_kernel_gradient", None, C.c_void_p)
+    f("ale_density_pressure", None, C.c_void_p, C.c_double)
+    f("ale_leapfrog_one", None, C.c_void_p, C.c_double)
+    f("ale_leapfrog_two", None, C.c_void_p, C.c_double, C.c_int)
     if pfx == "orc_":
+        f("ale_density_pressure", C.c_int, C.c_void_p, C.c_double)
+        f("ale_compute_density_limited", None, C.c_void_p)
+        f("ale_compute_acceleration_limited", None, C.c_void_p)
+        f("minmod_phi", C.c_double, *([C.c_double] * 6))
         f("init_ale", None, C.c_void_p)
         f("rale_forces", None, C.c_void_p)
         f("set_threads", None, C.c_void_p, C.c_int)

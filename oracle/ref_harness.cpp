@@ -29,6 +29,10 @@
 #endif
 #if defined(REF_RALE)
 #include "RALESPH_integration.h"
+#include "ALESPH_density_and_omega_bt.h"
+#include "ALESPH_acceleration_bt.h"
+#include "ALESPH_kernel_gradient.h"
+#include "ALESPH_integration.h"
 #else
 #include "SPH_integration.h"
 #endif
@@ -61,7 +65,8 @@ struct RefCase {
 
 // Field ids: 0 r, 1 v, 2 a, 3 p, 4 rho, 5 drho, 6 r_o, 7 v_o, 8 rho_o, 9 gnr,
 // 10 n, ALE: 11 omega, 12 omegarho, 13 omegav, 14 omega_o, 15 omegarho_o,
-// 16 omegav_o, 17 domega, 18 domegarho, 19 domegav.
+// 16 omegav_o, 17 domega, 18 domegarho, 19 domegav, MUSCL: 20 gradrho,
+// 21 gradvx, 22 gradvy.
 std::vector<realvec>* vec2_field(RefCase* c, int f) {
   switch (f) {
     case 0: return &c->ps.data.r;
@@ -74,6 +79,9 @@ std::vector<realvec>* vec2_field(RefCase* c, int f) {
     case 13: return &c->ps.special.omegav;
     case 16: return &c->ps.special.omegav_o;
     case 19: return &c->ps.special.domegav;
+    case 20: return &c->ps.special.gradrho;
+    case 21: return &c->ps.special.gradvx;
+    case 22: return &c->ps.special.gradvy;
     default: return nullptr;
   }
 }
@@ -209,6 +217,21 @@ void ref_rale_leapfrog_one(void* vc, double dt) {
 }
 void ref_rale_leapfrog_two(void* vc, double dt, int step) {
   Integrate_LeapFrog_partTwo(static_cast<RefCase*>(vc)->ps, dt, step);
+}
+// ALE-SPH with MUSCL reconstruction (cases/ALESPH_dambreakLong/SPH_simulation.h:78-125)
+void ref_ale_compute_density(void* vc) { Compute_Density(static_cast<RefCase*>(vc)->ps); }
+void ref_ale_compute_acceleration(void* vc) {
+  Compute_Acceleration_BT(static_cast<RefCase*>(vc)->ps);
+}
+void ref_ale_kernel_gradient(void* vc) { Kernel_gradient_approx(static_cast<RefCase*>(vc)->ps); }
+void ref_ale_density_pressure(void* vc, double dt) {
+  Integrate_density_compute_pressure(static_cast<RefCase*>(vc)->ps, dt);
+}
+void ref_ale_leapfrog_one(void* vc, double dt) {
+  Integrate_LeapFrog_partOne_ALESPH(static_cast<RefCase*>(vc)->ps, dt);
+}
+void ref_ale_leapfrog_two(void* vc, double dt, int step) {
+  Integrate_LeapFrog_partTwo_ALESPH(static_cast<RefCase*>(vc)->ps, dt, step);
 }
 #else
 void ref_predictor(void* vc, double dt) { Integrate_SymplecticPredictor(static_cast<RefCase*>(vc)->ps, dt); }

@@ -37,7 +37,7 @@ inline V2 operator*(V2 a, double b) { return {a.x * b, a.y * b}; }
 inline V2 operator/(V2 a, double b) { return {a.x / b, a.y / b}; }
 
 // Particle type ids, src/SPH_defs.h:44-65
-enum { T_FLUID = 0, T_WALL = 1, T_INLET = 6, T_OUTLET = 11 };
+enum { T_FLUID = 0, T_WALL = 1, T_INLET = 6, T_OUTLET = 11, T_OUTLETF = 17 };
 const double EPS = 0.01;  // src/SPH_defs.h:69
 
 struct Params {  // SPH_constants + Simulation_data, src/SPH_data.h:30-46,111-125
@@ -55,6 +55,8 @@ struct Case {
   // ALE-SPH volume variables (SPH_special, src/SPH_data.h:54-67)
   std::vector<double> omega, omegarho, omega_o, omegarho_o, domega, domegarho;
   std::vector<V2> omegav, omegav_o, domegav;
+  // ALE-SPH MUSCL gradients (src/SPH_data.h:68-70)
+  std::vector<V2> gradrho, gradvx, gradvy;
   int ncx = 0, ncy = 0;
   std::vector<int> cstart, citems;  // CSR cell lists
   int nthreads = 1;
@@ -547,6 +549,301 @@ void rale_leapfrog_two(Case& s, double dt, int step) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// ALE-SPH with MUSCL reconstruction, the pipeline of
+// cases/ALESPH_dambreakLong/SPH_simulation.h:78-125:
+//   Integrate_LeapFrog_partOne_ALESPH  src/ALESPH_integration.h:9-31
+//   Compute_Density                    src/ALESPH_density_and_omega_bt.h:9-281
+//   Kernel_gradient_approx             src/ALESPH_kernel_gradient.h:10-190
+//   Integrate_density_compute_pressure src/ALESPH_integration.h:86-135
+//   Compute_Acceleration_BT            src/ALESPH_acceleration_bt.h:12-295
+//   Integrate_LeapFrog_partTwo_ALESPH  src/ALESPH_integration.h:33-83
+// ---------------------------------------------------------------------------
+
+// MUSCLleft / MUSCLright with weight alpha, src/ALESPH_riemann_solver.h:195-235
+inline double muscl_left(double A, V2 g, V2 dr, double alpha) {
+  return A + alpha * 0.5 * (g.x * dr.x + g.y * dr.y);
+}
+inline double muscl_right(double A, V2 g, V2 dr, double alpha) {
+  return A - alpha * 0.5 * (g.x * dr.x + g.y * dr.y);
+}
+// minmod_r, src/ALESPH_riemann_solver.h:180-191, with r_CD the vector from
+// the reconstructing particle C to the other particle D.  The reference never
+// calls it; for equal values (aD == aC) the ratio is undefined there and the
+// restatement takes phi = 0 (no reconstruction).
+inline double minmod_phi(double aC, double aD, V2 gC, V2 rCD) {
+  if (aD - aC == 0) return 0.;
+  const double r = 2 * (gC.x * rCD.x + gC.y * rCD.y) / (aD - aC) - 1.;
+  const double mn = 1 < r ? 1 : r;  // MIN(1, r)
+  return 0 > mn ? 0 : mn;            // MAX(0, MIN(1, r))
+}
+// cs, src/ALESPH_riemann_solver.h:148-159 (gamma = 2)
+inline double ale_cs(double rho, double rho0, double c0) {
+  const double gamma = 2.;
+  return c0 * std::pow((rho / rho0), 0.5 * (gamma - 1));
+}
+
+// The reconstructed left/right states of one pair.  limited = 0 follows the
+// reference (alpha = 1); limited = 1 weights each reconstruction by the
+// minmod limiter of its own variable.
+struct MusclLR { double rhoL, rhoR; V2 vL, vR; };
+inline MusclLR muscl_states(const Case& s, int ai, int ni, V2 dr, int limited) {
+  const V2 ga = s.gradrho[ai], gn = s.gradrho[ni];
+  const V2 gax = s.gradvx[ai], gnx = s.gradvx[ni], gay = s.gradvy[ai], gny = s.gradvy[ni];
+  const double arho = s.rho[ai], nrho = s.rho[ni];
+  const V2 av = s.v[ai], nv = s.v[ni];
+  const V2 rij = V2{0., 0.} - dr;  // r_j - r_i
+  double a1 = 1, a2 = 1, a3 = 1, a4 = 1, a5 = 1, a6 = 1;
+  if (limited) {
+    a1 = minmod_phi(arho, nrho, ga, rij);
+    a2 = minmod_phi(nrho, arho, gn, dr);
+    a3 = minmod_phi(av.x, nv.x, gax, rij);
+    a4 = minmod_phi(av.y, nv.y, gay, rij);
+    a5 = minmod_phi(nv.x, av.x, gnx, dr);
+    a6 = minmod_phi(nv.y, av.y, gny, dr);
+  }
+  MusclLR o;
+  o.rhoL = muscl_left(arho, ga, dr, a1);
+  o.rhoR = muscl_right(nrho, gn, dr, a2);
+  o.vL.x = muscl_left(av.x, gax, dr, a3);
+  o.vL.y = muscl_left(av.y, gay, dr, a4);
+  o.vR.x = muscl_right(nv.x, gnx, dr, a5);
+  o.vR.y = muscl_right(nv.y, gny, dr, a6);
+  return o;
+}
+
+// Compute_Density, src/ALESPH_density_and_omega_bt.h:9-281.  Active: every
+// particle except outletf / outlet / inlet / wall (:58-61).  Quirk kept: the
+// fluid-neighbour volume flux reads `dvdW`, which the reference declares but
+// never assigns (:135, :237); its own build (g++ -O3, src/Makefile flags)
+// evaluates that term as zero, and so does this restatement -- only wall
+// neighbours change omega (:219-232).
+void ale_compute_density(Case& s, int limited) {
+  const Params& q = s.q;
+  const int nc = ncells(s);
+  const double h = q.h, m = q.m, c0 = q.cs, dp = q.dp;
+#pragma omp parallel for schedule(dynamic, 64) num_threads(s.nthreads)
+  for (int c = 0; c < nc; ++c) {
+    int ac[9];
+    stencil(c, s.ncy, ac);
+    for (int ii = s.cstart[c]; ii < s.cstart[c + 1]; ++ii) {
+      const int ai = s.citems[ii];
+      const int at = s.type[ai];
+      if (at == T_OUTLETF || at == T_OUTLET || at == T_INLET || at == T_WALL) continue;
+      const V2 ar = s.r[ai], av = s.v[ai];
+      const double aomega = s.omega[ai];
+      double domega_temp = 0, domegarho_temp = 0, gamma = 0;
+      for (int k = 0; k < 9; ++k) {
+        const int cl = ac[k];
+        if (cl < 0) continue;
+        for (int jj = s.cstart[cl]; jj < s.cstart[cl + 1]; ++jj) {
+          const int ni = s.citems[jj];
+          if (ai == ni) continue;
+          const V2 nr = s.r[ni], nv = s.v[ni];
+          const double nrho = s.rho[ni], nomega = s.omega[ni];
+          const V2 dr = ar - nr, dv = av - nv;
+          const double drs = std::sqrt((dr.x * dr.x) + (dr.y * dr.y));
+          const V2 drn = dr * (1 / drs) * (-1);
+          const KW kw = wendland(drs, h);
+          const double W = kw.w;
+          const V2 dW = dr * kw.f;
+          const MusclLR lr = muscl_states(s, ai, ni, dr, limited);
+          const double vl = lr.vL.x * drn.x + lr.vL.y * drn.y;
+          const double vr = lr.vR.x * drn.x + lr.vR.y * drn.y;
+          const double avgc = c0;
+          const double rhos = dens_riemann_lim(lr.rhoL, lr.rhoR, vl, vr, 0.5 * (lr.rhoL + lr.rhoR), avgc);
+          const double vss = vel_riemann(lr.rhoL, lr.rhoR, vl, vr, 0.5 * (lr.rhoL + lr.rhoR), avgc);
+          const V2 vs = drn * vss + ((av + nv) * 0.5 - drn * (vl + vr) * 0.5);
+          const V2 vz = av;
+          const double vsdW = (vs.x - vz.x) * dW.x + (vs.y - vz.y) * dW.y;
+          if (s.type[ni] == T_WALL) {
+            const V2 nb = s.n[ni] * (-1);
+            const double dvn = dv.x * nb.x + dv.y * nb.y;
+            domega_temp -= dvn * W * dp;
+          } else {
+            domega_temp -= 0. * nomega;  // the unassigned dvdW, as the reference build evaluates it
+            domegarho_temp -= 2 * rhos * nomega * vsdW;
+          }
+          gamma += W * m / nrho;
+        }
+      }
+      if (gamma == 0) {
+        s.domega[ai] = 0.;
+        s.domegarho[ai] = 0.;
+      } else {
+        s.domega[ai] = domega_temp * aomega;
+        s.domegarho[ai] = domegarho_temp * aomega;
+      }
+    }
+  }
+}
+
+// Kernel_gradient_approx, src/ALESPH_kernel_gradient.h:10-190 (fluid only)
+void ale_kernel_gradient(Case& s) {
+  const Params& q = s.q;
+  const int nc = ncells(s);
+  const double h = q.h, m = q.m, dp = q.dp;
+#pragma omp parallel for schedule(dynamic, 64) num_threads(s.nthreads)
+  for (int c = 0; c < nc; ++c) {
+    int ac[9];
+    stencil(c, s.ncy, ac);
+    for (int ii = s.cstart[c]; ii < s.cstart[c + 1]; ++ii) {
+      const int ai = s.citems[ii];
+      if (s.type[ai] != T_FLUID) continue;
+      const V2 ar = s.r[ai], av = s.v[ai];
+      const double arho = s.rho[ai];
+      V2 grt{0., 0.}, gxt{0., 0.}, gyt{0., 0.};
+      double gamma = 0;
+      for (int k = 0; k < 9; ++k) {
+        const int cl = ac[k];
+        if (cl < 0) continue;
+        for (int jj = s.cstart[cl]; jj < s.cstart[cl + 1]; ++jj) {
+          const int ni = s.citems[jj];
+          if (ai == ni) continue;
+          const V2 nr = s.r[ni], nv = s.v[ni];
+          const double nrho = s.rho[ni], nomega = s.omega[ni];
+          const V2 dr = ar - nr;
+          const double drs = std::sqrt((dr.x * dr.x) + (dr.y * dr.y));
+          const KW kw = wendland(drs, h);
+          const double W = kw.w;
+          const V2 dW = dr * kw.f;
+          if (s.type[ni] == T_WALL) {
+            const V2 nb = s.n[ni] * (-1);
+            grt.x += (nrho - arho) * nb.x * W * dp;
+            grt.y += (nrho - arho) * nb.y * W * dp;
+            gxt.x += (nv.x - av.x) * nb.x * W * dp;
+            gxt.y += (nv.x - av.x) * nb.y * W * dp;
+            gyt.x += (nv.y - av.y) * nb.x * W * dp;
+            gyt.y += (nv.y - av.y) * nb.y * W * dp;
+          } else {
+            grt.x += (nrho - arho) * nomega * dW.x;
+            grt.y += (nrho - arho) * nomega * dW.y;
+            gxt.x += (nv.x - av.x) * nomega * dW.x;
+            gxt.y += (nv.x - av.x) * nomega * dW.y;
+            gyt.x += (nv.y - av.y) * nomega * dW.x;
+            gyt.y += (nv.y - av.y) * nomega * dW.y;
+          }
+          gamma += W * m / nrho;
+        }
+      }
+      if (gamma == 0) {
+        s.gradrho[ai] = s.gradvx[ai] = s.gradvy[ai] = V2{0., 0.};
+      } else {
+        s.gradrho[ai] = grt * 2.;
+        s.gradvx[ai] = gxt * 2.;
+        s.gradvy[ai] = gyt * 2.;
+      }
+    }
+  }
+}
+
+// Integrate_density_compute_pressure, src/ALESPH_integration.h:86-135 (fluid
+// only; linear EOS Compute_Pressure2, src/SPH_pressure.h:24-34, clamped at 0).
+// Returns the count of densities outside [0, 1500] (the reference prints them).
+int ale_density_pressure(Case& s, double dt) {
+  int bad = 0;
+  for (int p = 0; p < s.np; ++p) {
+    if (s.type[p] != T_FLUID) continue;  // :95-101
+    s.omega[p] = s.omega[p] + s.domega[p] * dt;
+    s.omegarho[p] = s.omegarho[p] + s.domegarho[p] * dt;
+    s.rho[p] = s.omegarho[p] / (s.omega[p]);
+    if (s.rho[p] > 1500. || s.rho[p] < 0.) ++bad;
+    s.p[p] = s.q.cs * s.q.cs * (s.rho[p] - s.q.rho0);
+    if (s.p[p] < 0) s.p[p] = 0;
+  }
+  return bad;
+}
+
+// Compute_Acceleration_BT, src/ALESPH_acceleration_bt.h:12-295 (fluid only).
+// Sound speed of the pair: the sum cs(rho_i) + cs(rho_j) (:178); v* from the
+// unreconstructed densities with the reconstructed average (:185).
+void ale_compute_acceleration(Case& s, int limited) {
+  const Params& q = s.q;
+  const int nc = ncells(s);
+  const double h = q.h, m = q.m, dp = q.dp, c0 = q.cs, rho0 = q.rho0;
+  const V2 g{q.gx, q.gy};
+#pragma omp parallel for schedule(dynamic, 64) num_threads(s.nthreads)
+  for (int c = 0; c < nc; ++c) {
+    int ac[9];
+    stencil(c, s.ncy, ac);
+    for (int ii = s.cstart[c]; ii < s.cstart[c + 1]; ++ii) {
+      const int ai = s.citems[ii];
+      if (s.type[ai] != T_FLUID) continue;
+      const V2 ar = s.r[ai], av = s.v[ai];
+      const double arho = s.rho[ai], aomega = s.omega[ai];
+      V2 omegaa_temp{0., 0.};
+      double gamma = 0;
+      for (int k = 0; k < 9; ++k) {
+        const int cl = ac[k];
+        if (cl < 0) continue;
+        for (int jj = s.cstart[cl]; jj < s.cstart[cl + 1]; ++jj) {
+          const int ni = s.citems[jj];
+          if (ai == ni) continue;
+          const V2 nr = s.r[ni], nv = s.v[ni];
+          const double nrho = s.rho[ni], nomega = s.omega[ni];
+          const V2 dr = ar - nr;
+          const double drs = std::sqrt((dr.x * dr.x) + (dr.y * dr.y));
+          const V2 drn = dr * (1 / drs) * (-1);
+          const KW kw = wendland(drs, h);
+          const double W = kw.w;
+          const V2 dW = dr * kw.f;
+          const MusclLR lr = muscl_states(s, ai, ni, dr, limited);
+          const double vl = lr.vL.x * drn.x + lr.vL.y * drn.y;
+          const double vr = lr.vR.x * drn.x + lr.vR.y * drn.y;
+          const double avgc = ale_cs(arho, rho0, c0) + ale_cs(nrho, rho0, c0);
+          double rhos = dens_riemann_lim(lr.rhoL, lr.rhoR, vl, vr, 0.5 * (lr.rhoL + lr.rhoR), avgc);
+          const double vss = vel_riemann(arho, nrho, vl, vr, 0.5 * (lr.rhoL + lr.rhoR), avgc);
+          const V2 vs = drn * vss + ((av + nv) * 0.5 - drn * (vl + vr) * 0.5);
+          double ps = c0 * c0 * (rhos - rho0);
+          const V2 vz = av;
+          if (s.type[ni] == T_WALL) {
+            const V2 nb = s.n[ni] * (-1);
+            rhos = lr.rhoL - ((0 - lr.vL.x) * nb.x + (0 - lr.vL.y) * nb.y) * lr.rhoL / c0;
+            ps = c0 * c0 * (rhos - 1000.) * 1;
+            omegaa_temp.x -= 2 * W * dp * (ps * nb.x);
+            omegaa_temp.y -= 2 * W * dp * (ps * nb.y);
+          } else {
+            omegaa_temp.x -= 2 * nomega * ((rhos * vs.x * (vs.x - vz.x) + ps) * dW.x +
+                                           (rhos * vs.x * (vs.y - vz.y)) * dW.y);
+            omegaa_temp.y -= 2 * nomega * ((rhos * vs.y * (vs.x - vz.x)) * dW.x +
+                                           (rhos * vs.y * (vs.y - vz.y) + ps) * dW.y);
+          }
+          gamma += W * m / nrho;
+        }
+      }
+      if (gamma == 0)
+        s.domegav[ai] = V2{0., 0.} - g * aomega * arho;
+      else
+        s.domegav[ai] = omegaa_temp * aomega - g * aomega * arho;
+    }
+  }
+}
+
+// Integrate_LeapFrog_partOne_ALESPH, src/ALESPH_integration.h:9-31 (fluid only)
+void ale_leapfrog_one(Case& s, double dt) {
+  for (int p = 0; p < s.np; ++p) {
+    if (s.type[p] != T_FLUID) continue;
+    s.omegav[p] = s.omegav[p] + s.domegav[p] * dt * 0.5;
+    s.v[p] = s.omegav[p] / (s.rho[p] * s.omega[p]);
+  }
+}
+
+// Integrate_LeapFrog_partTwo_ALESPH, src/ALESPH_integration.h:33-83 (fluid only)
+void ale_leapfrog_two(Case& s, double dt, int step) {
+  for (int p = 0; p < s.np; ++p) {
+    if (s.type[p] != T_FLUID) continue;
+    if (step == 1) {
+      s.omegav[p] = s.omegav[p] + s.domegav[p] * dt * 0.5;
+      s.v[p] = s.omegav[p] / (s.rho[p] * s.omega[p]);
+      s.r[p] = s.r[p] + s.v[p] * dt * 0.5;
+    } else {
+      s.omegav[p] = s.omegav[p] + s.domegav[p] * dt;
+      s.v[p] = s.omegav[p] / (s.rho[p] * s.omega[p]);
+      s.r[p] = s.r[p] + s.v[p] * dt;
+    }
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -601,7 +898,8 @@ void orc_load(void* vc, int np, const int* type, const double* p, const double* 
 static std::vector<V2>* v2f(Case& s, int f) {
   switch (f) { case 0: return &s.r; case 1: return &s.v; case 2: return &s.a; case 6: return &s.r_o;
     case 7: return &s.v_o; case 9: return &s.gnr; case 10: return &s.n; case 13: return &s.omegav;
-    case 16: return &s.omegav_o; case 19: return &s.domegav; default: return nullptr; }
+    case 16: return &s.omegav_o; case 19: return &s.domegav; case 20: return &s.gradrho;
+    case 21: return &s.gradvx; case 22: return &s.gradvy; default: return nullptr; }
 }
 static std::vector<double>* sf(Case& s, int f) {
   switch (f) { case 3: return &s.p; case 4: return &s.rho; case 5: return &s.drho; case 8: return &s.rho_o;
@@ -628,6 +926,24 @@ void orc_init_ale(void* vc) {
   s.domega.assign(s.np, 0.);
   s.domegarho.assign(s.np, 0.);
   s.domegav.assign(s.np, V2{0., 0.});
+  s.gradrho.assign(s.np, V2{0., 0.});
+  s.gradvx.assign(s.np, V2{0., 0.});
+  s.gradvy.assign(s.np, V2{0., 0.});
+}
+void orc_ale_compute_density(void* vc) { ale_compute_density(*static_cast<Case*>(vc), 0); }
+void orc_ale_compute_density_limited(void* vc) { ale_compute_density(*static_cast<Case*>(vc), 1); }
+void orc_ale_compute_acceleration(void* vc) { ale_compute_acceleration(*static_cast<Case*>(vc), 0); }
+void orc_ale_compute_acceleration_limited(void* vc) {
+  ale_compute_acceleration(*static_cast<Case*>(vc), 1);
+}
+void orc_ale_kernel_gradient(void* vc) { ale_kernel_gradient(*static_cast<Case*>(vc)); }
+int orc_ale_density_pressure(void* vc, double dt) { return ale_density_pressure(*static_cast<Case*>(vc), dt); }
+void orc_ale_leapfrog_one(void* vc, double dt) { ale_leapfrog_one(*static_cast<Case*>(vc), dt); }
+void orc_ale_leapfrog_two(void* vc, double dt, int step) {
+  ale_leapfrog_two(*static_cast<Case*>(vc), dt, step);
+}
+double orc_minmod_phi(double aC, double aD, double gx, double gy, double rx, double ry) {
+  return minmod_phi(aC, aD, V2{gx, gy}, V2{rx, ry});
 }
 void orc_rale_forces(void* vc) { rale_forces(*static_cast<Case*>(vc)); }
 void orc_rale_leapfrog_one(void* vc, double dt) { rale_leapfrog_one(*static_cast<Case*>(vc), dt); }
@@ -764,6 +1080,7 @@ int orc_remove_out_of_domain(void* vc) {
       if (!s.omega.empty()) {
         er(s.omega); er(s.omegarho); er(s.omegav); er(s.omega_o); er(s.omegarho_o);
         er(s.omegav_o); er(s.domega); er(s.domegarho); er(s.domegav);
+        er(s.gradrho); er(s.gradvx); er(s.gradvy);
       }
       s.np--; s.np_out++; removed++;
     }

@@ -172,6 +172,78 @@ def test_rale_steps_bit_exact():
     assert_same(R, O, (po.F_R, po.F_V, po.F_P, po.F_RHO) + po.ALE_FIELDS)
 
 
+def ale_step(C, step, dt, oracle, gradients=False):
+    """cases/ALESPH_dambreakLong/SPH_simulation.h:78-125; with gradients=True the
+    commented-out Kernel_gradient_approx call at :103 (after Compute_Density) is
+    made."""
+    if step > 1:
+        C.call("ale_leapfrog_one", dt)
+    C.call("clear_cells")
+    C.call("particle_to_cell")
+    C.call("ale_compute_density")
+    if gradients:
+        C.call("ale_kernel_gradient")
+    C.call("ale_density_pressure", dt)
+    C.call("ale_compute_acceleration")
+    C.call("ale_leapfrog_two", dt, step)
+    C.call("remove_out_of_domain")
+
+
+ALE_ALL = (po.F_R, po.F_V, po.F_P, po.F_RHO) + po.ALE_FIELDS + po.GRAD_FIELDS
+
+
+@pytest.mark.skipif(not po.ref_available("rale"), reason="oracle/_ref/ref_rale.so not built")
+@pytest.mark.parametrize("gradients", [False, True])
+def test_ale_muscl_steps_bit_exact(gradients):
+    """ALE-SPH (Compute_Density / Compute_Acceleration_BT, leap-frog halves and
+    Integrate_density_compute_pressure), first order as the live driver runs
+    it, and with the MUSCL gradients of Kernel_gradient_approx."""
+    R, q, dt = dam_break_rale()
+    O = oracle_from_ref(R, q, po.F_N)
+    O.call("init_ale")
+    for f in ALE_ALL:
+        assert np.array_equal(R.get(f), O.get(f)), f
+    for c in (R, O):
+        c.call("divide_to_cells")
+        c.call("particle_to_cell")
+    for step in range(1, 16):
+        ale_step(R, step, dt, False, gradients)
+        ale_step(O, step, dt, True, gradients)
+    assert_same(R, O, ALE_ALL)
+    if gradients:
+        assert np.abs(O.get(po.F_GRADRHO)).max() > 0
+
+
+@pytest.mark.skipif(not po.ref_available("rale"), reason="oracle/_ref/ref_rale.so not built")
+def test_ale_density_pass_fluid_volume_flux_is_zero():
+    """Compute_Density reads a never-assigned `dvdW` for fluid neighbours
+    (src/ALESPH_density_and_omega_bt.h:135,237).  The reference build evaluates
+    that term as zero: with random velocities, interior fluid particles get
+    domega == 0 while the RALE pass (which assigns it, src/RALESPH_interaction.h:135)
+    does not.  The oracle reproduces the reference build."""
+    R, q, dt = dam_break_rale()
+    rng = np.random.default_rng(1)
+    v = R.get(po.F_V)
+    v[:] = rng.normal(0, 0.3, v.shape)
+    R.set(po.F_V, v)
+    O = oracle_from_ref(R, q, po.F_N)
+    O.call("init_ale")
+    for c in (R, O):
+        c.call("divide_to_cells")
+        c.call("particle_to_cell")
+        c.call("density_to_pressure")
+    R.call("ale_compute_density")
+    O.call("ale_compute_density")
+    assert np.array_equal(R.get(po.F_DOMEGA), O.get(po.F_DOMEGA))
+    assert np.array_equal(R.get(po.F_DOMEGARHO), O.get(po.F_DOMEGARHO))
+    r = R.get(po.F_R)
+    interior = (R.types() == 0) & (r[:, 0] > 0.2) & (r[:, 0] < 0.4) & (r[:, 1] > 0.1) & (r[:, 1] < 0.2)
+    assert interior.sum() > 10
+    assert np.all(R.get(po.F_DOMEGA)[interior] == 0)
+    R.call("compute_forces")  # RALE pass assigns dv.dW
+    assert np.any(R.get(po.F_DOMEGA)[interior] != 0)
+
+
 @pytest.mark.skipif(not po.ref_available("rale"), reason="oracle/_ref/ref_rale.so not built")
 def test_rale_isolated_particle_branch():
     """The reference never resets gamma_sum between the particles of a cell

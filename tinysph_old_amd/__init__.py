@@ -4,6 +4,6 @@ C ABI (include/tinysph.h).  This package is the Python host layer."""
 from ._lib import (  # noqa: F401
     CASE_ALE_DAM_BREAK, CASE_DAM_BREAK_BT, CASE_DAM_BREAK_MDBC, CASE_STILL_WATER_DBC, FLUID, FORCES_BT, FORCES_DBC,
     FORCES_WCSPH, PREC_FP64, PREC_MIXED, WALL, WALL_BT, WALL_DBC, WALL_MDBC, SCHEME_RALE,
-    SCHEME_SYMPLECTIC, TsphError, lib,
+    SCHEME_SYMPLECTIC, MUSCL_LINEAR, MUSCL_MINMOD, MUSCL_NONE, TsphError, lib,
 )
 from .system import CaseData, ParticleSystem, build_case, forces_variant, system_for_case  # noqa: F401

@@ -50,8 +50,9 @@ class CaseC(C.Structure):
 
 
 ALE_NAMES = ("omega", "omegarho", "omegav", "omega_o", "omegarho_o", "omegav_o", "domega",
-             "domegarho", "domegav")
-ALE_VEC = ("omegav", "omegav_o", "domegav")
+             "domegarho", "domegav", "gradrho", "gradvx", "gradvy")
+ALE_VEC = ("omegav", "omegav_o", "domegav", "gradrho", "gradvx", "gradvy")
+MUSCL_NONE, MUSCL_LINEAR, MUSCL_MINMOD = 0, 1, 2
 
 
 class AleFields(C.Structure):
@@ -97,6 +98,13 @@ EXPORTS = {
     "tsph_rale_leapfrog_one": (C.c_int, [C.c_void_p, C.c_double]),
     "tsph_rale_leapfrog_two": (C.c_int, [C.c_void_p, C.c_double, C.c_int]),
     "tsph_step_rale": (C.c_int, [C.c_void_p, C.c_double, C.c_int, C.c_int]),
+    "tsph_ale_compute_density": (C.c_int, [C.c_void_p, C.c_int]),
+    "tsph_ale_kernel_gradient": (C.c_int, [C.c_void_p]),
+    "tsph_ale_density_pressure": (C.c_int, [C.c_void_p, C.c_double]),
+    "tsph_ale_compute_acceleration": (C.c_int, [C.c_void_p, C.c_int]),
+    "tsph_ale_leapfrog_one": (C.c_int, [C.c_void_p, C.c_double]),
+    "tsph_ale_leapfrog_two": (C.c_int, [C.c_void_p, C.c_double, C.c_int]),
+    "tsph_step_ale": (C.c_int, [C.c_void_p, C.c_double, C.c_int, C.c_int, C.c_int]),
     "tsph_case_build": (C.c_int, [C.c_int, C.c_double, C.c_double, C.c_uint64,
                                   C.POINTER(C.POINTER(CaseC))]),
     "tsph_case_free": (C.c_int, [C.POINTER(CaseC)]),

@@ -22,7 +22,8 @@ PROBE_LIB = os.path.join(PKG, "libtsph_copy_probe.so")
 PROBE_SOURCE = "tsph_copy_probe.hip"
 OBJDIR = os.path.join(PKG, "_build")
 
-HIP_SOURCES = ["tsph_api.hip", "tsph_cells.hip", "tsph_pairs.hip", "tsph_stages.hip", "tsph_ale.hip"]
+HIP_SOURCES = ["tsph_api.hip", "tsph_cells.hip", "tsph_pairs.hip", "tsph_stages.hip", "tsph_ale.hip",
+               "tsph_alemuscl.hip"]
 HOST_SOURCES = ["tsph_cases.cpp"]
 
 ARCH = os.environ.get("TSPH_OFFLOAD_ARCH", "gfx950")

@@ -426,6 +426,7 @@ __global__ void k_ale_init(Soa d, const int *__restrict__ scal, double dp) {
   d.domega[i] = 0.;
   d.domegarho[i] = 0.;
   d.domegav[i] = make_double2(0., 0.);
+  d.gradrho[i] = d.gradvx[i] = d.gradvy[i] = make_double2(0., 0.);
 }
 
 // Integrate_LeapFrog_partOne, src/RALESPH_integration.h:10-36

@@ -51,14 +51,16 @@ static int alloc_soa(Soa &s, int cap, bool gnr, bool n) {
 static void free_soa(Soa &s) {
   void *ps[] = {s.r, s.v, s.a, s.r_o, s.v_o, s.gnr, s.n, s.rho, s.drho, s.p, s.rho_o,
                 s.type, s.seq, s.index, s.omega, s.omegarho, s.omega_o, s.omegarho_o,
-                s.domega, s.domegarho, s.omegav, s.omegav_o, s.domegav};
+                s.domega, s.domegarho, s.omegav, s.omegav_o, s.domegav, s.gradrho, s.gradvx,
+                s.gradvy};
   for (void *p : ps)
     if (p) (void)hipFree(p);
   s = Soa{};
 }
 
 static void drop_graph(tsph_ctx *c) {
-  for (Graph *g : {&c->graph[0], &c->graph[1], &c->rale_graph[0], &c->rale_graph[1]}) {
+  for (Graph *g : {&c->graph[0], &c->graph[1], &c->rale_graph[0], &c->rale_graph[1],
+                   &c->ale_graph[0], &c->ale_graph[1]}) {
     if (g->exec) (void)hipGraphExecDestroy(g->exec);
     *g = Graph{};
   }
@@ -150,7 +152,8 @@ static int alloc_ale(Soa &s, int cap) {
   if ((e = dalloc(&s.omega, cap)) || (e = dalloc(&s.omegarho, cap)) || (e = dalloc(&s.omega_o, cap)) ||
       (e = dalloc(&s.omegarho_o, cap)) || (e = dalloc(&s.domega, cap)) ||
       (e = dalloc(&s.domegarho, cap)) || (e = dalloc(&s.omegav, cap)) ||
-      (e = dalloc(&s.omegav_o, cap)) || (e = dalloc(&s.domegav, cap)))
+      (e = dalloc(&s.omegav_o, cap)) || (e = dalloc(&s.domegav, cap)) ||
+      (e = dalloc(&s.gradrho, cap)) || (e = dalloc(&s.gradvx, cap)) || (e = dalloc(&s.gradvy, cap)))
     return e;
   return 0;
 }
@@ -161,6 +164,19 @@ static int record_rale_step(tsph_ctx *c, double dt, int step) {
   if (step > 1 && (e = launch_rale_leapfrog_one(c, dt))) return e;
   if ((e = launch_cell_list(c)) || (e = launch_eos(c)) || (e = launch_rale_forces(c))) return e;
   if ((e = launch_rale_leapfrog_two(c, dt, step)) || (e = launch_remove(c))) return e;
+  return 0;
+}
+
+// one ALE-SPH step, cases/ALESPH_dambreakLong/SPH_simulation.h:78-125; with
+// muscl > 0 the gradients are computed where the driver has the (commented)
+// Kernel_gradient_approx call, right after Compute_Density (:103)
+static int record_ale_step(tsph_ctx *c, double dt, int step, int muscl) {
+  int e = 0;
+  if (step > 1 && (e = launch_ale_leapfrog_one(c, dt))) return e;
+  if ((e = launch_cell_list(c)) || (e = launch_ale_density(c, muscl))) return e;
+  if (muscl && (e = launch_ale_gradient(c))) return e;
+  if ((e = launch_ale_density_pressure(c, dt)) || (e = launch_ale_acceleration(c, muscl))) return e;
+  if ((e = launch_ale_leapfrog_two(c, dt, step)) || (e = launch_remove(c))) return e;
   return 0;
 }
 
@@ -553,7 +569,8 @@ int tsph_ale_upload(tsph_ctx *c, const tsph_ale_fields *h) {
       (e = put2(d.omegav, h->omegav)) || (e = put1(d.omega_o, h->omega_o)) ||
       (e = put1(d.omegarho_o, h->omegarho_o)) || (e = put2(d.omegav_o, h->omegav_o)) ||
       (e = put1(d.domega, h->domega)) || (e = put1(d.domegarho, h->domegarho)) ||
-      (e = put2(d.domegav, h->domegav)))
+      (e = put2(d.domegav, h->domegav)) || (e = put2(d.gradrho, h->gradrho)) ||
+      (e = put2(d.gradvx, h->gradvx)) || (e = put2(d.gradvy, h->gradvy)))
     return e;
   return 0;
 }
@@ -589,7 +606,8 @@ int tsph_ale_download(tsph_ctx *c, tsph_ale_fields *h) {
       (e = get2(h->omegav, d.omegav)) || (e = get1(h->omega_o, d.omega_o)) ||
       (e = get1(h->omegarho_o, d.omegarho_o)) || (e = get2(h->omegav_o, d.omegav_o)) ||
       (e = get1(h->domega, d.domega)) || (e = get1(h->domegarho, d.domegarho)) ||
-      (e = get2(h->domegav, d.domegav)))
+      (e = get2(h->domegav, d.domegav)) || (e = get2(h->gradrho, d.gradrho)) ||
+      (e = get2(h->gradvx, d.gradvx)) || (e = get2(h->gradvy, d.gradvy)))
     return e;
   h->np = m;
   return 0;
@@ -627,6 +645,53 @@ int tsph_step_rale(tsph_ctx *c, double dt, int step, int nsteps) {
     if (int e = record_rale_step(c, dt, s)) return e;
   if (s >= last) return 0;
   return replay_step(c, c->rale_graph, 100, dt, last - s, [&] { return record_rale_step(c, dt, 2); });
+}
+
+int tsph_ale_compute_density(tsph_ctx *c, int limited) {
+  if (int e = rale_ready(c)) return e;
+  if (!c->cells_valid) return fail(TSPH_ESTATE, "no cell list: call tsph_particle_to_cell first");
+  return launch_ale_density(c, limited ? 2 : 1);
+}
+
+int tsph_ale_kernel_gradient(tsph_ctx *c) {
+  if (int e = rale_ready(c)) return e;
+  if (!c->cells_valid) return fail(TSPH_ESTATE, "no cell list: call tsph_particle_to_cell first");
+  return launch_ale_gradient(c);
+}
+
+int tsph_ale_density_pressure(tsph_ctx *c, double dt) {
+  if (int e = rale_ready(c)) return e;
+  return launch_ale_density_pressure(c, dt);
+}
+
+int tsph_ale_compute_acceleration(tsph_ctx *c, int limited) {
+  if (int e = rale_ready(c)) return e;
+  if (!c->cells_valid) return fail(TSPH_ESTATE, "no cell list: call tsph_particle_to_cell first");
+  return launch_ale_acceleration(c, limited ? 2 : 1);
+}
+
+int tsph_ale_leapfrog_one(tsph_ctx *c, double dt) {
+  if (int e = rale_ready(c)) return e;
+  return launch_ale_leapfrog_one(c, dt);
+}
+
+int tsph_ale_leapfrog_two(tsph_ctx *c, double dt, int step) {
+  if (int e = rale_ready(c)) return e;
+  return launch_ale_leapfrog_two(c, dt, step);
+}
+
+int tsph_step_ale(tsph_ctx *c, double dt, int step, int nsteps, int muscl) {
+  if (int e = rale_ready(c)) return e;
+  if (step < 1 || nsteps < 0) return fail(TSPH_EINVAL, "steps are numbered from 1");
+  if (muscl < TSPH_MUSCL_NONE || muscl > TSPH_MUSCL_MINMOD) return fail(TSPH_EINVAL, "unknown MUSCL mode");
+  int s = step;
+  const int last = step + nsteps;
+  // the first step differs (src/ALESPH_integration.h:37-58): launched eagerly
+  for (; s < last && (s == 1 || c->time_pairs); ++s)
+    if (int e = record_ale_step(c, dt, s, muscl)) return e;
+  if (s >= last) return 0;
+  return replay_step(c, c->ale_graph, 200 + muscl, dt, last - s,
+                     [&] { return record_ale_step(c, dt, 2, muscl); });
 }
 
 int tsph_water_elevation(tsph_ctx *c, double x, double y0, int inverse, double *level) {

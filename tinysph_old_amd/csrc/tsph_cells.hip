@@ -273,6 +273,11 @@ __global__ void k_reorder(const int *__restrict__ cell_start, int ncells, const 
       dst.domega[k] = src.domega[j];
       dst.domegarho[k] = src.domegarho[j];
       dst.domegav[k] = src.domegav[j];
+      if (src.gradrho) {
+        dst.gradrho[k] = src.gradrho[j];
+        dst.gradvx[k] = src.gradvx[j];
+        dst.gradvy[k] = src.gradvy[j];
+      }
     }
   }
   const unsigned long long m = __ballot(wall);

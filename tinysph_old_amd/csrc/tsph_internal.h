@@ -42,6 +42,8 @@ struct Soa {
   double *omega = nullptr, *omegarho = nullptr, *omega_o = nullptr, *omegarho_o = nullptr;
   double *domega = nullptr, *domegarho = nullptr;
   double2 *omegav = nullptr, *omegav_o = nullptr, *domegav = nullptr;
+  // ALE-SPH MUSCL gradients of rho, vx, vy (allocated with the volume variables)
+  double2 *gradrho = nullptr, *gradvx = nullptr, *gradvy = nullptr;
 };
 
 // FP32 pair records of the mixed path, rebuilt after every reorder.
@@ -121,6 +123,7 @@ struct tsph_ctx {
   int parity = 0;             // flips whenever cur and alt are swapped
   tsph::Graph graph[2];       // symplectic step, by parity at the step's start
   tsph::Graph rale_graph[2];  // Riemann-ALE step, by parity at the step's start
+  tsph::Graph ale_graph[2];   // ALE-SPH MUSCL step, by parity at the step's start
 };
 
 namespace tsph {
@@ -156,5 +159,12 @@ int launch_rale_forces(tsph_ctx *c);
 int launch_rale_leapfrog_one(tsph_ctx *c, double dt);
 int launch_rale_leapfrog_two(tsph_ctx *c, double dt, int step);
 int run_interpolate_point(tsph_ctx *c, double x, double y, double *out4);
+// tsph_alemuscl.hip (muscl: 0 first order, 1 alpha = 1, 2 minmod-limited)
+int launch_ale_density(tsph_ctx *c, int muscl);
+int launch_ale_acceleration(tsph_ctx *c, int muscl);
+int launch_ale_gradient(tsph_ctx *c);
+int launch_ale_density_pressure(tsph_ctx *c, double dt);
+int launch_ale_leapfrog_one(tsph_ctx *c, double dt);
+int launch_ale_leapfrog_two(tsph_ctx *c, double dt, int step);
 
 }  // namespace tsph

@@ -105,6 +105,201 @@ __device__ __forceinline__ int slice_of(const Tile &t, int l) {
   return l >= t.lq[2] ? 2 : (l >= t.lq[1] ? 1 : 0);
 }
 
+// --------------------------------------------------------------------------
+// Generic tile walks.  A pass supplies a policy P:
+//   P::Lds            pointers into the staged-slot area of LDS
+//   P::lds(base, cap) carves that area (double2 / float2 arrays first)
+//   P::kSlotBytes     bytes per staged slot
+//   stage(L, l, j[, ox, oy])  slot l <- particle j (FP32 walk: tile-frame origin)
+//   active(i)         whether particle i is active in the pass
+//   Own own(i[, ox, oy]); Acc{} accumulators
+//   pair_lds(acc, own, L, l[, drx, dry, r2]) / pair_glob(acc, own, j[, ...])
+//   finish(acc, own, i)
+// walk64 visits neighbours in the reference order (9-cell stencil order,
+// ascending index in a cell) and tests support exactly as the reference does;
+// walk32 visits the three column slices in memory order with an FP32 test.
+// --------------------------------------------------------------------------
+template <class P>
+__device__ __forceinline__ void walk64(const P &pol, const Consts &k, const int *__restrict__ cell_start,
+                                       const int *__restrict__ cellof, int rows,
+                                       const int *__restrict__ scal, const int *__restrict__ tile_list,
+                                       int cap, unsigned long long *__restrict__ counters) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  int *tab = reinterpret_cast<int *>(smem);
+  uint16_t *s_list = reinterpret_cast<uint16_t *>(smem + kTabBytes);
+  const typename P::Lds L = P::lds(smem + kTabBytes + sizeof(uint16_t) * kList * kWave, cap);
+  if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&counters[C_PASSES], 1ull);
+  unsigned npair = 0, ncand = 0, nact = 0;
+  int it, it_end, it_step;
+  band_of_wave(scal[S_NTILES], it, it_end, it_step);
+  for (; it < it_end; it += it_step) {
+    __syncthreads();
+    Tile t;
+    tile_begin(k, cell_start, tile_list[it], rows, tab, t);
+    const bool staged = t.lq[3] <= cap;
+    if (staged)
+      for (int l = threadIdx.x; l < t.lq[3]; l += blockDim.x) pol.stage(L, l, l - sel3(t.base, slice_of(t, l)));
+    __syncthreads();
+    uint16_t *my_list = s_list + threadIdx.x;
+    for (int i0 = t.own0; i0 < t.own1; i0 += kWave) {
+      const int i = i0 + threadIdx.x;
+      int li = 0, cy = 0;
+      bool act = false;
+      if (i < t.own1) {
+        li = i + t.base[1];
+        act = pol.active(i);
+        cy = cellof[i] - t.cx * k.ncy;
+      }
+      nact += act;
+      const typename P::Own o = pol.own(act ? i : -1);
+      typename P::Acc acc{};
+      if (!staged) {
+        if (act) {
+          const int c = cellof[i];
+          for (int q = 0; q < 9; ++q) {
+            int dx, dy;
+            stencil_dxy(q, dx, dy);
+            if (t.cx + dx < 0 || t.cx + dx >= k.ncx || cy + dy < 0 || cy + dy >= k.ncy) continue;
+            const int cl = c + dy + dx * k.ncy;
+            for (int j = cell_start[cl]; j < cell_start[cl + 1]; ++j) {
+              if (j == i) continue;
+              ++ncand;
+              const double2 nr = pol.pos_glob(j);
+              if (!in_support64(o.r.x - nr.x, o.r.y - nr.y, k.h)) continue;
+              ++npair;
+              pol.pair_glob(acc, o, j);
+            }
+          }
+        }
+      } else {
+        int q = -1, lj = 0, lend = 0;
+        bool more = act;
+        while (__any(more)) {
+          int cnt = 0;
+          while (more && cnt < kList) {
+            if (lj >= lend) {
+              ++q;
+              if (q == 9) { more = false; break; }
+              int dx, dy;
+              stencil_dxy(q, dx, dy);
+              const int row = cy + dy;
+              if (row < 0 || row >= k.ncy) { lend = lj; continue; }
+              const int e = (dx + 1) * t.w + (row - t.row_lo);
+              const int bq = sel3(t.base, dx + 1);
+              lj = tab[e] + bq;
+              lend = tab[e + 1] + bq;
+              continue;
+            }
+            const int j = lj++;
+            if (j == li) continue;
+            ++ncand;
+            const double2 nr = pol.pos_lds(L, j);
+            if (in_support64(o.r.x - nr.x, o.r.y - nr.y, k.h)) my_list[(cnt++) * kWave] = (uint16_t)j;
+          }
+          npair += cnt;
+          for (int e = 0; e < cnt; ++e) pol.pair_lds(acc, o, L, my_list[e * kWave]);
+        }
+      }
+      if (act) pol.finish(acc, o, i);
+    }
+  }
+  wave_count(npair, ncand, nact, counters);
+}
+
+template <class P>
+__device__ __forceinline__ void walk32(const P &pol, const Consts &k, const int *__restrict__ cell_start,
+                                       const int *__restrict__ cellof, int rows,
+                                       const int *__restrict__ scal, const int *__restrict__ tile_list,
+                                       int cap, unsigned long long *__restrict__ counters) {
+#pragma clang fp contract(fast)
+  extern __shared__ __align__(16) unsigned char smem[];
+  int *tab = reinterpret_cast<int *>(smem);
+  uint16_t *s_list = reinterpret_cast<uint16_t *>(smem + kTabBytes);
+  const typename P::Lds L = P::lds(smem + kTabBytes + sizeof(uint16_t) * kList * kWave, cap);
+  const float r2cut = 4.f * (float)k.h * (float)k.h;
+  if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&counters[C_PASSES], 1ull);
+  unsigned npair = 0, ncand = 0, nact = 0;
+  int it, it_end, it_step;
+  band_of_wave(scal[S_NTILES], it, it_end, it_step);
+  for (; it < it_end; it += it_step) {
+    __syncthreads();
+    Tile t;
+    tile_begin(k, cell_start, tile_list[it], rows, tab, t);
+    const bool staged = t.lq[3] <= cap;
+    const int mid = t.row_lo + (t.nrows >> 1);
+    const double ox = k.x0 - k.of + t.cx * k.kh, oy = k.y0 - k.of + mid * k.kh;
+    if (staged)
+      for (int l = threadIdx.x; l < t.lq[3]; l += blockDim.x)
+        pol.stage(L, l, l - sel3(t.base, slice_of(t, l)), ox, oy);
+    __syncthreads();
+    uint16_t *my_list = s_list + threadIdx.x;
+    for (int i0 = t.own0; i0 < t.own1; i0 += kWave) {
+      const int i = i0 + threadIdx.x;
+      int li = 0, ri = 0;
+      bool act = false;
+      if (i < t.own1) {
+        li = i + t.base[1];
+        act = pol.active(i);
+        ri = cellof[i] - t.cx * k.ncy - t.row_lo;
+      }
+      nact += act;
+      const typename P::Own o = pol.own(act ? i : -1, ox, oy);
+      typename P::Acc acc{};
+      if (!staged) {
+        if (act) {
+          const int cc = cellof[i];
+          const int cy = cc - t.cx * k.ncy;
+          for (int dx = -1; dx <= 1; ++dx) {
+            if (t.cx + dx < 0 || t.cx + dx >= k.ncx) continue;
+            for (int dy = -1; dy <= 1; ++dy) {
+              if (cy + dy < 0 || cy + dy >= k.ncy) continue;
+              const int cl = cc + dy + dx * k.ncy;
+              for (int j = cell_start[cl]; j < cell_start[cl + 1]; ++j) {
+                if (j == i) continue;
+                ++ncand;
+                const float2 pj = pol.pos_glob(j, ox, oy);
+                const float drx = o.p.x - pj.x, dry = o.p.y - pj.y;
+                const float r2 = drx * drx + dry * dry;
+                if (!(r2 < r2cut)) continue;
+                ++npair;
+                pol.pair_glob(acc, o, j, drx, dry, r2);
+              }
+            }
+          }
+        }
+      } else {
+        const int rlo = max(ri - 1, 0), rhi = min(ri + 2, t.nrows);
+        const int a0 = tab[rlo] + t.base[0], n0 = tab[rhi] + t.base[0] - a0;
+        const int a1 = tab[t.w + rlo] + t.base[1], n1 = tab[t.w + rhi] + t.base[1] - a1;
+        const int a2 = tab[2 * t.w + rlo] + t.base[2], n2 = tab[2 * t.w + rhi] + t.base[2] - a2;
+        const int tot = act ? n0 + n1 + n2 : 0;
+        int cur = 0;
+        while (__any(cur < tot)) {
+          int cnt = 0;
+          while (cur < tot && cnt < kList) {
+            const int j = cur < n0 ? a0 + cur : (cur < n0 + n1 ? a1 + (cur - n0) : a2 + (cur - n0 - n1));
+            ++cur;
+            if (j == li) continue;
+            ++ncand;
+            const float2 pj = pol.pos_lds(L, j);
+            const float drx = o.p.x - pj.x, dry = o.p.y - pj.y;
+            if (drx * drx + dry * dry < r2cut) my_list[(cnt++) * kWave] = (uint16_t)j;
+          }
+          npair += cnt;
+          for (int e = 0; e < cnt; ++e) {
+            const int j = my_list[e * kWave];
+            const float2 pj = pol.pos_lds(L, j);
+            const float drx = o.p.x - pj.x, dry = o.p.y - pj.y;
+            pol.pair_lds(acc, o, L, j, drx, dry, drx * drx + dry * dry);
+          }
+        }
+      }
+      if (act) pol.finish(acc, o, i);
+    }
+  }
+  wave_count(npair, ncand, nact, counters);
+}
+
 // a pair-loop launch, bracketed by HIP events while pair timing is enabled
 template <typename K>
 int launch_timed(tsph_ctx *c, K kernel_launch) {

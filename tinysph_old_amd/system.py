@@ -280,6 +280,28 @@ class ParticleSystem:
     def step_rale(self, dt: float, step: int, nsteps: int = 1):
         L.check(self.lib.tsph_step_rale(self.ctx, dt, step, nsteps), "tsph_step_rale")
 
+    # ---- ALE-SPH with MUSCL reconstruction (src/ALESPH_*.h) ----
+    def ale_compute_density(self, limited: bool = False):
+        L.check(self.lib.tsph_ale_compute_density(self.ctx, int(limited)), "Compute_Density")
+
+    def ale_kernel_gradient(self):
+        L.check(self.lib.tsph_ale_kernel_gradient(self.ctx), "Kernel_gradient_approx")
+
+    def ale_density_pressure(self, dt: float):
+        L.check(self.lib.tsph_ale_density_pressure(self.ctx, dt), "Integrate_density_compute_pressure")
+
+    def ale_compute_acceleration(self, limited: bool = False):
+        L.check(self.lib.tsph_ale_compute_acceleration(self.ctx, int(limited)), "Compute_Acceleration_BT")
+
+    def ale_leapfrog_one(self, dt: float):
+        L.check(self.lib.tsph_ale_leapfrog_one(self.ctx, dt), "Integrate_LeapFrog_partOne_ALESPH")
+
+    def ale_leapfrog_two(self, dt: float, step: int):
+        L.check(self.lib.tsph_ale_leapfrog_two(self.ctx, dt, step), "Integrate_LeapFrog_partTwo_ALESPH")
+
+    def step_ale(self, dt: float, step: int, nsteps: int = 1, muscl: int = 0):
+        L.check(self.lib.tsph_step_ale(self.ctx, dt, step, nsteps, muscl), "tsph_step_ale")
+
     def set_waves_per_cu(self, waves: int):
         L.check(self.lib.tsph_set_waves_per_cu(self.ctx, waves), "tsph_set_waves_per_cu")
 
