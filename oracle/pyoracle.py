@@ -103,6 +103,9 @@ def _declare(lib: C.CDLL, pfx: str) -> None:
     f("ale_leapfrog_one", None, C.c_void_p, C.c_double)
     f("ale_leapfrog_two", None, C.c_void_p, C.c_double, C.c_int)
     if pfx == "orc_":
+        f("add_zone", C.c_int, C.c_void_p, *([C.c_double] * 8), C.c_int)
+        f("bz_integrate_inlet", C.c_int, C.c_void_p, C.c_int, C.c_double)
+        f("bz_integrate_outlet", C.c_int, C.c_void_p, C.c_int, C.c_double)
         f("ale_density_pressure", C.c_int, C.c_void_p, C.c_double)
         f("ale_compute_density_limited", None, C.c_void_p)
         f("ale_compute_acceleration_limited", None, C.c_void_p)
@@ -122,6 +125,16 @@ def _declare(lib: C.CDLL, pfx: str) -> None:
         f("diffusion_fourtakas", C.c_double, *([C.c_double] * 11))
     else:
         f("add", None, C.c_void_p, C.c_int, *([C.c_double] * 6))
+        f("draw_create_inlet_y", None, C.c_void_p, *([C.c_double] * 3), C.c_int, C.c_int,
+          C.c_double, C.c_double, C.c_int)
+        f("draw_create_inlet_x", None, C.c_void_p, *([C.c_double] * 3), C.c_int, C.c_int,
+          C.c_double, C.c_double, C.c_int)
+        f("draw_create_outlet_y", None, C.c_void_p, *([C.c_double] * 3), C.c_int, C.c_int,
+          C.c_double, C.c_double, C.c_int, C.c_int)
+        f("zone_count", C.c_int, C.c_void_p)
+        f("zone_get", None, C.c_void_p, C.c_int, _dp)
+        f("bz_integrate_inlet", None, C.c_void_p, C.c_int, C.c_double)
+        f("bz_integrate_outlet", None, C.c_void_p, C.c_int, C.c_double)
         f("add_ale", None, C.c_void_p, C.c_int, *([C.c_double] * 6))
         f("push_gnr", None, C.c_void_p, C.c_double, C.c_double)
         f("push_n", None, C.c_void_p, C.c_double, C.c_double)
@@ -182,6 +195,11 @@ class Case:
     def types(self) -> np.ndarray:
         out = np.zeros(max(self.np, 1), dtype=np.int32)
         self.call("get_type", _ptr(out, C.c_int))
+        return out[: self.np]
+
+    def index(self) -> np.ndarray:
+        out = np.zeros(max(self.np, 1), dtype=np.int32)
+        self.call("get_index", _ptr(out, C.c_int))
         return out[: self.np]
 
     def grid(self):

@@ -291,6 +291,35 @@ void ref_create_block_of_fluid(void* vc, double x0, double y0, double x1, double
   Create_Block_of_Fluid(static_cast<RefCase*>(vc)->ps, x0, y0, x1, y1);
 }
 #endif
+#if defined(REF_WCSPH)
+// ---- buffer zones (src/SPH_buffer_zones.h, src/SPH_create.h:439-542) ----
+void ref_draw_create_inlet_y(void* vc, double x, double y0, double y1, int o, int nbl, double vx,
+                             double vy, int type) {
+  DRAW_CreateInletY(static_cast<RefCase*>(vc)->ps, x, y0, y1, o, nbl, {vx, vy}, type);
+}
+void ref_draw_create_inlet_x(void* vc, double y, double x0, double x1, int o, int nbl, double vx,
+                             double vy, int type) {
+  DRAW_CreateInletX(static_cast<RefCase*>(vc)->ps, y, x0, x1, o, nbl, {vx, vy}, type);
+}
+void ref_draw_create_outlet_y(void* vc, double x, double y0, double y1, int o, int nbl, double vx,
+                              double vy, int type, int empty) {
+  DRAW_CreateOutletY(static_cast<RefCase*>(vc)->ps, x, y0, y1, o, nbl, {vx, vy}, type, empty != 0);
+}
+int ref_zone_count(void* vc) { return (int)static_cast<RefCase*>(vc)->ps.zones.size(); }
+// {x0, xm, y0, ym, nbl, bl, bh, btype, vbx, vby, ox, oy}
+void ref_zone_get(void* vc, int i, double* out) {
+  const BufferZone& z = static_cast<RefCase*>(vc)->ps.zones[i];
+  const double v[12] = {z.x0, z.xm, z.y0, z.ym, (double)z.nbl, z.bl, z.bh, (double)z.btype,
+                        z.v_b.x, z.v_b.y, z.orientation.x, z.orientation.y};
+  for (int k = 0; k < 12; ++k) out[k] = v[k];
+}
+void ref_bz_integrate_inlet(void* vc, int zone, double dt) {
+  BZ_Integrate_inlet(static_cast<RefCase*>(vc)->ps, (unsigned)zone, dt);
+}
+void ref_bz_integrate_outlet(void* vc, int zone, double dt) {
+  BZ_Integrate_outlet(static_cast<RefCase*>(vc)->ps, (unsigned)zone, dt);
+}
+#endif
 void ref_create_block_of_fluid_wp(void* vc, double x0, double y0, double x1, double y1) {
   Create_Block_of_FluidWP(static_cast<RefCase*>(vc)->ps, x0, y0, x1, y1);
 }

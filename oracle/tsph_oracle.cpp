@@ -46,8 +46,17 @@ struct Params {  // SPH_constants + Simulation_data, src/SPH_data.h:30-46,111-12
   double m, gx, gy;
 };
 
+// BufferZone, src/SPH_inlet_outlet_data.h:18-46 (nbl is never stored there)
+struct Zone {
+  V2 orientation;
+  double x0, y0, xm, ym, bl, bh;
+  int btype;
+  V2 v_b;
+};
+
 struct Case {
   Params q;
+  std::vector<Zone> zones;
   int np = 0, np_out = 0;
   std::vector<int> type, index;
   std::vector<double> p, rho, drho, rho_o;
@@ -844,6 +853,144 @@ void ale_leapfrog_two(Case& s, double dt, int step) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Buffer zones, src/SPH_buffer_zones.h:10-206
+// ---------------------------------------------------------------------------
+// Particle_system::Add_particle, src/SPH_particle_system.h:25-50 (wall extras
+// are kept per particle here and get a zero entry; ALE variables, when
+// present, the ALESPH_Add_particle values)
+void add_particle(Case& s, int type, double p, double rho, V2 r, V2 v, V2 a) {
+  s.np++;
+  s.type.push_back(type);
+  s.index.push_back(s.np - 1);
+  s.p.push_back(p);
+  s.rho.push_back(rho);
+  s.drho.push_back(0.);
+  s.r.push_back(r);
+  s.v.push_back(v);
+  s.a.push_back(a);
+  s.r_o.push_back(r);
+  s.rho_o.push_back(rho);
+  s.v_o.push_back(v);
+  s.gnr.push_back(V2{0., 0.});
+  s.n.push_back(V2{0., 0.});
+  if (!s.omega.empty()) {
+    const double w = s.q.dp * s.q.dp;
+    s.omega.push_back(w); s.omegarho.push_back(rho * w); s.omegav.push_back(v * w);
+    s.omega_o.push_back(w); s.omegarho_o.push_back(rho * w); s.omegav_o.push_back(v * w);
+    s.domega.push_back(0.); s.domegarho.push_back(0.); s.domegav.push_back(V2{0., 0.});
+    s.gradrho.push_back(V2{0., 0.}); s.gradvx.push_back(V2{0., 0.}); s.gradvy.push_back(V2{0., 0.});
+  }
+}
+
+// Remove_particle, src/SPH_particle_system.h:110-146 (erases every per-particle field)
+void erase_particle(Case& s, int p) {
+  auto er = [p](auto& vec) { if ((int)vec.size() > p) vec.erase(vec.begin() + p); };
+  er(s.type); er(s.index); er(s.p); er(s.rho); er(s.drho); er(s.r); er(s.v); er(s.a);
+  er(s.r_o); er(s.v_o); er(s.rho_o); er(s.gnr); er(s.n);
+  if (!s.omega.empty()) {
+    er(s.omega); er(s.omegarho); er(s.omegav); er(s.omega_o); er(s.omegarho_o); er(s.omegav_o);
+    er(s.domega); er(s.domegarho); er(s.domegav); er(s.gradrho); er(s.gradvx); er(s.gradvy);
+  }
+  s.np--;
+  s.np_out++;
+}
+
+// BZ_Integrate_inlet, src/SPH_buffer_zones.h:10-107.  The loop bound grows
+// with every appended particle, so particles appended during the pass are
+// visited in the same pass.  Returns the number of particles appended.
+int bz_integrate_inlet(Case& s, int zi, double dt) {
+  const Zone& z = s.zones[zi];
+  const double b_len = z.bl;
+  const V2 v_inl = z.v_b;
+  const int btype = z.btype;
+  int added = 0;
+  if (z.orientation.x != 0) {
+    const double x_b = z.x0;
+    const int orientation = (int)z.orientation.x;
+    if (orientation < 0) {
+      for (int p = 0; p < s.np; p++) {
+        if (s.type[p] != btype) continue;
+        if (s.r[p].x > x_b) {
+          s.type[p] = T_FLUID;
+          add_particle(s, btype, 0, 1000, V2{s.r[p].x - b_len, s.r[p].y}, v_inl, V2{0., 0.});
+          ++added;
+        }
+      }
+    } else if (orientation > 0) {
+      for (int p = 0; p < s.np; p++) {
+        if (s.type[p] != btype) continue;
+        s.r[p].x = s.r[p].x + s.v[p].x * dt;
+        if (s.r[p].x < x_b) {
+          s.type[p] = T_FLUID;
+          add_particle(s, btype, 0, 1000, V2{s.r[p].x + b_len, s.r[p].y}, v_inl, V2{0., 0.});
+          ++added;
+        }
+      }
+    }
+  } else if (z.orientation.y != 0) {
+    const double x_b = z.y0;
+    const int orientation = (int)z.orientation.y;
+    if (orientation < 0) {
+      for (int p = 0; p < s.np; p++) {
+        if (s.type[p] != btype) continue;
+        s.r[p].y = s.r[p].y + s.v[p].y * dt;
+        if (s.r[p].y > x_b) {
+          s.type[p] = T_FLUID;
+          add_particle(s, btype, 0, 1000, V2{s.r[p].x, s.r[p].y - b_len}, v_inl, V2{0., 0.});
+          ++added;
+        }
+      }
+    } else if (orientation > 0) {
+      for (int p = 0; p < s.np; p++) {
+        if (s.type[p] != btype) continue;
+        s.r[p].y = s.r[p].y + s.v[p].y * dt;
+        if (s.r[p].y < x_b) {
+          s.type[p] = T_FLUID;
+          add_particle(s, btype, 0, 1000, V2{s.r[p].x, s.r[p].y + b_len}, v_inl, V2{0., 0.});
+          ++added;
+        }
+      }
+    }
+  }
+  return added;
+}
+
+// BZ_Integrate_outlet, src/SPH_buffer_zones.h:111-206: fluid beyond the zone
+// start becomes the zone type (a = 0, drho = 0); zone particles past the zone
+// or below dp/2 are erased inside the ascending loop, so the particle that
+// slides into the slot is not examined.  y-oriented zones do nothing.
+// Returns the number of particles removed.
+int bz_integrate_outlet(Case& s, int zi, double dt) {
+  (void)dt;
+  const Zone& z = s.zones[zi];
+  const double dp = s.q.dp;
+  const double b_len = z.bl;
+  const int btype = z.btype;
+  int removed = 0;
+  if (z.orientation.x != 0) {
+    const double x_b = z.x0;
+    const int orientation = (int)z.orientation.x;
+    for (int p = 0; p < s.np; p++) {
+      if (s.type[p] == T_FLUID) {
+        if (orientation > 0 ? s.r[p].x > x_b : s.r[p].x < x_b) {
+          s.type[p] = btype;
+          s.a[p] = V2{0., 0};
+          s.drho[p] = 0.;
+        }
+      } else if (s.type[p] == btype) {
+        const bool out = orientation > 0 ? (s.r[p].x > x_b + b_len || s.r[p].y < dp / 2)
+                                         : (s.r[p].x < x_b - b_len || s.r[p].y < dp / 2);
+        if (out) {
+          erase_particle(s, p);
+          ++removed;
+        }
+      }
+    }
+  }
+  return removed;
+}
+
 }  // namespace
 
 extern "C" {
@@ -946,6 +1093,26 @@ double orc_minmod_phi(double aC, double aD, double gx, double gy, double rx, dou
   return minmod_phi(aC, aD, V2{gx, gy}, V2{rx, ry});
 }
 void orc_rale_forces(void* vc) { rale_forces(*static_cast<Case*>(vc)); }
+// BufferZone(x0, xm, y0, ym, nbl, v0, orientation, btype), src/SPH_inlet_outlet_data.h:32-44
+int orc_add_zone(void* vc, double x0, double xm, double y0, double ym, double vx, double vy,
+                 double ox, double oy, int btype) {
+  Case& s = *static_cast<Case*>(vc);
+  Zone z;
+  z.x0 = x0; z.xm = xm; z.y0 = y0; z.ym = ym; z.v_b = V2{vx, vy}; z.orientation = V2{ox, oy};
+  z.btype = btype;
+  z.bl = std::fabs(xm - x0);
+  z.bh = std::fabs(ym - y0);
+  if (z.orientation.x != 0) { z.bl = std::fabs(xm - x0); z.bh = std::fabs(ym - y0); }
+  else if (z.orientation.y != 0) { z.bl = std::fabs(ym - y0); z.bh = std::fabs(xm - x0); }
+  s.zones.push_back(z);
+  return (int)s.zones.size() - 1;
+}
+int orc_bz_integrate_inlet(void* vc, int zone, double dt) {
+  return bz_integrate_inlet(*static_cast<Case*>(vc), zone, dt);
+}
+int orc_bz_integrate_outlet(void* vc, int zone, double dt) {
+  return bz_integrate_outlet(*static_cast<Case*>(vc), zone, dt);
+}
 void orc_rale_leapfrog_one(void* vc, double dt) { rale_leapfrog_one(*static_cast<Case*>(vc), dt); }
 void orc_rale_leapfrog_two(void* vc, double dt, int step) {
   rale_leapfrog_two(*static_cast<Case*>(vc), dt, step);

@@ -314,3 +314,132 @@ def test_removal_bit_exact():
     assert R.np == O.np
     assert np.array_equal(R.get(po.F_R), O.get(po.F_R))
     assert np.array_equal(R.types(), O.types())
+
+
+# ---------------------------------------------------------------------------
+# Buffer zones (src/SPH_buffer_zones.h), reference build REF_WCSPH
+# ---------------------------------------------------------------------------
+INLET1, OUTLET1 = 7, 12  # cases/inflowSlab_mDBC/geometry.h:15-17 (btype values)
+
+
+def copy_zones(R, O):
+    out = np.zeros(12)
+    for i in range(R.lib.ref_zone_count(R.h)):
+        R.lib.ref_zone_get(R.h, i, out.ctypes.data_as(po._dp))
+        x0, xm, y0, ym, _nbl, bl, bh, bt, vx, vy, ox, oy = out
+        k = O.lib.orc_add_zone(O.h, x0, xm, y0, ym, vx, vy, ox, oy, int(bt))
+        assert k == i
+
+
+def inflow_slab(dp=0.01):
+    """cases/inflowSlab_mDBC/geometry.h with the driver's constants
+    (cases/inflowSlab_mDBC/constants.h)"""
+    h, kap, cs = math.sqrt(2 * dp * dp), 2.1, 42.48
+    nvl = math.ceil(kap * h / dp)
+    q = [h, kap, cs, 1000., 0.1, 0.1, dp, -nvl * dp, 2. + nvl * dp, -nvl * dp, 1. + nvl * dp]
+    R = po.Case(po.ref_lib("wcsph"), "ref_", q)
+    R.call("create_wall_x_mdbc", 0.0, 0.4 - 3 * dp, 0.1, -1, 0.1 + dp / 2)
+    R.call("create_wall_y_mdbc", 0.4, 0 + dp, 0.1 - 3 * dp, -1, 0.4 + dp / 2)
+    R.call("create_wall_x_mdbc", 0.4 + dp, 1.2, 0., -1, 0 + dp / 2)
+    R.call("create_corner_mdbc", -1, -1, 0.4 + dp / 2, 0. + dp / 2)
+    R.call("create_corner_mdbc", -1, -1, 0.4 + dp / 2, 0.1 + dp / 2)
+    R.call("draw_create_inlet_y", 0.1, 0.1 + dp, 0.2, -1, 4, 0.75, 0., INLET1)
+    R.call("draw_create_outlet_y", 1.0, 0.0 + dp, 0.5, +1, 4, 0.5, 0., OUTLET1, 1)
+    return R, q, 0.00005 * dp / 0.005
+
+
+def inflow_step(C, dt, oracle):
+    """cases/inflowSlab_mDBC/SPH_simulationSymplectic.h:57-123"""
+    C.call("wall_density" if not oracle else "mdbc_density")
+    C.call("density_to_pressure")
+    C.call("predictor", dt)
+    C.call("bz_integrate_inlet", 0, dt)
+    C.call("bz_integrate_outlet", 1, dt)
+    C.call("clear_cells"); C.call("particle_to_cell"); C.call("density_to_pressure")
+    C.call("compute_forces", 0) if oracle else C.call("compute_forces")
+    C.call("corrector", dt)
+    C.call("remove_out_of_domain")
+    C.call("clear_cells"); C.call("particle_to_cell"); C.call("density_to_pressure")
+    C.call("compute_forces", 0) if oracle else C.call("compute_forces")
+
+
+def test_inflow_slab_steps_bit_exact():
+    """Inlet conversions and appends (in loop order) inside whole driver steps."""
+    R, q, dt = inflow_slab()
+    O = oracle_from_ref(R, q, po.F_GNR)
+    copy_zones(R, O)
+    n0 = R.np
+    for c in (R, O):
+        c.call("divide_to_cells")
+        c.call("particle_to_cell")
+    for _ in range(12):
+        inflow_step(R, dt, False)
+        inflow_step(O, dt, True)
+    assert R.np > n0  # the inlet appended particles
+    assert_same(R, O, FIELDS)
+    assert np.array_equal(R.index(), O.index())
+
+
+def outlet_scene(orientation):
+    """Zone particles in runs of 1, 2 and 3 removal candidates, each followed by
+    a fluid particle beyond the zone start (converted only if examined), plus
+    candidates below dp/2 and kept zone particles."""
+    dp = 0.01
+    q = [dp * 1.4, 2.1, 40.0, 1000., 0.1, 0.1, dp, -0.5, 2.5, -0.5, 1.5]
+    R = po.Case(po.ref_lib("wcsph"), "ref_", q)
+    R.call("draw_create_outlet_y", 1.0, dp, 0.5, orientation, 4, 0.5, 0., OUTLET1, 1)
+    s = orientation
+    beyond = 1.0 + s * 0.1          # past x_b + b_len (b_len = 0.04)
+    inside = 1.0 + s * 0.02         # inside the zone
+    fluid_past = 1.0 + s * 0.01     # fluid beyond x_b: converts when examined
+    seq = [
+        (OUTLET1, beyond, 0.2), (0, fluid_past, 0.2),                           # run of 1
+        (OUTLET1, beyond, 0.2), (OUTLET1, beyond, 0.25), (0, fluid_past, 0.25),  # run of 2
+        (OUTLET1, beyond, 0.3), (OUTLET1, inside, 0.001), (OUTLET1, beyond, 0.35),
+        (0, fluid_past, 0.3),                                                   # run of 3
+        (OUTLET1, inside, 0.2), (0, fluid_past, 0.35), (0, 1.0 - s * 0.2, 0.2),
+        (OUTLET1, inside, 0.004), (0, fluid_past, 0.4),
+    ]
+    for t, x, y in seq:
+        R.call("add", t, 0.0, 1000.0, x, y, 0.3 * s, 0.0)
+    return R, q
+
+
+@pytest.mark.parametrize("orientation", [+1, -1])
+def test_outlet_in_loop_erase_bit_exact(orientation):
+    R, q = outlet_scene(orientation)
+    O = oracle_from_ref(R, q)
+    copy_zones(R, O)
+    R.call("bz_integrate_outlet", 0, 1e-4)
+    O.call("bz_integrate_outlet", 0, 1e-4)
+    assert R.np < 14 and R.np == O.np
+    assert_same(R, O, FIELDS)
+    assert np.array_equal(R.index(), O.index())
+
+
+@pytest.mark.parametrize("axis,orientation", [("y", -1), ("y", +1), ("x", -1), ("x", +1)])
+def test_inlet_orientations_bit_exact(axis, orientation):
+    """All four inlet branches (three of them advance the zone particles inside
+    the loop), with particles appended during the pass visited by it."""
+    dp = 0.01
+    q = [dp * 1.4, 2.1, 40.0, 1000., 0.1, 0.1, dp, -0.5, 2.5, -0.5, 1.5]
+    R = po.Case(po.ref_lib("wcsph"), "ref_", q)
+    if axis == "y":   # zone normal along x (DRAW_CreateInletY)
+        R.call("draw_create_inlet_y", 1.0, 0.2, 0.3, orientation, 4, -0.9 * orientation, 0., INLET1)
+    else:             # zone normal along y (DRAW_CreateInletX)
+        R.call("draw_create_inlet_x", 1.0, 0.2, 0.3, orientation, 4, 0., -0.9 * orientation, INLET1)
+    if (axis, orientation) == ("y", -1):
+        # this branch does not advance the zone particles: move them across
+        r = R.get(po.F_R)
+        r[:, 0] += 0.015
+        R.set(po.F_R, r)
+        R.set(po.F_RO, r)
+    O = oracle_from_ref(R, q)
+    copy_zones(R, O)
+    n0 = R.np
+    for dt in (0.004, 0.008, 0.02):
+        R.call("bz_integrate_inlet", 0, dt)
+        O.call("bz_integrate_inlet", 0, dt)
+        assert_same(R, O, FIELDS)
+        assert np.array_equal(R.index(), O.index())
+    assert R.np > n0 and (R.types() == 0).sum() == R.np - n0
