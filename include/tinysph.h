@@ -89,6 +89,9 @@ typedef struct tsph_counters {
   uint64_t density_out_of_range; /* rho > 1500 or < 0 seen by the EOS (src/SPH_pressure.h:76-80) */
   uint64_t outside_grid; /* particles that fell outside the cell grid */
   uint64_t removed;      /* particles removed out of the domain (np_out) */
+  uint64_t appended;     /* particles appended by inlet buffer zones */
+  uint64_t zone_rejected;/* inlet appends refused: capacity or the per-pass list limit */
+  uint64_t zone_chained; /* appended particles already past the zone start (not re-examined) */
 } tsph_counters;
 
 typedef struct tsph_ctx tsph_ctx;
@@ -218,6 +221,36 @@ int tsph_ale_leapfrog_two(tsph_ctx *ctx, double dt, int step);
 /* nsteps whole ALE-SPH steps numbered from `step`; muscl = TSPH_MUSCL_* */
 int tsph_step_ale(tsph_ctx *ctx, double dt, int step, int nsteps, int muscl);
 
+/* ---- buffer zones (src/SPH_buffer_zones.h, src/SPH_inlet_outlet_data.h) ---- */
+/* BufferZone, src/SPH_inlet_outlet_data.h:18-46: the region as DRAW_Create*
+ * passes it (src/SPH_create.h:439-542), the buffer velocity, the orientation
+ * {+-1, 0} (zone normal along x) or {0, +-1} (along y), and the particle type
+ * of the zone's particles. */
+typedef struct tsph_zone {
+  double x0, xm, y0, ym;
+  double vx, vy;
+  int ox, oy;
+  int btype;
+} tsph_zone;
+#define TSPH_MAX_ZONES 8
+/* particles.zones.push_back(BufferZone(...)); *index = the zone's index */
+int tsph_bz_add_zone(tsph_ctx *ctx, const tsph_zone *zone, int *index);
+/* BZ_Integrate_inlet, src/SPH_buffer_zones.h:10-107: zone particles past the
+ * zone start become fluid and a new zone particle is appended one buffer length
+ * behind each, in particle order (at most 4096 per call); the context capacity
+ * must leave room for them.  Takes effect for pair loops at the next
+ * tsph_particle_to_cell. */
+int tsph_bz_integrate_inlet(tsph_ctx *ctx, int zone, double dt);
+/* BZ_Integrate_outlet, src/SPH_buffer_zones.h:111-206: fluid past the zone start
+ * becomes the zone type (a = 0, drho = 0); zone particles past the zone end or
+ * below dp/2 are erased, with the reference's in-loop skip of the next particle. */
+int tsph_bz_integrate_outlet(tsph_ctx *ctx, int zone, double dt);
+/* nsteps whole steps of the inflow driver
+ * (cases/inflowSlab_mDBC/SPH_simulationSymplectic.h:57-123): the symplectic step
+ * with BZ_Integrate_inlet / _outlet after the predictor (zone -1: none). */
+int tsph_step_inflow(tsph_ctx *ctx, int wall, double dt, int nsteps, int inlet_zone,
+                     int outlet_zone);
+
 /* ---- case setup (host only; no GPU needed) ----
  * Presets built with the reference's geometry routines (src/SPH_create.h,
  * src/SPH_create_bt.h) in the order of the reference case geometry files. */
@@ -229,6 +262,13 @@ int tsph_step_ale(tsph_ctx *ctx, double dt, int step, int nsteps, int muscl);
 #define TSPH_CASE_ALE_DAM_BREAK 4   /* cases/ALESPH_dambreakLong/geometry.h (BT walls, ALE
                                        particles) in the 3.22x1.8 m tank with the 1.2x0.6 m
                                        column, dx = 2 mm */
+#define TSPH_CASE_INFLOW_SLAB 5     /* cases/inflowSlab_mDBC/geometry.h + constants.h: mDBC
+                                       step, inlet (zone 0, type 7) and empty outlet (zone 1,
+                                       type 12), dx = 5 mm */
+#define TSPH_CASE_INFLOW_CHANNEL 6  /* the inflow driver on an 8 x 0.5 m channel: 0.125 m of
+                                       water moving at 1 m/s over an mDBC floor, inlet at
+                                       x = 0 (zone 0), outlet at x = 7.9 (zone 1), dx = 0.5 mm
+                                       (about 4M particles) */
 
 /* time-stepping scheme of a preset */
 #define TSPH_SCHEME_SYMPLECTIC 0 /* WCSPH, tsph_step_symplectic */
@@ -242,6 +282,8 @@ typedef struct tsph_case {
   int scheme;                /* TSPH_SCHEME_* */
   int n_fluid, n_wall;
   tsph_particles particles;  /* arrays owned by the case */
+  int n_zones;               /* buffer zones of the preset, in index order */
+  tsph_zone zones[4];
 } tsph_case;
 
 /* dx <= 0 selects the preset's spacing; jitter is the uniform position

@@ -109,8 +109,26 @@ struct Simulation_data {
 };
 
 // ---- src/SPH_particle_system.h:7-213 ----
+// BufferZone, src/SPH_inlet_outlet_data.h:18-46 (the fields the buffer-zone
+// stages read)
+struct BufferZone {
+  realvec orientation;
+  real x0, y0, xm, ym;
+  real bl, bh;
+  unsigned int btype;
+  realvec v_b;
+  BufferZone(real _x0, real _xm, real _y0, real _ym, unsigned int, realvec _v0, realvec _orientation,
+             unsigned int _btype)
+      : orientation(_orientation), x0(_x0), y0(_y0), xm(_xm), ym(_ym), btype(_btype), v_b(_v0) {
+    bl = std::fabs(xm - x0);
+    bh = std::fabs(ym - y0);
+    if (orientation.y != 0 && orientation.x == 0) { bl = std::fabs(ym - y0); bh = std::fabs(xm - x0); }
+  }
+};
+
 struct Particle_system {
   unsigned int np;
+  std::vector<BufferZone> zones;  // src/SPH_particle_system.h:22
   SPH_data data;
   SPH_constants data_const;
   SPH_special special;
@@ -123,6 +141,7 @@ struct Particle_system {
   int device_capacity = 0;  // 0: the particle count at Divide_To_Cells
   bool host_dirty = true;   // host vectors newer than the device
   bool dev_newer = false;   // device newer than the host vectors
+  size_t zones_on_device = 0;
 
   void Add_particle(short unsigned int _type, real _p, real _rho, realvec _r, realvec _v, realvec _a) {
     np++;
@@ -197,8 +216,19 @@ inline std::vector<double> tsph_expand_extra(const Particle_system &ps, const st
   return out;
 }
 
+inline void tsph_push_zones(Particle_system &ps) {
+  for (; ps.zones_on_device < ps.zones.size(); ++ps.zones_on_device) {
+    const BufferZone &b = ps.zones[ps.zones_on_device];
+    tsph_zone z{};
+    z.x0 = b.x0; z.xm = b.xm; z.y0 = b.y0; z.ym = b.ym; z.vx = b.v_b.x; z.vy = b.v_b.y;
+    z.ox = (int)b.orientation.x; z.oy = (int)b.orientation.y; z.btype = (int)b.btype;
+    tsph_check(tsph_bz_add_zone(ps.dev, &z, nullptr), "tsph_bz_add_zone");
+  }
+}
+
 inline void tsph_push(Particle_system &ps) {
   if (!ps.dev) throw tsph_error("Divide_To_Cells must be called before the first stage");
+  tsph_push_zones(ps);
   if (!ps.host_dirty) return;
   const int n = (int)ps.np;
   std::vector<int32_t> type(n), index(n);
@@ -309,6 +339,7 @@ inline void Divide_To_Cells(Particle_system &ps, Simulation_data sd) {
   prm.capacity = ps.device_capacity > 0 ? ps.device_capacity : (int)(ps.np > 0 ? ps.np : 1);
   if (ps.dev) tsph_destroy(ps.dev);
   ps.dev = nullptr;
+  ps.zones_on_device = 0;
   tsph_check(tsph_create(&prm, &ps.dev), "tsph_create");
   int ncx = 0, ncy = 0;
   tsph_check(tsph_grid(ps.dev, &ncx, &ncy), "tsph_grid");
@@ -416,6 +447,19 @@ inline real Pressure_in_point(Particle_system &ps, Simulation_data, realvec ar) 
   double out[4];
   tsph_check(tsph_interpolate_point(ps.dev, ar.x, ar.y, out), "Pressure_in_point");
   return out[1];
+}
+
+// src/SPH_buffer_zones.h:10-107
+inline void BZ_Integrate_inlet(Particle_system &ps, unsigned int zoneIndex, double dt) {
+  tsph_push(ps);
+  tsph_check(tsph_bz_integrate_inlet(ps.dev, (int)zoneIndex, dt), "BZ_Integrate_inlet");
+  tsph_ran(ps);
+}
+// src/SPH_buffer_zones.h:111-206
+inline void BZ_Integrate_outlet(Particle_system &ps, unsigned int zoneIndex, double dt) {
+  tsph_push(ps);
+  tsph_check(tsph_bz_integrate_outlet(ps.dev, (int)zoneIndex, dt), "BZ_Integrate_outlet");
+  tsph_ran(ps);
 }
 
 // ---------------------------------------------------------------------------
@@ -570,4 +614,63 @@ inline void Create_corner_bt(Particle_system &ps, int orientationX, int orientat
       ps.special.n.push_back(n);
     }
   }
+}
+
+// src/SPH_create.h:439-470
+inline void DRAW_CreateInletY(Particle_system &ps, double x, double y_0, double y_1, int orientation,
+                              int nbl, realvec v0, int type) {
+  const real dp = ps.data_const.dp;
+  const real len = LEN(x, y_0, x, y_1);
+  const unsigned int npart = std::ceil(len / dp) + 1;
+  realvec r, v, a;
+  for (int l = 0; l < nbl; l++) {
+    r.x = x + dp * l * orientation;
+    for (unsigned h = 0; h < npart; h++) {
+      r.y = y_0 + h * dp;
+      v.x = v0.x; v.y = v0.y;
+      a.x = 0; a.y = 0;
+      ps.Add_particle(type, 0, ps.data_const.rho0, r, v, a);
+    }
+  }
+  ps.zones.push_back(BufferZone(x, x + nbl * dp * orientation, y_0, y_1, nbl, v0, {(real)orientation, 0}, type));
+}
+
+// src/SPH_create.h:472-503
+inline void DRAW_CreateInletX(Particle_system &ps, double y, double x_0, double x_1, int orientation,
+                              int nbl, realvec v0, int type) {
+  const real dp = ps.data_const.dp;
+  const real len = LEN(x_0, y, x_1, y);
+  const unsigned int npart = std::ceil(len / dp) + 1;
+  realvec r, v, a;
+  for (int l = 0; l < nbl; l++) {
+    r.y = y + dp * l * orientation;
+    for (unsigned h = 0; h < npart; h++) {
+      r.x = x_0 + h * dp;
+      v.x = v0.x; v.y = v0.y;
+      a.x = 0; a.y = 0;
+      ps.Add_particle(type, 0, ps.data_const.rho0, r, v, a);
+    }
+  }
+  ps.zones.push_back(BufferZone(x_0, x_1, y, y + nbl * dp * orientation, nbl, v0, {0, (real)orientation}, type));
+}
+
+// src/SPH_create.h:508-542 (a non-empty outlet is filled with particles of type inlet)
+inline void DRAW_CreateOutletY(Particle_system &ps, double x, double y_0, double y_1, int orientation,
+                               int nbl, realvec v0, int type, bool empty) {
+  const real dp = ps.data_const.dp;
+  if (empty == false) {
+    const real len = LEN(x, y_0, x, y_1);
+    const unsigned int npart = std::ceil(len / dp) + 1;
+    realvec r, v, a;
+    for (int l = 0; l < nbl; l++) {
+      r.x = x + dp * l * orientation;
+      for (unsigned h = 0; h < npart; h++) {
+        r.y = y_0 + h * dp;
+        v.x = v0.x; v.y = v0.y;
+        a.x = 0; a.y = 0;
+        ps.Add_particle(inlet, 0, ps.data_const.rho0, r, v, a);
+      }
+    }
+  }
+  ps.zones.push_back(BufferZone(x, x + nbl * dp * orientation, y_0, y_1, nbl, v0, {(real)orientation, 0}, type));
 }

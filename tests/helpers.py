@@ -81,3 +81,37 @@ def brute_force_pairs(r: np.ndarray, active: np.ndarray, h: float) -> int:
         m[i] = False
         total += int(m.sum())
     return total
+
+
+def oracle_add_zones(O, zones) -> None:
+    """Register buffer zones (dicts as CaseData.zones) with an oracle case."""
+    for i, z in enumerate(zones):
+        k = O.lib.orc_add_zone(O.h, z["x0"], z["xm"], z["y0"], z["ym"], z["vx"], z["vy"],
+                               float(z["ox"]), float(z["oy"]), int(z["btype"]))
+        assert k == i
+
+
+def oracle_inflow_step(O, wall: int, dt: float, inlet: int = 0, outlet: int = 1) -> None:
+    """One step of the inflow driver (cases/inflowSlab_mDBC/SPH_simulationSymplectic.h:57-123)."""
+    var = oracle_variant(wall)
+    if wall == WALL_MDBC:
+        O.call("mdbc_density")
+        O.call("density_to_pressure")
+    elif wall == WALL_BT:
+        O.call("bt_density")
+        O.call("density_to_pressure")
+    O.call("predictor", dt)
+    if inlet >= 0:
+        O.call("bz_integrate_inlet", inlet, dt)
+    if outlet >= 0:
+        O.call("bz_integrate_outlet", outlet, dt)
+    O.call("clear_cells")
+    assert O.call("particle_to_cell") == 0
+    O.call("density_to_pressure")
+    O.call("compute_forces", var)
+    O.call("corrector", dt)
+    O.call("remove_out_of_domain")
+    O.call("clear_cells")
+    assert O.call("particle_to_cell") == 0
+    O.call("density_to_pressure")
+    O.call("compute_forces", var)

@@ -20,6 +20,8 @@ PREC_FP64, PREC_MIXED = 0, 1
 FORCES_WCSPH, FORCES_DBC, FORCES_BT = 0, 1, 2
 WALL_DBC, WALL_MDBC, WALL_BT = 0, 1, 2
 CASE_STILL_WATER_DBC, CASE_DAM_BREAK_MDBC, CASE_DAM_BREAK_BT, CASE_ALE_DAM_BREAK = 1, 2, 3, 4
+CASE_INFLOW_SLAB, CASE_INFLOW_CHANNEL = 5, 6
+INLET1, OUTLET1 = 7, 12
 SCHEME_SYMPLECTIC, SCHEME_RALE = 0, 1
 
 _dp = C.POINTER(C.c_double)
@@ -40,13 +42,19 @@ class Particles(C.Structure):
 
 class Counters(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in ("pairs", "candidates", "active", "pair_passes",
-                                           "density_out_of_range", "outside_grid", "removed")]
+                                           "density_out_of_range", "outside_grid", "removed",
+                                           "appended", "zone_rejected", "zone_chained")]
+
+
+class Zone(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("x0", "xm", "y0", "ym", "vx", "vy")] + [
+        ("ox", C.c_int), ("oy", C.c_int), ("btype", C.c_int)]
 
 
 class CaseC(C.Structure):
     _fields_ = [("prm", Params), ("dt", C.c_double), ("wall", C.c_int), ("steps", C.c_int),
                 ("scheme", C.c_int), ("n_fluid", C.c_int), ("n_wall", C.c_int),
-                ("particles", Particles)]
+                ("particles", Particles), ("n_zones", C.c_int), ("zones", Zone * 4)]
 
 
 ALE_NAMES = ("omega", "omegarho", "omegav", "omega_o", "omegarho_o", "omegav_o", "domega",
@@ -105,6 +113,10 @@ EXPORTS = {
     "tsph_ale_leapfrog_one": (C.c_int, [C.c_void_p, C.c_double]),
     "tsph_ale_leapfrog_two": (C.c_int, [C.c_void_p, C.c_double, C.c_int]),
     "tsph_step_ale": (C.c_int, [C.c_void_p, C.c_double, C.c_int, C.c_int, C.c_int]),
+    "tsph_bz_add_zone": (C.c_int, [C.c_void_p, C.POINTER(Zone), C.POINTER(C.c_int)]),
+    "tsph_bz_integrate_inlet": (C.c_int, [C.c_void_p, C.c_int, C.c_double]),
+    "tsph_bz_integrate_outlet": (C.c_int, [C.c_void_p, C.c_int, C.c_double]),
+    "tsph_step_inflow": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int]),
     "tsph_case_build": (C.c_int, [C.c_int, C.c_double, C.c_double, C.c_uint64,
                                   C.POINTER(C.POINTER(CaseC))]),
     "tsph_case_free": (C.c_int, [C.POINTER(CaseC)]),

@@ -60,7 +60,7 @@ static void free_soa(Soa &s) {
 
 static void drop_graph(tsph_ctx *c) {
   for (Graph *g : {&c->graph[0], &c->graph[1], &c->rale_graph[0], &c->rale_graph[1],
-                   &c->ale_graph[0], &c->ale_graph[1]}) {
+                   &c->ale_graph[0], &c->ale_graph[1], &c->inflow_graph[0], &c->inflow_graph[1]}) {
     if (g->exec) (void)hipGraphExecDestroy(g->exec);
     *g = Graph{};
   }
@@ -180,8 +180,10 @@ static int record_ale_step(tsph_ctx *c, double dt, int step, int muscl) {
   return 0;
 }
 
-// one symplectic step in the drivers' stage order
-static int record_step(tsph_ctx *c, int wall, double dt) {
+// one symplectic step in the drivers' stage order; inlet / outlet >= 0 add the
+// buffer zones of the inflow driver after the predictor
+// (cases/inflowSlab_mDBC/SPH_simulationSymplectic.h:71-76)
+static int record_step(tsph_ctx *c, int wall, double dt, int inlet = -1, int outlet = -1) {
   const int variant = wall == TSPH_WALL_DBC    ? TSPH_FORCES_DBC
                       : wall == TSPH_WALL_MDBC ? TSPH_FORCES_WCSPH
                                                : TSPH_FORCES_BT;
@@ -194,6 +196,8 @@ static int record_step(tsph_ctx *c, int wall, double dt) {
     if ((e = launch_bt_density(c)) || (e = launch_eos(c))) return e;
   }
   if ((e = launch_predictor(c, dt))) return e;
+  if (inlet >= 0 && (e = launch_bz_inlet(c, inlet, dt))) return e;
+  if (outlet >= 0 && (e = launch_bz_outlet(c, outlet, dt))) return e;
   if ((e = launch_cell_list(c)) || (e = launch_eos(c)) || (e = launch_forces(c, variant))) return e;
   if ((e = launch_corrector(c, dt)) || (e = launch_remove(c))) return e;
   if ((e = launch_cell_list(c)) || (e = launch_eos(c)) || (e = launch_forces(c, variant))) return e;
@@ -247,19 +251,21 @@ int tsph_create(const tsph_params *prm, tsph_ctx **out) {
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
     return bail(fail(TSPH_EHIP, "hipStreamCreate failed"));
   const int cap = c->capacity;
+  c->seq_cap = 2 * cap + kBzMax;
   if ((e = alloc_soa(c->cur, cap, false, false)) || (e = alloc_soa(c->alt, cap, false, false)))
     return bail(e);
   if ((e = dalloc(&c->scal, S_NSCAL)) || (e = dalloc(&c->counters, C_NCOUNT)) ||
       (e = dalloc(&c->cell_count, k.ncells + 1)) || (e = dalloc(&c->cell_start, k.ncells + 2)) ||
       (e = dalloc(&c->cellof, cap)) || (e = dalloc(&c->hrank, cap)) || (e = dalloc(&c->hcell, cap)) ||
       (e = dalloc(&c->perm, cap)) || (e = dalloc(&c->scan_tmp, (k.ncells + 1) / 1024 + 2)) ||
-      (e = dalloc(&c->seqstate, cap)) || (e = dalloc(&c->probe_out, 4)) ||
+      (e = dalloc(&c->seqstate, (size_t)2 * cap + kBzMax)) || (e = dalloc(&c->probe_out, 4)) ||
+      (e = dalloc(&c->bz_list, 2 * kBzMax)) ||
       (e = dalloc(&c->tile_list, k.ncells + 8)) || (e = dalloc(&c->wall_list, cap)))
     return bail(e);
   // the per-cell counters and removal states start cleared; later builds clear
   // them as they consume them
   if (hipMemsetAsync(c->cell_count, 0, sizeof(int) * (k.ncells + 1), c->stream) != hipSuccess ||
-      hipMemsetAsync(c->seqstate, 0, cap, c->stream) != hipSuccess)
+      hipMemsetAsync(c->seqstate, 0, (size_t)2 * cap + kBzMax, c->stream) != hipSuccess)
     return bail(fail(TSPH_EHIP, "initialisation failed"));
   {
     int dev = 0;
@@ -287,7 +293,7 @@ int tsph_destroy(tsph_ctx *c) {
   free_soa(c->alt);
   void *ps[] = {c->scal, c->counters, c->cell_count, c->cell_start, c->cellof, c->hrank,
                 c->hcell, c->perm, c->scan_tmp, c->seqstate, c->probe_out, c->mx.rv,
-                c->mx.rp, c->mx.nrm, c->tile_list, c->wall_list};
+                c->mx.rp, c->mx.nrm, c->tile_list, c->wall_list, c->bz_list};
   for (void *p : ps)
     if (p) (void)hipFree(p);
   for (hipEvent_t ev : c->tev) (void)hipEventDestroy(ev);
@@ -354,8 +360,11 @@ int tsph_upload(tsph_ctx *c, const tsph_particles *h) {
     if (h->n) TSPH_HIP(cp(d.n, h->n, s2));
     else if (c->has_n) TSPH_HIP(cp(d.n, zeros.data(), s2));
   }
-  int sc[S_NSCAL] = {n, n, 0, 0};
+  int sc[S_NSCAL] = {};
+  sc[S_NP] = n;
+  sc[S_MAXSEQ] = n;
   TSPH_HIP(hipMemcpy(c->scal, sc, sizeof sc, hipMemcpyHostToDevice));
+  TSPH_HIP(hipMemset(c->seqstate, 0, (size_t)c->seq_cap));
   c->n_ub = n;
   c->n_wall = 0;
   for (int i = 0; i < n; ++i) c->n_wall += t[i] == TSPH_WALL;
@@ -694,6 +703,57 @@ int tsph_step_ale(tsph_ctx *c, double dt, int step, int nsteps, int muscl) {
                      [&] { return record_ale_step(c, dt, 2, muscl); });
 }
 
+int tsph_bz_add_zone(tsph_ctx *c, const tsph_zone *z, int *index) {
+  if (!c || !z) return fail(TSPH_EINVAL, "null argument");
+  if (c->n_zones >= TSPH_MAX_ZONES) return fail(TSPH_ECAPACITY, "too many buffer zones");
+  if ((z->ox != 0) == (z->oy != 0)) return fail(TSPH_EINVAL, "zone orientation must be along x or y");
+  Zone &q = c->zones[c->n_zones];
+  q.x0 = z->x0; q.xm = z->xm; q.y0 = z->y0; q.ym = z->ym; q.vx = z->vx; q.vy = z->vy;
+  q.ox = z->ox; q.oy = z->oy; q.btype = z->btype;
+  // src/SPH_inlet_outlet_data.h:37-42
+  if (q.ox != 0) { q.bl = std::fabs(q.xm - q.x0); q.bh = std::fabs(q.ym - q.y0); }
+  else { q.bl = std::fabs(q.ym - q.y0); q.bh = std::fabs(q.xm - q.x0); }
+  if (index) *index = c->n_zones;
+  c->n_zones++;
+  drop_graph(c);
+  return 0;
+}
+
+static int zone_ok(tsph_ctx *c, int zone) {
+  if (!c) return fail(TSPH_EINVAL, "null argument");
+  if (zone < 0 || zone >= c->n_zones) return fail(TSPH_EINVAL, "no such buffer zone");
+  return 0;
+}
+
+int tsph_bz_integrate_inlet(tsph_ctx *c, int zone, double dt) {
+  if (int e = zone_ok(c, zone)) return e;
+  return launch_bz_inlet(c, zone, dt);
+}
+
+int tsph_bz_integrate_outlet(tsph_ctx *c, int zone, double dt) {
+  if (int e = zone_ok(c, zone)) return e;
+  return launch_bz_outlet(c, zone, dt);
+}
+
+int tsph_step_inflow(tsph_ctx *c, int wall, double dt, int nsteps, int inlet, int outlet) {
+  if (!c) return fail(TSPH_EINVAL, "null argument");
+  if (wall < TSPH_WALL_DBC || wall > TSPH_WALL_BT || nsteps < 0)
+    return fail(TSPH_EINVAL, "invalid wall treatment or step count");
+  if (inlet >= 0 && (inlet >= c->n_zones)) return fail(TSPH_EINVAL, "no such inlet zone");
+  if (outlet >= 0 && (outlet >= c->n_zones)) return fail(TSPH_EINVAL, "no such outlet zone");
+  if (!c->cells_valid) return fail(TSPH_ESTATE, "no cell list: call tsph_particle_to_cell first");
+  if (wall == TSPH_WALL_MDBC && !c->has_gnr) return fail(TSPH_ESTATE, "mDBC needs ghost nodes (gnr)");
+  if (wall == TSPH_WALL_BT && !c->has_n) return fail(TSPH_ESTATE, "BT needs wall normals (n)");
+  if (inlet >= 0) c->n_ub = c->capacity;
+  if (c->time_pairs) {
+    for (int s = 0; s < nsteps; ++s)
+      if (int e = record_step(c, wall, dt, inlet, outlet)) return e;
+    return 0;
+  }
+  return replay_step(c, c->inflow_graph, 300 + 16 * wall + 4 * (inlet + 1) + (outlet + 1), dt,
+                     nsteps, [&] { return record_step(c, wall, dt, inlet, outlet); });
+}
+
 int tsph_water_elevation(tsph_ctx *c, double x, double y0, int inverse, double *level) {
   if (!c || !level) return fail(TSPH_EINVAL, "null argument");
   if (!c->cells_valid) return fail(TSPH_ESTATE, "no cell list: call tsph_particle_to_cell first");
@@ -718,6 +778,9 @@ int tsph_get_counters(tsph_ctx *c, tsph_counters *out) {
   out->density_out_of_range = v[C_BADRHO];
   out->outside_grid = v[C_OUTGRID];
   out->removed = v[C_REMOVED];
+  out->appended = v[C_APPENDED];
+  out->zone_rejected = v[C_BZREJECT];
+  out->zone_chained = v[C_BZCHAIN];
   return 0;
 }
 

@@ -5,6 +5,7 @@
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
+#include <algorithm>
 #include <memory>
 
 #include "../../include/tinysph_dropin.hpp"
@@ -67,7 +68,9 @@ int export_case(const Spec &s, Particle_system &ps, tsph_case **out) {
   c->prm.avisc = s.visco; c->prm.delta = s.delta; c->prm.dp = s.dp;
   c->prm.x_0 = sd.x_0; c->prm.x_m = sd.x_m; c->prm.y_0 = sd.y_0; c->prm.y_m = sd.y_m;
   c->prm.precision = TSPH_PREC_FP64;
-  c->prm.capacity = (int)ps.np;
+  // room for the particles an inlet appends: one buffer layer per pass is
+  // the most a stable time step converts
+  c->prm.capacity = (int)ps.np + (ps.zones.empty() ? 0 : std::max(4096, (int)ps.np / 4));
   c->dt = s.dt;
   c->wall = s.wall;
   c->steps = s.steps;
@@ -95,6 +98,12 @@ int export_case(const Spec &s, Particle_system &ps, tsph_case **out) {
   h.rho_o = dup(ps.data.rho_o);
   if (!ps.special.gnr.empty()) h.gnr = dup(tsph_expand_extra(ps, ps.special.gnr));
   if (!ps.special.n.empty()) h.n = dup(tsph_expand_extra(ps, ps.special.n));
+  for (const BufferZone &b : ps.zones) {
+    if (c->n_zones >= 4) break;
+    tsph_zone &z = c->zones[c->n_zones++];
+    z.x0 = b.x0; z.xm = b.xm; z.y0 = b.y0; z.ym = b.ym; z.vx = b.v_b.x; z.vy = b.v_b.y;
+    z.ox = (int)b.orientation.x; z.oy = (int)b.orientation.y; z.btype = (int)b.btype;
+  }
   *out = c;
   return 0;
 }
@@ -196,6 +205,52 @@ extern "C" int tsph_case_build(int preset, double dx, double jitter, uint64_t se
       Create_corner_bt(ps, -1, -1, {0., 0.}, {std::sqrt(2), std::sqrt(2)});
       Create_corner_bt(ps, +1, -1, {L, 0.}, {-std::sqrt(2), std::sqrt(2)});
       Create_Block_of_Fluid(ps, 0.0 + dp, 0 + dp, CW, CH);
+      jitter_fluid(ps, (jitter > 0 ? jitter : 0.0) * s.dp, seed);
+      return export_case(s, ps, out);
+    }
+    case TSPH_CASE_INFLOW_SLAB: {
+      // cases/inflowSlab_mDBC/geometry.h with cases/inflowSlab_mDBC/constants.h
+      // (dt scaled with dx from 5e-5 at dx = 5 mm); inlet zone 0, outlet zone 1
+      s.dp = dx > 0 ? dx : 0.005;
+      s.hh = 1.0 * std::sqrt(2 * s.dp * s.dp); s.kap = 2.1; s.cs = 42.48;
+      s.visco = 0.1; s.delta = 0.1; s.dt = 0.00005 * s.dp / 0.005;
+      s.x_0 = 0.; s.x_1 = 2.; s.y_0 = 0.; s.y_1 = 1.;
+      s.steps = 50000; s.wall = TSPH_WALL_MDBC;
+      Particle_system ps(s.hh, s.kap, s.cs, 1000., s.visco, s.delta, s.dp);
+      const double dp = s.dp;
+      Create_Wall_x_mdbc(ps, 0.0, 0.4 - 3 * dp, 0.1, -1, 0.1 + dp / 2);
+      Create_Wall_y_mdbc(ps, 0.4, 0 + dp, 0.1 - 3 * dp, -1, 0.4 + dp / 2);
+      Create_Wall_x_mdbc(ps, 0.4 + dp, 1.2, 0., -1, 0 + dp / 2);
+      Create_corner_mdbc(ps, -1, -1, {0.4 + dp / 2, 0. + dp / 2});
+      Create_corner_mdbc(ps, -1, -1, {0.4 + dp / 2, 0.1 + dp / 2});
+      DRAW_CreateInletY(ps, 0.1, 0.1 + dp, 0.2, -1, 4, {0.75, 0.}, inlet1);
+      DRAW_CreateOutletY(ps, 1.0, 0.0 + dp, 0.5, +1, 4, {0.5, 0.}, outlet1, true);
+      return export_case(s, ps, out);
+    }
+    case TSPH_CASE_INFLOW_CHANNEL: {
+      // the inflow driver on an 8 x 0.5 m channel: an mDBC floor, 0.125 m of
+      // water (hydrostatic density) moving at 1 m/s, a 4-layer inlet at x = 0
+      // feeding at 1 m/s (zone 0) and an empty outlet from x = 7.9 (zone 1);
+      // c0 = 10 sqrt(g H), dt = 0.2 h / c0
+      const double L = 8.0, H = 0.125, Hc = 0.5, xo = 7.9, U = 1.0;
+      s.dp = dx > 0 ? dx : 0.0005;
+      s.hh = 1.0 * std::sqrt(2 * s.dp * s.dp); s.kap = 2.1; s.cs = 10. * std::sqrt(g * H);
+      s.visco = 0.02; s.delta = 0.1; s.dt = 0.2 * s.hh / s.cs;
+      const double dp = s.dp;
+      const int nvl = std::ceil(s.kap * s.hh / s.dp);
+      s.x_0 = -(4 + nvl) * dp; s.x_1 = L + 0.1; s.y_0 = 0.; s.y_1 = Hc;
+      s.steps = 1000; s.wall = TSPH_WALL_MDBC;
+      Particle_system ps(s.hh, s.kap, s.cs, 1000., s.visco, s.delta, s.dp);
+      Create_Wall_x_mdbc(ps, -(4 + nvl) * dp, L, 0., -1, 0 + dp / 2);
+      DRAW_CreateInletY(ps, 0.0, 0.0 + dp, H, -1, 4, {U, 0.}, inlet1);
+      const unsigned first_fluid = ps.np;
+      Create_Block_of_FluidWP(ps, dp, dp, xo - dp, H);
+      for (unsigned i = first_fluid; i < ps.np; ++i) {
+        ps.data.v[i] = {U, 0.};
+        ps.data.v_o[i] = ps.data.v[i];
+        ps.data.v_oo[i] = ps.data.v[i];
+      }
+      DRAW_CreateOutletY(ps, xo, 0.0 + dp, Hc, +1, 4, {U, 0.}, outlet1, true);
       jitter_fluid(ps, (jitter > 0 ? jitter : 0.0) * s.dp, seed);
       return export_case(s, ps, out);
     }

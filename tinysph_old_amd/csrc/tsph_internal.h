@@ -24,12 +24,19 @@ namespace tsph {
 constexpr uint8_t kRemoved = 255;
 
 // device scalar slots
-enum { S_NP = 0, S_MAXSEQ = 1, S_NTILES = 2, S_NWALL = 3, S_NSCAL = 4 };
+enum { S_NP = 0, S_MAXSEQ = 1, S_NTILES = 2, S_NWALL = 3, S_BZCNT = 4, S_NSCAL = 5 };
 // device counter slots (uint64)
 enum {
   C_PAIRS = 0, C_CAND = 1, C_ACTIVE = 2, C_PASSES = 3, C_BADRHO = 4, C_OUTGRID = 5,
-  C_REMOVED = 6, C_NCOUNT = 8
+  C_REMOVED = 6, C_APPENDED = 7, C_BZREJECT = 8, C_BZCHAIN = 9, C_NCOUNT = 10
 };
+
+// buffer zone (BufferZone, src/SPH_inlet_outlet_data.h:18-46)
+struct Zone {
+  double x0, xm, y0, ym, bl, bh, vx, vy;
+  int ox, oy, btype;
+};
+constexpr int kBzMax = 4096;  // zone particles an inlet pass may convert
 
 struct Soa {
   double2 *r = nullptr, *v = nullptr, *a = nullptr, *r_o = nullptr, *v_o = nullptr;
@@ -124,6 +131,11 @@ struct tsph_ctx {
   tsph::Graph graph[2];       // symplectic step, by parity at the step's start
   tsph::Graph rale_graph[2];  // Riemann-ALE step, by parity at the step's start
   tsph::Graph ale_graph[2];   // ALE-SPH MUSCL step, by parity at the step's start
+  tsph::Graph inflow_graph[2];  // symplectic step with buffer zones
+  tsph::Zone zones[TSPH_MAX_ZONES];
+  int n_zones = 0;
+  int *bz_list = nullptr;     // 2 * kBzMax ints: {seq, slot} of the crossing zone particles
+  int seq_cap = 0;            // size of seqstate (reference positions ever handed out)
 };
 
 namespace tsph {
@@ -166,5 +178,8 @@ int launch_ale_gradient(tsph_ctx *c);
 int launch_ale_density_pressure(tsph_ctx *c, double dt);
 int launch_ale_leapfrog_one(tsph_ctx *c, double dt);
 int launch_ale_leapfrog_two(tsph_ctx *c, double dt, int step);
+// tsph_zones.hip
+int launch_bz_inlet(tsph_ctx *c, int zone, double dt);
+int launch_bz_outlet(tsph_ctx *c, int zone, double dt);
 
 }  // namespace tsph

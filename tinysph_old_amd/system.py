@@ -27,6 +27,7 @@ class CaseData:
     n_wall: int
     arrays: dict = field(default_factory=dict)
     scheme: int = 0
+    zones: list = field(default_factory=list)
 
     @property
     def np(self) -> int:
@@ -66,7 +67,8 @@ def build_case(preset: int, dx: float = 0.0, jitter: float = -1.0, seed: int = 1
             arrays["gnr"] = arr(h.gnr, 2 * n, np.float64).reshape(n, 2)
         if h.n:
             arrays["n"] = arr(h.n, 2 * n, np.float64).reshape(n, 2)
-        return CaseData(prm, c.dt, c.wall, c.steps, c.n_fluid, c.n_wall, arrays, c.scheme)
+        zones = [{n: getattr(c.zones[i], n) for n, _ in L.Zone._fields_} for i in range(c.n_zones)]
+        return CaseData(prm, c.dt, c.wall, c.steps, c.n_fluid, c.n_wall, arrays, c.scheme, zones)
     finally:
         lib.tsph_case_free(out)
 
@@ -280,6 +282,24 @@ class ParticleSystem:
     def step_rale(self, dt: float, step: int, nsteps: int = 1):
         L.check(self.lib.tsph_step_rale(self.ctx, dt, step, nsteps), "tsph_step_rale")
 
+    # ---- buffer zones (src/SPH_buffer_zones.h) ----
+    def bz_add_zone(self, zone: dict) -> int:
+        z = L.Zone()
+        for n, _ in L.Zone._fields_:
+            setattr(z, n, zone[n])
+        idx = C.c_int()
+        L.check(self.lib.tsph_bz_add_zone(self.ctx, C.byref(z), C.byref(idx)), "tsph_bz_add_zone")
+        return idx.value
+
+    def bz_integrate_inlet(self, zone: int, dt: float):
+        L.check(self.lib.tsph_bz_integrate_inlet(self.ctx, zone, dt), "BZ_Integrate_inlet")
+
+    def bz_integrate_outlet(self, zone: int, dt: float):
+        L.check(self.lib.tsph_bz_integrate_outlet(self.ctx, zone, dt), "BZ_Integrate_outlet")
+
+    def step_inflow(self, wall: int, dt: float, nsteps: int = 1, inlet: int = 0, outlet: int = 1):
+        L.check(self.lib.tsph_step_inflow(self.ctx, wall, dt, nsteps, inlet, outlet), "tsph_step_inflow")
+
     # ---- ALE-SPH with MUSCL reconstruction (src/ALESPH_*.h) ----
     def ale_compute_density(self, limited: bool = False):
         L.check(self.lib.tsph_ale_compute_density(self.ctx, int(limited)), "Compute_Density")
@@ -323,9 +343,12 @@ class ParticleSystem:
 
 def system_for_case(case: CaseData, precision: int = L.PREC_FP64, capacity: int | None = None
                     ) -> ParticleSystem:
-    ps = ParticleSystem(case.params, precision=precision,
-                        capacity=capacity if capacity is not None else case.np)
+    if capacity is None:
+        capacity = max(case.np, int(case.params.get("capacity", 0)))
+    ps = ParticleSystem(case.params, precision=precision, capacity=capacity)
     ps.upload(case.arrays)
+    for z in case.zones:
+        ps.bz_add_zone(z)
     return ps
 
 
