@@ -15,10 +15,13 @@
 // by #include: define TSPH_DROPIN_FORCES to TSPH_FORCES_WCSPH (default,
 // src/SPH_interaction.h), TSPH_FORCES_DBC (src/SPH_interactionDBC.h) or
 // TSPH_FORCES_BT (src/SPH_interactionBT.h) before including this header.  Define
-// TSPH_DROPIN_RALE instead for the Riemann-ALE drivers
-// (cases/ALESPH_dambreakLong/SPH_simulationINT.h): Compute_Forces is then
-// src/RALESPH_interaction.h's, Integrate_LeapFrog_partOne/partTwo are provided
-// and the geometry routines create ALE particles (src/ALESPH_create.h).
+// TSPH_DROPIN_RALE instead for the ALE drivers (cases/ALESPH_dambreakLong/):
+// Compute_Forces is then src/RALESPH_interaction.h's, the leap-frog halves of
+// both ALE schemes and the ALE-SPH MUSCL stages (Compute_Density,
+// Kernel_gradient_approx, Integrate_density_compute_pressure,
+// Compute_Acceleration_BT) are provided, and the geometry routines create ALE
+// particles (src/ALESPH_create.h).  Buffer zones: DRAW_CreateInletY/X,
+// DRAW_CreateOutletY, BZ_Integrate_inlet/outlet (src/SPH_buffer_zones.h).
 //
 // Data stays on the device between stages.  The host vectors are refreshed by
 // tsph_sync_host(particles); host edits after the first stage call must be
@@ -391,6 +394,47 @@ inline void Integrate_LeapFrog_partOne(Particle_system &ps, double dt) {
 inline void Integrate_LeapFrog_partTwo(Particle_system &ps, double dt, int step) {
   tsph_push(ps);
   tsph_check(tsph_rale_leapfrog_two(ps.dev, dt, step), "Integrate_LeapFrog_partTwo");
+  tsph_ran(ps);
+}
+// ALE-SPH with MUSCL reconstruction (cases/ALESPH_dambreakLong/SPH_simulation.h).
+// TSPH_DROPIN_MINMOD selects the minmod-weighted reconstruction.
+#ifndef TSPH_DROPIN_MINMOD
+#define TSPH_DROPIN_MINMOD 0
+#endif
+// src/ALESPH_density_and_omega_bt.h:9-281
+inline void Compute_Density(Particle_system &ps) {
+  tsph_push(ps);
+  tsph_check(tsph_ale_compute_density(ps.dev, TSPH_DROPIN_MINMOD), "Compute_Density");
+  tsph_ran(ps);
+}
+// src/ALESPH_kernel_gradient.h:10-190
+inline void Kernel_gradient_approx(Particle_system &ps) {
+  tsph_push(ps);
+  tsph_check(tsph_ale_kernel_gradient(ps.dev), "Kernel_gradient_approx");
+  tsph_ran(ps);
+}
+// src/ALESPH_integration.h:86-135
+inline void Integrate_density_compute_pressure(Particle_system &ps, double dt) {
+  tsph_push(ps);
+  tsph_check(tsph_ale_density_pressure(ps.dev, dt), "Integrate_density_compute_pressure");
+  tsph_ran(ps);
+}
+// src/ALESPH_acceleration_bt.h:12-295
+inline void Compute_Acceleration_BT(Particle_system &ps) {
+  tsph_push(ps);
+  tsph_check(tsph_ale_compute_acceleration(ps.dev, TSPH_DROPIN_MINMOD), "Compute_Acceleration_BT");
+  tsph_ran(ps);
+}
+// src/ALESPH_integration.h:9-31
+inline void Integrate_LeapFrog_partOne_ALESPH(Particle_system &ps, double dt) {
+  tsph_push(ps);
+  tsph_check(tsph_ale_leapfrog_one(ps.dev, dt), "Integrate_LeapFrog_partOne_ALESPH");
+  tsph_ran(ps);
+}
+// src/ALESPH_integration.h:33-83
+inline void Integrate_LeapFrog_partTwo_ALESPH(Particle_system &ps, double dt, int step) {
+  tsph_push(ps);
+  tsph_check(tsph_ale_leapfrog_two(ps.dev, dt, step), "Integrate_LeapFrog_partTwo_ALESPH");
   tsph_ran(ps);
 }
 #endif
