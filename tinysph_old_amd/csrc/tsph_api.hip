@@ -162,7 +162,7 @@ static int alloc_ale(Soa &s, int cap) {
 static int record_rale_step(tsph_ctx *c, double dt, int step) {
   int e = 0;
   if (step > 1 && (e = launch_rale_leapfrog_one(c, dt))) return e;
-  if ((e = launch_cell_list(c)) || (e = launch_eos(c)) || (e = launch_rale_forces(c))) return e;
+  if ((e = launch_cell_list(c, true)) || (e = launch_rale_forces(c))) return e;
   if ((e = launch_rale_leapfrog_two(c, dt, step)) || (e = launch_remove(c))) return e;
   return 0;
 }
@@ -198,9 +198,9 @@ static int record_step(tsph_ctx *c, int wall, double dt, int inlet = -1, int out
   if ((e = launch_predictor(c, dt))) return e;
   if (inlet >= 0 && (e = launch_bz_inlet(c, inlet, dt))) return e;
   if (outlet >= 0 && (e = launch_bz_outlet(c, outlet, dt))) return e;
-  if ((e = launch_cell_list(c)) || (e = launch_eos(c)) || (e = launch_forces(c, variant))) return e;
+  if ((e = launch_cell_list(c, true)) || (e = launch_forces(c, variant))) return e;
   if ((e = launch_corrector(c, dt)) || (e = launch_remove(c))) return e;
-  if ((e = launch_cell_list(c)) || (e = launch_eos(c)) || (e = launch_forces(c, variant))) return e;
+  if ((e = launch_cell_list(c, true)) || (e = launch_forces(c, variant))) return e;
   return 0;
 }
 

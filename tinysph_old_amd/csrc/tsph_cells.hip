@@ -239,20 +239,29 @@ __global__ void k_order(const int *__restrict__ cell_start, int ncells,
 // Gathers every field into sorted order (the state after the build is the
 // reference's state, permuted), lists the wall particles, and sets the new
 // particle count (removed particles sit after n_new and are dropped).
+// With EOS the pressure is recomputed as Density_to_pressure does
+// (src/SPH_pressure.h:65-89) and the FP32 records of the mixed path are
+// written, for the step sequences in which that stage follows the cell build.
+template <bool EOS>
 __global__ void k_reorder(const int *__restrict__ cell_start, int ncells, const int *__restrict__ perm,
-                          Soa src, Soa dst, int *__restrict__ scal, int *__restrict__ wall_list) {
+                          Soa src, Soa dst, int *__restrict__ scal, int *__restrict__ wall_list,
+                          const int *__restrict__ cellof, Consts kc, Mixed mx,
+                          unsigned long long *__restrict__ counters) {
   const int n_new = cell_start[ncells];
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (blockIdx.x == 0 && threadIdx.x == 0) scal[S_NP] = n_new;
   int wall = 0;
+  unsigned bad = 0;
   if (k < n_new) {
     const int j = perm[k];
     const uint8_t t = src.type[j];
     wall = t == TSPH_WALL;
-    dst.r[k] = src.r[j];
-    dst.v[k] = src.v[j];
+    const double2 r = src.r[j], v = src.v[j];
+    const double rho = src.rho[j];
+    dst.r[k] = r;
+    dst.v[k] = v;
     dst.a[k] = src.a[j];
-    dst.rho[k] = src.rho[j];
+    dst.rho[k] = rho;
     dst.drho[k] = src.drho[j];
     dst.type[k] = t;
     dst.seq[k] = src.seq[j];
@@ -260,9 +269,19 @@ __global__ void k_reorder(const int *__restrict__ cell_start, int ncells, const 
     dst.r_o[k] = src.r_o[j];
     dst.v_o[k] = src.v_o[j];
     dst.rho_o[k] = src.rho_o[j];
-    dst.p[k] = src.p[j];
+    double p = src.p[j];
+    if (EOS && t != TSPH_OUTLET && t != TSPH_INLET) {
+      bad = rho > 1500. || rho < 0.;
+      p = tait64(rho, kc.rho0, kc.cs);
+    }
+    dst.p[k] = p;
     if (src.gnr) dst.gnr[k] = src.gnr[j];
-    if (src.n) dst.n[k] = src.n[j];
+    double2 nv;
+    if (src.n) {
+      nv = src.n[j];
+      dst.n[k] = nv;
+    }
+    if (EOS && mx.rv) write_mixed_record(mx, kc, cellof[k], k, r, v, rho, p, src.n ? &nv : nullptr);
     if (src.omega) {
       dst.omega[k] = src.omega[j];
       dst.omegarho[k] = src.omegarho[j];
@@ -288,6 +307,10 @@ __global__ void k_reorder(const int *__restrict__ cell_start, int ncells, const 
     base = __shfl(base, __builtin_ctzll(m), 64);
     if (wall) wall_list[base + __popcll(m & lanemask_lt())] = k;
   }
+  if (EOS) {
+    const unsigned long long w = __ballot(bad);
+    if (w && (threadIdx.x & 63) == 0) atomicAdd(&counters[C_BADRHO], (unsigned long long)__popcll(w));
+  }
 }
 
 int scan_exclusive(int *in, int len, int *out, int *partial, int *scal, hipStream_t s) {
@@ -301,7 +324,7 @@ int scan_exclusive(int *in, int len, int *out, int *partial, int *scal, hipStrea
 
 }  // namespace
 
-int launch_cell_list(tsph_ctx *c) {
+int launch_cell_list(tsph_ctx *c, bool eos) {
   const Consts &k = c->k;
   const int nb = 256;
   const int g = grid_for(c->n_ub, nb);
@@ -314,15 +337,20 @@ int launch_cell_list(tsph_ctx *c) {
                      c->cell_start, c->perm, c->cellof);
   hipLaunchKernelGGL(k_order, dim3(grid_for(k.ncells, nb)), dim3(nb), 0, s, c->cell_start,
                      k.ncells, c->cur.seq, c->perm);
-  hipLaunchKernelGGL(k_reorder, dim3(g), dim3(nb), 0, s, c->cell_start, k.ncells, c->perm, c->cur,
-                     c->alt, c->scal, c->wall_list);
+  const Mixed mx = c->prm.precision == TSPH_PREC_MIXED ? c->mx : Mixed{};
+  if (eos)
+    hipLaunchKernelGGL(k_reorder<true>, dim3(g), dim3(nb), 0, s, c->cell_start, k.ncells, c->perm,
+                       c->cur, c->alt, c->scal, c->wall_list, c->cellof, k, mx, c->counters);
+  else
+    hipLaunchKernelGGL(k_reorder<false>, dim3(g), dim3(nb), 0, s, c->cell_start, k.ncells, c->perm,
+                       c->cur, c->alt, c->scal, c->wall_list, c->cellof, k, mx, c->counters);
   TSPH_HIP(hipGetLastError());
   std::swap(c->cur, c->alt);
   c->parity ^= 1;
   c->cells_valid = true;
   c->tiles_valid = false;
   c->tiles_zeroed = true;
-  c->records_valid = false;
+  c->records_valid = eos && c->prm.precision == TSPH_PREC_MIXED;
   return 0;
 }
 

@@ -152,7 +152,9 @@ int hip_fail(hipError_t e, const char *what, const char *file, int line);
 inline int grid_for(int n, int block) { return n <= 0 ? 1 : (n + block - 1) / block; }
 
 // tsph_cells.hip
-int launch_cell_list(tsph_ctx *c);  // hash, count, scan, scatter, order, reorder
+// hash, count, scan, scatter, order, reorder; eos: the reorder also runs
+// Density_to_pressure (and writes the mixed records) for the moved particles
+int launch_cell_list(tsph_ctx *c, bool eos = false);
 // tsph_pairs.hip
 int ensure_tile_list(tsph_ctx *c);  // list of non-empty pair-loop tiles
 int launch_forces(tsph_ctx *c, int variant);

@@ -501,23 +501,60 @@ __global__ __launch_bounds__(kWave) void k_tile32(Soa d, Mixed mx, const int *__
     const int a1 = tab[t.w + rlo] + t.base[1], n1 = tab[t.w + rhi] + t.base[1] - a1;
     const int a2 = tab[2 * t.w + rlo] + t.base[2], n2 = tab[2 * t.w + rhi] + t.base[2] - a2;
     const int tot = act ? n0 + n1 + n2 : 0;
+    // candidate c of the flattened range lives at LDS slot c + off(c)
+    const int n01 = n0 + n1, off1 = a1 - n0, off2 = a2 - n01;
+    const bool dbc_wall = VAR == TSPH_FORCES_DBC && at == TSPH_WALL;
     int cur = 0;
     Acc32 s;
     while (__any(cur < tot)) {
       int cnt = 0;
+      // four candidates at a time while the list has room for four: their LDS
+      // loads are issued together
+      while (cur + 4 <= tot && cnt <= kList - 4) {
+        const int c0 = cur, c1 = cur + 1, c2 = cur + 2, c3 = cur + 3;
+        const int j0 = c0 + (c0 < n0 ? a0 : (c0 < n01 ? off1 : off2));
+        const int j1 = c1 + (c1 < n0 ? a0 : (c1 < n01 ? off1 : off2));
+        const int j2 = c2 + (c2 < n0 ? a0 : (c2 < n01 ? off1 : off2));
+        const int j3 = c3 + (c3 < n0 ? a0 : (c3 < n01 ? off1 : off2));
+        const float2 p0 = s_pos[j0], p1 = s_pos[j1], p2 = s_pos[j2], p3 = s_pos[j3];
+        const float x0 = pi.x - p0.x, y0 = pi.y - p0.y, x1 = pi.x - p1.x, y1 = pi.y - p1.y;
+        const float x2 = pi.x - p2.x, y2 = pi.y - p2.y, x3 = pi.x - p3.x, y3 = pi.y - p3.y;
+        bool k0 = x0 * x0 + y0 * y0 < cc.r2cut && j0 != li;
+        bool k1 = x1 * x1 + y1 * y1 < cc.r2cut && j1 != li;
+        bool k2 = x2 * x2 + y2 * y2 < cc.r2cut && j2 != li;
+        bool k3 = x3 * x3 + y3 * y3 < cc.r2cut && j3 != li;
+        if (VAR == TSPH_FORCES_DBC) {
+          const bool w0 = dbc_wall && s_t[j0] == TSPH_WALL, w1 = dbc_wall && s_t[j1] == TSPH_WALL;
+          const bool w2 = dbc_wall && s_t[j2] == TSPH_WALL, w3 = dbc_wall && s_t[j3] == TSPH_WALL;
+          ncand += (j0 != li && !w0) + (j1 != li && !w1) + (j2 != li && !w2) + (j3 != li && !w3);
+          k0 = k0 && !w0; k1 = k1 && !w1; k2 = k2 && !w2; k3 = k3 && !w3;
+        }
+        if (k0) my_list[(cnt++) * kWave] = (uint16_t)j0;
+        if (k1) my_list[(cnt++) * kWave] = (uint16_t)j1;
+        if (k2) my_list[(cnt++) * kWave] = (uint16_t)j2;
+        if (k3) my_list[(cnt++) * kWave] = (uint16_t)j3;
+        cur += 4;
+      }
       while (cur < tot && cnt < kList) {
-        const int j = cur < n0 ? a0 + cur : (cur < n0 + n1 ? a1 + (cur - n0) : a2 + (cur - n0 - n1));
-        ++cur;
-        if (j == li) continue;
-        if (VAR == TSPH_FORCES_DBC && at == TSPH_WALL && s_t[j] == TSPH_WALL) continue;
-        ++ncand;
+        const int c = cur++;
+        const int j = c + (c < n0 ? a0 : (c < n01 ? off1 : off2));
         const float2 pj = s_pos[j];
         const float drx = pi.x - pj.x, dry = pi.y - pj.y;
-        if (drx * drx + dry * dry < cc.r2cut) my_list[(cnt++) * kWave] = (uint16_t)j;
+        bool in = drx * drx + dry * dry < cc.r2cut && j != li;
+        if (VAR == TSPH_FORCES_DBC) {
+          const bool ww = dbc_wall && s_t[j] == TSPH_WALL;
+          ncand += j != li && !ww;
+          in = in && !ww;
+        }
+        if (in) my_list[(cnt++) * kWave] = (uint16_t)j;
       }
       npair += cnt;
+      // pair terms in list order; the next entry's records are loaded while
+      // the current one is evaluated
+      int jn = cnt > 0 ? my_list[0] : 0;
       for (int e = 0; e < cnt; ++e) {
-        const int j = my_list[e * kWave];
+        const int j = jn;
+        if (e + 1 < cnt) jn = my_list[(e + 1) * kWave];
         const float2 pj = s_pos[j], dpj = s_dp[j];
         const float drx = pi.x - pj.x, dry = pi.y - pj.y;
         const int nt = VAR != TSPH_FORCES_WCSPH ? s_t[j] : 0;
@@ -526,6 +563,9 @@ __global__ __launch_bounds__(kWave) void k_tile32(Soa d, Mixed mx, const int *__
                     dpj.y, s_ir[j], nt, nb);
       }
     }
+    // candidates: the stencil minus the particle itself (DBC: wall-wall pairs
+    // are not candidates either, counted as the reference skips them)
+    if (VAR != TSPH_FORCES_DBC) ncand += tot > 0 ? tot - 1 : 0;
     if (act) {
       double2 a;
       double dr;

@@ -99,16 +99,20 @@ __device__ __forceinline__ double tait64(double rho, double rho0, double c0) {
 // resolves rho - rho0 to ~1e-7 of its size, while FP32 rho itself, ulp 6e-5 at
 // 1000, would lose the small density differences the diffusion term is built
 // from)
+__device__ __forceinline__ void write_mixed_record(const Mixed &mx, const Consts &k, int c, int i,
+                                                   double2 r, double2 v, double rho, double p,
+                                                   const double2 *n) {
+  const int cx = c / k.ncy, cy = c - cx * k.ncy;
+  const double ox = k.x0 - k.of + cx * k.kh, oy = k.y0 - k.of + cy * k.kh;
+  mx.rv[i] = make_float4((float)(r.x - ox), (float)(r.y - oy), (float)v.x, (float)v.y);
+  mx.rp[i] = make_float2((float)(rho - k.rho0), (float)p);
+  if (mx.nrm && n) mx.nrm[i] = make_float2((float)n->x, (float)n->y);
+}
+
 __device__ __forceinline__ void write_mixed_record(const Soa &d, const Mixed &mx,
                                                    const int *__restrict__ cellof, const Consts &k,
                                                    int i, double rho, double p) {
-  const int c = cellof[i];
-  const int cx = c / k.ncy, cy = c - cx * k.ncy;
-  const double ox = k.x0 - k.of + cx * k.kh, oy = k.y0 - k.of + cy * k.kh;
-  const double2 r = d.r[i], v = d.v[i];
-  mx.rv[i] = make_float4((float)(r.x - ox), (float)(r.y - oy), (float)v.x, (float)v.y);
-  mx.rp[i] = make_float2((float)(rho - k.rho0), (float)p);
-  if (mx.nrm && d.n) mx.nrm[i] = make_float2((float)d.n[i].x, (float)d.n[i].y);
+  write_mixed_record(mx, k, cellof[i], i, d.r[i], d.v[i], rho, p, d.n ? &d.n[i] : nullptr);
 }
 
 __device__ __forceinline__ float hydro_drho_f32(float x, float rho0) {

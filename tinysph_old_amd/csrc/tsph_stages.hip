@@ -91,41 +91,81 @@ __device__ __forceinline__ int point_cell(const Consts &k, double x, double y) {
   return cell_coord(y, k.y0, k.of, k.kh) + cell_coord(x, k.x0, k.of, k.kh) * k.ncy;
 }
 
-__device__ double mdbc_density_at(const Soa &d, const int *__restrict__ cell_start, const Consts &k,
-                                  double2 ar, double2 gnd) {
-  const double h = k.h, m = k.m;
-  double a11 = 0, a12 = 0, a13 = 0, a21 = 0, a22 = 0, a23 = 0, a31 = 0, a32 = 0, a33 = 0;
-  double drhodx = 0, drhody = 0, rhowsum = 0;
-  const double r2cut = 4.0 * h * h * (1.0 + 1e-12);
+// Wall densities: one lane per wall particle.  The walk over the 9 cells
+// loads the records of eight consecutive candidates together (their loads are
+// in flight at once) and then adds their terms in candidate order, so every
+// sum is formed in the reference order.  Candidates that are not fluid or lie
+// outside the support add nothing, as in the reference.
+constexpr int kWallBatch = 8;
+
+template <int NS, typename Term>
+__device__ __forceinline__ void wall_sums(const Soa &d, const int *__restrict__ cell_start,
+                                          const Consts &k, double2 ar, double (&acc)[NS],
+                                          Term term) {
+  const double r2cut = 4.0 * k.h * k.h * (1.0 + 1e-12);
   const int c = point_cell(k, ar.x, ar.y);
+#pragma unroll
+  for (int q = 0; q < NS; ++q) acc[q] = 0.;
   for (int s = 0; s < 9; ++s) {
     const int cl = c + stencil_off(s, k.ncy);
     if (cl < 0 || cl >= k.ncells) continue;
-    const int j1 = cell_start[cl + 1];
-    for (int j = cell_start[cl]; j < j1; ++j) {
-      if (d.type[j] != TSPH_FLUID) continue;
-      const double2 nr = d.r[j];
-      const double drx = ar.x - nr.x, dry = ar.y - nr.y;
-      const double r2 = drx * drx + dry * dry;
-      if (r2 > r2cut) continue;
-      const double drs = sqrt(r2);
-      if (drs / h > 2.0) continue;
-      const KW64 kw = wendland64(drs, h, k.awen, k.bwen);
-      const double nrho = d.rho[j];
-      rhowsum += m * kw.w;
-      drhodx += drx * kw.f * m;
-      drhody += dry * kw.f * m;
-      a11 += kw.w * (m / nrho);
-      a12 += drx * kw.w * (m / nrho);
-      a13 += dry * kw.w * (m / nrho);
-      a21 += drx * kw.f * (m / nrho);
-      a22 += drx * drx * kw.f * (m / nrho);
-      a23 += drx * dry * kw.f * (m / nrho);
-      a31 += dry * kw.f * (m / nrho);
-      a32 += dry * drx * kw.f * (m / nrho);
-      a33 += dry * dry * kw.f * (m / nrho);
+    const int j0 = cell_start[cl], j1 = cell_start[cl + 1];
+    for (int jb = j0; jb < j1; jb += kWallBatch) {
+      int tt[kWallBatch];
+      double2 rr[kWallBatch];
+      double hh[kWallBatch];
+#pragma unroll
+      for (int u = 0; u < kWallBatch; ++u) {
+        const int j = jb + u < j1 ? jb + u : j1 - 1;
+        tt[u] = jb + u < j1 ? (int)d.type[j] : TSPH_WALL;
+        rr[u] = d.r[j];
+        hh[u] = d.rho[j];
+      }
+#pragma unroll
+      for (int u = 0; u < kWallBatch; ++u) {
+        const double drx = ar.x - rr[u].x, dry = ar.y - rr[u].y;
+        const double r2 = drx * drx + dry * dry;
+        if (tt[u] == TSPH_FLUID && r2 <= r2cut) {
+          const double drs = sqrt(r2);
+          if (drs / k.h <= 2.0) term(hh[u], drx, dry, drs, acc);
+        }
+      }
     }
   }
+}
+
+__global__ __launch_bounds__(64) void k_mdbc(Soa d, const int *__restrict__ cell_start, const int *__restrict__ scal,
+                       const int *__restrict__ wall_list, Consts k) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= scal[S_NWALL]) return;
+  const int i = wall_list[w];
+  const double2 gn = d.gnr[i], r = d.r[i];
+  const double m = k.m;
+  // sums of src/SPH_kernel_approx.h:1141-1342 in its order:
+  // rhowsum, drhodx, drhody, a11, a12, a13, a21, a22, a23, a31, a32, a33
+  double acc[12];
+  wall_sums<12>(d, cell_start, k, gn, acc,
+                [&](double nrho, double drx, double dry, double drs, double (&a)[12]) {
+                  const KW64 kw = wendland64(drs, k.h, k.awen, k.bwen);
+                  const double mr = m / nrho;
+                  a[0] += m * kw.w;
+                  a[1] += drx * kw.f * m;
+                  a[2] += dry * kw.f * m;
+                  a[3] += kw.w * mr;
+                  a[4] += drx * kw.w * mr;
+                  a[5] += dry * kw.w * mr;
+                  a[6] += drx * kw.f * mr;
+                  a[7] += drx * drx * kw.f * mr;
+                  a[8] += drx * dry * kw.f * mr;
+                  a[9] += dry * kw.f * mr;
+                  a[10] += dry * drx * kw.f * mr;
+                  a[11] += dry * dry * kw.f * mr;
+                });
+  const double rhowsum = acc[0], drhodx = acc[1], drhody = acc[2];
+  const double a11 = acc[3], a12 = acc[4], a13 = acc[5], a21 = acc[6], a22 = acc[7], a23 = acc[8];
+  const double a31 = acc[9], a32 = acc[10], a33 = acc[11];
+  const double2 gnd = make_double2(r.x - gn.x, r.y - gn.y);
+  double rho = 1000.;
   // determinant, src/SPH_kernel_approx.h:663-675
   const double det = (a11 * a22 * a33 + a12 * a23 * a31 + a13 * a21 * a32 - a31 * a22 * a13 -
                       a32 * a23 * a11 - a33 * a21 * a12);
@@ -142,54 +182,28 @@ __device__ double mdbc_density_at(const Soa &d, const int *__restrict__ cell_sta
     const double dtemp = rhowsum * i11 + drhodx * i12 + drhody * i13;
     const double grx = -(rhowsum * i21 + drhodx * i22 + drhody * i23);
     const double gry = -(rhowsum * i31 + drhodx * i32 + drhody * i33);
-    return (dtemp + grx * gnd.x + gry * gnd.y);
+    rho = dtemp + grx * gnd.x + gry * gnd.y;
   } else if (a11 > 0) {
-    return rhowsum / a11;
+    rho = rhowsum / a11;
   }
-  return 1000.;
-}
-
-// one thread per wall particle of the wall list built with the cell list
-__global__ void k_mdbc(Soa d, const int *__restrict__ cell_start, const int *__restrict__ scal,
-                       const int *__restrict__ wall_list, Consts k) {
-  const int w = blockIdx.x * blockDim.x + threadIdx.x;
-  if (w >= scal[S_NWALL]) return;
-  const int i = wall_list[w];
-  const double2 gn = d.gnr[i], r = d.r[i];
-  const double2 gnd = make_double2(r.x - gn.x, r.y - gn.y);
-  double rho = mdbc_density_at(d, cell_start, k, gn, gnd);
-  if (rho < 1000.) rho = 1000.;
+  if (rho < 1000.) rho = 1000.;  // src/SPH_mDBC.h:44
   d.rho[i] = rho;
 }
 
-__global__ void k_bt_density(Soa d, const int *__restrict__ cell_start, const int *__restrict__ scal,
+__global__ __launch_bounds__(64) void k_bt_density(Soa d, const int *__restrict__ cell_start, const int *__restrict__ scal,
                              const int *__restrict__ wall_list, Consts k) {
   const int w = blockIdx.x * blockDim.x + threadIdx.x;
   if (w >= scal[S_NWALL]) return;
   const int i = wall_list[w];
-  const double h = k.h, m = k.m;
-  const double r2cut = 4.0 * h * h * (1.0 + 1e-12);
-  const double2 ar = d.r[i];
-  double wsum = 0, rhowsum = 0;
-  const int c = point_cell(k, ar.x, ar.y);
-  for (int s = 0; s < 9; ++s) {
-    const int cl = c + stencil_off(s, k.ncy);
-    if (cl < 0 || cl >= k.ncells) continue;
-    const int j1 = cell_start[cl + 1];
-    for (int j = cell_start[cl]; j < j1; ++j) {
-      if (d.type[j] != TSPH_FLUID) continue;
-      const double2 nr = d.r[j];
-      const double drx = ar.x - nr.x, dry = ar.y - nr.y;
-      const double r2 = drx * drx + dry * dry;
-      if (r2 > r2cut) continue;
-      const double drs = sqrt(r2);
-      if (drs / h > 2.0) continue;
-      const KW64 kw = wendland64(drs, h, k.awen, k.bwen);
-      wsum += kw.w * m / d.rho[j];
-      rhowsum += m * kw.w;
-    }
-  }
-  double rho = wsum > 0. ? rhowsum / wsum : 1000.;
+  const double m = k.m;
+  double acc[2];  // wsum, rhowsum (src/SPH_kernel_approx.h:1838-1953)
+  wall_sums<2>(d, cell_start, k, d.r[i], acc,
+               [&](double nrho, double, double, double drs, double (&a)[2]) {
+                 const KW64 kw = wendland64(drs, k.h, k.awen, k.bwen);
+                 a[0] += kw.w * m / nrho;
+                 a[1] += m * kw.w;
+               });
+  double rho = acc[0] > 0. ? acc[1] / acc[0] : 1000.;
   if (rho < 1000.) rho = 1000.;
   d.rho[i] = rho;
 }
@@ -365,7 +379,7 @@ int launch_corrector(tsph_ctx *c, double dt) {
 }
 
 int launch_mdbc(tsph_ctx *c) {
-  hipLaunchKernelGGL(k_mdbc, dim3(grid_for(c->n_wall, 256)), dim3(256), 0, c->stream, c->cur,
+  hipLaunchKernelGGL(k_mdbc, dim3(grid_for(c->n_wall, 64)), dim3(64), 0, c->stream, c->cur,
                      c->cell_start, c->scal, c->wall_list, c->k);
   TSPH_HIP(hipGetLastError());
   c->records_valid = false;
@@ -373,7 +387,7 @@ int launch_mdbc(tsph_ctx *c) {
 }
 
 int launch_bt_density(tsph_ctx *c) {
-  hipLaunchKernelGGL(k_bt_density, dim3(grid_for(c->n_wall, 256)), dim3(256), 0, c->stream, c->cur,
+  hipLaunchKernelGGL(k_bt_density, dim3(grid_for(c->n_wall, 64)), dim3(64), 0, c->stream, c->cur,
                      c->cell_start, c->scal, c->wall_list, c->k);
   TSPH_HIP(hipGetLastError());
   c->records_valid = false;

@@ -31,10 +31,14 @@ __device__ __forceinline__ void wave_count(unsigned pairs, unsigned cand, unsign
   }
 }
 
-// exact reference inclusion test: qq = |dr|/h <= 2 (src/SPH_kernel.h:63)
+// exact reference inclusion test: qq = |dr|/h <= 2 (src/SPH_kernel.h:63).  The
+// rounded sqrt and division are only needed within a relative 1e-12 of the
+// boundary (their errors are ~1e-16); elsewhere r^2 against 4h^2 decides.
 __device__ __forceinline__ bool in_support64(double drx, double dry, double h) {
   const double r2 = (drx * drx) + (dry * dry);
-  if (r2 > 4.0 * h * h * (1.0 + 1e-12)) return false;
+  const double c2 = 4.0 * h * h;
+  if (r2 > c2 * (1.0 + 1e-12)) return false;
+  if (r2 < c2 * (1.0 - 1e-12)) return true;
   return sqrt(r2) / h <= 2.0;
 }
 
