@@ -251,6 +251,46 @@ int tsph_bz_integrate_outlet(tsph_ctx *ctx, int zone, double dt);
 int tsph_step_inflow(tsph_ctx *ctx, int wall, double dt, int nsteps, int inlet_zone,
                      int outlet_zone);
 
+/* ---- batched ensembles ----
+ * Many independent cases (each with its own constants, domain, time step and
+ * particle count) held in one context and advanced together: every stage is
+ * one launch over all cases.  The cell grids of the cases are laid end to end
+ * in one cell range (a segmented cell list) and the pair loops walk one list
+ * of tiles over all grids, so uneven cases share the persistent waves.  Each
+ * case keeps the semantics of a single-case context run on its own, including
+ * its reference particle order and the removal rule.  The per-case stage
+ * calls tsph_particle_to_cell, tsph_density_to_pressure, tsph_compute_forces,
+ * tsph_mdbc_density, tsph_bt_density and tsph_remove_out_of_domain act on all
+ * cases; entry points that take one case's state or one time step (upload,
+ * download, predictor/corrector, ALE, buffer zones, probes) are refused.  No
+ * counterpart in the reference, which runs one case per executable. */
+#define TSPH_BATCH_MAX_CASES 255
+/* prm[q], dt[q]: constants, padded domain, capacity and time step of case q */
+int tsph_batch_create(const tsph_params *prm, const double *dt, int ncases, tsph_ctx **out);
+/* *ncases = the case count (0 for a single-case context) */
+int tsph_batch_cases(tsph_ctx *ctx, int *ncases);
+/* grid (src/SPH_find_pairs.h:19-20) and particle mass of case icase; NULL outputs skipped */
+int tsph_batch_grid(tsph_ctx *ctx, int icase, int *ncx, int *ncy, double *mass);
+/* host[q] for q < ncases, in each case's reference order (as tsph_upload) */
+int tsph_batch_upload(tsph_ctx *ctx, const tsph_particles *host);
+/* np[q] = live particles of case q (ncases entries) */
+int tsph_batch_get_np(tsph_ctx *ctx, int *np);
+/* case icase's state in its reference order (as tsph_download) */
+int tsph_batch_download(tsph_ctx *ctx, int icase, tsph_particles *host);
+/* nsteps symplectic steps of every case, each by its own dt, replayed from a
+ * captured graph (the stage order of tsph_step_symplectic) */
+int tsph_batch_step(tsph_ctx *ctx, int wall, int nsteps);
+
+/* ---- autotuner ----
+ * Times the pair pass `variant` on the current state for each tile depth
+ * (cells per tile: 4, 5, 6, 7, 8, 10, 12) and persistent-wave count (waves per
+ * CU: 4, 6, 8, 12, 16), three passes each, and keeps the fastest pair; the
+ * workgroup is one wavefront by design and each lane owns one particle of its
+ * tile.  Needs a cell list; leaves a, drho as one pair pass with the chosen
+ * setting leaves them.
+ * Outputs may be NULL.  No counterpart in the reference. */
+int tsph_autotune(tsph_ctx *ctx, int variant, int *tile_rows, int *waves_per_cu, double *best_ms);
+
 /* ---- case setup (host only; no GPU needed) ----
  * Presets built with the reference's geometry routines (src/SPH_create.h,
  * src/SPH_create_bt.h) in the order of the reference case geometry files. */
@@ -291,6 +331,15 @@ typedef struct tsph_case {
  * seed drives the jitter (SplitMix64). */
 int tsph_case_build(int preset, double dx, double jitter, uint64_t seed, tsph_case **out);
 int tsph_case_free(tsph_case *c);
+
+/* The batched-ensemble workload: ncases tanks drawn from `seed` (SplitMix64),
+ * each with mDBC walls and the dam-break constants (h = sqrt(2) dx, kap = 2.1,
+ * c0 = 10 sqrt(g depth), dt = 0.2 h / c0, 500 steps): tank width W ~ U[0.5, 3] m,
+ * tank height = 2 W, fill ~ U[0.3, 0.8] (water depth = fill * 2 W), dx ~ U[2, 10] mm,
+ * and with probability 1/2 each still water over the whole width or a dam-break
+ * column W/2 wide; dx is then rescaled so that the tank holds between nmin and
+ * nmax fluid particles.  Fills cases[0..ncases-1]; free each with tsph_case_free. */
+int tsph_ensemble_build(int ncases, uint64_t seed, int nmin, int nmax, tsph_case **cases);
 
 #ifdef __cplusplus
 }

@@ -6,4 +6,7 @@ from ._lib import (  # noqa: F401
     FORCES_WCSPH, PREC_FP64, PREC_MIXED, WALL, WALL_BT, WALL_DBC, WALL_MDBC, SCHEME_RALE,
     SCHEME_SYMPLECTIC, MUSCL_LINEAR, MUSCL_MINMOD, MUSCL_NONE, TsphError, lib,
 )
-from .system import CaseData, ParticleSystem, build_case, forces_variant, system_for_case  # noqa: F401
+from .system import (  # noqa: F401
+    BatchSystem, CaseData, ParticleSystem, autotune, batch_for_cases, build_case, build_ensemble,
+    forces_variant, system_for_case,
+)

@@ -120,6 +120,18 @@ EXPORTS = {
     "tsph_case_build": (C.c_int, [C.c_int, C.c_double, C.c_double, C.c_uint64,
                                   C.POINTER(C.POINTER(CaseC))]),
     "tsph_case_free": (C.c_int, [C.POINTER(CaseC)]),
+    "tsph_ensemble_build": (C.c_int, [C.c_int, C.c_uint64, C.c_int, C.c_int,
+                                      C.POINTER(C.POINTER(CaseC))]),
+    "tsph_batch_create": (C.c_int, [C.POINTER(Params), _dp, C.c_int, C.POINTER(C.c_void_p)]),
+    "tsph_batch_cases": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
+    "tsph_batch_grid": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                  _dp]),
+    "tsph_batch_upload": (C.c_int, [C.c_void_p, C.POINTER(Particles)]),
+    "tsph_batch_get_np": (C.c_int, [C.c_void_p, _i32p]),
+    "tsph_batch_download": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(Particles)]),
+    "tsph_batch_step": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    "tsph_autotune": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                _dp]),
 }
 
 _LIB = None

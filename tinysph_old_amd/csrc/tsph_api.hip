@@ -49,7 +49,7 @@ static int alloc_soa(Soa &s, int cap, bool gnr, bool n) {
 }
 
 static void free_soa(Soa &s) {
-  void *ps[] = {s.r, s.v, s.a, s.r_o, s.v_o, s.gnr, s.n, s.rho, s.drho, s.p, s.rho_o,
+  void *ps[] = {s.cid, s.r, s.v, s.a, s.r_o, s.v_o, s.gnr, s.n, s.rho, s.drho, s.p, s.rho_o,
                 s.type, s.seq, s.index, s.omega, s.omegarho, s.omega_o, s.omegarho_o,
                 s.domega, s.domegarho, s.omegav, s.omegav_o, s.domegav, s.gradrho, s.gradvx,
                 s.gradvy};
@@ -213,19 +213,18 @@ extern "C" {
 const char *tsph_last_error(void) { return g_err.c_str(); }
 int tsph_abi_version(void) { return TSPH_ABI_VERSION; }
 
-int tsph_create(const tsph_params *prm, tsph_ctx **out) {
-  if (!prm || !out) return fail(TSPH_EINVAL, "null argument");
-  *out = nullptr;
+}  // extern "C"
+
+namespace tsph {
+
+// Particle_system constructor + Divide_To_Cells for one case's constants
+static int make_consts(const tsph_params *prm, Consts &k) {
   if (!(prm->h > 0) || !(prm->kap > 0) || !(prm->dp > 0) || prm->capacity <= 0 ||
       !(prm->x_m > prm->x_0) || !(prm->y_m > prm->y_0))
     return fail(TSPH_EINVAL, "invalid parameters");
   if (prm->precision != TSPH_PREC_FP64 && prm->precision != TSPH_PREC_MIXED)
     return fail(TSPH_EINVAL, "unknown precision");
-  tsph_ctx *c = new tsph_ctx();
-  c->prm = *prm;
-  c->capacity = prm->capacity;
-  c->tile_rows = 7;
-  Consts &k = c->k;
+  k = Consts{};
   k.h = prm->h; k.kap = prm->kap; k.cs = prm->cs; k.rho0 = prm->rho0; k.avisc = prm->avisc;
   k.delta = prm->delta; k.dp = prm->dp;
   k.m = prm->rho0 / (1 / prm->dp * 1 / prm->dp);  // src/SPH_particle_system.h:207
@@ -238,35 +237,33 @@ int tsph_create(const tsph_params *prm, tsph_ctx **out) {
   k.ncx = (int)((k.xm - k.x0 + k.of) / k.kh) + 1;  // src/SPH_find_pairs.h:19-20
   k.ncy = (int)((k.ym - k.y0 + k.of) / k.kh) + 1;
   const long long nc = (long long)k.ncx * k.ncy;
-  if (nc <= 0 || nc > (1ll << 30)) {
-    delete c;
-    return fail(TSPH_EINVAL, "cell grid too large");
-  }
+  if (nc <= 0 || nc > (1ll << 30)) return fail(TSPH_EINVAL, "cell grid too large");
   k.ncells = (int)nc;
+  return 0;
+}
+
+// device buffers of a context holding `cap` particles over `ncells_tot` cells
+static int alloc_context(tsph_ctx *c) {
   int e = 0;
-  auto bail = [&](int code) {
-    tsph_destroy(c);
-    return code;
-  };
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
-    return bail(fail(TSPH_EHIP, "hipStreamCreate failed"));
-  const int cap = c->capacity;
-  c->seq_cap = 2 * cap + kBzMax;
+    return fail(TSPH_EHIP, "hipStreamCreate failed");
+  const int cap = c->capacity, nct = c->ncells_tot;
+  c->seq_cap = 2 * cap + kBzMax + c->ncases;
   if ((e = alloc_soa(c->cur, cap, false, false)) || (e = alloc_soa(c->alt, cap, false, false)))
-    return bail(e);
+    return e;
   if ((e = dalloc(&c->scal, S_NSCAL)) || (e = dalloc(&c->counters, C_NCOUNT)) ||
-      (e = dalloc(&c->cell_count, k.ncells + 1)) || (e = dalloc(&c->cell_start, k.ncells + 2)) ||
+      (e = dalloc(&c->cell_count, nct + 1)) || (e = dalloc(&c->cell_start, nct + 2)) ||
       (e = dalloc(&c->cellof, cap)) || (e = dalloc(&c->hrank, cap)) || (e = dalloc(&c->hcell, cap)) ||
-      (e = dalloc(&c->perm, cap)) || (e = dalloc(&c->scan_tmp, (k.ncells + 1) / 1024 + 2)) ||
-      (e = dalloc(&c->seqstate, (size_t)2 * cap + kBzMax)) || (e = dalloc(&c->probe_out, 4)) ||
+      (e = dalloc(&c->perm, cap)) || (e = dalloc(&c->scan_tmp, (nct + 1) / 1024 + 2)) ||
+      (e = dalloc(&c->seqstate, (size_t)c->seq_cap)) || (e = dalloc(&c->probe_out, 4)) ||
       (e = dalloc(&c->bz_list, 2 * kBzMax)) ||
-      (e = dalloc(&c->tile_list, k.ncells + 8)) || (e = dalloc(&c->wall_list, cap)))
-    return bail(e);
+      (e = dalloc(&c->tile_list, nct + 8)) || (e = dalloc(&c->wall_list, cap)))
+    return e;
   // the per-cell counters and removal states start cleared; later builds clear
   // them as they consume them
-  if (hipMemsetAsync(c->cell_count, 0, sizeof(int) * (k.ncells + 1), c->stream) != hipSuccess ||
-      hipMemsetAsync(c->seqstate, 0, (size_t)2 * cap + kBzMax, c->stream) != hipSuccess)
-    return bail(fail(TSPH_EHIP, "initialisation failed"));
+  if (hipMemsetAsync(c->cell_count, 0, sizeof(int) * (nct + 1), c->stream) != hipSuccess ||
+      hipMemsetAsync(c->seqstate, 0, (size_t)c->seq_cap, c->stream) != hipSuccess)
+    return fail(TSPH_EHIP, "initialisation failed");
   {
     int dev = 0;
     hipDeviceProp_t prop;
@@ -274,13 +271,55 @@ int tsph_create(const tsph_params *prm, tsph_ctx **out) {
         prop.multiProcessorCount > 0)
       c->num_cu = prop.multiProcessorCount;
   }
-  if (prm->precision == TSPH_PREC_MIXED) {
-    if ((e = dalloc(&c->mx.rv, cap)) || (e = dalloc(&c->mx.rp, cap))) return bail(e);
+  if (c->prm.precision == TSPH_PREC_MIXED) {
+    if ((e = dalloc(&c->mx.rv, cap)) || (e = dalloc(&c->mx.rp, cap))) return e;
   }
   if (hipMemsetAsync(c->scal, 0, S_NSCAL * sizeof(int), c->stream) != hipSuccess ||
       hipMemsetAsync(c->counters, 0, C_NCOUNT * sizeof(unsigned long long), c->stream) != hipSuccess ||
       hipStreamSynchronize(c->stream) != hipSuccess)
-    return bail(fail(TSPH_EHIP, "initialisation failed"));
+    return fail(TSPH_EHIP, "initialisation failed");
+  return 0;
+}
+
+// entry points that act on one case's state are refused on batched contexts
+static int single_only(const tsph_ctx *c) {
+  return c->cases ? fail(TSPH_ESTATE, "not available on a batched context") : 0;
+}
+
+// first global pair-loop tile of each case of a batched context at the
+// current tile_rows (host and device copies; no-op for a single case)
+static int update_tile_base(tsph_ctx *c) {
+  if (!c->cases) return 0;
+  std::vector<int> &tb = c->host_tile_base;
+  tb.assign(c->ncases + 1, 0);
+  for (int q = 0; q < c->ncases; ++q) {
+    const Consts &k = c->host_cases[q];
+    tb[q + 1] = tb[q] + k.ncx * ((k.ncy + c->tile_rows - 1) / c->tile_rows);
+  }
+  TSPH_HIP(hipStreamSynchronize(c->stream));
+  TSPH_HIP(hipMemcpy(c->tile_base, tb.data(), sizeof(int) * (c->ncases + 1), hipMemcpyHostToDevice));
+  return 0;
+}
+
+}  // namespace tsph
+
+extern "C" {
+
+int tsph_create(const tsph_params *prm, tsph_ctx **out) {
+  if (!prm || !out) return fail(TSPH_EINVAL, "null argument");
+  *out = nullptr;
+  Consts k;
+  if (int e = make_consts(prm, k)) return e;
+  tsph_ctx *c = new tsph_ctx();
+  c->prm = *prm;
+  c->capacity = prm->capacity;
+  c->tile_rows = 7;
+  c->k = k;
+  c->ncells_tot = k.ncells;
+  if (int e = alloc_context(c)) {
+    tsph_destroy(c);
+    return e;
+  }
   *out = c;
   return 0;
 }
@@ -293,7 +332,8 @@ int tsph_destroy(tsph_ctx *c) {
   free_soa(c->alt);
   void *ps[] = {c->scal, c->counters, c->cell_count, c->cell_start, c->cellof, c->hrank,
                 c->hcell, c->perm, c->scan_tmp, c->seqstate, c->probe_out, c->mx.rv,
-                c->mx.rp, c->mx.nrm, c->tile_list, c->wall_list, c->bz_list};
+                c->mx.rp, c->mx.nrm, c->tile_list, c->wall_list, c->bz_list, c->cases,
+                c->tile_base};
   for (void *p : ps)
     if (p) (void)hipFree(p);
   for (hipEvent_t ev : c->tev) (void)hipEventDestroy(ev);
@@ -304,6 +344,7 @@ int tsph_destroy(tsph_ctx *c) {
 
 int tsph_grid(tsph_ctx *c, int *ncx, int *ncy) {
   if (!c || !ncx || !ncy) return fail(TSPH_EINVAL, "null argument");
+  if (int e_ = single_only(c)) return e_;
   *ncx = c->k.ncx;
   *ncy = c->k.ncy;
   return 0;
@@ -311,12 +352,21 @@ int tsph_grid(tsph_ctx *c, int *ncx, int *ncy) {
 
 int tsph_mass(tsph_ctx *c, double *m) {
   if (!c || !m) return fail(TSPH_EINVAL, "null argument");
+  if (int e_ = single_only(c)) return e_;
   *m = c->k.m;
   return 0;
 }
 
-int tsph_upload(tsph_ctx *c, const tsph_particles *h) {
-  if (!c || !h) return fail(TSPH_EINVAL, "null argument");
+}  // extern "C"
+
+namespace tsph {
+
+// Writes n particles (reference order, fields as in tsph_particles) into the
+// device state.  seq / cid / index: per-particle reference position, case and
+// reference index (nullptr: i, none, h->index or i).  maxseq: first unused
+// reference position.
+static int upload_impl(tsph_ctx *c, const tsph_particles *h, const uint32_t *seq_in,
+                       const uint8_t *cid, int maxseq) {
   const int n = h->np;
   if (n < 0) return fail(TSPH_EINVAL, "negative particle count");
   if (n > c->capacity) return fail(TSPH_ECAPACITY, "particle count exceeds capacity");
@@ -331,7 +381,7 @@ int tsph_upload(tsph_ctx *c, const tsph_particles *h) {
   for (int i = 0; i < n; ++i) {
     if (h->type[i] < 0 || h->type[i] > 254) return fail(TSPH_EINVAL, "particle type out of range");
     t[i] = (uint8_t)h->type[i];
-    seq[i] = (uint32_t)i;
+    seq[i] = seq_in ? seq_in[i] : (uint32_t)i;
     idx[i] = h->index ? h->index[i] : i;
   }
   std::vector<double> zeros(2 * (size_t)n, 0.0);
@@ -355,6 +405,7 @@ int tsph_upload(tsph_ctx *c, const tsph_particles *h) {
     TSPH_HIP(cp(d.type, t.data(), n));
     TSPH_HIP(cp(d.seq, seq.data(), 4 * (size_t)n));
     TSPH_HIP(cp(d.index, idx.data(), 4 * (size_t)n));
+    if (cid) TSPH_HIP(cp(d.cid, cid, n));
     if (h->gnr) TSPH_HIP(cp(d.gnr, h->gnr, s2));
     else if (c->has_gnr) TSPH_HIP(cp(d.gnr, zeros.data(), s2));
     if (h->n) TSPH_HIP(cp(d.n, h->n, s2));
@@ -362,7 +413,7 @@ int tsph_upload(tsph_ctx *c, const tsph_particles *h) {
   }
   int sc[S_NSCAL] = {};
   sc[S_NP] = n;
-  sc[S_MAXSEQ] = n;
+  sc[S_MAXSEQ] = maxseq;
   TSPH_HIP(hipMemcpy(c->scal, sc, sizeof sc, hipMemcpyHostToDevice));
   TSPH_HIP(hipMemset(c->seqstate, 0, (size_t)c->seq_cap));
   c->n_ub = n;
@@ -376,6 +427,16 @@ int tsph_upload(tsph_ctx *c, const tsph_particles *h) {
   return 0;
 }
 
+}  // namespace tsph
+
+extern "C" {
+
+int tsph_upload(tsph_ctx *c, const tsph_particles *h) {
+  if (!c || !h) return fail(TSPH_EINVAL, "null argument");
+  if (int e = single_only(c)) return e;
+  return upload_impl(c, h, nullptr, nullptr, h->np);
+}
+
 int tsph_get_np(tsph_ctx *c, int *np) {
   if (!c || !np) return fail(TSPH_EINVAL, "null argument");
   TSPH_HIP(hipStreamSynchronize(c->stream));
@@ -383,30 +444,40 @@ int tsph_get_np(tsph_ctx *c, int *np) {
   return 0;
 }
 
-// order of the live slots in reference order (by seq), skipping removed ones
-static int live_order(tsph_ctx *c, int n, std::vector<int> &order, std::vector<uint32_t> &seq,
-                      std::vector<uint8_t> &type) {
+}  // extern "C"
+
+namespace tsph {
+
+// order of the live slots in reference order (by seq), skipping removed ones;
+// icase >= 0 keeps the slots of that case of a batched context only
+static int live_order(tsph_ctx *c, int n, int icase, std::vector<int> &order,
+                      std::vector<uint32_t> &seq, std::vector<uint8_t> &type) {
   seq.resize(n);
   type.resize(n);
+  std::vector<uint8_t> cid;
   if (n > 0) {
     TSPH_HIP(hipMemcpy(seq.data(), c->cur.seq, 4 * (size_t)n, hipMemcpyDeviceToHost));
     TSPH_HIP(hipMemcpy(type.data(), c->cur.type, n, hipMemcpyDeviceToHost));
+    if (icase >= 0) {
+      cid.resize(n);
+      TSPH_HIP(hipMemcpy(cid.data(), c->cur.cid, n, hipMemcpyDeviceToHost));
+    }
   }
   order.clear();
   for (int i = 0; i < n; ++i)
-    if (type[i] != kRemoved) order.push_back(i);
+    if (type[i] != kRemoved && (icase < 0 || cid[i] == icase)) order.push_back(i);
   std::sort(order.begin(), order.end(), [&](int a, int b) { return seq[a] < seq[b]; });
   return 0;
 }
 
-int tsph_download(tsph_ctx *c, tsph_particles *h) {
-  if (!c || !h) return fail(TSPH_EINVAL, "null argument");
+static int download_impl(tsph_ctx *c, tsph_particles *h, int icase) {
   int n = 0, e = 0;
-  if ((e = tsph_get_np(c, &n))) return e;
+  TSPH_HIP(hipStreamSynchronize(c->stream));
+  TSPH_HIP(hipMemcpy(&n, &c->scal[S_NP], sizeof(int), hipMemcpyDeviceToHost));
   std::vector<int> order;
   std::vector<uint32_t> seq;
   std::vector<uint8_t> type;
-  if ((e = live_order(c, n, order, seq, type))) return e;
+  if ((e = live_order(c, n, icase, order, seq, type))) return e;
   const int m = (int)order.size();
   h->np = m;
   std::vector<double> b1(n), b2(2 * (size_t)n);
@@ -446,6 +517,16 @@ int tsph_download(tsph_ctx *c, tsph_particles *h) {
   return 0;
 }
 
+}  // namespace tsph
+
+extern "C" {
+
+int tsph_download(tsph_ctx *c, tsph_particles *h) {
+  if (!c || !h) return fail(TSPH_EINVAL, "null argument");
+  if (int e = single_only(c)) return e;
+  return download_impl(c, h, -1);
+}
+
 int tsph_clear_cells(tsph_ctx *c) {
   if (!c) return fail(TSPH_EINVAL, "null argument");
   c->cells_valid = false;  // the cell list is rebuilt whole by tsph_particle_to_cell
@@ -459,17 +540,18 @@ int tsph_particle_to_cell(tsph_ctx *c) {
 
 int tsph_get_cell_lists(tsph_ctx *c, int32_t *start, int32_t *items) {
   if (!c || !start || !items) return fail(TSPH_EINVAL, "null argument");
+  if (int e_ = single_only(c)) return e_;
   if (!c->cells_valid) return fail(TSPH_ESTATE, "no cell list: call tsph_particle_to_cell first");
   int n = 0, e = 0;
   if ((e = tsph_get_np(c, &n))) return e;
   std::vector<int> order;
   std::vector<uint32_t> seq;
   std::vector<uint8_t> type;
-  if ((e = live_order(c, n, order, seq, type))) return e;
+  if ((e = live_order(c, n, -1, order, seq, type))) return e;
   // reference index of slot = its rank in reference order
   std::vector<int> rank(n, -1);
   for (size_t k = 0; k < order.size(); ++k) rank[order[k]] = (int)k;
-  TSPH_HIP(hipMemcpy(start, c->cell_start, sizeof(int32_t) * (c->k.ncells + 1), hipMemcpyDeviceToHost));
+  TSPH_HIP(hipMemcpy(start, c->cell_start, sizeof(int32_t) * (c->ncells_tot + 1), hipMemcpyDeviceToHost));
   for (int k = 0; k < n; ++k) items[k] = rank[k];
   return 0;
 }
@@ -502,11 +584,13 @@ int tsph_bt_density(tsph_ctx *c) {
 
 int tsph_symplectic_predictor(tsph_ctx *c, double dt) {
   if (!c) return fail(TSPH_EINVAL, "null argument");
+  if (int e_ = single_only(c)) return e_;
   return launch_predictor(c, dt);
 }
 
 int tsph_symplectic_corrector(tsph_ctx *c, double dt) {
   if (!c) return fail(TSPH_EINVAL, "null argument");
+  if (int e_ = single_only(c)) return e_;
   return launch_corrector(c, dt);
 }
 
@@ -517,6 +601,7 @@ int tsph_remove_out_of_domain(tsph_ctx *c) {
 
 int tsph_step_symplectic(tsph_ctx *c, int wall, double dt, int nsteps) {
   if (!c) return fail(TSPH_EINVAL, "null argument");
+  if (int e_ = single_only(c)) return e_;
   if (wall < TSPH_WALL_DBC || wall > TSPH_WALL_BT) return fail(TSPH_EINVAL, "unknown wall treatment");
   if (!c->cells_valid) return fail(TSPH_ESTATE, "no cell list: call tsph_particle_to_cell first");
   if (wall == TSPH_WALL_MDBC && !c->has_gnr) return fail(TSPH_ESTATE, "mDBC needs ghost nodes (gnr)");
@@ -533,6 +618,7 @@ int tsph_step_symplectic(tsph_ctx *c, int wall, double dt, int nsteps) {
 
 int tsph_ale_enable(tsph_ctx *c) {
   if (!c) return fail(TSPH_EINVAL, "null argument");
+  if (int e_ = single_only(c)) return e_;
   int e = 0;
   if (!c->has_ale) {
     if ((e = alloc_ale(c->cur, c->capacity)) || (e = alloc_ale(c->alt, c->capacity)) ||
@@ -546,12 +632,13 @@ int tsph_ale_enable(tsph_ctx *c) {
 
 int tsph_ale_upload(tsph_ctx *c, const tsph_ale_fields *h) {
   if (!c || !h) return fail(TSPH_EINVAL, "null argument");
+  if (int e_ = single_only(c)) return e_;
   int e = 0, n = 0;
   if ((e = tsph_ale_enable(c)) || (e = tsph_get_np(c, &n))) return e;
   std::vector<int> order;
   std::vector<uint32_t> seq;
   std::vector<uint8_t> type;
-  if ((e = live_order(c, n, order, seq, type))) return e;
+  if ((e = live_order(c, n, -1, order, seq, type))) return e;
   const int m = (int)order.size();
   if (h->np != m) return fail(TSPH_EINVAL, "ALE field count differs from the particle count");
   std::vector<double> b(2 * (size_t)n + 2, 0.0);
@@ -586,13 +673,14 @@ int tsph_ale_upload(tsph_ctx *c, const tsph_ale_fields *h) {
 
 int tsph_ale_download(tsph_ctx *c, tsph_ale_fields *h) {
   if (!c || !h) return fail(TSPH_EINVAL, "null argument");
+  if (int e_ = single_only(c)) return e_;
   if (!c->has_ale) return fail(TSPH_ESTATE, "ALE variables not enabled (tsph_ale_enable)");
   int n = 0, e = 0;
   if ((e = tsph_get_np(c, &n))) return e;
   std::vector<int> order;
   std::vector<uint32_t> seq;
   std::vector<uint8_t> type;
-  if ((e = live_order(c, n, order, seq, type))) return e;
+  if ((e = live_order(c, n, -1, order, seq, type))) return e;
   const int m = (int)order.size();
   std::vector<double> b(2 * (size_t)n + 2);
   auto get1 = [&](double *dst, const double *src) -> int {
@@ -705,6 +793,7 @@ int tsph_step_ale(tsph_ctx *c, double dt, int step, int nsteps, int muscl) {
 
 int tsph_bz_add_zone(tsph_ctx *c, const tsph_zone *z, int *index) {
   if (!c || !z) return fail(TSPH_EINVAL, "null argument");
+  if (int e_ = single_only(c)) return e_;
   if (c->n_zones >= TSPH_MAX_ZONES) return fail(TSPH_ECAPACITY, "too many buffer zones");
   if ((z->ox != 0) == (z->oy != 0)) return fail(TSPH_EINVAL, "zone orientation must be along x or y");
   Zone &q = c->zones[c->n_zones];
@@ -737,6 +826,7 @@ int tsph_bz_integrate_outlet(tsph_ctx *c, int zone, double dt) {
 
 int tsph_step_inflow(tsph_ctx *c, int wall, double dt, int nsteps, int inlet, int outlet) {
   if (!c) return fail(TSPH_EINVAL, "null argument");
+  if (int e_ = single_only(c)) return e_;
   if (wall < TSPH_WALL_DBC || wall > TSPH_WALL_BT || nsteps < 0)
     return fail(TSPH_EINVAL, "invalid wall treatment or step count");
   if (inlet >= 0 && (inlet >= c->n_zones)) return fail(TSPH_EINVAL, "no such inlet zone");
@@ -756,12 +846,14 @@ int tsph_step_inflow(tsph_ctx *c, int wall, double dt, int nsteps, int inlet, in
 
 int tsph_water_elevation(tsph_ctx *c, double x, double y0, int inverse, double *level) {
   if (!c || !level) return fail(TSPH_EINVAL, "null argument");
+  if (int e_ = single_only(c)) return e_;
   if (!c->cells_valid) return fail(TSPH_ESTATE, "no cell list: call tsph_particle_to_cell first");
   return run_water_elevation(c, x, y0, inverse, level);
 }
 
 int tsph_interpolate_point(tsph_ctx *c, double x, double y, double *out4) {
   if (!c || !out4) return fail(TSPH_EINVAL, "null argument");
+  if (int e_ = single_only(c)) return e_;
   if (!c->cells_valid) return fail(TSPH_ESTATE, "no cell list: call tsph_particle_to_cell first");
   return run_interpolate_point(c, x, y, out4);
 }
@@ -800,7 +892,7 @@ int tsph_set_tile_rows(tsph_ctx *c, int rows) {
     c->tiles_valid = false;
   }
   c->tile_rows = rows;
-  return 0;
+  return update_tile_base(c);
 }
 
 int tsph_set_waves_per_cu(tsph_ctx *c, int waves) {
@@ -853,6 +945,247 @@ int tsph_pair_timing_read(tsph_ctx *c, double *total_ms, int *launches) {
   *total_ms = tot;
   *launches = c->tev_used / 2;
   c->tev_used = 0;
+  return 0;
+}
+
+
+// ---- batched ensembles -------------------------------------------------------
+
+int tsph_batch_create(const tsph_params *prm, const double *dt, int ncases, tsph_ctx **out) {
+  if (!prm || !dt || !out) return fail(TSPH_EINVAL, "null argument");
+  *out = nullptr;
+  if (ncases < 1 || ncases > TSPH_BATCH_MAX_CASES)
+    return fail(TSPH_EINVAL, "case count must be 1..255");
+  std::vector<Consts> ks(ncases);
+  std::vector<int> caps(ncases);
+  long long cells = 0, cap = 0;
+  for (int q = 0; q < ncases; ++q) {
+    if (prm[q].precision != prm[0].precision) return fail(TSPH_EINVAL, "cases differ in precision");
+    if (!(dt[q] > 0.)) return fail(TSPH_EINVAL, "time steps must be positive");
+    if (prm[q].capacity < 0) return fail(TSPH_EINVAL, "negative capacity");
+    if (int e = make_consts(&prm[q], ks[q])) return e;
+    ks[q].cell_base = (int)cells;
+    ks[q].dt = dt[q];
+    cells += ks[q].ncells;
+    cap += prm[q].capacity;
+    caps[q] = prm[q].capacity;
+    if (cells > (1ll << 30)) return fail(TSPH_EINVAL, "cell grids too large");
+    if (cap > (1ll << 30)) return fail(TSPH_ECAPACITY, "capacity too large");
+  }
+  tsph_ctx *c = new tsph_ctx();
+  c->prm = prm[0];
+  c->prm.capacity = (int)cap;
+  c->capacity = (int)cap;
+  c->tile_rows = 7;
+  c->k = ks[0];
+  c->ncells_tot = (int)cells;
+  c->ncases = ncases;
+  c->host_cases = ks;
+  c->case_cap = caps;
+  int e = 0;
+  if ((e = alloc_context(c)) || (e = dalloc(&c->cases, ncases)) ||
+      (e = dalloc(&c->tile_base, ncases + 1)) || (e = dalloc(&c->cur.cid, c->capacity)) ||
+      (e = dalloc(&c->alt.cid, c->capacity))) {
+    tsph_destroy(c);
+    return e;
+  }
+  if (hipMemcpy(c->cases, ks.data(), sizeof(Consts) * ncases, hipMemcpyHostToDevice) != hipSuccess) {
+    tsph_destroy(c);
+    return fail(TSPH_EHIP, "initialisation failed");
+  }
+  if ((e = update_tile_base(c))) {
+    tsph_destroy(c);
+    return e;
+  }
+  *out = c;
+  return 0;
+}
+
+int tsph_batch_cases(tsph_ctx *c, int *ncases) {
+  if (!c || !ncases) return fail(TSPH_EINVAL, "null argument");
+  *ncases = c->cases ? c->ncases : 0;
+  return 0;
+}
+
+int tsph_batch_grid(tsph_ctx *c, int icase, int *ncx, int *ncy, double *mass) {
+  if (!c) return fail(TSPH_EINVAL, "null argument");
+  if (!c->cases) return fail(TSPH_ESTATE, "not a batched context");
+  if (icase < 0 || icase >= c->ncases) return fail(TSPH_EINVAL, "no such case");
+  const Consts &k = c->host_cases[icase];
+  if (ncx) *ncx = k.ncx;
+  if (ncy) *ncy = k.ncy;
+  if (mass) *mass = k.m;
+  return 0;
+}
+
+int tsph_batch_upload(tsph_ctx *c, const tsph_particles *h) {
+  if (!c || !h) return fail(TSPH_EINVAL, "null argument");
+  if (!c->cases) return fail(TSPH_ESTATE, "not a batched context");
+  const int nc = c->ncases;
+  long long tot = 0;
+  bool any_gnr = false, any_n = false;
+  for (int q = 0; q < nc; ++q) {
+    if (h[q].np < 0 || h[q].np > c->case_cap[q])
+      return fail(TSPH_ECAPACITY, "case particle count outside 0..its capacity");
+    if (h[q].np > 0 && (!h[q].type || !h[q].rho || !h[q].r))
+      return fail(TSPH_EINVAL, "type, rho and r are required");
+    tot += h[q].np;
+    any_gnr |= h[q].gnr != nullptr;
+    any_n |= h[q].n != nullptr;
+  }
+  // the cases one after another.  Each case's reference positions start one
+  // past the previous case's last; the position between them holds the state
+  // "kept", so a run of removal candidates never continues across cases.
+  const int n = (int)tot;
+  std::vector<int32_t> type(n), index(n);
+  std::vector<uint32_t> seq(n);
+  std::vector<uint8_t> cid(n);
+  std::vector<double> p(n), rho(n), drho(n), rho_o(n);
+  std::vector<double> r(2 * (size_t)n), v(2 * (size_t)n), a(2 * (size_t)n), r_o(2 * (size_t)n),
+      v_o(2 * (size_t)n), gnr(any_gnr ? 2 * (size_t)n : 0), nrm(any_n ? 2 * (size_t)n : 0);
+  c->seq_base.assign(nc + 1, 0);
+  int o = 0;
+  for (int q = 0; q < nc; ++q) {
+    const tsph_particles &x = h[q];
+    const int m = x.np;
+    for (int i = 0; i < m; ++i, ++o) {
+      type[o] = x.type[i];
+      index[o] = x.index ? x.index[i] : i;
+      seq[o] = (uint32_t)(c->seq_base[q] + i);
+      cid[o] = (uint8_t)q;
+      rho[o] = x.rho[i];
+      p[o] = x.p ? x.p[i] : 0.;
+      drho[o] = x.drho ? x.drho[i] : 0.;
+      rho_o[o] = x.rho_o ? x.rho_o[i] : x.rho[i];
+      for (int u = 0; u < 2; ++u) {
+        const size_t g = 2 * (size_t)o + u, l = 2 * (size_t)i + u;
+        r[g] = x.r[l];
+        v[g] = x.v ? x.v[l] : 0.;
+        a[g] = x.a ? x.a[l] : 0.;
+        r_o[g] = x.r_o ? x.r_o[l] : x.r[l];
+        v_o[g] = x.v_o ? x.v_o[l] : v[g];
+        if (any_gnr) gnr[g] = x.gnr ? x.gnr[l] : 0.;
+        if (any_n) nrm[g] = x.n ? x.n[l] : 0.;
+      }
+    }
+    c->seq_base[q + 1] = c->seq_base[q] + m + 1;
+  }
+  tsph_particles all{};
+  all.np = n;
+  all.type = type.data();
+  all.index = index.data();
+  all.p = p.data();
+  all.rho = rho.data();
+  all.drho = drho.data();
+  all.r = r.data();
+  all.v = v.data();
+  all.a = a.data();
+  all.r_o = r_o.data();
+  all.v_o = v_o.data();
+  all.rho_o = rho_o.data();
+  all.gnr = any_gnr ? gnr.data() : nullptr;
+  all.n = any_n ? nrm.data() : nullptr;
+  if (int e = upload_impl(c, &all, seq.data(), cid.data(), c->seq_base[nc])) return e;
+  const uint8_t kept = 1;
+  for (int q = 1; q <= nc; ++q)
+    TSPH_HIP(hipMemcpy(c->seqstate + c->seq_base[q] - 1, &kept, 1, hipMemcpyHostToDevice));
+  return 0;
+}
+
+int tsph_batch_get_np(tsph_ctx *c, int *np) {
+  if (!c || !np) return fail(TSPH_EINVAL, "null argument");
+  if (!c->cases) return fail(TSPH_ESTATE, "not a batched context");
+  int n = 0;
+  TSPH_HIP(hipStreamSynchronize(c->stream));
+  TSPH_HIP(hipMemcpy(&n, &c->scal[S_NP], sizeof(int), hipMemcpyDeviceToHost));
+  std::vector<uint8_t> type(n), cid(n);
+  if (n > 0) {
+    TSPH_HIP(hipMemcpy(type.data(), c->cur.type, n, hipMemcpyDeviceToHost));
+    TSPH_HIP(hipMemcpy(cid.data(), c->cur.cid, n, hipMemcpyDeviceToHost));
+  }
+  std::fill(np, np + c->ncases, 0);
+  for (int i = 0; i < n; ++i)
+    if (type[i] != kRemoved) np[cid[i]]++;
+  return 0;
+}
+
+int tsph_batch_download(tsph_ctx *c, int icase, tsph_particles *h) {
+  if (!c || !h) return fail(TSPH_EINVAL, "null argument");
+  if (!c->cases) return fail(TSPH_ESTATE, "not a batched context");
+  if (icase < 0 || icase >= c->ncases) return fail(TSPH_EINVAL, "no such case");
+  return download_impl(c, h, icase);
+}
+
+int tsph_batch_step(tsph_ctx *c, int wall, int nsteps) {
+  if (!c) return fail(TSPH_EINVAL, "null argument");
+  if (!c->cases) return fail(TSPH_ESTATE, "not a batched context");
+  if (wall < TSPH_WALL_DBC || wall > TSPH_WALL_BT) return fail(TSPH_EINVAL, "unknown wall treatment");
+  if (!c->cells_valid) return fail(TSPH_ESTATE, "no cell list: call tsph_particle_to_cell first");
+  if (wall == TSPH_WALL_MDBC && !c->has_gnr) return fail(TSPH_ESTATE, "mDBC needs ghost nodes (gnr)");
+  if (wall == TSPH_WALL_BT && !c->has_n) return fail(TSPH_ESTATE, "BT needs wall normals (n)");
+  if (nsteps <= 0) return 0;
+  // each case advances by its own time step (its entry of the case table)
+  if (c->time_pairs) {
+    for (int s = 0; s < nsteps; ++s)
+      if (int e = record_step(c, wall, 0.)) return e;
+    return 0;
+  }
+  return replay_step(c, c->graph, 400 + wall, 0., nsteps, [&] { return record_step(c, wall, 0.); });
+}
+
+// ---- autotuner -----------------------------------------------------------------
+
+int tsph_autotune(tsph_ctx *c, int variant, int *tile_rows, int *waves_per_cu, double *best_ms) {
+  if (!c) return fail(TSPH_EINVAL, "null argument");
+  if (variant < TSPH_FORCES_WCSPH || variant > TSPH_FORCES_BT) return fail(TSPH_EINVAL, "unknown variant");
+  if (!c->cells_valid) return fail(TSPH_ESTATE, "no cell list: call tsph_particle_to_cell first");
+  if (variant == TSPH_FORCES_BT && !c->has_n) return fail(TSPH_ESTATE, "BT forces need wall normals (n)");
+  if (c->time_pairs) return fail(TSPH_ESTATE, "pair timing is active");
+  static const int kRows[] = {4, 5, 6, 7, 8, 10, 12};
+  static const int kWaves[] = {4, 6, 8, 12, 16};
+  constexpr int kReps = 3;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  TSPH_HIP(hipEventCreate(&e0));
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return fail(TSPH_EHIP, "hipEventCreate failed");
+  }
+  int best_r = c->tile_rows, best_w = c->waves_per_cu, e = 0;
+  float best = 1e30f;
+  // every candidate recomputes the same pair pass: its outputs a and drho are
+  // overwritten from the unchanged state
+  for (int r : kRows) {
+    if ((e = tsph_set_tile_rows(c, r))) break;
+    for (int w : kWaves) {
+      if ((e = tsph_set_waves_per_cu(c, w)) || (e = launch_forces(c, variant))) break;  // warm-up
+      if (hipEventRecord(e0, c->stream) != hipSuccess) { e = fail(TSPH_EHIP, "event record failed"); break; }
+      for (int k = 0; k < kReps && !e; ++k) e = launch_forces(c, variant);
+      if (e) break;
+      float ms = 0.f;
+      if (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+          hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+        e = fail(TSPH_EHIP, "event timing failed");
+        break;
+      }
+      if (ms < best) {
+        best = ms;
+        best_r = r;
+        best_w = w;
+      }
+    }
+    if (e) break;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (e) return e;
+  // a last pass with the chosen setting, so that a and drho are what a pair
+  // pass with it leaves (the mixed path rounds in a tile-relative frame)
+  if ((e = tsph_set_tile_rows(c, best_r)) || (e = tsph_set_waves_per_cu(c, best_w)) ||
+      (e = launch_forces(c, variant)))
+    return e;
+  if (tile_rows) *tile_rows = best_r;
+  if (waves_per_cu) *waves_per_cu = best_w;
+  if (best_ms) *best_ms = best / kReps;
   return 0;
 }
 

@@ -259,6 +259,67 @@ extern "C" int tsph_case_build(int preset, double dx, double jitter, uint64_t se
   }
 }
 
+namespace {
+
+// One tank of the batched ensemble: the dam-break set-up of
+// cases/damBreakLong_mDBC/geometry.h (mDBC floor, side walls and corners) in a
+// W x Ht tank, with either still water over the whole width or a dam-break
+// column half the tank wide.  Constants as the mDBC dam break: h = sqrt(2) dx,
+// kap = 2.1, c0 = 10 sqrt(g depth), dt = 0.2 h / c0.
+int build_tank(double W, double Ht, double depth, bool dam, double dx, uint64_t seed,
+               tsph_case **out) {
+  const double g = 9.81;
+  Spec s{};
+  s.dp = dx;
+  s.hh = 1.0 * std::sqrt(2 * s.dp * s.dp); s.kap = 2.1; s.cs = 10. * std::sqrt(g * depth);
+  s.visco = 0.02; s.delta = 0.1; s.dt = 0.2 * s.hh / s.cs;
+  s.x_0 = 0.; s.x_1 = W + 0.1; s.y_0 = 0.; s.y_1 = Ht;
+  s.steps = 500; s.wall = TSPH_WALL_MDBC;
+  Particle_system ps(s.hh, s.kap, s.cs, 1000., s.visco, s.delta, s.dp);
+  const double dp = s.dp;
+  Create_Wall_x_mdbc(ps, 0. + dp, W, 0., -1, dp / 2);
+  Create_Wall_y_mdbc(ps, 0., 0 + dp, Ht, -1, dp / 2);
+  Create_Wall_y_mdbc(ps, W, 0 + dp, Ht, +1, W - dp / 2);
+  Create_corner_mdbc(ps, -1, -1, {0. + dp / 2, 0. + dp / 2});
+  Create_corner_mdbc(ps, +1, -1, {W - dp / 2, 0. + dp / 2});
+  if (dam) Create_Block_of_Fluid(ps, 0 + dp, 0 + dp, 0.5 * W, depth);
+  else Create_Block_of_Fluid(ps, 0 + dp, 0 + dp, W - dp, depth);
+  jitter_fluid(ps, 0.01 * s.dp, seed);
+  return export_case(s, ps, out);
+}
+
+}  // namespace
+
+extern "C" int tsph_ensemble_build(int ncases, uint64_t seed, int nmin, int nmax,
+                                   tsph_case **cases) {
+  if (!cases || ncases < 1 || nmin < 1 || nmax < nmin) return TSPH_EINVAL;
+  for (int q = 0; q < ncases; ++q) cases[q] = nullptr;
+  SplitMix64 g{seed};
+  for (int q = 0; q < ncases; ++q) {
+    const double W = 0.5 + 2.5 * g.uniform();
+    const double fill = 0.3 + 0.5 * g.uniform();
+    double dx = 0.002 + 0.008 * g.uniform();
+    const bool dam = g.uniform() < 0.5;
+    const uint64_t jseed = g.next();
+    const double Ht = 2.0 * W;  // tank height rule: twice the width
+    const double depth = fill * Ht;
+    // fluid particles of the block on a dx lattice; dx is rescaled so that the
+    // count lies in [nmin, nmax]
+    const double area = (dam ? 0.5 * W : W - dx) * depth;
+    const double est = area / (dx * dx);
+    if (est > nmax) dx = std::sqrt(area / nmax);
+    else if (est < nmin) dx = std::sqrt(area / nmin);
+    if (int e = build_tank(W, Ht, depth, dam, dx, jseed, &cases[q])) {
+      for (int u = 0; u < q; ++u) {
+        tsph_case_free(cases[u]);
+        cases[u] = nullptr;
+      }
+      return e;
+    }
+  }
+  return 0;
+}
+
 extern "C" int tsph_case_free(tsph_case *c) {
   if (!c) return 0;
   tsph_particles &h = c->particles;

@@ -38,8 +38,9 @@ __device__ __forceinline__ unsigned long long lanemask_lt() {
 }
 
 __global__ void k_hash(const double2 *__restrict__ r, const uint8_t *__restrict__ type,
-                       const uint32_t *__restrict__ seq, const int *__restrict__ scal, Consts k,
-                       int *__restrict__ cell_count, int *__restrict__ hcell,
+                       const uint32_t *__restrict__ seq, const int *__restrict__ scal, Consts k0,
+                       const Consts *__restrict__ cases, const uint8_t *__restrict__ cid,
+                       int ncells_tot, int *__restrict__ cell_count, int *__restrict__ hcell,
                        int *__restrict__ hrank, uint8_t *__restrict__ seqstate,
                        unsigned long long *__restrict__ counters) {
   const int n = scal[S_NP];
@@ -48,9 +49,10 @@ __global__ void k_hash(const double2 *__restrict__ r, const uint8_t *__restrict_
   int c = -1;
   if (live) {
     if (type[i] == kRemoved) {
-      c = k.ncells;
+      c = ncells_tot;
       seqstate[seq[i]] = 0;  // its reference position is vacated
     } else {
+      const Consts k = case_consts(k0, cases, cid, i);
       const double2 ri = r[i];
       const int cx = cell_coord(ri.x, k.x0, k.of, k.kh);
       const int cy = cell_coord(ri.y, k.y0, k.of, k.kh);
@@ -60,6 +62,7 @@ __global__ void k_hash(const double2 *__restrict__ r, const uint8_t *__restrict_
         atomicAdd(&counters[C_OUTGRID], 1ull);
         c = c < 0 ? 0 : k.ncells - 1;
       }
+      c += k.cell_base;  // the case's segment of the global cell range
     }
   }
   // group the lanes by cell; the lowest lane of each group does the atomic
@@ -245,7 +248,8 @@ __global__ void k_order(const int *__restrict__ cell_start, int ncells,
 template <bool EOS>
 __global__ void k_reorder(const int *__restrict__ cell_start, int ncells, const int *__restrict__ perm,
                           Soa src, Soa dst, int *__restrict__ scal, int *__restrict__ wall_list,
-                          const int *__restrict__ cellof, Consts kc, Mixed mx,
+                          const int *__restrict__ cellof, Consts k0,
+                          const Consts *__restrict__ cases, Mixed mx,
                           unsigned long long *__restrict__ counters) {
   const int n_new = cell_start[ncells];
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -258,6 +262,7 @@ __global__ void k_reorder(const int *__restrict__ cell_start, int ncells, const 
     wall = t == TSPH_WALL;
     const double2 r = src.r[j], v = src.v[j];
     const double rho = src.rho[j];
+    if (src.cid) dst.cid[k] = src.cid[j];
     dst.r[k] = r;
     dst.v[k] = v;
     dst.a[k] = src.a[j];
@@ -270,6 +275,7 @@ __global__ void k_reorder(const int *__restrict__ cell_start, int ncells, const 
     dst.v_o[k] = src.v_o[j];
     dst.rho_o[k] = src.rho_o[j];
     double p = src.p[j];
+    const Consts kc = EOS ? case_consts(k0, cases, src.cid, j) : k0;
     if (EOS && t != TSPH_OUTLET && t != TSPH_INLET) {
       bad = rho > 1500. || rho < 0.;
       p = tait64(rho, kc.rho0, kc.cs);
@@ -281,7 +287,8 @@ __global__ void k_reorder(const int *__restrict__ cell_start, int ncells, const 
       nv = src.n[j];
       dst.n[k] = nv;
     }
-    if (EOS && mx.rv) write_mixed_record(mx, kc, cellof[k], k, r, v, rho, p, src.n ? &nv : nullptr);
+    if (EOS && mx.rv)
+      write_mixed_record(mx, kc, cellof[k] - kc.cell_base, k, r, v, rho, p, src.n ? &nv : nullptr);
     if (src.omega) {
       dst.omega[k] = src.omega[j];
       dst.omegarho[k] = src.omegarho[j];
@@ -329,21 +336,23 @@ int launch_cell_list(tsph_ctx *c, bool eos) {
   const int nb = 256;
   const int g = grid_for(c->n_ub, nb);
   hipStream_t s = c->stream;
+  const int nct = c->ncells_tot;
   hipLaunchKernelGGL(k_hash, dim3(g), dim3(nb), 0, s, c->cur.r, c->cur.type, c->cur.seq, c->scal, k,
-                     c->cell_count, c->hcell, c->hrank, c->seqstate, c->counters);
-  if (int e = scan_exclusive(c->cell_count, k.ncells + 1, c->cell_start, c->scan_tmp, c->scal, s))
+                     c->cases, c->cur.cid, nct, c->cell_count, c->hcell, c->hrank, c->seqstate,
+                     c->counters);
+  if (int e = scan_exclusive(c->cell_count, nct + 1, c->cell_start, c->scan_tmp, c->scal, s))
     return e;
   hipLaunchKernelGGL(k_scatter, dim3(g), dim3(nb), 0, s, c->scal, c->hcell, c->hrank,
                      c->cell_start, c->perm, c->cellof);
-  hipLaunchKernelGGL(k_order, dim3(grid_for(k.ncells, nb)), dim3(nb), 0, s, c->cell_start,
-                     k.ncells, c->cur.seq, c->perm);
+  hipLaunchKernelGGL(k_order, dim3(grid_for(nct, nb)), dim3(nb), 0, s, c->cell_start, nct,
+                     c->cur.seq, c->perm);
   const Mixed mx = c->prm.precision == TSPH_PREC_MIXED ? c->mx : Mixed{};
   if (eos)
-    hipLaunchKernelGGL(k_reorder<true>, dim3(g), dim3(nb), 0, s, c->cell_start, k.ncells, c->perm,
-                       c->cur, c->alt, c->scal, c->wall_list, c->cellof, k, mx, c->counters);
+    hipLaunchKernelGGL(k_reorder<true>, dim3(g), dim3(nb), 0, s, c->cell_start, nct, c->perm,
+                       c->cur, c->alt, c->scal, c->wall_list, c->cellof, k, c->cases, mx, c->counters);
   else
-    hipLaunchKernelGGL(k_reorder<false>, dim3(g), dim3(nb), 0, s, c->cell_start, k.ncells, c->perm,
-                       c->cur, c->alt, c->scal, c->wall_list, c->cellof, k, mx, c->counters);
+    hipLaunchKernelGGL(k_reorder<false>, dim3(g), dim3(nb), 0, s, c->cell_start, nct, c->perm,
+                       c->cur, c->alt, c->scal, c->wall_list, c->cellof, k, c->cases, mx, c->counters);
   TSPH_HIP(hipGetLastError());
   std::swap(c->cur, c->alt);
   c->parity ^= 1;

@@ -39,6 +39,7 @@ struct Zone {
 constexpr int kBzMax = 4096;  // zone particles an inlet pass may convert
 
 struct Soa {
+  uint8_t *cid = nullptr;  // case of each particle (batched ensembles only)
   double2 *r = nullptr, *v = nullptr, *a = nullptr, *r_o = nullptr, *v_o = nullptr;
   double2 *gnr = nullptr, *n = nullptr;
   double *rho = nullptr, *drho = nullptr, *p = nullptr, *rho_o = nullptr;
@@ -67,7 +68,17 @@ struct Consts {
   double kh, of;
   double awen, bwen;  // float-rounded Wendland constants, src/SPH_kernel.h:56-57
   int ncx, ncy, ncells;
+  // batched ensembles: the case's first global cell id and its time step
+  // (single cases: 0 and unused)
+  int cell_base;
+  double dt;
 };
+
+// constants of particle i: its case's entry of the batch table, or k0
+__device__ __forceinline__ Consts case_consts(const Consts &k0, const Consts *__restrict__ cases,
+                                              const uint8_t *__restrict__ cid, int i) {
+  return cases ? cases[cid[i]] : k0;
+}
 
 // host-side flags that describe what the device buffers hold
 struct HostState {
@@ -94,6 +105,15 @@ struct tsph_ctx {
   tsph::Consts k;
   int capacity = 0;
   int n_ub = 0;  // host upper bound of the device particle count
+  int ncells_tot = 0;            // cells of all cases (the single case: k.ncells)
+  // batched ensembles: per-case constants on the device (nullptr for one case)
+  int ncases = 1;
+  tsph::Consts *cases = nullptr;
+  std::vector<tsph::Consts> host_cases;
+  int *tile_base = nullptr;      // ncases+1 first global tile id per case (current tile_rows)
+  std::vector<int> host_tile_base;  // host copy of tile_base
+  std::vector<int> seq_base;     // first reference position per case (ncases+1)
+  std::vector<int> case_cap;     // particle slots reserved per case
   bool has_gnr = false, has_n = false;
   bool cells_valid = false;
   int tile_rows = 16;  // cells per pair-loop tile (one grid-column run)
@@ -150,6 +170,12 @@ int hip_fail(hipError_t e, const char *what, const char *file, int line);
   } while (0)
 
 inline int grid_for(int n, int block) { return n <= 0 ? 1 : (n + block - 1) / block; }
+
+// pair-loop tiles of the cell grid(s) at the current tile_rows
+inline int total_tiles(const tsph_ctx *c) {
+  if (!c->cases) return c->k.ncx * ((c->k.ncy + c->tile_rows - 1) / c->tile_rows);
+  return c->host_tile_base[c->ncases];
+}
 
 // tsph_cells.hip
 // hash, count, scan, scatter, order, reorder; eos: the reorder also runs

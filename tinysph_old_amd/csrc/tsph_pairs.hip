@@ -219,7 +219,7 @@ __device__ void particle64_global(const Soa &d, const int *__restrict__ cell_sta
   const int at = d.type[i];
   const double2 ar = d.r[i], av = d.v[i];
   const double arho = d.rho[i], ap = d.p[i];
-  const int c = cellof[i];
+  const int c = cellof[i] - k.cell_base;
   const int cx = c / k.ncy, cy = c - cx * k.ncy;
   Acc64 s;
   for (int q = 0; q < 9; ++q) {
@@ -252,7 +252,7 @@ __device__ void particle32_global(const Soa &d, const Mixed &mx, const int *__re
   const float4 ai = mx.rv[i];
   const float2 api = mx.rp[i];
   const float air = __builtin_amdgcn_rcpf(cc.rho0 + api.x);
-  const int c = cellof[i];
+  const int c = cellof[i] - k.cell_base;
   const int cx = c / k.ncy, cy = c - cx * k.ncy;
   Acc32 s;
   for (int dx = -1; dx <= 1; ++dx) {
@@ -292,8 +292,11 @@ __global__ void k_zero_tiles(int *scal) {
   if (threadIdx.x == 0 && blockIdx.x == 0) scal[S_NTILES] = 0;
 }
 
-// list of the non-empty tiles, in tile order within each workgroup's chunk
-__global__ __launch_bounds__(256) void k_tile_list(const int *__restrict__ cell_start, Consts k,
+// list of the non-empty tiles, in tile order within each workgroup's chunk;
+// for a batched ensemble the tile ids run over the cases (tile_base)
+__global__ __launch_bounds__(256) void k_tile_list(const int *__restrict__ cell_start, Consts k0,
+                                                   const Consts *__restrict__ cases,
+                                                   const int *__restrict__ tile_base, int ncases,
                                                    int rows, int ntiles, int *__restrict__ scal,
                                                    int *__restrict__ tile_list) {
   __shared__ int lds[4];
@@ -301,9 +304,17 @@ __global__ __launch_bounds__(256) void k_tile_list(const int *__restrict__ cell_
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   int flag = 0;
   if (t < ntiles) {
+    Consts k = k0;
+    int lt = t;
+    if (cases) {
+      const int cse = case_of_tile(tile_base, ncases, t);
+      lt = t - tile_base[cse];
+      k = cases[cse];
+    }
+    const int *cs = cell_start + k.cell_base;
     const int nty = (k.ncy + rows - 1) / rows;
-    const int cx = t / nty, cy0 = (t - cx * nty) * rows, cy1 = min(cy0 + rows, k.ncy);
-    flag = cell_start[cx * k.ncy + cy1] > cell_start[cx * k.ncy + cy0];
+    const int cx = lt / nty, cy0 = (lt - cx * nty) * rows, cy1 = min(cy0 + rows, k.ncy);
+    flag = cs[cx * k.ncy + cy1] > cs[cx * k.ncy + cy0];
   }
   // block-wide exclusive scan of the flags (4 waves)
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -327,8 +338,10 @@ __global__ __launch_bounds__(256) void k_tile_list(const int *__restrict__ cell_
 // --------------------------------------------------------------------------
 template <int VAR>
 __global__ __launch_bounds__(kWave) void k_tile64(Soa d, const int *__restrict__ cell_start,
-                                                  const int *__restrict__ cellof, Consts k, int rows,
-                                                  const int *__restrict__ scal,
+                                                  const int *__restrict__ cellof, Consts k0,
+                                                  const Consts *__restrict__ cases,
+                                                  const int *__restrict__ tile_base, int ncases,
+                                                  int rows, const int *__restrict__ scal,
                                                   const int *__restrict__ tile_list, int cap,
                                                   unsigned long long *__restrict__ counters) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -346,13 +359,22 @@ __global__ __launch_bounds__(kWave) void k_tile64(Soa d, const int *__restrict__
   band_of_wave(scal[S_NTILES], it, it_end, it_step);
   for (; it < it_end; it += it_step) {
   __syncthreads();  // the previous tile's LDS is no longer read
+  // the tile's case: its constants and its segment of the cell range
+  int tile = tile_list[it];
+  Consts k = k0;
+  if (cases) {
+    const int cse = case_of_tile(tile_base, ncases, tile);
+    tile -= tile_base[cse];
+    k = cases[cse];
+  }
+  const int *cs = cell_start + k.cell_base;
   Tile t;
-  tile_begin(k, cell_start, tile_list[it], rows, tab, t);
+  tile_begin(k, cs, tile, rows, tab, t);
   if (t.lq[3] > cap) {
     for (int i = t.own0 + threadIdx.x; i < t.own1; i += blockDim.x) {
       if (!is_active<VAR>(d.type[i])) continue;
       ++nact;
-      particle64_global<VAR>(d, cell_start, cellof, k, i, npair, ncand);
+      particle64_global<VAR>(d, cs, cellof, k, i, npair, ncand);
     }
     continue;
   }
@@ -376,7 +398,7 @@ __global__ __launch_bounds__(kWave) void k_tile64(Soa d, const int *__restrict__
       li = i + t.base[1];
       at = s_t[li];
       act = is_active<VAR>(at);
-      cy = cellof[i] - t.cx * k.ncy;
+      cy = cellof[i] - k.cell_base - t.cx * k.ncy;
     }
     nact += act;
     const double2 ar = s_r[li], av = s_v[li], arp = s_rp[li];
@@ -432,8 +454,10 @@ __global__ __launch_bounds__(kWave) void k_tile64(Soa d, const int *__restrict__
 // --------------------------------------------------------------------------
 template <int VAR>
 __global__ __launch_bounds__(kWave) void k_tile32(Soa d, Mixed mx, const int *__restrict__ cell_start,
-                                                  const int *__restrict__ cellof, Consts k, int rows,
-                                                  const int *__restrict__ scal,
+                                                  const int *__restrict__ cellof, Consts k0,
+                                                  const Consts *__restrict__ cases,
+                                                  const int *__restrict__ tile_base, int ncases,
+                                                  int rows, const int *__restrict__ scal,
                                                   const int *__restrict__ tile_list, int cap,
                                                   unsigned long long *__restrict__ counters) {
 #pragma clang fp contract(fast)
@@ -448,19 +472,28 @@ __global__ __launch_bounds__(kWave) void k_tile32(Soa d, Mixed mx, const int *__
   uint8_t *s_t = reinterpret_cast<uint8_t *>(s_list + kList * kWave);
 
   if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&counters[C_PASSES], 1ull);
-  const C32 cc = make_c32(k);
   unsigned npair = 0, ncand = 0, nact = 0;
   int it, it_end, it_step;
   band_of_wave(scal[S_NTILES], it, it_end, it_step);
   for (; it < it_end; it += it_step) {
   __syncthreads();  // the previous tile's LDS is no longer read
+  // the tile's case: its constants and its segment of the cell range
+  int tile = tile_list[it];
+  Consts k = k0;
+  if (cases) {
+    const int cse = case_of_tile(tile_base, ncases, tile);
+    tile -= tile_base[cse];
+    k = cases[cse];
+  }
+  const int *cs = cell_start + k.cell_base;
+  const C32 cc = make_c32(k);
   Tile t;
-  tile_begin(k, cell_start, tile_list[it], rows, tab, t);
+  tile_begin(k, cs, tile, rows, tab, t);
   if (t.lq[3] > cap) {
     for (int i = t.own0 + threadIdx.x; i < t.own1; i += blockDim.x) {
       if (!is_active<VAR>(d.type[i])) continue;
       ++nact;
-      particle32_global<VAR>(d, mx, cell_start, cellof, k, cc, i, npair, ncand);
+      particle32_global<VAR>(d, mx, cs, cellof, k, cc, i, npair, ncand);
     }
     continue;
   }
@@ -470,7 +503,7 @@ __global__ __launch_bounds__(kWave) void k_tile32(Soa d, Mixed mx, const int *__
     const int j = l - sel3(t.base, q);
     const float4 rv = mx.rv[j];
     const float2 rp = mx.rp[j];
-    const int row = cellof[j] - (t.cx + q - 1) * k.ncy;
+    const int row = cellof[j] - k.cell_base - (t.cx + q - 1) * k.ncy;
     s_pos[l] = make_float2(rv.x + (float)(q - 1) * cc.kh, rv.y + (float)(row - mid) * cc.kh);
     s_vel[l] = make_float2(rv.z, rv.w);
     s_dp[l] = rp;
@@ -490,7 +523,7 @@ __global__ __launch_bounds__(kWave) void k_tile32(Soa d, Mixed mx, const int *__
       li = i + t.base[1];
       at = s_t[li];
       act = is_active<VAR>(at);
-      ri = cellof[i] - t.cx * k.ncy - t.row_lo;
+      ri = cellof[i] - k.cell_base - t.cx * k.ncy - t.row_lo;
     }
     nact += act;
     const float2 pi = s_pos[li], vi = s_vel[li], dpi = s_dp[li];
@@ -579,11 +612,11 @@ __global__ __launch_bounds__(kWave) void k_tile32(Soa d, Mixed mx, const int *__
 }
 
 __global__ void k_mixed_records(Soa d, const int *__restrict__ cellof, const int *__restrict__ scal,
-                                Consts k, Mixed mx) {
+                                Consts k0, const Consts *__restrict__ cases, Mixed mx) {
   const int n = scal[S_NP];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  write_mixed_record(d, mx, cellof, k, i, d.rho[i], d.p[i]);
+  write_mixed_record(d, mx, cellof, case_consts(k0, cases, d.cid, i), i, d.rho[i], d.p[i]);
 }
 
 
@@ -592,7 +625,7 @@ __global__ void k_mixed_records(Soa d, const int *__restrict__ cellof, const int
 int launch_mixed_records(tsph_ctx *c) {
   const int nb = 256;
   hipLaunchKernelGGL(k_mixed_records, dim3(grid_for(c->n_ub, nb)), dim3(nb), 0, c->stream, c->cur,
-                     c->cellof, c->scal, c->k, c->mx);
+                     c->cellof, c->scal, c->k, c->cases, c->mx);
   TSPH_HIP(hipGetLastError());
   return 0;
 }
@@ -600,14 +633,14 @@ int launch_mixed_records(tsph_ctx *c) {
 int ensure_tile_list(tsph_ctx *c) {
   const Consts &k = c->k;
   const int rows = c->tile_rows;
-  const int ntiles = k.ncx * ((k.ncy + rows - 1) / rows);
+  const int ntiles = total_tiles(c);
   hipStream_t s = c->stream;
   if (!c->tiles_valid) {
     if (!c->tiles_zeroed) {
       hipLaunchKernelGGL(k_zero_tiles, dim3(1), dim3(64), 0, s, c->scal);
     }
     hipLaunchKernelGGL(k_tile_list, dim3(grid_for(ntiles, 256)), dim3(256), 0, s, c->cell_start, k,
-                       rows, ntiles, c->scal, c->tile_list);
+                       c->cases, c->tile_base, c->ncases, rows, ntiles, c->scal, c->tile_list);
     TSPH_HIP(hipGetLastError());
     c->tiles_valid = true;
     c->tiles_zeroed = false;
@@ -636,15 +669,18 @@ int launch_forces(tsph_ctx *c, int variant) {
       switch (variant) {
         case TSPH_FORCES_WCSPH:
           hipLaunchKernelGGL(k_tile32<TSPH_FORCES_WCSPH>, g, b, lds, s, c->cur, c->mx, c->cell_start,
-                             c->cellof, k, rows, sc, tl, cap, c->counters);
+                             c->cellof, k, c->cases, c->tile_base, c->ncases, rows, sc, tl, cap,
+                             c->counters);
           break;
         case TSPH_FORCES_DBC:
           hipLaunchKernelGGL(k_tile32<TSPH_FORCES_DBC>, g, b, lds, s, c->cur, c->mx, c->cell_start,
-                             c->cellof, k, rows, sc, tl, cap, c->counters);
+                             c->cellof, k, c->cases, c->tile_base, c->ncases, rows, sc, tl, cap,
+                             c->counters);
           break;
         default:
           hipLaunchKernelGGL(k_tile32<TSPH_FORCES_BT>, g, b, lds, s, c->cur, c->mx, c->cell_start,
-                             c->cellof, k, rows, sc, tl, cap, c->counters);
+                             c->cellof, k, c->cases, c->tile_base, c->ncases, rows, sc, tl, cap,
+                             c->counters);
       }
     });
   }
@@ -653,15 +689,15 @@ int launch_forces(tsph_ctx *c, int variant) {
     switch (variant) {
       case TSPH_FORCES_WCSPH:
         hipLaunchKernelGGL(k_tile64<TSPH_FORCES_WCSPH>, g, b, lds, s, c->cur, c->cell_start, c->cellof,
-                           k, rows, sc, tl, cap, c->counters);
+                           k, c->cases, c->tile_base, c->ncases, rows, sc, tl, cap, c->counters);
         break;
       case TSPH_FORCES_DBC:
         hipLaunchKernelGGL(k_tile64<TSPH_FORCES_DBC>, g, b, lds, s, c->cur, c->cell_start, c->cellof,
-                           k, rows, sc, tl, cap, c->counters);
+                           k, c->cases, c->tile_base, c->ncases, rows, sc, tl, cap, c->counters);
         break;
       default:
         hipLaunchKernelGGL(k_tile64<TSPH_FORCES_BT>, g, b, lds, s, c->cur, c->cell_start, c->cellof,
-                           k, rows, sc, tl, cap, c->counters);
+                           k, c->cases, c->tile_base, c->ncases, rows, sc, tl, cap, c->counters);
     }
   });
 }

@@ -112,7 +112,8 @@ __device__ __forceinline__ void write_mixed_record(const Mixed &mx, const Consts
 __device__ __forceinline__ void write_mixed_record(const Soa &d, const Mixed &mx,
                                                    const int *__restrict__ cellof, const Consts &k,
                                                    int i, double rho, double p) {
-  write_mixed_record(mx, k, cellof[i], i, d.r[i], d.v[i], rho, p, d.n ? &d.n[i] : nullptr);
+  write_mixed_record(mx, k, cellof[i] - k.cell_base, i, d.r[i], d.v[i], rho, p,
+                     d.n ? &d.n[i] : nullptr);
 }
 
 __device__ __forceinline__ float hydro_drho_f32(float x, float rho0) {

@@ -25,11 +25,13 @@ namespace {
 // EOS; on the mixed path it also writes the FP32 pair records of every
 // particle (mx.rv != nullptr), which the pair loop that follows reads
 __global__ void k_eos(Soa d, Mixed mx, const int *__restrict__ cellof, const int *__restrict__ scal,
-                      Consts k, unsigned long long *__restrict__ counters) {
+                      Consts k0, const Consts *__restrict__ cases,
+                      unsigned long long *__restrict__ counters) {
   const int n = scal[S_NP];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   unsigned bad = 0;
   if (i < n) {
+    const Consts k = case_consts(k0, cases, d.cid, i);
     const int t = d.type[i];
     const double rho = d.rho[i];
     double p;
@@ -46,10 +48,12 @@ __global__ void k_eos(Soa d, Mixed mx, const int *__restrict__ cellof, const int
   if (w && (threadIdx.x & 63) == 0) atomicAdd(&counters[C_BADRHO], (unsigned long long)__popcll(w));
 }
 
-__global__ void k_predictor(Soa d, const int *__restrict__ scal, double dt) {
+__global__ void k_predictor(Soa d, const int *__restrict__ scal, double dt0,
+                            const Consts *__restrict__ cases) {
   const int n = scal[S_NP];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  const double dt = cases ? cases[d.cid[i]].dt : dt0;
   const double rho = d.rho[i];
   const double2 r = d.r[i], v = d.v[i], a = d.a[i];
   d.rho_o[i] = rho;
@@ -71,11 +75,14 @@ __device__ __forceinline__ uint8_t removal_state(const Consts &k, int type, doub
 
 // corrector; also records the removal state for the RemoveParticlesOutOfDomain
 // that follows it in every driver
-__global__ void k_corrector(Soa d, const int *__restrict__ scal, Consts k, double dt,
+__global__ void k_corrector(Soa d, const int *__restrict__ scal, Consts k0,
+                            const Consts *__restrict__ cases, double dt0,
                             uint8_t *__restrict__ state) {
   const int n = scal[S_NP];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  const Consts k = case_consts(k0, cases, d.cid, i);
+  const double dt = cases ? k.dt : dt0;
   const double e = (-d.drho[i] / d.rho[i]) * dt;
   d.rho[i] = d.rho_o[i] * ((2. - e) / (2. + e));
   const double2 vo = d.v_o[i], a = d.a[i], ro = d.r_o[i];
@@ -135,16 +142,18 @@ __device__ __forceinline__ void wall_sums(const Soa &d, const int *__restrict__ 
 }
 
 __global__ __launch_bounds__(64) void k_mdbc(Soa d, const int *__restrict__ cell_start, const int *__restrict__ scal,
-                       const int *__restrict__ wall_list, Consts k) {
+                       const int *__restrict__ wall_list, Consts k0,
+                       const Consts *__restrict__ cases) {
   const int w = blockIdx.x * blockDim.x + threadIdx.x;
   if (w >= scal[S_NWALL]) return;
   const int i = wall_list[w];
+  const Consts k = case_consts(k0, cases, d.cid, i);
   const double2 gn = d.gnr[i], r = d.r[i];
   const double m = k.m;
   // sums of src/SPH_kernel_approx.h:1141-1342 in its order:
   // rhowsum, drhodx, drhody, a11, a12, a13, a21, a22, a23, a31, a32, a33
   double acc[12];
-  wall_sums<12>(d, cell_start, k, gn, acc,
+  wall_sums<12>(d, cell_start + k.cell_base, k, gn, acc,
                 [&](double nrho, double drx, double dry, double drs, double (&a)[12]) {
                   const KW64 kw = wendland64(drs, k.h, k.awen, k.bwen);
                   const double mr = m / nrho;
@@ -191,13 +200,15 @@ __global__ __launch_bounds__(64) void k_mdbc(Soa d, const int *__restrict__ cell
 }
 
 __global__ __launch_bounds__(64) void k_bt_density(Soa d, const int *__restrict__ cell_start, const int *__restrict__ scal,
-                             const int *__restrict__ wall_list, Consts k) {
+                             const int *__restrict__ wall_list, Consts k0,
+                             const Consts *__restrict__ cases) {
   const int w = blockIdx.x * blockDim.x + threadIdx.x;
   if (w >= scal[S_NWALL]) return;
   const int i = wall_list[w];
+  const Consts k = case_consts(k0, cases, d.cid, i);
   const double m = k.m;
   double acc[2];  // wsum, rhowsum (src/SPH_kernel_approx.h:1838-1953)
-  wall_sums<2>(d, cell_start, k, d.r[i], acc,
+  wall_sums<2>(d, cell_start + k.cell_base, k, d.r[i], acc,
                [&](double nrho, double, double, double drs, double (&a)[2]) {
                  const KW64 kw = wendland64(drs, k.h, k.awen, k.bwen);
                  a[0] += kw.w * m / nrho;
@@ -214,12 +225,12 @@ __global__ __launch_bounds__(64) void k_bt_density(Soa d, const int *__restrict_
 // ...) is removed.  State per reference position (seq): 0 absent, 1 kept,
 // 2 candidate.
 // marks for a removal that does not directly follow a corrector
-__global__ void k_rm_mark(Soa d, const int *__restrict__ scal, Consts k,
-                          uint8_t *__restrict__ state) {
+__global__ void k_rm_mark(Soa d, const int *__restrict__ scal, Consts k0,
+                          const Consts *__restrict__ cases, uint8_t *__restrict__ state) {
   const int n = scal[S_NP];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  state[d.seq[i]] = removal_state(k, d.type[i], d.r[i]);
+  state[d.seq[i]] = removal_state(case_consts(k0, cases, d.cid, i), d.type[i], d.r[i]);
 }
 
 __global__ void k_rm_decide(Soa d, const int *__restrict__ scal, const uint8_t *__restrict__ state,
@@ -353,7 +364,7 @@ __global__ void k_interp_point(Soa d, const int *__restrict__ cell_start, Consts
 int launch_eos(tsph_ctx *c) {
   Mixed mx = c->prm.precision == TSPH_PREC_MIXED ? c->mx : Mixed{};
   hipLaunchKernelGGL(k_eos, dim3(grid_for(c->n_ub, 256)), dim3(256), 0, c->stream, c->cur, mx,
-                     c->cellof, c->scal, c->k, c->counters);
+                     c->cellof, c->scal, c->k, c->cases, c->counters);
   TSPH_HIP(hipGetLastError());
   // the records are only meaningful for the cell list they were built on
   c->records_valid = c->prm.precision == TSPH_PREC_MIXED && c->cells_valid;
@@ -362,7 +373,7 @@ int launch_eos(tsph_ctx *c) {
 
 int launch_predictor(tsph_ctx *c, double dt) {
   hipLaunchKernelGGL(k_predictor, dim3(grid_for(c->n_ub, 256)), dim3(256), 0, c->stream, c->cur,
-                     c->scal, dt);
+                     c->scal, dt, c->cases);
   TSPH_HIP(hipGetLastError());
   c->records_valid = false;
   c->marks_valid = false;
@@ -371,7 +382,7 @@ int launch_predictor(tsph_ctx *c, double dt) {
 
 int launch_corrector(tsph_ctx *c, double dt) {
   hipLaunchKernelGGL(k_corrector, dim3(grid_for(c->n_ub, 256)), dim3(256), 0, c->stream, c->cur,
-                     c->scal, c->k, dt, c->seqstate);
+                     c->scal, c->k, c->cases, dt, c->seqstate);
   TSPH_HIP(hipGetLastError());
   c->records_valid = false;
   c->marks_valid = true;
@@ -380,7 +391,7 @@ int launch_corrector(tsph_ctx *c, double dt) {
 
 int launch_mdbc(tsph_ctx *c) {
   hipLaunchKernelGGL(k_mdbc, dim3(grid_for(c->n_wall, 64)), dim3(64), 0, c->stream, c->cur,
-                     c->cell_start, c->scal, c->wall_list, c->k);
+                     c->cell_start, c->scal, c->wall_list, c->k, c->cases);
   TSPH_HIP(hipGetLastError());
   c->records_valid = false;
   return 0;
@@ -388,7 +399,7 @@ int launch_mdbc(tsph_ctx *c) {
 
 int launch_bt_density(tsph_ctx *c) {
   hipLaunchKernelGGL(k_bt_density, dim3(grid_for(c->n_wall, 64)), dim3(64), 0, c->stream, c->cur,
-                     c->cell_start, c->scal, c->wall_list, c->k);
+                     c->cell_start, c->scal, c->wall_list, c->k, c->cases);
   TSPH_HIP(hipGetLastError());
   c->records_valid = false;
   return 0;
@@ -397,7 +408,7 @@ int launch_bt_density(tsph_ctx *c) {
 int launch_remove(tsph_ctx *c) {
   if (!c->marks_valid) {
     hipLaunchKernelGGL(k_rm_mark, dim3(grid_for(c->n_ub, 256)), dim3(256), 0, c->stream, c->cur,
-                       c->scal, c->k, c->seqstate);
+                       c->scal, c->k, c->cases, c->seqstate);
   }
   hipLaunchKernelGGL(k_rm_decide, dim3(grid_for(c->n_ub, 256)), dim3(256), 0, c->stream, c->cur,
                      c->scal, c->seqstate, c->counters);

@@ -105,6 +105,17 @@ __device__ __forceinline__ bool tile_begin(const Consts &k, const int *__restric
   return t.own1 > t.own0;
 }
 
+// case of global tile t in a batched ensemble (tile_base: ncases+1 bounds)
+__device__ __forceinline__ int case_of_tile(const int *__restrict__ tile_base, int ncases, int t) {
+  int lo = 0, hi = ncases;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (tile_base[mid] <= t) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
 __device__ __forceinline__ int slice_of(const Tile &t, int l) {
   return l >= t.lq[2] ? 2 : (l >= t.lq[1] ? 1 : 0);
 }

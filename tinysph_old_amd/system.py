@@ -39,42 +39,110 @@ class CaseData:
                 p["x_0"], p["x_m"], p["y_0"], p["y_m"]]
 
 
+def _case_from_c(c) -> CaseData:
+    prm = {n: getattr(c.prm, n) for n, _ in L.Params._fields_}
+    h = c.particles
+    n = h.np
+
+    def arr(ptr, count, dtype):
+        if not ptr:
+            return None
+        return np.ctypeslib.as_array(ptr, shape=(count,)).astype(dtype, copy=True)
+
+    arrays = {
+        "type": arr(h.type, n, np.int32),
+        "index": arr(h.index, n, np.int32),
+        "p": arr(h.p, n, np.float64),
+        "rho": arr(h.rho, n, np.float64),
+        "r": arr(h.r, 2 * n, np.float64).reshape(n, 2),
+        "v": arr(h.v, 2 * n, np.float64).reshape(n, 2),
+    }
+    if h.gnr:
+        arrays["gnr"] = arr(h.gnr, 2 * n, np.float64).reshape(n, 2)
+    if h.n:
+        arrays["n"] = arr(h.n, 2 * n, np.float64).reshape(n, 2)
+    zones = [{n: getattr(c.zones[i], n) for n, _ in L.Zone._fields_} for i in range(c.n_zones)]
+    return CaseData(prm, c.dt, c.wall, c.steps, c.n_fluid, c.n_wall, arrays, c.scheme, zones)
+
+
 def build_case(preset: int, dx: float = 0.0, jitter: float = -1.0, seed: int = 12345) -> CaseData:
     lib = L.lib()
     out = C.POINTER(L.CaseC)()
     L.check(lib.tsph_case_build(preset, float(dx), float(jitter), int(seed), C.byref(out)),
             "tsph_case_build")
     try:
-        c = out.contents
-        prm = {n: getattr(c.prm, n) for n, _ in L.Params._fields_}
-        h = c.particles
-        n = h.np
-
-        def arr(ptr, count, dtype):
-            if not ptr:
-                return None
-            return np.ctypeslib.as_array(ptr, shape=(count,)).astype(dtype, copy=True)
-
-        arrays = {
-            "type": arr(h.type, n, np.int32),
-            "index": arr(h.index, n, np.int32),
-            "p": arr(h.p, n, np.float64),
-            "rho": arr(h.rho, n, np.float64),
-            "r": arr(h.r, 2 * n, np.float64).reshape(n, 2),
-            "v": arr(h.v, 2 * n, np.float64).reshape(n, 2),
-        }
-        if h.gnr:
-            arrays["gnr"] = arr(h.gnr, 2 * n, np.float64).reshape(n, 2)
-        if h.n:
-            arrays["n"] = arr(h.n, 2 * n, np.float64).reshape(n, 2)
-        zones = [{n: getattr(c.zones[i], n) for n, _ in L.Zone._fields_} for i in range(c.n_zones)]
-        return CaseData(prm, c.dt, c.wall, c.steps, c.n_fluid, c.n_wall, arrays, c.scheme, zones)
+        return _case_from_c(out.contents)
     finally:
         lib.tsph_case_free(out)
 
 
+def build_ensemble(ncases: int, seed: int, nmin: int = 2000, nmax: int = 400000) -> list:
+    """The batched-ensemble workload (tsph_ensemble_build): ncases tanks drawn
+    from seed, each holding nmin..nmax fluid particles."""
+    lib = L.lib()
+    arr = (C.POINTER(L.CaseC) * ncases)()
+    L.check(lib.tsph_ensemble_build(int(ncases), int(seed), int(nmin), int(nmax), arr),
+            "tsph_ensemble_build")
+    try:
+        return [_case_from_c(arr[q].contents) for q in range(ncases)]
+    finally:
+        for q in range(ncases):
+            if arr[q]:
+                lib.tsph_case_free(arr[q])
+
+
 VEC = ("r", "v", "a", "r_o", "v_o", "gnr", "n")
 SCAL = ("p", "rho", "drho", "rho_o")
+
+
+DOWNLOAD_FIELDS = ("type", "index", "p", "rho", "drho", "r", "v", "a", "r_o", "v_o", "rho_o")
+
+
+def host_particles(arrays: dict):
+    """tsph_particles over the numpy arrays of a case (plus the arrays it keeps alive)."""
+    n = len(arrays["type"])
+    keep = {}
+    h = L.Particles()
+    h.np = n
+    t = np.ascontiguousarray(arrays["type"], dtype=np.int32)
+    keep["type"] = t
+    h.type = L.iptr(t)
+    if arrays.get("index") is not None:
+        ix = np.ascontiguousarray(arrays["index"], dtype=np.int32)
+        keep["index"] = ix
+        h.index = L.iptr(ix)
+    for k in VEC + SCAL:
+        a = arrays.get(k)
+        if a is None:
+            continue
+        a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+        keep[k] = a
+        setattr(h, k, L.dptr(a))
+    return h, keep
+
+
+def download_buffers(n: int, fields):
+    out = {}
+    h = L.Particles()
+    for k in fields:
+        if k in ("type", "index"):
+            a = np.zeros(max(n, 1), dtype=np.int32)
+            setattr(h, k, L.iptr(a))
+        elif k in VEC:
+            a = np.zeros(max(2 * n, 1), dtype=np.float64)
+            setattr(h, k, L.dptr(a))
+        else:
+            a = np.zeros(max(n, 1), dtype=np.float64)
+            setattr(h, k, L.dptr(a))
+        out[k] = a
+    return h, out
+
+
+def trim_download(h, out: dict) -> dict:
+    m = h.np
+    for k in list(out):
+        out[k] = out[k][: 2 * m].reshape(m, 2) if k in VEC else out[k][:m]
+    return out
 
 
 class ParticleSystem:
@@ -112,47 +180,13 @@ class ParticleSystem:
 
     # ---- transfer ----
     def upload(self, arrays: dict) -> None:
-        n = len(arrays["type"])
-        keep = {}
-        h = L.Particles()
-        h.np = n
-        t = np.ascontiguousarray(arrays["type"], dtype=np.int32)
-        keep["type"] = t
-        h.type = L.iptr(t)
-        if arrays.get("index") is not None:
-            ix = np.ascontiguousarray(arrays["index"], dtype=np.int32)
-            keep["index"] = ix
-            h.index = L.iptr(ix)
-        for k in VEC + SCAL:
-            a = arrays.get(k)
-            if a is None:
-                continue
-            a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
-            keep[k] = a
-            setattr(h, k, L.dptr(a))
+        h, keep = host_particles(arrays)
         L.check(self.lib.tsph_upload(self.ctx, C.byref(h)), "tsph_upload")
 
-    def download(self, fields=("type", "index", "p", "rho", "drho", "r", "v", "a", "r_o", "v_o",
-                               "rho_o")) -> dict:
-        n = self.np_upper()
-        out = {}
-        h = L.Particles()
-        for k in fields:
-            if k in ("type", "index"):
-                a = np.zeros(max(n, 1), dtype=np.int32)
-                setattr(h, k, L.iptr(a))
-            elif k in VEC:
-                a = np.zeros(max(2 * n, 1), dtype=np.float64)
-                setattr(h, k, L.dptr(a))
-            else:
-                a = np.zeros(max(n, 1), dtype=np.float64)
-                setattr(h, k, L.dptr(a))
-            out[k] = a
+    def download(self, fields=DOWNLOAD_FIELDS) -> dict:
+        h, out = download_buffers(self.np_upper(), fields)
         L.check(self.lib.tsph_download(self.ctx, C.byref(h)), "tsph_download")
-        m = h.np
-        for k in list(out):
-            out[k] = out[k][: 2 * m].reshape(m, 2) if k in VEC else out[k][:m]
-        return out
+        return trim_download(h, out)
 
     def np_upper(self) -> int:
         return self.np
@@ -354,3 +388,70 @@ def system_for_case(case: CaseData, precision: int = L.PREC_FP64, capacity: int 
 
 def forces_variant(wall: int) -> int:
     return {L.WALL_DBC: L.FORCES_DBC, L.WALL_MDBC: L.FORCES_WCSPH, L.WALL_BT: L.FORCES_BT}[wall]
+
+
+class BatchSystem(ParticleSystem):
+    """A batched ensemble: many independent cases advanced together, one launch
+    per stage (tsph_batch_*).  The stage methods of ParticleSystem that act on
+    all cases (particle_to_cell, density_to_pressure, compute_forces,
+    mdbc_density, bt_density, remove_out_of_domain, counters, tuning) apply."""
+
+    def __init__(self, cases: list, precision: int = L.PREC_FP64, capacity: list | None = None):
+        self.lib = L.lib()
+        n = len(cases)
+        prms = (L.Params * n)()
+        for q, case in enumerate(cases):
+            for name, _ in L.Params._fields_:
+                if name in case.params and name not in ("precision", "capacity"):
+                    setattr(prms[q], name, float(case.params[name]))
+            prms[q].precision = precision
+            prms[q].capacity = int(capacity[q] if capacity is not None else case.np)
+        dts = np.ascontiguousarray([c.dt for c in cases], dtype=np.float64)
+        self.ncases = n
+        self.capacity = sum(int(prms[q].capacity) for q in range(n))
+        self.precision = precision
+        self.ctx = C.c_void_p()
+        L.check(self.lib.tsph_batch_create(prms, L.dptr(dts), n, C.byref(self.ctx)),
+                "tsph_batch_create")
+
+    def upload(self, arrays_per_case: list) -> None:
+        hs = (L.Particles * self.ncases)()
+        keep = []
+        for q, arrays in enumerate(arrays_per_case):
+            h, k = host_particles(arrays)
+            hs[q] = h
+            keep.append(k)
+        L.check(self.lib.tsph_batch_upload(self.ctx, hs), "tsph_batch_upload")
+
+    def np_per_case(self) -> np.ndarray:
+        out = np.zeros(self.ncases, dtype=np.int32)
+        L.check(self.lib.tsph_batch_get_np(self.ctx, L.iptr(out)), "tsph_batch_get_np")
+        return out
+
+    def download_case(self, q: int, fields=DOWNLOAD_FIELDS) -> dict:
+        h, out = download_buffers(int(self.np_per_case()[q]), fields)
+        L.check(self.lib.tsph_batch_download(self.ctx, int(q), C.byref(h)), "tsph_batch_download")
+        return trim_download(h, out)
+
+    def grid_of(self, q: int):
+        x, y, m = C.c_int(), C.c_int(), C.c_double()
+        L.check(self.lib.tsph_batch_grid(self.ctx, int(q), C.byref(x), C.byref(y), C.byref(m)),
+                "tsph_batch_grid")
+        return x.value, y.value, m.value
+
+    def step(self, wall: int, nsteps: int = 1):
+        L.check(self.lib.tsph_batch_step(self.ctx, wall, int(nsteps)), "tsph_batch_step")
+
+
+def batch_for_cases(cases: list, precision: int = L.PREC_FP64) -> BatchSystem:
+    bs = BatchSystem(cases, precision=precision)
+    bs.upload([c.arrays for c in cases])
+    return bs
+
+
+def autotune(ps: ParticleSystem, variant: int):
+    """tsph_autotune: (tile_rows, waves_per_cu, ms per pair pass) of the fastest setting."""
+    r, w, ms = C.c_int(), C.c_int(), C.c_double()
+    L.check(ps.lib.tsph_autotune(ps.ctx, variant, C.byref(r), C.byref(w), C.byref(ms)),
+            "tsph_autotune")
+    return r.value, w.value, ms.value
