@@ -7,6 +7,7 @@ the cancellation in p = B((rho/rho0)^7 - 1).  The stated bounds are:
   FP64 pair loop, one evaluation  : rel. error <= 1e-9 on a and dRho
   FP64, 20 symplectic steps       : rel. error <= 1e-9 on r, v, rho; <= 1e-6 on p
   mixed (FP32 pairs), one pass    : rel. L2 <= 1e-4 on a and dRho
+  mixed, 1000 symplectic steps    : rel. L2 <= 1e-4 on rho (FP64, same run: <= 1e-8)
 """
 import math
 
@@ -268,3 +269,29 @@ def test_remove_out_of_domain_alternation():
     assert np.array_equal(got["r"], O.get(po.F_R))
     s_o, it_o = O.cell_csr()
     assert len(got["r"]) == O.np
+
+
+def test_mixed_density_error_after_1000_steps():
+    """The mixed path's density after 1000 dam-break steps against the oracle
+    (dx = 4 mm, 46k particles, jittered lattice): relative L2 <= 1e-4."""
+    case = T.build_case(T.CASE_DAM_BREAK_MDBC, dx=0.004, jitter=0.01, seed=11)
+    nsteps = 1000
+    print(f"[case] dam break dx=4mm np={case.np}, {nsteps} steps", flush=True)
+    O = oracle_for(case)
+    oracle_init(O)
+    for _ in range(nsteps):
+        oracle_step(O, case.wall, case.dt)
+    ref = oracle_fields(O)
+    got = {}
+    for prec in (T.PREC_FP64, T.PREC_MIXED):
+        with T.system_for_case(case, precision=prec) as ps:
+            ps.particle_to_cell()
+            ps.step(case.wall, case.dt, nsteps)
+            got[prec] = ps.download()
+    errs = {}
+    for prec, g in got.items():
+        assert np.array_equal(g["type"], ref["type"])
+        errs[prec] = {k: rel_l2(g[k], ref[k]) for k in ("rho", "p", "v", "r")}
+    print(f"[errors] fp64 {errs[T.PREC_FP64]} mixed {errs[T.PREC_MIXED]}", flush=True)
+    assert errs[T.PREC_FP64]["rho"] <= 1e-8
+    assert errs[T.PREC_MIXED]["rho"] <= 1e-4

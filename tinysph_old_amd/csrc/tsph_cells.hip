@@ -282,13 +282,13 @@ __global__ void k_reorder(const int *__restrict__ cell_start, int ncells, const 
     }
     dst.p[k] = p;
     if (src.gnr) dst.gnr[k] = src.gnr[j];
-    double2 nv;
+    double2 nv = make_double2(0., 0.);
     if (src.n) {
       nv = src.n[j];
       dst.n[k] = nv;
     }
     if (EOS && mx.rv)
-      write_mixed_record(mx, kc, cellof[k] - kc.cell_base, k, r, v, rho, p, src.n ? &nv : nullptr);
+      write_mixed_record(mx, kc, cellof[k] - kc.cell_base, k, r, v, rho, p, src.n != nullptr, nv);
     if (src.omega) {
       dst.omega[k] = src.omega[j];
       dst.omegarho[k] = src.omegarho[j];

@@ -99,21 +99,23 @@ __device__ __forceinline__ double tait64(double rho, double rho0, double c0) {
 // resolves rho - rho0 to ~1e-7 of its size, while FP32 rho itself, ulp 6e-5 at
 // 1000, would lose the small density differences the diffusion term is built
 // from)
+// (has_n: the particle carries a wall normal n)
 __device__ __forceinline__ void write_mixed_record(const Mixed &mx, const Consts &k, int c, int i,
                                                    double2 r, double2 v, double rho, double p,
-                                                   const double2 *n) {
+                                                   bool has_n, double2 n) {
   const int cx = c / k.ncy, cy = c - cx * k.ncy;
   const double ox = k.x0 - k.of + cx * k.kh, oy = k.y0 - k.of + cy * k.kh;
   mx.rv[i] = make_float4((float)(r.x - ox), (float)(r.y - oy), (float)v.x, (float)v.y);
   mx.rp[i] = make_float2((float)(rho - k.rho0), (float)p);
-  if (mx.nrm && n) mx.nrm[i] = make_float2((float)n->x, (float)n->y);
+  if (mx.nrm && has_n) mx.nrm[i] = make_float2((float)n.x, (float)n.y);
 }
 
 __device__ __forceinline__ void write_mixed_record(const Soa &d, const Mixed &mx,
                                                    const int *__restrict__ cellof, const Consts &k,
                                                    int i, double rho, double p) {
-  write_mixed_record(mx, k, cellof[i] - k.cell_base, i, d.r[i], d.v[i], rho, p,
-                     d.n ? &d.n[i] : nullptr);
+  const bool has_n = d.n != nullptr;
+  write_mixed_record(mx, k, cellof[i] - k.cell_base, i, d.r[i], d.v[i], rho, p, has_n,
+                     has_n ? d.n[i] : make_double2(0., 0.));
 }
 
 __device__ __forceinline__ float hydro_drho_f32(float x, float rho0) {
