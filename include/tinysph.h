@@ -282,14 +282,20 @@ int tsph_batch_download(tsph_ctx *ctx, int icase, tsph_particles *host);
 int tsph_batch_step(tsph_ctx *ctx, int wall, int nsteps);
 
 /* ---- autotuner ----
- * Times the pair pass `variant` on the current state for each tile depth
- * (cells per tile: 4, 5, 6, 7, 8, 10, 12) and persistent-wave count (waves per
- * CU: 4, 6, 8, 12, 16), three passes each, and keeps the fastest pair; the
- * workgroup is one wavefront by design and each lane owns one particle of its
- * tile.  Needs a cell list; leaves a, drho as one pair pass with the chosen
- * setting leaves them.
+ * Times the pair pass `variant` on the current state for each tile walk
+ * (balanced or static, see tsph_set_balanced), tile depth (cells per tile: 5,
+ * 6, 7, 8, 10, 12) and persistent-wave count (waves per CU: 4, 8, 12, 16),
+ * three passes each, and keeps the fastest; the workgroup is one wavefront by
+ * design and each lane owns one particle of its tile.  Needs a cell list;
+ * leaves a, drho as one pair pass with the chosen setting leaves them.
  * Outputs may be NULL.  No counterpart in the reference. */
-int tsph_autotune(tsph_ctx *ctx, int variant, int *tile_rows, int *waves_per_cu, double *best_ms);
+int tsph_autotune(tsph_ctx *ctx, int variant, int *tile_rows, int *waves_per_cu, int *balanced,
+                  double *best_ms);
+/* Tile walk of the pair loops for the context's precision: 1 = balanced (the
+ * waves of each XCD take tiles from an atomic ticket counter and help other
+ * XCDs' tiles once theirs are done), 0 = static striding.  Results do not
+ * depend on it.  Defaults: balanced for FP64, static for mixed. */
+int tsph_set_balanced(tsph_ctx *ctx, int balanced);
 
 /* ---- case setup (host only; no GPU needed) ----
  * Presets built with the reference's geometry routines (src/SPH_create.h,

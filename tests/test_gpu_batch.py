@@ -167,8 +167,8 @@ def test_autotune_keeps_results(batch):
         with mk() as ps:
             ps.particle_to_cell()
             if tune:
-                rows, waves, ms = T.autotune(ps, T.FORCES_WCSPH)
-                assert rows in (4, 5, 6, 7, 8, 10, 12) and waves in (4, 6, 8, 12, 16)
+                rows, waves, bal, ms = T.autotune(ps, T.FORCES_WCSPH)
+                assert rows in (5, 6, 7, 8, 10, 12) and waves in (4, 8, 12, 16) and bal in (0, 1)
                 assert ms > 0
                 assert ps.tile_rows == rows
             else:
@@ -192,8 +192,8 @@ def test_ensemble_64_tanks_full_size():
     print(f"64 tanks, {int(n0.sum())} particles", flush=True)
     with T.batch_for_cases(cases, T.PREC_MIXED) as bs:
         bs.particle_to_cell()
-        rows, waves, _ = T.autotune(bs, T.FORCES_WCSPH)
-        print(f"tuned: {rows} cells per tile, {waves} waves per CU", flush=True)
+        rows, waves, bal, _ = T.autotune(bs, T.FORCES_WCSPH)
+        print(f"tuned: {rows} cells per tile, {waves} waves per CU, balanced {bal}", flush=True)
         bs.reset_counters()
         bs.step(T.WALL_MDBC, steps)
         cnt = bs.counters()

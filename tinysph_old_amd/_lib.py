@@ -131,7 +131,8 @@ EXPORTS = {
     "tsph_batch_download": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(Particles)]),
     "tsph_batch_step": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "tsph_autotune": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
-                                _dp]),
+                                C.POINTER(C.c_int), _dp]),
+    "tsph_set_balanced": (C.c_int, [C.c_void_p, C.c_int]),
 }
 
 _LIB = None

@@ -1135,14 +1135,23 @@ int tsph_batch_step(tsph_ctx *c, int wall, int nsteps) {
 
 // ---- autotuner -----------------------------------------------------------------
 
-int tsph_autotune(tsph_ctx *c, int variant, int *tile_rows, int *waves_per_cu, double *best_ms) {
+int tsph_set_balanced(tsph_ctx *c, int on) {
+  if (!c) return fail(TSPH_EINVAL, "null argument");
+  bool &b = c->balanced[c->prm.precision == TSPH_PREC_MIXED ? 1 : 0];
+  if (b != (on != 0)) drop_graph(c);
+  b = on != 0;
+  return 0;
+}
+
+int tsph_autotune(tsph_ctx *c, int variant, int *tile_rows, int *waves_per_cu, int *balanced,
+                  double *best_ms) {
   if (!c) return fail(TSPH_EINVAL, "null argument");
   if (variant < TSPH_FORCES_WCSPH || variant > TSPH_FORCES_BT) return fail(TSPH_EINVAL, "unknown variant");
   if (!c->cells_valid) return fail(TSPH_ESTATE, "no cell list: call tsph_particle_to_cell first");
   if (variant == TSPH_FORCES_BT && !c->has_n) return fail(TSPH_ESTATE, "BT forces need wall normals (n)");
   if (c->time_pairs) return fail(TSPH_ESTATE, "pair timing is active");
-  static const int kRows[] = {4, 5, 6, 7, 8, 10, 12};
-  static const int kWaves[] = {4, 6, 8, 12, 16};
+  static const int kRows[] = {5, 6, 7, 8, 10, 12};
+  static const int kWaves[] = {4, 8, 12, 16};
   constexpr int kReps = 3;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   TSPH_HIP(hipEventCreate(&e0));
@@ -1150,41 +1159,46 @@ int tsph_autotune(tsph_ctx *c, int variant, int *tile_rows, int *waves_per_cu, d
     (void)hipEventDestroy(e0);
     return fail(TSPH_EHIP, "hipEventCreate failed");
   }
-  int best_r = c->tile_rows, best_w = c->waves_per_cu, e = 0;
+  int best_r = c->tile_rows, best_w = c->waves_per_cu, best_b = 0, e = 0;
   float best = 1e30f;
   // every candidate recomputes the same pair pass: its outputs a and drho are
   // overwritten from the unchanged state
-  for (int r : kRows) {
-    if ((e = tsph_set_tile_rows(c, r))) break;
-    for (int w : kWaves) {
-      if ((e = tsph_set_waves_per_cu(c, w)) || (e = launch_forces(c, variant))) break;  // warm-up
-      if (hipEventRecord(e0, c->stream) != hipSuccess) { e = fail(TSPH_EHIP, "event record failed"); break; }
-      for (int k = 0; k < kReps && !e; ++k) e = launch_forces(c, variant);
+  for (int bal = 0; bal < 2 && !e; ++bal) {
+    if ((e = tsph_set_balanced(c, bal))) break;
+    for (int r : kRows) {
+      if ((e = tsph_set_tile_rows(c, r))) break;
+      for (int w : kWaves) {
+        if ((e = tsph_set_waves_per_cu(c, w)) || (e = launch_forces(c, variant))) break;  // warm-up
+        if (hipEventRecord(e0, c->stream) != hipSuccess) { e = fail(TSPH_EHIP, "event record failed"); break; }
+        for (int k = 0; k < kReps && !e; ++k) e = launch_forces(c, variant);
+        if (e) break;
+        float ms = 0.f;
+        if (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+          e = fail(TSPH_EHIP, "event timing failed");
+          break;
+        }
+        if (ms < best) {
+          best = ms;
+          best_r = r;
+          best_w = w;
+          best_b = bal;
+        }
+      }
       if (e) break;
-      float ms = 0.f;
-      if (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-          hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
-        e = fail(TSPH_EHIP, "event timing failed");
-        break;
-      }
-      if (ms < best) {
-        best = ms;
-        best_r = r;
-        best_w = w;
-      }
     }
-    if (e) break;
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (e) return e;
   // a last pass with the chosen setting, so that a and drho are what a pair
   // pass with it leaves (the mixed path rounds in a tile-relative frame)
-  if ((e = tsph_set_tile_rows(c, best_r)) || (e = tsph_set_waves_per_cu(c, best_w)) ||
-      (e = launch_forces(c, variant)))
+  if ((e = tsph_set_balanced(c, best_b)) || (e = tsph_set_tile_rows(c, best_r)) ||
+      (e = tsph_set_waves_per_cu(c, best_w)) || (e = launch_forces(c, variant)))
     return e;
   if (tile_rows) *tile_rows = best_r;
   if (waves_per_cu) *waves_per_cu = best_w;
+  if (balanced) *balanced = best_b;
   if (best_ms) *best_ms = best / kReps;
   return 0;
 }

@@ -24,7 +24,12 @@ namespace tsph {
 constexpr uint8_t kRemoved = 255;
 
 // device scalar slots
-enum { S_NP = 0, S_MAXSEQ = 1, S_NTILES = 2, S_NWALL = 3, S_BZCNT = 4, S_NSCAL = 5 };
+// (S_TQ: the pair loops' tile queue, 8 band ticket counters and 9 completion
+// counters, each in its own 128-byte line, kTqStride ints apart)
+constexpr int kTqStride = 32;
+constexpr int kBalancedStatic8 = 0;  // eighths of each band walked statically when balanced
+enum { S_NP = 0, S_MAXSEQ = 1, S_NTILES = 2, S_NWALL = 3, S_BZCNT = 4, S_TQ = 32,
+       S_NSCAL = S_TQ + 17 * kTqStride };
 // device counter slots (uint64)
 enum {
   C_PAIRS = 0, C_CAND = 1, C_ACTIVE = 2, C_PASSES = 3, C_BADRHO = 4, C_OUTGRID = 5,
@@ -119,6 +124,10 @@ struct tsph_ctx {
   int tile_rows = 16;  // cells per pair-loop tile (one grid-column run)
   int waves_per_cu = 8;  // persistent pair-loop waves per compute unit
   int num_cu = 256;      // compute units of the device
+  // pair-loop tile walk per precision (FP64, mixed): balanced (a static
+  // prefix of each band, then atomic tickets) or the plain static walk.  Long
+  // FP64 tiles gain from balancing; short mixed tiles lose more to the tickets.
+  bool balanced[2] = {true, false};
   int *tile_list = nullptr;  // non-empty tiles of the current cell list
   bool tiles_valid = false;  // tile_list matches the cell list and tile_rows
   bool tiles_zeroed = false; // the device tile counter is 0 (set by the cell-list build)

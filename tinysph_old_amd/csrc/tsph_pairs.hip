@@ -336,14 +336,14 @@ __global__ __launch_bounds__(256) void k_tile_list(const int *__restrict__ cell_
 // --------------------------------------------------------------------------
 // FP64 tile kernel
 // --------------------------------------------------------------------------
-template <int VAR>
+template <int VAR, bool DYN>
 __global__ __launch_bounds__(kWave) void k_tile64(Soa d, const int *__restrict__ cell_start,
                                                   const int *__restrict__ cellof, Consts k0,
                                                   const Consts *__restrict__ cases,
                                                   const int *__restrict__ tile_base, int ncases,
-                                                  int rows, const int *__restrict__ scal,
+                                                  int rows, int *__restrict__ scal,
                                                   const int *__restrict__ tile_list, int cap,
-                                                  unsigned long long *__restrict__ counters) {
+                                                  int stat8, unsigned long long *__restrict__ counters) {
   extern __shared__ __align__(16) unsigned char smem[];
   int *tab = reinterpret_cast<int *>(smem);
   double2 *s_r = reinterpret_cast<double2 *>(smem + kTabBytes);
@@ -352,13 +352,16 @@ __global__ __launch_bounds__(kWave) void k_tile64(Soa d, const int *__restrict__
   double2 *s_n = s_rp + cap;  // BT only
   uint16_t *s_list = reinterpret_cast<uint16_t *>(s_n + (VAR == TSPH_FORCES_BT ? cap : 0));
   uint8_t *s_t = reinterpret_cast<uint8_t *>(s_list + kList * kWave);
+  uint8_t *s_row = s_t + cap;  // row of the slot within the tile's rows
 
   if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&counters[C_PASSES], 1ull);
   unsigned npair = 0, ncand = 0, nact = 0;
-  int it, it_end, it_step;
-  band_of_wave(scal[S_NTILES], it, it_end, it_step);
-  for (; it < it_end; it += it_step) {
+  TileQueue tq = tq_begin(scal + S_TQ, scal[S_NTILES], stat8);
+  for (int it = tq_take<DYN>(tq, tq_ticket<DYN>(tq)); it >= 0;) {
   __syncthreads();  // the previous tile's LDS is no longer read
+  // the next tile's ticket: its atomic is in flight with this tile's loads,
+  // and it is read once they have been waited for
+  const int ticket = tq_ticket<DYN>(tq);
   // the tile's case: its constants and its segment of the cell range
   int tile = tile_list[it];
   Consts k = k0;
@@ -371,22 +374,27 @@ __global__ __launch_bounds__(kWave) void k_tile64(Soa d, const int *__restrict__
   Tile t;
   tile_begin(k, cs, tile, rows, tab, t);
   if (t.lq[3] > cap) {
+    const int next = tq_take<DYN>(tq, ticket);
     for (int i = t.own0 + threadIdx.x; i < t.own1; i += blockDim.x) {
       if (!is_active<VAR>(d.type[i])) continue;
       ++nact;
       particle64_global<VAR>(d, cs, cellof, k, i, npair, ncand);
     }
+    it = next;
     continue;
   }
   for (int l = threadIdx.x; l < t.lq[3]; l += blockDim.x) {
-    const int j = l - sel3(t.base, slice_of(t, l));
+    const int q = slice_of(t, l);
+    const int j = l - sel3(t.base, q);
     s_r[l] = d.r[j];
     s_v[l] = d.v[j];
     s_rp[l] = make_double2(d.rho[j], d.p[j]);
     s_t[l] = d.type[j];
+    s_row[l] = (uint8_t)(cellof[j] - k.cell_base - (t.cx + q - 1) * k.ncy - t.row_lo);
     if (VAR == TSPH_FORCES_BT) s_n[l] = d.n[j];
   }
   __syncthreads();
+  const int next = tq_take<DYN>(tq, ticket);
 
   const int lane = threadIdx.x;
   uint16_t *my_list = s_list + lane;  // entry e at my_list[e * kWave]
@@ -398,7 +406,7 @@ __global__ __launch_bounds__(kWave) void k_tile64(Soa d, const int *__restrict__
       li = i + t.base[1];
       at = s_t[li];
       act = is_active<VAR>(at);
-      cy = cellof[i] - k.cell_base - t.cx * k.ncy;
+      cy = s_row[li] + t.row_lo;
     }
     nact += act;
     const double2 ar = s_r[li], av = s_v[li], arp = s_rp[li];
@@ -445,21 +453,23 @@ __global__ __launch_bounds__(kWave) void k_tile64(Soa d, const int *__restrict__
       d.drho[i] = dr;
     }
   }
+  it = next;
   }  // tiles
+  tq_end<DYN>(tq);
   wave_count(npair, ncand, nact, counters);
 }
 
 // --------------------------------------------------------------------------
 // mixed tile kernel
 // --------------------------------------------------------------------------
-template <int VAR>
+template <int VAR, bool DYN>
 __global__ __launch_bounds__(kWave) void k_tile32(Soa d, Mixed mx, const int *__restrict__ cell_start,
                                                   const int *__restrict__ cellof, Consts k0,
                                                   const Consts *__restrict__ cases,
                                                   const int *__restrict__ tile_base, int ncases,
-                                                  int rows, const int *__restrict__ scal,
+                                                  int rows, int *__restrict__ scal,
                                                   const int *__restrict__ tile_list, int cap,
-                                                  unsigned long long *__restrict__ counters) {
+                                                  int stat8, unsigned long long *__restrict__ counters) {
 #pragma clang fp contract(fast)
   extern __shared__ __align__(16) unsigned char smem[];
   int *tab = reinterpret_cast<int *>(smem);
@@ -470,13 +480,16 @@ __global__ __launch_bounds__(kWave) void k_tile32(Soa d, Mixed mx, const int *__
   float *s_ir = reinterpret_cast<float *>(s_n + (VAR == TSPH_FORCES_BT ? cap : 0));  // 1/rho
   uint16_t *s_list = reinterpret_cast<uint16_t *>(s_ir + cap);
   uint8_t *s_t = reinterpret_cast<uint8_t *>(s_list + kList * kWave);
+  uint8_t *s_row = s_t + cap;  // row of the slot within the tile's rows
 
   if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&counters[C_PASSES], 1ull);
   unsigned npair = 0, ncand = 0, nact = 0;
-  int it, it_end, it_step;
-  band_of_wave(scal[S_NTILES], it, it_end, it_step);
-  for (; it < it_end; it += it_step) {
+  TileQueue tq = tq_begin(scal + S_TQ, scal[S_NTILES], stat8);
+  for (int it = tq_take<DYN>(tq, tq_ticket<DYN>(tq)); it >= 0;) {
   __syncthreads();  // the previous tile's LDS is no longer read
+  // the next tile's ticket: its atomic is in flight with this tile's loads,
+  // and it is read once they have been waited for
+  const int ticket = tq_ticket<DYN>(tq);
   // the tile's case: its constants and its segment of the cell range
   int tile = tile_list[it];
   Consts k = k0;
@@ -490,11 +503,13 @@ __global__ __launch_bounds__(kWave) void k_tile32(Soa d, Mixed mx, const int *__
   Tile t;
   tile_begin(k, cs, tile, rows, tab, t);
   if (t.lq[3] > cap) {
+    const int next = tq_take<DYN>(tq, ticket);
     for (int i = t.own0 + threadIdx.x; i < t.own1; i += blockDim.x) {
       if (!is_active<VAR>(d.type[i])) continue;
       ++nact;
       particle32_global<VAR>(d, mx, cs, cellof, k, cc, i, npair, ncand);
     }
+    it = next;
     continue;
   }
   const int mid = t.row_lo + (t.nrows >> 1);
@@ -509,9 +524,11 @@ __global__ __launch_bounds__(kWave) void k_tile32(Soa d, Mixed mx, const int *__
     s_dp[l] = rp;
     s_ir[l] = __builtin_amdgcn_rcpf(cc.rho0 + rp.x);
     s_t[l] = d.type[j];
+    s_row[l] = (uint8_t)(row - t.row_lo);
     if (VAR == TSPH_FORCES_BT) s_n[l] = mx.nrm[j];
   }
   __syncthreads();
+  const int next = tq_take<DYN>(tq, ticket);
 
   const int lane = threadIdx.x;
   uint16_t *my_list = s_list + lane;
@@ -523,7 +540,7 @@ __global__ __launch_bounds__(kWave) void k_tile32(Soa d, Mixed mx, const int *__
       li = i + t.base[1];
       at = s_t[li];
       act = is_active<VAR>(at);
-      ri = cellof[i] - k.cell_base - t.cx * k.ncy - t.row_lo;
+      ri = s_row[li];
     }
     nact += act;
     const float2 pi = s_pos[li], vi = s_vel[li], dpi = s_dp[li];
@@ -607,7 +624,9 @@ __global__ __launch_bounds__(kWave) void k_tile32(Soa d, Mixed mx, const int *__
       d.drho[i] = dr;
     }
   }
+  it = next;
   }  // tiles
+  tq_end<DYN>(tq);
   wave_count(npair, ncand, nact, counters);
 }
 
@@ -648,56 +667,56 @@ int ensure_tile_list(tsph_ctx *c) {
   return 0;
 }
 
+namespace {
+
+// one pair pass: the kernel for the variant, the precision and the tile walk
+// (balanced: static prefix + atomic tickets; otherwise the plain static walk)
+template <int VAR, bool DYN>
+void pair_launch(tsph_ctx *c, dim3 g, size_t lds, int rows, int cap) {
+  int *sc = c->scal;
+  if (c->prm.precision == TSPH_PREC_MIXED)
+    hipLaunchKernelGGL((k_tile32<VAR, DYN>), g, dim3(kWave), lds, c->stream, c->cur, c->mx,
+                       c->cell_start, c->cellof, c->k, c->cases, c->tile_base, c->ncases, rows, sc,
+                       c->tile_list, cap, kBalancedStatic8, c->counters);
+  else
+    hipLaunchKernelGGL((k_tile64<VAR, DYN>), g, dim3(kWave), lds, c->stream, c->cur, c->cell_start,
+                       c->cellof, c->k, c->cases, c->tile_base, c->ncases, rows, sc, c->tile_list,
+                       cap, kBalancedStatic8, c->counters);
+}
+
+template <int VAR>
+void pair_launch(tsph_ctx *c, bool balanced, dim3 g, size_t lds, int rows, int cap) {
+  if (balanced) pair_launch<VAR, true>(c, g, lds, rows, cap);
+  else pair_launch<VAR, false>(c, g, lds, rows, cap);
+}
+
+}  // namespace
+
 int launch_forces(tsph_ctx *c, int variant) {
-  const Consts &k = c->k;
   const int rows = c->tile_rows;
   const int cap = 3 * (rows + 2) * kCellCap;
   const bool bt = variant == TSPH_FORCES_BT;
-  hipStream_t s = c->stream;
   if (int e = ensure_tile_list(c)) return e;
-  const dim3 g(8 * ((c->num_cu * c->waves_per_cu + 7) / 8)), b(kWave);
+  const dim3 g(8 * ((c->num_cu * c->waves_per_cu + 7) / 8));
   const size_t list = sizeof(uint16_t) * kList * kWave;
-  const int *sc = c->scal, *tl = c->tile_list;
-  if (c->prm.precision == TSPH_PREC_MIXED) {
+  const bool mixed = c->prm.precision == TSPH_PREC_MIXED;
+  size_t lds;
+  if (mixed) {
     if (!c->records_valid) {
       if (int e = launch_mixed_records(c)) return e;
       c->records_valid = true;
     }
-    const size_t lds = kTabBytes + list +
-                       (size_t)cap * (3 * sizeof(float2) + sizeof(float) + 1 + (bt ? sizeof(float2) : 0));
-    return launch_timed(c, [&] {
-      switch (variant) {
-        case TSPH_FORCES_WCSPH:
-          hipLaunchKernelGGL(k_tile32<TSPH_FORCES_WCSPH>, g, b, lds, s, c->cur, c->mx, c->cell_start,
-                             c->cellof, k, c->cases, c->tile_base, c->ncases, rows, sc, tl, cap,
-                             c->counters);
-          break;
-        case TSPH_FORCES_DBC:
-          hipLaunchKernelGGL(k_tile32<TSPH_FORCES_DBC>, g, b, lds, s, c->cur, c->mx, c->cell_start,
-                             c->cellof, k, c->cases, c->tile_base, c->ncases, rows, sc, tl, cap,
-                             c->counters);
-          break;
-        default:
-          hipLaunchKernelGGL(k_tile32<TSPH_FORCES_BT>, g, b, lds, s, c->cur, c->mx, c->cell_start,
-                             c->cellof, k, c->cases, c->tile_base, c->ncases, rows, sc, tl, cap,
-                             c->counters);
-      }
-    });
+    lds = kTabBytes + list +
+          (size_t)cap * (3 * sizeof(float2) + sizeof(float) + 2 + (bt ? sizeof(float2) : 0));
+  } else {
+    lds = kTabBytes + list + (size_t)cap * (3 * sizeof(double2) + 2 + (bt ? sizeof(double2) : 0));
   }
-  const size_t lds = kTabBytes + list + (size_t)cap * (3 * sizeof(double2) + 1 + (bt ? sizeof(double2) : 0));
+  const bool bal = c->balanced[mixed ? 1 : 0];
   return launch_timed(c, [&] {
     switch (variant) {
-      case TSPH_FORCES_WCSPH:
-        hipLaunchKernelGGL(k_tile64<TSPH_FORCES_WCSPH>, g, b, lds, s, c->cur, c->cell_start, c->cellof,
-                           k, c->cases, c->tile_base, c->ncases, rows, sc, tl, cap, c->counters);
-        break;
-      case TSPH_FORCES_DBC:
-        hipLaunchKernelGGL(k_tile64<TSPH_FORCES_DBC>, g, b, lds, s, c->cur, c->cell_start, c->cellof,
-                           k, c->cases, c->tile_base, c->ncases, rows, sc, tl, cap, c->counters);
-        break;
-      default:
-        hipLaunchKernelGGL(k_tile64<TSPH_FORCES_BT>, g, b, lds, s, c->cur, c->cell_start, c->cellof,
-                           k, c->cases, c->tile_base, c->ncases, rows, sc, tl, cap, c->counters);
+      case TSPH_FORCES_WCSPH: pair_launch<TSPH_FORCES_WCSPH>(c, bal, g, lds, rows, cap); break;
+      case TSPH_FORCES_DBC: pair_launch<TSPH_FORCES_DBC>(c, bal, g, lds, rows, cap); break;
+      default: pair_launch<TSPH_FORCES_BT>(c, bal, g, lds, rows, cap);
     }
   });
 }

@@ -74,6 +74,120 @@ __device__ __forceinline__ void band_of_wave(int ntl, int &it, int &end, int &st
   step = gridDim.x >> 3;
 }
 
+// Balanced form of the same banding.  Each band is split in two parts:
+//   * a static part, the first J whole rounds of the band (J = the rounds that
+//     fit in stat8/8 of it): the waves of XCD x stride through it as above,
+//     with no atomics;
+//   * a dynamic part, the rest: waves take its tiles in list order from an
+//     atomic ticket counter per band, and when their own band is empty they
+//     help the other bands in turn.
+// A wave ends when all eight counters have passed their bands' ends; counters
+// only grow, so every wave ends.  The last wave to finish
+// resets the counters for the next launch (tq_end), so no clearing pass or
+// memset is needed.  A wave takes its next ticket at the start
+// of a tile and reads it after the tile's staging loads have been waited for,
+// so the atomic's latency hides behind theirs.
+struct TileQueue {
+  int *q;      // 17 counters, kTqStride ints apart
+  int ntl;     // tiles in the list
+  int band;    // band of the current ticket
+  int nwb;     // waves per band
+  int wb;      // this wave's rank in its band
+  int j;       // static round of the current tile (kDynamic once in the dynamic part)
+  int J;       // static rounds of the own band
+  int stat8;   // eighths of each band assigned statically
+};
+constexpr int kDynamic = 0x3fffffff;
+
+__device__ __forceinline__ int band_lo(int ntl, int b) { return (int)(((long long)ntl * b) >> 3); }
+
+__device__ __forceinline__ int static_rounds(const TileQueue &tq, int b) {
+  const int size = band_lo(tq.ntl, b + 1) - band_lo(tq.ntl, b);
+  return (int)(((long long)size * tq.stat8) >> 3) / tq.nwb;
+}
+
+// first tile of band b's dynamic part
+__device__ __forceinline__ int dyn_lo(const TileQueue &tq, int b) {
+  return band_lo(tq.ntl, b) + static_rounds(tq, b) * tq.nwb;
+}
+
+__device__ __forceinline__ bool tq_static_next(const TileQueue &tq) { return tq.j + 1 < tq.J; }
+
+// The functions below take DYN = false for the purely static walk (every tile
+// of the band by striding, no atomics at all: band_of_wave's order).
+
+// lane 0 takes a ticket of the current band when the next tile is a dynamic
+// one (the value is read with tq_take)
+template <bool DYN>
+__device__ __forceinline__ int tq_ticket(const TileQueue &tq) {
+  int t = 0;
+  if (DYN && !tq_static_next(tq) && threadIdx.x == 0) t = atomicAdd(&tq.q[tq.band * kTqStride], 1);
+  return t;
+}
+
+// list index of the wave's next tile (raw: the value from tq_ticket), or -1
+// when every band is exhausted
+template <bool DYN>
+__device__ __forceinline__ int tq_take(TileQueue &tq, int raw) {
+  if (!DYN) {
+    const int idx = band_lo(tq.ntl, tq.band) + tq.wb + (++tq.j) * tq.nwb;
+    return idx < band_lo(tq.ntl, tq.band + 1) ? idx : -1;
+  }
+  if (tq_static_next(tq)) {
+    ++tq.j;
+    return band_lo(tq.ntl, tq.band) + tq.wb + tq.j * tq.nwb;
+  }
+  tq.j = kDynamic;
+  int ticket = __builtin_amdgcn_readfirstlane(raw);
+  while (true) {
+    const int lo = dyn_lo(tq, tq.band), hi = band_lo(tq.ntl, tq.band + 1);
+    if (lo + ticket < hi) return lo + ticket;
+    // this band is exhausted: lanes 0-7 read the eight band counters at once
+    // (plain reads, so exhausted bands take no further atomics) and the wave
+    // moves on to the next band, in cyclic order, that still has tiles
+    bool open = false;
+    if (threadIdx.x < 8) {
+      const int b = threadIdx.x;
+      open = __hip_atomic_load(&tq.q[b * kTqStride], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <
+             band_lo(tq.ntl, b + 1) - dyn_lo(tq, b);
+    }
+    const unsigned m = (unsigned)__ballot(open) & 0xffu;
+    if (!m) return -1;
+    const unsigned rot = ((m >> tq.band) | (m << (8 - tq.band))) & 0xffu;  // bit i: band + i
+    tq.band = (tq.band + __builtin_ctz(rot)) & 7;
+    ticket = __builtin_amdgcn_readfirstlane(tq_ticket<DYN>(tq));
+  }
+}
+
+__device__ __forceinline__ TileQueue tq_begin(int *q, int ntl, int stat8) {
+  TileQueue tq;
+  tq.q = q;
+  tq.ntl = ntl;
+  tq.band = blockIdx.x & 7;
+  tq.nwb = gridDim.x >> 3;
+  tq.wb = blockIdx.x >> 3;
+  tq.stat8 = stat8;
+  tq.j = -1;
+  tq.J = static_rounds(tq, tq.band);
+  return tq;
+}
+
+// after the wave's last tq_take: count it finished.  The waves of a band
+// count on the band's own counter (q[8 + 1 + band]); the last of each band
+// counts on q[8], and the last of those resets every counter for the next
+// launch.  Two levels keep the end-of-launch atomics spread over nine lines.
+// No fence is needed: the wave's last ticket has returned (its value was read)
+// before these atomics are issued, and the tiles' outputs are for later launches.
+template <bool DYN>
+__device__ __forceinline__ void tq_end(const TileQueue &tq) {
+  if (DYN && threadIdx.x == 0) {
+    const int band = blockIdx.x & 7;
+    if (atomicAdd(&tq.q[(9 + band) * kTqStride], 1) == tq.nwb - 1 &&
+        atomicAdd(&tq.q[8 * kTqStride], 1) == 7)
+      for (int b = 0; b < 17; ++b) atomicExch(&tq.q[b * kTqStride], 0);
+  }
+}
+
 // Loads the cell offsets of the three slices (one global load per lane, all in
 // flight together) and derives the tile ranges.  False for an empty tile.
 __device__ __forceinline__ bool tile_begin(const Consts &k, const int *__restrict__ cell_start,

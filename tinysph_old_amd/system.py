@@ -356,6 +356,9 @@ class ParticleSystem:
     def step_ale(self, dt: float, step: int, nsteps: int = 1, muscl: int = 0):
         L.check(self.lib.tsph_step_ale(self.ctx, dt, step, nsteps, muscl), "tsph_step_ale")
 
+    def set_balanced(self, on: bool):
+        L.check(self.lib.tsph_set_balanced(self.ctx, int(bool(on))), "tsph_set_balanced")
+
     def set_waves_per_cu(self, waves: int):
         L.check(self.lib.tsph_set_waves_per_cu(self.ctx, waves), "tsph_set_waves_per_cu")
 
@@ -450,8 +453,9 @@ def batch_for_cases(cases: list, precision: int = L.PREC_FP64) -> BatchSystem:
 
 
 def autotune(ps: ParticleSystem, variant: int):
-    """tsph_autotune: (tile_rows, waves_per_cu, ms per pair pass) of the fastest setting."""
-    r, w, ms = C.c_int(), C.c_int(), C.c_double()
-    L.check(ps.lib.tsph_autotune(ps.ctx, variant, C.byref(r), C.byref(w), C.byref(ms)),
-            "tsph_autotune")
-    return r.value, w.value, ms.value
+    """tsph_autotune: (tile_rows, waves_per_cu, balanced, ms per pair pass) of the
+    fastest setting."""
+    r, w, b, ms = C.c_int(), C.c_int(), C.c_int(), C.c_double()
+    L.check(ps.lib.tsph_autotune(ps.ctx, variant, C.byref(r), C.byref(w), C.byref(b),
+                                 C.byref(ms)), "tsph_autotune")
+    return r.value, w.value, b.value, ms.value
