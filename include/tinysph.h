@@ -293,8 +293,9 @@ int tsph_autotune(tsph_ctx *ctx, int variant, int *tile_rows, int *waves_per_cu,
                   double *best_ms);
 /* Tile walk of the pair loops for the context's precision: 1 = balanced (the
  * waves of each XCD take tiles from an atomic ticket counter and help other
- * XCDs' tiles once theirs are done), 0 = static striding.  Results do not
- * depend on it.  Defaults: balanced for FP64, static for mixed. */
+ * XCDs' tiles once theirs are done), 0 = static striding.  Grids with fewer
+ * than 8 tiles per persistent wave always stride.  Results do not depend on
+ * it.  Defaults: balanced for FP64, static for mixed. */
 int tsph_set_balanced(tsph_ctx *ctx, int balanced);
 
 /* ---- case setup (host only; no GPU needed) ----

@@ -95,9 +95,10 @@ __device__ __forceinline__ void rale_finish(const Soa &d, const Consts &k, int i
 }
 
 // FP64 tile kernel, reference summation order (9 cells, ascending index)
+template <bool DYN>
 __global__ __launch_bounds__(kWave) void k_rale64(Soa d, const int *__restrict__ cell_start,
                                                   const int *__restrict__ cellof, Consts k, int rows,
-                                                  const int *__restrict__ scal,
+                                                  int *__restrict__ scal,
                                                   const int *__restrict__ tile_list, int cap,
                                                   uint8_t *__restrict__ gflag,
                                                   unsigned long long *__restrict__ counters) {
@@ -112,10 +113,10 @@ __global__ __launch_bounds__(kWave) void k_rale64(Soa d, const int *__restrict__
 
   if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&counters[C_PASSES], 1ull);
   unsigned npair = 0, ncand = 0, nact = 0;
-  int it, it_end, it_step;
-  band_of_wave(scal[S_NTILES], it, it_end, it_step);
-  for (; it < it_end; it += it_step) {
+  TileQueue tq = tq_begin(scal + S_TQ, scal[S_NTILES], kBalancedStatic8);
+  for (int it = tq_take<DYN>(tq, tq_ticket<DYN>(tq)); it >= 0;) {
     __syncthreads();
+    const int ticket = tq_ticket<DYN>(tq);  // read once this tile's loads are waited for
     Tile t;
     tile_begin(k, cell_start, tile_list[it], rows, tab, t);
     const bool staged = t.lq[3] <= cap;
@@ -130,6 +131,7 @@ __global__ __launch_bounds__(kWave) void k_rale64(Soa d, const int *__restrict__
       }
     }
     __syncthreads();
+    const int next = tq_take<DYN>(tq, ticket);
     const int lane = threadIdx.x;
     uint16_t *my_list = s_list + lane;
     for (int i0 = t.own0; i0 < t.own1; i0 += kWave) {
@@ -202,7 +204,9 @@ __global__ __launch_bounds__(kWave) void k_rale64(Soa d, const int *__restrict__
       }
       if (act) rale_finish(d, k, i, ao, arho, s.dom, s.domrho, s.dovx, s.dovy, s.gamma != 0, gflag);
     }
+    it = next;
   }
+  tq_end<DYN>(tq);
   wave_count(npair, ncand, nact, counters);
 }
 
@@ -264,9 +268,10 @@ __device__ __forceinline__ void rale_pair32(const CA32 &c, AccA32 &s, float drx,
   s.gamma += W * c.m * __builtin_amdgcn_rcpf(nrho);
 }
 
+template <bool DYN>
 __global__ __launch_bounds__(kWave) void k_rale32(Soa d, const int *__restrict__ cell_start,
                                                   const int *__restrict__ cellof, Consts k, int rows,
-                                                  const int *__restrict__ scal,
+                                                  int *__restrict__ scal,
                                                   const int *__restrict__ tile_list, int cap,
                                                   uint8_t *__restrict__ gflag,
                                                   unsigned long long *__restrict__ counters) {
@@ -287,10 +292,10 @@ __global__ __launch_bounds__(kWave) void k_rale32(Soa d, const int *__restrict__
 
   if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&counters[C_PASSES], 1ull);
   unsigned npair = 0, ncand = 0, nact = 0;
-  int it, it_end, it_step;
-  band_of_wave(scal[S_NTILES], it, it_end, it_step);
-  for (; it < it_end; it += it_step) {
+  TileQueue tq = tq_begin(scal + S_TQ, scal[S_NTILES], kBalancedStatic8);
+  for (int it = tq_take<DYN>(tq, tq_ticket<DYN>(tq)); it >= 0;) {
     __syncthreads();
+    const int ticket = tq_ticket<DYN>(tq);  // read once this tile's loads are waited for
     Tile t;
     tile_begin(k, cell_start, tile_list[it], rows, tab, t);
     const bool staged = t.lq[3] <= cap;
@@ -308,6 +313,7 @@ __global__ __launch_bounds__(kWave) void k_rale32(Soa d, const int *__restrict__
       }
     }
     __syncthreads();
+    const int next = tq_take<DYN>(tq, ticket);
     const int lane = threadIdx.x;
     uint16_t *my_list = s_list + lane;
     for (int i0 = t.own0; i0 < t.own1; i0 += kWave) {
@@ -386,7 +392,9 @@ __global__ __launch_bounds__(kWave) void k_rale32(Soa d, const int *__restrict__
                     (double)s.dovy, s.gamma != 0.f, gflag);
       }
     }
+    it = next;
   }
+  tq_end<DYN>(tq);
   wave_count(npair, ncand, nact, counters);
 }
 
@@ -509,14 +517,22 @@ int launch_rale_forces(tsph_ctx *c) {
   if (c->prm.precision == TSPH_PREC_MIXED) {
     const size_t lds = kTabBytes + list + (size_t)cap * (4 * sizeof(float2) + 1);
     e = launch_timed(c, [&] {
-      hipLaunchKernelGGL(k_rale32, g, b, lds, s, c->cur, c->cell_start, c->cellof, k, rows, c->scal,
-                         c->tile_list, cap, c->ale_flag, c->counters);
+      if (walk_balanced(c, true, (int)g.x))
+        hipLaunchKernelGGL(k_rale32<true>, g, b, lds, s, c->cur, c->cell_start, c->cellof, k, rows,
+                           c->scal, c->tile_list, cap, c->ale_flag, c->counters);
+      else
+        hipLaunchKernelGGL(k_rale32<false>, g, b, lds, s, c->cur, c->cell_start, c->cellof, k, rows,
+                           c->scal, c->tile_list, cap, c->ale_flag, c->counters);
     });
   } else {
     const size_t lds = kTabBytes + list + (size_t)cap * (4 * sizeof(double2) + 1);
     e = launch_timed(c, [&] {
-      hipLaunchKernelGGL(k_rale64, g, b, lds, s, c->cur, c->cell_start, c->cellof, k, rows, c->scal,
-                         c->tile_list, cap, c->ale_flag, c->counters);
+      if (walk_balanced(c, false, (int)g.x))
+        hipLaunchKernelGGL(k_rale64<true>, g, b, lds, s, c->cur, c->cell_start, c->cellof, k, rows,
+                           c->scal, c->tile_list, cap, c->ale_flag, c->counters);
+      else
+        hipLaunchKernelGGL(k_rale64<false>, g, b, lds, s, c->cur, c->cell_start, c->cellof, k, rows,
+                           c->scal, c->tile_list, cap, c->ale_flag, c->counters);
     });
   }
   if (e) return e;

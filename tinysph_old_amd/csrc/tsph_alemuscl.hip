@@ -679,22 +679,22 @@ struct Acc32P {
 // ---------------------------------------------------------------------------
 // kernels
 // ---------------------------------------------------------------------------
-template <class P>
+template <class P, bool DYN>
 __global__ __launch_bounds__(kWave) void k_ale_pass64(P pol, const int *__restrict__ cell_start,
                                                       const int *__restrict__ cellof, int rows,
-                                                      const int *__restrict__ scal,
+                                                      int *__restrict__ scal,
                                                       const int *__restrict__ tile_list, int cap,
                                                       unsigned long long *__restrict__ counters) {
-  walk64(pol, pol.k, cell_start, cellof, rows, scal, tile_list, cap, counters);
+  walk64<DYN>(pol, pol.k, cell_start, cellof, rows, scal, tile_list, cap, counters);
 }
 
-template <class P>
+template <class P, bool DYN>
 __global__ __launch_bounds__(kWave) void k_ale_pass32(P pol, const int *__restrict__ cell_start,
                                                       const int *__restrict__ cellof, int rows,
-                                                      const int *__restrict__ scal,
+                                                      int *__restrict__ scal,
                                                       const int *__restrict__ tile_list, int cap,
                                                       unsigned long long *__restrict__ counters) {
-  walk32(pol, pol.k, cell_start, cellof, rows, scal, tile_list, cap, counters);
+  walk32<DYN>(pol, pol.k, cell_start, cellof, rows, scal, tile_list, cap, counters);
 }
 
 // Integrate_density_compute_pressure, src/ALESPH_integration.h:86-135 (fluid
@@ -759,13 +759,23 @@ int launch_pass(tsph_ctx *c, const P &pol) {
   if (int e = ensure_tile_list(c)) return e;
   const dim3 g(8 * ((c->num_cu * c->waves_per_cu + 7) / 8)), b(kWave);
   const size_t lds = kFixed + (size_t)cap * SLOT;
+  const bool bal = walk_balanced(c, FP32, (int)g.x);
   return launch_timed(c, [&] {
-    if constexpr (FP32)
-      hipLaunchKernelGGL(k_ale_pass32<P>, g, b, lds, c->stream, pol, c->cell_start, c->cellof, rows,
-                         c->scal, c->tile_list, cap, c->counters);
-    else
-      hipLaunchKernelGGL(k_ale_pass64<P>, g, b, lds, c->stream, pol, c->cell_start, c->cellof, rows,
-                         c->scal, c->tile_list, cap, c->counters);
+    if constexpr (FP32) {
+      if (bal)
+        hipLaunchKernelGGL((k_ale_pass32<P, true>), g, b, lds, c->stream, pol, c->cell_start,
+                           c->cellof, rows, c->scal, c->tile_list, cap, c->counters);
+      else
+        hipLaunchKernelGGL((k_ale_pass32<P, false>), g, b, lds, c->stream, pol, c->cell_start,
+                           c->cellof, rows, c->scal, c->tile_list, cap, c->counters);
+    } else {
+      if (bal)
+        hipLaunchKernelGGL((k_ale_pass64<P, true>), g, b, lds, c->stream, pol, c->cell_start,
+                           c->cellof, rows, c->scal, c->tile_list, cap, c->counters);
+      else
+        hipLaunchKernelGGL((k_ale_pass64<P, false>), g, b, lds, c->stream, pol, c->cell_start,
+                           c->cellof, rows, c->scal, c->tile_list, cap, c->counters);
+    }
   });
 }
 

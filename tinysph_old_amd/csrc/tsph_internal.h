@@ -186,6 +186,13 @@ inline int total_tiles(const tsph_ctx *c) {
   return c->host_tile_base[c->ncases];
 }
 
+// whether a pair launch of `waves` persistent waves walks its tiles balanced:
+// the precision's setting, for grids of at least 8 tiles per wave (below that
+// the tickets cost more than the balance gains)
+inline bool walk_balanced(const tsph_ctx *c, bool fp32, int waves) {
+  return c->balanced[fp32 ? 1 : 0] && total_tiles(c) >= 8 * waves;
+}
+
 // tsph_cells.hip
 // hash, count, scan, scatter, order, reorder; eos: the reorder also runs
 // Density_to_pressure (and writes the mixed records) for the moved particles

@@ -711,7 +711,7 @@ int launch_forces(tsph_ctx *c, int variant) {
   } else {
     lds = kTabBytes + list + (size_t)cap * (3 * sizeof(double2) + 2 + (bt ? sizeof(double2) : 0));
   }
-  const bool bal = c->balanced[mixed ? 1 : 0];
+  const bool bal = walk_balanced(c, mixed, (int)g.x);
   return launch_timed(c, [&] {
     switch (variant) {
       case TSPH_FORCES_WCSPH: pair_launch<TSPH_FORCES_WCSPH>(c, bal, g, lds, rows, cap); break;

@@ -64,23 +64,17 @@ __device__ __forceinline__ int sel3(const int (&a)[3], int q) {
 }
 
 // Persistent waves: workgroup b runs on XCD b % 8, so the list of non-empty
-// tiles (in column order) is cut into 8 contiguous bands and the waves of
-// XCD x stride through band x; neighbouring tiles, which share two of their
-// three slices, are then staged from the same L2.
-__device__ __forceinline__ void band_of_wave(int ntl, int &it, int &end, int &step) {
-  const int x = blockIdx.x & 7;
-  it = (int)(((long long)ntl * x) >> 3) + (blockIdx.x >> 3);
-  end = (int)(((long long)ntl * (x + 1)) >> 3);
-  step = gridDim.x >> 3;
-}
-
-// Balanced form of the same banding.  Each band is split in two parts:
-//   * a static part, the first J whole rounds of the band (J = the rounds that
-//     fit in stat8/8 of it): the waves of XCD x stride through it as above,
-//     with no atomics;
-//   * a dynamic part, the rest: waves take its tiles in list order from an
-//     atomic ticket counter per band, and when their own band is empty they
-//     help the other bands in turn.
+// tiles is cut into 8 contiguous bands and the waves of XCD x walk band x;
+// tiles that are neighbours in the list, which share slices, are then staged
+// from the same L2.  Two walks:
+//   * static (DYN = false): the waves of XCD x stride through band x, no
+//     atomics;
+//   * balanced (DYN = true): each band is split in two parts:
+//       - a static part, the first J whole rounds of the band (J = the rounds
+//         that fit in stat8/8 of it), strided as in the static walk;
+//       - a dynamic part, the rest: waves take its tiles in list order from
+//         an atomic ticket counter per band, and when their own band is empty
+//         they help the other bands in turn.
 // A wave ends when all eight counters have passed their bands' ends; counters
 // only grow, so every wave ends.  The last wave to finish
 // resets the counters for the next launch (tq_end), so no clearing pass or
@@ -113,8 +107,6 @@ __device__ __forceinline__ int dyn_lo(const TileQueue &tq, int b) {
 
 __device__ __forceinline__ bool tq_static_next(const TileQueue &tq) { return tq.j + 1 < tq.J; }
 
-// The functions below take DYN = false for the purely static walk (every tile
-// of the band by striding, no atomics at all: band_of_wave's order).
 
 // lane 0 takes a ticket of the current band when the next tile is a dynamic
 // one (the value is read with tq_take)
@@ -248,10 +240,10 @@ __device__ __forceinline__ int slice_of(const Tile &t, int l) {
 // ascending index in a cell) and tests support exactly as the reference does;
 // walk32 visits the three column slices in memory order with an FP32 test.
 // --------------------------------------------------------------------------
-template <class P>
+template <bool DYN, class P>
 __device__ __forceinline__ void walk64(const P &pol, const Consts &k, const int *__restrict__ cell_start,
                                        const int *__restrict__ cellof, int rows,
-                                       const int *__restrict__ scal, const int *__restrict__ tile_list,
+                                       int *__restrict__ scal, const int *__restrict__ tile_list,
                                        int cap, unsigned long long *__restrict__ counters) {
   extern __shared__ __align__(16) unsigned char smem[];
   int *tab = reinterpret_cast<int *>(smem);
@@ -259,16 +251,17 @@ __device__ __forceinline__ void walk64(const P &pol, const Consts &k, const int 
   const typename P::Lds L = P::lds(smem + kTabBytes + sizeof(uint16_t) * kList * kWave, cap);
   if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&counters[C_PASSES], 1ull);
   unsigned npair = 0, ncand = 0, nact = 0;
-  int it, it_end, it_step;
-  band_of_wave(scal[S_NTILES], it, it_end, it_step);
-  for (; it < it_end; it += it_step) {
+  TileQueue tq = tq_begin(scal + S_TQ, scal[S_NTILES], kBalancedStatic8);
+  for (int it = tq_take<DYN>(tq, tq_ticket<DYN>(tq)); it >= 0;) {
     __syncthreads();
+    const int ticket = tq_ticket<DYN>(tq);  // read once this tile's loads are waited for
     Tile t;
     tile_begin(k, cell_start, tile_list[it], rows, tab, t);
     const bool staged = t.lq[3] <= cap;
     if (staged)
       for (int l = threadIdx.x; l < t.lq[3]; l += blockDim.x) pol.stage(L, l, l - sel3(t.base, slice_of(t, l)));
     __syncthreads();
+    const int next = tq_take<DYN>(tq, ticket);
     uint16_t *my_list = s_list + threadIdx.x;
     for (int i0 = t.own0; i0 < t.own1; i0 += kWave) {
       const int i = i0 + threadIdx.x;
@@ -331,14 +324,16 @@ __device__ __forceinline__ void walk64(const P &pol, const Consts &k, const int 
       }
       if (act) pol.finish(acc, o, i);
     }
+    it = next;
   }
+  tq_end<DYN>(tq);
   wave_count(npair, ncand, nact, counters);
 }
 
-template <class P>
+template <bool DYN, class P>
 __device__ __forceinline__ void walk32(const P &pol, const Consts &k, const int *__restrict__ cell_start,
                                        const int *__restrict__ cellof, int rows,
-                                       const int *__restrict__ scal, const int *__restrict__ tile_list,
+                                       int *__restrict__ scal, const int *__restrict__ tile_list,
                                        int cap, unsigned long long *__restrict__ counters) {
 #pragma clang fp contract(fast)
   extern __shared__ __align__(16) unsigned char smem[];
@@ -348,10 +343,10 @@ __device__ __forceinline__ void walk32(const P &pol, const Consts &k, const int 
   const float r2cut = 4.f * (float)k.h * (float)k.h;
   if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&counters[C_PASSES], 1ull);
   unsigned npair = 0, ncand = 0, nact = 0;
-  int it, it_end, it_step;
-  band_of_wave(scal[S_NTILES], it, it_end, it_step);
-  for (; it < it_end; it += it_step) {
+  TileQueue tq = tq_begin(scal + S_TQ, scal[S_NTILES], kBalancedStatic8);
+  for (int it = tq_take<DYN>(tq, tq_ticket<DYN>(tq)); it >= 0;) {
     __syncthreads();
+    const int ticket = tq_ticket<DYN>(tq);  // read once this tile's loads are waited for
     Tile t;
     tile_begin(k, cell_start, tile_list[it], rows, tab, t);
     const bool staged = t.lq[3] <= cap;
@@ -361,6 +356,7 @@ __device__ __forceinline__ void walk32(const P &pol, const Consts &k, const int 
       for (int l = threadIdx.x; l < t.lq[3]; l += blockDim.x)
         pol.stage(L, l, l - sel3(t.base, slice_of(t, l)), ox, oy);
     __syncthreads();
+    const int next = tq_take<DYN>(tq, ticket);
     uint16_t *my_list = s_list + threadIdx.x;
     for (int i0 = t.own0; i0 < t.own1; i0 += kWave) {
       const int i = i0 + threadIdx.x;
@@ -425,7 +421,9 @@ __device__ __forceinline__ void walk32(const P &pol, const Consts &k, const int 
       }
       if (act) pol.finish(acc, o, i);
     }
+    it = next;
   }
+  tq_end<DYN>(tq);
   wave_count(npair, ncand, nact, counters);
 }
 
