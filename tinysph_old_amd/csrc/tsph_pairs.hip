@@ -558,45 +558,47 @@ __global__ __launch_bounds__(kWave) void k_tile32(Soa d, Mixed mx, const int *__
     Acc32 s;
     while (__any(cur < tot)) {
       int cnt = 0;
-      // four candidates at a time while the list has room for four: their LDS
-      // loads are issued together
-      while (cur + 4 <= tot && cnt <= kList - 4) {
-        const int c0 = cur, c1 = cur + 1, c2 = cur + 2, c3 = cur + 3;
-        const int j0 = c0 + (c0 < n0 ? a0 : (c0 < n01 ? off1 : off2));
-        const int j1 = c1 + (c1 < n0 ? a0 : (c1 < n01 ? off1 : off2));
-        const int j2 = c2 + (c2 < n0 ? a0 : (c2 < n01 ? off1 : off2));
-        const int j3 = c3 + (c3 < n0 ? a0 : (c3 < n01 ? off1 : off2));
+      // Candidates in groups of up to four from one column slice (so one slot
+      // offset serves the group), while the list has room for four: their LDS
+      // loads are issued together, and the group is appended to the list
+      // without branches (every slot is written, the count advances by the
+      // test).  The particle itself is the candidate at distance exactly 0;
+      // "0 < r^2" drops it (and only a coincident particle with it, whose
+      // reference kernel gradient would be 0/0).
+      while (cur < tot && cnt <= kList - 4) {
+        const bool in0 = cur < n0, in01 = cur < n01;
+        const int end = in0 ? n0 : (in01 ? n01 : tot);
+        const int j0 = cur + (in0 ? a0 : (in01 ? off1 : off2));
+        const int m = min(4, end - cur);
+        // slots past the group's end (m < 4) are read but masked out: they lie
+        // within the LDS allocation (the staged records are followed by the
+        // other staged arrays)
+        const int j1 = j0 + 1, j2 = j0 + 2, j3 = j0 + 3;
         const float2 p0 = s_pos[j0], p1 = s_pos[j1], p2 = s_pos[j2], p3 = s_pos[j3];
         const float x0 = pi.x - p0.x, y0 = pi.y - p0.y, x1 = pi.x - p1.x, y1 = pi.y - p1.y;
         const float x2 = pi.x - p2.x, y2 = pi.y - p2.y, x3 = pi.x - p3.x, y3 = pi.y - p3.y;
-        bool k0 = x0 * x0 + y0 * y0 < cc.r2cut && j0 != li;
-        bool k1 = x1 * x1 + y1 * y1 < cc.r2cut && j1 != li;
-        bool k2 = x2 * x2 + y2 * y2 < cc.r2cut && j2 != li;
-        bool k3 = x3 * x3 + y3 * y3 < cc.r2cut && j3 != li;
+        const float q0 = x0 * x0 + y0 * y0, q1 = x1 * x1 + y1 * y1;
+        const float q2 = x2 * x2 + y2 * y2, q3 = x3 * x3 + y3 * y3;
+        bool k0 = (q0 > 0.f) & (q0 < cc.r2cut);
+        bool k1 = (m > 1) & (q1 > 0.f) & (q1 < cc.r2cut);
+        bool k2 = (m > 2) & (q2 > 0.f) & (q2 < cc.r2cut);
+        bool k3 = (m > 3) & (q3 > 0.f) & (q3 < cc.r2cut);
         if (VAR == TSPH_FORCES_DBC) {
           const bool w0 = dbc_wall && s_t[j0] == TSPH_WALL, w1 = dbc_wall && s_t[j1] == TSPH_WALL;
           const bool w2 = dbc_wall && s_t[j2] == TSPH_WALL, w3 = dbc_wall && s_t[j3] == TSPH_WALL;
-          ncand += (j0 != li && !w0) + (j1 != li && !w1) + (j2 != li && !w2) + (j3 != li && !w3);
+          ncand += (j0 != li && !w0) + (m > 1 && j1 != li && !w1) + (m > 2 && j2 != li && !w2) +
+                   (m > 3 && j3 != li && !w3);
           k0 = k0 && !w0; k1 = k1 && !w1; k2 = k2 && !w2; k3 = k3 && !w3;
         }
-        if (k0) my_list[(cnt++) * kWave] = (uint16_t)j0;
-        if (k1) my_list[(cnt++) * kWave] = (uint16_t)j1;
-        if (k2) my_list[(cnt++) * kWave] = (uint16_t)j2;
-        if (k3) my_list[(cnt++) * kWave] = (uint16_t)j3;
-        cur += 4;
-      }
-      while (cur < tot && cnt < kList) {
-        const int c = cur++;
-        const int j = c + (c < n0 ? a0 : (c < n01 ? off1 : off2));
-        const float2 pj = s_pos[j];
-        const float drx = pi.x - pj.x, dry = pi.y - pj.y;
-        bool in = drx * drx + dry * dry < cc.r2cut && j != li;
-        if (VAR == TSPH_FORCES_DBC) {
-          const bool ww = dbc_wall && s_t[j] == TSPH_WALL;
-          ncand += j != li && !ww;
-          in = in && !ww;
-        }
-        if (in) my_list[(cnt++) * kWave] = (uint16_t)j;
+        my_list[cnt * kWave] = (uint16_t)j0;
+        cnt += k0;
+        my_list[cnt * kWave] = (uint16_t)j1;
+        cnt += k1;
+        my_list[cnt * kWave] = (uint16_t)j2;
+        cnt += k2;
+        my_list[cnt * kWave] = (uint16_t)j3;
+        cnt += k3;
+        cur += m;
       }
       npair += cnt;
       // pair terms in list order; the next entry's records are loaded while
