@@ -92,6 +92,8 @@ typedef struct tsph_counters {
   uint64_t appended;     /* particles appended by inlet buffer zones */
   uint64_t zone_rejected;/* inlet appends refused: capacity or the per-pass list limit */
   uint64_t zone_chained; /* appended particles already past the zone start (not re-examined) */
+  uint64_t reinit_out_of_range; /* fluid densities outside [990, 1010] when reinitialised
+                                   (src/SPH_density_reinitialization.h:133-134) */
 } tsph_counters;
 
 typedef struct tsph_ctx tsph_ctx;
@@ -250,6 +252,47 @@ int tsph_bz_integrate_outlet(tsph_ctx *ctx, int zone, double dt);
  * with BZ_Integrate_inlet / _outlet after the predictor (zone -1: none). */
 int tsph_step_inflow(tsph_ctx *ctx, int wall, double dt, int nsteps, int inlet_zone,
                      int outlet_zone);
+
+/* ---- other WCSPH integrators and density reinitialisation ----
+ * The Verlet drivers keep a second time layer (v_oo, rho_oo) and the
+ * reinitialisation a per-particle buffer (SPH_special::rhoTemp); they exist on
+ * the device once tsph_history_enable has run, which sets them as
+ * Add_particle does (v_oo = v, rho_oo = rho, rhoTemp = rho). */
+typedef struct tsph_history {
+  int np;
+  double *v_oo;      /* 2*np */
+  double *rho_oo;
+  double *rho_temp;  /* rhoTemp */
+} tsph_history;
+int tsph_history_enable(tsph_ctx *ctx);
+/* in reference order; NULL arrays keep the device values */
+int tsph_history_upload(tsph_ctx *ctx, const tsph_history *host);
+int tsph_history_download(tsph_ctx *ctx, tsph_history *host);
+/* Integrate_Verlet / Integrate_Euler, src/SPH_integration.h:103-158 */
+int tsph_verlet(tsph_ctx *ctx, double dt);
+int tsph_euler(tsph_ctx *ctx, double dt);
+/* Integrate_LeapFrog_partOne/partTwo_withDensity, src/SPH_integration.h:161-230 */
+int tsph_leapfrog_one(tsph_ctx *ctx, double dt);
+int tsph_leapfrog_two(tsph_ctx *ctx, double dt, int step);
+/* Density_reinitialization_sumation / _overwrite,
+ * src/SPH_density_reinitialization.h:8-139 (the sum needs a cell list) */
+int tsph_density_reinit_sum(tsph_ctx *ctx);
+int tsph_density_reinit_overwrite(tsph_ctx *ctx);
+/* nsteps whole steps of the Verlet driver, steps step .. step+nsteps-1
+ * (cases/damBreakLong_mDBC/SPH_simulationVerlet.h:64-116 and the DBC / BT
+ * variants): wall density, cell list, EOS, pair loop, then Euler on step 1 and
+ * every 20th step, Verlet otherwise, then removal */
+int tsph_step_verlet(tsph_ctx *ctx, int wall, double dt, int step, int nsteps);
+/* nsteps whole steps of the leap-frog driver
+ * (cases/damBreakLong_mDBC/SPH_simulationLeapFrog.h and the DBC / BT variants):
+ * part one (from step 2), cell list, wall density, EOS, pair loop, part two,
+ * removal */
+int tsph_step_leapfrog(tsph_ctx *ctx, int wall, double dt, int step, int nsteps);
+/* nsteps symplectic steps, steps step .. step+nsteps-1, each followed on every
+ * `interval`-th step by the density reinitialisation
+ * (cases/damBreakLong_mDBC/SPH_simulationSymplectic_reinit.h, interval 20) */
+int tsph_step_symplectic_reinit(tsph_ctx *ctx, int wall, double dt, int step, int nsteps,
+                                int interval);
 
 /* ---- batched ensembles ----
  * Many independent cases (each with its own constants, domain, time step and

@@ -23,8 +23,11 @@ F_R, F_V, F_A, F_P, F_RHO, F_DRHO, F_RO, F_VO, F_RHOO, F_GNR, F_N = range(11)
  F_DOMEGAV) = range(11, 20)
 F_GRADRHO, F_GRADVX, F_GRADVY = 20, 21, 22
 GRAD_FIELDS = (F_GRADRHO, F_GRADVX, F_GRADVY)
+# Verlet second time layer and the density-reinitialisation buffer
+F_VOO, F_RHOOO, F_RHOTMP = 23, 24, 25
+HISTORY_FIELDS = (F_VOO, F_RHOOO, F_RHOTMP)
 VEC_FIELDS = {F_R, F_V, F_A, F_RO, F_VO, F_GNR, F_N, F_OMEGAV, F_OMEGAV_O, F_DOMEGAV, F_GRADRHO,
-              F_GRADVX, F_GRADVY}
+              F_GRADVX, F_GRADVY, F_VOO}
 ALE_FIELDS = tuple(range(11, 20))
 
 _dp = C.POINTER(C.c_double)
@@ -102,6 +105,13 @@ def _declare(lib: C.CDLL, pfx: str) -> None:
     f("ale_density_pressure", None, C.c_void_p, C.c_double)
     f("ale_leapfrog_one", None, C.c_void_p, C.c_double)
     f("ale_leapfrog_two", None, C.c_void_p, C.c_double, C.c_int)
+    # alternate WCSPH integrators and density reinitialisation
+    f("verlet", None, C.c_void_p, C.c_double)
+    f("euler", None, C.c_void_p, C.c_double)
+    f("leapfrog_one", None, C.c_void_p, C.c_double)
+    f("leapfrog_two", None, C.c_void_p, C.c_double, C.c_int)
+    f("reinit_sum", None, C.c_void_p)
+    f("reinit_overwrite", C.c_int if pfx == "orc_" else None, C.c_void_p)
     if pfx == "orc_":
         f("add_zone", C.c_int, C.c_void_p, *([C.c_double] * 8), C.c_int)
         f("bz_integrate_inlet", C.c_int, C.c_void_p, C.c_int, C.c_double)

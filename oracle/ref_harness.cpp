@@ -35,6 +35,7 @@
 #include "ALESPH_integration.h"
 #else
 #include "SPH_integration.h"
+#include "SPH_density_reinitialization.h"
 #endif
 #include "SPH_btdebug.h"
 #include "SPH_water_level.h"
@@ -66,7 +67,7 @@ struct RefCase {
 // Field ids: 0 r, 1 v, 2 a, 3 p, 4 rho, 5 drho, 6 r_o, 7 v_o, 8 rho_o, 9 gnr,
 // 10 n, ALE: 11 omega, 12 omegarho, 13 omegav, 14 omega_o, 15 omegarho_o,
 // 16 omegav_o, 17 domega, 18 domegarho, 19 domegav, MUSCL: 20 gradrho,
-// 21 gradvx, 22 gradvy.
+// 21 gradvx, 22 gradvy, history: 23 v_oo, 24 rho_oo, 25 rhoTemp.
 std::vector<realvec>* vec2_field(RefCase* c, int f) {
   switch (f) {
     case 0: return &c->ps.data.r;
@@ -82,6 +83,7 @@ std::vector<realvec>* vec2_field(RefCase* c, int f) {
     case 20: return &c->ps.special.gradrho;
     case 21: return &c->ps.special.gradvx;
     case 22: return &c->ps.special.gradvy;
+    case 23: return &c->ps.data.v_oo;
     default: return nullptr;
   }
 }
@@ -98,6 +100,8 @@ std::vector<real>* scal_field(RefCase* c, int f) {
     case 15: return &c->ps.special.omegarho_o;
     case 17: return &c->ps.special.domega;
     case 18: return &c->ps.special.domegarho;
+    case 24: return &c->ps.data.rho_oo;
+    case 25: return &c->ps.special.rhoTemp;
     default: return nullptr;
   }
 }
@@ -236,6 +240,16 @@ void ref_ale_leapfrog_two(void* vc, double dt, int step) {
 #else
 void ref_predictor(void* vc, double dt) { Integrate_SymplecticPredictor(static_cast<RefCase*>(vc)->ps, dt); }
 void ref_corrector(void* vc, double dt) { Integrate_SymplecticCorrector(static_cast<RefCase*>(vc)->ps, dt); }
+void ref_verlet(void* vc, double dt) { Integrate_Verlet(static_cast<RefCase*>(vc)->ps, dt); }
+void ref_euler(void* vc, double dt) { Integrate_Euler(static_cast<RefCase*>(vc)->ps, dt); }
+void ref_leapfrog_one(void* vc, double dt) {
+  Integrate_LeapFrog_partOne_withDensity(static_cast<RefCase*>(vc)->ps, dt);
+}
+void ref_leapfrog_two(void* vc, double dt, int step) {
+  Integrate_LeapFrog_partTwo_withDensity(static_cast<RefCase*>(vc)->ps, dt, step);
+}
+void ref_reinit_sum(void* vc) { Density_reinitialization_sumation(static_cast<RefCase*>(vc)->ps); }
+void ref_reinit_overwrite(void* vc) { Density_reinitialization_overwrite(static_cast<RefCase*>(vc)->ps); }
 #endif
 void ref_remove_out_of_domain(void* vc) {
   RefCase* c = static_cast<RefCase*>(vc);

@@ -66,6 +66,10 @@ struct Case {
   std::vector<V2> omegav, omegav_o, domegav;
   // ALE-SPH MUSCL gradients (src/SPH_data.h:68-70)
   std::vector<V2> gradrho, gradvx, gradvy;
+  // second time layer of the Verlet integrator (src/SPH_data.h:23,25) and the
+  // density-reinitialisation buffer (SPH_special::rhoTemp, src/SPH_data.h:82)
+  std::vector<V2> v_oo;
+  std::vector<double> rho_oo, rho_tmp;
   int ncx = 0, ncy = 0;
   std::vector<int> cstart, citems;  // CSR cell lists
   int nthreads = 1;
@@ -871,7 +875,10 @@ void add_particle(Case& s, int type, double p, double rho, V2 r, V2 v, V2 a) {
   s.a.push_back(a);
   s.r_o.push_back(r);
   s.rho_o.push_back(rho);
+  s.rho_oo.push_back(rho);
   s.v_o.push_back(v);
+  s.v_oo.push_back(v);
+  s.rho_tmp.push_back(rho);
   s.gnr.push_back(V2{0., 0.});
   s.n.push_back(V2{0., 0.});
   if (!s.omega.empty()) {
@@ -888,6 +895,7 @@ void erase_particle(Case& s, int p) {
   auto er = [p](auto& vec) { if ((int)vec.size() > p) vec.erase(vec.begin() + p); };
   er(s.type); er(s.index); er(s.p); er(s.rho); er(s.drho); er(s.r); er(s.v); er(s.a);
   er(s.r_o); er(s.v_o); er(s.rho_o); er(s.gnr); er(s.n);
+  er(s.v_oo); er(s.rho_oo); er(s.rho_tmp);
   if (!s.omega.empty()) {
     er(s.omega); er(s.omegarho); er(s.omegav); er(s.omega_o); er(s.omegarho_o); er(s.omegav_o);
     er(s.domega); er(s.domegarho); er(s.domegav); er(s.gradrho); er(s.gradvx); er(s.gradvy);
@@ -1032,6 +1040,7 @@ void orc_load(void* vc, int np, const int* type, const double* p, const double* 
     s.v[i] = {v[2 * i], v[2 * i + 1]};
   }
   s.r_o = s.r; s.v_o = s.v;
+  s.v_oo = s.v; s.rho_oo = s.rho; s.rho_tmp = s.rho;
   s.gnr.assign(np, V2{0., 0.});
   s.n.assign(np, V2{0., 0.});
   if (gnr) for (int i = 0; i < np; ++i) s.gnr[i] = {gnr[2 * i], gnr[2 * i + 1]};
@@ -1041,18 +1050,20 @@ void orc_load(void* vc, int np, const int* type, const double* p, const double* 
 // Field ids as the reference harness: 0 r, 1 v, 2 a, 3 p, 4 rho, 5 drho, 6 r_o,
 // 7 v_o, 8 rho_o, 9 gnr, 10 n.
 // ALE fields: 11 omega, 12 omegarho, 13 omegav, 14 omega_o, 15 omegarho_o,
-// 16 omegav_o, 17 domega, 18 domegarho, 19 domegav
+// 16 omegav_o, 17 domega, 18 domegarho, 19 domegav, MUSCL: 20 gradrho,
+// 21 gradvx, 22 gradvy; history: 23 v_oo, 24 rho_oo, 25 rhoTemp
 static std::vector<V2>* v2f(Case& s, int f) {
   switch (f) { case 0: return &s.r; case 1: return &s.v; case 2: return &s.a; case 6: return &s.r_o;
     case 7: return &s.v_o; case 9: return &s.gnr; case 10: return &s.n; case 13: return &s.omegav;
     case 16: return &s.omegav_o; case 19: return &s.domegav; case 20: return &s.gradrho;
-    case 21: return &s.gradvx; case 22: return &s.gradvy; default: return nullptr; }
+    case 21: return &s.gradvx; case 22: return &s.gradvy; case 23: return &s.v_oo;
+    default: return nullptr; }
 }
 static std::vector<double>* sf(Case& s, int f) {
   switch (f) { case 3: return &s.p; case 4: return &s.rho; case 5: return &s.drho; case 8: return &s.rho_o;
     case 11: return &s.omega; case 12: return &s.omegarho; case 14: return &s.omega_o;
     case 15: return &s.omegarho_o; case 17: return &s.domega; case 18: return &s.domegarho;
-    default: return nullptr; }
+    case 24: return &s.rho_oo; case 25: return &s.rho_tmp; default: return nullptr; }
 }
 
 // ALE volume variables as ALESPH_Add_particle sets them
@@ -1224,6 +1235,118 @@ void orc_corrector(void* vc, double dt) {
   }
 }
 
+// Integrate_Verlet, src/SPH_integration.h:103-128 (all particle types)
+void orc_verlet(void* vc, double dt) {
+  Case& s = *static_cast<Case*>(vc);
+#pragma omp parallel for schedule(static) num_threads(s.nthreads)
+  for (int p = 0; p < s.np; ++p) {
+    s.v[p] = s.v_oo[p] + s.a[p] * dt * 2;
+    s.r[p] = s.r[p] + s.v_o[p] * dt + s.a[p] * dt * dt * 0.5;
+    s.rho[p] = s.rho_oo[p] + s.drho[p] * dt * 2;
+    s.v_oo[p] = s.v_o[p];
+    s.v_o[p] = s.v[p];
+    s.rho_oo[p] = s.rho_o[p];
+    s.rho_o[p] = s.rho[p];
+  }
+}
+
+// Integrate_Euler, src/SPH_integration.h:130-158 (all particle types)
+void orc_euler(void* vc, double dt) {
+  Case& s = *static_cast<Case*>(vc);
+#pragma omp parallel for schedule(static) num_threads(s.nthreads)
+  for (int p = 0; p < s.np; ++p) {
+    s.v[p] = s.v[p] + s.a[p] * dt;
+    s.r[p] = s.r[p] + s.v_o[p] * dt + s.a[p] * dt * dt * 0.5;
+    s.rho[p] = s.rho[p] + s.drho[p] * dt;
+    s.v_oo[p] = s.v_o[p];
+    s.v_o[p] = s.v[p];
+    s.rho_oo[p] = s.rho_o[p];
+    s.rho_o[p] = s.rho[p];
+  }
+}
+
+// Integrate_LeapFrog_partOne_withDensity, src/SPH_integration.h:161-183
+void orc_leapfrog_one(void* vc, double dt) {
+  Case& s = *static_cast<Case*>(vc);
+#pragma omp parallel for schedule(static) num_threads(s.nthreads)
+  for (int p = 0; p < s.np; ++p) {
+    s.v_o[p] = s.v[p];
+    s.rho_o[p] = s.rho[p];
+    s.v[p] = s.v[p] + s.a[p] * dt * 0.5;
+    s.rho[p] += s.drho[p] * dt * 0.5;
+  }
+}
+
+// Integrate_LeapFrog_partTwo_withDensity, src/SPH_integration.h:185-230
+void orc_leapfrog_two(void* vc, double dt, int step) {
+  Case& s = *static_cast<Case*>(vc);
+  if (step == 1) {
+    for (int p = 0; p < s.np; ++p) {
+      s.v[p] = s.v[p] + s.a[p] * dt * 0.5;
+      s.r[p] = s.r[p] + s.v[p] * dt * 0.5;
+    }
+  } else {
+#pragma omp parallel for schedule(static) num_threads(s.nthreads)
+    for (int p = 0; p < s.np; ++p) {
+      s.r[p] = s.r[p] + s.v[p] * dt;
+      s.v[p] = s.v_o[p] + s.a[p] * dt;
+      s.rho[p] = s.rho_o[p] + s.drho[p] * dt;
+    }
+  }
+}
+
+// Density_reinitialization_sumation, src/SPH_density_reinitialization.h:8-121:
+// per fluid particle, over the 9 cells in stencil order (cl < 0 skipped; the
+// reference reads past the last cell without a check, which no case reaches)
+// and ascending index in a cell, every other particle adds m W to the density
+// sum and m W / rho_j to the volume sum (W = 0 outside the support, W = awen
+// for a coincident particle); rhoTemp = their ratio where the volume sum is
+// non-zero, otherwise rhoTemp keeps its value.
+void orc_reinit_sum(void* vc) {
+  Case& s = *static_cast<Case*>(vc);
+  const double h = s.q.h, m = s.q.m;
+  const int nc = ncells(s);
+#pragma omp parallel for schedule(dynamic, 64) num_threads(s.nthreads)
+  for (int c = 0; c < nc; ++c) {
+    int ac[9];
+    stencil(c, s.ncy, ac);
+    for (int k = s.cstart[c]; k < s.cstart[c + 1]; ++k) {
+      const int ai = s.citems[k];
+      if (s.type[ai] != T_FLUID) continue;
+      const V2 ar = s.r[ai];
+      double rho_sum = 0, wv_sum = 0;
+      for (int q = 0; q < 9; ++q) {
+        const int cl = ac[q];
+        if (cl < 0 || cl >= nc) continue;
+        for (int u = s.cstart[cl]; u < s.cstart[cl + 1]; ++u) {
+          const int ni = s.citems[u];
+          if (ai == ni) continue;
+          const V2 dr = ar - s.r[ni];
+          const double drs = std::sqrt((dr.x * dr.x) + (dr.y * dr.y));
+          const double W = wendland(drs, h).w;
+          rho_sum += m * W;
+          wv_sum += m * W / s.rho[ni];
+        }
+      }
+      if (wv_sum != 0) s.rho_tmp[ai] = rho_sum / wv_sum;
+    }
+  }
+}
+
+// Density_reinitialization_overwrite, src/SPH_density_reinitialization.h:123-139;
+// returns the fluid particles whose density before the overwrite lay outside
+// [990, 1010] (the reference prints a warning for each)
+int orc_reinit_overwrite(void* vc) {
+  Case& s = *static_cast<Case*>(vc);
+  int bad = 0;
+  for (int p = 0; p < s.np; ++p) {
+    if (s.type[p] != T_FLUID) continue;
+    if (s.rho[p] < 990 || s.rho[p] > 1010) ++bad;
+    s.rho[p] = s.rho_tmp[p];
+  }
+  return bad;
+}
+
 // src/SPH_btdebug.h:4-31 with Remove_particle, src/SPH_particle_system.h:110-146.
 // The erase happens inside the ascending loop without stepping back, so the
 // particle that slides into slot p is never examined.  gnr / n are not erased
@@ -1241,6 +1364,7 @@ int orc_remove_out_of_domain(void* vc) {
       auto er = [p](auto& vec) { vec.erase(vec.begin() + p); };
       er(s.type); er(s.index); er(s.p); er(s.rho); er(s.drho); er(s.r); er(s.v); er(s.a);
       er(s.r_o); er(s.v_o); er(s.rho_o); er(s.gnr); er(s.n);
+      er(s.v_oo); er(s.rho_oo); er(s.rho_tmp);  // src/SPH_particle_system.h:139-144
       // ALE fields erased with the particle (ALESPH_Remove_particle semantics,
       // src/SPH_particle_system.h:148-184; the ALE driver's plain removal
       // leaves them unerased, which misaligns them in the reference)

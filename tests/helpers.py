@@ -115,3 +115,43 @@ def oracle_inflow_step(O, wall: int, dt: float, inlet: int = 0, outlet: int = 1)
     assert O.call("particle_to_cell") == 0
     O.call("density_to_pressure")
     O.call("compute_forces", var)
+
+
+def oracle_verlet_step(O, wall: int, dt: float, step: int) -> None:
+    """One Verlet-driver step (cases/damBreakLong_mDBC/SPH_simulationVerlet.h:64-116;
+    the BT variant adds an EOS after the wall density, the DBC variant has none):
+    Euler on step 1 and every 20th step, Verlet otherwise."""
+    if wall == WALL_MDBC:
+        O.call("mdbc_density")
+    elif wall == WALL_BT:
+        O.call("bt_density")
+        O.call("density_to_pressure")
+    O.call("clear_cells")
+    assert O.call("particle_to_cell") == 0
+    O.call("density_to_pressure")
+    O.call("compute_forces", oracle_variant(wall))
+    O.call("verlet" if step > 1 and step % 20 != 0 else "euler", dt)
+    O.call("remove_out_of_domain")
+
+
+def oracle_leapfrog_step(O, wall: int, dt: float, step: int) -> None:
+    """One leap-frog-driver step (cases/damBreakLong_mDBC/SPH_simulationLeapFrog.h,
+    cases/stillWater_DBC/SPH_simulationLeapFrog.h)."""
+    if step > 1:
+        O.call("leapfrog_one", dt)
+    O.call("clear_cells")
+    assert O.call("particle_to_cell") == 0
+    if wall == WALL_MDBC:
+        O.call("mdbc_density")
+    elif wall == WALL_BT:
+        O.call("bt_density")
+    O.call("density_to_pressure")
+    O.call("compute_forces", oracle_variant(wall))
+    O.call("leapfrog_two", dt, step)
+    O.call("remove_out_of_domain")
+
+
+def oracle_reinit(O) -> int:
+    """Density_reinitialization_sumation + _overwrite; returns the out-of-range count."""
+    O.call("reinit_sum")
+    return O.call("reinit_overwrite")

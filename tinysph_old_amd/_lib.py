@@ -43,7 +43,13 @@ class Particles(C.Structure):
 class Counters(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in ("pairs", "candidates", "active", "pair_passes",
                                            "density_out_of_range", "outside_grid", "removed",
-                                           "appended", "zone_rejected", "zone_chained")]
+                                           "appended", "zone_rejected", "zone_chained",
+                                           "reinit_out_of_range")]
+
+
+class History(C.Structure):
+    _fields_ = [("np", C.c_int), ("v_oo", C.POINTER(C.c_double)), ("rho_oo", C.POINTER(C.c_double)),
+                ("rho_temp", C.POINTER(C.c_double))]
 
 
 class Zone(C.Structure):
@@ -133,6 +139,19 @@ EXPORTS = {
     "tsph_autotune": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                 C.POINTER(C.c_int), _dp]),
     "tsph_set_balanced": (C.c_int, [C.c_void_p, C.c_int]),
+    "tsph_history_enable": (C.c_int, [C.c_void_p]),
+    "tsph_history_upload": (C.c_int, [C.c_void_p, C.POINTER(History)]),
+    "tsph_history_download": (C.c_int, [C.c_void_p, C.POINTER(History)]),
+    "tsph_verlet": (C.c_int, [C.c_void_p, C.c_double]),
+    "tsph_euler": (C.c_int, [C.c_void_p, C.c_double]),
+    "tsph_leapfrog_one": (C.c_int, [C.c_void_p, C.c_double]),
+    "tsph_leapfrog_two": (C.c_int, [C.c_void_p, C.c_double, C.c_int]),
+    "tsph_density_reinit_sum": (C.c_int, [C.c_void_p]),
+    "tsph_density_reinit_overwrite": (C.c_int, [C.c_void_p]),
+    "tsph_step_verlet": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_int]),
+    "tsph_step_leapfrog": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_int]),
+    "tsph_step_symplectic_reinit": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_int,
+                                              C.c_int]),
 }
 
 _LIB = None

@@ -52,7 +52,7 @@ static void free_soa(Soa &s) {
   void *ps[] = {s.cid, s.r, s.v, s.a, s.r_o, s.v_o, s.gnr, s.n, s.rho, s.drho, s.p, s.rho_o,
                 s.type, s.seq, s.index, s.omega, s.omegarho, s.omega_o, s.omegarho_o,
                 s.domega, s.domegarho, s.omegav, s.omegav_o, s.domegav, s.gradrho, s.gradvx,
-                s.gradvy};
+                s.gradvy, s.v_oo, s.rho_oo, s.rho_tmp};
   for (void *p : ps)
     if (p) (void)hipFree(p);
   s = Soa{};
@@ -60,7 +60,9 @@ static void free_soa(Soa &s) {
 
 static void drop_graph(tsph_ctx *c) {
   for (Graph *g : {&c->graph[0], &c->graph[1], &c->rale_graph[0], &c->rale_graph[1],
-                   &c->ale_graph[0], &c->ale_graph[1], &c->inflow_graph[0], &c->inflow_graph[1]}) {
+                   &c->ale_graph[0], &c->ale_graph[1], &c->inflow_graph[0], &c->inflow_graph[1],
+                   &c->verlet_graph[0], &c->verlet_graph[1], &c->euler_graph[0],
+                   &c->euler_graph[1], &c->lf_graph[0], &c->lf_graph[1]}) {
     if (g->exec) (void)hipGraphExecDestroy(g->exec);
     *g = Graph{};
   }
@@ -201,6 +203,43 @@ static int record_step(tsph_ctx *c, int wall, double dt, int inlet = -1, int out
   if ((e = launch_cell_list(c, true)) || (e = launch_forces(c, variant))) return e;
   if ((e = launch_corrector(c, dt)) || (e = launch_remove(c))) return e;
   if ((e = launch_cell_list(c, true)) || (e = launch_forces(c, variant))) return e;
+  return 0;
+}
+
+static int forces_variant_of(int wall) {
+  return wall == TSPH_WALL_DBC ? TSPH_FORCES_DBC : wall == TSPH_WALL_MDBC ? TSPH_FORCES_WCSPH
+                                                                         : TSPH_FORCES_BT;
+}
+
+// one Verlet-driver step, cases/damBreakLong_mDBC/SPH_simulationVerlet.h:64-116
+// (mDBC: no EOS before the cell list), cases/damBreakLong_BT/SPH_simulationVerlet.h
+// (BT: wall density and EOS), cases/stillWater_DBC/SPH_simulationVerlet.h (DBC)
+static int record_verlet_step(tsph_ctx *c, int wall, double dt, bool euler) {
+  int e = 0;
+  if (wall == TSPH_WALL_MDBC) {
+    if ((e = launch_mdbc(c))) return e;
+  } else if (wall == TSPH_WALL_BT) {
+    if ((e = launch_bt_density(c)) || (e = launch_eos(c))) return e;
+  }
+  if ((e = launch_cell_list(c, true)) || (e = launch_forces(c, forces_variant_of(wall)))) return e;
+  if ((e = launch_verlet(c, dt, euler)) || (e = launch_remove(c))) return e;
+  return 0;
+}
+
+// one leap-frog-driver step, cases/damBreakLong_mDBC/SPH_simulationLeapFrog.h
+// (the wall density follows the cell list), cases/stillWater_DBC/SPH_simulationLeapFrog.h
+static int record_leapfrog_step(tsph_ctx *c, int wall, double dt, int step) {
+  int e = 0;
+  if (step > 1 && (e = launch_leapfrog_one(c, dt))) return e;
+  if (wall == TSPH_WALL_DBC) {
+    if ((e = launch_cell_list(c, true))) return e;
+  } else {
+    if ((e = launch_cell_list(c, false))) return e;
+    if ((e = wall == TSPH_WALL_MDBC ? launch_mdbc(c) : launch_bt_density(c)) || (e = launch_eos(c)))
+      return e;
+  }
+  if ((e = launch_forces(c, forces_variant_of(wall)))) return e;
+  if ((e = launch_leapfrog_two(c, dt, step)) || (e = launch_remove(c))) return e;
   return 0;
 }
 
@@ -873,6 +912,7 @@ int tsph_get_counters(tsph_ctx *c, tsph_counters *out) {
   out->appended = v[C_APPENDED];
   out->zone_rejected = v[C_BZREJECT];
   out->zone_chained = v[C_BZCHAIN];
+  out->reinit_out_of_range = v[C_REINIT];
   return 0;
 }
 
@@ -948,6 +988,172 @@ int tsph_pair_timing_read(tsph_ctx *c, double *total_ms, int *launches) {
   return 0;
 }
 
+
+// ---- other WCSPH integrators and density reinitialisation ---------------------
+
+int tsph_history_enable(tsph_ctx *c) {
+  if (!c) return fail(TSPH_EINVAL, "null argument");
+  if (int e = single_only(c)) return e;
+  int e = 0;
+  if (!c->has_history) {
+    for (Soa *d : {&c->cur, &c->alt})
+      if ((e = dalloc(&d->v_oo, c->capacity)) || (e = dalloc(&d->rho_oo, c->capacity)) ||
+          (e = dalloc(&d->rho_tmp, c->capacity)))
+        return e;
+    c->has_history = true;
+  }
+  drop_graph(c);
+  return launch_history_init(c);
+}
+
+static int history_ready(tsph_ctx *c) {
+  if (!c) return fail(TSPH_EINVAL, "null argument");
+  if (int e = single_only(c)) return e;
+  if (!c->has_history) return fail(TSPH_ESTATE, "history fields not enabled (tsph_history_enable)");
+  return 0;
+}
+
+int tsph_history_upload(tsph_ctx *c, const tsph_history *h) {
+  if (int e = history_ready(c)) return e;
+  if (!h) return fail(TSPH_EINVAL, "null argument");
+  int e = 0, n = 0;
+  TSPH_HIP(hipStreamSynchronize(c->stream));
+  TSPH_HIP(hipMemcpy(&n, &c->scal[S_NP], sizeof(int), hipMemcpyDeviceToHost));
+  std::vector<int> order;
+  std::vector<uint32_t> seq;
+  std::vector<uint8_t> type;
+  if ((e = live_order(c, n, -1, order, seq, type))) return e;
+  const int m = (int)order.size();
+  if (h->np != m) return fail(TSPH_EINVAL, "history field count differs from the particle count");
+  std::vector<double> b(2 * (size_t)n + 2, 0.0);
+  auto put = [&](void *dst, const double *src, int w) -> int {
+    if (!src || !n) return 0;
+    TSPH_HIP(hipMemcpy(b.data(), dst, w * sizeof(double) * n, hipMemcpyDeviceToHost));
+    for (int k = 0; k < m; ++k)
+      for (int u = 0; u < w; ++u) b[(size_t)w * order[k] + u] = src[(size_t)w * k + u];
+    TSPH_HIP(hipMemcpy(dst, b.data(), w * sizeof(double) * n, hipMemcpyHostToDevice));
+    return 0;
+  };
+  const Soa &d = c->cur;
+  if ((e = put(d.v_oo, h->v_oo, 2)) || (e = put(d.rho_oo, h->rho_oo, 1)) ||
+      (e = put(d.rho_tmp, h->rho_temp, 1)))
+    return e;
+  return 0;
+}
+
+int tsph_history_download(tsph_ctx *c, tsph_history *h) {
+  if (int e = history_ready(c)) return e;
+  if (!h) return fail(TSPH_EINVAL, "null argument");
+  int e = 0, n = 0;
+  TSPH_HIP(hipStreamSynchronize(c->stream));
+  TSPH_HIP(hipMemcpy(&n, &c->scal[S_NP], sizeof(int), hipMemcpyDeviceToHost));
+  std::vector<int> order;
+  std::vector<uint32_t> seq;
+  std::vector<uint8_t> type;
+  if ((e = live_order(c, n, -1, order, seq, type))) return e;
+  const int m = (int)order.size();
+  h->np = m;
+  std::vector<double> b(2 * (size_t)n + 2);
+  auto get = [&](double *dst, const void *src, int w) -> int {
+    if (!dst || !n) return 0;
+    TSPH_HIP(hipMemcpy(b.data(), src, w * sizeof(double) * n, hipMemcpyDeviceToHost));
+    for (int k = 0; k < m; ++k)
+      for (int u = 0; u < w; ++u) dst[(size_t)w * k + u] = b[(size_t)w * order[k] + u];
+    return 0;
+  };
+  const Soa &d = c->cur;
+  if ((e = get(h->v_oo, d.v_oo, 2)) || (e = get(h->rho_oo, d.rho_oo, 1)) ||
+      (e = get(h->rho_temp, d.rho_tmp, 1)))
+    return e;
+  return 0;
+}
+
+int tsph_verlet(tsph_ctx *c, double dt) {
+  if (int e = history_ready(c)) return e;
+  return launch_verlet(c, dt, false);
+}
+
+int tsph_euler(tsph_ctx *c, double dt) {
+  if (int e = history_ready(c)) return e;
+  return launch_verlet(c, dt, true);
+}
+
+int tsph_leapfrog_one(tsph_ctx *c, double dt) {
+  if (!c) return fail(TSPH_EINVAL, "null argument");
+  if (int e = single_only(c)) return e;
+  return launch_leapfrog_one(c, dt);
+}
+
+int tsph_leapfrog_two(tsph_ctx *c, double dt, int step) {
+  if (!c) return fail(TSPH_EINVAL, "null argument");
+  if (int e = single_only(c)) return e;
+  return launch_leapfrog_two(c, dt, step);
+}
+
+int tsph_density_reinit_sum(tsph_ctx *c) {
+  if (int e = history_ready(c)) return e;
+  if (!c->cells_valid) return fail(TSPH_ESTATE, "no cell list: call tsph_particle_to_cell first");
+  return launch_reinit_sum(c);
+}
+
+int tsph_density_reinit_overwrite(tsph_ctx *c) {
+  if (int e = history_ready(c)) return e;
+  return launch_reinit_overwrite(c);
+}
+
+static int step_checks(tsph_ctx *c, int wall, int step, int nsteps) {
+  if (!c) return fail(TSPH_EINVAL, "null argument");
+  if (int e = single_only(c)) return e;
+  if (wall < TSPH_WALL_DBC || wall > TSPH_WALL_BT) return fail(TSPH_EINVAL, "unknown wall treatment");
+  if (step < 1 || nsteps < 0) return fail(TSPH_EINVAL, "steps are numbered from 1");
+  if (!c->cells_valid) return fail(TSPH_ESTATE, "no cell list: call tsph_particle_to_cell first");
+  if (wall == TSPH_WALL_MDBC && !c->has_gnr) return fail(TSPH_ESTATE, "mDBC needs ghost nodes (gnr)");
+  if (wall == TSPH_WALL_BT && !c->has_n) return fail(TSPH_ESTATE, "BT needs wall normals (n)");
+  return 0;
+}
+
+int tsph_step_verlet(tsph_ctx *c, int wall, double dt, int step, int nsteps) {
+  if (int e = step_checks(c, wall, step, nsteps)) return e;
+  if (!c->has_history) return fail(TSPH_ESTATE, "history fields not enabled (tsph_history_enable)");
+  for (int s = step; s < step + nsteps; ++s) {
+    // src: `if(step > 1 && step%20 != 0) Verlet else Euler`
+    const bool euler = !(s > 1 && s % 20 != 0);
+    if (c->time_pairs) {
+      if (int e = record_verlet_step(c, wall, dt, euler)) return e;
+      continue;
+    }
+    if (int e = replay_step(c, euler ? c->euler_graph : c->verlet_graph, 500 + 4 * wall + euler, dt,
+                            1, [&] { return record_verlet_step(c, wall, dt, euler); }))
+      return e;
+  }
+  return 0;
+}
+
+int tsph_step_leapfrog(tsph_ctx *c, int wall, double dt, int step, int nsteps) {
+  if (int e = step_checks(c, wall, step, nsteps)) return e;
+  int s = step;
+  const int last = step + nsteps;
+  // the first step differs (no part one; part two's first-step branch): launched eagerly
+  for (; s < last && (s == 1 || c->time_pairs); ++s)
+    if (int e = record_leapfrog_step(c, wall, dt, s)) return e;
+  if (s >= last) return 0;
+  return replay_step(c, c->lf_graph, 520 + wall, dt, last - s,
+                     [&] { return record_leapfrog_step(c, wall, dt, 2); });
+}
+
+int tsph_step_symplectic_reinit(tsph_ctx *c, int wall, double dt, int step, int nsteps,
+                                int interval) {
+  if (int e = step_checks(c, wall, step, nsteps)) return e;
+  if (!c->has_history) return fail(TSPH_ESTATE, "history fields not enabled (tsph_history_enable)");
+  if (interval < 1) return fail(TSPH_EINVAL, "reinitialisation interval must be positive");
+  for (int s = step; s < step + nsteps; ++s) {
+    int e = c->time_pairs ? record_step(c, wall, dt)
+                          : replay_step(c, c->graph, wall, dt, 1, [&] { return record_step(c, wall, dt); });
+    if (e) return e;
+    if (s % interval == 0 && ((e = launch_reinit_sum(c)) || (e = launch_reinit_overwrite(c)))) return e;
+  }
+  return 0;
+}
 
 // ---- batched ensembles -------------------------------------------------------
 

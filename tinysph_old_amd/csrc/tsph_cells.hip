@@ -305,6 +305,11 @@ __global__ void k_reorder(const int *__restrict__ cell_start, int ncells, const 
         dst.gradvy[k] = src.gradvy[j];
       }
     }
+    if (src.v_oo) {
+      dst.v_oo[k] = src.v_oo[j];
+      dst.rho_oo[k] = src.rho_oo[j];
+      dst.rho_tmp[k] = src.rho_tmp[j];
+    }
   }
   const unsigned long long m = __ballot(wall);
   if (m) {

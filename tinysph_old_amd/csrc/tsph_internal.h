@@ -33,7 +33,7 @@ enum { S_NP = 0, S_MAXSEQ = 1, S_NTILES = 2, S_NWALL = 3, S_BZCNT = 4, S_TQ = 32
 // device counter slots (uint64)
 enum {
   C_PAIRS = 0, C_CAND = 1, C_ACTIVE = 2, C_PASSES = 3, C_BADRHO = 4, C_OUTGRID = 5,
-  C_REMOVED = 6, C_APPENDED = 7, C_BZREJECT = 8, C_BZCHAIN = 9, C_NCOUNT = 10
+  C_REMOVED = 6, C_APPENDED = 7, C_BZREJECT = 8, C_BZCHAIN = 9, C_REINIT = 10, C_NCOUNT = 11
 };
 
 // buffer zone (BufferZone, src/SPH_inlet_outlet_data.h:18-46)
@@ -57,6 +57,10 @@ struct Soa {
   double2 *omegav = nullptr, *omegav_o = nullptr, *domegav = nullptr;
   // ALE-SPH MUSCL gradients of rho, vx, vy (allocated with the volume variables)
   double2 *gradrho = nullptr, *gradvx = nullptr, *gradvy = nullptr;
+  // Verlet second time layer and the density-reinitialisation buffer (rhoTemp)
+  // (allocated by tsph_history_enable)
+  double2 *v_oo = nullptr;
+  double *rho_oo = nullptr, *rho_tmp = nullptr;
 };
 
 // FP32 pair records of the mixed path, rebuilt after every reorder.
@@ -161,6 +165,10 @@ struct tsph_ctx {
   tsph::Graph rale_graph[2];  // Riemann-ALE step, by parity at the step's start
   tsph::Graph ale_graph[2];   // ALE-SPH MUSCL step, by parity at the step's start
   tsph::Graph inflow_graph[2];  // symplectic step with buffer zones
+  tsph::Graph verlet_graph[2];  // Verlet-driver step (Verlet integrator)
+  tsph::Graph euler_graph[2];   // Verlet-driver step on its Euler steps
+  tsph::Graph lf_graph[2];      // leap-frog-driver step (after the first)
+  bool has_history = false;     // v_oo, rho_oo, rhoTemp allocated
   tsph::Zone zones[TSPH_MAX_ZONES];
   int n_zones = 0;
   int *bz_list = nullptr;     // 2 * kBzMax ints: {seq, slot} of the crossing zone particles
@@ -221,6 +229,13 @@ int launch_ale_acceleration(tsph_ctx *c, int muscl);
 int launch_ale_gradient(tsph_ctx *c);
 int launch_ale_density_pressure(tsph_ctx *c, double dt);
 int launch_ale_leapfrog_one(tsph_ctx *c, double dt);
+// tsph_integrators.hip
+int launch_history_init(tsph_ctx *c);
+int launch_verlet(tsph_ctx *c, double dt, bool euler);
+int launch_leapfrog_one(tsph_ctx *c, double dt);
+int launch_leapfrog_two(tsph_ctx *c, double dt, int step);
+int launch_reinit_sum(tsph_ctx *c);
+int launch_reinit_overwrite(tsph_ctx *c);
 int launch_ale_leapfrog_two(tsph_ctx *c, double dt, int step);
 // tsph_zones.hip
 int launch_bz_inlet(tsph_ctx *c, int zone, double dt);

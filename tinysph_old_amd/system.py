@@ -356,6 +356,48 @@ class ParticleSystem:
     def step_ale(self, dt: float, step: int, nsteps: int = 1, muscl: int = 0):
         L.check(self.lib.tsph_step_ale(self.ctx, dt, step, nsteps, muscl), "tsph_step_ale")
 
+    # ---- other WCSPH integrators and density reinitialisation
+    # (src/SPH_integration.h:103-230, src/SPH_density_reinitialization.h) ----
+    def history_enable(self):
+        L.check(self.lib.tsph_history_enable(self.ctx), "tsph_history_enable")
+
+    def history_download(self) -> dict:
+        n = self.np
+        voo, roo, rt = np.zeros(max(2 * n, 1)), np.zeros(max(n, 1)), np.zeros(max(n, 1))
+        h = L.History(n, L.dptr(voo), L.dptr(roo), L.dptr(rt))
+        L.check(self.lib.tsph_history_download(self.ctx, C.byref(h)), "tsph_history_download")
+        m = h.np
+        return {"v_oo": voo[: 2 * m].reshape(m, 2), "rho_oo": roo[:m], "rho_temp": rt[:m]}
+
+    def verlet(self, dt: float):
+        L.check(self.lib.tsph_verlet(self.ctx, dt), "Integrate_Verlet")
+
+    def euler(self, dt: float):
+        L.check(self.lib.tsph_euler(self.ctx, dt), "Integrate_Euler")
+
+    def leapfrog_one(self, dt: float):
+        L.check(self.lib.tsph_leapfrog_one(self.ctx, dt), "Integrate_LeapFrog_partOne_withDensity")
+
+    def leapfrog_two(self, dt: float, step: int):
+        L.check(self.lib.tsph_leapfrog_two(self.ctx, dt, step),
+                "Integrate_LeapFrog_partTwo_withDensity")
+
+    def density_reinit(self):
+        L.check(self.lib.tsph_density_reinit_sum(self.ctx), "Density_reinitialization_sumation")
+        L.check(self.lib.tsph_density_reinit_overwrite(self.ctx),
+                "Density_reinitialization_overwrite")
+
+    def step_verlet(self, wall: int, dt: float, step: int, nsteps: int = 1):
+        L.check(self.lib.tsph_step_verlet(self.ctx, wall, dt, step, nsteps), "tsph_step_verlet")
+
+    def step_leapfrog(self, wall: int, dt: float, step: int, nsteps: int = 1):
+        L.check(self.lib.tsph_step_leapfrog(self.ctx, wall, dt, step, nsteps), "tsph_step_leapfrog")
+
+    def step_symplectic_reinit(self, wall: int, dt: float, step: int, nsteps: int = 1,
+                               interval: int = 20):
+        L.check(self.lib.tsph_step_symplectic_reinit(self.ctx, wall, dt, step, nsteps, interval),
+                "tsph_step_symplectic_reinit")
+
     def set_balanced(self, on: bool):
         L.check(self.lib.tsph_set_balanced(self.ctx, int(bool(on))), "tsph_set_balanced")
 
