@@ -97,6 +97,7 @@ struct SPH_special {
   std::vector<realvec> omegav_o;
   std::vector<real> domega, domegarho;
   std::vector<realvec> domegav;
+  std::vector<real> rhoTemp;  // density-reinitialisation buffer, one entry per particle
 };
 
 struct Interaction_data {
@@ -145,6 +146,7 @@ struct Particle_system {
   bool host_dirty = true;   // host vectors newer than the device
   bool dev_newer = false;   // device newer than the host vectors
   size_t zones_on_device = 0;
+  bool history_on_device = false;  // v_oo, rho_oo, rhoTemp held on the device
 
   void Add_particle(short unsigned int _type, real _p, real _rho, realvec _r, realvec _v, realvec _a) {
     np++;
@@ -161,6 +163,7 @@ struct Particle_system {
     data.rho_oo.push_back(_rho);
     data.v_o.push_back(_v);
     data.v_oo.push_back(_v);
+    special.rhoTemp.push_back(_rho);
     host_dirty = true;
   }
 
@@ -273,7 +276,38 @@ inline void tsph_push(Particle_system &ps) {
     f.omegav = ov.data(); f.omegav_o = ovo.data(); f.domegav = dov.data();
     tsph_check(tsph_ale_upload(ps.dev, &f), "tsph_ale_upload");
   }
+  if (ps.history_on_device) {
+    std::vector<double> voo = flat(ps.data.v_oo);
+    tsph_history hh{};
+    hh.np = n;
+    hh.v_oo = voo.data();
+    hh.rho_oo = ps.data.rho_oo.data();
+    hh.rho_temp = ps.special.rhoTemp.data();
+    tsph_check(tsph_history_upload(ps.dev, &hh), "tsph_history_upload");
+  }
   ps.host_dirty = false;
+}
+
+// The Verlet time layer and the reinitialisation buffer move to the device on
+// the first stage that needs them.  The device sets them from its current state
+// (v_oo = v, rho_oo = rho, rhoTemp = rho); the host's values, which are still
+// Add_particle's, replace them while the particle set is the host's (no
+// particle removed or added on the device yet).
+inline void tsph_need_history(Particle_system &ps) {
+  tsph_push(ps);
+  if (ps.history_on_device) return;
+  tsph_check(tsph_history_enable(ps.dev), "tsph_history_enable");
+  ps.history_on_device = true;
+  const int n = (int)ps.np;
+  std::vector<double> voo(2 * (size_t)n);
+  for (int i = 0; i < n; ++i) { voo[2 * i] = ps.data.v_oo[i].x; voo[2 * i + 1] = ps.data.v_oo[i].y; }
+  tsph_history hh{};
+  hh.np = n;
+  hh.v_oo = voo.data();
+  hh.rho_oo = ps.data.rho_oo.data();
+  hh.rho_temp = ps.special.rhoTemp.data();
+  const int rc = tsph_history_upload(ps.dev, &hh);
+  if (rc != TSPH_OK && rc != TSPH_EINVAL) tsph_check(rc, "tsph_history_upload");
 }
 
 inline void tsph_sync_host(Particle_system &ps) {
@@ -305,6 +339,16 @@ inline void tsph_sync_host(Particle_system &ps) {
   ps.data.r = vec(r); ps.data.v = vec(v); ps.data.a = vec(a); ps.data.r_o = vec(ro); ps.data.v_o = vec(vo);
   ps.data.v_oo.resize(m);
   ps.data.rho_oo.resize(m);
+  ps.special.rhoTemp.resize(m);
+  if (ps.history_on_device) {
+    std::vector<double> voo(2 * (size_t)n);
+    tsph_history hh{};
+    hh.v_oo = voo.data();
+    hh.rho_oo = ps.data.rho_oo.data();
+    hh.rho_temp = ps.special.rhoTemp.data();
+    tsph_check(tsph_history_download(ps.dev, &hh), "tsph_history_download");
+    ps.data.v_oo = vec(voo);
+  }
   if (!ps.special.omega.empty()) {
     SPH_special &sp = ps.special;
     std::vector<double> om(n), orho(n), omo(n), orhoo(n), dom(n), dorho(n), ov(2 * (size_t)n),
@@ -464,6 +508,46 @@ inline void Integrate_SymplecticCorrector(Particle_system &ps, double dt) {
   tsph_check(tsph_symplectic_corrector(ps.dev, dt), "Integrate_SymplecticCorrector");
   tsph_ran(ps);
 }
+
+#ifndef TSPH_DROPIN_RALE
+// src/SPH_integration.h:103-128
+inline void Integrate_Verlet(Particle_system &ps, double dt) {
+  tsph_need_history(ps);
+  tsph_check(tsph_verlet(ps.dev, dt), "Integrate_Verlet");
+  tsph_ran(ps);
+}
+// src/SPH_integration.h:130-158
+inline void Integrate_Euler(Particle_system &ps, double dt) {
+  tsph_need_history(ps);
+  tsph_check(tsph_euler(ps.dev, dt), "Integrate_Euler");
+  tsph_ran(ps);
+}
+// src/SPH_integration.h:161-183
+inline void Integrate_LeapFrog_partOne_withDensity(Particle_system &ps, double dt) {
+  tsph_push(ps);
+  tsph_check(tsph_leapfrog_one(ps.dev, dt), "Integrate_LeapFrog_partOne_withDensity");
+  tsph_ran(ps);
+}
+// src/SPH_integration.h:185-230
+inline void Integrate_LeapFrog_partTwo_withDensity(Particle_system &ps, double dt, int step) {
+  tsph_push(ps);
+  tsph_check(tsph_leapfrog_two(ps.dev, dt, step), "Integrate_LeapFrog_partTwo_withDensity");
+  tsph_ran(ps);
+}
+// src/SPH_density_reinitialization.h:8-121
+inline void Density_reinitialization_sumation(Particle_system &ps) {
+  tsph_need_history(ps);
+  tsph_check(tsph_density_reinit_sum(ps.dev), "Density_reinitialization_sumation");
+  tsph_ran(ps);
+}
+// src/SPH_density_reinitialization.h:123-139 (the warnings are counted in
+// tsph_counters.reinit_out_of_range)
+inline void Density_reinitialization_overwrite(Particle_system &ps) {
+  tsph_need_history(ps);
+  tsph_check(tsph_density_reinit_overwrite(ps.dev), "Density_reinitialization_overwrite");
+  tsph_ran(ps);
+}
+#endif
 
 // src/SPH_btdebug.h:4-31
 inline void RemoveParticlesOutOfDomain(Particle_system &ps, Simulation_data) {

@@ -10,10 +10,11 @@ import numpy as np
 import pytest
 
 from oracle import pyoracle as po
-from tests.helpers import oracle_init, oracle_step, rel_err
+from tests.helpers import oracle_init, oracle_reinit, oracle_step, oracle_verlet_step, rel_err
 
 PKG = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "tinysph_old_amd")
-DRIVERS = {"mdbc": os.path.join(PKG, "tsph_dropin_mdbc"), "rale": os.path.join(PKG, "tsph_dropin_rale")}
+DRIVERS = {"mdbc": os.path.join(PKG, "tsph_dropin_mdbc"), "rale": os.path.join(PKG, "tsph_dropin_rale"),
+           "verlet": os.path.join(PKG, "tsph_dropin_verlet")}
 DP = 0.02
 
 
@@ -122,6 +123,26 @@ def test_driver_mdbc_matches_oracle(tmp_path):
     dt = 0.2 * math.sqrt(2 * DP * DP) / 34.3
     for _ in range(20):
         oracle_step(O, 1, dt)
+    assert np.array_equal(d1["type"], O.types())
+    assert rel_err(d1["r"], O.get(po.F_R)) <= 1e-9
+    assert rel_err(d1["v"], O.get(po.F_V)) <= 1e-9
+    assert rel_err(d1["rho"], O.get(po.F_RHO)) <= 1e-9
+    assert rel_err(d1["p"], O.get(po.F_P)) <= 1e-6
+
+
+@pytest.mark.gpu
+def test_driver_verlet_matches_oracle(tmp_path):
+    """22 steps of the Verlet driver with density reinitialisation (Euler on
+    steps 1 and 20, reinitialisation after step 20) against the oracle."""
+    _, init, final = run_driver("verlet", 22, tmp_path)
+    d0, d1 = load(init, False), load(final, False)
+    O = po.oracle_case(params(), with_extra(d0, "mdbc"))
+    oracle_init(O)
+    dt = 0.2 * math.sqrt(2 * DP * DP) / 34.3
+    for step in range(1, 23):
+        oracle_verlet_step(O, 1, dt, step)
+        if step % 20 == 0:
+            oracle_reinit(O)
     assert np.array_equal(d1["type"], O.types())
     assert rel_err(d1["r"], O.get(po.F_R)) <= 1e-9
     assert rel_err(d1["v"], O.get(po.F_V)) <= 1e-9

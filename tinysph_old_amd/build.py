@@ -60,6 +60,7 @@ DRIVER_SOURCE = "dropin_driver.cpp"
 DRIVERS = {
     os.path.join(PKG, "tsph_dropin_mdbc"): [],
     os.path.join(PKG, "tsph_dropin_rale"): ["-DTSPH_DROPIN_RALE"],
+    os.path.join(PKG, "tsph_dropin_verlet"): ["-DTSPH_DROPIN_VERLET"],
 }
 
 

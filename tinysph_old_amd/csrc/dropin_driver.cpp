@@ -6,10 +6,13 @@
 //   tsph_dropin_rale : the Riemann-ALE dam break (-DTSPH_DROPIN_RALE), stage
 //                      order of cases/ALESPH_dambreakLong/SPH_simulationINT.h:84-132
 //                      and geometry of cases/ALESPH_dambreakLong/geometry.h
+//   tsph_dropin_verlet : the mDBC dam break with the Verlet integrator and the
+//                      density reinitialisation (-DTSPH_DROPIN_VERLET), stage order
+//                      of cases/damBreakLong_mDBC/SPH_simulationVerlet_reinit.h
 // Apart from the state dumps, the bodies of INIT/RUN are the reference drivers'
 // call sequences unchanged.
 //
-//   tsph_dropin_{mdbc,rale} DP STEPS INIT.bin FINAL.bin
+//   tsph_dropin_{mdbc,rale,verlet} DP STEPS INIT.bin FINAL.bin
 //
 // Dump format (little endian): int32 np, int32 n_extra, int32 type[np],
 // f64 r[2np], v[2np], rho[np], p[np], extra[2*n_extra] (gnr for mDBC, n for
@@ -114,6 +117,27 @@ struct SPH_simulation {
       Compute_Forces(particles);
       Integrate_LeapFrog_partTwo(particles, dt, step);
       RemoveParticlesOutOfDomain(particles, simulation_data);
+    }
+  }
+#elif defined(TSPH_DROPIN_VERLET)
+  void RUN() {
+    while (step < step_end) {
+      step++;
+      mDBC_compute_density_mdbcGeo(particles, simulation_data);
+      Clear_cells(particles);
+      Particle_To_Cell(particles, simulation_data);
+      Density_to_pressure(particles);
+      Compute_Forces(particles);
+      if (step > 1 && step % 20 != 0) {
+        Integrate_Verlet(particles, dt);
+      } else {
+        Integrate_Euler(particles, dt);
+      }
+      RemoveParticlesOutOfDomain(particles, simulation_data);
+      if (step % 20 == 0) {
+        Density_reinitialization_sumation(particles);
+        Density_reinitialization_overwrite(particles);
+      }
     }
   }
 #else
