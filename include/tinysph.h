@@ -294,6 +294,39 @@ int tsph_step_leapfrog(tsph_ctx *ctx, int wall, double dt, int step, int nsteps)
 int tsph_step_symplectic_reinit(tsph_ctx *ctx, int wall, double dt, int step, int nsteps,
                                 int interval);
 
+/* ---- output, snapshot and resume ----
+ * write_to_ASCII_VTK, src/SPH_data_to_vtk.h:4-51: the particles in reference
+ * order as legacy ASCII VTK polydata (type, pressure, density, velocity), the
+ * same text as the reference's writer.  The device-to-host copies are queued on
+ * the context's stream at the call (they see the stages queued before it); a
+ * host thread formats and writes the file while later stages run.
+ * tsph_io_wait joins the pending writes and reports the first failure;
+ * tsph_destroy joins them too. */
+int tsph_write_vtk(tsph_ctx *ctx, const char *path);
+int tsph_io_wait(tsph_ctx *ctx);
+/* GenerateInterpol, src/SPH_dataInterpolation.h:13-91: Pressure_in_point's
+ * interpolation (src/testing.h:6-148) on the grid x0 + i dh, y0 + j dh with
+ * dh = h/4, ncx = (int)((xm-x0)/h)*4, ncy = (int)((ym-y0)/h)*4, value of point
+ * (i, j) at j + i*ncy.  With rho, p and v all NULL only the sizes are returned. */
+int tsph_interpolate_grid(tsph_ctx *ctx, double x0, double y0, double xm, double ym, int *ncx,
+                          int *ncy, double *rho, double *p, double *v);
+/* GenerateInterpol's file: write_INTERPOLATED_mesh_to_ASCII_VTK,
+ * src/SPH_grid_to_vtk.h:139-243 */
+int tsph_write_interpolated_vtk(tsph_ctx *ctx, double x0, double y0, double xm, double ym,
+                                const char *path);
+/* The two writers' formatting on host arrays (no device needed). */
+int tsph_format_vtk(const tsph_particles *host, const char *path);
+int tsph_format_interpolated_vtk(int ncx, int ncy, double dh, double x0, double y0,
+                                 const double *rho, const double *p, const double *v,
+                                 const char *path);
+/* Binary FP64 snapshot of every per-particle field in reference order (with
+ * the ALE variables and the history fields when present), and its reload into
+ * a context created with the same tsph_params.  After a reload, call
+ * tsph_particle_to_cell; the run then continues as it would have.  No
+ * counterpart in the reference. */
+int tsph_save_state(tsph_ctx *ctx, const char *path);
+int tsph_load_state(tsph_ctx *ctx, const char *path);
+
 /* ---- batched ensembles ----
  * Many independent cases (each with its own constants, domain, time step and
  * particle count) held in one context and advanced together: every stage is

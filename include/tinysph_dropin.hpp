@@ -577,6 +577,40 @@ inline real Pressure_in_point(Particle_system &ps, Simulation_data, realvec ar) 
   return out[1];
 }
 
+// src/SPH_data_to_vtk.h:4-51.  With the state on the device the copies are
+// queued at the call and the file is written in the background
+// (tsph_io_wait(particles.dev) joins); before the first stage it is written
+// from the host vectors.
+inline void write_to_ASCII_VTK(Particle_system &ps, std::string output_file_name) {
+  if (ps.dev) {
+    tsph_push(ps);
+    tsph_check(tsph_write_vtk(ps.dev, output_file_name.c_str()), "write_to_ASCII_VTK");
+    return;
+  }
+  const int n = (int)ps.np;
+  std::vector<int32_t> type(n);
+  std::vector<double> r(2 * (size_t)n), v(2 * (size_t)n);
+  for (int i = 0; i < n; ++i) {
+    type[i] = ps.data.part_type[i];
+    r[2 * i] = ps.data.r[i].x; r[2 * i + 1] = ps.data.r[i].y;
+    v[2 * i] = ps.data.v[i].x; v[2 * i + 1] = ps.data.v[i].y;
+  }
+  tsph_particles h{};
+  h.np = n;
+  h.type = type.data(); h.r = r.data(); h.v = v.data();
+  h.p = ps.data.p.data(); h.rho = ps.data.rho.data();
+  tsph_check(tsph_format_vtk(&h, output_file_name.c_str()), "write_to_ASCII_VTK");
+}
+
+// src/SPH_dataInterpolation.h:13-91 (grid on the device, file as
+// write_INTERPOLATED_mesh_to_ASCII_VTK, src/SPH_grid_to_vtk.h:139-243)
+inline void GenerateInterpol(Particle_system &ps, Simulation_data, std::string filename, real x0,
+                             real y0, real xm, real ym) {
+  tsph_push(ps);
+  tsph_check(tsph_write_interpolated_vtk(ps.dev, x0, y0, xm, ym, filename.c_str()),
+             "GenerateInterpol");
+}
+
 // src/SPH_buffer_zones.h:10-107
 inline void BZ_Integrate_inlet(Particle_system &ps, unsigned int zoneIndex, double dt) {
   tsph_push(ps);

@@ -49,6 +49,8 @@
 #endif
 #if defined(REF_WCSPH)
 #include "SPH_buffer_zones.h"
+#include "SPH_data_to_vtk.h"
+#include "SPH_grid_to_vtk.h"
 #endif
 
 namespace {
@@ -332,6 +334,19 @@ void ref_bz_integrate_inlet(void* vc, int zone, double dt) {
 }
 void ref_bz_integrate_outlet(void* vc, int zone, double dt) {
   BZ_Integrate_outlet(static_cast<RefCase*>(vc)->ps, (unsigned)zone, dt);
+}
+// write_to_ASCII_VTK, src/SPH_data_to_vtk.h:4-51
+void ref_write_vtk(void* vc, const char* path) {
+  write_to_ASCII_VTK(static_cast<RefCase*>(vc)->ps, std::string(path));
+}
+// write_INTERPOLATED_mesh_to_ASCII_VTK, src/SPH_grid_to_vtk.h:139-243
+void ref_write_grid_vtk(int ncx, int ncy, double kh, double x0, double y0, const double* v,
+                        const double* rho, const double* p, const char* path) {
+  const int n = ncx * ncy;
+  std::vector<realvec> vv(n);
+  for (int t = 0; t < n; ++t) vv[t] = {v[2 * t], v[2 * t + 1]};
+  std::vector<real> rr(rho, rho + n), pp(p, p + n);
+  write_INTERPOLATED_mesh_to_ASCII_VTK(ncx, ncy, kh, x0, y0, vv, rr, pp, std::string(path));
 }
 #endif
 void ref_create_block_of_fluid_wp(void* vc, double x0, double y0, double x1, double y1) {

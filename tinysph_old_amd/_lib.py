@@ -139,6 +139,17 @@ EXPORTS = {
     "tsph_autotune": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                 C.POINTER(C.c_int), _dp]),
     "tsph_set_balanced": (C.c_int, [C.c_void_p, C.c_int]),
+    "tsph_write_vtk": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "tsph_io_wait": (C.c_int, [C.c_void_p]),
+    "tsph_interpolate_grid": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_double,
+                                        C.POINTER(C.c_int), C.POINTER(C.c_int), _dp, _dp, _dp]),
+    "tsph_write_interpolated_vtk": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_double,
+                                              C.c_double, C.c_char_p]),
+    "tsph_format_vtk": (C.c_int, [C.POINTER(Particles), C.c_char_p]),
+    "tsph_format_interpolated_vtk": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_double,
+                                               C.c_double, _dp, _dp, _dp, C.c_char_p]),
+    "tsph_save_state": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "tsph_load_state": (C.c_int, [C.c_void_p, C.c_char_p]),
     "tsph_history_enable": (C.c_int, [C.c_void_p]),
     "tsph_history_upload": (C.c_int, [C.c_void_p, C.POINTER(History)]),
     "tsph_history_download": (C.c_int, [C.c_void_p, C.POINTER(History)]),

@@ -23,7 +23,7 @@ PROBE_SOURCE = "tsph_copy_probe.hip"
 OBJDIR = os.path.join(PKG, "_build")
 
 HIP_SOURCES = ["tsph_api.hip", "tsph_cells.hip", "tsph_pairs.hip", "tsph_stages.hip", "tsph_ale.hip",
-               "tsph_alemuscl.hip", "tsph_zones.hip", "tsph_integrators.hip"]
+               "tsph_alemuscl.hip", "tsph_zones.hip", "tsph_integrators.hip", "tsph_io.hip"]
 HOST_SOURCES = ["tsph_cases.cpp"]
 
 ARCH = os.environ.get("TSPH_OFFLOAD_ARCH", "gfx950")

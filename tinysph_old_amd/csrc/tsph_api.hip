@@ -365,6 +365,7 @@ int tsph_create(const tsph_params *prm, tsph_ctx **out) {
 
 int tsph_destroy(tsph_ctx *c) {
   if (!c) return 0;
+  (void)tsph_io_wait(c);  // writer threads hold copies of this context's state
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   drop_graph(c);
   free_soa(c->cur);
@@ -1151,6 +1152,153 @@ int tsph_step_symplectic_reinit(tsph_ctx *c, int wall, double dt, int step, int 
                           : replay_step(c, c->graph, wall, dt, 1, [&] { return record_step(c, wall, dt); });
     if (e) return e;
     if (s % interval == 0 && ((e = launch_reinit_sum(c)) || (e = launch_reinit_overwrite(c)))) return e;
+  }
+  return 0;
+}
+
+// ---- binary snapshot and resume -------------------------------------------------
+// File: "TSPHSNP1", int32 version 1, np, flags (1 gnr, 2 n, 4 ALE variables,
+// 8 history fields), 0; then in reference order int32 type[np], index[np];
+// f64 r, v, a, r_o, v_o [2np each], p, rho, drho, rho_o [np each]; gnr, n
+// [2np each, if flagged]; ALE omega, omegarho, omega_o, omegarho_o, domega,
+// domegarho [np each], omegav, omegav_o, domegav, gradrho, gradvx, gradvy [2np
+// each]; history v_oo [2np], rho_oo, rhoTemp [np each].  Little endian.
+
+static const char kSnapMagic[8] = {'T', 'S', 'P', 'H', 'S', 'N', 'P', '1'};
+
+int tsph_save_state(tsph_ctx *c, const char *path) {
+  if (!c || !path) return fail(TSPH_EINVAL, "null argument");
+  if (int e = single_only(c)) return e;
+  int n = 0, e = 0;
+  TSPH_HIP(hipStreamSynchronize(c->stream));
+  TSPH_HIP(hipMemcpy(&n, &c->scal[S_NP], sizeof(int), hipMemcpyDeviceToHost));
+  const size_t N = (size_t)std::max(n, 1);
+  std::vector<int32_t> type(N), index(N);
+  std::vector<double> s1[4], s2[7];
+  for (auto &x : s1) x.resize(N);
+  for (auto &x : s2) x.resize(2 * N);
+  tsph_particles h{};
+  h.type = type.data(); h.index = index.data();
+  h.p = s1[0].data(); h.rho = s1[1].data(); h.drho = s1[2].data(); h.rho_o = s1[3].data();
+  h.r = s2[0].data(); h.v = s2[1].data(); h.a = s2[2].data(); h.r_o = s2[3].data();
+  h.v_o = s2[4].data();
+  h.gnr = c->has_gnr ? s2[5].data() : nullptr;
+  h.n = c->has_n ? s2[6].data() : nullptr;
+  if ((e = download_impl(c, &h, -1))) return e;
+  const int m = h.np;
+  std::vector<double> a1[6], a2[6], h1[2], h2;
+  if (c->has_ale) {
+    for (auto &x : a1) x.resize(N);
+    for (auto &x : a2) x.resize(2 * N);
+    tsph_ale_fields f{};
+    f.omega = a1[0].data(); f.omegarho = a1[1].data(); f.omega_o = a1[2].data();
+    f.omegarho_o = a1[3].data(); f.domega = a1[4].data(); f.domegarho = a1[5].data();
+    f.omegav = a2[0].data(); f.omegav_o = a2[1].data(); f.domegav = a2[2].data();
+    f.gradrho = a2[3].data(); f.gradvx = a2[4].data(); f.gradvy = a2[5].data();
+    if ((e = tsph_ale_download(c, &f))) return e;
+  }
+  if (c->has_history) {
+    for (auto &x : h1) x.resize(N);
+    h2.resize(2 * N);
+    tsph_history hh{};
+    hh.v_oo = h2.data(); hh.rho_oo = h1[0].data(); hh.rho_temp = h1[1].data();
+    if ((e = tsph_history_download(c, &hh))) return e;
+  }
+  FILE *fp = std::fopen(path, "wb");
+  if (!fp) return fail(TSPH_EINVAL, (std::string("cannot write ") + path).c_str());
+  const int32_t head[4] = {1, m, (c->has_gnr ? 1 : 0) | (c->has_n ? 2 : 0) | (c->has_ale ? 4 : 0) |
+                                     (c->has_history ? 8 : 0), 0};
+  bool ok = std::fwrite(kSnapMagic, 1, 8, fp) == 8 && std::fwrite(head, 4, 4, fp) == 4;
+  auto w = [&](const void *src, size_t bytes) {
+    if (ok && bytes) ok = std::fwrite(src, 1, bytes, fp) == bytes;
+  };
+  const size_t b1 = 8 * (size_t)m, b2 = 16 * (size_t)m;
+  w(type.data(), 4 * (size_t)m);
+  w(index.data(), 4 * (size_t)m);
+  for (int k = 0; k < 5; ++k) w(s2[k].data(), b2);
+  for (int k = 0; k < 4; ++k) w(s1[k].data(), b1);
+  if (c->has_gnr) w(s2[5].data(), b2);
+  if (c->has_n) w(s2[6].data(), b2);
+  if (c->has_ale) {
+    for (int k = 0; k < 6; ++k) w(a1[k].data(), b1);
+    for (int k = 0; k < 6; ++k) w(a2[k].data(), b2);
+  }
+  if (c->has_history) {
+    w(h2.data(), b2);
+    w(h1[0].data(), b1);
+    w(h1[1].data(), b1);
+  }
+  ok = (std::fclose(fp) == 0) && ok;
+  if (!ok) return fail(TSPH_EINVAL, (std::string("write failed: ") + path).c_str());
+  return 0;
+}
+
+int tsph_load_state(tsph_ctx *c, const char *path) {
+  if (!c || !path) return fail(TSPH_EINVAL, "null argument");
+  if (int e = single_only(c)) return e;
+  FILE *fp = std::fopen(path, "rb");
+  if (!fp) return fail(TSPH_EINVAL, (std::string("cannot read ") + path).c_str());
+  char magic[8];
+  int32_t head[4];
+  bool ok = std::fread(magic, 1, 8, fp) == 8 && std::fread(head, 4, 4, fp) == 4 &&
+            std::memcmp(magic, kSnapMagic, 8) == 0 && head[0] == 1 && head[1] >= 0;
+  const int m = ok ? head[1] : 0;
+  const int flags = ok ? head[2] : 0;
+  const size_t N = (size_t)std::max(m, 1);
+  std::vector<int32_t> type(N), index(N);
+  std::vector<double> s1[4], s2[7], a1[6], a2[6], h1[2], h2;
+  auto r = [&](void *dst, size_t bytes) {
+    if (ok && bytes) ok = std::fread(dst, 1, bytes, fp) == bytes;
+  };
+  const size_t b1 = 8 * (size_t)m, b2 = 16 * (size_t)m;
+  for (auto &x : s1) x.resize(N);
+  for (auto &x : s2) x.resize(2 * N);
+  r(type.data(), 4 * (size_t)m);
+  r(index.data(), 4 * (size_t)m);
+  for (int k = 0; k < 5; ++k) r(s2[k].data(), b2);
+  for (int k = 0; k < 4; ++k) r(s1[k].data(), b1);
+  if (flags & 1) r(s2[5].data(), b2);
+  if (flags & 2) r(s2[6].data(), b2);
+  if (flags & 4) {
+    for (auto &x : a1) x.resize(N);
+    for (auto &x : a2) x.resize(2 * N);
+    for (int k = 0; k < 6; ++k) r(a1[k].data(), b1);
+    for (int k = 0; k < 6; ++k) r(a2[k].data(), b2);
+  }
+  if (flags & 8) {
+    for (auto &x : h1) x.resize(N);
+    h2.resize(2 * N);
+    r(h2.data(), b2);
+    r(h1[0].data(), b1);
+    r(h1[1].data(), b1);
+  }
+  std::fclose(fp);
+  if (!ok) return fail(TSPH_EINVAL, (std::string("not a complete snapshot: ") + path).c_str());
+  tsph_particles h{};
+  h.np = m;
+  h.type = type.data(); h.index = index.data();
+  h.p = s1[0].data(); h.rho = s1[1].data(); h.drho = s1[2].data(); h.rho_o = s1[3].data();
+  h.r = s2[0].data(); h.v = s2[1].data(); h.a = s2[2].data(); h.r_o = s2[3].data();
+  h.v_o = s2[4].data();
+  h.gnr = (flags & 1) ? s2[5].data() : nullptr;
+  h.n = (flags & 2) ? s2[6].data() : nullptr;
+  int e = 0;
+  if ((e = tsph_upload(c, &h))) return e;
+  if (flags & 4) {
+    tsph_ale_fields f{};
+    f.np = m;
+    f.omega = a1[0].data(); f.omegarho = a1[1].data(); f.omega_o = a1[2].data();
+    f.omegarho_o = a1[3].data(); f.domega = a1[4].data(); f.domegarho = a1[5].data();
+    f.omegav = a2[0].data(); f.omegav_o = a2[1].data(); f.domegav = a2[2].data();
+    f.gradrho = a2[3].data(); f.gradvx = a2[4].data(); f.gradvy = a2[5].data();
+    if ((e = tsph_ale_upload(c, &f))) return e;
+  }
+  if (flags & 8) {
+    if ((e = tsph_history_enable(c))) return e;
+    tsph_history hh{};
+    hh.np = m;
+    hh.v_oo = h2.data(); hh.rho_oo = h1[0].data(); hh.rho_temp = h1[1].data();
+    if ((e = tsph_history_upload(c, &hh))) return e;
   }
   return 0;
 }

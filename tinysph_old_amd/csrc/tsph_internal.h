@@ -14,7 +14,9 @@
 
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/tinysph.h"
@@ -169,6 +171,10 @@ struct tsph_ctx {
   tsph::Graph euler_graph[2];   // Verlet-driver step on its Euler steps
   tsph::Graph lf_graph[2];      // leap-frog-driver step (after the first)
   bool has_history = false;     // v_oo, rho_oo, rhoTemp allocated
+  // asynchronous output (tsph_write_vtk): writer threads and their first error
+  std::vector<std::thread> writers;
+  std::mutex io_mu;
+  std::string io_error;
   tsph::Zone zones[TSPH_MAX_ZONES];
   int n_zones = 0;
   int *bz_list = nullptr;     // 2 * kBzMax ints: {seq, slot} of the crossing zone particles
@@ -217,6 +223,7 @@ int launch_mdbc(tsph_ctx *c);
 int launch_bt_density(tsph_ctx *c);
 int launch_remove(tsph_ctx *c);
 int run_water_elevation(tsph_ctx *c, double x, double y0, int inverse, double *out);
+int launch_interp_grid(tsph_ctx *c, double x0, double y0, double dh, int ncx, int ncy, double *out);
 // tsph_ale.hip
 int launch_ale_init(tsph_ctx *c);
 int launch_rale_forces(tsph_ctx *c);

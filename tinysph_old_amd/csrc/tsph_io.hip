@@ -1,0 +1,262 @@
+// Output path (host code of the library):
+//   * write_to_ASCII_VTK, src/SPH_data_to_vtk.h:4-51: the particles as legacy
+//     ASCII VTK polydata (type, pressure, density, velocity).  tsph_write_vtk
+//     queues the device-to-host copies on the context's stream into pinned
+//     buffers and leaves the formatting and the file to a host thread, so the
+//     following steps run while the file is written.
+//   * GenerateInterpol, src/SPH_dataInterpolation.h:13-91, with
+//     write_INTERPOLATED_mesh_to_ASCII_VTK, src/SPH_grid_to_vtk.h:139-243: the
+//     interpolation grid is computed on the device (one lane per point).
+// The text is produced with the same stream operations as the reference
+// writers, so the files are byte-identical for the same values.
+#include <algorithm>
+#include <fstream>
+#include <numeric>
+
+#include "tsph_internal.h"
+
+namespace tsph {
+
+namespace {
+
+int io_fail(int code, const std::string &msg) {
+  set_error(msg);
+  return code;
+}
+
+// the particle file of write_to_ASCII_VTK (np particles in reference order)
+bool format_particles(const char *path, int np, const int32_t *type, const double *r,
+                      const double *p, const double *rho, const double *v) {
+  std::ofstream file(path);
+  if (!file) return false;
+  const unsigned n = (unsigned)np;
+  file << "# vtk DataFile Version 3.0" << '\n' << "vtk output" << '\n' << "ASCII" << '\n'
+       << "DATASET POLYDATA" << '\n';
+  file << "POINTS " << n << " float" << '\n';
+  for (int i = 0; i < np; ++i) file << r[2 * i] << " " << r[2 * i + 1] << " 0" << '\n';
+  file << "POINT_DATA " << n << '\n' << "FIELDS FieldData 4" << '\n';
+  file << "type 1 " << n << " int" << '\n';
+  for (int i = 0; i < np; ++i) file << (int)type[i] << '\n';
+  file << "pressure 1 " << n << " float" << '\n';
+  for (int i = 0; i < np; ++i) file << p[i] << '\n';
+  file << "density 1 " << n << " float" << '\n';
+  for (int i = 0; i < np; ++i) file << rho[i] << '\n';
+  file << "velocity 3 " << n << " float" << '\n';
+  for (int i = 0; i < np; ++i) file << v[2 * i] << " " << v[2 * i + 1] << " 0" << '\n';
+  file.close();
+  return !file.fail();
+}
+
+// the grid file of write_INTERPOLATED_mesh_to_ASCII_VTK; values at
+// POS(i, j) = j + i ncy (src/SPH_defs.h:36), points in row order
+bool format_grid(const char *path, int ncx, int ncy, double dh, double x0, double y0,
+                 const double *rho, const double *p, const double *v) {
+  std::ofstream file(path);
+  if (!file) return false;
+  const int cells = ncx * ncy;
+  file << "# vtk DataFile Version 3.0" << '\n';
+  file << "vtk output" << '\n';
+  file << "ASCII" << '\n';
+  file << "DATASET STRUCTURED_GRID" << '\n';
+  file << "DIMENSIONS " << ncx << " " << ncy << " 1" << '\n';
+  file << "POINTS " << ncx * ncy << " float" << '\n';
+  // float counters, as the reference's loop
+  for (float j = 0; j < ncy; j++)
+    for (float i = 0; i < ncx; i++) file << x0 + i * dh << " " << y0 + j * dh << " 0" << '\n';
+  file << "POINT_DATA " << cells << '\n';
+  file << "FIELD FieldData " << 3 << '\n';
+  file << "DENSITY 1 " << cells << " float" << '\n';
+  for (int j = 0; j < ncy; j++)
+    for (int i = 0; i < ncx; i++) file << rho[j + i * ncy] << '\n';
+  file << "PRESSURE 1 " << cells << " float" << '\n';
+  for (int j = 0; j < ncy; j++)
+    for (int i = 0; i < ncx; i++) file << p[j + i * ncy] << '\n';
+  file << "VELOCITY 3 " << cells << " float" << '\n';
+  for (int j = 0; j < ncy; j++)
+    for (int i = 0; i < ncx; i++) {
+      const int t = j + i * ncy;
+      file << v[2 * t] << " " << v[2 * t + 1] << " 0" << '\n';
+    }
+  file.close();
+  return !file.fail();
+}
+
+// one asynchronous particle file: pinned copies of the slot arrays, the event
+// that marks them complete, and the target path
+struct VtkJob {
+  std::string path;
+  int slots = 0;
+  hipEvent_t done = nullptr;
+  int *np = nullptr;
+  uint32_t *seq = nullptr;
+  uint8_t *type = nullptr;
+  double2 *r = nullptr, *v = nullptr;
+  double *p = nullptr, *rho = nullptr;
+};
+
+void free_job(VtkJob &j) {
+  void *ps[] = {j.np, j.seq, j.type, j.r, j.v, j.p, j.rho};
+  for (void *q : ps)
+    if (q) (void)hipHostFree(q);
+  if (j.done) (void)hipEventDestroy(j.done);
+}
+
+std::string write_job(VtkJob &j) {
+  if (hipEventSynchronize(j.done) != hipSuccess) return "tsph_write_vtk: copy failed";
+  const int n = std::min(*j.np, j.slots);
+  // live slots in reference order (removed particles skipped)
+  std::vector<int> order;
+  order.reserve(n);
+  for (int i = 0; i < n; ++i)
+    if (j.type[i] != kRemoved) order.push_back(i);
+  std::sort(order.begin(), order.end(), [&](int a, int b) { return j.seq[a] < j.seq[b]; });
+  const int m = (int)order.size();
+  std::vector<int32_t> type(m);
+  std::vector<double> r(2 * (size_t)m), v(2 * (size_t)m), p(m), rho(m);
+  for (int k = 0; k < m; ++k) {
+    const int i = order[k];
+    type[k] = j.type[i];
+    r[2 * k] = j.r[i].x; r[2 * k + 1] = j.r[i].y;
+    v[2 * k] = j.v[i].x; v[2 * k + 1] = j.v[i].y;
+    p[k] = j.p[i];
+    rho[k] = j.rho[i];
+  }
+  if (!format_particles(j.path.c_str(), m, type.data(), r.data(), p.data(), rho.data(), v.data()))
+    return "tsph_write_vtk: cannot write " + j.path;
+  return std::string();
+}
+
+}  // namespace
+
+}  // namespace tsph
+
+using namespace tsph;
+
+extern "C" {
+
+int tsph_format_vtk(const tsph_particles *h, const char *path) {
+  if (!h || !path || (h->np > 0 && (!h->type || !h->r || !h->p || !h->rho || !h->v)))
+    return io_fail(TSPH_EINVAL, "tsph_format_vtk: null argument");
+  if (!format_particles(path, h->np, h->type, h->r, h->p, h->rho, h->v))
+    return io_fail(TSPH_EINVAL, std::string("tsph_format_vtk: cannot write ") + path);
+  return 0;
+}
+
+int tsph_format_interpolated_vtk(int ncx, int ncy, double dh, double x0, double y0,
+                                 const double *rho, const double *p, const double *v,
+                                 const char *path) {
+  if (!path || ncx < 0 || ncy < 0 || (ncx * ncy > 0 && (!rho || !p || !v)))
+    return io_fail(TSPH_EINVAL, "tsph_format_interpolated_vtk: invalid argument");
+  if (!format_grid(path, ncx, ncy, dh, x0, y0, rho, p, v))
+    return io_fail(TSPH_EINVAL, std::string("tsph_format_interpolated_vtk: cannot write ") + path);
+  return 0;
+}
+
+int tsph_write_vtk(tsph_ctx *c, const char *path) {
+  if (!c || !path) return io_fail(TSPH_EINVAL, "null argument");
+  if (c->cases) return io_fail(TSPH_ESTATE, "not available on a batched context");
+  VtkJob j;
+  j.path = path;
+  j.slots = c->n_ub;
+  const size_t n = (size_t)std::max(j.slots, 1);
+  bool ok = hipEventCreateWithFlags(&j.done, hipEventDisableTiming) == hipSuccess &&
+            hipHostMalloc((void **)&j.np, sizeof(int)) == hipSuccess &&
+            hipHostMalloc((void **)&j.seq, 4 * n) == hipSuccess &&
+            hipHostMalloc((void **)&j.type, n) == hipSuccess &&
+            hipHostMalloc((void **)&j.r, 16 * n) == hipSuccess &&
+            hipHostMalloc((void **)&j.v, 16 * n) == hipSuccess &&
+            hipHostMalloc((void **)&j.p, 8 * n) == hipSuccess &&
+            hipHostMalloc((void **)&j.rho, 8 * n) == hipSuccess;
+  if (ok) {
+    const Soa &d = c->cur;
+    const size_t s = (size_t)j.slots;
+    hipStream_t st = c->stream;
+    ok = hipMemcpyAsync(j.np, &c->scal[S_NP], sizeof(int), hipMemcpyDeviceToHost, st) == hipSuccess &&
+         (s == 0 ||
+          (hipMemcpyAsync(j.seq, d.seq, 4 * s, hipMemcpyDeviceToHost, st) == hipSuccess &&
+           hipMemcpyAsync(j.type, d.type, s, hipMemcpyDeviceToHost, st) == hipSuccess &&
+           hipMemcpyAsync(j.r, d.r, 16 * s, hipMemcpyDeviceToHost, st) == hipSuccess &&
+           hipMemcpyAsync(j.v, d.v, 16 * s, hipMemcpyDeviceToHost, st) == hipSuccess &&
+           hipMemcpyAsync(j.p, d.p, 8 * s, hipMemcpyDeviceToHost, st) == hipSuccess &&
+           hipMemcpyAsync(j.rho, d.rho, 8 * s, hipMemcpyDeviceToHost, st) == hipSuccess)) &&
+         hipEventRecord(j.done, st) == hipSuccess;
+  }
+  if (!ok) {
+    (void)hipStreamSynchronize(c->stream);
+    free_job(j);
+    return io_fail(TSPH_EHIP, "tsph_write_vtk: copy setup failed");
+  }
+  c->writers.emplace_back([c, j]() mutable {
+    const std::string err = write_job(j);
+    free_job(j);
+    if (!err.empty()) {
+      std::lock_guard<std::mutex> lk(c->io_mu);
+      if (c->io_error.empty()) c->io_error = err;
+    }
+  });
+  return 0;
+}
+
+int tsph_io_wait(tsph_ctx *c) {
+  if (!c) return io_fail(TSPH_EINVAL, "null argument");
+  for (std::thread &t : c->writers)
+    if (t.joinable()) t.join();
+  c->writers.clear();
+  std::lock_guard<std::mutex> lk(c->io_mu);
+  if (!c->io_error.empty()) {
+    const std::string e = c->io_error;
+    c->io_error.clear();
+    return io_fail(TSPH_EINVAL, e);
+  }
+  return 0;
+}
+
+int tsph_interpolate_grid(tsph_ctx *c, double x0, double y0, double xm, double ym, int *ncx,
+                          int *ncy, double *rho, double *p, double *v) {
+  if (!c || !ncx || !ncy) return io_fail(TSPH_EINVAL, "null argument");
+  if (c->cases) return io_fail(TSPH_ESTATE, "not available on a batched context");
+  // src/SPH_dataInterpolation.h:31-36: sf = 4, dh = h / sf
+  const double h = c->k.h, sf = 4;
+  const double dh = h / sf;
+  const int nx = (int)((xm - x0) / h) * sf;
+  const int ny = (int)((ym - y0) / h) * sf;
+  if (nx < 0 || ny < 0 || (long long)nx * ny > (1ll << 28))
+    return io_fail(TSPH_EINVAL, "interpolation grid too large");
+  *ncx = nx;
+  *ncy = ny;
+  if (!rho && !p && !v) return 0;
+  if (!c->cells_valid) return io_fail(TSPH_ESTATE, "no cell list: call tsph_particle_to_cell first");
+  const size_t cnt = (size_t)nx * ny;
+  if (cnt == 0) return 0;
+  double *dout = nullptr;
+  TSPH_HIP(hipMalloc((void **)&dout, 4 * sizeof(double) * cnt));
+  std::vector<double> out(4 * cnt);
+  int e = launch_interp_grid(c, x0, y0, dh, nx, ny, dout);
+  if (!e && (hipMemcpyAsync(out.data(), dout, 4 * sizeof(double) * cnt, hipMemcpyDeviceToHost,
+                            c->stream) != hipSuccess ||
+             hipStreamSynchronize(c->stream) != hipSuccess))
+    e = io_fail(TSPH_EHIP, "tsph_interpolate_grid: copy failed");
+  (void)hipFree(dout);
+  if (e) return e;
+  for (size_t t = 0; t < cnt; ++t) {
+    if (rho) rho[t] = out[4 * t];
+    if (p) p[t] = out[4 * t + 1];
+    if (v) { v[2 * t] = out[4 * t + 2]; v[2 * t + 1] = out[4 * t + 3]; }
+  }
+  return 0;
+}
+
+int tsph_write_interpolated_vtk(tsph_ctx *c, double x0, double y0, double xm, double ym,
+                                const char *path) {
+  if (!path) return io_fail(TSPH_EINVAL, "null argument");
+  int nx = 0, ny = 0;
+  if (int e = tsph_interpolate_grid(c, x0, y0, xm, ym, &nx, &ny, nullptr, nullptr, nullptr)) return e;
+  const size_t cnt = (size_t)nx * ny;
+  std::vector<double> rho(cnt + 1), p(cnt + 1), v(2 * cnt + 2);
+  if (int e = tsph_interpolate_grid(c, x0, y0, xm, ym, &nx, &ny, rho.data(), p.data(), v.data()))
+    return e;
+  return tsph_format_interpolated_vtk(nx, ny, c->k.h / 4, x0, y0, rho.data(), p.data(), v.data(),
+                                      path);
+}
+
+}  // extern "C"

@@ -321,9 +321,10 @@ __global__ __launch_bounds__(256) void k_water_level(Soa d, const int *__restric
   }
 }
 
-__global__ void k_interp_point(Soa d, const int *__restrict__ cell_start, Consts k, double x,
-                               double y, double *__restrict__ out) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// Kernel_velocity_approximation_TEST, src/testing.h:6-148, at one point:
+// out4 = {rho, p, vx, vy}
+__device__ void interpolate_at(const Soa &d, const int *__restrict__ cell_start, const Consts &k,
+                               double x, double y, double *out4) {
   const double kh = k.kh, of = k.of;
   const unsigned npx = (unsigned)(int)((x - k.x0 + of) / kh);
   const unsigned npy = (unsigned)(int)((y - k.y0 + of) / kh);
@@ -356,7 +357,29 @@ __global__ void k_interp_point(Soa d, const int *__restrict__ cell_start, Consts
     p = 0 + pw / wsum;
   }
   if (wsum == 0) { vx = 0.; vy = 0.; }
-  out[0] = rho; out[1] = p; out[2] = vx; out[3] = vy;
+  out4[0] = rho; out4[1] = p; out4[2] = vx; out4[3] = vy;
+}
+
+__global__ void k_interp_point(Soa d, const int *__restrict__ cell_start, Consts k, double x,
+                               double y, double *__restrict__ out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  interpolate_at(d, cell_start, k, x, y, out);
+}
+
+// GenerateInterpol's grid, src/SPH_dataInterpolation.h:13-91: point (gx, gy) at
+// (x0 + gx dh, y0 + gy dh), stored at POS(gx, gy) = gy + gx ncy
+// (src/SPH_defs.h:36), one lane per point
+__global__ void k_interp_grid(Soa d, const int *__restrict__ cell_start, Consts k, double x0,
+                              double y0, double dh, int ncx, int ncy, double *__restrict__ out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ncx * ncy) return;
+  const int gx = t / ncy, gy = t - gx * ncy;
+  double v[4];
+  interpolate_at(d, cell_start, k, x0 + gx * dh, y0 + gy * dh, v);
+  out[4 * t] = v[0];
+  out[4 * t + 1] = v[1];
+  out[4 * t + 2] = v[2];
+  out[4 * t + 3] = v[3];
 }
 
 }  // namespace
@@ -424,6 +447,14 @@ int run_water_elevation(tsph_ctx *c, double x, double y0, int inverse, double *o
   TSPH_HIP(hipGetLastError());
   TSPH_HIP(hipMemcpyAsync(out, dout, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   TSPH_HIP(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+// out: 4 doubles per grid point, device memory
+int launch_interp_grid(tsph_ctx *c, double x0, double y0, double dh, int ncx, int ncy, double *out) {
+  hipLaunchKernelGGL(k_interp_grid, dim3(grid_for(ncx * ncy, 64)), dim3(64), 0, c->stream, c->cur,
+                     c->cell_start, c->k, x0, y0, dh, ncx, ncy, out);
+  TSPH_HIP(hipGetLastError());
   return 0;
 }
 

@@ -356,6 +356,36 @@ class ParticleSystem:
     def step_ale(self, dt: float, step: int, nsteps: int = 1, muscl: int = 0):
         L.check(self.lib.tsph_step_ale(self.ctx, dt, step, nsteps, muscl), "tsph_step_ale")
 
+    # ---- output, snapshot, resume (src/SPH_data_to_vtk.h, src/SPH_dataInterpolation.h) ----
+    def write_vtk(self, path: str):
+        L.check(self.lib.tsph_write_vtk(self.ctx, str(path).encode()), "write_to_ASCII_VTK")
+
+    def io_wait(self):
+        L.check(self.lib.tsph_io_wait(self.ctx), "tsph_io_wait")
+
+    def interpolate_grid(self, x0: float, y0: float, xm: float, ym: float):
+        """GenerateInterpol's grid: (rho, p, v) as (ncx, ncy) / (ncx, ncy, 2) arrays."""
+        nx, ny = C.c_int(), C.c_int()
+        L.check(self.lib.tsph_interpolate_grid(self.ctx, x0, y0, xm, ym, C.byref(nx), C.byref(ny),
+                                               None, None, None), "tsph_interpolate_grid")
+        n = nx.value * ny.value
+        rho, p, v = np.zeros(max(n, 1)), np.zeros(max(n, 1)), np.zeros(max(2 * n, 1))
+        L.check(self.lib.tsph_interpolate_grid(self.ctx, x0, y0, xm, ym, C.byref(nx), C.byref(ny),
+                                               L.dptr(rho), L.dptr(p), L.dptr(v)),
+                "tsph_interpolate_grid")
+        shp = (nx.value, ny.value)
+        return rho[:n].reshape(shp), p[:n].reshape(shp), v[:2 * n].reshape(shp + (2,))
+
+    def write_interpolated_vtk(self, x0: float, y0: float, xm: float, ym: float, path: str):
+        L.check(self.lib.tsph_write_interpolated_vtk(self.ctx, x0, y0, xm, ym, str(path).encode()),
+                "GenerateInterpol")
+
+    def save_state(self, path: str):
+        L.check(self.lib.tsph_save_state(self.ctx, str(path).encode()), "tsph_save_state")
+
+    def load_state(self, path: str):
+        L.check(self.lib.tsph_load_state(self.ctx, str(path).encode()), "tsph_load_state")
+
     # ---- other WCSPH integrators and density reinitialisation
     # (src/SPH_integration.h:103-230, src/SPH_density_reinitialization.h) ----
     def history_enable(self):
