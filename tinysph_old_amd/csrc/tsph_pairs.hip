@@ -558,46 +558,43 @@ __global__ __launch_bounds__(kWave) void k_tile32(Soa d, Mixed mx, const int *__
     Acc32 s;
     while (__any(cur < tot)) {
       int cnt = 0;
-      // Candidates in groups of up to four from one column slice (so one slot
-      // offset serves the group), while the list has room for four: their LDS
-      // loads are issued together, and the group is appended to the list
+      // Candidates in groups of up to kGroup from one column slice (so one slot
+      // offset serves the group), while the list has room for a group: their
+      // LDS loads are issued together, and the group is appended to the list
       // without branches (every slot is written, the count advances by the
-      // test).  The particle itself is the candidate at distance exactly 0;
-      // "0 < r^2" drops it (and only a coincident particle with it, whose
-      // reference kernel gradient would be 0/0).
-      while (cur < tot && cnt <= kList - 4) {
+      // test).  Slots past the group's end are read but masked out; they lie
+      // within the LDS allocation (the staged positions are followed by the
+      // other staged arrays).  The particle itself is the candidate at
+      // distance exactly 0; "0 < r^2" drops it (and only a coincident particle
+      // with it, whose reference kernel gradient would be 0/0).
+      while (cur < tot && cnt <= kList - kGroup) {
         const bool in0 = cur < n0, in01 = cur < n01;
         const int end = in0 ? n0 : (in01 ? n01 : tot);
         const int j0 = cur + (in0 ? a0 : (in01 ? off1 : off2));
-        const int m = min(4, end - cur);
-        // slots past the group's end (m < 4) are read but masked out: they lie
-        // within the LDS allocation (the staged records are followed by the
-        // other staged arrays)
-        const int j1 = j0 + 1, j2 = j0 + 2, j3 = j0 + 3;
-        const float2 p0 = s_pos[j0], p1 = s_pos[j1], p2 = s_pos[j2], p3 = s_pos[j3];
-        const float x0 = pi.x - p0.x, y0 = pi.y - p0.y, x1 = pi.x - p1.x, y1 = pi.y - p1.y;
-        const float x2 = pi.x - p2.x, y2 = pi.y - p2.y, x3 = pi.x - p3.x, y3 = pi.y - p3.y;
-        const float q0 = x0 * x0 + y0 * y0, q1 = x1 * x1 + y1 * y1;
-        const float q2 = x2 * x2 + y2 * y2, q3 = x3 * x3 + y3 * y3;
-        bool k0 = (q0 > 0.f) & (q0 < cc.r2cut);
-        bool k1 = (m > 1) & (q1 > 0.f) & (q1 < cc.r2cut);
-        bool k2 = (m > 2) & (q2 > 0.f) & (q2 < cc.r2cut);
-        bool k3 = (m > 3) & (q3 > 0.f) & (q3 < cc.r2cut);
-        if (VAR == TSPH_FORCES_DBC) {
-          const bool w0 = dbc_wall && s_t[j0] == TSPH_WALL, w1 = dbc_wall && s_t[j1] == TSPH_WALL;
-          const bool w2 = dbc_wall && s_t[j2] == TSPH_WALL, w3 = dbc_wall && s_t[j3] == TSPH_WALL;
-          ncand += (j0 != li && !w0) + (m > 1 && j1 != li && !w1) + (m > 2 && j2 != li && !w2) +
-                   (m > 3 && j3 != li && !w3);
-          k0 = k0 && !w0; k1 = k1 && !w1; k2 = k2 && !w2; k3 = k3 && !w3;
+        const int m = min(kGroup, end - cur);
+        float2 pp[kGroup];
+#pragma unroll
+        for (int u = 0; u < kGroup; ++u) pp[u] = s_pos[j0 + u];
+        bool kk[kGroup];
+#pragma unroll
+        for (int u = 0; u < kGroup; ++u) {
+          const float x = pi.x - pp[u].x, y = pi.y - pp[u].y;
+          const float q = x * x + y * y;
+          kk[u] = (u < m) & (q > 0.f) & (q < cc.r2cut);
         }
-        my_list[cnt * kWave] = (uint16_t)j0;
-        cnt += k0;
-        my_list[cnt * kWave] = (uint16_t)j1;
-        cnt += k1;
-        my_list[cnt * kWave] = (uint16_t)j2;
-        cnt += k2;
-        my_list[cnt * kWave] = (uint16_t)j3;
-        cnt += k3;
+        if (VAR == TSPH_FORCES_DBC) {
+#pragma unroll
+          for (int u = 0; u < kGroup; ++u) {
+            const bool w = dbc_wall && s_t[j0 + u] == TSPH_WALL;
+            ncand += (u < m) && (j0 + u != li) && !w;
+            kk[u] = kk[u] && !w;
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < kGroup; ++u) {
+          my_list[cnt * kWave] = (uint16_t)(j0 + u);
+          cnt += kk[u];
+        }
         cur += m;
       }
       npair += cnt;
