@@ -222,6 +222,34 @@ def test_tile_size_does_not_change_results(precision):
             assert rel_l2(o["drho"], outs[0]["drho"]) <= 1e-5
 
 
+@pytest.mark.parametrize("precision", [T.PREC_FP64, T.PREC_MIXED])
+def test_tile_walks_agree(precision):
+    """The static and the pooled tile walk (and the waves per CU) only change
+    which wavefront computes a tile, never the tile frame or the summation
+    order: identical bits and counts.  dx = 4 mm and 1-2 waves per CU give the
+    grid enough tiles per wave for the pooled walk to be taken."""
+    case = T.build_case(T.CASE_DAM_BREAK_MDBC, dx=0.004, jitter=0.01, seed=11)
+    print(f"[case] dam_break_mdbc dx=0.004 precision={precision}", flush=True)
+    outs = []
+    with T.system_for_case(case, precision=precision) as ps:
+        ps.particle_to_cell()
+        ps.mdbc_density()
+        ps.density_to_pressure()
+        for bal, waves in ((0, 8), (1, 1), (1, 2), (0, 2)):
+            ps.set_balanced(bal)
+            ps.set_waves_per_cu(waves)
+            ps.reset_counters()
+            ps.compute_forces(T.FORCES_WCSPH)
+            out = ps.download(("a", "drho"))
+            out["counters"] = ps.counters()
+            outs.append(out)
+    for o in outs[1:]:
+        assert np.array_equal(o["a"], outs[0]["a"])
+        assert np.array_equal(o["drho"], outs[0]["drho"])
+        for key in ("pairs", "candidates", "active", "pair_passes"):
+            assert o["counters"][key] == outs[0]["counters"][key], key
+
+
 def test_water_level_and_probe():
     case = make("dam_break_mdbc")
     O = oracle_for(case)

@@ -96,15 +96,17 @@ __device__ __forceinline__ void rale_finish(const Soa &d, const Consts &k, int i
 
 // FP64 tile kernel, reference summation order (9 cells, ascending index)
 template <bool DYN>
-__global__ __launch_bounds__(kWave) void k_rale64(Soa d, const int *__restrict__ cell_start,
+__global__ __launch_bounds__(kWave * kPoolWaves) void k_rale64(Soa d, const int *__restrict__ cell_start,
                                                   const int *__restrict__ cellof, Consts k, int rows,
                                                   int *__restrict__ scal,
                                                   const int *__restrict__ tile_list, int cap,
+                                                  const int *__restrict__ bound, int lds_wave,
                                                   uint8_t *__restrict__ gflag,
                                                   unsigned long long *__restrict__ counters) {
   extern __shared__ __align__(16) unsigned char smem[];
-  int *tab = reinterpret_cast<int *>(smem);
-  double2 *s_r = reinterpret_cast<double2 *>(smem + kTabBytes);
+  unsigned char *wl = wave_lds(smem, lds_wave);
+  int *tab = reinterpret_cast<int *>(wl);
+  double2 *s_r = reinterpret_cast<double2 *>(wl + kTabBytes);
   double2 *s_v = s_r + cap;
   double2 *s_rw = s_v + cap;  // {rho, omega}
   double2 *s_n = s_rw + cap;
@@ -113,15 +115,15 @@ __global__ __launch_bounds__(kWave) void k_rale64(Soa d, const int *__restrict__
 
   if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&counters[C_PASSES], 1ull);
   unsigned npair = 0, ncand = 0, nact = 0;
-  TileQueue tq = tq_begin(scal + S_TQ, scal[S_NTILES], kBalancedStatic8);
+  TileQueue tq = tq_begin<DYN>(scal, bound, smem);
   for (int it = tq_take<DYN>(tq, tq_ticket<DYN>(tq)); it >= 0;) {
-    __syncthreads();
+    wave_sync();
     const int ticket = tq_ticket<DYN>(tq);  // read once this tile's loads are waited for
     Tile t;
     tile_begin(k, cell_start, tile_list[it], rows, tab, t);
     const bool staged = t.lq[3] <= cap;
     if (staged) {
-      for (int l = threadIdx.x; l < t.lq[3]; l += blockDim.x) {
+      for (int l = tile_lane(); l < t.lq[3]; l += kWave) {
         const int j = l - sel3(t.base, slice_of(t, l));
         s_r[l] = d.r[j];
         s_v[l] = d.v[j];
@@ -130,9 +132,9 @@ __global__ __launch_bounds__(kWave) void k_rale64(Soa d, const int *__restrict__
         s_n[l] = d.n ? d.n[j] : make_double2(0., 0.);
       }
     }
-    __syncthreads();
+    wave_sync();
     const int next = tq_take<DYN>(tq, ticket);
-    const int lane = threadIdx.x;
+    const int lane = tile_lane();
     uint16_t *my_list = s_list + lane;
     for (int i0 = t.own0; i0 < t.own1; i0 += kWave) {
       const int i = i0 + lane;
@@ -206,7 +208,6 @@ __global__ __launch_bounds__(kWave) void k_rale64(Soa d, const int *__restrict__
     }
     it = next;
   }
-  tq_end<DYN>(tq);
   wave_count(npair, ncand, nact, counters);
 }
 
@@ -269,16 +270,18 @@ __device__ __forceinline__ void rale_pair32(const CA32 &c, AccA32 &s, float drx,
 }
 
 template <bool DYN>
-__global__ __launch_bounds__(kWave) void k_rale32(Soa d, const int *__restrict__ cell_start,
+__global__ __launch_bounds__(kWave * kPoolWaves) void k_rale32(Soa d, const int *__restrict__ cell_start,
                                                   const int *__restrict__ cellof, Consts k, int rows,
                                                   int *__restrict__ scal,
                                                   const int *__restrict__ tile_list, int cap,
+                                                  const int *__restrict__ bound, int lds_wave,
                                                   uint8_t *__restrict__ gflag,
                                                   unsigned long long *__restrict__ counters) {
 #pragma clang fp contract(fast)
   extern __shared__ __align__(16) unsigned char smem[];
-  int *tab = reinterpret_cast<int *>(smem);
-  float2 *s_pos = reinterpret_cast<float2 *>(smem + kTabBytes);
+  unsigned char *wl = wave_lds(smem, lds_wave);
+  int *tab = reinterpret_cast<int *>(wl);
+  float2 *s_pos = reinterpret_cast<float2 *>(wl + kTabBytes);
   float2 *s_vel = s_pos + cap;
   float2 *s_dw = s_vel + cap;  // {rho - rho0, omega}
   float2 *s_n = s_dw + cap;
@@ -292,9 +295,9 @@ __global__ __launch_bounds__(kWave) void k_rale32(Soa d, const int *__restrict__
 
   if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&counters[C_PASSES], 1ull);
   unsigned npair = 0, ncand = 0, nact = 0;
-  TileQueue tq = tq_begin(scal + S_TQ, scal[S_NTILES], kBalancedStatic8);
+  TileQueue tq = tq_begin<DYN>(scal, bound, smem);
   for (int it = tq_take<DYN>(tq, tq_ticket<DYN>(tq)); it >= 0;) {
-    __syncthreads();
+    wave_sync();
     const int ticket = tq_ticket<DYN>(tq);  // read once this tile's loads are waited for
     Tile t;
     tile_begin(k, cell_start, tile_list[it], rows, tab, t);
@@ -302,7 +305,7 @@ __global__ __launch_bounds__(kWave) void k_rale32(Soa d, const int *__restrict__
     const int mid = t.row_lo + (t.nrows >> 1);
     const double ox = k.x0 - k.of + t.cx * k.kh, oy = k.y0 - k.of + mid * k.kh;
     if (staged) {
-      for (int l = threadIdx.x; l < t.lq[3]; l += blockDim.x) {
+      for (int l = tile_lane(); l < t.lq[3]; l += kWave) {
         const int j = l - sel3(t.base, slice_of(t, l));
         const double2 r = d.r[j], v = d.v[j];
         s_pos[l] = make_float2((float)(r.x - ox), (float)(r.y - oy));
@@ -312,9 +315,9 @@ __global__ __launch_bounds__(kWave) void k_rale32(Soa d, const int *__restrict__
         s_n[l] = d.n ? make_float2((float)d.n[j].x, (float)d.n[j].y) : make_float2(0.f, 0.f);
       }
     }
-    __syncthreads();
+    wave_sync();
     const int next = tq_take<DYN>(tq, ticket);
-    const int lane = threadIdx.x;
+    const int lane = tile_lane();
     uint16_t *my_list = s_list + lane;
     for (int i0 = t.own0; i0 < t.own1; i0 += kWave) {
       const int i = i0 + lane;
@@ -394,7 +397,6 @@ __global__ __launch_bounds__(kWave) void k_rale32(Soa d, const int *__restrict__
     }
     it = next;
   }
-  tq_end<DYN>(tq);
   wave_count(npair, ncand, nact, counters);
 }
 
@@ -510,29 +512,33 @@ int launch_rale_forces(tsph_ctx *c) {
   const int rows = c->tile_rows;
   const int cap = 3 * (rows + 2) * kCellCap;
   hipStream_t s = c->stream;
-  if (int e = ensure_tile_list(c)) return e;
-  const dim3 g(8 * ((c->num_cu * c->waves_per_cu + 7) / 8)), b(kWave);
   const size_t list = sizeof(uint16_t) * kList * kWave;
   int e = 0;
   if (c->prm.precision == TSPH_PREC_MIXED) {
-    const size_t lds = kTabBytes + list + (size_t)cap * (4 * sizeof(float2) + 1);
+    const TileGrid tg = tile_grid(c, true, kTabBytes + list + (size_t)cap * (4 * sizeof(float2) + 1));
+    if ((e = ensure_tile_list(c, tg.pooled ? tg.groups : 0))) return e;
     e = launch_timed(c, [&] {
-      if (walk_balanced(c, true, (int)g.x))
-        hipLaunchKernelGGL(k_rale32<true>, g, b, lds, s, c->cur, c->cell_start, c->cellof, k, rows,
-                           c->scal, c->tile_list, cap, c->ale_flag, c->counters);
+      if (tg.pooled)
+        hipLaunchKernelGGL(k_rale32<true>, tg.grid, tg.block, tg.lds, s, c->cur, c->cell_start,
+                           c->cellof, k, rows, c->scal, c->tile_list, cap, c->tile_bound,
+                           tg.lds_wave, c->ale_flag, c->counters);
       else
-        hipLaunchKernelGGL(k_rale32<false>, g, b, lds, s, c->cur, c->cell_start, c->cellof, k, rows,
-                           c->scal, c->tile_list, cap, c->ale_flag, c->counters);
+        hipLaunchKernelGGL(k_rale32<false>, tg.grid, tg.block, tg.lds, s, c->cur, c->cell_start,
+                           c->cellof, k, rows, c->scal, c->tile_list, cap, c->tile_bound,
+                           tg.lds_wave, c->ale_flag, c->counters);
     });
   } else {
-    const size_t lds = kTabBytes + list + (size_t)cap * (4 * sizeof(double2) + 1);
+    const TileGrid tg = tile_grid(c, false, kTabBytes + list + (size_t)cap * (4 * sizeof(double2) + 1));
+    if ((e = ensure_tile_list(c, tg.pooled ? tg.groups : 0))) return e;
     e = launch_timed(c, [&] {
-      if (walk_balanced(c, false, (int)g.x))
-        hipLaunchKernelGGL(k_rale64<true>, g, b, lds, s, c->cur, c->cell_start, c->cellof, k, rows,
-                           c->scal, c->tile_list, cap, c->ale_flag, c->counters);
+      if (tg.pooled)
+        hipLaunchKernelGGL(k_rale64<true>, tg.grid, tg.block, tg.lds, s, c->cur, c->cell_start,
+                           c->cellof, k, rows, c->scal, c->tile_list, cap, c->tile_bound,
+                           tg.lds_wave, c->ale_flag, c->counters);
       else
-        hipLaunchKernelGGL(k_rale64<false>, g, b, lds, s, c->cur, c->cell_start, c->cellof, k, rows,
-                           c->scal, c->tile_list, cap, c->ale_flag, c->counters);
+        hipLaunchKernelGGL(k_rale64<false>, tg.grid, tg.block, tg.lds, s, c->cur, c->cell_start,
+                           c->cellof, k, rows, c->scal, c->tile_list, cap, c->tile_bound,
+                           tg.lds_wave, c->ale_flag, c->counters);
     });
   }
   if (e) return e;

@@ -680,21 +680,25 @@ struct Acc32P {
 // kernels
 // ---------------------------------------------------------------------------
 template <class P, bool DYN>
-__global__ __launch_bounds__(kWave) void k_ale_pass64(P pol, const int *__restrict__ cell_start,
+__global__ __launch_bounds__(kWave * kPoolWaves) void k_ale_pass64(P pol, const int *__restrict__ cell_start,
                                                       const int *__restrict__ cellof, int rows,
                                                       int *__restrict__ scal,
-                                                      const int *__restrict__ tile_list, int cap,
+                                                      const int *__restrict__ tile_list,
+                                                      const int *__restrict__ bound, int cap,
+                                                      int lds_wave,
                                                       unsigned long long *__restrict__ counters) {
-  walk64<DYN>(pol, pol.k, cell_start, cellof, rows, scal, tile_list, cap, counters);
+  walk64<DYN>(pol, pol.k, cell_start, cellof, rows, scal, tile_list, bound, cap, lds_wave, counters);
 }
 
 template <class P, bool DYN>
-__global__ __launch_bounds__(kWave) void k_ale_pass32(P pol, const int *__restrict__ cell_start,
+__global__ __launch_bounds__(kWave * kPoolWaves) void k_ale_pass32(P pol, const int *__restrict__ cell_start,
                                                       const int *__restrict__ cellof, int rows,
                                                       int *__restrict__ scal,
-                                                      const int *__restrict__ tile_list, int cap,
+                                                      const int *__restrict__ tile_list,
+                                                      const int *__restrict__ bound, int cap,
+                                                      int lds_wave,
                                                       unsigned long long *__restrict__ counters) {
-  walk32<DYN>(pol, pol.k, cell_start, cellof, rows, scal, tile_list, cap, counters);
+  walk32<DYN>(pol, pol.k, cell_start, cellof, rows, scal, tile_list, bound, cap, lds_wave, counters);
 }
 
 // Integrate_density_compute_pressure, src/ALESPH_integration.h:86-135 (fluid
@@ -752,29 +756,31 @@ __global__ void k_ale_lf2(Soa d, const int *__restrict__ scal, double dt, int fi
 template <class P, int SLOT, bool FP32>
 int launch_pass(tsph_ctx *c, const P &pol) {
   const int rows = c->tile_rows;
-  // slots budgeted per tile, within 64 KiB of LDS per workgroup; a tile whose
+  // slots budgeted per tile, within 64 KiB of LDS per wavefront; a tile whose
   // neighbourhood does not fit is gathered from global memory instead
   constexpr int kFixed = kTabBytes + sizeof(uint16_t) * kList * kWave;
-  const int cap = std::min(3 * (rows + 2) * kCellCap, (65536 - kFixed) / SLOT);
-  if (int e = ensure_tile_list(c)) return e;
-  const dim3 g(8 * ((c->num_cu * c->waves_per_cu + 7) / 8)), b(kWave);
-  const size_t lds = kFixed + (size_t)cap * SLOT;
-  const bool bal = walk_balanced(c, FP32, (int)g.x);
+  const int cap = std::min(3 * (rows + 2) * kCellCap, (65536 - kPoolHdr - kFixed) / SLOT);
+  const TileGrid tg = tile_grid(c, FP32, kFixed + (size_t)cap * SLOT);
+  if (int e = ensure_tile_list(c, tg.pooled ? tg.groups : 0)) return e;
   return launch_timed(c, [&] {
     if constexpr (FP32) {
-      if (bal)
-        hipLaunchKernelGGL((k_ale_pass32<P, true>), g, b, lds, c->stream, pol, c->cell_start,
-                           c->cellof, rows, c->scal, c->tile_list, cap, c->counters);
+      if (tg.pooled)
+        hipLaunchKernelGGL((k_ale_pass32<P, true>), tg.grid, tg.block, tg.lds, c->stream, pol,
+                           c->cell_start, c->cellof, rows, c->scal, c->tile_list, c->tile_bound,
+                           cap, tg.lds_wave, c->counters);
       else
-        hipLaunchKernelGGL((k_ale_pass32<P, false>), g, b, lds, c->stream, pol, c->cell_start,
-                           c->cellof, rows, c->scal, c->tile_list, cap, c->counters);
+        hipLaunchKernelGGL((k_ale_pass32<P, false>), tg.grid, tg.block, tg.lds, c->stream, pol,
+                           c->cell_start, c->cellof, rows, c->scal, c->tile_list, c->tile_bound,
+                           cap, tg.lds_wave, c->counters);
     } else {
-      if (bal)
-        hipLaunchKernelGGL((k_ale_pass64<P, true>), g, b, lds, c->stream, pol, c->cell_start,
-                           c->cellof, rows, c->scal, c->tile_list, cap, c->counters);
+      if (tg.pooled)
+        hipLaunchKernelGGL((k_ale_pass64<P, true>), tg.grid, tg.block, tg.lds, c->stream, pol,
+                           c->cell_start, c->cellof, rows, c->scal, c->tile_list, c->tile_bound,
+                           cap, tg.lds_wave, c->counters);
       else
-        hipLaunchKernelGGL((k_ale_pass64<P, false>), g, b, lds, c->stream, pol, c->cell_start,
-                           c->cellof, rows, c->scal, c->tile_list, cap, c->counters);
+        hipLaunchKernelGGL((k_ale_pass64<P, false>), tg.grid, tg.block, tg.lds, c->stream, pol,
+                           c->cell_start, c->cellof, rows, c->scal, c->tile_list, c->tile_bound,
+                           cap, tg.lds_wave, c->counters);
     }
   });
 }

@@ -69,13 +69,12 @@ static void drop_graph(tsph_ctx *c) {
 }
 
 static HostState host_state(const tsph_ctx *c) {
-  return {c->cells_valid, c->tiles_valid, c->tiles_zeroed, c->records_valid, c->marks_valid};
+  return {c->cells_valid, c->tiles_valid, c->records_valid, c->marks_valid};
 }
 
 static void set_host_state(tsph_ctx *c, const HostState &h) {
   c->cells_valid = h.cells_valid;
   c->tiles_valid = h.tiles_valid;
-  c->tiles_zeroed = h.tiles_zeroed;
   c->records_valid = h.records_valid;
   c->marks_valid = h.marks_valid;
 }
@@ -290,7 +289,7 @@ static int alloc_context(tsph_ctx *c) {
   c->seq_cap = 2 * cap + kBzMax + c->ncases;
   if ((e = alloc_soa(c->cur, cap, false, false)) || (e = alloc_soa(c->alt, cap, false, false)))
     return e;
-  if ((e = dalloc(&c->scal, S_NSCAL)) || (e = dalloc(&c->counters, C_NCOUNT)) ||
+  if ((e = dalloc(&c->scal, S_NSCAL)) ||
       (e = dalloc(&c->cell_count, nct + 1)) || (e = dalloc(&c->cell_start, nct + 2)) ||
       (e = dalloc(&c->cellof, cap)) || (e = dalloc(&c->hrank, cap)) || (e = dalloc(&c->hcell, cap)) ||
       (e = dalloc(&c->perm, cap)) || (e = dalloc(&c->scan_tmp, (nct + 1) / 1024 + 2)) ||
@@ -306,15 +305,25 @@ static int alloc_context(tsph_ctx *c) {
   {
     int dev = 0;
     hipDeviceProp_t prop;
-    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess &&
-        prop.multiProcessorCount > 0)
-      c->num_cu = prop.multiProcessorCount;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess) {
+      if (prop.multiProcessorCount > 0) c->num_cu = prop.multiProcessorCount;
+      if (prop.sharedMemPerBlock > 0) c->max_lds_block = (int)prop.sharedMemPerBlock;
+    }
   }
+  // the tile-list build: one block per 256 tiles (tiles <= cells); pooled
+  // walks: one chunk per workgroup, at most one workgroup per wave
+  c->list_part_cap = (nct + 8) / 256 + 2;
+  c->bound_cap = 8 * ((c->num_cu * 32 + 7) / 8);
+  // tile launches have at most num_cu * 32 + 31 wavefronts (tile_grid)
+  c->wave_ctr_cap = c->num_cu * 32 + 32;
+  if ((e = dalloc(&c->list_part, 2 * c->list_part_cap)) || (e = dalloc(&c->tile_bound, c->bound_cap + 1)) ||
+      (e = dalloc(&c->counters, counter_words(c))))
+    return e;
   if (c->prm.precision == TSPH_PREC_MIXED) {
     if ((e = dalloc(&c->mx.rv, cap)) || (e = dalloc(&c->mx.rp, cap))) return e;
   }
   if (hipMemsetAsync(c->scal, 0, S_NSCAL * sizeof(int), c->stream) != hipSuccess ||
-      hipMemsetAsync(c->counters, 0, C_NCOUNT * sizeof(unsigned long long), c->stream) != hipSuccess ||
+      hipMemsetAsync(c->counters, 0, counter_words(c) * sizeof(unsigned long long), c->stream) != hipSuccess ||
       hipStreamSynchronize(c->stream) != hipSuccess)
     return fail(TSPH_EHIP, "initialisation failed");
   return 0;
@@ -373,7 +382,7 @@ int tsph_destroy(tsph_ctx *c) {
   void *ps[] = {c->scal, c->counters, c->cell_count, c->cell_start, c->cellof, c->hrank,
                 c->hcell, c->perm, c->scan_tmp, c->seqstate, c->probe_out, c->mx.rv,
                 c->mx.rp, c->mx.nrm, c->tile_list, c->wall_list, c->bz_list, c->cases,
-                c->tile_base};
+                c->tile_base, c->list_part, c->tile_bound};
   for (void *p : ps)
     if (p) (void)hipFree(p);
   for (hipEvent_t ev : c->tev) (void)hipEventDestroy(ev);
@@ -461,7 +470,6 @@ static int upload_impl(tsph_ctx *c, const tsph_particles *h, const uint32_t *seq
   for (int i = 0; i < n; ++i) c->n_wall += t[i] == TSPH_WALL;
   c->cells_valid = false;
   c->tiles_valid = false;
-  c->tiles_zeroed = true;  // the upload above zeroed the device tile counter
   c->records_valid = false;
   c->marks_valid = false;
   return 0;
@@ -900,9 +908,11 @@ int tsph_interpolate_point(tsph_ctx *c, double x, double y, double *out4) {
 
 int tsph_get_counters(tsph_ctx *c, tsph_counters *out) {
   if (!c || !out) return fail(TSPH_EINVAL, "null argument");
-  unsigned long long v[C_NCOUNT];
+  std::vector<unsigned long long> v(counter_words(c));
   TSPH_HIP(hipStreamSynchronize(c->stream));
-  TSPH_HIP(hipMemcpy(v, c->counters, sizeof v, hipMemcpyDeviceToHost));
+  TSPH_HIP(hipMemcpy(v.data(), c->counters, v.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+  for (int w = 0; w < c->wave_ctr_cap; ++w)
+    for (int q = 0; q < 3; ++q) v[C_PAIRS + q] += v[kWaveCtr0 + 4 * w + q];  // C_PAIRS, C_CAND, C_ACTIVE
   out->pairs = v[C_PAIRS];
   out->candidates = v[C_CAND];
   out->active = v[C_ACTIVE];
@@ -920,8 +930,7 @@ int tsph_get_counters(tsph_ctx *c, tsph_counters *out) {
 int tsph_reset_counters(tsph_ctx *c) {
   if (!c) return fail(TSPH_EINVAL, "null argument");
   TSPH_HIP(hipStreamSynchronize(c->stream));
-  unsigned long long z[C_NCOUNT] = {};
-  TSPH_HIP(hipMemcpy(c->counters, z, sizeof z, hipMemcpyHostToDevice));
+  TSPH_HIP(hipMemset(c->counters, 0, counter_words(c) * sizeof(unsigned long long)));
   return 0;
 }
 

@@ -363,7 +363,6 @@ int launch_cell_list(tsph_ctx *c, bool eos) {
   c->parity ^= 1;
   c->cells_valid = true;
   c->tiles_valid = false;
-  c->tiles_zeroed = true;
   c->records_valid = eos && c->prm.precision == TSPH_PREC_MIXED;
   return 0;
 }

@@ -26,17 +26,18 @@ namespace tsph {
 constexpr uint8_t kRemoved = 255;
 
 // device scalar slots
-// (S_TQ: the pair loops' tile queue, 8 band ticket counters and 9 completion
-// counters, each in its own 128-byte line, kTqStride ints apart)
-constexpr int kTqStride = 32;
-constexpr int kBalancedStatic8 = 0;  // eighths of each band walked statically when balanced
-enum { S_NP = 0, S_MAXSEQ = 1, S_NTILES = 2, S_NWALL = 3, S_BZCNT = 4, S_TQ = 32,
-       S_NSCAL = S_TQ + 17 * kTqStride };
+enum { S_NP = 0, S_MAXSEQ = 1, S_NTILES = 2, S_NWALL = 3, S_BZCNT = 4, S_NSCAL = 32 };
 // device counter slots (uint64)
 enum {
   C_PAIRS = 0, C_CAND = 1, C_ACTIVE = 2, C_PASSES = 3, C_BADRHO = 4, C_OUTGRID = 5,
   C_REMOVED = 6, C_APPENDED = 7, C_BZREJECT = 8, C_BZCHAIN = 9, C_REINIT = 10, C_NCOUNT = 11
 };
+// The pair loops' counts (pairs, candidates, active) go to per-wavefront
+// slots after the counters above, 4 words per wavefront, and are summed when
+// read: one device-wide atomic per wavefront on three shared words, all at the
+// end of a launch, serialises there and delays the launch's last tiles.
+constexpr int kWaveCtr0 = 16;
+
 
 // buffer zone (BufferZone, src/SPH_inlet_outlet_data.h:18-46)
 struct Zone {
@@ -93,7 +94,7 @@ __device__ __forceinline__ Consts case_consts(const Consts &k0, const Consts *__
 
 // host-side flags that describe what the device buffers hold
 struct HostState {
-  bool cells_valid, tiles_valid, tiles_zeroed, records_valid, marks_valid;
+  bool cells_valid, tiles_valid, records_valid, marks_valid;
 };
 
 // one captured step.  A step may rebuild the cell list an odd number of times
@@ -130,13 +131,20 @@ struct tsph_ctx {
   int tile_rows = 16;  // cells per pair-loop tile (one grid-column run)
   int waves_per_cu = 8;  // persistent pair-loop waves per compute unit
   int num_cu = 256;      // compute units of the device
-  // pair-loop tile walk per precision (FP64, mixed): balanced (a static
-  // prefix of each band, then atomic tickets) or the plain static walk.  Long
-  // FP64 tiles gain from balancing; short mixed tiles lose more to the tickets.
-  bool balanced[2] = {true, false};
-  int *tile_list = nullptr;  // non-empty tiles of the current cell list
+  int max_lds_block = 65536;  // LDS bytes a workgroup may allocate
+  // pair-loop tile walk per precision (FP64, mixed): pooled (work-balanced
+  // chunks per workgroup, tiles taken from an LDS counter by the workgroup's
+  // waves) or the plain static walk (the default: on the dam-break workloads
+  // the static walk at 8 waves per CU is as fast, and the autotuner tries both)
+  bool balanced[2] = {false, false};
+  int *tile_list = nullptr;  // non-empty tiles of the current cell list, in tile order
   bool tiles_valid = false;  // tile_list matches the cell list and tile_rows
-  bool tiles_zeroed = false; // the device tile counter is 0 (set by the cell-list build)
+  int *list_part = nullptr;  // per list-build block: tiles, work
+  int list_part_cap = 0;     // list-build blocks list_part holds
+  int *tile_bound = nullptr; // pooled walk: first list entry of each workgroup's chunk
+  int wave_ctr_cap = 0;      // wavefronts with count slots in `counters`
+  int bound_cap = 0;         // chunks tile_bound holds
+  int list_groups = 0;       // chunks the current tile_bound was cut for (0: none)
   int *wall_list = nullptr;  // slots of the wall particles of the current cell list
   int n_wall = 0;            // wall particles (walls are never created or removed)
   bool records_valid = false;  // mixed-path FP32 records match the FP64 state
@@ -194,25 +202,43 @@ int hip_fail(hipError_t e, const char *what, const char *file, int line);
 
 inline int grid_for(int n, int block) { return n <= 0 ? 1 : (n + block - 1) / block; }
 
+// words of the counters buffer: the counters, then the per-wavefront slots
+inline size_t counter_words(const tsph_ctx *c) { return kWaveCtr0 + 4 * (size_t)c->wave_ctr_cap; }
+
 // pair-loop tiles of the cell grid(s) at the current tile_rows
 inline int total_tiles(const tsph_ctx *c) {
   if (!c->cases) return c->k.ncx * ((c->k.ncy + c->tile_rows - 1) / c->tile_rows);
   return c->host_tile_base[c->ncases];
 }
 
-// whether a pair launch of `waves` persistent waves walks its tiles balanced:
+// whether a pair launch of `waves` persistent waves takes the pooled walk:
 // the precision's setting, for grids of at least 8 tiles per wave (below that
-// the tickets cost more than the balance gains)
+// the waves have about a tile each and nothing to balance)
 inline bool walk_balanced(const tsph_ctx *c, bool fp32, int waves) {
   return c->balanced[fp32 ? 1 : 0] && total_tiles(c) >= 8 * waves;
 }
+
+// launch geometry of a tile kernel: `groups` workgroups (a multiple of 8) of
+// block.x / 64 wavefronts, each wavefront with lds_wave bytes of LDS
+struct TileGrid {
+  dim3 grid, block;
+  size_t lds = 0;    // dynamic LDS per workgroup
+  int lds_wave = 0;
+  int groups = 0;
+  bool pooled = false;
+};
 
 // tsph_cells.hip
 // hash, count, scan, scatter, order, reorder; eos: the reorder also runs
 // Density_to_pressure (and writes the mixed records) for the moved particles
 int launch_cell_list(tsph_ctx *c, bool eos = false);
 // tsph_pairs.hip
-int ensure_tile_list(tsph_ctx *c);  // list of non-empty pair-loop tiles
+// list of the non-empty pair-loop tiles; groups > 0: also its chunk bounds
+// for a pooled walk of that many workgroups
+int ensure_tile_list(tsph_ctx *c, int groups);
+// geometry of a tile-kernel launch at the context's waves per CU (pooled when
+// walk_balanced holds for the precision)
+TileGrid tile_grid(tsph_ctx *c, bool fp32, size_t lds_wave);
 int launch_forces(tsph_ctx *c, int variant);
 int launch_mixed_records(tsph_ctx *c);
 // tsph_stages.hip

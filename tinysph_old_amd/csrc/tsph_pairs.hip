@@ -288,65 +288,151 @@ __device__ void particle32_global(const Soa &d, const Mixed &mx, const int *__re
 }
 
 
-__global__ void k_zero_tiles(int *scal) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) scal[S_NTILES] = 0;
+// The list of non-empty tiles in tile order, built in two passes over blocks
+// of kListBlock tiles: k_tile_count sums each block's non-empty tiles and
+// their work (own particles + kTileWork0 per tile); k_tile_emit adds up the
+// sums of the blocks before its own (the block count is small), scans its
+// block and writes the list.  For the pooled walk it also cuts the list into
+// `groups` chunks of about equal work: chunk g starts at the entry whose work
+// interval [W0, W1) holds floor(g W / groups) (W: total work).
+constexpr int kListBlock = 256;
+constexpr int kTileWork0 = 16;  // per-tile work (slice staging, set-up) in own particles
+
+__device__ __forceinline__ int tile_own(const int *__restrict__ cell_start, const Consts &k0,
+                                        const Consts *__restrict__ cases,
+                                        const int *__restrict__ tile_base, int ncases, int rows,
+                                        int t) {
+  int lt = t, cell_base = k0.cell_base, ncy = k0.ncy;
+  if (cases) {
+    const int cse = case_of_tile(tile_base, ncases, t);
+    lt = t - tile_base[cse];
+    cell_base = cases[cse].cell_base;
+    ncy = cases[cse].ncy;
+  }
+  const int *cs = cell_start + cell_base;
+  const int nty = (ncy + rows - 1) / rows;
+  const int cx = lt / nty, cy0 = (lt - cx * nty) * rows, cy1 = min(cy0 + rows, ncy);
+  return cs[cx * ncy + cy1] - cs[cx * ncy + cy0];
 }
 
-// list of the non-empty tiles, in tile order within each workgroup's chunk;
-// for a batched ensemble the tile ids run over the cases (tile_base)
-__global__ __launch_bounds__(256) void k_tile_list(const int *__restrict__ cell_start, Consts k0,
-                                                   const Consts *__restrict__ cases,
-                                                   const int *__restrict__ tile_base, int ncases,
-                                                   int rows, int ntiles, int *__restrict__ scal,
-                                                   int *__restrict__ tile_list) {
-  __shared__ int lds[4];
-  __shared__ int base;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  int flag = 0;
-  if (t < ntiles) {
-    Consts k = k0;
-    int lt = t;
-    if (cases) {
-      const int cse = case_of_tile(tile_base, ncases, t);
-      lt = t - tile_base[cse];
-      k = cases[cse];
+// sums of a and b over the block (kListBlock threads)
+__device__ __forceinline__ void block_sum2(int &a, int &b, int *lds) {
+  for (int o = 32; o > 0; o >>= 1) {
+    a += __shfl_xor(a, o, 64);
+    b += __shfl_xor(b, o, 64);
+  }
+  const int wid = threadIdx.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) {
+    lds[2 * wid] = a;
+    lds[2 * wid + 1] = b;
+  }
+  __syncthreads();
+  a = b = 0;
+  for (int w = 0; w < kListBlock / 64; ++w) {
+    a += lds[2 * w];
+    b += lds[2 * w + 1];
+  }
+}
+
+__global__ __launch_bounds__(kListBlock) void k_tile_count(const int *__restrict__ cell_start, Consts k0,
+                                                           const Consts *__restrict__ cases,
+                                                           const int *__restrict__ tile_base,
+                                                           int ncases, int rows, int ntiles,
+                                                           int *__restrict__ part) {
+  __shared__ int lds[2 * kListBlock / 64];
+  const int t = blockIdx.x * kListBlock + threadIdx.x;
+  const int own = t < ntiles ? tile_own(cell_start, k0, cases, tile_base, ncases, rows, t) : 0;
+  int n = own > 0, w = own > 0 ? own + kTileWork0 : 0;
+  block_sum2(n, w, lds);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = n;
+    part[2 * blockIdx.x + 1] = w;
+  }
+}
+
+__global__ __launch_bounds__(kListBlock) void k_tile_emit(const int *__restrict__ cell_start, Consts k0,
+                                                          const Consts *__restrict__ cases,
+                                                          const int *__restrict__ tile_base,
+                                                          int ncases, int rows, int ntiles,
+                                                          const int *__restrict__ part, int groups,
+                                                          int *__restrict__ scal,
+                                                          int *__restrict__ tile_list,
+                                                          int *__restrict__ bound) {
+  __shared__ int lds[2 * kListBlock / 64];
+  const int nb = gridDim.x;
+  // tiles and work of the blocks before this one, and of all blocks
+  int bn = 0, bw = 0, tn = 0, tw = 0;
+  for (int b = threadIdx.x; b < nb; b += kListBlock) {
+    const int n = part[2 * b], w = part[2 * b + 1];
+    if (b < (int)blockIdx.x) {
+      bn += n;
+      bw += w;
     }
-    const int *cs = cell_start + k.cell_base;
-    const int nty = (k.ncy + rows - 1) / rows;
-    const int cx = lt / nty, cy0 = (lt - cx * nty) * rows, cy1 = min(cy0 + rows, k.ncy);
-    flag = cs[cx * k.ncy + cy1] > cs[cx * k.ncy + cy0];
+    tn += n;
+    tw += w;
   }
-  // block-wide exclusive scan of the flags (4 waves)
+  block_sum2(bn, bw, lds);
+  block_sum2(tn, tw, lds);
+  const int t = blockIdx.x * kListBlock + threadIdx.x;
+  const int own = t < ntiles ? tile_own(cell_start, k0, cases, tile_base, ncases, rows, t) : 0;
+  const int flag = own > 0, work = flag ? own + kTileWork0 : 0;
+  // exclusive scans of flag and work over the block
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  const unsigned long long m = __ballot(flag);
-  const int before = __popcll(m & ((1ull << lane) - 1ull));
-  if (lane == 0) lds[wid] = __popcll(m);
-  __syncthreads();
-  int off = 0, tot = 0;
-  for (int w = 0; w < 4; ++w) {
-    if (w < wid) off += lds[w];
-    tot += lds[w];
+  int xn = flag, xw = work;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int yn = __shfl_up(xn, o, 64), yw = __shfl_up(xw, o, 64);
+    if (lane >= o) {
+      xn += yn;
+      xw += yw;
+    }
   }
-  if (threadIdx.x == 0) base = tot ? atomicAdd(&scal[S_NTILES], tot) : 0;
   __syncthreads();
-  if (flag) tile_list[base + off + before] = t;
+  if (lane == 63) {
+    lds[2 * wid] = xn;
+    lds[2 * wid + 1] = xw;
+  }
+  __syncthreads();
+  for (int w = 0; w < wid; ++w) {
+    xn += lds[2 * w];
+    xw += lds[2 * w + 1];
+  }
+  if (flag) {
+    const int idx = bn + xn - 1;
+    tile_list[idx] = t;
+    if (groups > 0) {
+      const long long w0 = bw + xw - work, w1 = w0 + work;
+      const long long g0 = (w0 * groups + tw - 1) / tw, g1 = (w1 * groups + tw - 1) / tw;
+      for (long long g = g0; g < g1 && g < groups; ++g) bound[g] = idx;
+    }
+  }
+  if (blockIdx.x == 0) {
+    if (threadIdx.x == 0) scal[S_NTILES] = tn;
+    if (groups > 0) {
+      if (tn == 0)
+        for (int g = threadIdx.x; g <= groups; g += kListBlock) bound[g] = 0;
+      else if (threadIdx.x == 0)
+        bound[groups] = tn;
+    }
+  }
 }
-
 
 // --------------------------------------------------------------------------
 // FP64 tile kernel
 // --------------------------------------------------------------------------
 template <int VAR, bool DYN>
-__global__ __launch_bounds__(kWave) void k_tile64(Soa d, const int *__restrict__ cell_start,
+__global__ __launch_bounds__(kWave * kPoolWaves) void k_tile64(Soa d, const int *__restrict__ cell_start,
                                                   const int *__restrict__ cellof, Consts k0,
                                                   const Consts *__restrict__ cases,
                                                   const int *__restrict__ tile_base, int ncases,
                                                   int rows, int *__restrict__ scal,
                                                   const int *__restrict__ tile_list, int cap,
-                                                  int stat8, unsigned long long *__restrict__ counters) {
+                                                  const int *__restrict__ bound, int lds_wave,
+                                                  unsigned long long *__restrict__ counters) {
   extern __shared__ __align__(16) unsigned char smem[];
-  int *tab = reinterpret_cast<int *>(smem);
-  double2 *s_r = reinterpret_cast<double2 *>(smem + kTabBytes);
+  unsigned char *wl = wave_lds(smem, lds_wave);
+  int *tab = reinterpret_cast<int *>(wl);
+  double2 *s_r = reinterpret_cast<double2 *>(wl + kTabBytes);
   double2 *s_v = s_r + cap;
   double2 *s_rp = s_v + cap;  // {rho, p}
   double2 *s_n = s_rp + cap;  // BT only
@@ -356,9 +442,9 @@ __global__ __launch_bounds__(kWave) void k_tile64(Soa d, const int *__restrict__
 
   if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&counters[C_PASSES], 1ull);
   unsigned npair = 0, ncand = 0, nact = 0;
-  TileQueue tq = tq_begin(scal + S_TQ, scal[S_NTILES], stat8);
+  TileQueue tq = tq_begin<DYN>(scal, bound, smem);
   for (int it = tq_take<DYN>(tq, tq_ticket<DYN>(tq)); it >= 0;) {
-  __syncthreads();  // the previous tile's LDS is no longer read
+  wave_sync();  // the previous tile's LDS is no longer read
   // the next tile's ticket: its atomic is in flight with this tile's loads,
   // and it is read once they have been waited for
   const int ticket = tq_ticket<DYN>(tq);
@@ -375,7 +461,7 @@ __global__ __launch_bounds__(kWave) void k_tile64(Soa d, const int *__restrict__
   tile_begin(k, cs, tile, rows, tab, t);
   if (t.lq[3] > cap) {
     const int next = tq_take<DYN>(tq, ticket);
-    for (int i = t.own0 + threadIdx.x; i < t.own1; i += blockDim.x) {
+    for (int i = t.own0 + tile_lane(); i < t.own1; i += kWave) {
       if (!is_active<VAR>(d.type[i])) continue;
       ++nact;
       particle64_global<VAR>(d, cs, cellof, k, i, npair, ncand);
@@ -383,7 +469,7 @@ __global__ __launch_bounds__(kWave) void k_tile64(Soa d, const int *__restrict__
     it = next;
     continue;
   }
-  for (int l = threadIdx.x; l < t.lq[3]; l += blockDim.x) {
+  for (int l = tile_lane(); l < t.lq[3]; l += kWave) {
     const int q = slice_of(t, l);
     const int j = l - sel3(t.base, q);
     s_r[l] = d.r[j];
@@ -393,10 +479,10 @@ __global__ __launch_bounds__(kWave) void k_tile64(Soa d, const int *__restrict__
     s_row[l] = (uint8_t)(cellof[j] - k.cell_base - (t.cx + q - 1) * k.ncy - t.row_lo);
     if (VAR == TSPH_FORCES_BT) s_n[l] = d.n[j];
   }
-  __syncthreads();
+  wave_sync();
   const int next = tq_take<DYN>(tq, ticket);
 
-  const int lane = threadIdx.x;
+  const int lane = tile_lane();
   uint16_t *my_list = s_list + lane;  // entry e at my_list[e * kWave]
   for (int i0 = t.own0; i0 < t.own1; i0 += kWave) {
     const int i = i0 + lane;
@@ -455,7 +541,6 @@ __global__ __launch_bounds__(kWave) void k_tile64(Soa d, const int *__restrict__
   }
   it = next;
   }  // tiles
-  tq_end<DYN>(tq);
   wave_count(npair, ncand, nact, counters);
 }
 
@@ -463,17 +548,19 @@ __global__ __launch_bounds__(kWave) void k_tile64(Soa d, const int *__restrict__
 // mixed tile kernel
 // --------------------------------------------------------------------------
 template <int VAR, bool DYN>
-__global__ __launch_bounds__(kWave) void k_tile32(Soa d, Mixed mx, const int *__restrict__ cell_start,
+__global__ __launch_bounds__(kWave * kPoolWaves) void k_tile32(Soa d, Mixed mx, const int *__restrict__ cell_start,
                                                   const int *__restrict__ cellof, Consts k0,
                                                   const Consts *__restrict__ cases,
                                                   const int *__restrict__ tile_base, int ncases,
                                                   int rows, int *__restrict__ scal,
                                                   const int *__restrict__ tile_list, int cap,
-                                                  int stat8, unsigned long long *__restrict__ counters) {
+                                                  const int *__restrict__ bound, int lds_wave,
+                                                  unsigned long long *__restrict__ counters) {
 #pragma clang fp contract(fast)
   extern __shared__ __align__(16) unsigned char smem[];
-  int *tab = reinterpret_cast<int *>(smem);
-  float2 *s_pos = reinterpret_cast<float2 *>(smem + kTabBytes);  // tile frame
+  unsigned char *wl = wave_lds(smem, lds_wave);
+  int *tab = reinterpret_cast<int *>(wl);
+  float2 *s_pos = reinterpret_cast<float2 *>(wl + kTabBytes);  // tile frame
   float2 *s_vel = s_pos + cap;
   float2 *s_dp = s_vel + cap;   // {rho - rho0, p}
   float2 *s_n = s_dp + cap;     // BT only
@@ -484,9 +571,12 @@ __global__ __launch_bounds__(kWave) void k_tile32(Soa d, Mixed mx, const int *__
 
   if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&counters[C_PASSES], 1ull);
   unsigned npair = 0, ncand = 0, nact = 0;
-  TileQueue tq = tq_begin(scal + S_TQ, scal[S_NTILES], stat8);
+  int pend_i = -1;  // particle whose results are still to be stored
+  double2 pend_a = make_double2(0., 0.);
+  double pend_dr = 0.;
+  TileQueue tq = tq_begin<DYN>(scal, bound, smem);
   for (int it = tq_take<DYN>(tq, tq_ticket<DYN>(tq)); it >= 0;) {
-  __syncthreads();  // the previous tile's LDS is no longer read
+  wave_sync();  // the previous tile's LDS is no longer read
   // the next tile's ticket: its atomic is in flight with this tile's loads,
   // and it is read once they have been waited for
   const int ticket = tq_ticket<DYN>(tq);
@@ -504,7 +594,7 @@ __global__ __launch_bounds__(kWave) void k_tile32(Soa d, Mixed mx, const int *__
   tile_begin(k, cs, tile, rows, tab, t);
   if (t.lq[3] > cap) {
     const int next = tq_take<DYN>(tq, ticket);
-    for (int i = t.own0 + threadIdx.x; i < t.own1; i += blockDim.x) {
+    for (int i = t.own0 + tile_lane(); i < t.own1; i += kWave) {
       if (!is_active<VAR>(d.type[i])) continue;
       ++nact;
       particle32_global<VAR>(d, mx, cs, cellof, k, cc, i, npair, ncand);
@@ -513,24 +603,53 @@ __global__ __launch_bounds__(kWave) void k_tile32(Soa d, Mixed mx, const int *__
     continue;
   }
   const int mid = t.row_lo + (t.nrows >> 1);
-  for (int l = threadIdx.x; l < t.lq[3]; l += blockDim.x) {
-    const int q = slice_of(t, l);
-    const int j = l - sel3(t.base, q);
-    const float4 rv = mx.rv[j];
-    const float2 rp = mx.rp[j];
-    const int row = cellof[j] - k.cell_base - (t.cx + q - 1) * k.ncy;
-    s_pos[l] = make_float2(rv.x + (float)(q - 1) * cc.kh, rv.y + (float)(row - mid) * cc.kh);
-    s_vel[l] = make_float2(rv.z, rv.w);
-    s_dp[l] = rp;
-    s_ir[l] = __builtin_amdgcn_rcpf(cc.rho0 + rp.x);
-    s_t[l] = d.type[j];
-    s_row[l] = (uint8_t)(row - t.row_lo);
-    if (VAR == TSPH_FORCES_BT) s_n[l] = mx.nrm[j];
+  // staging: the records of kStageU slots per lane are loaded together (slots
+  // past the end load the last slot again, so every load is consumed on every
+  // path and no wait is left for the compiler to place conservatively).  The
+  // previous tile's last results are stored after the staging: their write
+  // acknowledgements then return while this tile is computed, instead of
+  // delaying the wait for the staging loads (one counter covers both).
+  for (int l0 = 0; l0 < t.lq[3]; l0 += kStageU * kWave) {
+    float4 rv[kStageU];
+    float2 rp[kStageU], nr[kStageU];
+    int cl[kStageU], ty[kStageU];
+#pragma unroll
+    for (int u = 0; u < kStageU; ++u) {
+      const int l = min(l0 + u * kWave + tile_lane(), t.lq[3] - 1);
+      const int j = l - sel3(t.base, slice_of(t, l));
+      rv[u] = mx.rv[j];
+      rp[u] = mx.rp[j];
+      cl[u] = cellof[j];
+      ty[u] = d.type[j];
+      if (VAR == TSPH_FORCES_BT) nr[u] = mx.nrm[j];
+    }
+#pragma unroll
+    for (int u = 0; u < kStageU; ++u) {
+      const int l = l0 + u * kWave + tile_lane();
+      const int q = slice_of(t, min(l, t.lq[3] - 1));
+      const int row = cl[u] - k.cell_base - (t.cx + q - 1) * k.ncy;
+      const float2 pos = make_float2(rv[u].x + (float)(q - 1) * cc.kh, rv[u].y + (float)(row - mid) * cc.kh);
+      const float ir = __builtin_amdgcn_rcpf(cc.rho0 + rp[u].x);
+      if (l < t.lq[3]) {
+        s_pos[l] = pos;
+        s_vel[l] = make_float2(rv[u].z, rv[u].w);
+        s_dp[l] = rp[u];
+        s_ir[l] = ir;
+        s_t[l] = (uint8_t)ty[u];
+        s_row[l] = (uint8_t)(row - t.row_lo);
+        if (VAR == TSPH_FORCES_BT) s_n[l] = nr[u];
+      }
+    }
   }
-  __syncthreads();
+  if (pend_i >= 0) {
+    d.a[pend_i] = pend_a;
+    d.drho[pend_i] = pend_dr;
+    pend_i = -1;
+  }
+  wave_sync();
   const int next = tq_take<DYN>(tq, ticket);
 
-  const int lane = threadIdx.x;
+  const int lane = tile_lane();
   uint16_t *my_list = s_list + lane;
   for (int i0 = t.own0; i0 < t.own1; i0 += kWave) {
     const int i = i0 + lane;
@@ -615,17 +734,23 @@ __global__ __launch_bounds__(kWave) void k_tile32(Soa d, Mixed mx, const int *__
     // candidates: the stencil minus the particle itself (DBC: wall-wall pairs
     // are not candidates either, counted as the reference skips them)
     if (VAR != TSPH_FORCES_DBC) ncand += tot > 0 ? tot - 1 : 0;
+    // results are stored one chunk later (or after the next tile's staging)
+    if (pend_i >= 0) {
+      d.a[pend_i] = pend_a;
+      d.drho[pend_i] = pend_dr;
+    }
+    pend_i = -1;
     if (act) {
-      double2 a;
-      double dr;
-      finish32<VAR>(k, s, at, a, dr);
-      d.a[i] = a;
-      d.drho[i] = dr;
+      finish32<VAR>(k, s, at, pend_a, pend_dr);
+      pend_i = i;
     }
   }
   it = next;
   }  // tiles
-  tq_end<DYN>(tq);
+  if (pend_i >= 0) {
+    d.a[pend_i] = pend_a;
+    d.drho[pend_i] = pend_dr;
+  }
   wave_count(npair, ncand, nact, counters);
 }
 
@@ -648,45 +773,66 @@ int launch_mixed_records(tsph_ctx *c) {
   return 0;
 }
 
-int ensure_tile_list(tsph_ctx *c) {
+int ensure_tile_list(tsph_ctx *c, int groups) {
+  if (c->tiles_valid && (groups == 0 || groups == c->list_groups)) return 0;
   const Consts &k = c->k;
   const int rows = c->tile_rows;
   const int ntiles = total_tiles(c);
-  hipStream_t s = c->stream;
-  if (!c->tiles_valid) {
-    if (!c->tiles_zeroed) {
-      hipLaunchKernelGGL(k_zero_tiles, dim3(1), dim3(64), 0, s, c->scal);
-    }
-    hipLaunchKernelGGL(k_tile_list, dim3(grid_for(ntiles, 256)), dim3(256), 0, s, c->cell_start, k,
-                       c->cases, c->tile_base, c->ncases, rows, ntiles, c->scal, c->tile_list);
-    TSPH_HIP(hipGetLastError());
-    c->tiles_valid = true;
-    c->tiles_zeroed = false;
+  const int nb = grid_for(ntiles, kListBlock);
+  if (nb > c->list_part_cap || groups > c->bound_cap) {
+    set_error("tile list: more tiles or pooled workgroups than allocated");
+    return TSPH_ESTATE;
   }
+  hipStream_t s = c->stream;
+  hipLaunchKernelGGL(k_tile_count, dim3(nb), dim3(kListBlock), 0, s, c->cell_start, k, c->cases,
+                     c->tile_base, c->ncases, rows, ntiles, c->list_part);
+  hipLaunchKernelGGL(k_tile_emit, dim3(nb), dim3(kListBlock), 0, s, c->cell_start, k, c->cases,
+                     c->tile_base, c->ncases, rows, ntiles, c->list_part, groups, c->scal,
+                     c->tile_list, c->tile_bound);
+  TSPH_HIP(hipGetLastError());
+  c->tiles_valid = true;
+  c->list_groups = groups;
   return 0;
+}
+
+TileGrid tile_grid(tsph_ctx *c, bool fp32, size_t lds_wave) {
+  TileGrid tg;
+  const int waves = c->num_cu * c->waves_per_cu;
+  tg.lds_wave = (int)((lds_wave + 15) / 16 * 16);
+  tg.pooled = walk_balanced(c, fp32, waves);
+  int pw = 1;
+  if (tg.pooled) {
+    pw = (c->max_lds_block - kPoolHdr) / tg.lds_wave;
+    pw = pw < 1 ? 1 : (pw > kPoolWaves ? kPoolWaves : pw);
+  }
+  tg.groups = 8 * ((waves + 8 * pw - 1) / (8 * pw));
+  tg.grid = dim3(tg.groups);
+  tg.block = dim3(kWave * pw);
+  tg.lds = kPoolHdr + (size_t)pw * tg.lds_wave;
+  return tg;
 }
 
 namespace {
 
 // one pair pass: the kernel for the variant, the precision and the tile walk
-// (balanced: static prefix + atomic tickets; otherwise the plain static walk)
+// (pooled or static)
 template <int VAR, bool DYN>
-void pair_launch(tsph_ctx *c, dim3 g, size_t lds, int rows, int cap) {
+void pair_launch(tsph_ctx *c, const TileGrid &tg, int rows, int cap) {
   int *sc = c->scal;
   if (c->prm.precision == TSPH_PREC_MIXED)
-    hipLaunchKernelGGL((k_tile32<VAR, DYN>), g, dim3(kWave), lds, c->stream, c->cur, c->mx,
+    hipLaunchKernelGGL((k_tile32<VAR, DYN>), tg.grid, tg.block, tg.lds, c->stream, c->cur, c->mx,
                        c->cell_start, c->cellof, c->k, c->cases, c->tile_base, c->ncases, rows, sc,
-                       c->tile_list, cap, kBalancedStatic8, c->counters);
+                       c->tile_list, cap, c->tile_bound, tg.lds_wave, c->counters);
   else
-    hipLaunchKernelGGL((k_tile64<VAR, DYN>), g, dim3(kWave), lds, c->stream, c->cur, c->cell_start,
-                       c->cellof, c->k, c->cases, c->tile_base, c->ncases, rows, sc, c->tile_list,
-                       cap, kBalancedStatic8, c->counters);
+    hipLaunchKernelGGL((k_tile64<VAR, DYN>), tg.grid, tg.block, tg.lds, c->stream, c->cur,
+                       c->cell_start, c->cellof, c->k, c->cases, c->tile_base, c->ncases, rows, sc,
+                       c->tile_list, cap, c->tile_bound, tg.lds_wave, c->counters);
 }
 
 template <int VAR>
-void pair_launch(tsph_ctx *c, bool balanced, dim3 g, size_t lds, int rows, int cap) {
-  if (balanced) pair_launch<VAR, true>(c, g, lds, rows, cap);
-  else pair_launch<VAR, false>(c, g, lds, rows, cap);
+void pair_launch(tsph_ctx *c, const TileGrid &tg, int rows, int cap) {
+  if (tg.pooled) pair_launch<VAR, true>(c, tg, rows, cap);
+  else pair_launch<VAR, false>(c, tg, rows, cap);
 }
 
 }  // namespace
@@ -695,8 +841,6 @@ int launch_forces(tsph_ctx *c, int variant) {
   const int rows = c->tile_rows;
   const int cap = 3 * (rows + 2) * kCellCap;
   const bool bt = variant == TSPH_FORCES_BT;
-  if (int e = ensure_tile_list(c)) return e;
-  const dim3 g(8 * ((c->num_cu * c->waves_per_cu + 7) / 8));
   const size_t list = sizeof(uint16_t) * kList * kWave;
   const bool mixed = c->prm.precision == TSPH_PREC_MIXED;
   size_t lds;
@@ -710,12 +854,13 @@ int launch_forces(tsph_ctx *c, int variant) {
   } else {
     lds = kTabBytes + list + (size_t)cap * (3 * sizeof(double2) + 2 + (bt ? sizeof(double2) : 0));
   }
-  const bool bal = walk_balanced(c, mixed, (int)g.x);
+  const TileGrid tg = tile_grid(c, mixed, lds);
+  if (int e = ensure_tile_list(c, tg.pooled ? tg.groups : 0)) return e;
   return launch_timed(c, [&] {
     switch (variant) {
-      case TSPH_FORCES_WCSPH: pair_launch<TSPH_FORCES_WCSPH>(c, bal, g, lds, rows, cap); break;
-      case TSPH_FORCES_DBC: pair_launch<TSPH_FORCES_DBC>(c, bal, g, lds, rows, cap); break;
-      default: pair_launch<TSPH_FORCES_BT>(c, bal, g, lds, rows, cap);
+      case TSPH_FORCES_WCSPH: pair_launch<TSPH_FORCES_WCSPH>(c, tg, rows, cap); break;
+      case TSPH_FORCES_DBC: pair_launch<TSPH_FORCES_DBC>(c, tg, rows, cap); break;
+      default: pair_launch<TSPH_FORCES_BT>(c, tg, rows, cap);
     }
   });
 }

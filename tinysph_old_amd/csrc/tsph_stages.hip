@@ -98,62 +98,132 @@ __device__ __forceinline__ int point_cell(const Consts &k, double x, double y) {
   return cell_coord(y, k.y0, k.of, k.kh) + cell_coord(x, k.x0, k.of, k.kh) * k.ncy;
 }
 
-// Wall densities: one lane per wall particle.  The walk over the 9 cells
-// loads the records of eight consecutive candidates together (their loads are
-// in flight at once) and then adds their terms in candidate order, so every
-// sum is formed in the reference order.  Candidates that are not fluid or lie
-// outside the support add nothing, as in the reference.
-constexpr int kWallBatch = 8;
+// Wall densities: kWG lanes per wall particle.  The 9 stencil cells (cells
+// outside the grid skipped) form one candidate sequence in the reference order
+// (stencil order, ascending index within a cell); lane u of the group takes
+// candidates u, u + kWG, u + 2 kWG, ...  The records of kWPre such rounds are
+// loaded together (all in flight at once); then, round by round, the group's
+// lanes evaluate their candidates' terms at once, put them in LDS, and lane u
+// adds sums q = u, u + kWG, ... over the round's candidates in candidate
+// order, so every sum is formed in the reference order.  A candidate that is
+// not a fluid neighbour within the support adds nothing, as in the reference.
+constexpr int kWG = 8;
+constexpr int kWPre = 10;
+constexpr int kWallsPerWave = 64 / kWG;
 
 template <int NS, typename Term>
 __device__ __forceinline__ void wall_sums(const Soa &d, const int *__restrict__ cell_start,
-                                          const Consts &k, double2 ar, double (&acc)[NS],
+                                          const Consts &k, double2 ar, bool live,
+                                          double (&acc)[NS], double *__restrict__ terms,
                                           Term term) {
+  constexpr int QPL = (NS + kWG - 1) / kWG;  // sums per lane
+  const int u = threadIdx.x & (kWG - 1);
+  const int g0 = threadIdx.x & 63 & ~(kWG - 1);  // the group's first lane
+  double *grp = terms + g0 * NS;                  // kWG x NS doubles of the group
   const double r2cut = 4.0 * k.h * k.h * (1.0 + 1e-12);
-  const int c = point_cell(k, ar.x, ar.y);
+  // candidate f lies in the last cell s with pre[s] <= f, at index f + dj[s]
+  int pre[9], dj[9], tot = 0;
 #pragma unroll
-  for (int q = 0; q < NS; ++q) acc[q] = 0.;
   for (int s = 0; s < 9; ++s) {
-    const int cl = c + stencil_off(s, k.ncy);
-    if (cl < 0 || cl >= k.ncells) continue;
-    const int j0 = cell_start[cl], j1 = cell_start[cl + 1];
-    for (int jb = j0; jb < j1; jb += kWallBatch) {
-      int tt[kWallBatch];
-      double2 rr[kWallBatch];
-      double hh[kWallBatch];
+    pre[s] = 0;
+    dj[s] = 0;
+  }
+  if (live) {
+    const int c = point_cell(k, ar.x, ar.y);
 #pragma unroll
-      for (int u = 0; u < kWallBatch; ++u) {
-        const int j = jb + u < j1 ? jb + u : j1 - 1;
-        tt[u] = jb + u < j1 ? (int)d.type[j] : TSPH_WALL;
-        rr[u] = d.r[j];
-        hh[u] = d.rho[j];
+    for (int s = 0; s < 9; ++s) {
+      const int cl = c + stencil_off(s, k.ncy);
+      int lo = 0, n = 0;
+      if (cl >= 0 && cl < k.ncells) {
+        lo = cell_start[cl];
+        n = cell_start[cl + 1] - lo;
       }
-#pragma unroll
-      for (int u = 0; u < kWallBatch; ++u) {
-        const double drx = ar.x - rr[u].x, dry = ar.y - rr[u].y;
-        const double r2 = drx * drx + dry * dry;
-        if (tt[u] == TSPH_FLUID && r2 <= r2cut) {
-          const double drs = sqrt(r2);
-          if (drs / k.h <= 2.0) term(hh[u], drx, dry, drs, acc);
-        }
-      }
+      pre[s] = tot;
+      dj[s] = lo - tot;
+      tot += n;
     }
   }
+  double mine[QPL];  // sums u + kWG x
+#pragma unroll
+  for (int x = 0; x < QPL; ++x) mine[x] = 0.;
+  for (int base = 0; base < tot; base += kWG * kWPre) {  // uniform within the group
+    int tt[kWPre];
+    double2 rr[kWPre];
+    double hh[kWPre];
+#pragma unroll
+    for (int v = 0; v < kWPre; ++v) {
+      const int f = base + v * kWG + u;
+      int j = f + dj[0];
+#pragma unroll
+      for (int s = 1; s < 9; ++s) j = f >= pre[s] ? f + dj[s] : j;
+      tt[v] = -1;
+      rr[v] = make_double2(0., 0.);
+      hh[v] = 1.;
+      if (f < tot) {
+        tt[v] = d.type[j];
+        rr[v] = d.r[j];
+        hh[v] = d.rho[j];
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < kWPre; ++v) {
+      if (base + v * kWG >= tot) break;
+      bool ok = false;
+      if (tt[v] == TSPH_FLUID) {
+        const double drx = ar.x - rr[v].x, dry = ar.y - rr[v].y;
+        const double r2 = drx * drx + dry * dry;
+        if (r2 <= r2cut) {
+          const double drs = sqrt(r2);
+          if (drs / k.h <= 2.0) {
+            double t[NS];
+#pragma unroll
+            for (int q = 0; q < NS; ++q) t[q] = 0.;
+            term(hh[v], drx, dry, drs, t);  // t = the candidate's terms (0 + x is x)
+#pragma unroll
+            for (int q = 0; q < NS; ++q) grp[u * NS + q] = t[q];
+            ok = true;
+          }
+        }
+      }
+      const unsigned gm = (unsigned)(__ballot(ok) >> g0) & ((1u << kWG) - 1u);
+      // LDS accesses of one wavefront complete in issue order; the fences keep
+      // the compiler from moving them across the hand-over
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      for (int w = 0; w < kWG; ++w) {
+        if (!((gm >> w) & 1u)) continue;
+#pragma unroll
+        for (int x = 0; x < QPL; ++x)
+          if (u + kWG * x < NS) mine[x] += grp[w * NS + u + kWG * x];
+      }
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+    }
+  }
+  // every lane of the group gets all NS sums
+#pragma unroll
+  for (int x = 0; x < QPL; ++x)
+    if (u + kWG * x < NS) grp[u + kWG * x] = mine[x];
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+#pragma unroll
+  for (int q = 0; q < NS; ++q) acc[q] = grp[q];
 }
 
 __global__ __launch_bounds__(64) void k_mdbc(Soa d, const int *__restrict__ cell_start, const int *__restrict__ scal,
                        const int *__restrict__ wall_list, Consts k0,
                        const Consts *__restrict__ cases) {
-  const int w = blockIdx.x * blockDim.x + threadIdx.x;
-  if (w >= scal[S_NWALL]) return;
-  const int i = wall_list[w];
-  const Consts k = case_consts(k0, cases, d.cid, i);
-  const double2 gn = d.gnr[i], r = d.r[i];
+  __shared__ __align__(16) double s_terms[64 * 12];
+  const int w = blockIdx.x * kWallsPerWave + (int)(threadIdx.x / kWG);
+  const bool live = w < scal[S_NWALL];
+  const int i = live ? wall_list[w] : 0;
+  const Consts k = live ? case_consts(k0, cases, d.cid, i) : k0;
+  const double2 gn = live ? d.gnr[i] : make_double2(0., 0.);
   const double m = k.m;
   // sums of src/SPH_kernel_approx.h:1141-1342 in its order:
   // rhowsum, drhodx, drhody, a11, a12, a13, a21, a22, a23, a31, a32, a33
   double acc[12];
-  wall_sums<12>(d, cell_start + k.cell_base, k, gn, acc,
+  wall_sums<12>(d, cell_start + k.cell_base, k, gn, live, acc, s_terms,
                 [&](double nrho, double drx, double dry, double drs, double (&a)[12]) {
                   const KW64 kw = wendland64(drs, k.h, k.awen, k.bwen);
                   const double mr = m / nrho;
@@ -170,6 +240,8 @@ __global__ __launch_bounds__(64) void k_mdbc(Soa d, const int *__restrict__ cell
                   a[10] += dry * drx * kw.f * mr;
                   a[11] += dry * dry * kw.f * mr;
                 });
+  if (!live || (threadIdx.x & (kWG - 1)) != 0) return;
+  const double2 r = d.r[i];
   const double rhowsum = acc[0], drhodx = acc[1], drhody = acc[2];
   const double a11 = acc[3], a12 = acc[4], a13 = acc[5], a21 = acc[6], a22 = acc[7], a23 = acc[8];
   const double a31 = acc[9], a32 = acc[10], a33 = acc[11];
@@ -202,18 +274,20 @@ __global__ __launch_bounds__(64) void k_mdbc(Soa d, const int *__restrict__ cell
 __global__ __launch_bounds__(64) void k_bt_density(Soa d, const int *__restrict__ cell_start, const int *__restrict__ scal,
                              const int *__restrict__ wall_list, Consts k0,
                              const Consts *__restrict__ cases) {
-  const int w = blockIdx.x * blockDim.x + threadIdx.x;
-  if (w >= scal[S_NWALL]) return;
-  const int i = wall_list[w];
-  const Consts k = case_consts(k0, cases, d.cid, i);
+  __shared__ __align__(16) double s_terms[64 * 2];
+  const int w = blockIdx.x * kWallsPerWave + (int)(threadIdx.x / kWG);
+  const bool live = w < scal[S_NWALL];
+  const int i = live ? wall_list[w] : 0;
+  const Consts k = live ? case_consts(k0, cases, d.cid, i) : k0;
   const double m = k.m;
   double acc[2];  // wsum, rhowsum (src/SPH_kernel_approx.h:1838-1953)
-  wall_sums<2>(d, cell_start + k.cell_base, k, d.r[i], acc,
-               [&](double nrho, double, double, double drs, double (&a)[2]) {
+  wall_sums<2>(d, cell_start + k.cell_base, k, live ? d.r[i] : make_double2(0., 0.), live, acc,
+               s_terms, [&](double nrho, double, double, double drs, double (&a)[2]) {
                  const KW64 kw = wendland64(drs, k.h, k.awen, k.bwen);
                  a[0] += kw.w * m / nrho;
                  a[1] += m * kw.w;
                });
+  if (!live || (threadIdx.x & (kWG - 1)) != 0) return;
   double rho = acc[0] > 0. ? acc[1] / acc[0] : 1000.;
   if (rho < 1000.) rho = 1000.;
   d.rho[i] = rho;
@@ -413,7 +487,7 @@ int launch_corrector(tsph_ctx *c, double dt) {
 }
 
 int launch_mdbc(tsph_ctx *c) {
-  hipLaunchKernelGGL(k_mdbc, dim3(grid_for(c->n_wall, 64)), dim3(64), 0, c->stream, c->cur,
+  hipLaunchKernelGGL(k_mdbc, dim3(grid_for(c->n_wall, kWallsPerWave)), dim3(64), 0, c->stream, c->cur,
                      c->cell_start, c->scal, c->wall_list, c->k, c->cases);
   TSPH_HIP(hipGetLastError());
   c->records_valid = false;
@@ -421,7 +495,7 @@ int launch_mdbc(tsph_ctx *c) {
 }
 
 int launch_bt_density(tsph_ctx *c) {
-  hipLaunchKernelGGL(k_bt_density, dim3(grid_for(c->n_wall, 64)), dim3(64), 0, c->stream, c->cur,
+  hipLaunchKernelGGL(k_bt_density, dim3(grid_for(c->n_wall, kWallsPerWave)), dim3(64), 0, c->stream, c->cur,
                      c->cell_start, c->scal, c->wall_list, c->k, c->cases);
   TSPH_HIP(hipGetLastError());
   c->records_valid = false;

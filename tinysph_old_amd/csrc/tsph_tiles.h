@@ -15,7 +15,10 @@ constexpr int kMaxTileRows = 24;   // upper bound of tile_rows (LDS offset table
 constexpr int kCellCap = 11;       // LDS records budgeted per cell of the slices
 constexpr int kList = 32;          // per-lane neighbour list entries in LDS
 constexpr int kGroup = 8;          // candidates tested together in the mixed pair loop
+constexpr int kStageU = 4;         // staged slots per lane whose loads are in flight together
 
+// adds the wavefront's counts to its own slots (kWaveCtr0 + 4 w, w: the
+// wavefront's index in the launch); no two wavefronts of a launch share a slot
 __device__ __forceinline__ void wave_count(unsigned pairs, unsigned cand, unsigned act,
                                            unsigned long long *counters) {
   unsigned long long a = pairs, b = cand, c = act;
@@ -26,9 +29,11 @@ __device__ __forceinline__ void wave_count(unsigned pairs, unsigned cand, unsign
     c += __shfl_xor(c, o, 64);
   }
   if ((threadIdx.x & 63) == 0) {
-    if (a) atomicAdd(&counters[C_PAIRS], a);
-    if (b) atomicAdd(&counters[C_CAND], b);
-    if (c) atomicAdd(&counters[C_ACTIVE], c);
+    const int w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    unsigned long long *p = counters + kWaveCtr0 + 4 * w;
+    p[0] += a;
+    p[1] += b;
+    p[2] += c;
   }
 }
 
@@ -64,121 +69,90 @@ __device__ __forceinline__ int sel3(const int (&a)[3], int q) {
   return q == 0 ? a[0] : (q == 1 ? a[1] : a[2]);
 }
 
-// Persistent waves: workgroup b runs on XCD b % 8, so the list of non-empty
-// tiles is cut into 8 contiguous bands and the waves of XCD x walk band x;
-// tiles that are neighbours in the list, which share slices, are then staged
-// from the same L2.  Two walks:
-//   * static (DYN = false): the waves of XCD x stride through band x, no
-//     atomics;
-//   * balanced (DYN = true): each band is split in two parts:
-//       - a static part, the first J whole rounds of the band (J = the rounds
-//         that fit in stat8/8 of it), strided as in the static walk;
-//       - a dynamic part, the rest: waves take its tiles in list order from
-//         an atomic ticket counter per band, and when their own band is empty
-//         they help the other bands in turn.
-// A wave ends when all eight counters have passed their bands' ends; counters
-// only grow, so every wave ends.  The last wave to finish
-// resets the counters for the next launch (tq_end), so no clearing pass or
-// memset is needed.  A wave takes its next ticket at the start
-// of a tile and reads it after the tile's staging loads have been waited for,
-// so the atomic's latency hides behind theirs.
+// Each wavefront of a tile kernel works on its own tile with its own LDS area;
+// a workgroup holds one wavefront (static walk) or up to kPoolWaves of them
+// (pooled walk), which share nothing but the pool counter.  LDS layout:
+// kPoolHdr bytes (the counter), then one area of lds_wave bytes per wavefront.
+constexpr int kPoolWaves = 4;
+constexpr int kPoolHdr = 16;
+
+__device__ __forceinline__ int tile_lane() { return threadIdx.x & (kWave - 1); }
+
+__device__ __forceinline__ unsigned char *wave_lds(unsigned char *smem, int lds_wave) {
+  return smem + kPoolHdr + (threadIdx.x >> 6) * lds_wave;
+}
+
+// Hand-over between the lanes of one wavefront through LDS (staged records,
+// lists): a wavefront's LDS instructions execute in issue order, so only the
+// compiler has to be kept from moving accesses across this point.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+// Persistent waves over the list of non-empty tiles (tile order).  Workgroup
+// b runs on XCD b % 8.  Two walks:
+//   * static (DYN = false, one wavefront per workgroup): the list is cut into
+//     8 contiguous bands and the waves of XCD x stride through band x, so
+//     tiles that are neighbours in the list, which share slices, are staged
+//     from the same L2;
+//   * pooled (DYN = true): the list is cut into one chunk per workgroup of
+//     about equal work (own particles plus a per-tile overhead, chunk bounds
+//     written by the list build), the chunks of XCD x again contiguous; the
+//     workgroup's wavefronts take the tiles of their chunk in order from a
+//     counter in LDS.  Waves of one compute unit thereby balance each other
+//     without any device-wide atomic.
 struct TileQueue {
-  int *q;      // 17 counters, kTqStride ints apart
-  int ntl;     // tiles in the list
-  int band;    // band of the current ticket
-  int nwb;     // waves per band
-  int wb;      // this wave's rank in its band
-  int j;       // static round of the current tile (kDynamic once in the dynamic part)
-  int J;       // static rounds of the own band
-  int stat8;   // eighths of each band assigned statically
+  int ntl;   // static: tiles in the list
+  int band;  // static: band of the wave
+  int nwb;   // static: waves per band
+  int wb;    // static: this wave's rank in its band
+  int j;     // static: round of the current tile
+  int hi;    // pooled: end of the chunk
+  int *ctr;  // pooled: the chunk's next tile (LDS)
 };
-constexpr int kDynamic = 0x3fffffff;
 
 __device__ __forceinline__ int band_lo(int ntl, int b) { return (int)(((long long)ntl * b) >> 3); }
 
-__device__ __forceinline__ int static_rounds(const TileQueue &tq, int b) {
-  const int size = band_lo(tq.ntl, b + 1) - band_lo(tq.ntl, b);
-  return (int)(((long long)size * tq.stat8) >> 3) / tq.nwb;
+template <bool DYN>
+__device__ __forceinline__ TileQueue tq_begin(const int *__restrict__ scal, const int *__restrict__ bound,
+                                              unsigned char *smem) {
+  TileQueue tq;
+  tq.ntl = 0;
+  tq.band = blockIdx.x & 7;
+  tq.nwb = gridDim.x >> 3;
+  tq.wb = blockIdx.x >> 3;
+  tq.j = -1;
+  tq.hi = 0;
+  tq.ctr = reinterpret_cast<int *>(smem);
+  if (!DYN) {
+    tq.ntl = scal[S_NTILES];
+  } else {
+    const int chunk = tq.band * tq.nwb + tq.wb;
+    tq.hi = bound[chunk + 1];
+    if (threadIdx.x == 0) *tq.ctr = bound[chunk];
+    __syncthreads();
+  }
+  return tq;
 }
 
-// first tile of band b's dynamic part
-__device__ __forceinline__ int dyn_lo(const TileQueue &tq, int b) {
-  return band_lo(tq.ntl, b) + static_rounds(tq, b) * tq.nwb;
-}
-
-__device__ __forceinline__ bool tq_static_next(const TileQueue &tq) { return tq.j + 1 < tq.J; }
-
-
-// lane 0 takes a ticket of the current band when the next tile is a dynamic
-// one (the value is read with tq_take)
+// pooled walk: lane 0 takes the next tile of the chunk (read with tq_take)
 template <bool DYN>
 __device__ __forceinline__ int tq_ticket(const TileQueue &tq) {
   int t = 0;
-  if (DYN && !tq_static_next(tq) && threadIdx.x == 0) t = atomicAdd(&tq.q[tq.band * kTqStride], 1);
+  if (DYN && tile_lane() == 0) t = atomicAdd(tq.ctr, 1);
   return t;
 }
 
 // list index of the wave's next tile (raw: the value from tq_ticket), or -1
-// when every band is exhausted
 template <bool DYN>
 __device__ __forceinline__ int tq_take(TileQueue &tq, int raw) {
   if (!DYN) {
     const int idx = band_lo(tq.ntl, tq.band) + tq.wb + (++tq.j) * tq.nwb;
     return idx < band_lo(tq.ntl, tq.band + 1) ? idx : -1;
   }
-  if (tq_static_next(tq)) {
-    ++tq.j;
-    return band_lo(tq.ntl, tq.band) + tq.wb + tq.j * tq.nwb;
-  }
-  tq.j = kDynamic;
-  int ticket = __builtin_amdgcn_readfirstlane(raw);
-  while (true) {
-    const int lo = dyn_lo(tq, tq.band), hi = band_lo(tq.ntl, tq.band + 1);
-    if (lo + ticket < hi) return lo + ticket;
-    // this band is exhausted: lanes 0-7 read the eight band counters at once
-    // (plain reads, so exhausted bands take no further atomics) and the wave
-    // moves on to the next band, in cyclic order, that still has tiles
-    bool open = false;
-    if (threadIdx.x < 8) {
-      const int b = threadIdx.x;
-      open = __hip_atomic_load(&tq.q[b * kTqStride], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <
-             band_lo(tq.ntl, b + 1) - dyn_lo(tq, b);
-    }
-    const unsigned m = (unsigned)__ballot(open) & 0xffu;
-    if (!m) return -1;
-    const unsigned rot = ((m >> tq.band) | (m << (8 - tq.band))) & 0xffu;  // bit i: band + i
-    tq.band = (tq.band + __builtin_ctz(rot)) & 7;
-    ticket = __builtin_amdgcn_readfirstlane(tq_ticket<DYN>(tq));
-  }
-}
-
-__device__ __forceinline__ TileQueue tq_begin(int *q, int ntl, int stat8) {
-  TileQueue tq;
-  tq.q = q;
-  tq.ntl = ntl;
-  tq.band = blockIdx.x & 7;
-  tq.nwb = gridDim.x >> 3;
-  tq.wb = blockIdx.x >> 3;
-  tq.stat8 = stat8;
-  tq.j = -1;
-  tq.J = static_rounds(tq, tq.band);
-  return tq;
-}
-
-// after the wave's last tq_take: count it finished.  The waves of a band
-// count on the band's own counter (q[8 + 1 + band]); the last of each band
-// counts on q[8], and the last of those resets every counter for the next
-// launch.  Two levels keep the end-of-launch atomics spread over nine lines.
-// No fence is needed: the wave's last ticket has returned (its value was read)
-// before these atomics are issued, and the tiles' outputs are for later launches.
-template <bool DYN>
-__device__ __forceinline__ void tq_end(const TileQueue &tq) {
-  if (DYN && threadIdx.x == 0) {
-    const int band = blockIdx.x & 7;
-    if (atomicAdd(&tq.q[(9 + band) * kTqStride], 1) == tq.nwb - 1 &&
-        atomicAdd(&tq.q[8 * kTqStride], 1) == 7)
-      for (int b = 0; b < 17; ++b) atomicExch(&tq.q[b * kTqStride], 0);
-  }
+  const int v = __builtin_amdgcn_readfirstlane(raw);
+  return v < tq.hi ? v : -1;
 }
 
 // Loads the cell offsets of the three slices (one global load per lane, all in
@@ -192,12 +166,12 @@ __device__ __forceinline__ bool tile_begin(const Consts &k, const int *__restric
   t.row_lo = max(t.cy0 - 1, 0);
   t.nrows = min(t.cy1 + 1, k.ncy) - t.row_lo;
   t.w = t.nrows + 1;
-  for (int e = threadIdx.x; e < 3 * t.w; e += blockDim.x) {
+  for (int e = tile_lane(); e < 3 * t.w; e += kWave) {
     const int q = e / t.w, r = e - q * t.w;
     const int col = t.cx + q - 1;
     tab[e] = (col >= 0 && col < k.ncx) ? cell_start[col * k.ncy + t.row_lo + r] : 0;
   }
-  __syncthreads();
+  wave_sync();
   t.own0 = tab[t.w + (t.cy0 - t.row_lo)];
   t.own1 = tab[t.w + (t.cy1 - t.row_lo)];
   int l = 0;
@@ -245,27 +219,30 @@ template <bool DYN, class P>
 __device__ __forceinline__ void walk64(const P &pol, const Consts &k, const int *__restrict__ cell_start,
                                        const int *__restrict__ cellof, int rows,
                                        int *__restrict__ scal, const int *__restrict__ tile_list,
-                                       int cap, unsigned long long *__restrict__ counters) {
+                                       const int *__restrict__ bound, int cap, int lds_wave,
+                                       unsigned long long *__restrict__ counters) {
   extern __shared__ __align__(16) unsigned char smem[];
-  int *tab = reinterpret_cast<int *>(smem);
-  uint16_t *s_list = reinterpret_cast<uint16_t *>(smem + kTabBytes);
-  const typename P::Lds L = P::lds(smem + kTabBytes + sizeof(uint16_t) * kList * kWave, cap);
+  unsigned char *wl = wave_lds(smem, lds_wave);
+  int *tab = reinterpret_cast<int *>(wl);
+  uint16_t *s_list = reinterpret_cast<uint16_t *>(wl + kTabBytes);
+  const typename P::Lds L = P::lds(wl + kTabBytes + sizeof(uint16_t) * kList * kWave, cap);
+  const int lane = tile_lane();
   if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&counters[C_PASSES], 1ull);
   unsigned npair = 0, ncand = 0, nact = 0;
-  TileQueue tq = tq_begin(scal + S_TQ, scal[S_NTILES], kBalancedStatic8);
+  TileQueue tq = tq_begin<DYN>(scal, bound, smem);
   for (int it = tq_take<DYN>(tq, tq_ticket<DYN>(tq)); it >= 0;) {
-    __syncthreads();
+    wave_sync();
     const int ticket = tq_ticket<DYN>(tq);  // read once this tile's loads are waited for
     Tile t;
     tile_begin(k, cell_start, tile_list[it], rows, tab, t);
     const bool staged = t.lq[3] <= cap;
     if (staged)
-      for (int l = threadIdx.x; l < t.lq[3]; l += blockDim.x) pol.stage(L, l, l - sel3(t.base, slice_of(t, l)));
-    __syncthreads();
+      for (int l = lane; l < t.lq[3]; l += kWave) pol.stage(L, l, l - sel3(t.base, slice_of(t, l)));
+    wave_sync();
     const int next = tq_take<DYN>(tq, ticket);
-    uint16_t *my_list = s_list + threadIdx.x;
+    uint16_t *my_list = s_list + lane;
     for (int i0 = t.own0; i0 < t.own1; i0 += kWave) {
-      const int i = i0 + threadIdx.x;
+      const int i = i0 + lane;
       int li = 0, cy = 0;
       bool act = false;
       if (i < t.own1) {
@@ -327,7 +304,6 @@ __device__ __forceinline__ void walk64(const P &pol, const Consts &k, const int 
     }
     it = next;
   }
-  tq_end<DYN>(tq);
   wave_count(npair, ncand, nact, counters);
 }
 
@@ -335,18 +311,21 @@ template <bool DYN, class P>
 __device__ __forceinline__ void walk32(const P &pol, const Consts &k, const int *__restrict__ cell_start,
                                        const int *__restrict__ cellof, int rows,
                                        int *__restrict__ scal, const int *__restrict__ tile_list,
-                                       int cap, unsigned long long *__restrict__ counters) {
+                                       const int *__restrict__ bound, int cap, int lds_wave,
+                                       unsigned long long *__restrict__ counters) {
 #pragma clang fp contract(fast)
   extern __shared__ __align__(16) unsigned char smem[];
-  int *tab = reinterpret_cast<int *>(smem);
-  uint16_t *s_list = reinterpret_cast<uint16_t *>(smem + kTabBytes);
-  const typename P::Lds L = P::lds(smem + kTabBytes + sizeof(uint16_t) * kList * kWave, cap);
+  unsigned char *wl = wave_lds(smem, lds_wave);
+  int *tab = reinterpret_cast<int *>(wl);
+  uint16_t *s_list = reinterpret_cast<uint16_t *>(wl + kTabBytes);
+  const typename P::Lds L = P::lds(wl + kTabBytes + sizeof(uint16_t) * kList * kWave, cap);
+  const int lane = tile_lane();
   const float r2cut = 4.f * (float)k.h * (float)k.h;
   if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&counters[C_PASSES], 1ull);
   unsigned npair = 0, ncand = 0, nact = 0;
-  TileQueue tq = tq_begin(scal + S_TQ, scal[S_NTILES], kBalancedStatic8);
+  TileQueue tq = tq_begin<DYN>(scal, bound, smem);
   for (int it = tq_take<DYN>(tq, tq_ticket<DYN>(tq)); it >= 0;) {
-    __syncthreads();
+    wave_sync();
     const int ticket = tq_ticket<DYN>(tq);  // read once this tile's loads are waited for
     Tile t;
     tile_begin(k, cell_start, tile_list[it], rows, tab, t);
@@ -354,13 +333,13 @@ __device__ __forceinline__ void walk32(const P &pol, const Consts &k, const int 
     const int mid = t.row_lo + (t.nrows >> 1);
     const double ox = k.x0 - k.of + t.cx * k.kh, oy = k.y0 - k.of + mid * k.kh;
     if (staged)
-      for (int l = threadIdx.x; l < t.lq[3]; l += blockDim.x)
+      for (int l = lane; l < t.lq[3]; l += kWave)
         pol.stage(L, l, l - sel3(t.base, slice_of(t, l)), ox, oy);
-    __syncthreads();
+    wave_sync();
     const int next = tq_take<DYN>(tq, ticket);
-    uint16_t *my_list = s_list + threadIdx.x;
+    uint16_t *my_list = s_list + lane;
     for (int i0 = t.own0; i0 < t.own1; i0 += kWave) {
-      const int i = i0 + threadIdx.x;
+      const int i = i0 + lane;
       int li = 0, ri = 0;
       bool act = false;
       if (i < t.own1) {
@@ -424,7 +403,6 @@ __device__ __forceinline__ void walk32(const P &pol, const Consts &k, const int 
     }
     it = next;
   }
-  tq_end<DYN>(tq);
   wave_count(npair, ncand, nact, counters);
 }
 
