@@ -69,12 +69,13 @@ static void drop_graph(tsph_ctx *c) {
 }
 
 static HostState host_state(const tsph_ctx *c) {
-  return {c->cells_valid, c->tiles_valid, c->records_valid, c->marks_valid};
+  return {c->cells_valid, c->tiles_valid, c->records_valid, c->marks_valid, c->list_groups};
 }
 
 static void set_host_state(tsph_ctx *c, const HostState &h) {
   c->cells_valid = h.cells_valid;
   c->tiles_valid = h.tiles_valid;
+  c->list_groups = h.list_groups;
   c->records_valid = h.records_valid;
   c->marks_valid = h.marks_valid;
 }
@@ -196,10 +197,16 @@ static int record_step(tsph_ctx *c, int wall, double dt, int inlet = -1, int out
   } else if (wall == TSPH_WALL_BT) {
     if ((e = launch_bt_density(c)) || (e = launch_eos(c))) return e;
   }
-  if ((e = launch_predictor(c, dt))) return e;
-  if (inlet >= 0 && (e = launch_bz_inlet(c, inlet, dt))) return e;
-  if (outlet >= 0 && (e = launch_bz_outlet(c, outlet, dt))) return e;
-  if ((e = launch_cell_list(c, true)) || (e = launch_forces(c, variant))) return e;
+  if (inlet >= 0 || outlet >= 0) {
+    // the buffer zones act on the predicted positions
+    if ((e = launch_predictor(c, dt))) return e;
+    if (inlet >= 0 && (e = launch_bz_inlet(c, inlet, dt))) return e;
+    if (outlet >= 0 && (e = launch_bz_outlet(c, outlet, dt))) return e;
+    if ((e = launch_cell_list(c, true))) return e;
+  } else if ((e = launch_cell_list(c, true, true, dt))) {  // predictor applied by the build
+    return e;
+  }
+  if ((e = launch_forces(c, variant))) return e;
   if ((e = launch_corrector(c, dt)) || (e = launch_remove(c))) return e;
   if ((e = launch_cell_list(c, true)) || (e = launch_forces(c, variant))) return e;
   return 0;
@@ -292,7 +299,8 @@ static int alloc_context(tsph_ctx *c) {
   if ((e = dalloc(&c->scal, S_NSCAL)) ||
       (e = dalloc(&c->cell_count, nct + 1)) || (e = dalloc(&c->cell_start, nct + 2)) ||
       (e = dalloc(&c->cellof, cap)) || (e = dalloc(&c->hrank, cap)) || (e = dalloc(&c->hcell, cap)) ||
-      (e = dalloc(&c->perm, cap)) || (e = dalloc(&c->scan_tmp, (nct + 1) / 1024 + 2)) ||
+      (e = dalloc(&c->perm, cap)) || (e = dalloc(&c->scan_tmp, 2 * ((nct + 1) / 1024 + 2))) ||
+      (e = dalloc(&c->tile_flag, nct + 8)) ||
       (e = dalloc(&c->seqstate, (size_t)c->seq_cap)) || (e = dalloc(&c->probe_out, 4)) ||
       (e = dalloc(&c->bz_list, 2 * kBzMax)) ||
       (e = dalloc(&c->tile_list, nct + 8)) || (e = dalloc(&c->wall_list, cap)))
@@ -382,7 +390,7 @@ int tsph_destroy(tsph_ctx *c) {
   void *ps[] = {c->scal, c->counters, c->cell_count, c->cell_start, c->cellof, c->hrank,
                 c->hcell, c->perm, c->scan_tmp, c->seqstate, c->probe_out, c->mx.rv,
                 c->mx.rp, c->mx.nrm, c->tile_list, c->wall_list, c->bz_list, c->cases,
-                c->tile_base, c->list_part, c->tile_bound};
+                c->tile_base, c->list_part, c->tile_bound, c->tile_flag};
   for (void *p : ps)
     if (p) (void)hipFree(p);
   for (hipEvent_t ev : c->tev) (void)hipEventDestroy(ev);

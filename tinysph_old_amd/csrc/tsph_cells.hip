@@ -37,9 +37,13 @@ __device__ __forceinline__ unsigned long long lanemask_lt() {
   return (1ull << (threadIdx.x & 63)) - 1ull;
 }
 
-__global__ void k_hash(const double2 *__restrict__ r, const uint8_t *__restrict__ type,
+// PRED: the cell of the position the symplectic predictor gives (the
+// predictor itself is applied by the reorder that follows)
+template <bool PRED>
+__global__ void k_hash(const double2 *__restrict__ r, const double2 *__restrict__ v,
+                       const uint8_t *__restrict__ type,
                        const uint32_t *__restrict__ seq, const int *__restrict__ scal, Consts k0,
-                       const Consts *__restrict__ cases, const uint8_t *__restrict__ cid,
+                       const Consts *__restrict__ cases, const uint8_t *__restrict__ cid, double dt0,
                        int ncells_tot, int *__restrict__ cell_count, int *__restrict__ hcell,
                        int *__restrict__ hrank, uint8_t *__restrict__ seqstate,
                        unsigned long long *__restrict__ counters) {
@@ -53,7 +57,8 @@ __global__ void k_hash(const double2 *__restrict__ r, const uint8_t *__restrict_
       seqstate[seq[i]] = 0;  // its reference position is vacated
     } else {
       const Consts k = case_consts(k0, cases, cid, i);
-      const double2 ri = r[i];
+      double2 ri = r[i];
+      if (PRED) ri = predicted_r(ri, v[i], cases ? k.dt : dt0);
       const int cx = cell_coord(ri.x, k.x0, k.of, k.kh);
       const int cy = cell_coord(ri.y, k.y0, k.of, k.kh);
       c = cy + cx * k.ncy;  // POS(x,y) = y + x*Y, src/SPH_defs.h:36
@@ -113,55 +118,129 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int *lds, int &total)
   return wo + x - v;
 }
 
+// The pair loops' tiles (runs of `rows` cells of a grid column, per case) are
+// listed by the scan too: a tile belongs to the scan block that holds its
+// first cell.  The first pass marks each tile (non-empty: one of its cells
+// holds a particle) and counts the marked tiles of each block, the second
+// lists them in tile order.  One pass reads the counts of a tile's cells past
+// its own block's end, so the marks are made before the second pass clears
+// the counts.
+struct TileScan {
+  Consts k0;
+  const Consts *cases;
+  const int *tile_base;
+  int ncases;
+  int rows;
+  uint8_t *flag;  // per tile
+  int *list;
+};
+
+// tile of cell c if c is a tile's first cell, else -1; n: the tile's cells
+__device__ __forceinline__ int tile_at(const TileScan &ts, int c, int &n) {
+  int cse = 0, cell_base = ts.k0.cell_base, ncy = ts.k0.ncy, ncells = ts.k0.ncells;
+  if (ts.cases) {
+    int lo = 0, hi = ts.ncases;
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (ts.cases[mid].cell_base <= c) lo = mid;
+      else hi = mid;
+    }
+    cse = lo;
+    cell_base = ts.cases[cse].cell_base;
+    ncy = ts.cases[cse].ncy;
+    ncells = ts.cases[cse].ncells;
+  }
+  const int lc = c - cell_base;
+  if (lc < 0 || lc >= ncells) return -1;
+  const int cx = lc / ncy, cy = lc - cx * ncy;
+  if (cy % ts.rows) return -1;
+  n = min(ts.rows, ncy - cy);
+  return (ts.cases ? ts.tile_base[cse] : 0) + cx * ((ncy + ts.rows - 1) / ts.rows) + cy / ts.rows;
+}
+
+// sums of the first pass over the blocks before this one (pre) and over all
+__device__ __forceinline__ void block_offsets(const int *__restrict__ partial, int &pre_c, int &pre_t,
+                                              int &all_c, int &all_t, int *lds) {
+  int v[4] = {0, 0, 0, 0};
+  for (int b = threadIdx.x; b < (int)gridDim.x; b += kScanBlock) {
+    const int pc = partial[2 * b], pt = partial[2 * b + 1];
+    if (b < (int)blockIdx.x) {
+      v[0] += pc;
+      v[1] += pt;
+    }
+    v[2] += pc;
+    v[3] += pt;
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+    for (int o = 32; o > 0; o >>= 1) v[q] += __shfl_xor(v[q], o, 64);
+  const int wid = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0)
+    for (int q = 0; q < 4; ++q) lds[4 * wid + q] = v[q];
+  __syncthreads();
+  pre_c = pre_t = all_c = all_t = 0;
+  for (int w = 0; w < kScanBlock / 64; ++w) {
+    pre_c += lds[4 * w];
+    pre_t += lds[4 * w + 1];
+    all_c += lds[4 * w + 2];
+    all_t += lds[4 * w + 3];
+  }
+  __syncthreads();
+}
+
 __global__ __launch_bounds__(kScanBlock) void k_scan_partials(const int *__restrict__ in, int len,
-                                                              int *__restrict__ partial) {
+                                                              int *__restrict__ partial, TileScan ts) {
   __shared__ int lds[kScanBlock / 64];
   const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
-  int s = 0;
+  int s = 0, nt = 0;
 #pragma unroll
-  for (int j = 0; j < kScanItems; ++j) s += (base + j < len) ? in[base + j] : 0;
-  int total;
+  for (int j = 0; j < kScanItems; ++j) {
+    if (base + j >= len) continue;
+    s += in[base + j];
+    int n = 0;
+    const int t = tile_at(ts, base + j, n);
+    if (t >= 0) {
+      int any = 0;
+      for (int r = 0; r < n; ++r) any |= in[base + j + r];
+      ts.flag[t] = any != 0;
+      nt += any != 0;
+    }
+  }
+  int total, tiles;
   block_exclusive_scan<kScanBlock>(s, lds, total);
-  if (threadIdx.x == 0) partial[blockIdx.x] = total;
-}
-
-// one block: exclusive scan of the block partials; writes out[len] = total and
-// clears the list counters that the reorder and pair passes rebuild
-__global__ __launch_bounds__(1024) void k_scan_top(int *__restrict__ partial, int nb,
-                                                   int *__restrict__ out, int len,
-                                                   int *__restrict__ scal) {
-  __shared__ int lds[1024 / 64];
-  int carry = 0;
-  for (int base = 0; base < nb; base += 1024) {
-    const int i = base + threadIdx.x;
-    const int v = i < nb ? partial[i] : 0;
-    int total;
-    const int ex = block_exclusive_scan<1024>(v, lds, total);
-    if (i < nb) partial[i] = carry + ex;
-    carry += total;
-  }
+  block_exclusive_scan<kScanBlock>(nt, lds, tiles);
   if (threadIdx.x == 0) {
-    out[len] = carry;
-    scal[S_NTILES] = 0;
-    scal[S_NWALL] = 0;
+    partial[2 * blockIdx.x] = total;
+    partial[2 * blockIdx.x + 1] = tiles;
   }
 }
 
-// final pass; also clears the counts it consumed for the next build
+// second pass: cell offsets and the tile list (each block adds up the first
+// pass's sums of the blocks before it); clears the counts it consumed for the
+// next build; block 0 writes out[len] (the total), the tile count and clears
+// the wall-list counter the reorder rebuilds
 __global__ __launch_bounds__(kScanBlock) void k_scan_final(int *__restrict__ in, int len,
                                                            const int *__restrict__ partial,
-                                                           int *__restrict__ out) {
-  __shared__ int lds[kScanBlock / 64];
+                                                           int *__restrict__ out,
+                                                           int *__restrict__ scal, TileScan ts) {
+  __shared__ int lds[4 * kScanBlock / 64];
+  int pre_c, pre_t, all_c, all_t;
+  block_offsets(partial, pre_c, pre_t, all_c, all_t, lds);
   const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
-  int v[kScanItems];
-  int s = 0;
+  int v[kScanItems], tl[kScanItems];
+  int s = 0, nt = 0;
 #pragma unroll
   for (int j = 0; j < kScanItems; ++j) {
     v[j] = (base + j < len) ? in[base + j] : 0;
     s += v[j];
+    int n = 0;
+    tl[j] = base + j < len ? tile_at(ts, base + j, n) : -1;
+    if (tl[j] >= 0 && !ts.flag[tl[j]]) tl[j] = -1;
+    nt += tl[j] >= 0;
   }
-  int total;
-  int ex = block_exclusive_scan<kScanBlock>(s, lds, total) + partial[blockIdx.x];
+  int total, tiles;
+  int ex = block_exclusive_scan<kScanBlock>(s, lds, total) + pre_c;
+  int et = block_exclusive_scan<kScanBlock>(nt, lds, tiles) + pre_t;
 #pragma unroll
   for (int j = 0; j < kScanItems; ++j) {
     if (base + j < len) {
@@ -169,6 +248,12 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_final(int *__restrict__ in,
       in[base + j] = 0;
     }
     ex += v[j];
+    if (tl[j] >= 0) ts.list[et++] = tl[j];
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    out[len] = all_c;
+    scal[S_NTILES] = all_t;
+    scal[S_NWALL] = 0;
   }
 }
 
@@ -245,11 +330,14 @@ __global__ void k_order(const int *__restrict__ cell_start, int ncells,
 // With EOS the pressure is recomputed as Density_to_pressure does
 // (src/SPH_pressure.h:65-89) and the FP32 records of the mixed path are
 // written, for the step sequences in which that stage follows the cell build.
-template <bool EOS>
+// With PRED the symplectic predictor (src/SPH_integration.h:266-287) is
+// applied to the moved particles, for the step sequences in which it
+// immediately precedes the cell build.
+template <bool EOS, bool PRED>
 __global__ void k_reorder(const int *__restrict__ cell_start, int ncells, const int *__restrict__ perm,
                           Soa src, Soa dst, int *__restrict__ scal, int *__restrict__ wall_list,
                           const int *__restrict__ cellof, Consts k0,
-                          const Consts *__restrict__ cases, Mixed mx,
+                          const Consts *__restrict__ cases, Mixed mx, double dt0,
                           unsigned long long *__restrict__ counters) {
   const int n_new = cell_start[ncells];
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -260,22 +348,32 @@ __global__ void k_reorder(const int *__restrict__ cell_start, int ncells, const 
     const int j = perm[k];
     const uint8_t t = src.type[j];
     wall = t == TSPH_WALL;
-    const double2 r = src.r[j], v = src.v[j];
-    const double rho = src.rho[j];
+    double2 r = src.r[j], v = src.v[j];
+    const double2 a = src.a[j];
+    double rho = src.rho[j];
+    const double drho = src.drho[j];
+    const Consts kc = EOS || PRED ? case_consts(k0, cases, src.cid, j) : k0;
     if (src.cid) dst.cid[k] = src.cid[j];
+    if (PRED) {
+      const double dt = cases ? kc.dt : dt0;
+      dst.r_o[k] = r;
+      dst.v_o[k] = v;
+      dst.rho_o[k] = rho;
+      predict(r, v, rho, a, drho, dt);
+    } else {
+      dst.r_o[k] = src.r_o[j];
+      dst.v_o[k] = src.v_o[j];
+      dst.rho_o[k] = src.rho_o[j];
+    }
     dst.r[k] = r;
     dst.v[k] = v;
-    dst.a[k] = src.a[j];
+    dst.a[k] = a;
     dst.rho[k] = rho;
-    dst.drho[k] = src.drho[j];
+    dst.drho[k] = drho;
     dst.type[k] = t;
     dst.seq[k] = src.seq[j];
     dst.index[k] = src.index[j];
-    dst.r_o[k] = src.r_o[j];
-    dst.v_o[k] = src.v_o[j];
-    dst.rho_o[k] = src.rho_o[j];
     double p = src.p[j];
-    const Consts kc = EOS ? case_consts(k0, cases, src.cid, j) : k0;
     if (EOS && t != TSPH_OUTLET && t != TSPH_INLET) {
       bad = rho > 1500. || rho < 0.;
       p = tait64(rho, kc.rho0, kc.cs);
@@ -325,45 +423,59 @@ __global__ void k_reorder(const int *__restrict__ cell_start, int ncells, const 
   }
 }
 
-int scan_exclusive(int *in, int len, int *out, int *partial, int *scal, hipStream_t s) {
+// exclusive scan of the cell counts into cell_start, with the tile list of the
+// current tile_rows
+int scan_cells(tsph_ctx *c, hipStream_t s) {
+  const int len = c->ncells_tot + 1;
   const int nb = (len + kScanTile - 1) / kScanTile;
-  hipLaunchKernelGGL(k_scan_partials, dim3(nb), dim3(kScanBlock), 0, s, in, len, partial);
-  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, s, partial, nb, out, len, scal);
-  hipLaunchKernelGGL(k_scan_final, dim3(nb), dim3(kScanBlock), 0, s, in, len, partial, out);
+  const TileScan ts{c->k, c->cases, c->tile_base, c->ncases, c->tile_rows, c->tile_flag, c->tile_list};
+  hipLaunchKernelGGL(k_scan_partials, dim3(nb), dim3(kScanBlock), 0, s, c->cell_count, len,
+                     c->scan_tmp, ts);
+  hipLaunchKernelGGL(k_scan_final, dim3(nb), dim3(kScanBlock), 0, s, c->cell_count, len,
+                     c->scan_tmp, c->cell_start, c->scal, ts);
   TSPH_HIP(hipGetLastError());
   return 0;
 }
 
 }  // namespace
 
-int launch_cell_list(tsph_ctx *c, bool eos) {
+int launch_cell_list(tsph_ctx *c, bool eos, bool predict, double dt) {
   const Consts &k = c->k;
   const int nb = 256;
   const int g = grid_for(c->n_ub, nb);
   hipStream_t s = c->stream;
   const int nct = c->ncells_tot;
-  hipLaunchKernelGGL(k_hash, dim3(g), dim3(nb), 0, s, c->cur.r, c->cur.type, c->cur.seq, c->scal, k,
-                     c->cases, c->cur.cid, nct, c->cell_count, c->hcell, c->hrank, c->seqstate,
-                     c->counters);
-  if (int e = scan_exclusive(c->cell_count, nct + 1, c->cell_start, c->scan_tmp, c->scal, s))
-    return e;
+  if (predict)
+    hipLaunchKernelGGL(k_hash<true>, dim3(g), dim3(nb), 0, s, c->cur.r, c->cur.v, c->cur.type,
+                       c->cur.seq, c->scal, k, c->cases, c->cur.cid, dt, nct, c->cell_count,
+                       c->hcell, c->hrank, c->seqstate, c->counters);
+  else
+    hipLaunchKernelGGL(k_hash<false>, dim3(g), dim3(nb), 0, s, c->cur.r, c->cur.v, c->cur.type,
+                       c->cur.seq, c->scal, k, c->cases, c->cur.cid, dt, nct, c->cell_count,
+                       c->hcell, c->hrank, c->seqstate, c->counters);
+  if (int e = scan_cells(c, s)) return e;
   hipLaunchKernelGGL(k_scatter, dim3(g), dim3(nb), 0, s, c->scal, c->hcell, c->hrank,
                      c->cell_start, c->perm, c->cellof);
   hipLaunchKernelGGL(k_order, dim3(grid_for(nct, nb)), dim3(nb), 0, s, c->cell_start, nct,
                      c->cur.seq, c->perm);
   const Mixed mx = c->prm.precision == TSPH_PREC_MIXED ? c->mx : Mixed{};
-  if (eos)
-    hipLaunchKernelGGL(k_reorder<true>, dim3(g), dim3(nb), 0, s, c->cell_start, nct, c->perm,
-                       c->cur, c->alt, c->scal, c->wall_list, c->cellof, k, c->cases, mx, c->counters);
-  else
-    hipLaunchKernelGGL(k_reorder<false>, dim3(g), dim3(nb), 0, s, c->cell_start, nct, c->perm,
-                       c->cur, c->alt, c->scal, c->wall_list, c->cellof, k, c->cases, mx, c->counters);
+#define TSPH_REORDER(E, P)                                                                    \
+  hipLaunchKernelGGL((k_reorder<E, P>), dim3(g), dim3(nb), 0, s, c->cell_start, nct, c->perm,      \
+                     c->cur, c->alt, c->scal, c->wall_list, c->cellof, k, c->cases, mx, dt,       \
+                     c->counters)
+  if (eos && predict) TSPH_REORDER(true, true);
+  else if (eos) TSPH_REORDER(true, false);
+  else if (predict) TSPH_REORDER(false, true);
+  else TSPH_REORDER(false, false);
+#undef TSPH_REORDER
   TSPH_HIP(hipGetLastError());
   std::swap(c->cur, c->alt);
   c->parity ^= 1;
   c->cells_valid = true;
-  c->tiles_valid = false;
+  c->tiles_valid = true;  // listed by the scan (no pooled-walk chunks)
+  c->list_groups = 0;
   c->records_valid = eos && c->prm.precision == TSPH_PREC_MIXED;
+  if (predict) c->marks_valid = false;
   return 0;
 }
 

@@ -95,6 +95,7 @@ __device__ __forceinline__ Consts case_consts(const Consts &k0, const Consts *__
 // host-side flags that describe what the device buffers hold
 struct HostState {
   bool cells_valid, tiles_valid, records_valid, marks_valid;
+  int list_groups;
 };
 
 // one captured step.  A step may rebuild the cell list an odd number of times
@@ -141,6 +142,7 @@ struct tsph_ctx {
   bool tiles_valid = false;  // tile_list matches the cell list and tile_rows
   int *list_part = nullptr;  // per list-build block: tiles, work
   int list_part_cap = 0;     // list-build blocks list_part holds
+  uint8_t *tile_flag = nullptr; // per tile: non-empty (the cell-list scan's marks)
   int *tile_bound = nullptr; // pooled walk: first list entry of each workgroup's chunk
   int wave_ctr_cap = 0;      // wavefronts with count slots in `counters`
   int bound_cap = 0;         // chunks tile_bound holds
@@ -231,7 +233,9 @@ struct TileGrid {
 // tsph_cells.hip
 // hash, count, scan, scatter, order, reorder; eos: the reorder also runs
 // Density_to_pressure (and writes the mixed records) for the moved particles
-int launch_cell_list(tsph_ctx *c, bool eos = false);
+// predict: the symplectic predictor (step dt; per case in a batch) is applied
+// on the way, in place of a launch_predictor immediately before
+int launch_cell_list(tsph_ctx *c, bool eos = false, bool predict = false, double dt = 0.);
 // tsph_pairs.hip
 // list of the non-empty pair-loop tiles; groups > 0: also its chunk bounds
 // for a pooled walk of that many workgroups

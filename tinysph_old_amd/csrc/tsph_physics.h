@@ -81,6 +81,18 @@ __device__ __forceinline__ double diff_fourtakas64(double h, double drs, double 
 }
 
 // Compute_Pressure (Tait, gamma = 7), src/SPH_pressure.h:7-19
+// Integrate_SymplecticPredictor's half step, src/SPH_integration.h:266-287
+// (the expressions in the reference's order; no contraction)
+__device__ __forceinline__ double2 predicted_r(double2 r, double2 v, double dt) {
+  return make_double2(r.x + v.x * dt * 0.5, r.y + v.y * dt * 0.5);
+}
+__device__ __forceinline__ void predict(double2 &r, double2 &v, double &rho, double2 a, double drho,
+                                        double dt) {
+  rho = rho + drho * dt * 0.5;
+  r = predicted_r(r, v, dt);
+  v = make_double2(v.x + a.x * dt * 0.5, v.y + a.y * dt * 0.5);
+}
+
 __device__ __forceinline__ double tait64(double rho, double rho0, double c0) {
   const double b = c0 * c0 * rho0 / 7.;
   const double rr = rho / rho0;

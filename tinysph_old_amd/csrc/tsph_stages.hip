@@ -54,14 +54,15 @@ __global__ void k_predictor(Soa d, const int *__restrict__ scal, double dt0,
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double dt = cases ? cases[d.cid[i]].dt : dt0;
-  const double rho = d.rho[i];
-  const double2 r = d.r[i], v = d.v[i], a = d.a[i];
+  double rho = d.rho[i];
+  double2 r = d.r[i], v = d.v[i];
   d.rho_o[i] = rho;
   d.r_o[i] = r;
   d.v_o[i] = v;
-  d.rho[i] = rho + d.drho[i] * dt * 0.5;
-  d.r[i] = make_double2(r.x + v.x * dt * 0.5, r.y + v.y * dt * 0.5);
-  d.v[i] = make_double2(v.x + a.x * dt * 0.5, v.y + a.y * dt * 0.5);
+  predict(r, v, rho, d.a[i], d.drho[i], dt);
+  d.rho[i] = rho;
+  d.r[i] = r;
+  d.v[i] = v;
 }
 
 // removal state of a particle at reference position seq: 0 absent, 1 kept,
