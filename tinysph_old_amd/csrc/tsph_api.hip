@@ -300,7 +300,6 @@ static int alloc_context(tsph_ctx *c) {
       (e = dalloc(&c->cell_count, nct + 1)) || (e = dalloc(&c->cell_start, nct + 2)) ||
       (e = dalloc(&c->cellof, cap)) || (e = dalloc(&c->hrank, cap)) || (e = dalloc(&c->hcell, cap)) ||
       (e = dalloc(&c->perm, cap)) || (e = dalloc(&c->scan_tmp, 2 * ((nct + 1) / 1024 + 2))) ||
-      (e = dalloc(&c->tile_flag, nct + 8)) ||
       (e = dalloc(&c->seqstate, (size_t)c->seq_cap)) || (e = dalloc(&c->probe_out, 4)) ||
       (e = dalloc(&c->bz_list, 2 * kBzMax)) ||
       (e = dalloc(&c->tile_list, nct + 8)) || (e = dalloc(&c->wall_list, cap)))
@@ -390,7 +389,7 @@ int tsph_destroy(tsph_ctx *c) {
   void *ps[] = {c->scal, c->counters, c->cell_count, c->cell_start, c->cellof, c->hrank,
                 c->hcell, c->perm, c->scan_tmp, c->seqstate, c->probe_out, c->mx.rv,
                 c->mx.rp, c->mx.nrm, c->tile_list, c->wall_list, c->bz_list, c->cases,
-                c->tile_base, c->list_part, c->tile_bound, c->tile_flag};
+                c->tile_base, c->list_part, c->tile_bound};
   for (void *p : ps)
     if (p) (void)hipFree(p);
   for (hipEvent_t ev : c->tev) (void)hipEventDestroy(ev);
@@ -944,7 +943,7 @@ int tsph_reset_counters(tsph_ctx *c) {
 
 int tsph_set_tile_rows(tsph_ctx *c, int rows) {
   if (!c) return fail(TSPH_EINVAL, "null argument");
-  if (rows < 1 || rows > 24) return fail(TSPH_EINVAL, "tile rows must be 1..24");
+  if (rows < 1 || rows > kMaxTileRows) return fail(TSPH_EINVAL, "tile rows must be 1..24");
   if (rows != c->tile_rows) {
     drop_graph(c);
     c->tiles_valid = false;

@@ -7,12 +7,13 @@
 //                 cell order from the previous build, so a wavefront spans only
 //                 a handful of cells); removed particles go to the extra
 //                 bucket `ncells`
-//   2. scan       exclusive scan of the per-cell counts -> cell_start (CSR)
+//   2. scan       exclusive scan of the per-cell counts -> cell_start (CSR);
+//                 also lists the non-empty pair-loop tiles
 //   3. k_scatter  sorted position = cell_start[cell] + rank
 //   4. k_order    per cell, order its slots by `seq` (reference particle
 //                 order), which makes the result identical to the reference's
 //                 serial push_back in ascending particle index and independent
-//                 of the atomic arrival order
+//                 of the atomic arrival order; clears the counts
 //   5. k_reorder  gather the SoA fields into (cell, seq) order; also lists the
 //                 wall particles for the wall-density gathers
 //
@@ -20,8 +21,9 @@
 // the cell count, so a single digit covers it and its scan is the cell-start
 // table the pair loops need anyway); step 4 supplies the stability.
 // All passes read the particle count from device memory, so the sequence can
-// be captured in a HIP graph.  The per-cell counters are cleared by the scan
-// that consumes them, so no separate clearing pass is needed.
+// be captured in a HIP graph.  The per-cell counters are cleared by the
+// per-cell pass that follows their scan (k_order), so no separate clearing
+// pass is needed.  The scan also lists the pair loops' non-empty tiles.
 #include "tsph_internal.h"
 #include "tsph_physics.h"
 
@@ -120,24 +122,23 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int *lds, int &total)
 
 // The pair loops' tiles (runs of `rows` cells of a grid column, per case) are
 // listed by the scan too: a tile belongs to the scan block that holds its
-// first cell.  The first pass marks each tile (non-empty: one of its cells
-// holds a particle) and counts the marked tiles of each block, the second
-// lists them in tile order.  One pass reads the counts of a tile's cells past
-// its own block's end, so the marks are made before the second pass clears
-// the counts.
+// first cell.  Both passes stage the block's counts in LDS with the counts of
+// the next kMaxTileRows cells after it, mark the tiles whose first cell is in
+// the block (non-empty: one of their cells holds a particle); the first pass
+// counts them per block, the second lists them in tile order.  The counts are
+// cleared later, by k_order, so both passes read them intact.
 struct TileScan {
-  Consts k0;
+  int ncy, ncells, cell_base;  // the single case (cases == nullptr)
   const Consts *cases;
   const int *tile_base;
   int ncases;
   int rows;
-  uint8_t *flag;  // per tile
   int *list;
 };
 
-// tile of cell c if c is a tile's first cell, else -1; n: the tile's cells
-__device__ __forceinline__ int tile_at(const TileScan &ts, int c, int &n) {
-  int cse = 0, cell_base = ts.k0.cell_base, ncy = ts.k0.ncy, ncells = ts.k0.ncells;
+// {tile, cells of the tile} if cell c is a tile's first cell, else {-1, 0}
+__device__ __forceinline__ int2 tile_at(const TileScan ts, int c) {
+  int cse = 0, cell_base = ts.cell_base, ncy = ts.ncy, ncells = ts.ncells;
   if (ts.cases) {
     int lo = 0, hi = ts.ncases;
     while (hi - lo > 1) {
@@ -151,11 +152,38 @@ __device__ __forceinline__ int tile_at(const TileScan &ts, int c, int &n) {
     ncells = ts.cases[cse].ncells;
   }
   const int lc = c - cell_base;
-  if (lc < 0 || lc >= ncells) return -1;
+  if (lc < 0 || lc >= ncells) return make_int2(-1, 0);
   const int cx = lc / ncy, cy = lc - cx * ncy;
-  if (cy % ts.rows) return -1;
-  n = min(ts.rows, ncy - cy);
-  return (ts.cases ? ts.tile_base[cse] : 0) + cx * ((ncy + ts.rows - 1) / ts.rows) + cy / ts.rows;
+  if (cy % ts.rows) return make_int2(-1, 0);
+  const int t = (ts.cases ? ts.tile_base[cse] : 0) + cx * ((ncy + ts.rows - 1) / ts.rows) + cy / ts.rows;
+  return make_int2(t, min(ts.rows, ncy - cy));
+}
+
+// this thread's counts (v) and the block's counts + halo staged in s_cnt;
+// returns the thread's tiles (tl[j] >= 0: cell base + j starts a non-empty tile)
+__device__ __forceinline__ int stage_counts(const int *__restrict__ in, int len, const TileScan ts,
+                                            int (&v)[kScanItems], int (&tl)[kScanItems], int *s_cnt) {
+  const int b0 = blockIdx.x * kScanTile, base = b0 + threadIdx.x * kScanItems;
+#pragma unroll
+  for (int j = 0; j < kScanItems; ++j) {
+    v[j] = (base + j < len) ? in[base + j] : 0;
+    s_cnt[threadIdx.x * kScanItems + j] = v[j];
+  }
+  if (threadIdx.x < kMaxTileRows) {
+    const int c = b0 + kScanTile + threadIdx.x;
+    s_cnt[kScanTile + threadIdx.x] = c < len ? in[c] : 0;
+  }
+  __syncthreads();
+  int nt = 0;
+#pragma unroll
+  for (int j = 0; j < kScanItems; ++j) {
+    const int2 tn = base + j < len ? tile_at(ts, base + j) : make_int2(-1, 0);
+    int any = 0;
+    for (int r = 0; r < tn.y; ++r) any |= s_cnt[threadIdx.x * kScanItems + j + r];
+    tl[j] = any ? tn.x : -1;
+    nt += any != 0;
+  }
+  return nt;
 }
 
 // sums of the first pass over the blocks before this one (pre) and over all
@@ -163,13 +191,13 @@ __device__ __forceinline__ void block_offsets(const int *__restrict__ partial, i
                                               int &all_c, int &all_t, int *lds) {
   int v[4] = {0, 0, 0, 0};
   for (int b = threadIdx.x; b < (int)gridDim.x; b += kScanBlock) {
-    const int pc = partial[2 * b], pt = partial[2 * b + 1];
+    const int2 pb = reinterpret_cast<const int2 *>(partial)[b];
     if (b < (int)blockIdx.x) {
-      v[0] += pc;
-      v[1] += pt;
+      v[0] += pb.x;
+      v[1] += pb.y;
     }
-    v[2] += pc;
-    v[3] += pt;
+    v[2] += pb.x;
+    v[3] += pb.y;
   }
 #pragma unroll
   for (int q = 0; q < 4; ++q)
@@ -191,21 +219,12 @@ __device__ __forceinline__ void block_offsets(const int *__restrict__ partial, i
 __global__ __launch_bounds__(kScanBlock) void k_scan_partials(const int *__restrict__ in, int len,
                                                               int *__restrict__ partial, TileScan ts) {
   __shared__ int lds[kScanBlock / 64];
-  const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
-  int s = 0, nt = 0;
+  __shared__ int s_cnt[kScanTile + kMaxTileRows];
+  int v[kScanItems], tl[kScanItems];
+  const int nt = stage_counts(in, len, ts, v, tl, s_cnt);
+  int s = 0;
 #pragma unroll
-  for (int j = 0; j < kScanItems; ++j) {
-    if (base + j >= len) continue;
-    s += in[base + j];
-    int n = 0;
-    const int t = tile_at(ts, base + j, n);
-    if (t >= 0) {
-      int any = 0;
-      for (int r = 0; r < n; ++r) any |= in[base + j + r];
-      ts.flag[t] = any != 0;
-      nt += any != 0;
-    }
-  }
+  for (int j = 0; j < kScanItems; ++j) s += v[j];
   int total, tiles;
   block_exclusive_scan<kScanBlock>(s, lds, total);
   block_exclusive_scan<kScanBlock>(nt, lds, tiles);
@@ -216,37 +235,28 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_partials(const int *__restr
 }
 
 // second pass: cell offsets and the tile list (each block adds up the first
-// pass's sums of the blocks before it); clears the counts it consumed for the
-// next build; block 0 writes out[len] (the total), the tile count and clears
-// the wall-list counter the reorder rebuilds
-__global__ __launch_bounds__(kScanBlock) void k_scan_final(int *__restrict__ in, int len,
+// pass's sums of the blocks before it); block 0 writes out[len] (the total),
+// the tile count and clears the wall-list counter the reorder rebuilds
+__global__ __launch_bounds__(kScanBlock) void k_scan_final(const int *__restrict__ in, int len,
                                                            const int *__restrict__ partial,
                                                            int *__restrict__ out,
                                                            int *__restrict__ scal, TileScan ts) {
   __shared__ int lds[4 * kScanBlock / 64];
+  __shared__ int s_cnt[kScanTile + kMaxTileRows];
   int pre_c, pre_t, all_c, all_t;
   block_offsets(partial, pre_c, pre_t, all_c, all_t, lds);
-  const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
   int v[kScanItems], tl[kScanItems];
-  int s = 0, nt = 0;
+  const int nt = stage_counts(in, len, ts, v, tl, s_cnt);
+  const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+  int s = 0;
 #pragma unroll
-  for (int j = 0; j < kScanItems; ++j) {
-    v[j] = (base + j < len) ? in[base + j] : 0;
-    s += v[j];
-    int n = 0;
-    tl[j] = base + j < len ? tile_at(ts, base + j, n) : -1;
-    if (tl[j] >= 0 && !ts.flag[tl[j]]) tl[j] = -1;
-    nt += tl[j] >= 0;
-  }
+  for (int j = 0; j < kScanItems; ++j) s += v[j];
   int total, tiles;
   int ex = block_exclusive_scan<kScanBlock>(s, lds, total) + pre_c;
   int et = block_exclusive_scan<kScanBlock>(nt, lds, tiles) + pre_t;
 #pragma unroll
   for (int j = 0; j < kScanItems; ++j) {
-    if (base + j < len) {
-      out[base + j] = ex;
-      in[base + j] = 0;
-    }
+    if (base + j < len) out[base + j] = ex;
     ex += v[j];
     if (tl[j] >= 0) ts.list[et++] = tl[j];
   }
@@ -272,10 +282,13 @@ __global__ void k_scatter(const int *__restrict__ scal, const int *__restrict__ 
 // Orders each cell's slots by reference order (seq).  Cells of up to 16
 // particles are sorted in registers (all loads in flight together, then an
 // odd-even transposition network); larger cells use an insertion sort.
+// Also clears the per-cell counts (ncells + 1 of them) for the next build.
 __global__ void k_order(const int *__restrict__ cell_start, int ncells,
-                        const uint32_t *__restrict__ seq, int *__restrict__ perm) {
+                        const uint32_t *__restrict__ seq, int *__restrict__ perm,
+                        int *__restrict__ cell_count) {
   constexpr int R = 16;
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c <= ncells) cell_count[c] = 0;
   if (c >= ncells) return;
   const int s = cell_start[c], e = cell_start[c + 1];
   const int cnt = e - s;
@@ -428,7 +441,8 @@ __global__ void k_reorder(const int *__restrict__ cell_start, int ncells, const 
 int scan_cells(tsph_ctx *c, hipStream_t s) {
   const int len = c->ncells_tot + 1;
   const int nb = (len + kScanTile - 1) / kScanTile;
-  const TileScan ts{c->k, c->cases, c->tile_base, c->ncases, c->tile_rows, c->tile_flag, c->tile_list};
+  const TileScan ts{c->k.ncy, c->k.ncells, c->k.cell_base, c->cases, c->tile_base, c->ncases,
+                    c->tile_rows, c->tile_list};
   hipLaunchKernelGGL(k_scan_partials, dim3(nb), dim3(kScanBlock), 0, s, c->cell_count, len,
                      c->scan_tmp, ts);
   hipLaunchKernelGGL(k_scan_final, dim3(nb), dim3(kScanBlock), 0, s, c->cell_count, len,
@@ -456,8 +470,8 @@ int launch_cell_list(tsph_ctx *c, bool eos, bool predict, double dt) {
   if (int e = scan_cells(c, s)) return e;
   hipLaunchKernelGGL(k_scatter, dim3(g), dim3(nb), 0, s, c->scal, c->hcell, c->hrank,
                      c->cell_start, c->perm, c->cellof);
-  hipLaunchKernelGGL(k_order, dim3(grid_for(nct, nb)), dim3(nb), 0, s, c->cell_start, nct,
-                     c->cur.seq, c->perm);
+  hipLaunchKernelGGL(k_order, dim3(grid_for(nct + 1, nb)), dim3(nb), 0, s, c->cell_start, nct,
+                     c->cur.seq, c->perm, c->cell_count);
   const Mixed mx = c->prm.precision == TSPH_PREC_MIXED ? c->mx : Mixed{};
 #define TSPH_REORDER(E, P)                                                                    \
   hipLaunchKernelGGL((k_reorder<E, P>), dim3(g), dim3(nb), 0, s, c->cell_start, nct, c->perm,      \

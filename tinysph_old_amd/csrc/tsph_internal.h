@@ -25,6 +25,8 @@ namespace tsph {
 
 constexpr uint8_t kRemoved = 255;
 
+constexpr int kMaxTileRows = 24;  // upper bound of tile_rows
+
 // device scalar slots
 enum { S_NP = 0, S_MAXSEQ = 1, S_NTILES = 2, S_NWALL = 3, S_BZCNT = 4, S_NSCAL = 32 };
 // device counter slots (uint64)
@@ -142,7 +144,6 @@ struct tsph_ctx {
   bool tiles_valid = false;  // tile_list matches the cell list and tile_rows
   int *list_part = nullptr;  // per list-build block: tiles, work
   int list_part_cap = 0;     // list-build blocks list_part holds
-  uint8_t *tile_flag = nullptr; // per tile: non-empty (the cell-list scan's marks)
   int *tile_bound = nullptr; // pooled walk: first list entry of each workgroup's chunk
   int wave_ctr_cap = 0;      // wavefronts with count slots in `counters`
   int bound_cap = 0;         // chunks tile_bound holds

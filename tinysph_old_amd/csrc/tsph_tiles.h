@@ -11,7 +11,6 @@ namespace tsph {
 namespace tiles {
 
 constexpr int kWave = 64;          // one wavefront per tile
-constexpr int kMaxTileRows = 24;   // upper bound of tile_rows (LDS offset table)
 constexpr int kCellCap = 11;       // LDS records budgeted per cell of the slices
 constexpr int kList = 32;          // per-lane neighbour list entries in LDS
 constexpr int kGroup = 8;          // candidates tested together in the mixed pair loop
