@@ -359,19 +359,20 @@ int tsph_batch_step(tsph_ctx *ctx, int wall, int nsteps);
 
 /* ---- autotuner ----
  * Times the pair pass `variant` on the current state for each tile walk
- * (balanced or static, see tsph_set_balanced), tile depth (cells per tile: 5,
+ * (pooled or static, see tsph_set_balanced), tile depth (cells per tile: 5,
  * 6, 7, 8, 10, 12) and persistent-wave count (waves per CU: 4, 8, 12, 16),
- * three passes each, and keeps the fastest; the workgroup is one wavefront by
- * design and each lane owns one particle of its tile.  Needs a cell list;
- * leaves a, drho as one pair pass with the chosen setting leaves them.
- * Outputs may be NULL.  No counterpart in the reference. */
+ * three passes each, and keeps the fastest; each wavefront owns one tile and
+ * each lane one particle of it by design.  Needs a cell list; leaves a, drho
+ * as one pair pass with the chosen setting leaves them.  Outputs may be
+ * NULL.  No counterpart in the reference. */
 int tsph_autotune(tsph_ctx *ctx, int variant, int *tile_rows, int *waves_per_cu, int *balanced,
                   double *best_ms);
-/* Tile walk of the pair loops for the context's precision: 1 = balanced (the
- * waves of each XCD take tiles from an atomic ticket counter and help other
- * XCDs' tiles once theirs are done), 0 = static striding.  Grids with fewer
- * than 8 tiles per persistent wave always stride.  Results do not depend on
- * it.  Defaults: balanced for FP64, static for mixed. */
+/* Tile walk of the pair loops for the context's precision: 1 = pooled
+ * (workgroups of up to 4 wavefronts, each with a chunk of the tile list of
+ * about equal work whose tiles its wavefronts take from a counter in LDS),
+ * 0 = static striding, one wavefront per workgroup (the default).  Grids with
+ * fewer than 8 tiles per persistent wave always stride.  Results do not depend
+ * on it.  No counterpart in the reference. */
 int tsph_set_balanced(tsph_ctx *ctx, int balanced);
 
 /* ---- case setup (host only; no GPU needed) ----
