@@ -68,6 +68,17 @@ static void drop_graph(tsph_ctx *c) {
   }
 }
 
+// default pair-loop launch: 7-cell tiles; the mixed path's shorter tiles hide
+// more latency with 12 waves per CU in the pooled walk (C2: 0.082 ms against
+// 0.088 ms for the static walk at 8), the FP64 path's larger LDS records allow
+// 8 waves per CU, where the static walk is the faster
+static void set_walk_defaults(tsph_ctx *c) {
+  const bool mixed = c->prm.precision == TSPH_PREC_MIXED;
+  c->waves_per_cu = mixed ? 12 : 8;
+  c->balanced[0] = false;
+  c->balanced[1] = mixed;
+}
+
 static HostState host_state(const tsph_ctx *c) {
   return {c->cells_valid, c->tiles_valid, c->records_valid, c->marks_valid, c->list_groups};
 }
@@ -299,7 +310,7 @@ static int alloc_context(tsph_ctx *c) {
   if ((e = dalloc(&c->scal, S_NSCAL)) ||
       (e = dalloc(&c->cell_count, nct + 1)) || (e = dalloc(&c->cell_start, nct + 2)) ||
       (e = dalloc(&c->cellof, cap)) || (e = dalloc(&c->hrank, cap)) || (e = dalloc(&c->hcell, cap)) ||
-      (e = dalloc(&c->perm, cap)) || (e = dalloc(&c->scan_tmp, 2 * ((nct + 1) / 1024 + 2))) ||
+      (e = dalloc(&c->perm, cap)) || (e = dalloc(&c->scan_tmp, 3 * ((nct + 1) / 1024 + 2))) ||
       (e = dalloc(&c->seqstate, (size_t)c->seq_cap)) || (e = dalloc(&c->probe_out, 4)) ||
       (e = dalloc(&c->bz_list, 2 * kBzMax)) ||
       (e = dalloc(&c->tile_list, nct + 8)) || (e = dalloc(&c->wall_list, cap)))
@@ -369,6 +380,7 @@ int tsph_create(const tsph_params *prm, tsph_ctx **out) {
   c->prm = *prm;
   c->capacity = prm->capacity;
   c->tile_rows = 7;
+  set_walk_defaults(c);
   c->k = k;
   c->ncells_tot = k.ncells;
   if (int e = alloc_context(c)) {
@@ -1347,6 +1359,7 @@ int tsph_batch_create(const tsph_params *prm, const double *dt, int ncases, tsph
   c->prm.capacity = (int)cap;
   c->capacity = (int)cap;
   c->tile_rows = 7;
+  set_walk_defaults(c);
   c->k = ks[0];
   c->ncells_tot = (int)cells;
   c->ncases = ncases;

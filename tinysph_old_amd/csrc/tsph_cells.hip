@@ -134,6 +134,8 @@ struct TileScan {
   int ncases;
   int rows;
   int *list;
+  int groups;  // > 0: also cut the list into chunks for a pooled walk of this many workgroups
+  int *bound;  // chunk starts (groups + 1)
 };
 
 // {tile, cells of the tile} if cell c is a tile's first cell, else {-1, 0}
@@ -160,9 +162,12 @@ __device__ __forceinline__ int2 tile_at(const TileScan ts, int c) {
 }
 
 // this thread's counts (v) and the block's counts + halo staged in s_cnt;
-// returns the thread's tiles (tl[j] >= 0: cell base + j starts a non-empty tile)
+// tl[j] >= 0: cell base + j starts a non-empty tile of work wk[j] (its own
+// particles + kTileWork0, the pooled walk's cost model); returns the number
+// of such tiles and their work (w)
 __device__ __forceinline__ int stage_counts(const int *__restrict__ in, int len, const TileScan ts,
-                                            int (&v)[kScanItems], int (&tl)[kScanItems], int *s_cnt) {
+                                            int (&v)[kScanItems], int (&tl)[kScanItems],
+                                            int (&wk)[kScanItems], int &w, int *s_cnt) {
   const int b0 = blockIdx.x * kScanTile, base = b0 + threadIdx.x * kScanItems;
 #pragma unroll
   for (int j = 0; j < kScanItems; ++j) {
@@ -175,44 +180,49 @@ __device__ __forceinline__ int stage_counts(const int *__restrict__ in, int len,
   }
   __syncthreads();
   int nt = 0;
+  w = 0;
 #pragma unroll
   for (int j = 0; j < kScanItems; ++j) {
     const int2 tn = base + j < len ? tile_at(ts, base + j) : make_int2(-1, 0);
-    int any = 0;
-    for (int r = 0; r < tn.y; ++r) any |= s_cnt[threadIdx.x * kScanItems + j + r];
-    tl[j] = any ? tn.x : -1;
-    nt += any != 0;
+    int own = 0;
+    for (int r = 0; r < tn.y; ++r) own += s_cnt[threadIdx.x * kScanItems + j + r];
+    tl[j] = own ? tn.x : -1;
+    wk[j] = own ? own + kTileWork0 : 0;
+    nt += own != 0;
+    w += wk[j];
   }
   return nt;
 }
 
 // sums of the first pass over the blocks before this one (pre) and over all
-__device__ __forceinline__ void block_offsets(const int *__restrict__ partial, int &pre_c, int &pre_t,
-                                              int &all_c, int &all_t, int *lds) {
-  int v[4] = {0, 0, 0, 0};
+// blocks (all): cells, tiles, tile work
+__device__ __forceinline__ void block_offsets(const int *__restrict__ partial, int (&pre)[3],
+                                              int (&all)[3], int *lds) {
+  int v[6] = {0, 0, 0, 0, 0, 0};
   for (int b = threadIdx.x; b < (int)gridDim.x; b += kScanBlock) {
-    const int2 pb = reinterpret_cast<const int2 *>(partial)[b];
+    const int pc = partial[3 * b], pt = partial[3 * b + 1], pw = partial[3 * b + 2];
     if (b < (int)blockIdx.x) {
-      v[0] += pb.x;
-      v[1] += pb.y;
+      v[0] += pc;
+      v[1] += pt;
+      v[2] += pw;
     }
-    v[2] += pb.x;
-    v[3] += pb.y;
+    v[3] += pc;
+    v[4] += pt;
+    v[5] += pw;
   }
 #pragma unroll
-  for (int q = 0; q < 4; ++q)
+  for (int q = 0; q < 6; ++q)
     for (int o = 32; o > 0; o >>= 1) v[q] += __shfl_xor(v[q], o, 64);
   const int wid = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0)
-    for (int q = 0; q < 4; ++q) lds[4 * wid + q] = v[q];
+    for (int q = 0; q < 6; ++q) lds[6 * wid + q] = v[q];
   __syncthreads();
-  pre_c = pre_t = all_c = all_t = 0;
-  for (int w = 0; w < kScanBlock / 64; ++w) {
-    pre_c += lds[4 * w];
-    pre_t += lds[4 * w + 1];
-    all_c += lds[4 * w + 2];
-    all_t += lds[4 * w + 3];
-  }
+  for (int q = 0; q < 3; ++q) pre[q] = all[q] = 0;
+  for (int w = 0; w < kScanBlock / 64; ++w)
+    for (int q = 0; q < 3; ++q) {
+      pre[q] += lds[6 * w + q];
+      all[q] += lds[6 * w + 3 + q];
+    }
   __syncthreads();
 }
 
@@ -220,50 +230,72 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_partials(const int *__restr
                                                               int *__restrict__ partial, TileScan ts) {
   __shared__ int lds[kScanBlock / 64];
   __shared__ int s_cnt[kScanTile + kMaxTileRows];
-  int v[kScanItems], tl[kScanItems];
-  const int nt = stage_counts(in, len, ts, v, tl, s_cnt);
+  int v[kScanItems], tl[kScanItems], wk[kScanItems], w;
+  const int nt = stage_counts(in, len, ts, v, tl, wk, w, s_cnt);
   int s = 0;
 #pragma unroll
   for (int j = 0; j < kScanItems; ++j) s += v[j];
-  int total, tiles;
+  int total, tiles, work;
   block_exclusive_scan<kScanBlock>(s, lds, total);
   block_exclusive_scan<kScanBlock>(nt, lds, tiles);
+  block_exclusive_scan<kScanBlock>(w, lds, work);
   if (threadIdx.x == 0) {
-    partial[2 * blockIdx.x] = total;
-    partial[2 * blockIdx.x + 1] = tiles;
+    partial[3 * blockIdx.x] = total;
+    partial[3 * blockIdx.x + 1] = tiles;
+    partial[3 * blockIdx.x + 2] = work;
   }
 }
 
 // second pass: cell offsets and the tile list (each block adds up the first
-// pass's sums of the blocks before it); block 0 writes out[len] (the total),
-// the tile count and clears the wall-list counter the reorder rebuilds
+// pass's sums of the blocks before it), and for a pooled walk the chunk
+// starts: chunk g starts at the tile whose work interval [W0, W1) holds
+// floor(g W / groups) (W: total work).  Block 0 writes out[len] (the total),
+// the tile count, the last chunk bound, and clears the wall-list counter the
+// reorder rebuilds.
 __global__ __launch_bounds__(kScanBlock) void k_scan_final(const int *__restrict__ in, int len,
                                                            const int *__restrict__ partial,
                                                            int *__restrict__ out,
                                                            int *__restrict__ scal, TileScan ts) {
-  __shared__ int lds[4 * kScanBlock / 64];
+  __shared__ int lds[6 * kScanBlock / 64];
   __shared__ int s_cnt[kScanTile + kMaxTileRows];
-  int pre_c, pre_t, all_c, all_t;
-  block_offsets(partial, pre_c, pre_t, all_c, all_t, lds);
-  int v[kScanItems], tl[kScanItems];
-  const int nt = stage_counts(in, len, ts, v, tl, s_cnt);
+  int pre[3], all[3];
+  block_offsets(partial, pre, all, lds);
+  int v[kScanItems], tl[kScanItems], wk[kScanItems], w;
+  const int nt = stage_counts(in, len, ts, v, tl, wk, w, s_cnt);
   const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
   int s = 0;
 #pragma unroll
   for (int j = 0; j < kScanItems; ++j) s += v[j];
-  int total, tiles;
-  int ex = block_exclusive_scan<kScanBlock>(s, lds, total) + pre_c;
-  int et = block_exclusive_scan<kScanBlock>(nt, lds, tiles) + pre_t;
+  int total, tiles, work;
+  int ex = block_exclusive_scan<kScanBlock>(s, lds, total) + pre[0];
+  int et = block_exclusive_scan<kScanBlock>(nt, lds, tiles) + pre[1];
+  long long ew = block_exclusive_scan<kScanBlock>(w, lds, work) + pre[2];
 #pragma unroll
   for (int j = 0; j < kScanItems; ++j) {
     if (base + j < len) out[base + j] = ex;
     ex += v[j];
-    if (tl[j] >= 0) ts.list[et++] = tl[j];
+    if (tl[j] >= 0) {
+      if (ts.groups > 0) {
+        const long long g0 = (ew * ts.groups + all[2] - 1) / all[2];
+        const long long g1 = ((ew + wk[j]) * ts.groups + all[2] - 1) / all[2];
+        for (long long g = g0; g < g1 && g < ts.groups; ++g) ts.bound[g] = et;
+      }
+      ts.list[et++] = tl[j];
+      ew += wk[j];
+    }
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    out[len] = all_c;
-    scal[S_NTILES] = all_t;
-    scal[S_NWALL] = 0;
+  if (blockIdx.x == 0) {
+    if (threadIdx.x == 0) {
+      out[len] = all[0];
+      scal[S_NTILES] = all[1];
+      scal[S_NWALL] = 0;
+    }
+    if (ts.groups > 0) {
+      if (all[1] == 0)
+        for (int g = threadIdx.x; g <= ts.groups; g += kScanBlock) ts.bound[g] = 0;
+      else if (threadIdx.x == 0)
+        ts.bound[ts.groups] = all[1];
+    }
   }
 }
 
@@ -441,13 +473,15 @@ __global__ void k_reorder(const int *__restrict__ cell_start, int ncells, const 
 int scan_cells(tsph_ctx *c, hipStream_t s) {
   const int len = c->ncells_tot + 1;
   const int nb = (len + kScanTile - 1) / kScanTile;
+  const int groups = pooled_groups(c);
   const TileScan ts{c->k.ncy, c->k.ncells, c->k.cell_base, c->cases, c->tile_base, c->ncases,
-                    c->tile_rows, c->tile_list};
+                    c->tile_rows, c->tile_list, groups, c->tile_bound};
   hipLaunchKernelGGL(k_scan_partials, dim3(nb), dim3(kScanBlock), 0, s, c->cell_count, len,
                      c->scan_tmp, ts);
   hipLaunchKernelGGL(k_scan_final, dim3(nb), dim3(kScanBlock), 0, s, c->cell_count, len,
                      c->scan_tmp, c->cell_start, c->scal, ts);
   TSPH_HIP(hipGetLastError());
+  c->list_groups = groups;
   return 0;
 }
 
@@ -486,8 +520,7 @@ int launch_cell_list(tsph_ctx *c, bool eos, bool predict, double dt) {
   std::swap(c->cur, c->alt);
   c->parity ^= 1;
   c->cells_valid = true;
-  c->tiles_valid = true;  // listed by the scan (no pooled-walk chunks)
-  c->list_groups = 0;
+  c->tiles_valid = true;  // listed (and cut for the pooled walk) by the scan
   c->records_valid = eos && c->prm.precision == TSPH_PREC_MIXED;
   if (predict) c->marks_valid = false;
   return 0;

@@ -26,6 +26,7 @@ namespace tsph {
 constexpr uint8_t kRemoved = 255;
 
 constexpr int kMaxTileRows = 24;  // upper bound of tile_rows
+constexpr int kTileWork0 = 16;    // pooled walk's cost model: per-tile work in own particles
 
 // device scalar slots
 enum { S_NP = 0, S_MAXSEQ = 1, S_NTILES = 2, S_NWALL = 3, S_BZCNT = 4, S_NSCAL = 32 };
@@ -137,8 +138,7 @@ struct tsph_ctx {
   int max_lds_block = 65536;  // LDS bytes a workgroup may allocate
   // pair-loop tile walk per precision (FP64, mixed): pooled (work-balanced
   // chunks per workgroup, tiles taken from an LDS counter by the workgroup's
-  // waves) or the plain static walk (the default: on the dam-break workloads
-  // the static walk at 8 waves per CU is as fast, and the autotuner tries both)
+  // waves) or the plain static walk; defaults set at creation
   bool balanced[2] = {false, false};
   int *tile_list = nullptr;  // non-empty tiles of the current cell list, in tile order
   bool tiles_valid = false;  // tile_list matches the cell list and tile_rows
@@ -244,6 +244,9 @@ int ensure_tile_list(tsph_ctx *c, int groups);
 // geometry of a tile-kernel launch at the context's waves per CU (pooled when
 // walk_balanced holds for the precision)
 TileGrid tile_grid(tsph_ctx *c, bool fp32, size_t lds_wave);
+// workgroups of the pooled walk that launch_forces would take now (0: static),
+// for the cell-list build to cut the tile list for
+int pooled_groups(tsph_ctx *c);
 int launch_forces(tsph_ctx *c, int variant);
 int launch_mixed_records(tsph_ctx *c);
 // tsph_stages.hip

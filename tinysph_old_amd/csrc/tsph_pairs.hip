@@ -296,7 +296,6 @@ __device__ void particle32_global(const Soa &d, const Mixed &mx, const int *__re
 // `groups` chunks of about equal work: chunk g starts at the entry whose work
 // interval [W0, W1) holds floor(g W / groups) (W: total work).
 constexpr int kListBlock = 256;
-constexpr int kTileWork0 = 16;  // per-tile work (slice staging, set-up) in own particles
 
 __device__ __forceinline__ int tile_own(const int *__restrict__ cell_start, const Consts &k0,
                                         const Consts *__restrict__ cases,
@@ -592,6 +591,18 @@ __global__ __launch_bounds__(kWave * kPoolWaves) void k_tile32(Soa d, Mixed mx, 
   const C32 cc = make_c32(k);
   Tile t;
   tile_begin(k, cs, tile, rows, tab, t);
+  // LDS layout with kGroup sentinels after each column slice's records: a
+  // candidate group may run past the end of its slice (the sentinels lie far
+  // away and fail every distance test) and need not be masked
+  int se[3];  // end of slice q's records
+#pragma unroll
+  for (int q = 0; q < 3; ++q) se[q] = t.lq[q + 1] + q * kGroup;
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    t.lq[q] += q * kGroup;
+    t.base[q] += q * kGroup;
+  }
+  t.lq[3] += 3 * kGroup;
   if (t.lq[3] > cap) {
     const int next = tq_take<DYN>(tq, ticket);
     for (int i = t.own0 + tile_lane(); i < t.own1; i += kWave) {
@@ -616,7 +627,9 @@ __global__ __launch_bounds__(kWave * kPoolWaves) void k_tile32(Soa d, Mixed mx, 
 #pragma unroll
     for (int u = 0; u < kStageU; ++u) {
       const int l = min(l0 + u * kWave + tile_lane(), t.lq[3] - 1);
-      const int j = l - sel3(t.base, slice_of(t, l));
+      const int q = slice_of(t, l);
+      // sentinel slots load the tile's first own particle (never used)
+      const int j = l < sel3(se, q) ? l - sel3(t.base, q) : t.own0;
       rv[u] = mx.rv[j];
       rp[u] = mx.rp[j];
       cl[u] = cellof[j];
@@ -628,7 +641,10 @@ __global__ __launch_bounds__(kWave * kPoolWaves) void k_tile32(Soa d, Mixed mx, 
       const int l = l0 + u * kWave + tile_lane();
       const int q = slice_of(t, min(l, t.lq[3] - 1));
       const int row = cl[u] - k.cell_base - (t.cx + q - 1) * k.ncy;
-      const float2 pos = make_float2(rv[u].x + (float)(q - 1) * cc.kh, rv[u].y + (float)(row - mid) * cc.kh);
+      const bool real = l < sel3(se, q);
+      const float2 pos = real ? make_float2(rv[u].x + (float)(q - 1) * cc.kh,
+                                            rv[u].y + (float)(row - mid) * cc.kh)
+                              : make_float2(kFar32, kFar32);
       const float ir = __builtin_amdgcn_rcpf(cc.rho0 + rp[u].x);
       if (l < t.lq[3]) {
         s_pos[l] = pos;
@@ -673,19 +689,21 @@ __global__ __launch_bounds__(kWave * kPoolWaves) void k_tile32(Soa d, Mixed mx, 
     // candidate c of the flattened range lives at LDS slot c + off(c)
     const int n01 = n0 + n1, off1 = a1 - n0, off2 = a2 - n01;
     const bool dbc_wall = VAR == TSPH_FORCES_DBC && at == TSPH_WALL;
+    const unsigned c2m1 = __float_as_uint(cc.r2cut) - 1u;
     int cur = 0;
     Acc32 s;
     while (__any(cur < tot)) {
       int cnt = 0;
-      // Candidates in groups of up to kGroup from one column slice (so one slot
+      // Candidates in groups of kGroup from one column slice (so one slot
       // offset serves the group), while the list has room for a group: their
       // LDS loads are issued together, and the group is appended to the list
       // without branches (every slot is written, the count advances by the
-      // test).  Slots past the group's end are read but masked out; they lie
-      // within the LDS allocation (the staged positions are followed by the
-      // other staged arrays).  The particle itself is the candidate at
-      // distance exactly 0; "0 < r^2" drops it (and only a coincident particle
-      // with it, whose reference kernel gradient would be 0/0).
+      // test).  Slots past the lane's range in the slice are rows at least
+      // one cell (> 2h) away, or the slice's far sentinels, and fail the test.
+      // The particle itself is the candidate at distance exactly 0; "0 < r^2"
+      // drops it (and only a coincident particle with it, whose reference
+      // kernel gradient would be 0/0).  0 < r^2 < (2h)^2 is one unsigned
+      // compare of the bit patterns (r^2 >= 0, so they order as the values).
       while (cur < tot && cnt <= kList - kGroup) {
         const bool in0 = cur < n0, in01 = cur < n01;
         const int end = in0 ? n0 : (in01 ? n01 : tot);
@@ -699,7 +717,7 @@ __global__ __launch_bounds__(kWave * kPoolWaves) void k_tile32(Soa d, Mixed mx, 
         for (int u = 0; u < kGroup; ++u) {
           const float x = pi.x - pp[u].x, y = pi.y - pp[u].y;
           const float q = x * x + y * y;
-          kk[u] = (u < m) & (q > 0.f) & (q < cc.r2cut);
+          kk[u] = __float_as_uint(q) - 1u < c2m1;
         }
         if (VAR == TSPH_FORCES_DBC) {
 #pragma unroll
@@ -812,6 +830,23 @@ TileGrid tile_grid(tsph_ctx *c, bool fp32, size_t lds_wave) {
   return tg;
 }
 
+// LDS bytes per wavefront of the pair pass of `variant` at the current tile rows
+static size_t forces_lds_wave(const tsph_ctx *c, int variant) {
+  const int cap = 3 * (c->tile_rows + 2) * kCellCap;
+  const size_t list = sizeof(uint16_t) * kList * kWave;
+  const bool bt = variant == TSPH_FORCES_BT;
+  if (c->prm.precision == TSPH_PREC_MIXED)
+    return kTabBytes + list +
+           (size_t)cap * (3 * sizeof(float2) + sizeof(float) + 2 + (bt ? sizeof(float2) : 0));
+  return kTabBytes + list + (size_t)cap * (3 * sizeof(double2) + 2 + (bt ? sizeof(double2) : 0));
+}
+
+int pooled_groups(tsph_ctx *c) {
+  const TileGrid tg = tile_grid(c, c->prm.precision == TSPH_PREC_MIXED,
+                                forces_lds_wave(c, TSPH_FORCES_WCSPH));
+  return tg.pooled ? tg.groups : 0;
+}
+
 namespace {
 
 // one pair pass: the kernel for the variant, the precision and the tile walk
@@ -840,21 +875,12 @@ void pair_launch(tsph_ctx *c, const TileGrid &tg, int rows, int cap) {
 int launch_forces(tsph_ctx *c, int variant) {
   const int rows = c->tile_rows;
   const int cap = 3 * (rows + 2) * kCellCap;
-  const bool bt = variant == TSPH_FORCES_BT;
-  const size_t list = sizeof(uint16_t) * kList * kWave;
   const bool mixed = c->prm.precision == TSPH_PREC_MIXED;
-  size_t lds;
-  if (mixed) {
-    if (!c->records_valid) {
-      if (int e = launch_mixed_records(c)) return e;
-      c->records_valid = true;
-    }
-    lds = kTabBytes + list +
-          (size_t)cap * (3 * sizeof(float2) + sizeof(float) + 2 + (bt ? sizeof(float2) : 0));
-  } else {
-    lds = kTabBytes + list + (size_t)cap * (3 * sizeof(double2) + 2 + (bt ? sizeof(double2) : 0));
+  if (mixed && !c->records_valid) {
+    if (int e = launch_mixed_records(c)) return e;
+    c->records_valid = true;
   }
-  const TileGrid tg = tile_grid(c, mixed, lds);
+  const TileGrid tg = tile_grid(c, mixed, forces_lds_wave(c, variant));
   if (int e = ensure_tile_list(c, tg.pooled ? tg.groups : 0)) return e;
   return launch_timed(c, [&] {
     switch (variant) {

@@ -15,6 +15,7 @@ constexpr int kCellCap = 11;       // LDS records budgeted per cell of the slice
 constexpr int kList = 32;          // per-lane neighbour list entries in LDS
 constexpr int kGroup = 8;          // candidates tested together in the mixed pair loop
 constexpr int kStageU = 4;         // staged slots per lane whose loads are in flight together
+constexpr float kFar32 = 1e30f;    // coordinate of the mixed loop's sentinel slots
 
 // adds the wavefront's counts to its own slots (kWaveCtr0 + 4 w, w: the
 // wavefront's index in the launch); no two wavefronts of a launch share a slot
