@@ -201,14 +201,17 @@ static int record_step(tsph_ctx *c, int wall, double dt, int inlet = -1, int out
                       : wall == TSPH_WALL_MDBC ? TSPH_FORCES_WCSPH
                                                : TSPH_FORCES_BT;
   int e = 0;
+  const bool zones = inlet >= 0 || outlet >= 0;
   // wall density + EOS (cases/damBreakLong_mDBC/SPH_simulationSymplectic.h:65-66,
-  // cases/damBreakLong_BT/SPH_simulationSymplectic.h:72-73); DBC has none
-  if (wall == TSPH_WALL_MDBC) {
-    if ((e = launch_mdbc(c)) || (e = launch_eos(c))) return e;
-  } else if (wall == TSPH_WALL_BT) {
-    if ((e = launch_bt_density(c)) || (e = launch_eos(c))) return e;
+  // cases/damBreakLong_BT/SPH_simulationSymplectic.h:72-73); DBC has none.
+  // Without buffer zones nothing reads the pressures (or the mixed records)
+  // this EOS writes before the cell build rewrites them all, so only its
+  // out-of-range density count is taken.
+  if (wall == TSPH_WALL_MDBC || wall == TSPH_WALL_BT) {
+    if ((e = wall == TSPH_WALL_MDBC ? launch_mdbc(c) : launch_bt_density(c))) return e;
+    if ((e = zones ? launch_eos(c) : launch_eos_count(c))) return e;
   }
-  if (inlet >= 0 || outlet >= 0) {
+  if (zones) {
     // the buffer zones act on the predicted positions
     if ((e = launch_predictor(c, dt))) return e;
     if (inlet >= 0 && (e = launch_bz_inlet(c, inlet, dt))) return e;

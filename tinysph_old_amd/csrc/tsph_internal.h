@@ -251,6 +251,7 @@ int launch_forces(tsph_ctx *c, int variant);
 int launch_mixed_records(tsph_ctx *c);
 // tsph_stages.hip
 int launch_eos(tsph_ctx *c);
+int launch_eos_count(tsph_ctx *c);  // the EOS's out-of-range density count only
 int launch_predictor(tsph_ctx *c, double dt);
 int launch_corrector(tsph_ctx *c, double dt);
 int launch_mdbc(tsph_ctx *c);

@@ -48,6 +48,21 @@ __global__ void k_eos(Soa d, Mixed mx, const int *__restrict__ cellof, const int
   if (w && (threadIdx.x & 63) == 0) atomicAdd(&counters[C_BADRHO], (unsigned long long)__popcll(w));
 }
 
+// the EOS pass's out-of-range density count alone (its pressures and records
+// unused: the next cell build recomputes them)
+__global__ void k_eos_count(Soa d, const int *__restrict__ scal,
+                            unsigned long long *__restrict__ counters) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  bool bad = false;
+  if (i < scal[S_NP]) {
+    const int t = d.type[i];
+    const double rho = d.rho[i];
+    bad = t != TSPH_OUTLET && t != TSPH_INLET && (rho > 1500. || rho < 0.);
+  }
+  const unsigned long long w = __ballot(bad);
+  if (w && (threadIdx.x & 63) == 0) atomicAdd(&counters[C_BADRHO], (unsigned long long)__popcll(w));
+}
+
 __global__ void k_predictor(Soa d, const int *__restrict__ scal, double dt0,
                             const Consts *__restrict__ cases) {
   const int n = scal[S_NP];
@@ -108,8 +123,8 @@ __device__ __forceinline__ int point_cell(const Consts &k, double x, double y) {
 // adds sums q = u, u + kWG, ... over the round's candidates in candidate
 // order, so every sum is formed in the reference order.  A candidate that is
 // not a fluid neighbour within the support adds nothing, as in the reference.
-constexpr int kWG = 8;
-constexpr int kWPre = 10;
+constexpr int kWG = 16;
+constexpr int kWPre = 5;
 constexpr int kWallsPerWave = 64 / kWG;
 
 template <int NS, typename Term>
@@ -466,6 +481,13 @@ int launch_eos(tsph_ctx *c) {
   TSPH_HIP(hipGetLastError());
   // the records are only meaningful for the cell list they were built on
   c->records_valid = c->prm.precision == TSPH_PREC_MIXED && c->cells_valid;
+  return 0;
+}
+
+int launch_eos_count(tsph_ctx *c) {
+  hipLaunchKernelGGL(k_eos_count, dim3(grid_for(c->n_ub, 256)), dim3(256), 0, c->stream, c->cur,
+                     c->scal, c->counters);
+  TSPH_HIP(hipGetLastError());
   return 0;
 }
 
