@@ -202,6 +202,9 @@ static int record_step(tsph_ctx *c, int wall, double dt, int inlet = -1, int out
                                                : TSPH_FORCES_BT;
   int e = 0;
   const bool zones = inlet >= 0 || outlet >= 0;
+  // each cell build is followed by a pair pass that rewrites a, drho of its
+  // active particles, so the builds need not move those
+  const unsigned active = forces_active_types(variant);
   // wall density + EOS (cases/damBreakLong_mDBC/SPH_simulationSymplectic.h:65-66,
   // cases/damBreakLong_BT/SPH_simulationSymplectic.h:72-73); DBC has none.
   // Without buffer zones nothing reads the pressures (or the mixed records)
@@ -216,13 +219,13 @@ static int record_step(tsph_ctx *c, int wall, double dt, int inlet = -1, int out
     if ((e = launch_predictor(c, dt))) return e;
     if (inlet >= 0 && (e = launch_bz_inlet(c, inlet, dt))) return e;
     if (outlet >= 0 && (e = launch_bz_outlet(c, outlet, dt))) return e;
-    if ((e = launch_cell_list(c, true))) return e;
-  } else if ((e = launch_cell_list(c, true, true, dt))) {  // predictor applied by the build
+    if ((e = launch_cell_list(c, true, false, 0., active))) return e;
+  } else if ((e = launch_cell_list(c, true, true, dt, active))) {  // predictor applied by the build
     return e;
   }
   if ((e = launch_forces(c, variant))) return e;
   if ((e = launch_corrector(c, dt)) || (e = launch_remove(c))) return e;
-  if ((e = launch_cell_list(c, true)) || (e = launch_forces(c, variant))) return e;
+  if ((e = launch_cell_list(c, true, false, 0., active)) || (e = launch_forces(c, variant))) return e;
   return 0;
 }
 

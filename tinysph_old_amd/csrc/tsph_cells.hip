@@ -377,13 +377,15 @@ __global__ void k_order(const int *__restrict__ cell_start, int ncells,
 // written, for the step sequences in which that stage follows the cell build.
 // With PRED the symplectic predictor (src/SPH_integration.h:266-287) is
 // applied to the moved particles, for the step sequences in which it
-// immediately precedes the cell build.
+// immediately precedes the cell build.  ad_skip (a type mask): a and drho of
+// those types are not moved, for the step sequences in which a pair pass that
+// overwrites them for exactly those types follows the build.
 template <bool EOS, bool PRED>
 __global__ void k_reorder(const int *__restrict__ cell_start, int ncells, const int *__restrict__ perm,
                           Soa src, Soa dst, int *__restrict__ scal, int *__restrict__ wall_list,
                           const int *__restrict__ cellof, Consts k0,
                           const Consts *__restrict__ cases, Mixed mx, double dt0,
-                          unsigned long long *__restrict__ counters) {
+                          unsigned ad_skip, unsigned long long *__restrict__ counters) {
   const int n_new = cell_start[ncells];
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (blockIdx.x == 0 && threadIdx.x == 0) scal[S_NP] = n_new;
@@ -394,9 +396,14 @@ __global__ void k_reorder(const int *__restrict__ cell_start, int ncells, const 
     const uint8_t t = src.type[j];
     wall = t == TSPH_WALL;
     double2 r = src.r[j], v = src.v[j];
-    const double2 a = src.a[j];
+    const bool move_ad = !((ad_skip >> t) & 1u);
+    double2 a = make_double2(0., 0.);
+    double drho = 0.;
+    if (PRED || move_ad) {
+      a = src.a[j];
+      drho = src.drho[j];
+    }
     double rho = src.rho[j];
-    const double drho = src.drho[j];
     const Consts kc = EOS || PRED ? case_consts(k0, cases, src.cid, j) : k0;
     if (src.cid) dst.cid[k] = src.cid[j];
     if (PRED) {
@@ -412,9 +419,11 @@ __global__ void k_reorder(const int *__restrict__ cell_start, int ncells, const 
     }
     dst.r[k] = r;
     dst.v[k] = v;
-    dst.a[k] = a;
+    if (move_ad) {
+      dst.a[k] = a;
+      dst.drho[k] = drho;
+    }
     dst.rho[k] = rho;
-    dst.drho[k] = drho;
     dst.type[k] = t;
     dst.seq[k] = src.seq[j];
     dst.index[k] = src.index[j];
@@ -487,7 +496,7 @@ int scan_cells(tsph_ctx *c, hipStream_t s) {
 
 }  // namespace
 
-int launch_cell_list(tsph_ctx *c, bool eos, bool predict, double dt) {
+int launch_cell_list(tsph_ctx *c, bool eos, bool predict, double dt, unsigned ad_skip) {
   const Consts &k = c->k;
   const int nb = 256;
   const int g = grid_for(c->n_ub, nb);
@@ -510,7 +519,7 @@ int launch_cell_list(tsph_ctx *c, bool eos, bool predict, double dt) {
 #define TSPH_REORDER(E, P)                                                                    \
   hipLaunchKernelGGL((k_reorder<E, P>), dim3(g), dim3(nb), 0, s, c->cell_start, nct, c->perm,      \
                      c->cur, c->alt, c->scal, c->wall_list, c->cellof, k, c->cases, mx, dt,       \
-                     c->counters)
+                     ad_skip, c->counters)
   if (eos && predict) TSPH_REORDER(true, true);
   else if (eos) TSPH_REORDER(true, false);
   else if (predict) TSPH_REORDER(false, true);

@@ -235,8 +235,15 @@ struct TileGrid {
 // hash, count, scan, scatter, order, reorder; eos: the reorder also runs
 // Density_to_pressure (and writes the mixed records) for the moved particles
 // predict: the symplectic predictor (step dt; per case in a batch) is applied
-// on the way, in place of a launch_predictor immediately before
-int launch_cell_list(tsph_ctx *c, bool eos = false, bool predict = false, double dt = 0.);
+// on the way, in place of a launch_predictor immediately before.  ad_skip: a
+// mask of particle types (bit t: type t) whose a, drho are left behind, for a
+// build that a pair pass overwriting exactly those follows (forces_active_types)
+int launch_cell_list(tsph_ctx *c, bool eos = false, bool predict = false, double dt = 0.,
+                     unsigned ad_skip = 0);
+// types whose a, drho a pair pass of `variant` writes (its active particles)
+inline unsigned forces_active_types(int variant) {
+  return variant == TSPH_FORCES_DBC ? (1u << TSPH_FLUID) | (1u << TSPH_WALL) : 1u << TSPH_FLUID;
+}
 // tsph_pairs.hip
 // list of the non-empty pair-loop tiles; groups > 0: also its chunk bounds
 // for a pooled walk of that many workgroups
