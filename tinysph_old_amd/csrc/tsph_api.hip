@@ -224,8 +224,11 @@ static int record_step(tsph_ctx *c, int wall, double dt, int inlet = -1, int out
     return e;
   }
   if ((e = launch_forces(c, variant))) return e;
-  if ((e = launch_corrector(c, dt)) || (e = launch_remove(c))) return e;
-  if ((e = launch_cell_list(c, true, false, 0., active)) || (e = launch_forces(c, variant))) return e;
+  // the corrector writes the removal states; the build takes the removal
+  // decision (RemoveParticlesOutOfDomain) on the way
+  if ((e = launch_corrector(c, dt))) return e;
+  if ((e = launch_cell_list(c, true, false, 0., active, true)) || (e = launch_forces(c, variant)))
+    return e;
   return 0;
 }
 

@@ -40,10 +40,14 @@ __device__ __forceinline__ unsigned long long lanemask_lt() {
 }
 
 // PRED: the cell of the position the symplectic predictor gives (the
-// predictor itself is applied by the reorder that follows)
-template <bool PRED>
+// predictor itself is applied by the reorder that follows).  DECIDE: the
+// removal decision of RemoveParticlesOutOfDomain (k_rm_decide's) is taken here,
+// from the removal states of the corrector that precedes the build.  The
+// reference positions of removed particles are marked absent by k_scatter, so
+// no decision in this pass reads a state that this pass changes.
+template <bool PRED, bool DECIDE>
 __global__ void k_hash(const double2 *__restrict__ r, const double2 *__restrict__ v,
-                       const uint8_t *__restrict__ type,
+                       uint8_t *__restrict__ type,
                        const uint32_t *__restrict__ seq, const int *__restrict__ scal, Consts k0,
                        const Consts *__restrict__ cases, const uint8_t *__restrict__ cid, double dt0,
                        int ncells_tot, int *__restrict__ cell_count, int *__restrict__ hcell,
@@ -54,9 +58,14 @@ __global__ void k_hash(const double2 *__restrict__ r, const double2 *__restrict_
   const bool live = i < n;
   int c = -1;
   if (live) {
-    if (type[i] == kRemoved) {
+    bool removed = type[i] == kRemoved;
+    if (DECIDE && !removed && removal_decided(seqstate, seq[i])) {
+      removed = true;
+      type[i] = kRemoved;
+      atomicAdd(&counters[C_REMOVED], 1ull);
+    }
+    if (removed) {
       c = ncells_tot;
-      seqstate[seq[i]] = 0;  // its reference position is vacated
     } else {
       const Consts k = case_consts(k0, cases, cid, i);
       double2 ri = r[i];
@@ -299,13 +308,17 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_final(const int *__restrict
   }
 }
 
+// also marks the reference positions of removed particles (the trailing
+// bucket) absent in the removal states
 __global__ void k_scatter(const int *__restrict__ scal, const int *__restrict__ hcell,
                           const int *__restrict__ hrank, const int *__restrict__ cell_start,
-                          int *__restrict__ perm, int *__restrict__ cellof) {
+                          int *__restrict__ perm, int *__restrict__ cellof, int ncells_tot,
+                          const uint32_t *__restrict__ seq, uint8_t *__restrict__ seqstate) {
   const int n = scal[S_NP];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int c = hcell[i];
+  if (c == ncells_tot) seqstate[seq[i]] = 0;
   const int pos = cell_start[c] + hrank[i];
   perm[pos] = i;
   cellof[pos] = c;
@@ -496,23 +509,25 @@ int scan_cells(tsph_ctx *c, hipStream_t s) {
 
 }  // namespace
 
-int launch_cell_list(tsph_ctx *c, bool eos, bool predict, double dt, unsigned ad_skip) {
+int launch_cell_list(tsph_ctx *c, bool eos, bool predict, double dt, unsigned ad_skip,
+                     bool decide) {
   const Consts &k = c->k;
   const int nb = 256;
   const int g = grid_for(c->n_ub, nb);
   hipStream_t s = c->stream;
   const int nct = c->ncells_tot;
-  if (predict)
-    hipLaunchKernelGGL(k_hash<true>, dim3(g), dim3(nb), 0, s, c->cur.r, c->cur.v, c->cur.type,
-                       c->cur.seq, c->scal, k, c->cases, c->cur.cid, dt, nct, c->cell_count,
-                       c->hcell, c->hrank, c->seqstate, c->counters);
-  else
-    hipLaunchKernelGGL(k_hash<false>, dim3(g), dim3(nb), 0, s, c->cur.r, c->cur.v, c->cur.type,
-                       c->cur.seq, c->scal, k, c->cases, c->cur.cid, dt, nct, c->cell_count,
-                       c->hcell, c->hrank, c->seqstate, c->counters);
+#define TSPH_HASH(P, D)                                                                        \
+  hipLaunchKernelGGL((k_hash<P, D>), dim3(g), dim3(nb), 0, s, c->cur.r, c->cur.v, c->cur.type,  \
+                     c->cur.seq, c->scal, k, c->cases, c->cur.cid, dt, nct, c->cell_count,      \
+                     c->hcell, c->hrank, c->seqstate, c->counters)
+  if (predict && decide) TSPH_HASH(true, true);
+  else if (predict) TSPH_HASH(true, false);
+  else if (decide) TSPH_HASH(false, true);
+  else TSPH_HASH(false, false);
+#undef TSPH_HASH
   if (int e = scan_cells(c, s)) return e;
   hipLaunchKernelGGL(k_scatter, dim3(g), dim3(nb), 0, s, c->scal, c->hcell, c->hrank,
-                     c->cell_start, c->perm, c->cellof);
+                     c->cell_start, c->perm, c->cellof, nct, c->cur.seq, c->seqstate);
   hipLaunchKernelGGL(k_order, dim3(grid_for(nct + 1, nb)), dim3(nb), 0, s, c->cell_start, nct,
                      c->cur.seq, c->perm, c->cell_count);
   const Mixed mx = c->prm.precision == TSPH_PREC_MIXED ? c->mx : Mixed{};
@@ -531,7 +546,7 @@ int launch_cell_list(tsph_ctx *c, bool eos, bool predict, double dt, unsigned ad
   c->cells_valid = true;
   c->tiles_valid = true;  // listed (and cut for the pooled walk) by the scan
   c->records_valid = eos && c->prm.precision == TSPH_PREC_MIXED;
-  if (predict) c->marks_valid = false;
+  if (predict || decide) c->marks_valid = false;
   return 0;
 }
 

@@ -238,8 +238,11 @@ struct TileGrid {
 // on the way, in place of a launch_predictor immediately before.  ad_skip: a
 // mask of particle types (bit t: type t) whose a, drho are left behind, for a
 // build that a pair pass overwriting exactly those follows (forces_active_types)
+// decide: RemoveParticlesOutOfDomain's decision is taken by the build, in
+// place of a launch_remove immediately before (the removal states must be
+// current: a corrector precedes)
 int launch_cell_list(tsph_ctx *c, bool eos = false, bool predict = false, double dt = 0.,
-                     unsigned ad_skip = 0);
+                     unsigned ad_skip = 0, bool decide = false);
 // types whose a, drho a pair pass of `variant` writes (its active particles)
 inline unsigned forces_active_types(int variant) {
   return variant == TSPH_FORCES_DBC ? (1u << TSPH_FLUID) | (1u << TSPH_WALL) : 1u << TSPH_FLUID;

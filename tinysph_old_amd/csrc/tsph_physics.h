@@ -81,6 +81,23 @@ __device__ __forceinline__ double diff_fourtakas64(double h, double drs, double 
 }
 
 // Compute_Pressure (Tait, gamma = 7), src/SPH_pressure.h:7-19
+// RemoveParticlesOutOfDomain's decision for the particle at reference
+// position s (src/SPH_btdebug.h:4-31): the reference erases inside an
+// ascending loop without stepping back, so in a run of consecutive candidates
+// (state 2; absent positions, state 0, are particles removed earlier and are
+// skipped) every other one, the 1st, 3rd, ..., is erased.
+__device__ __forceinline__ bool removal_decided(const uint8_t *__restrict__ state, uint32_t s) {
+  if (state[s] != 2) return false;
+  int run = 0;  // candidates immediately before s in reference order
+  for (int64_t j = (int64_t)s - 1; j >= 0; --j) {
+    const uint8_t st = state[j];
+    if (st == 0) continue;
+    if (st != 2) break;
+    ++run;
+  }
+  return (run & 1) == 0;
+}
+
 // Integrate_SymplecticPredictor's half step, src/SPH_integration.h:266-287
 // (the expressions in the reference's order; no contraction)
 __device__ __forceinline__ double2 predicted_r(double2 r, double2 v, double dt) {

@@ -328,18 +328,7 @@ __global__ void k_rm_decide(Soa d, const int *__restrict__ scal, const uint8_t *
   const int n = scal[S_NP];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const uint32_t s = d.seq[i];
-  if (state[s] != 2) return;
-  // count the candidates immediately before s in reference order (absent
-  // positions are skipped: they are particles removed earlier)
-  int run = 0;
-  for (int64_t j = (int64_t)s - 1; j >= 0; --j) {
-    const uint8_t st = state[j];
-    if (st == 0) continue;
-    if (st != 2) break;
-    ++run;
-  }
-  if ((run & 1) == 0) {
+  if (removal_decided(state, d.seq[i])) {
     d.type[i] = kRemoved;
     atomicAdd(&counters[C_REMOVED], 1ull);
   }
