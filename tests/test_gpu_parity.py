@@ -165,6 +165,36 @@ def test_graph_replay_equals_eager():
         assert np.array_equal(ga[k], gb[k]), k
 
 
+def test_step_counters_equal_stage_by_stage():
+    """The captured step folds the EOS count after the wall densities into the
+    wall pass and the removal decision into the cell build: the out-of-range
+    density and removal counts equal those of the stages run one by one."""
+    case = make("dam_break_mdbc")
+    rho = case.arrays["rho"].copy()
+    fluid = np.nonzero(case.arrays["type"] == 0)[0]
+    rho[fluid[::97]] = 1600.0  # out of range for the EOS warning (src/SPH_pressure.h:76-80)
+    case.arrays["rho"] = rho
+    with T.system_for_case(case) as a, T.system_for_case(case) as b:
+        a.particle_to_cell()
+        b.particle_to_cell()
+        a.reset_counters()
+        b.reset_counters()
+        a.step(case.wall, case.dt, 2)
+        for _ in range(2):
+            b.mdbc_density(); b.density_to_pressure(); b.predictor(case.dt)
+            b.clear_cells(); b.particle_to_cell(); b.density_to_pressure()
+            b.compute_forces(T.FORCES_WCSPH); b.corrector(case.dt); b.remove_out_of_domain()
+            b.clear_cells(); b.particle_to_cell(); b.density_to_pressure()
+            b.compute_forces(T.FORCES_WCSPH)
+        ca, cb = a.counters(), b.counters()
+        ga, gb = a.download(), b.download()
+    assert ca["density_out_of_range"] > 0
+    for key in ("density_out_of_range", "removed", "pairs", "active"):
+        assert ca[key] == cb[key], key
+    for k in ("r", "v", "rho", "p", "a", "drho"):
+        assert np.array_equal(ga[k], gb[k]), k
+
+
 @pytest.mark.parametrize("name", list(CASES))
 def test_forces_mixed(name):
     case = make(name)

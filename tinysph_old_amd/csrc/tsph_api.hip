@@ -209,10 +209,10 @@ static int record_step(tsph_ctx *c, int wall, double dt, int inlet = -1, int out
   // cases/damBreakLong_BT/SPH_simulationSymplectic.h:72-73); DBC has none.
   // Without buffer zones nothing reads the pressures (or the mixed records)
   // this EOS writes before the cell build rewrites them all, so only its
-  // out-of-range density count is taken.
+  // out-of-range density count is taken, by the wall-density pass.
   if (wall == TSPH_WALL_MDBC || wall == TSPH_WALL_BT) {
-    if ((e = wall == TSPH_WALL_MDBC ? launch_mdbc(c) : launch_bt_density(c))) return e;
-    if ((e = zones ? launch_eos(c) : launch_eos_count(c))) return e;
+    if ((e = wall == TSPH_WALL_MDBC ? launch_mdbc(c, !zones) : launch_bt_density(c, !zones))) return e;
+    if (zones && (e = launch_eos(c))) return e;
   }
   if (zones) {
     // the buffer zones act on the predicted positions

@@ -261,11 +261,12 @@ int launch_forces(tsph_ctx *c, int variant);
 int launch_mixed_records(tsph_ctx *c);
 // tsph_stages.hip
 int launch_eos(tsph_ctx *c);
-int launch_eos_count(tsph_ctx *c);  // the EOS's out-of-range density count only
 int launch_predictor(tsph_ctx *c, double dt);
 int launch_corrector(tsph_ctx *c, double dt);
-int launch_mdbc(tsph_ctx *c);
-int launch_bt_density(tsph_ctx *c);
+// count_density: also take the out-of-range density count of an EOS pass
+// right after (in place of that pass, where nothing reads its pressures)
+int launch_mdbc(tsph_ctx *c, bool count_density = false);
+int launch_bt_density(tsph_ctx *c, bool count_density = false);
 int launch_remove(tsph_ctx *c);
 int run_water_elevation(tsph_ctx *c, double x, double y0, int inverse, double *out);
 int launch_interp_grid(tsph_ctx *c, double x0, double y0, double dh, int ncx, int ncy, double *out);

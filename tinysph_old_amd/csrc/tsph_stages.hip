@@ -48,21 +48,6 @@ __global__ void k_eos(Soa d, Mixed mx, const int *__restrict__ cellof, const int
   if (w && (threadIdx.x & 63) == 0) atomicAdd(&counters[C_BADRHO], (unsigned long long)__popcll(w));
 }
 
-// the EOS pass's out-of-range density count alone (its pressures and records
-// unused: the next cell build recomputes them)
-__global__ void k_eos_count(Soa d, const int *__restrict__ scal,
-                            unsigned long long *__restrict__ counters) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  bool bad = false;
-  if (i < scal[S_NP]) {
-    const int t = d.type[i];
-    const double rho = d.rho[i];
-    bad = t != TSPH_OUTLET && t != TSPH_INLET && (rho > 1500. || rho < 0.);
-  }
-  const unsigned long long w = __ballot(bad);
-  if (w && (threadIdx.x & 63) == 0) atomicAdd(&counters[C_BADRHO], (unsigned long long)__popcll(w));
-}
-
 __global__ void k_predictor(Soa d, const int *__restrict__ scal, double dt0,
                             const Consts *__restrict__ cases) {
   const int n = scal[S_NP];
@@ -226,9 +211,28 @@ __device__ __forceinline__ void wall_sums(const Soa &d, const int *__restrict__ 
   for (int q = 0; q < NS; ++q) acc[q] = grp[q];
 }
 
+// The out-of-range density count of the EOS pass that follows the wall
+// densities in the symplectic step (k_eos's count; the pass itself is not
+// needed there): the wall densities just computed are counted by the lanes
+// that computed them (wall_bad), every other particle by a grid-stride sweep.
+__device__ __forceinline__ void count_bad_density(const Soa &d, const int *__restrict__ scal,
+                                                  bool wall_bad,
+                                                  unsigned long long *__restrict__ counters) {
+  const int n = scal[S_NP];
+  unsigned bad = wall_bad;
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+    const int t = d.type[p];
+    if (t == TSPH_WALL || t == TSPH_INLET || t == TSPH_OUTLET) continue;
+    const double rho = d.rho[p];
+    bad += rho > 1500. || rho < 0.;
+  }
+  for (int o = 32; o > 0; o >>= 1) bad += __shfl_xor(bad, o, 64);
+  if (bad && (threadIdx.x & 63) == 0) atomicAdd(&counters[C_BADRHO], (unsigned long long)bad);
+}
+
 __global__ __launch_bounds__(64) void k_mdbc(Soa d, const int *__restrict__ cell_start, const int *__restrict__ scal,
                        const int *__restrict__ wall_list, Consts k0,
-                       const Consts *__restrict__ cases) {
+                       const Consts *__restrict__ cases, unsigned long long *__restrict__ counters) {
   __shared__ __align__(16) double s_terms[64 * 12];
   const int w = blockIdx.x * kWallsPerWave + (int)(threadIdx.x / kWG);
   const bool live = w < scal[S_NWALL];
@@ -256,7 +260,8 @@ __global__ __launch_bounds__(64) void k_mdbc(Soa d, const int *__restrict__ cell
                   a[10] += dry * drx * kw.f * mr;
                   a[11] += dry * dry * kw.f * mr;
                 });
-  if (!live || (threadIdx.x & (kWG - 1)) != 0) return;
+  bool wall_bad = false;
+  if (live && (threadIdx.x & (kWG - 1)) == 0) {
   const double2 r = d.r[i];
   const double rhowsum = acc[0], drhodx = acc[1], drhody = acc[2];
   const double a11 = acc[3], a12 = acc[4], a13 = acc[5], a21 = acc[6], a22 = acc[7], a23 = acc[8];
@@ -285,11 +290,14 @@ __global__ __launch_bounds__(64) void k_mdbc(Soa d, const int *__restrict__ cell
   }
   if (rho < 1000.) rho = 1000.;  // src/SPH_mDBC.h:44
   d.rho[i] = rho;
+  wall_bad = rho > 1500. || rho < 0.;
+  }
+  if (counters) count_bad_density(d, scal, wall_bad, counters);
 }
 
 __global__ __launch_bounds__(64) void k_bt_density(Soa d, const int *__restrict__ cell_start, const int *__restrict__ scal,
                              const int *__restrict__ wall_list, Consts k0,
-                             const Consts *__restrict__ cases) {
+                             const Consts *__restrict__ cases, unsigned long long *__restrict__ counters) {
   __shared__ __align__(16) double s_terms[64 * 2];
   const int w = blockIdx.x * kWallsPerWave + (int)(threadIdx.x / kWG);
   const bool live = w < scal[S_NWALL];
@@ -303,10 +311,14 @@ __global__ __launch_bounds__(64) void k_bt_density(Soa d, const int *__restrict_
                  a[0] += kw.w * m / nrho;
                  a[1] += m * kw.w;
                });
-  if (!live || (threadIdx.x & (kWG - 1)) != 0) return;
-  double rho = acc[0] > 0. ? acc[1] / acc[0] : 1000.;
-  if (rho < 1000.) rho = 1000.;
-  d.rho[i] = rho;
+  bool wall_bad = false;
+  if (live && (threadIdx.x & (kWG - 1)) == 0) {
+    double rho = acc[0] > 0. ? acc[1] / acc[0] : 1000.;
+    if (rho < 1000.) rho = 1000.;
+    d.rho[i] = rho;
+    wall_bad = rho > 1500. || rho < 0.;
+  }
+  if (counters) count_bad_density(d, scal, wall_bad, counters);
 }
 
 // Removal.  The reference erases inside an ascending loop without stepping
@@ -473,13 +485,6 @@ int launch_eos(tsph_ctx *c) {
   return 0;
 }
 
-int launch_eos_count(tsph_ctx *c) {
-  hipLaunchKernelGGL(k_eos_count, dim3(grid_for(c->n_ub, 256)), dim3(256), 0, c->stream, c->cur,
-                     c->scal, c->counters);
-  TSPH_HIP(hipGetLastError());
-  return 0;
-}
-
 int launch_predictor(tsph_ctx *c, double dt) {
   hipLaunchKernelGGL(k_predictor, dim3(grid_for(c->n_ub, 256)), dim3(256), 0, c->stream, c->cur,
                      c->scal, dt, c->cases);
@@ -498,17 +503,19 @@ int launch_corrector(tsph_ctx *c, double dt) {
   return 0;
 }
 
-int launch_mdbc(tsph_ctx *c) {
+int launch_mdbc(tsph_ctx *c, bool count_density) {
   hipLaunchKernelGGL(k_mdbc, dim3(grid_for(c->n_wall, kWallsPerWave)), dim3(64), 0, c->stream, c->cur,
-                     c->cell_start, c->scal, c->wall_list, c->k, c->cases);
+                     c->cell_start, c->scal, c->wall_list, c->k, c->cases,
+                     count_density ? c->counters : nullptr);
   TSPH_HIP(hipGetLastError());
   c->records_valid = false;
   return 0;
 }
 
-int launch_bt_density(tsph_ctx *c) {
+int launch_bt_density(tsph_ctx *c, bool count_density) {
   hipLaunchKernelGGL(k_bt_density, dim3(grid_for(c->n_wall, kWallsPerWave)), dim3(64), 0, c->stream, c->cur,
-                     c->cell_start, c->scal, c->wall_list, c->k, c->cases);
+                     c->cell_start, c->scal, c->wall_list, c->k, c->cases,
+                     count_density ? c->counters : nullptr);
   TSPH_HIP(hipGetLastError());
   c->records_valid = false;
   return 0;
