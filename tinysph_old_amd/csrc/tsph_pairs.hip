@@ -132,13 +132,13 @@ struct Acc32 {
 };
 
 struct C32 {
-  float h, ih, m, c0, rho0, dp, kh, r2cut, epsh2, awen, bwen, visc, dcoef, hx;
+  float h, ih, m, c0, rho0, dp, dpm, kh, r2cut, epsh2, awen, bwen, visc, dcoef, hx;
 };
 
 __device__ __forceinline__ C32 make_c32(const Consts &k) {
   C32 c;
   c.h = (float)k.h; c.ih = 1.f / c.h; c.m = (float)k.m; c.c0 = (float)k.cs; c.rho0 = (float)k.rho0;
-  c.dp = (float)k.dp; c.kh = (float)k.kh; c.r2cut = 4.f * c.h * c.h;
+  c.dp = (float)k.dp; c.dpm = (float)(k.dp / k.m); c.kh = (float)k.kh; c.r2cut = 4.f * c.h * c.h;
   c.epsh2 = (float)kEps * c.h * c.h; c.awen = (float)k.awen; c.bwen = (float)k.bwen;
   // Pi = -alpha c0 (h drdv / (r^2 + eps h^2)) / (0.5 (rho_i + rho_j)) for drdv < 0
   c.visc = -2.f * (float)k.avisc * c.c0 * c.h;
@@ -148,7 +148,9 @@ __device__ __forceinline__ C32 make_c32(const Consts &k) {
 }
 
 // one pair of the mixed path; dr = r_i - r_j with r2 = |dr|^2 < (2h)^2; dev is
-// rho - rho0, ir is 1/rho
+// rho - rho0, ir is 1/rho.  Every sum is kept divided by the particle mass m
+// (the BT wall terms, which carry dp instead, by dp/m); finish32 applies m
+// once, or lets it cancel in the BT ratios.
 template <int VAR>
 __device__ __forceinline__ void pair32(const C32 &c, Acc32 &s, float drx, float dry, float r2,
                                        float2 vi, float adev, float ap, float air, float2 vj,
@@ -168,32 +170,34 @@ __device__ __forceinline__ void pair32(const C32 &c, Acc32 &s, float drx, float 
   if (VAR == TSPH_FORCES_BT && nt == TSPH_WALL) {
     const float w = c.awen * (qq + qq + 1.f) * wq2 * wq2;
     const float nrho = c.rho0 + ndev;
-    const float t = w * (p_temp + visco) * c.dp * nrho;
+    const float t = w * (p_temp + visco) * c.dpm * nrho;
     s.accx += t * nb.x;
     s.accy += t * nb.y;
-    s.drho_sum -= (dvx * nb.x + dvy * nb.y) * w * nrho * c.dp;
+    s.drho_sum -= (dvx * nb.x + dvy * nb.y) * w * nrho * c.dpm;
   } else {
-    const float dWx = drx * f, dWy = dry * f;
+    const float dWx = drx * f, dWy = dry * f;  // dr.dW = f r^2
     const float dvdW = dvx * dWx + dvy * dWy;
-    const float drdW = drx * dWx + dry * dWy;
+    const float drdW = f * r2;
     const float drhoH = hydro_drho_f32(dry * c.hx, c.rho0);
     const float psi = c.dcoef * ((ndev - adev) - drhoH) * rinv;
-    const float t = (p_temp + visco) * c.m;
+    const float t = p_temp + visco;
     s.accx -= dWx * t;
     s.accy -= dWy * t;
-    s.drho_sum += dvdW * c.m;
-    s.diff_sum -= psi * drdW * c.m * nir;
+    s.drho_sum += dvdW;
+    s.diff_sum -= psi * drdW * nir;
     if (VAR == TSPH_FORCES_BT && nt == TSPH_FLUID) {
       const float w = c.awen * (qq + qq + 1.f) * wq2 * wq2;
-      s.gamma += w * c.m * nir;
+      s.gamma += w * nir;
     }
   }
 }
 
+// the sums of pair32 are divided by m: m is applied here (in the BT ratios it
+// cancels; its Shepard sum is compared as m gamma)
 template <int VAR>
 __device__ __forceinline__ void finish32(const Consts &k, const Acc32 &s, int at, double2 &a, double &dr) {
   if (VAR == TSPH_FORCES_BT) {
-    if (s.gamma > (float)kEps) {
+    if ((double)s.gamma * k.m > kEps) {
       a = make_double2((double)(s.accx / s.gamma) - k.gx, (double)(s.accy / s.gamma) - k.gy);
       dr = (double)(s.drho_sum / s.gamma + s.diff_sum / s.gamma);
     } else {
@@ -202,10 +206,10 @@ __device__ __forceinline__ void finish32(const Consts &k, const Acc32 &s, int at
     }
   } else if (VAR == TSPH_FORCES_DBC && at == TSPH_WALL) {
     a = make_double2(0., 0.);
-    dr = (double)(s.drho_sum + s.diff_sum);
+    dr = (double)(s.drho_sum + s.diff_sum) * k.m;
   } else {
-    a = make_double2((double)s.accx - k.gx, (double)s.accy - k.gy);
-    dr = (double)(s.drho_sum + s.diff_sum);
+    a = make_double2((double)s.accx * k.m - k.gx, (double)s.accy * k.m - k.gy);
+    dr = (double)(s.drho_sum + s.diff_sum) * k.m;
   }
 }
 
@@ -737,10 +741,11 @@ __global__ __launch_bounds__(kWave * kPoolWaves) void k_tile32(Soa d, Mixed mx, 
       npair += cnt;
       // pair terms in list order; the next entry's records are loaded while
       // the current one is evaluated
-      int jn = cnt > 0 ? my_list[0] : 0;
+      int jn = my_list[0];
       for (int e = 0; e < cnt; ++e) {
         const int j = jn;
-        if (e + 1 < cnt) jn = my_list[(e + 1) * kWave];
+        // read unconditionally (past cnt it is an unused entry of the list)
+        jn = my_list[min(e + 1, kList - 1) * kWave];
         const float2 pj = s_pos[j], dpj = s_dp[j];
         const float drx = pi.x - pj.x, dry = pi.y - pj.y;
         const int nt = VAR != TSPH_FORCES_WCSPH ? s_t[j] : 0;
