@@ -328,13 +328,11 @@ __global__ void k_scatter(const int *__restrict__ scal, const int *__restrict__ 
 // particles are sorted in registers (all loads in flight together, then an
 // odd-even transposition network); larger cells use an insertion sort.
 // Also clears the per-cell counts (ncells + 1 of them) for the next build.
-__global__ void k_order(const int *__restrict__ cell_start, int ncells,
-                        const uint32_t *__restrict__ seq, int *__restrict__ perm,
-                        int *__restrict__ cell_count) {
+// orders cell c's slots by seq
+__device__ __forceinline__ void order_cell(const int *__restrict__ cell_start, int c,
+                                           const uint32_t *__restrict__ seq,
+                                           int *__restrict__ perm) {
   constexpr int R = 16;
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c <= ncells) cell_count[c] = 0;
-  if (c >= ncells) return;
   const int s = cell_start[c], e = cell_start[c + 1];
   const int cnt = e - s;
   if (cnt <= 1) return;
@@ -379,6 +377,36 @@ __global__ void k_order(const int *__restrict__ cell_start, int ncells,
       --b;
     }
     perm[b + 1] = pa;
+  }
+}
+
+__global__ void k_order(const int *__restrict__ cell_start, int ncells,
+                        const uint32_t *__restrict__ seq, int *__restrict__ perm,
+                        int *__restrict__ cell_count) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c <= ncells) cell_count[c] = 0;
+  if (c < ncells) order_cell(cell_start, c, seq, perm);
+}
+
+// The same over the cells of the listed (non-empty) tiles only, for a single
+// case: every occupied cell lies in a listed tile, so the other cells need
+// neither ordering nor clearing (their counts are 0).  A bounded grid strides
+// over the (tile, row) items.
+__global__ void k_order_tiles(const int *__restrict__ cell_start, const int *__restrict__ scal,
+                              const int *__restrict__ tile_list, int rows, int ncy, int ncells,
+                              const uint32_t *__restrict__ seq, int *__restrict__ perm,
+                              int *__restrict__ cell_count) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) cell_count[ncells] = 0;  // the removed bucket
+  const int items = scal[S_NTILES] * rows;
+  const int nty = (ncy + rows - 1) / rows;
+  for (int item = blockIdx.x * blockDim.x + threadIdx.x; item < items; item += gridDim.x * blockDim.x) {
+    const int it = item / rows, r = item - it * rows;
+    const int tile = tile_list[it];
+    const int cx = tile / nty, cy = (tile - cx * nty) * rows + r;
+    if (cy >= ncy) continue;
+    const int c = cx * ncy + cy;
+    cell_count[c] = 0;
+    order_cell(cell_start, c, seq, perm);
   }
 }
 
@@ -528,8 +556,12 @@ int launch_cell_list(tsph_ctx *c, bool eos, bool predict, double dt, unsigned ad
   if (int e = scan_cells(c, s)) return e;
   hipLaunchKernelGGL(k_scatter, dim3(g), dim3(nb), 0, s, c->scal, c->hcell, c->hrank,
                      c->cell_start, c->perm, c->cellof, nct, c->cur.seq, c->seqstate);
-  hipLaunchKernelGGL(k_order, dim3(grid_for(nct + 1, nb)), dim3(nb), 0, s, c->cell_start, nct,
-                     c->cur.seq, c->perm, c->cell_count);
+  if (c->cases)
+    hipLaunchKernelGGL(k_order, dim3(grid_for(nct + 1, nb)), dim3(nb), 0, s, c->cell_start, nct,
+                       c->cur.seq, c->perm, c->cell_count);
+  else
+    hipLaunchKernelGGL(k_order_tiles, dim3(2 * c->num_cu), dim3(nb), 0, s, c->cell_start, c->scal,
+                       c->tile_list, c->tile_rows, k.ncy, nct, c->cur.seq, c->perm, c->cell_count);
   const Mixed mx = c->prm.precision == TSPH_PREC_MIXED ? c->mx : Mixed{};
 #define TSPH_REORDER(E, P)                                                                    \
   hipLaunchKernelGGL((k_reorder<E, P>), dim3(g), dim3(nb), 0, s, c->cell_start, nct, c->perm,      \
