@@ -174,19 +174,24 @@ __device__ __forceinline__ int2 tile_at(const TileScan ts, int c) {
 // tl[j] >= 0: cell base + j starts a non-empty tile of work wk[j] (its own
 // particles + kTileWork0, the pooled walk's cost model); returns the number
 // of such tiles and their work (w)
-__device__ __forceinline__ int stage_counts(const int *__restrict__ in, int len, const TileScan ts,
-                                            int (&v)[kScanItems], int (&tl)[kScanItems],
+// the thread's counts (v) and, for the first kMaxTileRows threads, one count
+// of the next block (halo)
+__device__ __forceinline__ void load_counts(const int *__restrict__ in, int len,
+                                            int (&v)[kScanItems], int &halo) {
+  const int b0 = blockIdx.x * kScanTile, base = b0 + threadIdx.x * kScanItems;
+#pragma unroll
+  for (int j = 0; j < kScanItems; ++j) v[j] = (base + j < len) ? in[base + j] : 0;
+  const int c = b0 + kScanTile + threadIdx.x;
+  halo = threadIdx.x < kMaxTileRows && c < len ? in[c] : 0;
+}
+
+__device__ __forceinline__ int stage_counts(int len, const TileScan ts, const int (&v)[kScanItems],
+                                            int halo, int (&tl)[kScanItems],
                                             int (&wk)[kScanItems], int &w, int *s_cnt) {
   const int b0 = blockIdx.x * kScanTile, base = b0 + threadIdx.x * kScanItems;
 #pragma unroll
-  for (int j = 0; j < kScanItems; ++j) {
-    v[j] = (base + j < len) ? in[base + j] : 0;
-    s_cnt[threadIdx.x * kScanItems + j] = v[j];
-  }
-  if (threadIdx.x < kMaxTileRows) {
-    const int c = b0 + kScanTile + threadIdx.x;
-    s_cnt[kScanTile + threadIdx.x] = c < len ? in[c] : 0;
-  }
+  for (int j = 0; j < kScanItems; ++j) s_cnt[threadIdx.x * kScanItems + j] = v[j];
+  if (threadIdx.x < kMaxTileRows) s_cnt[kScanTile + threadIdx.x] = halo;
   __syncthreads();
   int nt = 0;
   w = 0;
@@ -239,8 +244,9 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_partials(const int *__restr
                                                               int *__restrict__ partial, TileScan ts) {
   __shared__ int lds[kScanBlock / 64];
   __shared__ int s_cnt[kScanTile + kMaxTileRows];
-  int v[kScanItems], tl[kScanItems], wk[kScanItems], w;
-  const int nt = stage_counts(in, len, ts, v, tl, wk, w, s_cnt);
+  int v[kScanItems], tl[kScanItems], wk[kScanItems], w, halo;
+  load_counts(in, len, v, halo);
+  const int nt = stage_counts(len, ts, v, halo, tl, wk, w, s_cnt);
   int s = 0;
 #pragma unroll
   for (int j = 0; j < kScanItems; ++j) s += v[j];
@@ -267,10 +273,13 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_final(const int *__restrict
                                                            int *__restrict__ scal, TileScan ts) {
   __shared__ int lds[6 * kScanBlock / 64];
   __shared__ int s_cnt[kScanTile + kMaxTileRows];
+  // the block's counts are loaded first, so their latency overlaps the sums
+  // of the partials
+  int v[kScanItems], tl[kScanItems], wk[kScanItems], w, halo;
+  load_counts(in, len, v, halo);
   int pre[3], all[3];
   block_offsets(partial, pre, all, lds);
-  int v[kScanItems], tl[kScanItems], wk[kScanItems], w;
-  const int nt = stage_counts(in, len, ts, v, tl, wk, w, s_cnt);
+  const int nt = stage_counts(len, ts, v, halo, tl, wk, w, s_cnt);
   const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
   int s = 0;
 #pragma unroll
