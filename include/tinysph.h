@@ -371,7 +371,7 @@ int tsph_autotune(tsph_ctx *ctx, int variant, int *tile_rows, int *waves_per_cu,
  * (workgroups of up to 4 wavefronts, each with a chunk of the tile list of
  * about equal work whose tiles its wavefronts take from a counter in LDS),
  * 0 = static striding, one wavefront per workgroup (the default).  Grids with
- * fewer than 8 tiles per persistent wave always stride.  Results do not depend
+ * fewer than 16 tiles per persistent wave always stride.  Results do not depend
  * on it.  No counterpart in the reference. */
 int tsph_set_balanced(tsph_ctx *ctx, int balanced);
 

@@ -335,6 +335,8 @@ static int alloc_context(tsph_ctx *c) {
     if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess) {
       if (prop.multiProcessorCount > 0) c->num_cu = prop.multiProcessorCount;
       if (prop.sharedMemPerBlock > 0) c->max_lds_block = (int)prop.sharedMemPerBlock;
+      if (prop.maxSharedMemoryPerMultiProcessor > 0)
+        c->lds_cu = (int)prop.maxSharedMemoryPerMultiProcessor;
     }
   }
   // the tile-list build: one block per 256 tiles (tiles <= cells); pooled

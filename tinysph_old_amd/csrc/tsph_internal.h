@@ -136,6 +136,7 @@ struct tsph_ctx {
   int waves_per_cu = 8;  // persistent pair-loop waves per compute unit
   int num_cu = 256;      // compute units of the device
   int max_lds_block = 65536;  // LDS bytes a workgroup may allocate
+  int lds_cu = 163840;        // LDS bytes of one compute unit
   // pair-loop tile walk per precision (FP64, mixed): pooled (work-balanced
   // chunks per workgroup, tiles taken from an LDS counter by the workgroup's
   // waves) or the plain static walk; defaults set at creation
@@ -215,10 +216,11 @@ inline int total_tiles(const tsph_ctx *c) {
 }
 
 // whether a pair launch of `waves` persistent waves takes the pooled walk:
-// the precision's setting, for grids of at least 8 tiles per wave (below that
-// the waves have about a tile each and nothing to balance)
+// the precision's setting, for grids of at least 16 tiles per wave (below that
+// the waves have a few tiles each and little to balance: a 183k-particle ALE
+// dam break with 8-10 tiles per wave ran 0.35 ms per step pooled, 0.28 static)
 inline bool walk_balanced(const tsph_ctx *c, bool fp32, int waves) {
-  return c->balanced[fp32 ? 1 : 0] && total_tiles(c) >= 8 * waves;
+  return c->balanced[fp32 ? 1 : 0] && total_tiles(c) >= 16 * waves;
 }
 
 // launch geometry of a tile kernel: `groups` workgroups (a multiple of 8) of

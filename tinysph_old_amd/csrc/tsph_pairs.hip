@@ -825,8 +825,19 @@ TileGrid tile_grid(tsph_ctx *c, bool fp32, size_t lds_wave) {
   tg.pooled = walk_balanced(c, fp32, waves);
   int pw = 1;
   if (tg.pooled) {
-    pw = (c->max_lds_block - kPoolHdr) / tg.lds_wave;
-    pw = pw < 1 ? 1 : (pw > kPoolWaves ? kPoolWaves : pw);
+    // the workgroup size (up to kPoolWaves wavefronts, within one workgroup's
+    // LDS) that keeps the most wavefronts of the grid resident on a compute
+    // unit at once; ties go to the larger workgroup.  Four 21 KB waves in one
+    // workgroup would leave a 160 KB compute unit with one workgroup (4 waves)
+    // where 1-wave workgroups give it 7.
+    int best = 0;
+    for (int p = 1; p <= kPoolWaves; ++p) {
+      const int block = kPoolHdr + p * tg.lds_wave;
+      if (block > c->max_lds_block) break;
+      const int fit = c->lds_cu / block, want = (c->waves_per_cu + p - 1) / p;
+      const int resident = (fit < want ? fit : want) * p;
+      if (resident >= best) best = resident, pw = p;
+    }
   }
   tg.groups = 8 * ((waves + 8 * pw - 1) / (8 * pw));
   tg.grid = dim3(tg.groups);
