@@ -309,6 +309,9 @@ def main() -> int:
                    "dt": case.dt, "pair_passes_per_step": 2, "path": "mixed, hipGraph step"},
         "steps_per_s": k / mixed["t_graph"],
         "pairs_per_step": mixed["pairs"] / k,
+        # cell-list scan blocks that fell back to summing the counts themselves
+        # (both paths, timed runs; expected 0)
+        "scan_stalls": mixed["counters"]["scan_stalls"] + fp64["counters"]["scan_stalls"],
         "fp64": {"pair_interactions_per_s": rate(fp64), "steps_per_s": k / fp64["t_graph"],
                  "ms_per_step": fp64["t_graph"] / k * 1e3, "pair_kernel_avg_ms": d_avg,
                  "roofline_frac": d_ach / HBM_PEAK_GBS},

@@ -94,6 +94,9 @@ typedef struct tsph_counters {
   uint64_t zone_chained; /* appended particles already past the zone start (not re-examined) */
   uint64_t reinit_out_of_range; /* fluid densities outside [990, 1010] when reinitialised
                                    (src/SPH_density_reinitialization.h:133-134) */
+  uint64_t scan_stalls;  /* cell-list scan blocks that outwaited their bounded wait for the
+                            blocks before them and summed the counts themselves (results
+                            unchanged; expected 0) */
 } tsph_counters;
 
 typedef struct tsph_ctx tsph_ctx;

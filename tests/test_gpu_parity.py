@@ -268,16 +268,23 @@ def test_tile_walks_agree(precision):
         for bal, waves in ((0, 8), (1, 1), (1, 2), (0, 2)):
             ps.set_balanced(bal)
             ps.set_waves_per_cu(waves)
-            ps.reset_counters()
-            ps.compute_forces(T.FORCES_WCSPH)
-            out = ps.download(("a", "drho"))
-            out["counters"] = ps.counters()
-            outs.append(out)
+            # first with the list re-cut for the new walk by the tile-list
+            # pass, then with the list and chunks the cell-list build cuts
+            for rebuild in (False, True):
+                if rebuild:
+                    ps.particle_to_cell()
+                    ps.density_to_pressure()
+                ps.reset_counters()
+                ps.compute_forces(T.FORCES_WCSPH)
+                out = ps.download(("a", "drho"))
+                out["counters"] = ps.counters()
+                outs.append(out)
     for o in outs[1:]:
         assert np.array_equal(o["a"], outs[0]["a"])
         assert np.array_equal(o["drho"], outs[0]["drho"])
         for key in ("pairs", "candidates", "active", "pair_passes"):
             assert o["counters"][key] == outs[0]["counters"][key], key
+        assert o["counters"]["scan_stalls"] == 0
 
 
 def test_water_level_and_probe():

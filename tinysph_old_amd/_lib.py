@@ -44,7 +44,7 @@ class Counters(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in ("pairs", "candidates", "active", "pair_passes",
                                            "density_out_of_range", "outside_grid", "removed",
                                            "appended", "zone_rejected", "zone_chained",
-                                           "reinit_out_of_range")]
+                                           "reinit_out_of_range", "scan_stalls")]
 
 
 class History(C.Structure):

@@ -322,11 +322,14 @@ static int alloc_context(tsph_ctx *c) {
       (e = dalloc(&c->perm, cap)) || (e = dalloc(&c->scan_tmp, 3 * ((nct + 1) / 1024 + 2))) ||
       (e = dalloc(&c->seqstate, (size_t)c->seq_cap)) || (e = dalloc(&c->probe_out, 4)) ||
       (e = dalloc(&c->bz_list, 2 * kBzMax)) ||
-      (e = dalloc(&c->tile_list, nct + 8)) || (e = dalloc(&c->wall_list, cap)))
+      (e = dalloc(&c->tile_list, nct + 8)) || (e = dalloc(&c->wall_list, cap)) ||
+      (e = dalloc(&c->scan_state, (nct + 1) / 1024 + 2)))
     return e;
   // the per-cell counters and removal states start cleared; later builds clear
-  // them as they consume them
+  // them as they consume them (the single-pass scan's words likewise)
   if (hipMemsetAsync(c->cell_count, 0, sizeof(int) * (nct + 1), c->stream) != hipSuccess ||
+      hipMemsetAsync(c->scan_state, 0, sizeof(unsigned long long) * ((nct + 1) / 1024 + 2),
+                     c->stream) != hipSuccess ||
       hipMemsetAsync(c->seqstate, 0, (size_t)c->seq_cap, c->stream) != hipSuccess)
     return fail(TSPH_EHIP, "initialisation failed");
   {
@@ -410,7 +413,7 @@ int tsph_destroy(tsph_ctx *c) {
   free_soa(c->cur);
   free_soa(c->alt);
   void *ps[] = {c->scal, c->counters, c->cell_count, c->cell_start, c->cellof, c->hrank,
-                c->hcell, c->perm, c->scan_tmp, c->seqstate, c->probe_out, c->mx.rv,
+                c->hcell, c->perm, c->scan_tmp, c->scan_state, c->seqstate, c->probe_out, c->mx.rv,
                 c->mx.rp, c->mx.nrm, c->tile_list, c->wall_list, c->bz_list, c->cases,
                 c->tile_base, c->list_part, c->tile_bound};
   for (void *p : ps)
@@ -494,7 +497,8 @@ static int upload_impl(tsph_ctx *c, const tsph_particles *h, const uint32_t *seq
   sc[S_NP] = n;
   sc[S_MAXSEQ] = maxseq;
   TSPH_HIP(hipMemcpy(c->scal, sc, sizeof sc, hipMemcpyHostToDevice));
-  TSPH_HIP(hipMemset(c->seqstate, 0, (size_t)c->seq_cap));
+  TSPH_HIP(hipMemsetAsync(c->seqstate, 0, (size_t)c->seq_cap, c->stream));
+  TSPH_HIP(hipStreamSynchronize(c->stream));
   c->n_ub = n;
   c->n_wall = 0;
   for (int i = 0; i < n; ++i) c->n_wall += t[i] == TSPH_WALL;
@@ -954,13 +958,17 @@ int tsph_get_counters(tsph_ctx *c, tsph_counters *out) {
   out->zone_rejected = v[C_BZREJECT];
   out->zone_chained = v[C_BZCHAIN];
   out->reinit_out_of_range = v[C_REINIT];
+  out->scan_stalls = v[C_SCANSTALL];
   return 0;
 }
 
 int tsph_reset_counters(tsph_ctx *c) {
   if (!c) return fail(TSPH_EINVAL, "null argument");
+  // on the context's stream: a hipMemset on the null stream is not ordered
+  // with the (non-blocking) context stream and could land after the next
+  // launch's first counts
+  TSPH_HIP(hipMemsetAsync(c->counters, 0, counter_words(c) * sizeof(unsigned long long), c->stream));
   TSPH_HIP(hipStreamSynchronize(c->stream));
-  TSPH_HIP(hipMemset(c->counters, 0, counter_words(c) * sizeof(unsigned long long)));
   return 0;
 }
 

@@ -176,19 +176,20 @@ __device__ __forceinline__ int2 tile_at(const TileScan ts, int c) {
 // of such tiles and their work (w)
 // the thread's counts (v) and, for the first kMaxTileRows threads, one count
 // of the next block (halo)
-__device__ __forceinline__ void load_counts(const int *__restrict__ in, int len,
+__device__ __forceinline__ void load_counts(const int *__restrict__ in, int len, int block,
                                             int (&v)[kScanItems], int &halo) {
-  const int b0 = blockIdx.x * kScanTile, base = b0 + threadIdx.x * kScanItems;
+  const int b0 = block * kScanTile, base = b0 + threadIdx.x * kScanItems;
 #pragma unroll
   for (int j = 0; j < kScanItems; ++j) v[j] = (base + j < len) ? in[base + j] : 0;
   const int c = b0 + kScanTile + threadIdx.x;
   halo = threadIdx.x < kMaxTileRows && c < len ? in[c] : 0;
 }
 
-__device__ __forceinline__ int stage_counts(int len, const TileScan ts, const int (&v)[kScanItems],
-                                            int halo, int (&tl)[kScanItems],
-                                            int (&wk)[kScanItems], int &w, int *s_cnt) {
-  const int b0 = blockIdx.x * kScanTile, base = b0 + threadIdx.x * kScanItems;
+__device__ __forceinline__ int stage_counts(int len, int block, const TileScan ts,
+                                            const int (&v)[kScanItems], int halo,
+                                            int (&tl)[kScanItems], int (&wk)[kScanItems], int &w,
+                                            int *s_cnt) {
+  const int b0 = block * kScanTile, base = b0 + threadIdx.x * kScanItems;
 #pragma unroll
   for (int j = 0; j < kScanItems; ++j) s_cnt[threadIdx.x * kScanItems + j] = v[j];
   if (threadIdx.x < kMaxTileRows) s_cnt[kScanTile + threadIdx.x] = halo;
@@ -245,8 +246,8 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_partials(const int *__restr
   __shared__ int lds[kScanBlock / 64];
   __shared__ int s_cnt[kScanTile + kMaxTileRows];
   int v[kScanItems], tl[kScanItems], wk[kScanItems], w, halo;
-  load_counts(in, len, v, halo);
-  const int nt = stage_counts(len, ts, v, halo, tl, wk, w, s_cnt);
+  load_counts(in, len, blockIdx.x, v, halo);
+  const int nt = stage_counts(len, blockIdx.x, ts, v, halo, tl, wk, w, s_cnt);
   int s = 0;
 #pragma unroll
   for (int j = 0; j < kScanItems; ++j) s += v[j];
@@ -276,10 +277,10 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_final(const int *__restrict
   // the block's counts are loaded first, so their latency overlaps the sums
   // of the partials
   int v[kScanItems], tl[kScanItems], wk[kScanItems], w, halo;
-  load_counts(in, len, v, halo);
+  load_counts(in, len, blockIdx.x, v, halo);
   int pre[3], all[3];
   block_offsets(partial, pre, all, lds);
-  const int nt = stage_counts(len, ts, v, halo, tl, wk, w, s_cnt);
+  const int nt = stage_counts(len, blockIdx.x, ts, v, halo, tl, wk, w, s_cnt);
   const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
   int s = 0;
 #pragma unroll
@@ -313,6 +314,107 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_final(const int *__restrict
         for (int g = threadIdx.x; g <= ts.groups; g += kScanBlock) ts.bound[g] = 0;
       else if (threadIdx.x == 0)
         ts.bound[ts.groups] = all[1];
+    }
+  }
+}
+
+// Single-pass form of the two kernels above, for a single-case grid.  Every
+// block publishes its sums (cells, tiles) in one 64-bit word (bits 0-30
+// cells, 31-61 tiles, 63 set), then adds up the words of all the blocks
+// before it.  k_order_tiles clears the words for the next build.  Blocks are
+// dispatched in index order, so a block waits only for blocks that have
+// started, and those publish before they wait themselves.  A block whose wait
+// exceeds kScanSpin polls all the same forms the sums from the counts of the
+// cells before it (and counts the stall): a late publisher costs time, never
+// correctness, and every wait ends.  The pooled walk's chunk bounds are cut
+// by k_order_tiles, from cell_start.
+constexpr int kScanSpin = 1 << 20;
+constexpr unsigned long long kScanField = 0x7fffffffull;
+
+__device__ __forceinline__ void block_sum2(int &a, int &b, int *lds) {
+  for (int o = 32; o > 0; o >>= 1) {
+    a += __shfl_xor(a, o, 64);
+    b += __shfl_xor(b, o, 64);
+  }
+  const int wid = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    lds[2 * wid] = a;
+    lds[2 * wid + 1] = b;
+  }
+  __syncthreads();
+  a = b = 0;
+  for (int w = 0; w < kScanBlock / 64; ++w) {
+    a += lds[2 * w];
+    b += lds[2 * w + 1];
+  }
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_single(const int *__restrict__ in, int len,
+                                                            unsigned long long *__restrict__ state,
+                                                            int *__restrict__ out,
+                                                            int *__restrict__ scal,
+                                                            unsigned long long *__restrict__ counters,
+                                                            TileScan ts) {
+  __shared__ int lds[2 * kScanBlock / 64];
+  __shared__ int s_cnt[kScanTile + kMaxTileRows];
+  const int nb = gridDim.x, blk = blockIdx.x;
+  int v[kScanItems], tl[kScanItems], wk[kScanItems], w, halo;
+  load_counts(in, len, blk, v, halo);
+  const int nt = stage_counts(len, blk, ts, v, halo, tl, wk, w, s_cnt);
+  int s = 0;
+#pragma unroll
+  for (int j = 0; j < kScanItems; ++j) s += v[j];
+  int total, tiles;
+  int ex = block_exclusive_scan<kScanBlock>(s, lds, total);
+  int et = block_exclusive_scan<kScanBlock>(nt, lds, tiles);
+  if (threadIdx.x == 0)
+    __hip_atomic_store(&state[blk],
+                       (1ull << 63) | ((unsigned long long)tiles << 31) | (unsigned long long)total,
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  int pc = 0, pt = 0;
+  bool stalled = false;
+  for (int b = threadIdx.x; b < blk && !stalled; b += kScanBlock) {
+    unsigned long long word = 0;
+    for (int spin = 0;; ++spin) {
+      word = __hip_atomic_load(&state[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (word >> 63) break;
+      if (spin == kScanSpin) {
+        stalled = true;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    pc += (int)(word & kScanField);
+    pt += (int)((word >> 31) & kScanField);
+  }
+  if (__syncthreads_or(stalled)) {
+    // the same sums from the counts of the cells before the block
+    pc = pt = 0;
+    for (int c = threadIdx.x; c < blk * kScanTile; c += kScanBlock) {
+      pc += in[c];
+      const int2 tn = tile_at(ts, c);
+      int own = 0;
+      for (int r = 0; r < tn.y; ++r) own += in[c + r];
+      pt += own != 0;
+    }
+    if (threadIdx.x == 0) atomicAdd(&counters[C_SCANSTALL], 1ull);
+  }
+  block_sum2(pc, pt, lds);
+  ex += pc;
+  et += pt;
+  const int base = blk * kScanTile + threadIdx.x * kScanItems;
+#pragma unroll
+  for (int j = 0; j < kScanItems; ++j) {
+    if (base + j < len) out[base + j] = ex;
+    ex += v[j];
+    if (tl[j] >= 0) ts.list[et++] = tl[j];
+  }
+  if (threadIdx.x == 0) {
+    if (blk == 0) scal[S_NWALL] = 0;
+    if (blk == nb - 1) {
+      out[len] = pc + total;
+      scal[S_NTILES] = pt + tiles;
     }
   }
 }
@@ -401,12 +503,28 @@ __global__ void k_order(const int *__restrict__ cell_start, int ncells,
 // case: every occupied cell lies in a listed tile, so the other cells need
 // neither ordering nor clearing (their counts are 0).  A bounded grid strides
 // over the (tile, row) items.
+// groups > 0: also the chunk starts of a pooled walk of that many workgroups.
+// Listed tile t's work interval is [W0, W0 + own + kTileWork0) with W0 =
+// cell_start[its first cell] + kTileWork0 t (every cell before it lies in a
+// tile before it) and W = cell_start[ncells] + kTileWork0 ntiles; chunk g
+// starts at the tile whose interval holds ceil(g W / groups).
 __global__ void k_order_tiles(const int *__restrict__ cell_start, const int *__restrict__ scal,
                               const int *__restrict__ tile_list, int rows, int ncy, int ncells,
                               const uint32_t *__restrict__ seq, int *__restrict__ perm,
-                              int *__restrict__ cell_count) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) cell_count[ncells] = 0;  // the removed bucket
-  const int items = scal[S_NTILES] * rows;
+                              int *__restrict__ cell_count, int groups, int *__restrict__ bound,
+                              unsigned long long *__restrict__ scan_state, int scan_blocks) {
+  const int ntiles = scal[S_NTILES];
+  for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < scan_blocks; b += gridDim.x * blockDim.x)
+    scan_state[b] = 0;  // k_scan_single's words, for the next build
+  if (blockIdx.x == 0) {
+    if (threadIdx.x == 0) {
+      cell_count[ncells] = 0;  // the removed bucket
+      if (groups > 0) bound[groups] = ntiles;
+    }
+    if (groups > 0 && ntiles == 0)
+      for (int g = threadIdx.x; g < groups; g += blockDim.x) bound[g] = 0;
+  }
+  const int items = ntiles * rows;
   const int nty = (ncy + rows - 1) / rows;
   for (int item = blockIdx.x * blockDim.x + threadIdx.x; item < items; item += gridDim.x * blockDim.x) {
     const int it = item / rows, r = item - it * rows;
@@ -414,6 +532,14 @@ __global__ void k_order_tiles(const int *__restrict__ cell_start, const int *__r
     const int cx = tile / nty, cy = (tile - cx * nty) * rows + r;
     if (cy >= ncy) continue;
     const int c = cx * ncy + cy;
+    if (groups > 0 && r == 0) {
+      const int n = min(rows, ncy - cy);
+      const long long w0 = cell_start[c] + (long long)kTileWork0 * it;
+      const long long w1 = cell_start[c + n] + (long long)kTileWork0 * (it + 1);
+      const long long wt = cell_start[ncells] + (long long)kTileWork0 * ntiles;
+      const long long g0 = (w0 * groups + wt - 1) / wt, g1 = (w1 * groups + wt - 1) / wt;
+      for (long long g = g0; g < g1 && g < groups; ++g) bound[g] = it;
+    }
     cell_count[c] = 0;
     order_cell(cell_start, c, seq, perm);
   }
@@ -535,10 +661,16 @@ int scan_cells(tsph_ctx *c, hipStream_t s) {
   const int groups = pooled_groups(c);
   const TileScan ts{c->k.ncy, c->k.ncells, c->k.cell_base, c->cases, c->tile_base, c->ncases,
                     c->tile_rows, c->tile_list, groups, c->tile_bound};
-  hipLaunchKernelGGL(k_scan_partials, dim3(nb), dim3(kScanBlock), 0, s, c->cell_count, len,
-                     c->scan_tmp, ts);
-  hipLaunchKernelGGL(k_scan_final, dim3(nb), dim3(kScanBlock), 0, s, c->cell_count, len,
-                     c->scan_tmp, c->cell_start, c->scal, ts);
+  if (!c->cases) {
+    // one pass; k_order_tiles cuts the pooled chunks
+    hipLaunchKernelGGL(k_scan_single, dim3(nb), dim3(kScanBlock), 0, s, c->cell_count, len,
+                       c->scan_state, c->cell_start, c->scal, c->counters, ts);
+  } else {
+    hipLaunchKernelGGL(k_scan_partials, dim3(nb), dim3(kScanBlock), 0, s, c->cell_count, len,
+                       c->scan_tmp, ts);
+    hipLaunchKernelGGL(k_scan_final, dim3(nb), dim3(kScanBlock), 0, s, c->cell_count, len,
+                       c->scan_tmp, c->cell_start, c->scal, ts);
+  }
   TSPH_HIP(hipGetLastError());
   c->list_groups = groups;
   return 0;
@@ -570,7 +702,9 @@ int launch_cell_list(tsph_ctx *c, bool eos, bool predict, double dt, unsigned ad
                        c->cur.seq, c->perm, c->cell_count);
   else
     hipLaunchKernelGGL(k_order_tiles, dim3(2 * c->num_cu), dim3(nb), 0, s, c->cell_start, c->scal,
-                       c->tile_list, c->tile_rows, k.ncy, nct, c->cur.seq, c->perm, c->cell_count);
+                       c->tile_list, c->tile_rows, k.ncy, nct, c->cur.seq, c->perm, c->cell_count,
+                       c->list_groups, c->tile_bound, c->scan_state,
+                       (nct + 1 + kScanTile - 1) / kScanTile);
   const Mixed mx = c->prm.precision == TSPH_PREC_MIXED ? c->mx : Mixed{};
 #define TSPH_REORDER(E, P)                                                                    \
   hipLaunchKernelGGL((k_reorder<E, P>), dim3(g), dim3(nb), 0, s, c->cell_start, nct, c->perm,      \

@@ -33,7 +33,8 @@ enum { S_NP = 0, S_MAXSEQ = 1, S_NTILES = 2, S_NWALL = 3, S_BZCNT = 4, S_NSCAL =
 // device counter slots (uint64)
 enum {
   C_PAIRS = 0, C_CAND = 1, C_ACTIVE = 2, C_PASSES = 3, C_BADRHO = 4, C_OUTGRID = 5,
-  C_REMOVED = 6, C_APPENDED = 7, C_BZREJECT = 8, C_BZCHAIN = 9, C_REINIT = 10, C_NCOUNT = 11
+  C_REMOVED = 6, C_APPENDED = 7, C_BZREJECT = 8, C_BZCHAIN = 9, C_REINIT = 10, C_SCANSTALL = 11,
+  C_NCOUNT = 12
 };
 // The pair loops' counts (pairs, candidates, active) go to per-wavefront
 // slots after the counters above, 4 words per wavefront, and are summed when
@@ -167,6 +168,8 @@ struct tsph_ctx {
   int *hcell = nullptr;                // cell from the hash pass (pre-sort order)
   int *perm = nullptr;                 // sorted position -> old slot
   int *scan_tmp = nullptr;             // block partials of the scan
+  // single-pass scan: one published word per block (zero: not yet published)
+  unsigned long long *scan_state = nullptr;
   uint8_t *seqstate = nullptr;         // removal: per-seq state
   double *probe_out = nullptr;         // 4 doubles for probe results
   // pair-kernel timing: one event pair per pair launch while enabled
