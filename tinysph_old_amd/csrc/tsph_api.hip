@@ -319,7 +319,7 @@ static int alloc_context(tsph_ctx *c) {
   if ((e = dalloc(&c->scal, S_NSCAL)) ||
       (e = dalloc(&c->cell_count, nct + 1)) || (e = dalloc(&c->cell_start, nct + 2)) ||
       (e = dalloc(&c->cellof, cap)) || (e = dalloc(&c->hrank, cap)) || (e = dalloc(&c->hcell, cap)) ||
-      (e = dalloc(&c->perm, cap)) || (e = dalloc(&c->scan_tmp, 3 * ((nct + 1) / 1024 + 2))) ||
+      (e = dalloc(&c->perm, cap)) ||
       (e = dalloc(&c->seqstate, (size_t)c->seq_cap)) || (e = dalloc(&c->probe_out, 4)) ||
       (e = dalloc(&c->bz_list, 2 * kBzMax)) ||
       (e = dalloc(&c->tile_list, nct + 8)) || (e = dalloc(&c->wall_list, cap)) ||
@@ -413,7 +413,7 @@ int tsph_destroy(tsph_ctx *c) {
   free_soa(c->cur);
   free_soa(c->alt);
   void *ps[] = {c->scal, c->counters, c->cell_count, c->cell_start, c->cellof, c->hrank,
-                c->hcell, c->perm, c->scan_tmp, c->scan_state, c->seqstate, c->probe_out, c->mx.rv,
+                c->hcell, c->perm, c->scan_state, c->seqstate, c->probe_out, c->mx.rv,
                 c->mx.rp, c->mx.nrm, c->tile_list, c->wall_list, c->bz_list, c->cases,
                 c->tile_base, c->list_part, c->tile_bound};
   for (void *p : ps)

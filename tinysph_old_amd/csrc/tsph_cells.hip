@@ -35,6 +35,9 @@ constexpr int kScanBlock = 256;
 constexpr int kScanItems = 4;
 constexpr int kScanTile = kScanBlock * kScanItems;
 
+// blocks of the cell-list scan (ncells + 1 counts)
+inline int scan_blocks(const tsph_ctx *c) { return (c->ncells_tot + 1 + kScanTile - 1) / kScanTile; }
+
 __device__ __forceinline__ unsigned long long lanemask_lt() {
   return (1ull << (threadIdx.x & 63)) - 1ull;
 }
@@ -131,11 +134,11 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int *lds, int &total)
 
 // The pair loops' tiles (runs of `rows` cells of a grid column, per case) are
 // listed by the scan too: a tile belongs to the scan block that holds its
-// first cell.  Both passes stage the block's counts in LDS with the counts of
-// the next kMaxTileRows cells after it, mark the tiles whose first cell is in
-// the block (non-empty: one of their cells holds a particle); the first pass
-// counts them per block, the second lists them in tile order.  The counts are
-// cleared later, by k_order, so both passes read them intact.
+// first cell.  A block stages its counts in LDS with the counts of the next
+// kMaxTileRows cells after it, marks the tiles whose first cell is in the
+// block (non-empty: one of their cells holds a particle), and lists them in
+// tile order.  The counts are cleared later, by k_order, so the scan reads
+// them intact (the fallback of a stalled block included).
 struct TileScan {
   int ncy, ncells, cell_base;  // the single case (cases == nullptr)
   const Consts *cases;
@@ -143,8 +146,6 @@ struct TileScan {
   int ncases;
   int rows;
   int *list;
-  int groups;  // > 0: also cut the list into chunks for a pooled walk of this many workgroups
-  int *bound;  // chunk starts (groups + 1)
 };
 
 // {tile, cells of the tile} if cell c is a tile's first cell, else {-1, 0}
@@ -170,10 +171,6 @@ __device__ __forceinline__ int2 tile_at(const TileScan ts, int c) {
   return make_int2(t, min(ts.rows, ncy - cy));
 }
 
-// this thread's counts (v) and the block's counts + halo staged in s_cnt;
-// tl[j] >= 0: cell base + j starts a non-empty tile of work wk[j] (its own
-// particles + kTileWork0, the pooled walk's cost model); returns the number
-// of such tiles and their work (w)
 // the thread's counts (v) and, for the first kMaxTileRows threads, one count
 // of the next block (halo)
 __device__ __forceinline__ void load_counts(const int *__restrict__ in, int len, int block,
@@ -185,208 +182,168 @@ __device__ __forceinline__ void load_counts(const int *__restrict__ in, int len,
   halo = threadIdx.x < kMaxTileRows && c < len ? in[c] : 0;
 }
 
+// the block's counts + halo staged in s_cnt; tl[j] >= 0: cell base + j
+// starts a non-empty tile; returns the number of such tiles
 __device__ __forceinline__ int stage_counts(int len, int block, const TileScan ts,
                                             const int (&v)[kScanItems], int halo,
-                                            int (&tl)[kScanItems], int (&wk)[kScanItems], int &w,
-                                            int *s_cnt) {
+                                            int (&tl)[kScanItems], int *s_cnt) {
   const int b0 = block * kScanTile, base = b0 + threadIdx.x * kScanItems;
 #pragma unroll
   for (int j = 0; j < kScanItems; ++j) s_cnt[threadIdx.x * kScanItems + j] = v[j];
   if (threadIdx.x < kMaxTileRows) s_cnt[kScanTile + threadIdx.x] = halo;
   __syncthreads();
   int nt = 0;
-  w = 0;
 #pragma unroll
   for (int j = 0; j < kScanItems; ++j) {
     const int2 tn = base + j < len ? tile_at(ts, base + j) : make_int2(-1, 0);
     int own = 0;
     for (int r = 0; r < tn.y; ++r) own += s_cnt[threadIdx.x * kScanItems + j + r];
     tl[j] = own ? tn.x : -1;
-    wk[j] = own ? own + kTileWork0 : 0;
     nt += own != 0;
-    w += wk[j];
   }
   return nt;
 }
 
-// sums of the first pass over the blocks before this one (pre) and over all
-// blocks (all): cells, tiles, tile work
-__device__ __forceinline__ void block_offsets(const int *__restrict__ partial, int (&pre)[3],
-                                              int (&all)[3], int *lds) {
-  int v[6] = {0, 0, 0, 0, 0, 0};
-  for (int b = threadIdx.x; b < (int)gridDim.x; b += kScanBlock) {
-    const int pc = partial[3 * b], pt = partial[3 * b + 1], pw = partial[3 * b + 2];
-    if (b < (int)blockIdx.x) {
-      v[0] += pc;
-      v[1] += pt;
-      v[2] += pw;
-    }
-    v[3] += pc;
-    v[4] += pt;
-    v[5] += pw;
-  }
-#pragma unroll
-  for (int q = 0; q < 6; ++q)
-    for (int o = 32; o > 0; o >>= 1) v[q] += __shfl_xor(v[q], o, 64);
-  const int wid = threadIdx.x >> 6;
-  if ((threadIdx.x & 63) == 0)
-    for (int q = 0; q < 6; ++q) lds[6 * wid + q] = v[q];
-  __syncthreads();
-  for (int q = 0; q < 3; ++q) pre[q] = all[q] = 0;
-  for (int w = 0; w < kScanBlock / 64; ++w)
-    for (int q = 0; q < 3; ++q) {
-      pre[q] += lds[6 * w + q];
-      all[q] += lds[6 * w + 3 + q];
-    }
-  __syncthreads();
-}
-
-__global__ __launch_bounds__(kScanBlock) void k_scan_partials(const int *__restrict__ in, int len,
-                                                              int *__restrict__ partial, TileScan ts) {
-  __shared__ int lds[kScanBlock / 64];
-  __shared__ int s_cnt[kScanTile + kMaxTileRows];
-  int v[kScanItems], tl[kScanItems], wk[kScanItems], w, halo;
-  load_counts(in, len, blockIdx.x, v, halo);
-  const int nt = stage_counts(len, blockIdx.x, ts, v, halo, tl, wk, w, s_cnt);
-  int s = 0;
-#pragma unroll
-  for (int j = 0; j < kScanItems; ++j) s += v[j];
-  int total, tiles, work;
-  block_exclusive_scan<kScanBlock>(s, lds, total);
-  block_exclusive_scan<kScanBlock>(nt, lds, tiles);
-  block_exclusive_scan<kScanBlock>(w, lds, work);
-  if (threadIdx.x == 0) {
-    partial[3 * blockIdx.x] = total;
-    partial[3 * blockIdx.x + 1] = tiles;
-    partial[3 * blockIdx.x + 2] = work;
-  }
-}
-
-// second pass: cell offsets and the tile list (each block adds up the first
-// pass's sums of the blocks before it), and for a pooled walk the chunk
-// starts: chunk g starts at the tile whose work interval [W0, W1) holds
-// floor(g W / groups) (W: total work).  Block 0 writes out[len] (the total),
-// the tile count, the last chunk bound, and clears the wall-list counter the
-// reorder rebuilds.
-__global__ __launch_bounds__(kScanBlock) void k_scan_final(const int *__restrict__ in, int len,
-                                                           const int *__restrict__ partial,
-                                                           int *__restrict__ out,
-                                                           int *__restrict__ scal, TileScan ts) {
-  __shared__ int lds[6 * kScanBlock / 64];
-  __shared__ int s_cnt[kScanTile + kMaxTileRows];
-  // the block's counts are loaded first, so their latency overlaps the sums
-  // of the partials
-  int v[kScanItems], tl[kScanItems], wk[kScanItems], w, halo;
-  load_counts(in, len, blockIdx.x, v, halo);
-  int pre[3], all[3];
-  block_offsets(partial, pre, all, lds);
-  const int nt = stage_counts(len, blockIdx.x, ts, v, halo, tl, wk, w, s_cnt);
-  const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
-  int s = 0;
-#pragma unroll
-  for (int j = 0; j < kScanItems; ++j) s += v[j];
-  int total, tiles, work;
-  int ex = block_exclusive_scan<kScanBlock>(s, lds, total) + pre[0];
-  int et = block_exclusive_scan<kScanBlock>(nt, lds, tiles) + pre[1];
-  long long ew = block_exclusive_scan<kScanBlock>(w, lds, work) + pre[2];
-#pragma unroll
-  for (int j = 0; j < kScanItems; ++j) {
-    if (base + j < len) out[base + j] = ex;
-    ex += v[j];
-    if (tl[j] >= 0) {
-      if (ts.groups > 0) {
-        const long long g0 = (ew * ts.groups + all[2] - 1) / all[2];
-        const long long g1 = ((ew + wk[j]) * ts.groups + all[2] - 1) / all[2];
-        for (long long g = g0; g < g1 && g < ts.groups; ++g) ts.bound[g] = et;
-      }
-      ts.list[et++] = tl[j];
-      ew += wk[j];
-    }
-  }
-  if (blockIdx.x == 0) {
-    if (threadIdx.x == 0) {
-      out[len] = all[0];
-      scal[S_NTILES] = all[1];
-      scal[S_NWALL] = 0;
-    }
-    if (ts.groups > 0) {
-      if (all[1] == 0)
-        for (int g = threadIdx.x; g <= ts.groups; g += kScanBlock) ts.bound[g] = 0;
-      else if (threadIdx.x == 0)
-        ts.bound[ts.groups] = all[1];
-    }
-  }
-}
-
-// Single-pass form of the two kernels above, for a single-case grid.  Every
-// block publishes its sums (cells, tiles) in one 64-bit word (bits 0-30
-// cells, 31-61 tiles, 63 set), then adds up the words of all the blocks
-// before it.  k_order_tiles clears the words for the next build.  Blocks are
-// dispatched in index order, so a block waits only for blocks that have
-// started, and those publish before they wait themselves.  A block whose wait
-// exceeds kScanSpin polls all the same forms the sums from the counts of the
-// cells before it (and counts the stall): a late publisher costs time, never
-// correctness, and every wait ends.  The pooled walk's chunk bounds are cut
-// by k_order_tiles, from cell_start.
+// Single-pass scan with decoupled look-back.  Every block publishes its sums
+// (cells, tiles) in one 64-bit word (bits 0-30 cells, 31-61 tiles, 62 set
+// once the word holds the inclusive prefix instead of the block's own sums,
+// 63 set once written).  A block then walks back over the words of the blocks
+// before it, 1024 at a time (four per thread), until it meets an inclusive
+// prefix, adds up what it passed, and publishes its own inclusive prefix
+// (a grid of at most 1024 blocks reads all its words in one round and
+// publishes no inclusive prefixes).
+// k_scatter clears the words for the next build.  Blocks are dispatched in
+// index order, so a block waits only for blocks that have started, and those
+// publish their own sums before they wait themselves.  A block whose wait
+// exceeds kScanSpin polls forms its sums from the counts of the cells before
+// it instead (and counts the stall): a late publisher costs time, never
+// correctness, and every wait ends.
 constexpr int kScanSpin = 1 << 20;
+constexpr int kLookBack = 4;  // words per thread and look-back round
 constexpr unsigned long long kScanField = 0x7fffffffull;
+constexpr unsigned long long kScanSet = 1ull << 63, kScanIncl = 1ull << 62;
 
-__device__ __forceinline__ void block_sum2(int &a, int &b, int *lds) {
+__device__ __forceinline__ unsigned long long scan_word(int cells, int tiles, bool incl) {
+  return kScanSet | (incl ? kScanIncl : 0ull) | ((unsigned long long)tiles << 31) |
+         (unsigned long long)cells;
+}
+
+// block-wide sums of a and b, and the minimum of m
+__device__ __forceinline__ void block_reduce3(int &a, int &b, int &m, int *lds) {
   for (int o = 32; o > 0; o >>= 1) {
     a += __shfl_xor(a, o, 64);
     b += __shfl_xor(b, o, 64);
+    m = min(m, __shfl_xor(m, o, 64));
   }
   const int wid = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0) {
-    lds[2 * wid] = a;
-    lds[2 * wid + 1] = b;
+    lds[3 * wid] = a;
+    lds[3 * wid + 1] = b;
+    lds[3 * wid + 2] = m;
   }
   __syncthreads();
   a = b = 0;
+  m = INT_MAX;
   for (int w = 0; w < kScanBlock / 64; ++w) {
-    a += lds[2 * w];
-    b += lds[2 * w + 1];
+    a += lds[3 * w];
+    b += lds[3 * w + 1];
+    m = min(m, lds[3 * w + 2]);
   }
   __syncthreads();
 }
 
-__global__ __launch_bounds__(kScanBlock) void k_scan_single(const int *__restrict__ in, int len,
-                                                            unsigned long long *__restrict__ state,
-                                                            int *__restrict__ out,
-                                                            int *__restrict__ scal,
-                                                            unsigned long long *__restrict__ counters,
-                                                            TileScan ts) {
-  __shared__ int lds[2 * kScanBlock / 64];
+__global__ __launch_bounds__(kScanBlock) void k_scan(const int *__restrict__ in, int len,
+                                                     unsigned long long *__restrict__ state,
+                                                     int *__restrict__ out, int *__restrict__ scal,
+                                                     unsigned long long *__restrict__ counters,
+                                                     TileScan ts) {
+  __shared__ int lds[3 * kScanBlock / 64];
   __shared__ int s_cnt[kScanTile + kMaxTileRows];
   const int nb = gridDim.x, blk = blockIdx.x;
-  int v[kScanItems], tl[kScanItems], wk[kScanItems], w, halo;
+  int v[kScanItems], tl[kScanItems], halo;
   load_counts(in, len, blk, v, halo);
-  const int nt = stage_counts(len, blk, ts, v, halo, tl, wk, w, s_cnt);
+  const int nt = stage_counts(len, blk, ts, v, halo, tl, s_cnt);
   int s = 0;
 #pragma unroll
   for (int j = 0; j < kScanItems; ++j) s += v[j];
   int total, tiles;
   int ex = block_exclusive_scan<kScanBlock>(s, lds, total);
   int et = block_exclusive_scan<kScanBlock>(nt, lds, tiles);
+  // a grid that one look-back round covers needs no inclusive prefixes
+  const bool chain = nb > kLookBack * kScanBlock;
   if (threadIdx.x == 0)
-    __hip_atomic_store(&state[blk],
-                       (1ull << 63) | ((unsigned long long)tiles << 31) | (unsigned long long)total,
-                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&state[blk], scan_word(total, tiles, chain && blk == 0), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  // pc, pt = sums of the blocks before this one.  Look-back: a round reads
+  // the words of the kLookBack * kScanBlock nearest blocks not yet covered
+  // (entry j = thread + kScanBlock q: block hi - j), all loads in flight
+  // together.
   int pc = 0, pt = 0;
   bool stalled = false;
-  for (int b = threadIdx.x; b < blk && !stalled; b += kScanBlock) {
-    unsigned long long word = 0;
-    for (int spin = 0;; ++spin) {
-      word = __hip_atomic_load(&state[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if (word >> 63) break;
-      if (spin == kScanSpin) {
-        stalled = true;
-        break;
+  if (!chain) {
+    // one round over all the blocks before this one, farthest first (they
+    // published first)
+    for (int b = threadIdx.x; b < blk && !stalled; b += kScanBlock) {
+      unsigned long long word = 0;
+      for (int spin = 0;; ++spin) {
+        word = __hip_atomic_load(&state[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (word & kScanSet) break;
+        if (spin == kScanSpin) {
+          stalled = true;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
       }
-      __builtin_amdgcn_s_sleep(1);
+      pc += (int)(word & kScanField);
+      pt += (int)((word >> 31) & kScanField);
     }
-    pc += (int)(word & kScanField);
-    pt += (int)((word >> 31) & kScanField);
+    if (!__syncthreads_or(stalled)) {
+      int unused = 0;
+      block_reduce3(pc, pt, unused, lds);
+    }
+  }
+  for (int hi = chain ? blk - 1 : -1; hi >= 0;) {
+    unsigned long long word[kLookBack];
+#pragma unroll
+    for (int q = 0; q < kLookBack; ++q) {
+      const int b = hi - (int)threadIdx.x - kScanBlock * q;
+      // before block 0: an empty inclusive prefix
+      word[q] = b >= 0 ? __hip_atomic_load(&state[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                       : kScanSet | kScanIncl;
+    }
+#pragma unroll
+    for (int q = 0; q < kLookBack; ++q) {
+      const int b = hi - (int)threadIdx.x - kScanBlock * q;
+      for (int spin = 0; !(word[q] & kScanSet) && !stalled; ++spin) {
+        if (spin == kScanSpin) {
+          stalled = true;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+        word[q] = __hip_atomic_load(&state[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+    if (__syncthreads_or(stalled)) break;
+    // the nearest inclusive prefix ends the walk; the words up to it count
+    int f = INT_MAX, c = 0, t = 0;
+    if (chain) {
+#pragma unroll
+      for (int q = 0; q < kLookBack; ++q)
+        if (word[q] & kScanIncl) f = min(f, (int)threadIdx.x + kScanBlock * q);
+      block_reduce3(c, t, f, lds);
+    }
+#pragma unroll
+    for (int q = 0; q < kLookBack; ++q)
+      if ((int)threadIdx.x + kScanBlock * q <= f) {
+        c += (int)(word[q] & kScanField);
+        t += (int)((word[q] >> 31) & kScanField);
+      }
+    int unused = 0;
+    block_reduce3(c, t, unused, lds);
+    pc += c;
+    pt += t;
+    if (f != INT_MAX) break;
+    hi -= kLookBack * kScanBlock;
   }
   if (__syncthreads_or(stalled)) {
     // the same sums from the counts of the cells before the block
@@ -398,9 +355,13 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_single(const int *__restric
       for (int r = 0; r < tn.y; ++r) own += in[c + r];
       pt += own != 0;
     }
+    int unused = 0;
+    block_reduce3(pc, pt, unused, lds);
     if (threadIdx.x == 0) atomicAdd(&counters[C_SCANSTALL], 1ull);
   }
-  block_sum2(pc, pt, lds);
+  if (chain && threadIdx.x == 0 && blk > 0)
+    __hip_atomic_store(&state[blk], scan_word(pc + total, pt + tiles, true), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
   ex += pc;
   et += pt;
   const int base = blk * kScanTile + threadIdx.x * kScanItems;
@@ -419,14 +380,63 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_single(const int *__restric
   }
 }
 
+// chunk starts of a pooled walk of `groups` workgroups over the listed tiles.
+// Listed tile t's work interval is [W0, W0 + own + kTileWork0) with W0 =
+// cell_start[its first cell] + kTileWork0 t (every cell before it lies in a
+// tile before it: tiles and cells are both in (case, column, row) order) and
+// W = cell_start[ncells] + kTileWork0 ntiles; chunk g starts at the tile whose
+// interval holds ceil(g W / groups).
+__device__ __forceinline__ void cut_chunks(const int *__restrict__ cell_start, int ncells, int t,
+                                           int ntiles, int c0, int n, int groups,
+                                           int *__restrict__ bound) {
+  const long long w0 = cell_start[c0] + (long long)kTileWork0 * t;
+  const long long w1 = cell_start[c0 + n] + (long long)kTileWork0 * (t + 1);
+  const long long wt = cell_start[ncells] + (long long)kTileWork0 * ntiles;
+  const long long g0 = (w0 * groups + wt - 1) / wt, g1 = (w1 * groups + wt - 1) / wt;
+  for (long long g = g0; g < g1 && g < groups; ++g) bound[g] = t;
+}
+
+// the last chunk bound (and all of them for an empty list)
+__device__ __forceinline__ void close_chunks(int ntiles, int groups, int *__restrict__ bound) {
+  if (threadIdx.x == 0) bound[groups] = ntiles;
+  if (ntiles == 0)
+    for (int g = threadIdx.x; g < groups; g += blockDim.x) bound[g] = 0;
+}
+
+// cut_chunks for batched grids (one thread per listed tile)
+__global__ void k_tile_bounds(const int *__restrict__ cell_start, const int *__restrict__ scal,
+                              const int *__restrict__ tile_list, const Consts *__restrict__ cases,
+                              const int *__restrict__ tile_base, int ncases, int rows, int ncells,
+                              int groups, int *__restrict__ bound) {
+  const int ntiles = scal[S_NTILES];
+  if (blockIdx.x == 0) close_chunks(ntiles, groups, bound);
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < ntiles; t += gridDim.x * blockDim.x) {
+    const int tile = tile_list[t];
+    int lo = 0, hi = ncases;
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (tile_base[mid] <= tile) lo = mid;
+      else hi = mid;
+    }
+    const Consts &k = cases[lo];
+    const int nty = (k.ncy + rows - 1) / rows, lt = tile - tile_base[lo];
+    const int cx = lt / nty, cy = (lt - cx * nty) * rows;
+    cut_chunks(cell_start, ncells, t, ntiles, k.cell_base + cx * k.ncy + cy,
+               min(rows, k.ncy - cy), groups, bound);
+  }
+}
+
 // also marks the reference positions of removed particles (the trailing
-// bucket) absent in the removal states
+// bucket) absent in the removal states, and clears k_scan's words for the
+// next build
 __global__ void k_scatter(const int *__restrict__ scal, const int *__restrict__ hcell,
                           const int *__restrict__ hrank, const int *__restrict__ cell_start,
                           int *__restrict__ perm, int *__restrict__ cellof, int ncells_tot,
-                          const uint32_t *__restrict__ seq, uint8_t *__restrict__ seqstate) {
+                          const uint32_t *__restrict__ seq, uint8_t *__restrict__ seqstate,
+                          unsigned long long *__restrict__ scan_state, int scan_blocks) {
   const int n = scal[S_NP];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  for (int b = i; b < scan_blocks; b += gridDim.x * blockDim.x) scan_state[b] = 0;
   if (i >= n) return;
   const int c = hcell[i];
   if (c == ncells_tot) seqstate[seq[i]] = 0;
@@ -503,26 +513,16 @@ __global__ void k_order(const int *__restrict__ cell_start, int ncells,
 // case: every occupied cell lies in a listed tile, so the other cells need
 // neither ordering nor clearing (their counts are 0).  A bounded grid strides
 // over the (tile, row) items.
-// groups > 0: also the chunk starts of a pooled walk of that many workgroups.
-// Listed tile t's work interval is [W0, W0 + own + kTileWork0) with W0 =
-// cell_start[its first cell] + kTileWork0 t (every cell before it lies in a
-// tile before it) and W = cell_start[ncells] + kTileWork0 ntiles; chunk g
-// starts at the tile whose interval holds ceil(g W / groups).
+// groups > 0: also the chunk starts of a pooled walk of that many workgroups
+// (cut_chunks).
 __global__ void k_order_tiles(const int *__restrict__ cell_start, const int *__restrict__ scal,
                               const int *__restrict__ tile_list, int rows, int ncy, int ncells,
                               const uint32_t *__restrict__ seq, int *__restrict__ perm,
-                              int *__restrict__ cell_count, int groups, int *__restrict__ bound,
-                              unsigned long long *__restrict__ scan_state, int scan_blocks) {
+                              int *__restrict__ cell_count, int groups, int *__restrict__ bound) {
   const int ntiles = scal[S_NTILES];
-  for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < scan_blocks; b += gridDim.x * blockDim.x)
-    scan_state[b] = 0;  // k_scan_single's words, for the next build
   if (blockIdx.x == 0) {
-    if (threadIdx.x == 0) {
-      cell_count[ncells] = 0;  // the removed bucket
-      if (groups > 0) bound[groups] = ntiles;
-    }
-    if (groups > 0 && ntiles == 0)
-      for (int g = threadIdx.x; g < groups; g += blockDim.x) bound[g] = 0;
+    if (threadIdx.x == 0) cell_count[ncells] = 0;  // the removed bucket
+    if (groups > 0) close_chunks(ntiles, groups, bound);
   }
   const int items = ntiles * rows;
   const int nty = (ncy + rows - 1) / rows;
@@ -532,14 +532,8 @@ __global__ void k_order_tiles(const int *__restrict__ cell_start, const int *__r
     const int cx = tile / nty, cy = (tile - cx * nty) * rows + r;
     if (cy >= ncy) continue;
     const int c = cx * ncy + cy;
-    if (groups > 0 && r == 0) {
-      const int n = min(rows, ncy - cy);
-      const long long w0 = cell_start[c] + (long long)kTileWork0 * it;
-      const long long w1 = cell_start[c + n] + (long long)kTileWork0 * (it + 1);
-      const long long wt = cell_start[ncells] + (long long)kTileWork0 * ntiles;
-      const long long g0 = (w0 * groups + wt - 1) / wt, g1 = (w1 * groups + wt - 1) / wt;
-      for (long long g = g0; g < g1 && g < groups; ++g) bound[g] = it;
-    }
+    if (groups > 0 && r == 0)
+      cut_chunks(cell_start, ncells, it, ntiles, c, min(rows, ncy - cy), groups, bound);
     cell_count[c] = 0;
     order_cell(cell_start, c, seq, perm);
   }
@@ -603,10 +597,13 @@ __global__ void k_reorder(const int *__restrict__ cell_start, int ncells, const 
     dst.type[k] = t;
     dst.seq[k] = src.seq[j];
     dst.index[k] = src.index[j];
-    double p = src.p[j];
+    // with the EOS the old pressure is read only where it is kept
+    double p;
     if (EOS && t != TSPH_OUTLET && t != TSPH_INLET) {
       bad = rho > 1500. || rho < 0.;
       p = tait64(rho, kc.rho0, kc.cs);
+    } else {
+      p = src.p[j];
     }
     dst.p[k] = p;
     if (src.gnr) dst.gnr[k] = src.gnr[j];
@@ -657,20 +654,16 @@ __global__ void k_reorder(const int *__restrict__ cell_start, int ncells, const 
 // current tile_rows
 int scan_cells(tsph_ctx *c, hipStream_t s) {
   const int len = c->ncells_tot + 1;
-  const int nb = (len + kScanTile - 1) / kScanTile;
   const int groups = pooled_groups(c);
   const TileScan ts{c->k.ncy, c->k.ncells, c->k.cell_base, c->cases, c->tile_base, c->ncases,
-                    c->tile_rows, c->tile_list, groups, c->tile_bound};
-  if (!c->cases) {
-    // one pass; k_order_tiles cuts the pooled chunks
-    hipLaunchKernelGGL(k_scan_single, dim3(nb), dim3(kScanBlock), 0, s, c->cell_count, len,
-                       c->scan_state, c->cell_start, c->scal, c->counters, ts);
-  } else {
-    hipLaunchKernelGGL(k_scan_partials, dim3(nb), dim3(kScanBlock), 0, s, c->cell_count, len,
-                       c->scan_tmp, ts);
-    hipLaunchKernelGGL(k_scan_final, dim3(nb), dim3(kScanBlock), 0, s, c->cell_count, len,
-                       c->scan_tmp, c->cell_start, c->scal, ts);
-  }
+                    c->tile_rows, c->tile_list};
+  hipLaunchKernelGGL(k_scan, dim3(scan_blocks(c)), dim3(kScanBlock), 0, s, c->cell_count, len,
+                     c->scan_state, c->cell_start, c->scal, c->counters, ts);
+  // a single grid's chunks are cut by k_order_tiles
+  if (c->cases && groups > 0)
+    hipLaunchKernelGGL(k_tile_bounds, dim3(2 * c->num_cu), dim3(256), 0, s, c->cell_start,
+                       c->scal, c->tile_list, c->cases, c->tile_base, c->ncases, c->tile_rows,
+                       c->ncells_tot, groups, c->tile_bound);
   TSPH_HIP(hipGetLastError());
   c->list_groups = groups;
   return 0;
@@ -696,15 +689,15 @@ int launch_cell_list(tsph_ctx *c, bool eos, bool predict, double dt, unsigned ad
 #undef TSPH_HASH
   if (int e = scan_cells(c, s)) return e;
   hipLaunchKernelGGL(k_scatter, dim3(g), dim3(nb), 0, s, c->scal, c->hcell, c->hrank,
-                     c->cell_start, c->perm, c->cellof, nct, c->cur.seq, c->seqstate);
+                     c->cell_start, c->perm, c->cellof, nct, c->cur.seq, c->seqstate,
+                     c->scan_state, scan_blocks(c));
   if (c->cases)
     hipLaunchKernelGGL(k_order, dim3(grid_for(nct + 1, nb)), dim3(nb), 0, s, c->cell_start, nct,
                        c->cur.seq, c->perm, c->cell_count);
   else
     hipLaunchKernelGGL(k_order_tiles, dim3(2 * c->num_cu), dim3(nb), 0, s, c->cell_start, c->scal,
                        c->tile_list, c->tile_rows, k.ncy, nct, c->cur.seq, c->perm, c->cell_count,
-                       c->list_groups, c->tile_bound, c->scan_state,
-                       (nct + 1 + kScanTile - 1) / kScanTile);
+                       c->list_groups, c->tile_bound);
   const Mixed mx = c->prm.precision == TSPH_PREC_MIXED ? c->mx : Mixed{};
 #define TSPH_REORDER(E, P)                                                                    \
   hipLaunchKernelGGL((k_reorder<E, P>), dim3(g), dim3(nb), 0, s, c->cell_start, nct, c->perm,      \

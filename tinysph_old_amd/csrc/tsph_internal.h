@@ -167,8 +167,7 @@ struct tsph_ctx {
   int *hrank = nullptr;                // rank inside the cell from the hash pass
   int *hcell = nullptr;                // cell from the hash pass (pre-sort order)
   int *perm = nullptr;                 // sorted position -> old slot
-  int *scan_tmp = nullptr;             // block partials of the scan
-  // single-pass scan: one published word per block (zero: not yet published)
+  // cell-list scan: one published word per block (zero: not yet published)
   unsigned long long *scan_state = nullptr;
   uint8_t *seqstate = nullptr;         // removal: per-seq state
   double *probe_out = nullptr;         // 4 doubles for probe results
