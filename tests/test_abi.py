@@ -203,3 +203,36 @@ int main() {
     res = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", f"-I{ROOT}/include", str(src)],
                          capture_output=True, text=True)
     assert res.returncode == 0, res.stderr
+
+
+# header struct <-> ctypes mirror: every field at the same offset, same size
+STRUCTS = {"tsph_params": "Params", "tsph_particles": "Particles", "tsph_counters": "Counters",
+           "tsph_history": "History", "tsph_zone": "Zone", "tsph_case": "CaseC",
+           "tsph_ale_fields": "AleFields"}
+
+
+def test_struct_layouts_match_header(tmp_path):
+    lines = ["#include <stddef.h>", "#include <stdio.h>", '#include "tinysph.h"',
+             "int main(void) {"]
+    for cname, pyname in STRUCTS.items():
+        cls = getattr(L, pyname)
+        lines.append(f'  printf("{pyname} size %zu\\n", sizeof({cname}));')
+        for field, _ in cls._fields_:
+            lines.append(f'  printf("{pyname} {field} %zu\\n", offsetof({cname}, {field}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o",
+                    str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {}
+    for line in out:
+        if line:
+            cls_name, key, value = line.split()
+            got[(cls_name, key)] = int(value)
+    for pyname in STRUCTS.values():
+        cls = getattr(L, pyname)
+        assert got[(pyname, "size")] == C.sizeof(cls), pyname
+        for field, _ in cls._fields_:
+            assert got[(pyname, field)] == getattr(cls, field).offset, (pyname, field)
