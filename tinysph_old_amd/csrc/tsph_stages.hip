@@ -215,16 +215,29 @@ __device__ __forceinline__ void wall_sums(const Soa &d, const int *__restrict__ 
 // densities in the symplectic step (k_eos's count; the pass itself is not
 // needed there): the wall densities just computed are counted by the lanes
 // that computed them (wall_bad), every other particle by a grid-stride sweep.
+// The sweep loads kSweep particles per thread at a time, type and density
+// together, so it costs one memory round trip per kSweep grid strides.
+constexpr int kSweep = 4;
+
 __device__ __forceinline__ void count_bad_density(const Soa &d, const int *__restrict__ scal,
                                                   bool wall_bad,
                                                   unsigned long long *__restrict__ counters) {
   const int n = scal[S_NP];
+  const int stride = gridDim.x * blockDim.x;
   unsigned bad = wall_bad;
-  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
-    const int t = d.type[p];
-    if (t == TSPH_WALL || t == TSPH_INLET || t == TSPH_OUTLET) continue;
-    const double rho = d.rho[p];
-    bad += rho > 1500. || rho < 0.;
+  for (int p0 = blockIdx.x * blockDim.x + threadIdx.x; p0 < n; p0 += kSweep * stride) {
+    int t[kSweep];
+    double rho[kSweep];
+#pragma unroll
+    for (int q = 0; q < kSweep; ++q) {
+      const int p = p0 + q * stride;
+      t[q] = p < n ? d.type[p] : TSPH_WALL;
+      rho[q] = p < n ? d.rho[p] : 0.;
+    }
+#pragma unroll
+    for (int q = 0; q < kSweep; ++q)
+      if (t[q] != TSPH_WALL && t[q] != TSPH_INLET && t[q] != TSPH_OUTLET)
+        bad += rho[q] > 1500. || rho[q] < 0.;
   }
   for (int o = 32; o > 0; o >>= 1) bad += __shfl_xor(bad, o, 64);
   if (bad && (threadIdx.x & 63) == 0) atomicAdd(&counters[C_BADRHO], (unsigned long long)bad);
