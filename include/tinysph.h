@@ -163,9 +163,15 @@ int tsph_reset_counters(tsph_ctx *ctx);
  * column (1..24, default 7). */
 int tsph_set_tile_rows(tsph_ctx *ctx, int rows);
 int tsph_get_tile_rows(tsph_ctx *ctx, int *rows);
-/* Pair-loop waves kept resident per compute unit (1..32, default 8); the
- * waves walk the list of non-empty tiles. */
+/* Pair-loop waves launched per compute unit (1..32; default 12 on the mixed
+ * path, 8 on the FP64 path); the waves walk the list of non-empty tiles. */
 int tsph_set_waves_per_cu(tsph_ctx *ctx, int waves);
+/* Polls a cell-list scan block makes for the published sums of an earlier
+ * block before it forms its prefix from the cell counts itself (counted in
+ * tsph_counters.scan_stalls; results are the same either way).  Default 2^20;
+ * -1: never poll, always sum the counts (slow: for testing that path).  No
+ * counterpart in the reference. */
+int tsph_set_scan_wait(tsph_ctx *ctx, int polls);
 /* wait for all work queued on the context */
 int tsph_synchronize(tsph_ctx *ctx);
 
@@ -372,8 +378,9 @@ int tsph_autotune(tsph_ctx *ctx, int variant, int *tile_rows, int *waves_per_cu,
                   double *best_ms);
 /* Tile walk of the pair loops for the context's precision: 1 = pooled
  * (workgroups of up to 4 wavefronts, each with a chunk of the tile list of
- * about equal work whose tiles its wavefronts take from a counter in LDS),
- * 0 = static striding, one wavefront per workgroup (the default).  Grids with
+ * about equal work whose tiles its wavefronts take from a counter in LDS; the
+ * mixed path's default), 0 = static striding, one wavefront per workgroup (the
+ * FP64 path's default).  Grids with
  * fewer than 16 tiles per persistent wave always stride.  Results do not depend
  * on it.  No counterpart in the reference. */
 int tsph_set_balanced(tsph_ctx *ctx, int balanced);

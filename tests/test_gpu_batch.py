@@ -215,3 +215,23 @@ def test_ensemble_64_tanks_full_size():
     for q in small:
         ref, _ = run_single(cases[q], T.PREC_MIXED, steps, prep)
         assert_same(got[q], ref, f"tank {q}")
+
+
+def test_batch_scan_fallback_gives_same_state():
+    """The batched grid's scan with every block forming its prefix from the
+    cell counts (scan wait -1) gives the same steps as the polled scan."""
+    cases = T.build_ensemble(4, SEED + 5, 2000, 12000)
+    got = []
+    for wait in (1 << 20, -1):
+        with T.batch_for_cases(cases, T.PREC_MIXED) as bs:
+            bs.set_scan_wait(wait)
+            bs.set_waves_per_cu(1)
+            bs.set_balanced(True)
+            bs.particle_to_cell()
+            bs.step(T.WALL_MDBC, 5)
+            got.append(([bs.download_case(q, FIELDS) for q in range(len(cases))], bs.counters()))
+    (d0, n0), (d1, n1) = got
+    for q in range(len(cases)):
+        assert_same(d1[q], d0[q], f"case {q}")
+    assert n0["pairs"] == n1["pairs"]
+    assert n0["scan_stalls"] == 0 and n1["scan_stalls"] > 0

@@ -362,3 +362,41 @@ def test_mixed_density_error_after_1000_steps():
     print(f"[errors] fp64 {errs[T.PREC_FP64]} mixed {errs[T.PREC_MIXED]}", flush=True)
     assert errs[T.PREC_FP64]["rho"] <= 1e-8
     assert errs[T.PREC_MIXED]["rho"] <= 1e-4
+
+
+@pytest.mark.parametrize("precision", [T.PREC_FP64, T.PREC_MIXED], ids=["fp64", "mixed"])
+def test_scan_fallback_gives_same_cell_lists(precision):
+    """A cell-list scan block that outwaits its bounded poll for the sums of
+    the blocks before it forms its prefix from the cell counts instead.  Forced
+    for every block (scan wait -1), the cell lists, the pooled chunks (through
+    the forces and pair counts) and whole graph-replayed steps are bit-identical
+    to the polled scan's, and the stalls are counted."""
+    case = T.build_case(T.CASE_DAM_BREAK_MDBC, dx=0.004, jitter=0.01, seed=11)
+    print(f"[case] dam_break_mdbc dx=0.004 np={case.np} precision={precision}", flush=True)
+    outs = []
+    for wait in (1 << 20, -1):
+        with T.system_for_case(case, precision=precision) as ps:
+            ps.set_scan_wait(wait)
+            ps.set_waves_per_cu(1)  # enough tiles per wave for the pooled walk
+            ps.set_balanced(True)
+            ps.particle_to_cell()
+            cells = ps.cell_lists()
+            ps.mdbc_density()
+            ps.density_to_pressure()
+            ps.reset_counters()
+            ps.compute_forces(T.FORCES_WCSPH)
+            first = ps.counters()
+            forces = ps.download(("a", "drho"))
+            ps.step(case.wall, case.dt, 5)
+            state = ps.download(("r", "v", "rho", "p", "a", "drho"))
+            outs.append((cells, first, forces, state, ps.counters()))
+    (c0, f0, a0, s0, n0), (c1, f1, a1, s1, n1) = outs
+    assert np.array_equal(c0[0], c1[0]) and np.array_equal(c0[1], c1[1])
+    for key in ("pairs", "candidates", "active"):
+        assert f0[key] == f1[key], key
+    for k in a0:
+        assert np.array_equal(a0[k], a1[k]), k
+    for k in s0:
+        assert np.array_equal(s0[k], s1[k]), k
+    assert n0["scan_stalls"] == 0
+    assert n1["scan_stalls"] > 0

@@ -102,6 +102,7 @@ EXPORTS = {
     "tsph_set_tile_rows": (C.c_int, [C.c_void_p, C.c_int]),
     "tsph_get_tile_rows": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "tsph_set_waves_per_cu": (C.c_int, [C.c_void_p, C.c_int]),
+    "tsph_set_scan_wait": (C.c_int, [C.c_void_p, C.c_int]),
     "tsph_synchronize": (C.c_int, [C.c_void_p]),
     "tsph_pair_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "tsph_pair_timing_read": (C.c_int, [C.c_void_p, _dp, C.POINTER(C.c_int)]),

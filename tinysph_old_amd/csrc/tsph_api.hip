@@ -991,6 +991,14 @@ int tsph_set_waves_per_cu(tsph_ctx *c, int waves) {
   return 0;
 }
 
+int tsph_set_scan_wait(tsph_ctx *c, int polls) {
+  if (!c) return fail(TSPH_EINVAL, "null argument");
+  if (polls < -1) return fail(TSPH_EINVAL, "scan polls must be -1 or more");
+  if (polls != c->scan_wait) drop_graph(c);
+  c->scan_wait = polls;
+  return 0;
+}
+
 int tsph_get_tile_rows(tsph_ctx *c, int *rows) {
   if (!c || !rows) return fail(TSPH_EINVAL, "null argument");
   *rows = c->tile_rows;

@@ -32,11 +32,16 @@ namespace tsph {
 namespace {
 
 constexpr int kScanBlock = 256;
-constexpr int kScanItems = 4;
-constexpr int kScanTile = kScanBlock * kScanItems;
+// cells per thread of the scan: 4 for grids of up to kScanWide cells, 16 above
+// (so that grids of up to 4M cells keep at most 1024 blocks: one look-back round)
+constexpr int kScanWide = 1 << 20;
+inline int scan_items(const tsph_ctx *c) { return c->ncells_tot + 1 <= kScanWide ? 4 : 16; }
 
 // blocks of the cell-list scan (ncells + 1 counts)
-inline int scan_blocks(const tsph_ctx *c) { return (c->ncells_tot + 1 + kScanTile - 1) / kScanTile; }
+inline int scan_blocks(const tsph_ctx *c) {
+  const int tile = kScanBlock * scan_items(c);
+  return (c->ncells_tot + 1 + tile - 1) / tile;
+}
 
 __device__ __forceinline__ unsigned long long lanemask_lt() {
   return (1ull << (threadIdx.x & 63)) - 1ull;
@@ -148,56 +153,87 @@ struct TileScan {
   int *list;
 };
 
-// {tile, cells of the tile} if cell c is a tile's first cell, else {-1, 0}
-__device__ __forceinline__ int2 tile_at(const TileScan ts, int c) {
-  int cse = 0, cell_base = ts.cell_base, ncy = ts.ncy, ncells = ts.ncells;
-  if (ts.cases) {
-    int lo = 0, hi = ts.ncases;
-    while (hi - lo > 1) {
-      const int mid = (lo + hi) >> 1;
-      if (ts.cases[mid].cell_base <= c) lo = mid;
-      else hi = mid;
-    }
-    cse = lo;
-    cell_base = ts.cases[cse].cell_base;
-    ncy = ts.cases[cse].ncy;
-    ncells = ts.cases[cse].ncells;
+// the case whose cell range holds cell c (batched grids)
+__device__ __forceinline__ int case_of(const TileScan &ts, int c) {
+  int lo = 0, hi = ts.ncases;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (ts.cases[mid].cell_base <= c) lo = mid;
+    else hi = mid;
   }
-  const int lc = c - cell_base;
-  if (lc < 0 || lc >= ncells) return make_int2(-1, 0);
-  const int cx = lc / ncy, cy = lc - cx * ncy;
+  return lo;
+}
+
+// the grid geometry of one case, in registers: cells [cell_base, next) belong
+// to it (next: the following case's first cell)
+struct CaseGrid {
+  int cse, cell_base, ncy, ncells, tile_base, next;
+};
+
+__device__ __forceinline__ CaseGrid case_grid(const TileScan &ts, int cse) {
+  if (!ts.cases) return {0, ts.cell_base, ts.ncy, ts.ncells, 0, INT_MAX};
+  const Consts &k = ts.cases[cse];
+  return {cse, k.cell_base, k.ncy, k.ncells, ts.tile_base[cse],
+          cse + 1 < ts.ncases ? ts.cases[cse + 1].cell_base : INT_MAX};
+}
+
+// the grid of cell c, starting from g (cells ascending); a reload only at a
+// case boundary
+__device__ __forceinline__ void advance_grid(const TileScan &ts, int c, CaseGrid &g) {
+  if (c < g.next) return;
+  int cse = g.cse;
+  while (cse + 1 < ts.ncases && ts.cases[cse + 1].cell_base <= c) ++cse;
+  g = case_grid(ts, cse);
+}
+
+// {tile, cells of the tile} if cell c (in grid g) is a tile's first cell,
+// else {-1, 0}
+__device__ __forceinline__ int2 tile_at(const TileScan &ts, int c, const CaseGrid &g) {
+  const int lc = c - g.cell_base;
+  if (lc < 0 || lc >= g.ncells) return make_int2(-1, 0);
+  const int cx = lc / g.ncy, cy = lc - cx * g.ncy;
   if (cy % ts.rows) return make_int2(-1, 0);
-  const int t = (ts.cases ? ts.tile_base[cse] : 0) + cx * ((ncy + ts.rows - 1) / ts.rows) + cy / ts.rows;
-  return make_int2(t, min(ts.rows, ncy - cy));
+  const int t = g.tile_base + cx * ((g.ncy + ts.rows - 1) / ts.rows) + cy / ts.rows;
+  return make_int2(t, min(ts.rows, g.ncy - cy));
 }
 
 // the thread's counts (v) and, for the first kMaxTileRows threads, one count
 // of the next block (halo)
+template <int IT>
 __device__ __forceinline__ void load_counts(const int *__restrict__ in, int len, int block,
-                                            int (&v)[kScanItems], int &halo) {
-  const int b0 = block * kScanTile, base = b0 + threadIdx.x * kScanItems;
+                                            int (&v)[IT], int &halo) {
+  constexpr int kTile = kScanBlock * IT;
+  const int b0 = block * kTile, base = b0 + threadIdx.x * IT;
 #pragma unroll
-  for (int j = 0; j < kScanItems; ++j) v[j] = (base + j < len) ? in[base + j] : 0;
-  const int c = b0 + kScanTile + threadIdx.x;
+  for (int j = 0; j < IT; ++j) v[j] = (base + j < len) ? in[base + j] : 0;
+  const int c = b0 + kTile + threadIdx.x;
   halo = threadIdx.x < kMaxTileRows && c < len ? in[c] : 0;
 }
 
 // the block's counts + halo staged in s_cnt; tl[j] >= 0: cell base + j
-// starts a non-empty tile; returns the number of such tiles
+// starts a non-empty tile; returns the number of such tiles.  On a batched
+// grid one thread finds the case of the block's first cell (s_case) and every
+// thread steps from there to its own cells.
+template <int IT>
 __device__ __forceinline__ int stage_counts(int len, int block, const TileScan ts,
-                                            const int (&v)[kScanItems], int halo,
-                                            int (&tl)[kScanItems], int *s_cnt) {
-  const int b0 = block * kScanTile, base = b0 + threadIdx.x * kScanItems;
+                                            const int (&v)[IT], int halo, int (&tl)[IT],
+                                            int *s_cnt, int *s_case) {
+  constexpr int kTile = kScanBlock * IT;
+  const int b0 = block * kTile, base = b0 + threadIdx.x * IT;
 #pragma unroll
-  for (int j = 0; j < kScanItems; ++j) s_cnt[threadIdx.x * kScanItems + j] = v[j];
-  if (threadIdx.x < kMaxTileRows) s_cnt[kScanTile + threadIdx.x] = halo;
+  for (int j = 0; j < IT; ++j) s_cnt[threadIdx.x * IT + j] = v[j];
+  if (threadIdx.x < kMaxTileRows) s_cnt[kTile + threadIdx.x] = halo;
+  if (ts.cases && threadIdx.x == 0) *s_case = case_of(ts, b0);
   __syncthreads();
   int nt = 0;
+  CaseGrid g = case_grid(ts, ts.cases ? *s_case : 0);
+  advance_grid(ts, base, g);
 #pragma unroll
-  for (int j = 0; j < kScanItems; ++j) {
-    const int2 tn = base + j < len ? tile_at(ts, base + j) : make_int2(-1, 0);
+  for (int j = 0; j < IT; ++j) {
+    advance_grid(ts, base + j, g);
+    const int2 tn = base + j < len ? tile_at(ts, base + j, g) : make_int2(-1, 0);
     int own = 0;
-    for (int r = 0; r < tn.y; ++r) own += s_cnt[threadIdx.x * kScanItems + j + r];
+    for (int r = 0; r < tn.y; ++r) own += s_cnt[threadIdx.x * IT + j + r];
     tl[j] = own ? tn.x : -1;
     nt += own != 0;
   }
@@ -211,14 +247,17 @@ __device__ __forceinline__ int stage_counts(int len, int block, const TileScan t
 // before it, 1024 at a time (four per thread), until it meets an inclusive
 // prefix, adds up what it passed, and publishes its own inclusive prefix
 // (a grid of at most 1024 blocks reads all its words in one round and
-// publishes no inclusive prefixes).
+// publishes no inclusive prefixes).  Blocks cover 256 IT cells: IT = 16 above
+// 1M cells keeps grids of up to 4M cells in one round (the polls are
+// device-coherent loads, which miss the L2: a round costs one load per block
+// pair).
 // k_scatter clears the words for the next build.  Blocks are dispatched in
 // index order, so a block waits only for blocks that have started, and those
 // publish their own sums before they wait themselves.  A block whose wait
-// exceeds kScanSpin polls forms its sums from the counts of the cells before
-// it instead (and counts the stall): a late publisher costs time, never
-// correctness, and every wait ends.
-constexpr int kScanSpin = 1 << 20;
+// exceeds spin_limit polls (the context's scan_wait; -1: it does not poll)
+// forms its sums from the counts of the cells before it instead (and counts
+// the stall): a late publisher costs time, never correctness, and every wait
+// ends.
 constexpr int kLookBack = 4;  // words per thread and look-back round
 constexpr unsigned long long kScanField = 0x7fffffffull;
 constexpr unsigned long long kScanSet = 1ull << 63, kScanIncl = 1ull << 62;
@@ -252,20 +291,23 @@ __device__ __forceinline__ void block_reduce3(int &a, int &b, int &m, int *lds) 
   __syncthreads();
 }
 
+template <int IT>
 __global__ __launch_bounds__(kScanBlock) void k_scan(const int *__restrict__ in, int len,
                                                      unsigned long long *__restrict__ state,
                                                      int *__restrict__ out, int *__restrict__ scal,
                                                      unsigned long long *__restrict__ counters,
-                                                     TileScan ts) {
+                                                     TileScan ts, int spin_limit) {
   __shared__ int lds[3 * kScanBlock / 64];
-  __shared__ int s_cnt[kScanTile + kMaxTileRows];
+  constexpr int kTile = kScanBlock * IT;
+  __shared__ int s_cnt[kTile + kMaxTileRows];
+  __shared__ int s_case;
   const int nb = gridDim.x, blk = blockIdx.x;
-  int v[kScanItems], tl[kScanItems], halo;
-  load_counts(in, len, blk, v, halo);
-  const int nt = stage_counts(len, blk, ts, v, halo, tl, s_cnt);
+  int v[IT], tl[IT], halo;
+  load_counts<IT>(in, len, blk, v, halo);
+  const int nt = stage_counts<IT>(len, blk, ts, v, halo, tl, s_cnt, &s_case);
   int s = 0;
 #pragma unroll
-  for (int j = 0; j < kScanItems; ++j) s += v[j];
+  for (int j = 0; j < IT; ++j) s += v[j];
   int total, tiles;
   int ex = block_exclusive_scan<kScanBlock>(s, lds, total);
   int et = block_exclusive_scan<kScanBlock>(nt, lds, tiles);
@@ -286,12 +328,12 @@ __global__ __launch_bounds__(kScanBlock) void k_scan(const int *__restrict__ in,
     for (int b = threadIdx.x; b < blk && !stalled; b += kScanBlock) {
       unsigned long long word = 0;
       for (int spin = 0;; ++spin) {
-        word = __hip_atomic_load(&state[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (word & kScanSet) break;
-        if (spin == kScanSpin) {
+        if (spin > spin_limit) {
           stalled = true;
           break;
         }
+        word = __hip_atomic_load(&state[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (word & kScanSet) break;
         __builtin_amdgcn_s_sleep(1);
       }
       pc += (int)(word & kScanField);
@@ -308,14 +350,16 @@ __global__ __launch_bounds__(kScanBlock) void k_scan(const int *__restrict__ in,
     for (int q = 0; q < kLookBack; ++q) {
       const int b = hi - (int)threadIdx.x - kScanBlock * q;
       // before block 0: an empty inclusive prefix
-      word[q] = b >= 0 ? __hip_atomic_load(&state[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                       : kScanSet | kScanIncl;
+      word[q] = b < 0 ? kScanSet | kScanIncl
+                : spin_limit >= 0 ? __hip_atomic_load(&state[b], __ATOMIC_RELAXED,
+                                                      __HIP_MEMORY_SCOPE_AGENT)
+                                  : 0ull;
     }
 #pragma unroll
     for (int q = 0; q < kLookBack; ++q) {
       const int b = hi - (int)threadIdx.x - kScanBlock * q;
-      for (int spin = 0; !(word[q] & kScanSet) && !stalled; ++spin) {
-        if (spin == kScanSpin) {
+      for (int spin = 1; !(word[q] & kScanSet) && !stalled; ++spin) {
+        if (spin > spin_limit) {
           stalled = true;
           break;
         }
@@ -348,9 +392,9 @@ __global__ __launch_bounds__(kScanBlock) void k_scan(const int *__restrict__ in,
   if (__syncthreads_or(stalled)) {
     // the same sums from the counts of the cells before the block
     pc = pt = 0;
-    for (int c = threadIdx.x; c < blk * kScanTile; c += kScanBlock) {
+    for (int c = threadIdx.x; c < blk * kTile; c += kScanBlock) {
       pc += in[c];
-      const int2 tn = tile_at(ts, c);
+      const int2 tn = tile_at(ts, c, case_grid(ts, ts.cases ? case_of(ts, c) : 0));
       int own = 0;
       for (int r = 0; r < tn.y; ++r) own += in[c + r];
       pt += own != 0;
@@ -364,9 +408,9 @@ __global__ __launch_bounds__(kScanBlock) void k_scan(const int *__restrict__ in,
                        __HIP_MEMORY_SCOPE_AGENT);
   ex += pc;
   et += pt;
-  const int base = blk * kScanTile + threadIdx.x * kScanItems;
+  const int base = blk * kTile + threadIdx.x * IT;
 #pragma unroll
-  for (int j = 0; j < kScanItems; ++j) {
+  for (int j = 0; j < IT; ++j) {
     if (base + j < len) out[base + j] = ex;
     ex += v[j];
     if (tl[j] >= 0) ts.list[et++] = tl[j];
@@ -657,8 +701,12 @@ int scan_cells(tsph_ctx *c, hipStream_t s) {
   const int groups = pooled_groups(c);
   const TileScan ts{c->k.ncy, c->k.ncells, c->k.cell_base, c->cases, c->tile_base, c->ncases,
                     c->tile_rows, c->tile_list};
-  hipLaunchKernelGGL(k_scan, dim3(scan_blocks(c)), dim3(kScanBlock), 0, s, c->cell_count, len,
-                     c->scan_state, c->cell_start, c->scal, c->counters, ts);
+  if (scan_items(c) == 4)
+    hipLaunchKernelGGL(k_scan<4>, dim3(scan_blocks(c)), dim3(kScanBlock), 0, s, c->cell_count,
+                       len, c->scan_state, c->cell_start, c->scal, c->counters, ts, c->scan_wait);
+  else
+    hipLaunchKernelGGL(k_scan<16>, dim3(scan_blocks(c)), dim3(kScanBlock), 0, s, c->cell_count,
+                       len, c->scan_state, c->cell_start, c->scal, c->counters, ts, c->scan_wait);
   // a single grid's chunks are cut by k_order_tiles
   if (c->cases && groups > 0)
     hipLaunchKernelGGL(k_tile_bounds, dim3(2 * c->num_cu), dim3(256), 0, s, c->cell_start,

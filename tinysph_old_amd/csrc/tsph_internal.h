@@ -167,8 +167,10 @@ struct tsph_ctx {
   int *hrank = nullptr;                // rank inside the cell from the hash pass
   int *hcell = nullptr;                // cell from the hash pass (pre-sort order)
   int *perm = nullptr;                 // sorted position -> old slot
-  // cell-list scan: one published word per block (zero: not yet published)
+  // cell-list scan: one published word per block (zero: not yet published),
+  // and the polls a block makes for an earlier block's word (-1: none)
   unsigned long long *scan_state = nullptr;
+  int scan_wait = 1 << 20;
   uint8_t *seqstate = nullptr;         // removal: per-seq state
   double *probe_out = nullptr;         // 4 doubles for probe results
   // pair-kernel timing: one event pair per pair launch while enabled

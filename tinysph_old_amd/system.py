@@ -434,6 +434,9 @@ class ParticleSystem:
     def set_waves_per_cu(self, waves: int):
         L.check(self.lib.tsph_set_waves_per_cu(self.ctx, waves), "tsph_set_waves_per_cu")
 
+    def set_scan_wait(self, polls: int):
+        L.check(self.lib.tsph_set_scan_wait(self.ctx, polls), "tsph_set_scan_wait")
+
     @property
     def tile_rows(self) -> int:
         r = C.c_int()
