@@ -400,3 +400,48 @@ def test_scan_fallback_gives_same_cell_lists(precision):
         assert np.array_equal(s0[k], s1[k]), k
     assert n0["scan_stalls"] == 0
     assert n1["scan_stalls"] > 0
+
+
+@pytest.mark.parametrize("precision", [T.PREC_FP64, T.PREC_MIXED], ids=["fp64", "mixed"])
+def test_empty_system(precision):
+    """No particles: every stage and whole graph-replayed steps run, count
+    nothing and leave an empty system."""
+    case = make("dam_break_mdbc")
+    empty = {k: np.asarray(v)[:0] for k, v in case.arrays.items() if v is not None}
+    with T.ParticleSystem(case.params, precision=precision, capacity=64) as ps:
+        ps.upload(empty)
+        ps.particle_to_cell()
+        ps.mdbc_density()
+        ps.density_to_pressure()
+        ps.compute_forces(T.FORCES_WCSPH)
+        ps.step(case.wall, case.dt, 3)
+        assert ps.np == 0
+        cnt = ps.counters()
+        for key in ("pairs", "candidates", "active", "outside_grid", "removed", "scan_stalls"):
+            assert cnt[key] == 0, key
+        start, items = ps.cell_lists()
+        assert not start.any() and len(items) == 0
+        assert len(ps.download(("r", "rho"))["rho"]) == 0
+
+
+def test_particle_outside_grid_is_counted_and_binned():
+    """A particle beyond the cell grid (undefined behaviour in the reference,
+    src/SPH_find_pairs.h:64-69) is counted, binned in the nearest edge cell and
+    paired like any other; every particle appears in the cell lists once."""
+    case = make("dam_break_mdbc")
+    arrays = {k: (np.array(v, copy=True) if v is not None else None) for k, v in case.arrays.items()}
+    fluid = np.flatnonzero(np.asarray(arrays["type"]) == T.FLUID)
+    r = arrays["r"].reshape(-1, 2)
+    r[fluid[0]] = (case.params["x_m"] + 1.0, case.params["y_m"] + 1.0)
+    with T.ParticleSystem(case.params, precision=T.PREC_FP64, capacity=case.np) as ps:
+        ps.upload(arrays)
+        ps.particle_to_cell()
+        cnt = ps.counters()
+        assert cnt["outside_grid"] == 1
+        start, items = ps.cell_lists()
+        assert start[-1] == case.np
+        assert np.array_equal(np.sort(items), np.arange(case.np))
+        ps.density_to_pressure()
+        ps.compute_forces(T.FORCES_WCSPH)
+        out = ps.download(("a", "drho"))
+        assert np.isfinite(out["a"]).all() and np.isfinite(out["drho"]).all()
