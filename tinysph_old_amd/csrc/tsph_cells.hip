@@ -248,9 +248,7 @@ __device__ __forceinline__ int stage_counts(int len, int block, const TileScan t
 // prefix, adds up what it passed, and publishes its own inclusive prefix
 // (a grid of at most 1024 blocks reads all its words in one round and
 // publishes no inclusive prefixes).  Blocks cover 256 IT cells: IT = 16 above
-// 1M cells keeps grids of up to 4M cells in one round (the polls are
-// device-coherent loads, which miss the L2: a round costs one load per block
-// pair).
+// 1M cells keeps grids of up to 4M cells in one round.
 // k_scatter clears the words for the next build.  Blocks are dispatched in
 // index order, so a block waits only for blocks that have started, and those
 // publish their own sums before they wait themselves.  A block whose wait
