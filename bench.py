@@ -235,8 +235,8 @@ def main() -> int:
                     help="run only the --workloads legs (for profiling them alone)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: every GPU box of this build has one MI355X, and the "
-              "pipeline is single-GPU (multi-GPU sharding is design-only)", file=sys.stderr)
+        print("bench.py measures one GPU (--gpus 1): the pipeline runs on a single MI355X; "
+              "sharding across GPUs is designed, not built", file=sys.stderr)
         return 2
 
     legs = {"still_water": (leg_still_water, 200), "ale": (leg_ale, 50),

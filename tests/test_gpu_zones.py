@@ -3,8 +3,8 @@ src/SPH_buffer_zones.h) against the oracle on the same inputs.
 
 Integer results (types, reference indices, particle counts, the order of
 appended particles) are compared exactly; FP64 fields within 1e-9 relative
-(1e-6 on p), as for the other FP64 parity tests.  The 4M-particle channel
-(BASELINE config 5 size) is checked through size-independent properties:
+(1e-6 on p), as for the other FP64 parity tests.  The inflow channel at its
+full size (4M particles) is checked through size-independent properties:
 particle bookkeeping (live = initial + appended - removed), no refused or
 chained appends, and the inflow rate against its closed form.
 """
