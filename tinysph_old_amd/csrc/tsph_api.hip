@@ -69,8 +69,9 @@ static void drop_graph(tsph_ctx *c) {
 }
 
 // default pair-loop launch: 7-cell tiles; the mixed path's shorter tiles hide
-// more latency with 12 waves per CU in the pooled walk (C2: 0.082 ms against
-// 0.088 ms for the static walk at 8), the FP64 path's larger LDS records allow
+// more latency with 12 waves per CU in the pooled walk (740k-particle dam
+// break: 0.080 ms against 0.088 ms for the static walk at 8; the autotuner
+// picks the same), the FP64 path's larger LDS records allow
 // 8 waves per CU, where the static walk is the faster
 static void set_walk_defaults(tsph_ctx *c) {
   const bool mixed = c->prm.precision == TSPH_PREC_MIXED;
