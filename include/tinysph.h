@@ -256,6 +256,13 @@ int tsph_bz_integrate_inlet(tsph_ctx *ctx, int zone, double dt);
  * becomes the zone type (a = 0, drho = 0); zone particles past the zone end or
  * below dp/2 are erased, with the reference's in-loop skip of the next particle. */
 int tsph_bz_integrate_outlet(tsph_ctx *ctx, int zone, double dt);
+/* Relabels the reference positions of the live particles 0..np-1, keeping
+ * their order.  No counterpart in the reference, where positions are vector
+ * indices that every erase renumbers (src/SPH_particle_system.h:125-145).
+ * Inlet appends take fresh positions and the library relabels by itself
+ * before they run out (tsph_bz_integrate_inlet, tsph_step_inflow); results
+ * are unchanged by a relabel. */
+int tsph_renumber(tsph_ctx *ctx);
 /* nsteps whole steps of the inflow driver
  * (cases/inflowSlab_mDBC/SPH_simulationSymplectic.h:57-123): the symplectic step
  * with BZ_Integrate_inlet / _outlet after the predictor (zone -1: none). */
@@ -328,6 +335,17 @@ int tsph_format_vtk(const tsph_particles *host, const char *path);
 int tsph_format_interpolated_vtk(int ncx, int ncy, double dh, double x0, double y0,
                                  const double *rho, const double *p, const double *v,
                                  const char *path);
+/* write_mesh_to_ASCII_VTK, src/SPH_grid_to_vtk.h:6-67: the cell grid of the
+ * last Particle_To_Cell with each cell's particle count; x_0, y_0 are the
+ * driver's (padded) domain origin, Simulation_data's x_0 and y_0. */
+int tsph_write_grid_vtk(tsph_ctx *ctx, double x_0, double y_0, const char *path);
+/* The same file from host counts (count of cell y + x*ncy). */
+int tsph_format_grid_vtk(int ncx, int ncy, double kh, double of, double x_0, double y_0,
+                         const int32_t *counts, const char *path);
+/* Output_hydrostatic_pressure, src/SPH_measure_pressure.h:17-34: per fluid
+ * particle in reference order, "p/(1000*9.81*0.5) y/0.5". */
+int tsph_write_hydrostatic_pressure(tsph_ctx *ctx, const char *path);
+int tsph_format_hydrostatic_pressure(const tsph_particles *host, const char *path);
 /* Binary FP64 snapshot of every per-particle field in reference order (with
  * the ALE variables and the history fields when present), and its reload into
  * a context created with the same tsph_params.  After a reload, call
@@ -403,6 +421,10 @@ int tsph_set_balanced(tsph_ctx *ctx, int balanced);
                                        water moving at 1 m/s over an mDBC floor, inlet at
                                        x = 0 (zone 0), outlet at x = 7.9 (zone 1), dx = 0.5 mm
                                        (about 4M particles) */
+#define TSPH_CASE_DAM_BREAK_MDBC_REF 7 /* cases/damBreakLong_mDBC/{constants,geometry}.h as
+                                       they stand: 1.61x1.0 m tank, 0.6x0.3 m column, dx = 5 mm,
+                                       c0 = 34.3 m/s, dt = 5e-5 s scaled with dx; the tank of
+                                       the reference's water-level gauges and wall sensors */
 
 /* time-stepping scheme of a preset */
 #define TSPH_SCHEME_SYMPLECTIC 0 /* WCSPH, tsph_step_symplectic */

@@ -28,9 +28,13 @@
 // followed by tsph_mark_host_dirty(particles).  Errors throw tsph_error.
 #pragma once
 
+#include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
+#include <fstream>
+#include <iostream>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -142,7 +146,11 @@ struct Particle_system {
   // device side
   tsph_ctx *dev = nullptr;
   int precision = TSPH_PREC_FP64;
-  int device_capacity = 0;  // 0: the particle count at Divide_To_Cells
+  // particle slots on the device; 0: the particle count at Divide_To_Cells,
+  // plus max(4096, np/4) slots for inlet appends when buffer zones exist
+  int device_capacity = 0;
+  uint64_t removed_seen = 0;   // device removal count already added to np_out
+  uint64_t rejected_seen = 0;  // device count of refused inlet appends already reported
   bool host_dirty = true;   // host vectors newer than the device
   bool dev_newer = false;   // device newer than the host vectors
   size_t zones_on_device = 0;
@@ -328,7 +336,13 @@ inline void tsph_sync_host(Particle_system &ps) {
     for (int i = 0; i < m; ++i) o[i] = {f[2 * i], f[2 * i + 1]};
     return o;
   };
-  ps.np_out += ps.np - (unsigned)m;
+  // np_out counts removals (src/SPH_particle_system.h:115), never appends:
+  // take the device's removal count since the last synchronisation
+  tsph_counters cnt{};
+  tsph_check(tsph_get_counters(ps.dev, &cnt), "tsph_get_counters");
+  if (cnt.removed < ps.removed_seen) ps.removed_seen = 0;  // counters were reset
+  ps.np_out += (unsigned)(cnt.removed - ps.removed_seen);
+  ps.removed_seen = cnt.removed;
   ps.np = (unsigned)m;
   ps.data.part_type.assign(type.begin(), type.begin() + m);
   ps.data.index.assign(index.begin(), index.begin() + m);
@@ -383,10 +397,15 @@ inline void Divide_To_Cells(Particle_system &ps, Simulation_data sd) {
   prm.dp = ps.data_const.dp;
   prm.x_0 = sd.x_0; prm.x_m = sd.x_m; prm.y_0 = sd.y_0; prm.y_m = sd.y_m;
   prm.precision = ps.precision;
-  prm.capacity = ps.device_capacity > 0 ? ps.device_capacity : (int)(ps.np > 0 ? ps.np : 1);
+  const int n0 = (int)(ps.np > 0 ? ps.np : 1);
+  // an inlet appends up to one buffer layer per pass, and appended particles
+  // stay until an outlet or the domain bound removes them
+  const int room = ps.zones.empty() ? 0 : std::max(4096, n0 / 4);
+  prm.capacity = ps.device_capacity > 0 ? ps.device_capacity : n0 + room;
   if (ps.dev) tsph_destroy(ps.dev);
   ps.dev = nullptr;
   ps.zones_on_device = 0;
+  ps.removed_seen = ps.rejected_seen = 0;
   tsph_check(tsph_create(&prm, &ps.dev), "tsph_create");
   int ncx = 0, ncy = 0;
   tsph_check(tsph_grid(ps.dev, &ncx, &ncy), "tsph_grid");
@@ -611,11 +630,79 @@ inline void GenerateInterpol(Particle_system &ps, Simulation_data, std::string f
              "GenerateInterpol");
 }
 
-// src/SPH_buffer_zones.h:10-107
+// write_mesh_to_ASCII_VTK, src/SPH_grid_to_vtk.h:6-67 (the reference takes the
+// system by value; the cell counts are the device's last Particle_To_Cell)
+inline void write_mesh_to_ASCII_VTK(Particle_system &ps, Simulation_data sd, std::string filename) {
+  tsph_push(ps);
+  tsph_check(tsph_write_grid_vtk(ps.dev, sd.x_0, sd.y_0, filename.c_str()), "write_mesh_to_ASCII_VTK");
+}
+
+// Output_hydrostatic_pressure, src/SPH_measure_pressure.h:17-34
+inline void Output_hydrostatic_pressure(Particle_system &ps, std::string file_name) {
+  tsph_push(ps);
+  tsph_check(tsph_write_hydrostatic_pressure(ps.dev, file_name.c_str()), "Output_hydrostatic_pressure");
+}
+
+// ---- the drivers' text output, src/SPH_output_info.h (host only) ----
+// Output_file, src/SPH_output_info.h:8-43: the constants and the padded domain
+inline void Output_file(SPH_constants constants, Simulation_data simulation_data,
+                        std::string info_file_name) {
+  std::ofstream f(info_file_name);
+  auto line = [&f](const char *label, real value) { f << label << value << '\n'; };
+  f << "--- SIMULATION INFO --- " << '\n' << "\n" << "-> SPH constants: " << '\n';
+  line("Alpha: ", constants.avisc);
+  line("Cs: ", constants.cs);
+  line("Dp: ", constants.dp);
+  f << "External forces: [" << constants.graviy.x << "," << constants.graviy.y << "]" << '\n';
+  line("h: ", constants.h);
+  line("Kap: ", constants.kap);
+  line("m: ", constants.m);
+  line("rho0: ", constants.rho0);
+  f << "\n" << "-> Simulation info: " << '\n';
+  line("h = h(dp): ", simulation_data.hh);
+  line("kh: ", simulation_data.kh);
+  f << "Number of virual layers (for walls) nvl: " << simulation_data.nvl << '\n';
+  f << "\n" << "-> Domain size (with respect to cells): " << '\n';
+  f << "[x_min, y_min]: [" << simulation_data.x_0 << " , " << simulation_data.y_0 << "] " << '\n';
+  f << "[x_max, y_max]: [" << simulation_data.x_m << " , " << simulation_data.y_m << "] " << '\n';
+  f.close();
+  std::cout << "Info file generated." << std::endl;
+}
+// src/SPH_output_info.h:46-56 (empty in the reference)
+inline void Add_to_output_file(std::string) {}
+inline void Write_basic_simulation_info() {}
+// src/SPH_output_info.h:58-85: one "time value" line appended per call
+inline void tsph_append_series(const std::string &path, real t, real value) {
+  std::ofstream f(path, std::ios_base::app);
+  f << t << " " << value << '\n';
+}
+inline void Write_water_level(real step, real wl, std::string output_file_name_wl) {
+  tsph_append_series(output_file_name_wl, step, wl);
+}
+inline void Write_pressure(real step, real p, std::string output_file_name_p) {
+  tsph_append_series(output_file_name_p, step, p);
+}
+inline void Write_kinetic_energy(real step, real p, std::string output_file_name_p) {
+  tsph_append_series(output_file_name_p, step, p);
+}
+
+// src/SPH_buffer_zones.h:10-107.  The reference appends without bound; the
+// device has a fixed number of slots (device_capacity), so a pass whose appends
+// do not fit is refused whole and reported here as an error instead of letting
+// the inlet silently stop feeding.
 inline void BZ_Integrate_inlet(Particle_system &ps, unsigned int zoneIndex, double dt) {
   tsph_push(ps);
   tsph_check(tsph_bz_integrate_inlet(ps.dev, (int)zoneIndex, dt), "BZ_Integrate_inlet");
   tsph_ran(ps);
+  tsph_counters cnt{};
+  tsph_check(tsph_get_counters(ps.dev, &cnt), "tsph_get_counters");
+  if (cnt.zone_rejected < ps.rejected_seen) ps.rejected_seen = 0;  // counters were reset
+  if (cnt.zone_rejected > ps.rejected_seen) {
+    const uint64_t k = cnt.zone_rejected - ps.rejected_seen;
+    ps.rejected_seen = cnt.zone_rejected;
+    throw tsph_error("BZ_Integrate_inlet: " + std::to_string(k) +
+                     " inlet appends refused (device capacity; raise Particle_system::device_capacity)");
+  }
 }
 // src/SPH_buffer_zones.h:111-206
 inline void BZ_Integrate_outlet(Particle_system &ps, unsigned int zoneIndex, double dt) {

@@ -51,6 +51,7 @@
 #include "SPH_buffer_zones.h"
 #include "SPH_data_to_vtk.h"
 #include "SPH_grid_to_vtk.h"
+#include "SPH_output_info.h"
 #endif
 
 namespace {
@@ -347,6 +348,31 @@ void ref_write_grid_vtk(int ncx, int ncy, double kh, double x0, double y0, const
   for (int t = 0; t < n; ++t) vv[t] = {v[2 * t], v[2 * t + 1]};
   std::vector<real> rr(rho, rho + n), pp(p, p + n);
   write_INTERPOLATED_mesh_to_ASCII_VTK(ncx, ncy, kh, x0, y0, vv, rr, pp, std::string(path));
+}
+// write_mesh_to_ASCII_VTK, src/SPH_grid_to_vtk.h:6-67 (cell counts of the last
+// Particle_To_Cell)
+void ref_write_mesh_vtk(void* vc, const char* path) {
+  RefCase* c = static_cast<RefCase*>(vc);
+  write_mesh_to_ASCII_VTK(c->ps, c->sd, std::string(path));
+}
+// Output_hydrostatic_pressure, src/SPH_measure_pressure.h:17-34
+void ref_output_hydrostatic_pressure(void* vc, const char* path) {
+  Output_hydrostatic_pressure(static_cast<RefCase*>(vc)->ps, std::string(path));
+}
+// Output_file, src/SPH_output_info.h:8-43, with the case's padded domain and
+// the given nvl
+void ref_output_file(void* vc, int nvl, const char* path) {
+  RefCase* c = static_cast<RefCase*>(vc);
+  Simulation_data sd = c->sd;
+  sd.nvl = nvl;
+  Output_file(c->ps.data_const, sd, std::string(path));
+}
+// Write_water_level / Write_pressure, src/SPH_output_info.h:58-76
+void ref_write_water_level(double t, double value, const char* path) {
+  Write_water_level(t, value, std::string(path));
+}
+void ref_write_pressure(double t, double value, const char* path) {
+  Write_pressure(t, value, std::string(path));
 }
 #endif
 void ref_create_block_of_fluid_wp(void* vc, double x0, double y0, double x1, double y1) {

@@ -1,0 +1,211 @@
+"""The reference's own measurements after many steps, GPU against the oracle.
+
+The mDBC dam break of cases/damBreakLong_mDBC (1.61 x 1.0 m tank, 0.6 x 0.3 m
+column) is run as its constants.h sets it up (dx = 5 mm, dt = 5e-5 s) for
+10000 steps, to t = 0.5 s: the front crosses the tank and hits the right wall
+at about t = 0.45 s.  Every 50 steps
+the eight water-level gauges of the driver are read
+(cases/damBreakLong_mDBC/SPH_simulationSymplectic.h:156-186, constants.h:58-83:
+Water_Elevation at x = 0.3 / 0.865 / 1.114 / 1.3625 m from y = 0, and
+Water_Elevation_inverse from y = 0.45 m, the last at wl_h4_pos.x as the driver
+has it), and every 10 steps the four wall sensors (:188-201, constants.h:85-94:
+Pressure_in_point at x = 1.61 m, y = 3 / 15 / 30 / 80 mm), in the driver's
+units (level / 0.3 m; pressure / (1000 * 9.81 * 0.3) Pa).
+
+Tolerances.  The FP64 path differs from the oracle only where the device pow()
+differs from glibc's (a few ulp per call); a dam break amplifies such
+differences (measured on the oracle alone: one ulp of every fluid density at
+t = 0 moves the wall pressures by 3e-3 of their scale at t = 0.5 s, and by 40 %
+at t = 0.58 s after the impact).  So:
+  * water levels: the gauge marches in steps that grow quadratically
+    (src/SPH_water_level.h:93-109), so a level can only move by whole steps;
+    every sample must agree to within one marching step at its height;
+  * wall pressures up to t = 0.5 s: within 1e-2 of the driver's pressure scale
+    (FP64) and 5e-2 (mixed);
+  * particle fields at t = 0.5 s: relative L2 bounds stated in each test.
+
+The still-water tank of the first workload (1.0 m, 0.5 m of water, dx = 10 mm,
+2000 steps) is compared field by field and through Output_hydrostatic_pressure
+(src/SPH_measure_pressure.h:17-34), the profile the reference writes for its
+still-water validation (cases/stillWater_DBC/SPH_simulationSymplectic.h:154-157).
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import tinysph_old_amd as T
+from oracle import pyoracle as po
+from tests.helpers import oracle_fields, oracle_for, oracle_init, oracle_step, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+DX, STEPS = 0.005, 10000
+WL_EVERY, P_EVERY = 50, 10
+# (x, y0, inverse): cases/damBreakLong_mDBC/constants.h:58-83 in the driver's call order
+GAUGES = [(0.3, 0.0, False), (0.3, 0.45, True), (0.865, 0.0, False), (0.865, 0.45, True),
+          (1.114, 0.0, False), (1.114, 0.45, True), (1.3625, 0.0, False), (1.3625, 0.45, True)]
+SENSORS = [(1.61, 0.003), (1.61, 0.015), (1.61, 0.03), (1.61, 0.08)]
+WL_SCALE = 0.3
+P_SCALE = 1000. * 9.81 * 0.3
+
+
+def threads() -> int:
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def marching_step(level: float, y0: float, dp: float) -> float:
+    """Largest sampling increment of Water_Elevation(_inverse) up to `level`
+    (src/SPH_water_level.h:93-109: the i-th move is i * dp * 5e-5)."""
+    dp_wl = dp * 0.00005
+    i = math.ceil(math.sqrt(2.0 * abs(level - y0) / dp_wl)) + 1
+    return i * dp_wl
+
+
+def run_series(step, wl, pr, nsteps):
+    """Drive `step(k)` for nsteps steps, sampling the gauges and sensors as the
+    driver does (after the step's last stage)."""
+    wls, ps = [], []
+    done = 0
+    while done < nsteps:
+        step(P_EVERY)
+        done += P_EVERY
+        ps.append([pr(x, y) for x, y in SENSORS])
+        if done % WL_EVERY == 0:
+            wls.append([wl(x, y0, inv) for x, y0, inv in GAUGES])
+    return np.array(wls) / WL_SCALE, np.array(ps) / P_SCALE
+
+
+@pytest.fixture(scope="module")
+def dam_break():
+    case = T.build_case(T.CASE_DAM_BREAK_MDBC_REF, dx=DX)
+    assert abs(case.dt - 5e-5) < 1e-15
+    print(f"[case] reference dam break dx={DX} np={case.np} steps={STEPS}", flush=True)
+    O = oracle_for(case, threads=threads())
+    oracle_init(O)
+
+    def ostep(k):
+        for _ in range(k):
+            oracle_step(O, case.wall, case.dt)
+
+    wl = lambda x, y0, inv: O.call("water_elevation_inverse" if inv else "water_elevation", x, y0)
+    pr = lambda x, y: O.interpolate_point(x, y)[1]
+    series = run_series(ostep, wl, pr, STEPS)
+    print(f"[oracle] done, np={O.np}", flush=True)
+    return case, series, oracle_fields(O)
+
+
+def gpu_series(case, precision):
+    with T.system_for_case(case, precision=precision) as ps:
+        ps.particle_to_cell()
+        wl = lambda x, y0, inv: ps.water_elevation(x, y0, inverse=inv)
+        pr = lambda x, y: ps.interpolate_point(x, y)[1]
+        series = run_series(lambda k: ps.step(case.wall, case.dt, k), wl, pr, STEPS)
+        return series, ps.download()
+
+
+def check_series(case, got, ref, p_tol):
+    (wl_g, p_g), (wl_o, p_o) = got, ref
+    assert wl_g.shape == wl_o.shape == (STEPS // WL_EVERY, len(GAUGES))
+    assert p_g.shape == p_o.shape == (STEPS // P_EVERY, len(SENSORS))
+    dp = case.params["dp"]
+    # every gauge sees water by the end (the front reaches x = 1.3625 m)
+    assert (wl_o[-1, 0::2] * WL_SCALE > 0.02).all(), wl_o[-1]
+    tol = np.array([[marching_step(v * WL_SCALE, y0, dp) for v, (_, y0, _) in zip(row, GAUGES)]
+                    for row in wl_o]) / WL_SCALE
+    dwl = np.abs(wl_g - wl_o)
+    exact = float((dwl == 0).mean())
+    dpr = np.abs(p_g - p_o)
+    print(f"[water level] max |d| {dwl.max() * WL_SCALE:.3e} m, {exact:.1%} of samples identical; "
+          f"[wall pressure] max |d| {dpr.max():.3e} of rho0 g 0.3, at step "
+          f"{(np.unravel_index(dpr.argmax(), dpr.shape)[0] + 1) * P_EVERY}; peak |p| "
+          f"{np.abs(p_o).max():.3f}", flush=True)
+    assert (dwl <= tol + 1e-12).all(), np.argwhere(dwl > tol + 1e-12)[:5]
+    # the sensors see the impact before the end
+    assert np.abs(p_o).max() > 0.5
+    assert dpr.max() <= p_tol, dpr.max()
+    return exact
+
+
+def field_errors(got, ref):
+    assert np.array_equal(got["type"], ref["type"])
+    return {"rho-rho0": rel_l2(got["rho"] - 1000., ref["rho"] - 1000.),
+            "p": rel_l2(got["p"], ref["p"]), "v": rel_l2(got["v"], ref["v"]),
+            "r": rel_l2(got["r"], ref["r"])}
+
+
+@pytest.mark.timeout(600)
+def test_water_levels_and_wall_pressures_fp64(dam_break):
+    case, ref_series, ref_fields = dam_break
+    series, fields = gpu_series(case, T.PREC_FP64)
+    exact = check_series(case, series, ref_series, p_tol=1e-2)
+    assert exact >= 0.9
+    errs = field_errors(fields, ref_fields)
+    print(f"[fields fp64 t=0.5 s] {errs}", flush=True)
+    assert errs["r"] <= 1e-4 and errs["v"] <= 1e-2
+    assert errs["rho-rho0"] <= 1e-2 and errs["p"] <= 1e-2
+
+
+@pytest.mark.timeout(600)
+def test_water_levels_and_wall_pressures_mixed(dam_break):
+    case, ref_series, ref_fields = dam_break
+    series, fields = gpu_series(case, T.PREC_MIXED)
+    check_series(case, series, ref_series, p_tol=5e-2)
+    errs = field_errors(fields, ref_fields)
+    print(f"[fields mixed t=0.5 s] {errs}", flush=True)
+    assert errs["r"] <= 1e-3 and errs["v"] <= 1e-1
+    assert errs["rho-rho0"] <= 1e-1 and errs["p"] <= 1e-1
+
+
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("precision", [T.PREC_FP64, T.PREC_MIXED], ids=["fp64", "mixed"])
+def test_still_water_2000_steps_and_hydrostatic_profile(precision, tmp_path):
+    """The still-water DBC tank at its configured size (1.0 m, 0.5 m of water,
+    dx = 10 mm, h = 1.5 dx, 2000 steps) against the oracle, and its
+    Output_hydrostatic_pressure profile."""
+    case = T.build_case(T.CASE_STILL_WATER_DBC, dx=0.01)
+    nsteps = case.steps
+    assert nsteps == 2000
+    print(f"[case] still water dx=0.01 np={case.np} steps={nsteps} precision={precision}", flush=True)
+    O = oracle_for(case, threads=threads())
+    oracle_init(O)
+    for _ in range(nsteps):
+        oracle_step(O, case.wall, case.dt)
+    ref = oracle_fields(O)
+    path = tmp_path / "hydrostatic.dat"
+    with T.system_for_case(case, precision=precision) as ps:
+        ps.particle_to_cell()
+        ps.step(case.wall, case.dt, nsteps)
+        got = ps.download()
+        ps.write_hydrostatic_pressure(str(path))
+    errs = field_errors(got, ref)
+    print(f"[fields] {errs}", flush=True)
+    # one ulp of every fluid density at t = 0 gives, on the oracle alone after
+    # these 2000 steps: rho - rho0 6e-8, p 6e-8, v 6e-6, r 4e-9 (rel. L2)
+    if precision == T.PREC_FP64:
+        assert errs["rho-rho0"] <= 1e-5 and errs["p"] <= 1e-5 and errs["r"] <= 1e-7
+        assert errs["v"] <= 1e-3
+    else:
+        assert errs["rho-rho0"] <= 1e-2 and errs["p"] <= 1e-2 and errs["r"] <= 1e-5
+        assert errs["v"] <= 3e-1
+    # the profile file: one line per fluid particle in reference order,
+    # p / (rho0 g 0.5) and y / 0.5 (src/SPH_measure_pressure.h:22-31)
+    prof = np.loadtxt(path)
+    fl = ref["type"] == 0
+    assert prof.shape == (int(fl.sum()), 2)
+    want = np.stack([ref["p"][fl] / (9.81 * 1000. * 0.5), ref["r"][fl, 1] / 0.5], axis=1)
+    # six significant digits as printed
+    assert np.allclose(prof, want, rtol=2e-5 if precision == T.PREC_FP64 else 1e-3, atol=2e-3)
+    # still water stays hydrostatic, p / (rho0 g H) = 1 - y / H, up to the
+    # acoustic oscillation of the weakly compressible column (period 4 H / c0 =
+    # 0.09 s; the oracle's own fit at t = 0.27 s: slope -0.97, intercept 0.94)
+    core = (want[:, 1] > 0.2) & (want[:, 1] < 0.8)
+    A = np.stack([prof[core, 1], np.ones(int(core.sum()))], axis=1)
+    slope, icpt = np.linalg.lstsq(A, prof[core, 0], rcond=None)[0]
+    print(f"[hydrostatic] p* = {slope:.4f} y* + {icpt:.4f}", flush=True)
+    assert -1.05 <= slope <= -0.9 and abs(icpt - 1.0) <= 0.1

@@ -20,7 +20,7 @@ PREC_FP64, PREC_MIXED = 0, 1
 FORCES_WCSPH, FORCES_DBC, FORCES_BT = 0, 1, 2
 WALL_DBC, WALL_MDBC, WALL_BT = 0, 1, 2
 CASE_STILL_WATER_DBC, CASE_DAM_BREAK_MDBC, CASE_DAM_BREAK_BT, CASE_ALE_DAM_BREAK = 1, 2, 3, 4
-CASE_INFLOW_SLAB, CASE_INFLOW_CHANNEL = 5, 6
+CASE_INFLOW_SLAB, CASE_INFLOW_CHANNEL, CASE_DAM_BREAK_MDBC_REF = 5, 6, 7
 INLET1, OUTLET1 = 7, 12
 SCHEME_SYMPLECTIC, SCHEME_RALE = 0, 1
 
@@ -123,6 +123,7 @@ EXPORTS = {
     "tsph_bz_add_zone": (C.c_int, [C.c_void_p, C.POINTER(Zone), C.POINTER(C.c_int)]),
     "tsph_bz_integrate_inlet": (C.c_int, [C.c_void_p, C.c_int, C.c_double]),
     "tsph_bz_integrate_outlet": (C.c_int, [C.c_void_p, C.c_int, C.c_double]),
+    "tsph_renumber": (C.c_int, [C.c_void_p]),
     "tsph_step_inflow": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int]),
     "tsph_case_build": (C.c_int, [C.c_int, C.c_double, C.c_double, C.c_uint64,
                                   C.POINTER(C.POINTER(CaseC))]),
@@ -147,6 +148,11 @@ EXPORTS = {
     "tsph_write_interpolated_vtk": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_double,
                                               C.c_double, C.c_char_p]),
     "tsph_format_vtk": (C.c_int, [C.POINTER(Particles), C.c_char_p]),
+    "tsph_write_grid_vtk": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_char_p]),
+    "tsph_format_grid_vtk": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                       C.c_double, C.POINTER(C.c_int32), C.c_char_p]),
+    "tsph_write_hydrostatic_pressure": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "tsph_format_hydrostatic_pressure": (C.c_int, [C.POINTER(Particles), C.c_char_p]),
     "tsph_format_interpolated_vtk": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_double,
                                                C.c_double, _dp, _dp, _dp, C.c_char_p]),
     "tsph_save_state": (C.c_int, [C.c_void_p, C.c_char_p]),

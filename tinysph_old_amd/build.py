@@ -61,6 +61,7 @@ DRIVERS = {
     os.path.join(PKG, "tsph_dropin_mdbc"): [],
     os.path.join(PKG, "tsph_dropin_rale"): ["-DTSPH_DROPIN_RALE"],
     os.path.join(PKG, "tsph_dropin_verlet"): ["-DTSPH_DROPIN_VERLET"],
+    os.path.join(PKG, "tsph_dropin_inflow"): ["-DTSPH_DROPIN_INFLOW"],
 }
 
 

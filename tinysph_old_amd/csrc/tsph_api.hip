@@ -497,6 +497,7 @@ static int upload_impl(tsph_ctx *c, const tsph_particles *h, const uint32_t *seq
   int sc[S_NSCAL] = {};
   sc[S_NP] = n;
   sc[S_MAXSEQ] = maxseq;
+  c->seq_ub = maxseq;
   TSPH_HIP(hipMemcpy(c->scal, sc, sizeof sc, hipMemcpyHostToDevice));
   TSPH_HIP(hipMemsetAsync(c->seqstate, 0, (size_t)c->seq_cap, c->stream));
   TSPH_HIP(hipStreamSynchronize(c->stream));
@@ -550,6 +551,60 @@ static int live_order(tsph_ctx *c, int n, int icase, std::vector<int> &order,
   for (int i = 0; i < n; ++i)
     if (type[i] != kRemoved && (icase < 0 || cid[i] == icase)) order.push_back(i);
   std::sort(order.begin(), order.end(), [&](int a, int b) { return seq[a] < seq[b]; });
+  return 0;
+}
+
+// Reference positions (seq) are only ever compared, never counted, so they may
+// be relabelled between stages as long as the live particles keep their order
+// (in the reference they are vector indices, renumbered by every erase,
+// src/SPH_particle_system.h:125-145).  Inlet appends take fresh positions from
+// S_MAXSEQ upwards and nothing hands positions back, so a long inflow run
+// would run out of removal states (seq_cap); this relabels the live particles
+// 0..m-1 in order, carrying each one's removal state along, gives the removed
+// slots still in the range the positions after them (state 0: absent), and
+// resets S_MAXSEQ.  Cell order, tiles and records stay valid: the in-cell
+// order by seq is unchanged.
+static int renumber_seq(tsph_ctx *c) {
+  int n = 0, e = 0;
+  TSPH_HIP(hipStreamSynchronize(c->stream));
+  TSPH_HIP(hipMemcpy(&n, &c->scal[S_NP], sizeof(int), hipMemcpyDeviceToHost));
+  std::vector<int> order;
+  std::vector<uint32_t> seq;
+  std::vector<uint8_t> type;
+  if ((e = live_order(c, n, -1, order, seq, type))) return e;
+  std::vector<uint8_t> st((size_t)c->seq_cap), nst((size_t)c->seq_cap, 0);
+  TSPH_HIP(hipMemcpy(st.data(), c->seqstate, (size_t)c->seq_cap, hipMemcpyDeviceToHost));
+  std::vector<uint32_t> nseq((size_t)std::max(n, 1));
+  uint32_t next = 0;
+  for (int i : order) {
+    nst[next] = st[seq[i]];
+    nseq[i] = next++;
+  }
+  for (int i = 0; i < n; ++i)
+    if (type[i] == kRemoved) nseq[i] = next++;
+  if (n > 0) TSPH_HIP(hipMemcpy(c->cur.seq, nseq.data(), 4 * (size_t)n, hipMemcpyHostToDevice));
+  TSPH_HIP(hipMemcpy(c->seqstate, nst.data(), (size_t)c->seq_cap, hipMemcpyHostToDevice));
+  const int ms = (int)next;
+  TSPH_HIP(hipMemcpy(&c->scal[S_MAXSEQ], &ms, sizeof(int), hipMemcpyHostToDevice));
+  c->seq_ub = ms;
+  return 0;
+}
+
+// Inlet passes (each hands out up to kBzMax positions) that fit before the
+// reference positions run out; reads the device's S_MAXSEQ, and relabels, only
+// when the host's bound allows less than one.  After a relabel at least one
+// fits: S_MAXSEQ <= capacity and seq_cap = 2 capacity + kBzMax + ncases.
+static int seq_room(tsph_ctx *c, int *passes) {
+  auto room = [c] { return (c->seq_cap - c->seq_ub) / kBzMax; };
+  if (room() < 1) {
+    int ms = 0;
+    TSPH_HIP(hipStreamSynchronize(c->stream));
+    TSPH_HIP(hipMemcpy(&ms, &c->scal[S_MAXSEQ], sizeof(int), hipMemcpyDeviceToHost));
+    c->seq_ub = ms;
+    if (room() < 1)
+      if (int e = renumber_seq(c)) return e;
+  }
+  *passes = room();
   return 0;
 }
 
@@ -899,7 +954,16 @@ static int zone_ok(tsph_ctx *c, int zone) {
 
 int tsph_bz_integrate_inlet(tsph_ctx *c, int zone, double dt) {
   if (int e = zone_ok(c, zone)) return e;
+  int passes = 0;
+  if (int e = seq_room(c, &passes)) return e;
+  c->seq_ub += kBzMax;
   return launch_bz_inlet(c, zone, dt);
+}
+
+int tsph_renumber(tsph_ctx *c) {
+  if (!c) return fail(TSPH_EINVAL, "null argument");
+  if (int e_ = single_only(c)) return e_;
+  return renumber_seq(c);
 }
 
 int tsph_bz_integrate_outlet(tsph_ctx *c, int zone, double dt) {
@@ -918,13 +982,27 @@ int tsph_step_inflow(tsph_ctx *c, int wall, double dt, int nsteps, int inlet, in
   if (wall == TSPH_WALL_MDBC && !c->has_gnr) return fail(TSPH_ESTATE, "mDBC needs ghost nodes (gnr)");
   if (wall == TSPH_WALL_BT && !c->has_n) return fail(TSPH_ESTATE, "BT needs wall normals (n)");
   if (inlet >= 0) c->n_ub = c->capacity;
-  if (c->time_pairs) {
-    for (int s = 0; s < nsteps; ++s)
-      if (int e = record_step(c, wall, dt, inlet, outlet)) return e;
-    return 0;
+  // one inlet pass per step: replay in runs that cannot exhaust the reference
+  // positions, relabelling them in between when needed
+  for (int done = 0; done < nsteps;) {
+    int k = nsteps - done;
+    if (inlet >= 0) {
+      int passes = 0;
+      if (int e = seq_room(c, &passes)) return e;
+      k = std::min(k, passes);
+      c->seq_ub += k * kBzMax;
+    }
+    if (c->time_pairs) {
+      for (int s = 0; s < k; ++s)
+        if (int e = record_step(c, wall, dt, inlet, outlet)) return e;
+    } else if (int e = replay_step(c, c->inflow_graph,
+                                   300 + 16 * wall + 4 * (inlet + 1) + (outlet + 1), dt, k,
+                                   [&] { return record_step(c, wall, dt, inlet, outlet); })) {
+      return e;
+    }
+    done += k;
   }
-  return replay_step(c, c->inflow_graph, 300 + 16 * wall + 4 * (inlet + 1) + (outlet + 1), dt,
-                     nsteps, [&] { return record_step(c, wall, dt, inlet, outlet); });
+  return 0;
 }
 
 int tsph_water_elevation(tsph_ctx *c, double x, double y0, int inverse, double *level) {

@@ -169,6 +169,27 @@ extern "C" int tsph_case_build(int preset, double dx, double jitter, uint64_t se
       jitter_fluid(ps, (jitter >= 0 ? jitter : 0.01) * s.dp, seed);
       return export_case(s, ps, out);
     }
+    case TSPH_CASE_DAM_BREAK_MDBC_REF: {
+      // cases/damBreakLong_mDBC/constants.h and geometry.h as they stand (the
+      // tank of the water-level gauges and the wall pressure sensors,
+      // cases/damBreakLong_mDBC/constants.h:58-94); dt = 5e-5 s at dx = 5 mm,
+      // scaled with dx
+      s.dp = dx > 0 ? dx : 0.005;
+      s.hh = 1.0 * std::sqrt(2 * s.dp * s.dp); s.kap = 2.1; s.cs = 34.3;
+      s.visco = 0.02; s.delta = 0.1; s.dt = 0.00005 * (s.dp / 0.005);
+      s.x_0 = 0.; s.x_1 = 2.; s.y_0 = 0.; s.y_1 = 1.;
+      s.steps = 45001; s.wall = TSPH_WALL_MDBC;
+      Particle_system ps(s.hh, s.kap, s.cs, 1000., s.visco, s.delta, s.dp);
+      const double dp = s.dp;
+      Create_Wall_x_mdbc(ps, 0. + dp, 1.61, 0., -1, dp / 2);
+      Create_Wall_y_mdbc(ps, 0., 0 + dp, 1., -1, dp / 2);
+      Create_Wall_y_mdbc(ps, 1.61, 0 + dp, 1., +1, 1.61 - dp / 2);
+      Create_corner_mdbc(ps, -1, -1, {0. + dp / 2, 0. + dp / 2});
+      Create_corner_mdbc(ps, +1, -1, {1.61 - dp / 2, 0. + dp / 2});
+      Create_Block_of_Fluid(ps, 0 + dp, 0 + dp, 0.6, 0.3);
+      jitter_fluid(ps, (jitter > 0 ? jitter : 0.0) * s.dp, seed);
+      return export_case(s, ps, out);
+    }
     case TSPH_CASE_DAM_BREAK_BT: {
       // cases/damBreakLong_BT/{constants,geometry}.h
       s.dp = dx > 0 ? dx : 0.005;

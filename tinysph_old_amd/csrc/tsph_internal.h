@@ -195,6 +195,7 @@ struct tsph_ctx {
   int n_zones = 0;
   int *bz_list = nullptr;     // 2 * kBzMax ints: {seq, slot} of the crossing zone particles
   int seq_cap = 0;            // size of seqstate (reference positions ever handed out)
+  int seq_ub = 0;             // host upper bound of scal[S_MAXSEQ]
 };
 
 namespace tsph {

@@ -81,6 +81,48 @@ bool format_grid(const char *path, int ncx, int ncy, double dh, double x0, doubl
   return !file.fail();
 }
 
+// the cell-grid file of write_mesh_to_ASCII_VTK, src/SPH_grid_to_vtk.h:6-67:
+// (ncx+1) x (ncy+1) grid points at x_0 + i kh - of, y_0 + j kh - of, then per
+// cell (in POS order, src/SPH_defs.h:36) its particle count and its 1-D index
+bool format_cells(const char *path, int ncx, int ncy, double kh, double of, double x_0,
+                  double y_0, const int32_t *count) {
+  std::ofstream file(path);
+  if (!file) return false;
+  const int px = ncx + 1, py = ncy + 1;
+  const int cells = ncx * ncy;
+  file << "# vtk DataFile Version 3.0" << '\n' << "vtk output" << '\n' << "ASCII" << '\n'
+       << "DATASET STRUCTURED_GRID" << '\n' << "DIMENSIONS " << px << " " << py << " 1" << '\n'
+       << "POINTS " << px * py << " float" << '\n';
+  for (int i = 0; i < px; i++)
+    for (int j = 0; j < py; j++) file << x_0 + i * kh - of << " " << y_0 + j * kh - of << " 0" << '\n';
+  file << "CELL_DATA " << cells << '\n' << "FIELD FieldData " << 2 << '\n';
+  file << "NumberOfParticles 1 " << cells << " int" << '\n';
+  for (int x = 0; x < ncx; x++)
+    for (int y = 0; y < ncy; y++) file << (unsigned)count[y + x * ncy] << '\n';
+  file << "Cell1DIndex 1 " << cells << " int" << '\n';
+  for (int x = 0; x < ncx; x++)
+    for (int y = 0; y < ncy; y++) file << y + x * ncy << '\n';
+  file.close();
+  return !file.fail();
+}
+
+// Output_hydrostatic_pressure, src/SPH_measure_pressure.h:17-34: one line per
+// fluid particle in reference order, p / (rho0 g H) and y / H for the 0.5 m
+// still-water column
+bool format_hydrostatic(const char *path, int np, const int32_t *type, const double *r,
+                        const double *p) {
+  std::ofstream file(path);
+  if (!file) return false;
+  const double H_coef = 1. / 0.5;
+  const double p_coef = 1. / (9.81 * 1000. * 0.5);
+  for (int i = 0; i < np; i++) {
+    if (type[i] != 0) continue;
+    file << p[i] * p_coef << " " << r[2 * i + 1] * H_coef << '\n';
+  }
+  file.close();
+  return !file.fail();
+}
+
 // one asynchronous particle file: pinned copies of the slot arrays, the event
 // that marks them complete, and the target path
 struct VtkJob {
@@ -150,6 +192,54 @@ int tsph_format_interpolated_vtk(int ncx, int ncy, double dh, double x0, double 
   if (!format_grid(path, ncx, ncy, dh, x0, y0, rho, p, v))
     return io_fail(TSPH_EINVAL, std::string("tsph_format_interpolated_vtk: cannot write ") + path);
   return 0;
+}
+
+int tsph_format_grid_vtk(int ncx, int ncy, double kh, double of, double x_0, double y_0,
+                         const int32_t *counts, const char *path) {
+  if (!path || ncx < 0 || ncy < 0 || (ncx * ncy > 0 && !counts))
+    return io_fail(TSPH_EINVAL, "tsph_format_grid_vtk: invalid argument");
+  if (!format_cells(path, ncx, ncy, kh, of, x_0, y_0, counts))
+    return io_fail(TSPH_EINVAL, std::string("tsph_format_grid_vtk: cannot write ") + path);
+  return 0;
+}
+
+int tsph_write_grid_vtk(tsph_ctx *c, double x_0, double y_0, const char *path) {
+  if (!c || !path) return io_fail(TSPH_EINVAL, "null argument");
+  if (c->cases) return io_fail(TSPH_ESTATE, "not available on a batched context");
+  if (!c->cells_valid) return io_fail(TSPH_ESTATE, "no cell list: call tsph_particle_to_cell first");
+  const int ncx = c->k.ncx, ncy = c->k.ncy, cells = ncx * ncy;
+  std::vector<int32_t> start((size_t)cells + 1), count((size_t)std::max(cells, 1));
+  if (hipMemcpyAsync(start.data(), c->cell_start, sizeof(int32_t) * ((size_t)cells + 1),
+                     hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+      hipStreamSynchronize(c->stream) != hipSuccess)
+    return io_fail(TSPH_EHIP, "tsph_write_grid_vtk: copy failed");
+  for (int q = 0; q < cells; ++q) count[q] = start[q + 1] - start[q];
+  // src/SPH_grid_to_vtk.h:16-18: kh = h kap, of = 0.273 h
+  return tsph_format_grid_vtk(ncx, ncy, c->k.h * c->k.kap, c->k.h * 0.273, x_0, y_0, count.data(),
+                              path);
+}
+
+int tsph_format_hydrostatic_pressure(const tsph_particles *h, const char *path) {
+  if (!h || !path || (h->np > 0 && (!h->type || !h->r || !h->p)))
+    return io_fail(TSPH_EINVAL, "tsph_format_hydrostatic_pressure: null argument");
+  if (!format_hydrostatic(path, h->np, h->type, h->r, h->p))
+    return io_fail(TSPH_EINVAL, std::string("tsph_format_hydrostatic_pressure: cannot write ") + path);
+  return 0;
+}
+
+int tsph_write_hydrostatic_pressure(tsph_ctx *c, const char *path) {
+  if (!c || !path) return io_fail(TSPH_EINVAL, "null argument");
+  if (c->cases) return io_fail(TSPH_ESTATE, "not available on a batched context");
+  int n = 0;
+  if (int e = tsph_get_np(c, &n)) return e;
+  std::vector<int32_t> type((size_t)std::max(n, 1));
+  std::vector<double> r(2 * (size_t)std::max(n, 1)), p((size_t)std::max(n, 1));
+  tsph_particles h{};
+  h.type = type.data();
+  h.r = r.data();
+  h.p = p.data();
+  if (int e = tsph_download(c, &h)) return e;
+  return tsph_format_hydrostatic_pressure(&h, path);
 }
 
 int tsph_write_vtk(tsph_ctx *c, const char *path) {

@@ -331,6 +331,10 @@ class ParticleSystem:
     def bz_integrate_outlet(self, zone: int, dt: float):
         L.check(self.lib.tsph_bz_integrate_outlet(self.ctx, zone, dt), "BZ_Integrate_outlet")
 
+    def renumber(self):
+        """Relabel the live particles' reference positions 0..np-1 (tsph_renumber)."""
+        L.check(self.lib.tsph_renumber(self.ctx), "tsph_renumber")
+
     def step_inflow(self, wall: int, dt: float, nsteps: int = 1, inlet: int = 0, outlet: int = 1):
         L.check(self.lib.tsph_step_inflow(self.ctx, wall, dt, nsteps, inlet, outlet), "tsph_step_inflow")
 
@@ -359,6 +363,16 @@ class ParticleSystem:
     # ---- output, snapshot, resume (src/SPH_data_to_vtk.h, src/SPH_dataInterpolation.h) ----
     def write_vtk(self, path: str):
         L.check(self.lib.tsph_write_vtk(self.ctx, str(path).encode()), "write_to_ASCII_VTK")
+
+    def write_grid_vtk(self, x_0: float, y_0: float, path: str):
+        """write_mesh_to_ASCII_VTK (src/SPH_grid_to_vtk.h:6-67)."""
+        L.check(self.lib.tsph_write_grid_vtk(self.ctx, float(x_0), float(y_0), str(path).encode()),
+                "write_mesh_to_ASCII_VTK")
+
+    def write_hydrostatic_pressure(self, path: str):
+        """Output_hydrostatic_pressure (src/SPH_measure_pressure.h:17-34)."""
+        L.check(self.lib.tsph_write_hydrostatic_pressure(self.ctx, str(path).encode()),
+                "Output_hydrostatic_pressure")
 
     def io_wait(self):
         L.check(self.lib.tsph_io_wait(self.ctx), "tsph_io_wait")
