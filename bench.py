@@ -46,18 +46,41 @@ KERNEL = "k_tile32<0>"  # the mixed WCSPH pair kernel (mDBC case: fluid-only int
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_dam_break_mdbc_740k.md")
 
 
-def measured_traffic(kernel: str):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summary (lines `traffic_bytes_per_launch <kernel> <bytes>`), or None."""
+SQ_PROFILE = os.path.join(ROOT, "profiles", "sq_counters_dam_break_mdbc_740k.md")
+# VALU issue: one wave64 instruction per 4 cycles per SIMD, 256 CUs x 4 SIMDs, 2.4 GHz
+VALU_ISSUE_PER_S = 256 * 4 * 2.4e9 / 4
+
+
+def profile_value(path: str, key: str, kernel: str):
+    """The value of a machine-readable line `<key> <kernel> <value>` of a
+    committed rocprofv3 summary, or None."""
     try:
-        with open(TRAFFIC_PROFILE) as f:
+        with open(path) as f:
             for ln in f:
                 t = ln.split()
-                if len(t) == 3 and t[0] == "traffic_bytes_per_launch" and t[1] == kernel:
+                if len(t) == 3 and t[0] == key and t[1] == kernel:
                     return float(t[2])
     except OSError:
         pass
     return None
+
+
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (lines `traffic_bytes_per_launch <kernel> <bytes>`), or None."""
+    return profile_value(TRAFFIC_PROFILE, "traffic_bytes_per_launch", kernel)
+
+
+def valu_ceiling(kernel: str, avg_ms: float) -> dict:
+    """The VALU-issue ceiling of `kernel`: SQ_INSTS_VALU per launch (committed
+    rocprofv3 summary) at one wave instruction per 4 cycles on every SIMD, and
+    the fraction of that floor the measured launch time reaches."""
+    insts = profile_value(SQ_PROFILE, "valu_insts_per_launch", kernel)
+    if insts is None:
+        return {"valu_insts_per_launch": None, "valu_floor_ms": None, "valu_frac": None}
+    floor_ms = insts / VALU_ISSUE_PER_S * 1e3
+    return {"valu_insts_per_launch": insts, "valu_floor_ms": floor_ms,
+            "valu_frac": floor_ms / avg_ms if avg_ms > 0 else None}
 
 
 def copy_probe(mib: int = 512, reps: int = 3):
@@ -146,6 +169,46 @@ def leg_still_water(steps):
     return {"fp64": leg_line(t, cnt, steps, case.np)}
 
 
+# ALE-SPH MUSCL pair passes with reconstruction (minmod): algorithmic bytes per
+# pair (the staged neighbour slot: r, v, {rho, omega}, n, type, and for the
+# density and acceleration passes the three gradients) and per active particle
+# (own slot + outputs: d omega, d omega rho; the three gradients; d omega v)
+ALE_PASS_BYTES = {
+    T.PREC_MIXED: {"density": (57, 57 + 16), "gradient": (33, 33 + 48), "acceleration": (57, 57 + 16)},
+    T.PREC_FP64: {"density": (113, 113 + 16), "gradient": (65, 65 + 48),
+                  "acceleration": (113, 113 + 16)},
+}
+
+
+def ale_pass_roofline(ps, prec, reps: int = 10) -> dict:
+    """Each ALE pair pass timed alone on the current state (HIP events around
+    each launch), with its pair and active-particle counts: achieved
+    algorithmic GB/s and the fraction of the HBM peak."""
+    calls = {"density": lambda: ps.ale_compute_density(limited=True),
+             "gradient": ps.ale_kernel_gradient,
+             "acceleration": lambda: ps.ale_compute_acceleration(limited=True)}
+    out = {}
+    for name, fn in calls.items():
+        fn()
+        ps.synchronize()
+        c0 = ps.counters()
+        ps.pair_timing(reps + 2)
+        for _ in range(reps):
+            fn()
+        ms, n = ps.pair_timing_read()
+        ps.pair_timing(0)
+        c1 = ps.counters()
+        pairs = (c1["pairs"] - c0["pairs"]) / max(n, 1)
+        act = (c1["active"] - c0["active"]) / max(n, 1)
+        bp, ba = ALE_PASS_BYTES[prec][name]
+        nbytes = pairs * bp + act * ba
+        avg = ms / max(n, 1)
+        gbs = nbytes / (avg * 1e-3) / 1e9
+        out[name] = {"avg_launch_ms": avg, "pairs_per_launch": pairs, "algorithmic_bytes_per_launch":
+                     nbytes, "achieved_gbs": gbs, "frac": gbs / HBM_PEAK_GBS}
+    return out
+
+
 def leg_ale(steps):
     """ALE-SPH dam break, BT walls, dx = 2 mm, minmod-limited MUSCL."""
     case = T.build_case(T.CASE_ALE_DAM_BREAK)
@@ -162,6 +225,7 @@ def leg_ale(steps):
 
             t, cnt = timed(ps, run, steps, 5)
             out[name] = leg_line(t, cnt, steps, case.np)
+            out[name]["roofline_passes"] = ale_pass_roofline(ps, prec)
     out["mixed_over_fp64"] = (out["mixed"]["pair_interactions_per_s"] /
                               out["fp64"]["pair_interactions_per_s"])
     return out
@@ -187,21 +251,69 @@ def leg_ensemble(steps):
 
 
 def leg_inflow(steps):
-    """8 x 0.5 m channel with inlet/outlet buffer zones, dx = 0.5 mm (4M particles)."""
+    """8 x 0.5 m channel with inlet/outlet buffer zones, dx = 0.5 mm (4M particles),
+    FP32 pairs with Kahan-compensated FP64 integration."""
     case = T.build_case(T.CASE_INFLOW_CHANNEL)
     with T.system_for_case(case, precision=T.PREC_MIXED) as ps:
+        ps.set_compensated(True)
         ps.particle_to_cell()
         t, cnt = timed(ps, lambda k: ps.step_inflow(case.wall, case.dt, k), steps, 3)
     return {"mixed": leg_line(t, cnt, steps, case.np, appended=cnt["appended"],
-                             removed=cnt["removed"], refused=cnt["zone_rejected"])}
+                             removed=cnt["removed"], refused=cnt["zone_rejected"],
+                             integration="fp64, Kahan-compensated r and rho")}
+
+
+def host_threads() -> int:
+    """Threads for the oracle: the host cores this process may use.  A GPU box
+    gives each GPU a share of the host and says so in OMP_NUM_THREADS; else the
+    CPU affinity mask."""
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except AttributeError:
+        aff = os.cpu_count() or 1
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    return max(1, min(int(env), aff) if env.isdigit() and int(env) > 0 else aff)
+
+
+def host_info() -> dict:
+    """The host CPU as lscpu and the kernel report it, and the share this
+    process may use (affinity, cgroup quota, OMP_NUM_THREADS)."""
+    import subprocess
+    info = {}
+    try:
+        txt = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        keys = {"Model name": "model", "Socket(s)": "sockets", "Core(s) per socket": "cores_per_socket",
+                "Thread(s) per core": "threads_per_core", "CPU(s)": "cpus"}
+        for ln in txt.splitlines():
+            k, _, v = ln.partition(":")
+            if k.strip() in keys and keys[k.strip()] not in info:
+                v = v.strip()
+                info[keys[k.strip()]] = int(v) if v.isdigit() else v
+    except (OSError, subprocess.SubprocessError):
+        pass
+    if isinstance(info.get("sockets"), int) and isinstance(info.get("cores_per_socket"), int):
+        info["physical_cores"] = info["sockets"] * info["cores_per_socket"]
+    try:
+        info["affinity"] = len(os.sched_getaffinity(0))
+    except AttributeError:
+        pass
+    try:
+        quota, period = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        info["cgroup_cpus"] = None if quota == "max" else float(quota) / float(period)
+    except (OSError, ValueError):
+        pass
+    info["omp_num_threads"] = os.environ.get("OMP_NUM_THREADS")
+    return info
 
 
 def cpu_baseline(case, budget_s: float = 20.0, threads: int = 0, max_steps: int = 50):
-    """Oracle (C++ restatement of the reference path, OpenMP) on the host cores."""
+    """Oracle (C++ restatement of the reference path, OpenMP) on the host cores.
+    Returns (steps, seconds, threads, oracle) so that the same run can go on as
+    the checker of the error legs."""
     from oracle import pyoracle as po
     from tests.helpers import oracle_init, oracle_step
 
-    threads = threads or min(16, os.cpu_count() or 1)
+    threads = threads or host_threads()
     O = po.oracle_case(case.oracle_params(), case.arrays, omp_threads=threads)
     oracle_init(O)
     O.call("density_to_pressure")
@@ -214,7 +326,180 @@ def cpu_baseline(case, budget_s: float = 20.0, threads: int = 0, max_steps: int 
         n += 1
         if time.perf_counter() - t0 > budget_s or n >= max_steps:
             break
-    return n, time.perf_counter() - t0, threads
+    return n, time.perf_counter() - t0, threads, O
+
+
+# ---------------------------------------------------------------------------
+# error of every reported field against the FP64 oracle, per workload.  The
+# oracle runs here only as the checker (the timed GPU legs never call it).
+# ---------------------------------------------------------------------------
+def field_errors(got: dict, ref: dict, rho0: float = 1000.0) -> dict:
+    """Relative L2 error of rho - rho0, p, v and r (and max |d| / max |ref|)."""
+    def l2(a, b):
+        a, b = np.asarray(a, np.float64).ravel(), np.asarray(b, np.float64).ravel()
+        return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+    def mx(a, b):
+        a, b = np.asarray(a, np.float64).ravel(), np.asarray(b, np.float64).ravel()
+        return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+    if len(got["type"]) != len(ref["type"]) or not np.array_equal(got["type"], ref["type"]):
+        return {"types_equal": False, "np": [int(len(got["type"])), int(len(ref["type"]))]}
+    return {"rho-rho0_rel_l2": l2(got["rho"] - rho0, ref["rho"] - rho0),
+            "p_rel_l2": l2(got["p"], ref["p"]), "v_rel_l2": l2(got["v"], ref["v"]),
+            "r_rel_l2": l2(got["r"], ref["r"]), "rho_rel_max": mx(got["rho"], ref["rho"]),
+            "r_rel_max": mx(got["r"], ref["r"])}
+
+
+def oracle_fields(O) -> dict:
+    from oracle import pyoracle as po
+    return {"type": O.types(), "r": O.get(po.F_R), "v": O.get(po.F_V), "p": O.get(po.F_P),
+            "rho": O.get(po.F_RHO)}
+
+
+def gpu_fields(case, precision, steps, kind="symplectic", muscl=0, compensated=False):
+    """The GPU state after `steps` driver steps from the case's initial state."""
+    with T.system_for_case(case, precision=precision) as ps:
+        if compensated:
+            ps.set_compensated(True)
+        if kind == "ale":
+            ps.ale_enable()
+            ps.step_ale(case.dt, 1, steps, muscl)
+        else:
+            ps.particle_to_cell()
+            if kind == "inflow":
+                ps.step_inflow(case.wall, case.dt, steps)
+            else:
+                ps.step(case.wall, case.dt, steps)
+        return ps.download(("type", "r", "v", "p", "rho")), ps.counters()
+
+
+def errors_vs_oracle(threads: int, O_full, full_case, full_steps: int, gpu_full: dict) -> dict:
+    """Each workload's mixed (and FP64) GPU state against the oracle after the
+    workload's step count, at the sizes stated in each entry."""
+    from tests.helpers import (oracle_add_zones, oracle_ale, oracle_ale_step, oracle_for,
+                               oracle_inflow_step, oracle_init, oracle_step)
+
+    out = {}
+    precs = (("mixed", T.PREC_MIXED), ("fp64", T.PREC_FP64))
+
+    def log(name, t0):
+        print(f"[bench] error leg {name} done {time.perf_counter() - t0:.1f}s", file=sys.stderr,
+              flush=True)
+
+    # dam break at full size (740k): the cpu_baseline run continued
+    t0 = time.perf_counter()
+    ref = oracle_fields(O_full)
+    out["dam_break_740k"] = {"particles": full_case.np, "after_steps": full_steps,
+                             **{k: field_errors(g, ref) for k, g in gpu_full.items()}}
+    log("dam_break_740k", t0)
+
+    # still water, the first workload at its configured size (dx 10 mm, 2000 steps)
+    t0 = time.perf_counter()
+    case = T.build_case(T.CASE_STILL_WATER_DBC, dx=0.01)
+    O = oracle_for(case, threads)
+    oracle_init(O)
+    for _ in range(case.steps):
+        oracle_step(O, case.wall, case.dt)
+    ref = oracle_fields(O)
+    out["still_water"] = {"particles": case.np, "after_steps": case.steps, "dx": 0.01,
+                          **{n: field_errors(gpu_fields(case, p, case.steps)[0], ref) for n, p in precs}}
+    log("still_water", t0)
+
+    # dam break, 1000 steps at dx = 4 mm (the 740k case at 1000 steps:
+    # profiles/error_vs_oracle_dam_break_740k.md)
+    t0 = time.perf_counter()
+    case = T.build_case(T.CASE_DAM_BREAK_MDBC, dx=0.004, jitter=0.01, seed=SEED)
+    O = oracle_for(case, threads)
+    oracle_init(O)
+    for _ in range(1000):
+        oracle_step(O, case.wall, case.dt)
+    ref = oracle_fields(O)
+    out["dam_break_dx4mm"] = {"particles": case.np, "after_steps": 1000, "dx": 0.004,
+                              **{n: field_errors(gpu_fields(case, p, 1000)[0], ref) for n, p in precs}}
+    log("dam_break_dx4mm", t0)
+
+    # ALE-SPH MUSCL (minmod) dam break, 1000 steps at dx = 8 mm
+    t0 = time.perf_counter()
+    case = T.build_case(T.CASE_ALE_DAM_BREAK, dx=0.008)
+    O = oracle_ale(case, threads)
+    for s in range(1, 1001):
+        oracle_ale_step(O, s, case.dt, T.MUSCL_MINMOD)
+    ref = oracle_fields(O)
+    out["ale_muscl_dx8mm"] = {
+        "particles": case.np, "after_steps": 1000, "dx": 0.008, "muscl": "minmod",
+        **{n: field_errors(gpu_fields(case, p, 1000, "ale", T.MUSCL_MINMOD)[0], ref)
+           for n, p in precs}}
+    log("ale_muscl_dx8mm", t0)
+
+    # inflow channel (8 x 0.5 m), 1000 steps at dx = 5 mm: fields and the mass
+    # fluxes through the inlet (appended) and the outlet plane (fluid turned
+    # outlet, src/SPH_buffer_zones.h:164-171), times m over the run time
+    t0 = time.perf_counter()
+    case = T.build_case(T.CASE_INFLOW_CHANNEL, dx=0.005)
+    O = oracle_for(case, threads)
+    oracle_add_zones(O, case.zones)
+    oracle_init(O)
+    steps = 1000
+    for _ in range(steps):
+        oracle_inflow_step(O, case.wall, case.dt)
+    ref = oracle_fields(O)
+    m = 1000.0 * case.params["dp"] ** 2
+    t_run = steps * case.dt
+    out_t = case.zones[1]["btype"]
+
+    def fluxes(types, removed):
+        n0 = len(case.arrays["type"])
+        conv = int((types == out_t).sum()) - int((case.arrays["type"] == out_t).sum()) + removed
+        app = len(types) - n0 + removed
+        return app * m / t_run, conv * m / t_run
+
+    ref_in, ref_out = fluxes(ref["type"], O.call("np_out"))
+    entry = {"particles": case.np, "after_steps": steps, "dx": 0.005,
+             "oracle_mass_flux_in_kg_per_s_m": ref_in, "oracle_mass_flux_out_kg_per_s_m": ref_out}
+    entry["mixed_integration"] = "fp64, Kahan-compensated r and rho"
+    for n, p in precs:
+        g, cnt = gpu_fields(case, p, steps, "inflow", compensated=p == T.PREC_MIXED)
+        f_in, f_out = fluxes(g["type"], cnt["removed"])
+        entry[n] = {**field_errors(g, ref),
+                    "mass_flux_in_rel_err": abs(f_in - ref_in) / max(abs(ref_in), 1e-300),
+                    "mass_flux_out_rel_err": abs(f_out - ref_out) / max(abs(ref_out), 1e-300)}
+    out["inflow_dx5mm"] = entry
+    log("inflow_dx5mm", t0)
+
+    # batched ensemble: its three smallest tanks, batched together on the GPU,
+    # each against its own oracle run (500 steps)
+    t0 = time.perf_counter()
+    tanks = sorted(T.build_ensemble(64, SEED), key=lambda c: c.np)[:3]
+    refs = []
+    for c in tanks:
+        O = oracle_for(c, threads)
+        oracle_init(O)
+        for _ in range(500):
+            oracle_step(O, c.wall, c.dt)
+        refs.append(oracle_fields(O))
+    entry = {"tanks": [c.np for c in tanks], "after_steps": 500}
+    for n, p in precs:
+        with T.batch_for_cases(tanks, p) as bs:
+            bs.particle_to_cell()
+            bs.step(T.WALL_MDBC, 500)
+            got = [bs.download_case(q, ("type", "r", "v", "p", "rho")) for q in range(len(tanks))]
+        entry[n] = [field_errors(g, r) for g, r in zip(got, refs)]
+    out["ensemble_smallest_3"] = entry
+    log("ensemble_smallest_3", t0)
+    return out
+
+
+def developed_run(dx: float = 0.004, t_end: float = 0.6) -> dict:
+    """The mixed path against the FP64 path (both on the GPU) over a physically
+    developed dam break: the 3.22 m tank at dx = 4 mm to t = 0.6 s, past the
+    front's impact on the right wall (about t = 0.55 s)."""
+    case = T.build_case(T.CASE_DAM_BREAK_MDBC, dx=dx, jitter=0.01, seed=SEED)
+    steps = int(round(t_end / case.dt))
+    st = {n: gpu_fields(case, p, steps)[0] for n, p in (("mixed", T.PREC_MIXED), ("fp64", T.PREC_FP64))}
+    front = {n: float(s["r"][s["type"] == 0, 0].max()) for n, s in st.items()}
+    return {"particles": case.np, "dx": dx, "steps": steps, "t_end_s": steps * case.dt,
+            "front_x_m": front, **field_errors(st["mixed"], st["fp64"])}
 
 
 def main() -> int:
@@ -229,6 +514,11 @@ def main() -> int:
                     help="also run a plain 512 MiB device copy (calibration for HBM counters)")
     ap.add_argument("--error-steps", type=int, default=1000,
                     help="step count at which the mixed path's error vs FP64 is quoted")
+    ap.add_argument("--oracle-steps", type=int, default=100,
+                    help="steps of the 740k-particle case at which its error against the "
+                         "oracle is quoted (0: no error legs against the oracle)")
+    ap.add_argument("--no-developed", dest="developed", action="store_false",
+                    help="skip the mixed-vs-FP64 run to the dam-break front's wall impact")
     ap.add_argument("--workloads", default="all",
                     help="comma list of still_water,ale,ensemble,inflow; 'none' or 'all'")
     ap.add_argument("--workloads-only", action="store_true",
@@ -314,27 +604,48 @@ def main() -> int:
         "scan_stalls": mixed["counters"]["scan_stalls"] + fp64["counters"]["scan_stalls"],
         "fp64": {"pair_interactions_per_s": rate(fp64), "steps_per_s": k / fp64["t_graph"],
                  "ms_per_step": fp64["t_graph"] / k * 1e3, "pair_kernel_avg_ms": d_avg,
-                 "roofline_frac": d_ach / HBM_PEAK_GBS},
+                 "roofline_frac": d_ach / HBM_PEAK_GBS, **valu_ceiling("k_tile64<0>", d_avg)},
         "mixed_over_fp64": rate(mixed) / rate(fp64),
         "error_vs_fp64": err,
+        # `frac` is the algorithmic-byte roofline (neighbour records gathered per
+        # pair, as if none were reused); the staged tiles serve most of them from
+        # LDS, so the kernel's own ceiling is VALU issue (`valu_frac`, from the
+        # SQ_INSTS_VALU count) and its measured HBM rate is `traffic_gbs`
         "roofline": {"bound": "hbm", "achieved": m_ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": m_ach / HBM_PEAK_GBS, "traffic": traffic,
                      "traffic_gbs": traffic / (m_avg * 1e-3) / 1e9 if traffic else None,
+                     "traffic_frac": traffic / (m_avg * 1e-3) / 1e9 / HBM_PEAK_GBS if traffic else None,
+                     "binding": "valu issue", **valu_ceiling(KERNEL, m_avg),
                      "kernel": KERNEL, "avg_launch_ms": m_avg,
                      "algorithmic_bytes_per_launch": m_bytes, "pairs_per_launch": m_ppl},
     }
     if want:
         line["workloads"] = run_workloads()
+    if args.developed:
+        line["error_mixed_vs_fp64_developed"] = developed_run()
+        print("[bench] developed run done", file=sys.stderr, flush=True)
     if not args.no_cpu:
-        n, t, threads = cpu_baseline(case, args.cpu_budget)
+        n, t, threads, O = cpu_baseline(case, args.cpu_budget)
         pairs_per_step = fp64["pairs"] / k
         line["cpu_baseline"] = {"value": n * pairs_per_step / t, "unit": "pair-interactions/s",
                                 "cores": threads, "kind": "port",
                                 "sample": f"{n} full symplectic steps of the same 740k-particle "
                                           f"dam break (oracle, OpenMP, {threads} threads), "
                                           f"{t:.1f} s",
-                                "steps_per_s": n / t}
-        n1, t1, _ = cpu_baseline(case, args.cpu_budget / 3, threads=1, max_steps=5)
+                                "steps_per_s": n / t, "host": host_info()}
+        print(f"[bench] cpu baseline done ({threads} threads)", file=sys.stderr, flush=True)
+        if args.oracle_steps > 0:
+            # the same oracle run goes on, as the checker, to the step count the
+            # full-size error is quoted at
+            for _ in range(max(args.oracle_steps - n, 0)):
+                from tests.helpers import oracle_step
+                oracle_step(O, case.wall, case.dt)
+            steps_full = max(args.oracle_steps, n)
+            gpu_full = {nm: gpu_fields(case, p, steps_full)[0]
+                        for nm, p in (("mixed", T.PREC_MIXED), ("fp64", T.PREC_FP64))}
+            line["error_vs_oracle"] = errors_vs_oracle(threads, O, case, steps_full, gpu_full)
+        del O
+        n1, t1, _, _ = cpu_baseline(case, args.cpu_budget / 3, threads=1, max_steps=5)
         line["cpu_baseline"]["single_thread"] = {
             "value": n1 * pairs_per_step / t1, "unit": "pair-interactions/s", "cores": 1,
             "sample": f"{n1} full symplectic steps, 1 thread, {t1:.1f} s"}

@@ -256,6 +256,15 @@ int tsph_bz_integrate_inlet(tsph_ctx *ctx, int zone, double dt);
  * becomes the zone type (a = 0, drho = 0); zone particles past the zone end or
  * below dp/2 are erased, with the reference's in-loop skip of the next particle. */
 int tsph_bz_integrate_outlet(tsph_ctx *ctx, int zone, double dt);
+/* Kahan-compensated FP64 position and density updates in the symplectic
+ * corrector (on != 0), where the reference's corrector
+ * (src/SPH_integration.h:287-309) rounds each update once: the rounding error
+ * of every step is carried to the next, so the drift of r and rho over long
+ * runs stays at the rounding level of one step instead of growing with the
+ * step count.  Results differ from the uncompensated path at that level.
+ * Adds 24 B per particle (two generations), moved with the particles; the
+ * terms start at zero (here and at every tsph_upload).  on = 0 frees them. */
+int tsph_set_compensated(tsph_ctx *ctx, int on);
 /* Relabels the reference positions of the live particles 0..np-1, keeping
  * their order.  No counterpart in the reference, where positions are vector
  * indices that every erase renumbers (src/SPH_particle_system.h:125-145).

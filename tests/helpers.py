@@ -155,3 +155,31 @@ def oracle_reinit(O) -> int:
     """Density_reinitialization_sumation + _overwrite; returns the out-of-range count."""
     O.call("reinit_sum")
     return O.call("reinit_overwrite")
+
+
+def oracle_ale(case, threads: int = 0):
+    """Oracle ALE case: ALESPH_Add_particle's volume variables, the grid and the
+    first cell list (cases/ALESPH_dambreakLong/SPH_simulation.h INIT)."""
+    O = oracle_for(case, threads)
+    O.call("init_ale")
+    O.call("divide_to_cells")
+    assert O.call("particle_to_cell") == 0
+    return O
+
+
+def oracle_ale_step(O, step, dt, muscl):
+    """cases/ALESPH_dambreakLong/SPH_simulation.h:78-125 (gradients after
+    Compute_Density where the driver has the commented call, :103).
+    muscl: 0 none, 1 linear, 2 minmod (TSPH_MUSCL_*)."""
+    sfx = "_limited" if muscl == 2 else ""
+    if step > 1:
+        O.call("ale_leapfrog_one", dt)
+    O.call("clear_cells")
+    assert O.call("particle_to_cell") == 0
+    O.call("ale_compute_density" + sfx)
+    if muscl:
+        O.call("ale_kernel_gradient")
+    O.call("ale_density_pressure", dt)
+    O.call("ale_compute_acceleration" + sfx)
+    O.call("ale_leapfrog_two", dt, step)
+    O.call("remove_out_of_domain")

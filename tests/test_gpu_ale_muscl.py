@@ -12,7 +12,7 @@ import pytest
 
 import tinysph_old_amd as T
 from oracle import pyoracle as po
-from tests.helpers import oracle_for, rel_err, rel_l2
+from tests.helpers import oracle_ale, oracle_ale_step, oracle_for, rel_err, rel_l2
 
 pytestmark = pytest.mark.gpu
 
@@ -28,31 +28,6 @@ def _oracle(oracle_built):
 def ale_case(dx=DX):
     print(f"[case] ale_dam_break dx={dx}", flush=True)
     return T.build_case(T.CASE_ALE_DAM_BREAK, dx=dx)
-
-
-def oracle_ale(case):
-    O = oracle_for(case)
-    O.call("init_ale")
-    O.call("divide_to_cells")
-    assert O.call("particle_to_cell") == 0
-    return O
-
-
-def oracle_ale_step(O, step, dt, muscl):
-    """cases/ALESPH_dambreakLong/SPH_simulation.h:78-125 (gradients after
-    Compute_Density where the driver has the commented call, :103)."""
-    sfx = "_limited" if muscl == T.MUSCL_MINMOD else ""
-    if step > 1:
-        O.call("ale_leapfrog_one", dt)
-    O.call("clear_cells")
-    assert O.call("particle_to_cell") == 0
-    O.call("ale_compute_density" + sfx)
-    if muscl:
-        O.call("ale_kernel_gradient")
-    O.call("ale_density_pressure", dt)
-    O.call("ale_compute_acceleration" + sfx)
-    O.call("ale_leapfrog_two", dt, step)
-    O.call("remove_out_of_domain")
 
 
 ALE_CMP = {"omega": po.F_OMEGA, "omegarho": po.F_OMEGARHO, "omegav": po.F_OMEGAV,

@@ -193,3 +193,76 @@ def test_verlet_driver_mixed():
     assert np.array_equal(g["type"], O.types())
     assert rel_l2(g["rho"], O.get(po.F_RHO)) <= 1e-4
     assert rel_l2(g["r"], O.get(po.F_R)) <= 1e-6
+
+
+def isolated_movers(n=256, seed=5):
+    """n fluid particles more than 2h apart (no pair terms: a = -g, dRho = 0),
+    moving at vx = 1 m/s, at x in [1, 8) m inside a tall, wide domain."""
+    rng = np.random.default_rng(seed)
+    dp = 0.01
+    h = 1.4 * dp
+    xs = 1.0 + 7.0 * rng.random(n)
+    ys = 40.0 + 0.1 * np.arange(n)  # 0.1 m > 2h apart in y
+    arrays = dict(type=np.zeros(n, np.int32), p=np.zeros(n), rho=np.full(n, 1000.0),
+                  r=np.stack([xs, ys], 1), v=np.tile([1.0, 0.0], (n, 1)))
+    params = dict(h=h, kap=2.0, cs=20.0, rho0=1000., avisc=0.01, delta=0.1, dp=dp,
+                  x_0=0.0, x_m=12.0, y_0=-5.0, y_m=70.0)
+    return params, arrays
+
+
+def test_compensated_corrector_keeps_position_at_rounding_level():
+    """Kahan-compensated corrector (tsph_set_compensated) against the exact
+    sum: particles moving at a constant vx = 1 m/s for 20000 symplectic steps
+    advance by exactly (v + v_o) dt / 2 = dt per step, so x_n = x_0 + n dt in
+    exact arithmetic (evaluated here with fractions).  The plain FP64 corrector
+    (src/SPH_integration.h:287-309) rounds every step's sum and drifts; the
+    compensated one stays within one ulp of the exact value."""
+    from fractions import Fraction
+    params, arrays = isolated_movers()
+    dt, steps = 1.234567e-4, 20000
+    out = {}
+    for comp in (False, True):
+        with T.ParticleSystem(params, precision=T.PREC_FP64, capacity=len(arrays["type"])) as ps:
+            ps.upload(arrays)
+            if comp:
+                ps.set_compensated(True)
+            ps.particle_to_cell()
+            ps.step(T.WALL_DBC, dt, steps)
+            got = ps.download(("type", "r", "v"))
+        assert np.array_equal(got["type"], arrays["type"])
+        assert np.all(got["v"][:, 0] == 1.0)
+        out[comp] = got["r"][:, 0]
+    fdt = Fraction(dt)
+    exact = np.array([float(Fraction(float(x0)) + steps * fdt) for x0 in arrays["r"][:, 0]])
+    ulp = np.spacing(exact)
+    err_plain = np.abs(out[False] - exact) / ulp
+    err_comp = np.abs(out[True] - exact) / ulp
+    print(f"[kahan] {steps} steps: plain mean {err_plain.mean():.1f} ulp (max {err_plain.max():.0f}), "
+          f"compensated mean {err_comp.mean():.2f} ulp (max {err_comp.max():.0f})", flush=True)
+    assert err_comp.max() <= 1.0
+    assert err_plain.mean() >= 10.0 * max(err_comp.mean(), 0.05)
+
+
+def test_compensated_inflow_matches_oracle():
+    """The compensated corrector on the inflow slab (appends, conversions,
+    erases) stays within the FP64 bounds of the oracle's uncompensated run."""
+    from tests.helpers import oracle_add_zones, oracle_inflow_step
+    case = T.build_case(T.CASE_INFLOW_SLAB, dx=0.01)
+    steps = 150
+    print(f"[case] inflow slab dx=0.01 np={case.np} steps={steps} compensated", flush=True)
+    O = oracle_for(case)
+    oracle_add_zones(O, case.zones)
+    oracle_init(O)
+    for _ in range(steps):
+        oracle_inflow_step(O, case.wall, case.dt)
+    with T.system_for_case(case) as ps:
+        ps.set_compensated(True)
+        ps.particle_to_cell()
+        ps.step_inflow(case.wall, case.dt, steps)
+        g = ps.download(("type", "r", "v", "rho", "p"))
+        assert ps.counters()["zone_rejected"] == 0
+    assert np.array_equal(g["type"], O.types())
+    assert rel_err(g["r"], O.get(po.F_R)) <= 1e-9
+    assert rel_err(g["v"], O.get(po.F_V)) <= 1e-9
+    assert rel_err(g["rho"], O.get(po.F_RHO)) <= 1e-9
+    assert rel_err(g["p"], O.get(po.F_P)) <= 1e-6

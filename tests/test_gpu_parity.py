@@ -148,20 +148,37 @@ def test_steps_fp64(name):
     assert rel_err(got["p"], ref["p"]) <= 1e-6
 
 
-def test_graph_replay_equals_eager():
-    case = make("dam_break_mdbc")
-    with T.system_for_case(case) as a, T.system_for_case(case) as b:
+def stage_step(ps, wall, dt):
+    """One symplectic step launched stage by stage, like the drivers
+    (cases/damBreakLong_mDBC/SPH_simulationSymplectic.h:59-116)."""
+    variant = T.forces_variant(wall)
+    if wall == T.WALL_MDBC:
+        ps.mdbc_density(); ps.density_to_pressure()
+    elif wall == T.WALL_BT:
+        ps.bt_density(); ps.density_to_pressure()
+    ps.predictor(dt)
+    ps.clear_cells(); ps.particle_to_cell(); ps.density_to_pressure()
+    ps.compute_forces(variant); ps.corrector(dt); ps.remove_out_of_domain()
+    ps.clear_cells(); ps.particle_to_cell(); ps.density_to_pressure()
+    ps.compute_forces(variant)
+
+
+@pytest.mark.parametrize("name", list(CASES))
+@pytest.mark.parametrize("precision", [T.PREC_FP64, T.PREC_MIXED], ids=["fp64", "mixed"])
+def test_graph_replay_equals_eager(name, precision):
+    """The captured step (wall densities on a second stream beside the first
+    cell build's sort; on the mixed path the corrector inside the first pair
+    pass) equals the stages launched one by one, bit for bit."""
+    case = make(name)
+    with T.system_for_case(case, precision) as a, T.system_for_case(case, precision) as b:
         a.particle_to_cell()
         b.particle_to_cell()
         a.step(case.wall, case.dt, 5)  # graph
-        for _ in range(5):  # stage by stage, like the driver
-            b.mdbc_density(); b.density_to_pressure(); b.predictor(case.dt)
-            b.clear_cells(); b.particle_to_cell(); b.density_to_pressure()
-            b.compute_forces(T.FORCES_WCSPH); b.corrector(case.dt); b.remove_out_of_domain()
-            b.clear_cells(); b.particle_to_cell(); b.density_to_pressure()
-            b.compute_forces(T.FORCES_WCSPH)
+        for _ in range(5):
+            stage_step(b, case.wall, case.dt)
         ga, gb = a.download(), b.download()
-    for k in ("r", "v", "rho", "p", "a", "drho"):
+        assert a.counters()["removed"] == b.counters()["removed"]
+    for k in ("type", "r", "v", "rho", "p", "a", "drho", "r_o", "v_o", "rho_o"):
         assert np.array_equal(ga[k], gb[k]), k
 
 

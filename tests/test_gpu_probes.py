@@ -12,16 +12,21 @@ has it), and every 10 steps the four wall sensors (:188-201, constants.h:85-94:
 Pressure_in_point at x = 1.61 m, y = 3 / 15 / 30 / 80 mm), in the driver's
 units (level / 0.3 m; pressure / (1000 * 9.81 * 0.3) Pa).
 
-Tolerances.  The FP64 path differs from the oracle only where the device pow()
-differs from glibc's (a few ulp per call); a dam break amplifies such
-differences (measured on the oracle alone: one ulp of every fluid density at
-t = 0 moves the wall pressures by 3e-3 of their scale at t = 0.5 s, and by 40 %
-at t = 0.58 s after the impact).  So:
-  * water levels: the gauge marches in steps that grow quadratically
-    (src/SPH_water_level.h:93-109), so a level can only move by whole steps;
-    every sample must agree to within one marching step at its height;
-  * wall pressures up to t = 0.5 s: within 1e-2 of the driver's pressure scale
-    (FP64) and 5e-2 (mixed);
+Tolerances.  A dam break amplifies round-off: the oracle run twice, once with
+every fluid density perturbed by one ulp, is the yardstick, and the fixture
+computes it next to the oracle's own run (at dx = 5 mm: gauges identical to
+t = 0.3 s and within 1.0 mm up to t = 0.5 s; wall pressures within 4e-5 of
+their scale up to the impact at t = 0.44 s and up to 9 times their scale after
+it, where even time-averaged pressures differ by 2-36 % between the two runs).
+The GPU against the oracle:
+  * before the front reaches the second gauge (t <= 0.25 s) every gauge within
+    one marching step (src/SPH_water_level.h:93-109: a level moves by whole
+    steps) and every sensor dry (zero);
+  * every gauge sample within three times the largest gauge difference of the
+    perturbed oracle run, or one marching step if that is larger;
+  * wall pressures up to the impact within 1e-3 of the driver's pressure scale
+    (FP64) and 1e-2 (mixed); after it, within three times the largest
+    difference of the perturbed oracle run;
   * particle fields at t = 0.5 s: relative L2 bounds stated in each test.
 
 The still-water tank of the first workload (1.0 m, 0.5 m of water, dx = 10 mm,
@@ -42,6 +47,8 @@ from tests.helpers import oracle_fields, oracle_for, oracle_init, oracle_step, r
 pytestmark = pytest.mark.gpu
 
 DX, STEPS = 0.005, 10000
+FRONT_ROW_WL = 100    # gauge samples up to t = 0.25 s: before the front reaches x = 0.865 m
+IMPACT_ROW = 880      # sensor samples up to t = 0.44 s: before the impact on the right wall
 WL_EVERY, P_EVERY = 50, 10
 # (x, y0, inverse): cases/damBreakLong_mDBC/constants.h:58-83 in the driver's call order
 GAUGES = [(0.3, 0.0, False), (0.3, 0.45, True), (0.865, 0.0, False), (0.865, 0.45, True),
@@ -81,12 +88,8 @@ def run_series(step, wl, pr, nsteps):
     return np.array(wls) / WL_SCALE, np.array(ps) / P_SCALE
 
 
-@pytest.fixture(scope="module")
-def dam_break():
-    case = T.build_case(T.CASE_DAM_BREAK_MDBC_REF, dx=DX)
-    assert abs(case.dt - 5e-5) < 1e-15
-    print(f"[case] reference dam break dx={DX} np={case.np} steps={STEPS}", flush=True)
-    O = oracle_for(case, threads=threads())
+def oracle_series(case, arrays):
+    O = po.oracle_case(case.oracle_params(), arrays, omp_threads=threads())
     oracle_init(O)
 
     def ostep(k):
@@ -95,9 +98,25 @@ def dam_break():
 
     wl = lambda x, y0, inv: O.call("water_elevation_inverse" if inv else "water_elevation", x, y0)
     pr = lambda x, y: O.interpolate_point(x, y)[1]
-    series = run_series(ostep, wl, pr, STEPS)
-    print(f"[oracle] done, np={O.np}", flush=True)
-    return case, series, oracle_fields(O)
+    return run_series(ostep, wl, pr, STEPS), oracle_fields(O)
+
+
+@pytest.fixture(scope="module")
+def dam_break():
+    """The oracle's series and fields, and the envelope of the same run with
+    every fluid density perturbed by one ulp (what round-off alone moves)."""
+    case = T.build_case(T.CASE_DAM_BREAK_MDBC_REF, dx=DX)
+    assert abs(case.dt - 5e-5) < 1e-15
+    print(f"[case] reference dam break dx={DX} np={case.np} steps={STEPS}", flush=True)
+    series, fields = oracle_series(case, case.arrays)
+    pert = {k: (np.array(v, copy=True) if v is not None else None) for k, v in case.arrays.items()}
+    pert["rho"][pert["type"] == 0] *= 1.0 + 2.0 ** -52
+    env_series, _ = oracle_series(case, pert)
+    env = (np.abs(env_series[0] - series[0]), np.abs(env_series[1] - series[1]))
+    print(f"[oracle] done, np={fields['type'].size}; one-ulp envelope: gauges max "
+          f"{env[0].max() * WL_SCALE:.3e} m, sensors max {env[1].max():.3e} (to the impact "
+          f"{env[1][:IMPACT_ROW].max():.3e})", flush=True)
+    return case, series, fields, env
 
 
 def gpu_series(case, precision):
@@ -109,26 +128,35 @@ def gpu_series(case, precision):
         return series, ps.download()
 
 
-def check_series(case, got, ref, p_tol):
-    (wl_g, p_g), (wl_o, p_o) = got, ref
+def check_series(case, got, ref, env, p_tol):
+    (wl_g, p_g), (wl_o, p_o), (env_wl, env_p) = got, ref, env
     assert wl_g.shape == wl_o.shape == (STEPS // WL_EVERY, len(GAUGES))
     assert p_g.shape == p_o.shape == (STEPS // P_EVERY, len(SENSORS))
     dp = case.params["dp"]
     # every gauge sees water by the end (the front reaches x = 1.3625 m)
     assert (wl_o[-1, 0::2] * WL_SCALE > 0.02).all(), wl_o[-1]
-    tol = np.array([[marching_step(v * WL_SCALE, y0, dp) for v, (_, y0, _) in zip(row, GAUGES)]
-                    for row in wl_o]) / WL_SCALE
+    step = np.array([[marching_step(v * WL_SCALE, y0, dp) for v, (_, y0, _) in zip(row, GAUGES)]
+                     for row in wl_o]) / WL_SCALE
     dwl = np.abs(wl_g - wl_o)
-    exact = float((dwl == 0).mean())
     dpr = np.abs(p_g - p_o)
-    print(f"[water level] max |d| {dwl.max() * WL_SCALE:.3e} m, {exact:.1%} of samples identical; "
-          f"[wall pressure] max |d| {dpr.max():.3e} of rho0 g 0.3, at step "
-          f"{(np.unravel_index(dpr.argmax(), dpr.shape)[0] + 1) * P_EVERY}; peak |p| "
-          f"{np.abs(p_o).max():.3f}", flush=True)
-    assert (dwl <= tol + 1e-12).all(), np.argwhere(dwl > tol + 1e-12)[:5]
-    # the sensors see the impact before the end
+    exact = float((dwl == 0).mean())
+    print(f"[water level] to t=0.25 s max |d| {dwl[:FRONT_ROW_WL].max() * WL_SCALE:.3e} m; whole run "
+          f"max |d| {dwl.max() * WL_SCALE:.3e} m, {exact:.1%} of samples identical "
+          f"(one-ulp oracle: {float((env_wl == 0).mean()):.1%}); [wall pressure] to the impact max |d| "
+          f"{dpr[:IMPACT_ROW].max():.3e}, after it {dpr[IMPACT_ROW:].max():.3e} of rho0 g 0.3; "
+          f"peak |p| {np.abs(p_o).max():.3f}", flush=True)
+    # before the front reaches the second gauge: within one marching step, sensors dry
+    early = dwl[:FRONT_ROW_WL]
+    assert (early <= step[:FRONT_ROW_WL] + 1e-12).all(), np.argwhere(early > step[:FRONT_ROW_WL] + 1e-12)[:5]
+    assert np.abs(p_o[:FRONT_ROW_WL * WL_EVERY // P_EVERY]).max() == 0.0
+    assert np.abs(p_g[:FRONT_ROW_WL * WL_EVERY // P_EVERY]).max() == 0.0
+    # the whole run: within three times what one ulp moves the oracle by
+    tol_wl = np.maximum(step, 3.0 * env_wl.max())
+    assert (dwl <= tol_wl + 1e-12).all(), np.argwhere(dwl > tol_wl + 1e-12)[:5]
+    # the sensors see the impact; pointwise parity up to it, chaos after it
     assert np.abs(p_o).max() > 0.5
-    assert dpr.max() <= p_tol, dpr.max()
+    assert dpr[:IMPACT_ROW].max() <= p_tol, dpr[:IMPACT_ROW].max()
+    assert dpr[IMPACT_ROW:].max() <= 3.0 * env_p.max() + p_tol
     return exact
 
 
@@ -141,10 +169,10 @@ def field_errors(got, ref):
 
 @pytest.mark.timeout(600)
 def test_water_levels_and_wall_pressures_fp64(dam_break):
-    case, ref_series, ref_fields = dam_break
+    case, ref_series, ref_fields, env = dam_break
     series, fields = gpu_series(case, T.PREC_FP64)
-    exact = check_series(case, series, ref_series, p_tol=1e-2)
-    assert exact >= 0.9
+    exact = check_series(case, series, ref_series, env, p_tol=1e-3)
+    assert exact >= 0.85
     errs = field_errors(fields, ref_fields)
     print(f"[fields fp64 t=0.5 s] {errs}", flush=True)
     assert errs["r"] <= 1e-4 and errs["v"] <= 1e-2
@@ -153,9 +181,9 @@ def test_water_levels_and_wall_pressures_fp64(dam_break):
 
 @pytest.mark.timeout(600)
 def test_water_levels_and_wall_pressures_mixed(dam_break):
-    case, ref_series, ref_fields = dam_break
+    case, ref_series, ref_fields, env = dam_break
     series, fields = gpu_series(case, T.PREC_MIXED)
-    check_series(case, series, ref_series, p_tol=5e-2)
+    check_series(case, series, ref_series, env, p_tol=1e-2)
     errs = field_errors(fields, ref_fields)
     print(f"[fields mixed t=0.5 s] {errs}", flush=True)
     assert errs["r"] <= 1e-3 and errs["v"] <= 1e-1

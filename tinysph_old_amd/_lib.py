@@ -124,6 +124,7 @@ EXPORTS = {
     "tsph_bz_integrate_inlet": (C.c_int, [C.c_void_p, C.c_int, C.c_double]),
     "tsph_bz_integrate_outlet": (C.c_int, [C.c_void_p, C.c_int, C.c_double]),
     "tsph_renumber": (C.c_int, [C.c_void_p]),
+    "tsph_set_compensated": (C.c_int, [C.c_void_p, C.c_int]),
     "tsph_step_inflow": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int]),
     "tsph_case_build": (C.c_int, [C.c_int, C.c_double, C.c_double, C.c_uint64,
                                   C.POINTER(C.POINTER(CaseC))]),

@@ -52,7 +52,7 @@ static void free_soa(Soa &s) {
   void *ps[] = {s.cid, s.r, s.v, s.a, s.r_o, s.v_o, s.gnr, s.n, s.rho, s.drho, s.p, s.rho_o,
                 s.type, s.seq, s.index, s.omega, s.omegarho, s.omega_o, s.omegarho_o,
                 s.domega, s.domegarho, s.omegav, s.omegav_o, s.domegav, s.gradrho, s.gradvx,
-                s.gradvy, s.v_oo, s.rho_oo, s.rho_tmp};
+                s.gradvy, s.v_oo, s.rho_oo, s.rho_tmp, s.rc, s.rhoc};
   for (void *p : ps)
     if (p) (void)hipFree(p);
   s = Soa{};
@@ -81,7 +81,8 @@ static void set_walk_defaults(tsph_ctx *c) {
 }
 
 static HostState host_state(const tsph_ctx *c) {
-  return {c->cells_valid, c->tiles_valid, c->records_valid, c->marks_valid, c->list_groups};
+  return {c->cells_valid, c->tiles_valid, c->records_valid, c->marks_valid, c->olds_stale,
+          c->list_groups};
 }
 
 static void set_host_state(tsph_ctx *c, const HostState &h) {
@@ -90,25 +91,41 @@ static void set_host_state(tsph_ctx *c, const HostState &h) {
   c->list_groups = h.list_groups;
   c->records_valid = h.records_valid;
   c->marks_valid = h.marks_valid;
+  c->olds_stale = h.olds_stale;
 }
 
 // Runs nsteps steps from graphs: the graph of the current buffer parity is
 // captured from `record` (one step's launches) unless it already holds a step
 // for the same tag, dt and launch shape.  Recording mutates the host state as
 // launching would (buffer swaps, flags); a replay applies the same changes.
+// Steps other than the symplectic ones (record_step) read or move r_o, v_o,
+// rho_o in their first launches, so a symplectic step before them must have
+// its left-behind olds gathered first, eagerly: a gather captured into their
+// graph would repeat on every replay.
+static bool step_reads_olds(const Graph *slots, const tsph_ctx *c) {
+  return slots != c->graph && slots != c->inflow_graph;
+}
+
 template <typename R>
 static int replay_step(tsph_ctx *c, Graph *slots, int tag, double dt, int nsteps, R record) {
+  if (step_reads_olds(slots, c))
+    if (int e = ensure_olds(c)) return e;
   for (int s = 0; s < nsteps; ++s) {
     Graph &g = slots[c->parity];
     if (!g.exec || g.wall != tag || g.dt != dt || g.n_ub != c->n_ub) {
       if (g.exec) (void)hipGraphExecDestroy(g.exec);
       g = Graph{};
       const Soa cur0 = c->cur, alt0 = c->alt;
+      int *const cs0 = c->cell_start, *const csn0 = c->cell_start_next;
+      const int nwall0 = c->nwall_idx;
       const int parity0 = c->parity;
       const HostState h0 = host_state(c);
       auto undo = [&] {  // nothing was launched: restore the host view
         c->cur = cur0;
         c->alt = alt0;
+        c->cell_start = cs0;
+        c->cell_start_next = csn0;
+        c->nwall_idx = nwall0;
         c->parity = parity0;
         set_host_state(c, h0);
       };
@@ -135,8 +152,10 @@ static int replay_step(tsph_ctx *c, Graph *slots, int tag, double dt, int nsteps
       g.flips = c->parity != parity0;
       g.post = host_state(c);
     } else {
-      if (g.flips) {
+      if (g.flips) {  // what an odd number of cell builds leaves (launch_cell_list)
         std::swap(c->cur, c->alt);
+        std::swap(c->cell_start, c->cell_start_next);
+        c->nwall_idx = c->nwall_idx == S_NWALL ? S_NWALL_B : S_NWALL;
         c->parity ^= 1;
       }
       set_host_state(c, g.post);
@@ -211,8 +230,25 @@ static int record_step(tsph_ctx *c, int wall, double dt, int inlet = -1, int out
   // Without buffer zones nothing reads the pressures (or the mixed records)
   // this EOS writes before the cell build rewrites them all, so only its
   // out-of-range density count is taken, by the wall-density pass.
+  // Without buffer zones the wall densities run on the side stream while the
+  // first cell build sorts: the pass reads the previous cell list and writes
+  // wall densities, the sort reads positions only and writes the other
+  // cell-offset buffer and wall-count slot; the build's reorder (which reads
+  // the densities) waits for it.
+  hipEvent_t walls_done = nullptr;
   if (wall == TSPH_WALL_MDBC || wall == TSPH_WALL_BT) {
-    if ((e = wall == TSPH_WALL_MDBC ? launch_mdbc(c, !zones) : launch_bt_density(c, !zones))) return e;
+    hipStream_t ws = nullptr;
+    if (!zones) {
+      TSPH_HIP(hipEventRecord(c->fork_ev, c->stream));
+      TSPH_HIP(hipStreamWaitEvent(c->side, c->fork_ev, 0));
+      ws = c->side;
+    }
+    if ((e = wall == TSPH_WALL_MDBC ? launch_mdbc(c, !zones, ws) : launch_bt_density(c, !zones, ws)))
+      return e;
+    if (!zones) {
+      TSPH_HIP(hipEventRecord(c->join_ev, c->side));
+      walls_done = c->join_ev;
+    }
     if (zones && (e = launch_eos(c))) return e;
   }
   if (zones) {
@@ -221,14 +257,21 @@ static int record_step(tsph_ctx *c, int wall, double dt, int inlet = -1, int out
     if (inlet >= 0 && (e = launch_bz_inlet(c, inlet, dt))) return e;
     if (outlet >= 0 && (e = launch_bz_outlet(c, outlet, dt))) return e;
     if ((e = launch_cell_list(c, true, false, 0., active))) return e;
-  } else if ((e = launch_cell_list(c, true, true, dt, active))) {  // predictor applied by the build
+  } else if ((e = launch_cell_list(c, true, true, dt, active, false, walls_done))) {
+    return e;  // (the predictor is applied by the build)
+  }
+  // the corrector writes the removal states; the build takes the removal
+  // decision (RemoveParticlesOutOfDomain) on the way.  The mixed pair pass
+  // applies the corrector itself (it never reads the FP64 state it updates).
+  if (c->prm.precision == TSPH_PREC_MIXED) {
+    if ((e = launch_forces(c, variant, dt))) return e;
+  } else if ((e = launch_forces(c, variant)) || (e = launch_corrector(c, dt))) {
     return e;
   }
-  if ((e = launch_forces(c, variant))) return e;
-  // the corrector writes the removal states; the build takes the removal
-  // decision (RemoveParticlesOutOfDomain) on the way
-  if ((e = launch_corrector(c, dt))) return e;
-  if ((e = launch_cell_list(c, true, false, 0., active, true)) || (e = launch_forces(c, variant)))
+  // r_o, v_o, rho_o are dead after the corrector until the next predictor
+  // rewrites them: this build leaves them behind
+  if ((e = launch_cell_list(c, true, false, 0., active, true, nullptr, true)) ||
+      (e = launch_forces(c, variant)))
     return e;
   return 0;
 }
@@ -311,7 +354,11 @@ static int make_consts(const tsph_params *prm, Consts &k) {
 // device buffers of a context holding `cap` particles over `ncells_tot` cells
 static int alloc_context(tsph_ctx *c) {
   int e = 0;
-  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+  // both streams at the default priority
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->join_ev, hipEventDisableTiming) != hipSuccess)
     return fail(TSPH_EHIP, "hipStreamCreate failed");
   const int cap = c->capacity, nct = c->ncells_tot;
   c->seq_cap = 2 * cap + kBzMax + c->ncases;
@@ -319,6 +366,7 @@ static int alloc_context(tsph_ctx *c) {
     return e;
   if ((e = dalloc(&c->scal, S_NSCAL)) ||
       (e = dalloc(&c->cell_count, nct + 1)) || (e = dalloc(&c->cell_start, nct + 2)) ||
+      (e = dalloc(&c->cell_start_next, nct + 2)) ||
       (e = dalloc(&c->cellof, cap)) || (e = dalloc(&c->hrank, cap)) || (e = dalloc(&c->hcell, cap)) ||
       (e = dalloc(&c->perm, cap)) ||
       (e = dalloc(&c->seqstate, (size_t)c->seq_cap)) || (e = dalloc(&c->probe_out, 4)) ||
@@ -413,13 +461,16 @@ int tsph_destroy(tsph_ctx *c) {
   drop_graph(c);
   free_soa(c->cur);
   free_soa(c->alt);
-  void *ps[] = {c->scal, c->counters, c->cell_count, c->cell_start, c->cellof, c->hrank,
+  void *ps[] = {c->scal, c->counters, c->cell_count, c->cell_start, c->cell_start_next, c->cellof, c->hrank,
                 c->hcell, c->perm, c->scan_state, c->seqstate, c->probe_out, c->mx.rv,
                 c->mx.rp, c->mx.nrm, c->tile_list, c->wall_list, c->bz_list, c->cases,
                 c->tile_base, c->list_part, c->tile_bound};
   for (void *p : ps)
     if (p) (void)hipFree(p);
   for (hipEvent_t ev : c->tev) (void)hipEventDestroy(ev);
+  if (c->fork_ev) (void)hipEventDestroy(c->fork_ev);
+  if (c->join_ev) (void)hipEventDestroy(c->join_ev);
+  if (c->side) (void)hipStreamDestroy(c->side);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return 0;
@@ -494,6 +545,10 @@ static int upload_impl(tsph_ctx *c, const tsph_particles *h, const uint32_t *seq
     if (h->n) TSPH_HIP(cp(d.n, h->n, s2));
     else if (c->has_n) TSPH_HIP(cp(d.n, zeros.data(), s2));
   }
+  if (d.rc) {  // compensation starts from zero
+    TSPH_HIP(hipMemsetAsync(d.rc, 0, sizeof(double2) * (size_t)c->capacity, c->stream));
+    TSPH_HIP(hipMemsetAsync(d.rhoc, 0, sizeof(double) * (size_t)c->capacity, c->stream));
+  }
   int sc[S_NSCAL] = {};
   sc[S_NP] = n;
   sc[S_MAXSEQ] = maxseq;
@@ -504,6 +559,7 @@ static int upload_impl(tsph_ctx *c, const tsph_particles *h, const uint32_t *seq
   c->n_ub = n;
   c->n_wall = 0;
   for (int i = 0; i < n; ++i) c->n_wall += t[i] == TSPH_WALL;
+  c->olds_stale = false;
   c->cells_valid = false;
   c->tiles_valid = false;
   c->records_valid = false;
@@ -610,6 +666,7 @@ static int seq_room(tsph_ctx *c, int *passes) {
 
 static int download_impl(tsph_ctx *c, tsph_particles *h, int icase) {
   int n = 0, e = 0;
+  if ((e = ensure_olds(c))) return e;
   TSPH_HIP(hipStreamSynchronize(c->stream));
   TSPH_HIP(hipMemcpy(&n, &c->scal[S_NP], sizeof(int), hipMemcpyDeviceToHost));
   std::vector<int> order;
@@ -1125,6 +1182,31 @@ int tsph_pair_timing_read(tsph_ctx *c, double *total_ms, int *launches) {
 
 
 // ---- other WCSPH integrators and density reinitialisation ---------------------
+
+int tsph_set_compensated(tsph_ctx *c, int on) {
+  if (!c) return fail(TSPH_EINVAL, "null argument");
+  TSPH_HIP(hipStreamSynchronize(c->stream));
+  drop_graph(c);
+  if (!on) {
+    for (Soa *d : {&c->cur, &c->alt}) {
+      if (d->rc) (void)hipFree(d->rc);
+      if (d->rhoc) (void)hipFree(d->rhoc);
+      d->rc = nullptr;
+      d->rhoc = nullptr;
+    }
+    return 0;
+  }
+  if (!c->cur.rc) {
+    for (Soa *d : {&c->cur, &c->alt})
+      if (int e = dalloc(&d->rc, c->capacity)) return e;
+    for (Soa *d : {&c->cur, &c->alt})
+      if (int e = dalloc(&d->rhoc, c->capacity)) return e;
+    TSPH_HIP(hipMemsetAsync(c->cur.rc, 0, sizeof(double2) * (size_t)c->capacity, c->stream));
+    TSPH_HIP(hipMemsetAsync(c->cur.rhoc, 0, sizeof(double) * (size_t)c->capacity, c->stream));
+    TSPH_HIP(hipStreamSynchronize(c->stream));
+  }
+  return 0;
+}
 
 int tsph_history_enable(tsph_ctx *c) {
   if (!c) return fail(TSPH_EINVAL, "null argument");

@@ -294,7 +294,7 @@ __global__ __launch_bounds__(kScanBlock) void k_scan(const int *__restrict__ in,
                                                      unsigned long long *__restrict__ state,
                                                      int *__restrict__ out, int *__restrict__ scal,
                                                      unsigned long long *__restrict__ counters,
-                                                     TileScan ts, int spin_limit) {
+                                                     TileScan ts, int spin_limit, int nwall_idx) {
   __shared__ int lds[3 * kScanBlock / 64];
   constexpr int kTile = kScanBlock * IT;
   __shared__ int s_cnt[kTile + kMaxTileRows];
@@ -414,7 +414,7 @@ __global__ __launch_bounds__(kScanBlock) void k_scan(const int *__restrict__ in,
     if (tl[j] >= 0) ts.list[et++] = tl[j];
   }
   if (threadIdx.x == 0) {
-    if (blk == 0) scal[S_NWALL] = 0;
+    if (blk == 0) scal[nwall_idx] = 0;  // the reorder that follows counts the walls
     if (blk == nb - 1) {
       out[len] = pc + total;
       scal[S_NTILES] = pt + tiles;
@@ -597,7 +597,8 @@ __global__ void k_reorder(const int *__restrict__ cell_start, int ncells, const 
                           Soa src, Soa dst, int *__restrict__ scal, int *__restrict__ wall_list,
                           const int *__restrict__ cellof, Consts k0,
                           const Consts *__restrict__ cases, Mixed mx, double dt0,
-                          unsigned ad_skip, unsigned long long *__restrict__ counters) {
+                          unsigned ad_skip, unsigned long long *__restrict__ counters,
+                          int nwall_idx, bool move_olds) {
   const int n_new = cell_start[ncells];
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (blockIdx.x == 0 && threadIdx.x == 0) scal[S_NP] = n_new;
@@ -624,7 +625,7 @@ __global__ void k_reorder(const int *__restrict__ cell_start, int ncells, const 
       dst.v_o[k] = v;
       dst.rho_o[k] = rho;
       predict(r, v, rho, a, drho, dt);
-    } else {
+    } else if (move_olds) {
       dst.r_o[k] = src.r_o[j];
       dst.v_o[k] = src.v_o[j];
       dst.rho_o[k] = src.rho_o[j];
@@ -677,12 +678,16 @@ __global__ void k_reorder(const int *__restrict__ cell_start, int ncells, const 
       dst.rho_oo[k] = src.rho_oo[j];
       dst.rho_tmp[k] = src.rho_tmp[j];
     }
+    if (src.rc) {
+      dst.rc[k] = src.rc[j];
+      dst.rhoc[k] = src.rhoc[j];
+    }
   }
   const unsigned long long m = __ballot(wall);
   if (m) {
     const int lane = threadIdx.x & 63;
     int base = 0;
-    if (lane == __builtin_ctzll(m)) base = atomicAdd(&scal[S_NWALL], __popcll(m));
+    if (lane == __builtin_ctzll(m)) base = atomicAdd(&scal[nwall_idx], __popcll(m));
     base = __shfl(base, __builtin_ctzll(m), 64);
     if (wall) wall_list[base + __popcll(m & lanemask_lt())] = k;
   }
@@ -692,22 +697,35 @@ __global__ void k_reorder(const int *__restrict__ cell_start, int ncells, const 
   }
 }
 
+// r_o, v_o, rho_o of slot k taken from the generation before the last build
+// (cur <- alt through perm), for a build that left them behind
+__global__ void k_gather_olds(const int *__restrict__ scal, const int *__restrict__ perm, Soa src,
+                              Soa dst) {
+  const int n = scal[S_NP];
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int j = perm[k];
+  dst.r_o[k] = src.r_o[j];
+  dst.v_o[k] = src.v_o[j];
+  dst.rho_o[k] = src.rho_o[j];
+}
+
 // exclusive scan of the cell counts into cell_start, with the tile list of the
 // current tile_rows
-int scan_cells(tsph_ctx *c, hipStream_t s) {
+int scan_cells(tsph_ctx *c, hipStream_t s, int *cs, int nwall_idx) {
   const int len = c->ncells_tot + 1;
   const int groups = pooled_groups(c);
   const TileScan ts{c->k.ncy, c->k.ncells, c->k.cell_base, c->cases, c->tile_base, c->ncases,
                     c->tile_rows, c->tile_list};
   if (scan_items(c) == 4)
     hipLaunchKernelGGL(k_scan<4>, dim3(scan_blocks(c)), dim3(kScanBlock), 0, s, c->cell_count,
-                       len, c->scan_state, c->cell_start, c->scal, c->counters, ts, c->scan_wait);
+                       len, c->scan_state, cs, c->scal, c->counters, ts, c->scan_wait, nwall_idx);
   else
     hipLaunchKernelGGL(k_scan<16>, dim3(scan_blocks(c)), dim3(kScanBlock), 0, s, c->cell_count,
-                       len, c->scan_state, c->cell_start, c->scal, c->counters, ts, c->scan_wait);
+                       len, c->scan_state, cs, c->scal, c->counters, ts, c->scan_wait, nwall_idx);
   // a single grid's chunks are cut by k_order_tiles
   if (c->cases && groups > 0)
-    hipLaunchKernelGGL(k_tile_bounds, dim3(2 * c->num_cu), dim3(256), 0, s, c->cell_start,
+    hipLaunchKernelGGL(k_tile_bounds, dim3(2 * c->num_cu), dim3(256), 0, s, cs,
                        c->scal, c->tile_list, c->cases, c->tile_base, c->ncases, c->tile_rows,
                        c->ncells_tot, groups, c->tile_bound);
   TSPH_HIP(hipGetLastError());
@@ -717,13 +735,30 @@ int scan_cells(tsph_ctx *c, hipStream_t s) {
 
 }  // namespace
 
+int ensure_olds(tsph_ctx *c) {
+  if (!c->olds_stale) return 0;
+  hipLaunchKernelGGL(k_gather_olds, dim3(grid_for(c->n_ub, 256)), dim3(256), 0, c->stream, c->scal,
+                     c->perm, c->alt, c->cur);
+  TSPH_HIP(hipGetLastError());
+  c->olds_stale = false;
+  return 0;
+}
+
 int launch_cell_list(tsph_ctx *c, bool eos, bool predict, double dt, unsigned ad_skip,
-                     bool decide) {
+                     bool decide, hipEvent_t before_reorder, bool skip_olds) {
+  // a build that moves r_o, v_o, rho_o needs them current (the predictor's
+  // build rewrites them instead)
+  if (!predict && !skip_olds)
+    if (int e = ensure_olds(c)) return e;
   const Consts &k = c->k;
   const int nb = 256;
   const int g = grid_for(c->n_ub, nb);
   hipStream_t s = c->stream;
   const int nct = c->ncells_tot;
+  // this build's cell offsets and wall count go to the buffers the current
+  // cell list does not use (swapped in at the end)
+  int *cs = c->cell_start_next;
+  const int wslot = c->nwall_idx == S_NWALL ? S_NWALL_B : S_NWALL;
 #define TSPH_HASH(P, D)                                                                        \
   hipLaunchKernelGGL((k_hash<P, D>), dim3(g), dim3(nb), 0, s, c->cur.r, c->cur.v, c->cur.type,  \
                      c->cur.seq, c->scal, k, c->cases, c->cur.cid, dt, nct, c->cell_count,      \
@@ -733,22 +768,23 @@ int launch_cell_list(tsph_ctx *c, bool eos, bool predict, double dt, unsigned ad
   else if (decide) TSPH_HASH(false, true);
   else TSPH_HASH(false, false);
 #undef TSPH_HASH
-  if (int e = scan_cells(c, s)) return e;
+  if (int e = scan_cells(c, s, cs, wslot)) return e;
   hipLaunchKernelGGL(k_scatter, dim3(g), dim3(nb), 0, s, c->scal, c->hcell, c->hrank,
-                     c->cell_start, c->perm, c->cellof, nct, c->cur.seq, c->seqstate,
+                     cs, c->perm, c->cellof, nct, c->cur.seq, c->seqstate,
                      c->scan_state, scan_blocks(c));
   if (c->cases)
-    hipLaunchKernelGGL(k_order, dim3(grid_for(nct + 1, nb)), dim3(nb), 0, s, c->cell_start, nct,
+    hipLaunchKernelGGL(k_order, dim3(grid_for(nct + 1, nb)), dim3(nb), 0, s, cs, nct,
                        c->cur.seq, c->perm, c->cell_count);
   else
-    hipLaunchKernelGGL(k_order_tiles, dim3(2 * c->num_cu), dim3(nb), 0, s, c->cell_start, c->scal,
+    hipLaunchKernelGGL(k_order_tiles, dim3(2 * c->num_cu), dim3(nb), 0, s, cs, c->scal,
                        c->tile_list, c->tile_rows, k.ncy, nct, c->cur.seq, c->perm, c->cell_count,
                        c->list_groups, c->tile_bound);
+  if (before_reorder) TSPH_HIP(hipStreamWaitEvent(s, before_reorder, 0));
   const Mixed mx = c->prm.precision == TSPH_PREC_MIXED ? c->mx : Mixed{};
 #define TSPH_REORDER(E, P)                                                                    \
-  hipLaunchKernelGGL((k_reorder<E, P>), dim3(g), dim3(nb), 0, s, c->cell_start, nct, c->perm,      \
+  hipLaunchKernelGGL((k_reorder<E, P>), dim3(g), dim3(nb), 0, s, cs, nct, c->perm,                 \
                      c->cur, c->alt, c->scal, c->wall_list, c->cellof, k, c->cases, mx, dt,       \
-                     ad_skip, c->counters)
+                     ad_skip, c->counters, wslot, !skip_olds)
   if (eos && predict) TSPH_REORDER(true, true);
   else if (eos) TSPH_REORDER(true, false);
   else if (predict) TSPH_REORDER(false, true);
@@ -756,6 +792,9 @@ int launch_cell_list(tsph_ctx *c, bool eos, bool predict, double dt, unsigned ad
 #undef TSPH_REORDER
   TSPH_HIP(hipGetLastError());
   std::swap(c->cur, c->alt);
+  std::swap(c->cell_start, c->cell_start_next);
+  c->nwall_idx = wslot;
+  c->olds_stale = skip_olds && !predict;
   c->parity ^= 1;
   c->cells_valid = true;
   c->tiles_valid = true;  // listed (and cut for the pooled walk) by the scan

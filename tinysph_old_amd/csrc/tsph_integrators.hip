@@ -140,6 +140,7 @@ __global__ void k_reinit_overwrite(Soa d, const int *__restrict__ scal,
 }  // namespace
 
 int launch_history_init(tsph_ctx *c) {
+  if (int e = ensure_olds(c)) return e;
   hipLaunchKernelGGL(k_history_init, dim3(grid_for(c->n_ub, 256)), dim3(256), 0, c->stream, c->cur,
                      c->scal);
   TSPH_HIP(hipGetLastError());
@@ -147,6 +148,7 @@ int launch_history_init(tsph_ctx *c) {
 }
 
 int launch_verlet(tsph_ctx *c, double dt, bool euler) {
+  if (int e = ensure_olds(c)) return e;
   if (euler)
     hipLaunchKernelGGL(k_euler, dim3(grid_for(c->n_ub, 256)), dim3(256), 0, c->stream, c->cur,
                        c->scal, dt);
@@ -160,6 +162,7 @@ int launch_verlet(tsph_ctx *c, double dt, bool euler) {
 }
 
 int launch_leapfrog_one(tsph_ctx *c, double dt) {
+  if (int e = ensure_olds(c)) return e;
   hipLaunchKernelGGL(k_lf1_density, dim3(grid_for(c->n_ub, 256)), dim3(256), 0, c->stream, c->cur,
                      c->scal, dt);
   TSPH_HIP(hipGetLastError());
@@ -169,6 +172,7 @@ int launch_leapfrog_one(tsph_ctx *c, double dt) {
 }
 
 int launch_leapfrog_two(tsph_ctx *c, double dt, int step) {
+  if (int e = ensure_olds(c)) return e;
   hipLaunchKernelGGL(k_lf2_density, dim3(grid_for(c->n_ub, 256)), dim3(256), 0, c->stream, c->cur,
                      c->scal, dt, (int)(step == 1));
   TSPH_HIP(hipGetLastError());

@@ -29,7 +29,9 @@ constexpr int kMaxTileRows = 24;  // upper bound of tile_rows
 constexpr int kTileWork0 = 16;    // pooled walk's cost model: per-tile work in own particles
 
 // device scalar slots
-enum { S_NP = 0, S_MAXSEQ = 1, S_NTILES = 2, S_NWALL = 3, S_BZCNT = 4, S_NSCAL = 32 };
+// (the wall count alternates between S_NWALL and S_NWALL_B with the cell-offset
+// buffers: a build fills the other one, see cell_start_next)
+enum { S_NP = 0, S_MAXSEQ = 1, S_NTILES = 2, S_NWALL = 3, S_BZCNT = 4, S_NWALL_B = 5, S_NSCAL = 32 };
 // device counter slots (uint64)
 enum {
   C_PAIRS = 0, C_CAND = 1, C_ACTIVE = 2, C_PASSES = 3, C_BADRHO = 4, C_OUTGRID = 5,
@@ -68,6 +70,10 @@ struct Soa {
   // (allocated by tsph_history_enable)
   double2 *v_oo = nullptr;
   double *rho_oo = nullptr, *rho_tmp = nullptr;
+  // Kahan compensation terms of the symplectic corrector's position and
+  // density updates (allocated by tsph_set_compensated)
+  double2 *rc = nullptr;
+  double *rhoc = nullptr;
 };
 
 // FP32 pair records of the mixed path, rebuilt after every reorder.
@@ -98,7 +104,7 @@ __device__ __forceinline__ Consts case_consts(const Consts &k0, const Consts *__
 
 // host-side flags that describe what the device buffers hold
 struct HostState {
-  bool cells_valid, tiles_valid, records_valid, marks_valid;
+  bool cells_valid, tiles_valid, records_valid, marks_valid, olds_stale;
   int list_groups;
 };
 
@@ -154,6 +160,10 @@ struct tsph_ctx {
   int n_wall = 0;            // wall particles (walls are never created or removed)
   bool records_valid = false;  // mixed-path FP32 records match the FP64 state
   bool marks_valid = false;    // removal marks match the current positions
+  // r_o, v_o, rho_o were not moved by the last cell build (the build that
+  // follows a symplectic corrector: the next predictor rewrites them); until
+  // rebuilt they are alt's values through perm (ensure_olds)
+  bool olds_stale = false;
   bool has_ale = false;        // ALE volume variables allocated
   uint8_t *ale_flag = nullptr; // per particle: its own ALE gamma sum is non-zero
   hipStream_t stream = nullptr;
@@ -163,6 +173,14 @@ struct tsph_ctx {
   unsigned long long *counters = nullptr;
   int *cell_count = nullptr;           // ncells+1 (last = removed bucket)
   int *cell_start = nullptr;           // ncells+2
+  // Every build writes its cell offsets and wall count into the buffers the
+  // previous build did not (cell_start_next, the other wall-count slot) and
+  // then swaps, so that the wall-density pass that reads the previous cell
+  // list can run while the next build sorts (record_step).
+  int *cell_start_next = nullptr;      // ncells+2
+  int nwall_idx = 3;                   // S_NWALL or S_NWALL_B: the current wall count
+  hipStream_t side = nullptr;          // second stream of a step graph's parallel branch
+  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
   int *cellof = nullptr;               // cell of each slot (sorted order)
   int *hrank = nullptr;                // rank inside the cell from the hash pass
   int *hcell = nullptr;                // cell from the hash pass (pre-sort order)
@@ -248,8 +266,17 @@ struct TileGrid {
 // decide: RemoveParticlesOutOfDomain's decision is taken by the build, in
 // place of a launch_remove immediately before (the removal states must be
 // current: a corrector precedes)
+// before_reorder: an event the reorder waits for (work on another stream
+// that reads the previous cell list and the particle state the reorder rewrites)
+// skip_olds: r_o, v_o, rho_o are not moved (olds_stale afterwards), for a build
+// after which they are dead until the next predictor rewrites them
 int launch_cell_list(tsph_ctx *c, bool eos = false, bool predict = false, double dt = 0.,
-                     unsigned ad_skip = 0, bool decide = false);
+                     unsigned ad_skip = 0, bool decide = false, hipEvent_t before_reorder = nullptr,
+                     bool skip_olds = false);
+// r_o, v_o, rho_o of the current generation, gathered from the previous one
+// through the last build's permutation if that build left them behind; every
+// launch that reads or moves them calls this first
+int ensure_olds(tsph_ctx *c);
 // types whose a, drho a pair pass of `variant` writes (its active particles)
 inline unsigned forces_active_types(int variant) {
   return variant == TSPH_FORCES_DBC ? (1u << TSPH_FLUID) | (1u << TSPH_WALL) : 1u << TSPH_FLUID;
@@ -264,7 +291,9 @@ TileGrid tile_grid(tsph_ctx *c, bool fp32, size_t lds_wave);
 // workgroups of the pooled walk that launch_forces would take now (0: static),
 // for the cell-list build to cut the tile list for
 int pooled_groups(tsph_ctx *c);
-int launch_forces(tsph_ctx *c, int variant);
+// corr_dt > 0 (mixed path): the pass also applies the symplectic corrector
+// with that time step (batched contexts: each case's own) to every particle
+int launch_forces(tsph_ctx *c, int variant, double corr_dt = 0.);
 int launch_mixed_records(tsph_ctx *c);
 // tsph_stages.hip
 int launch_eos(tsph_ctx *c);
@@ -272,8 +301,8 @@ int launch_predictor(tsph_ctx *c, double dt);
 int launch_corrector(tsph_ctx *c, double dt);
 // count_density: also take the out-of-range density count of an EOS pass
 // right after (in place of that pass, where nothing reads its pressures)
-int launch_mdbc(tsph_ctx *c, bool count_density = false);
-int launch_bt_density(tsph_ctx *c, bool count_density = false);
+int launch_mdbc(tsph_ctx *c, bool count_density = false, hipStream_t s = nullptr);
+int launch_bt_density(tsph_ctx *c, bool count_density = false, hipStream_t s = nullptr);
 int launch_remove(tsph_ctx *c);
 int run_water_elevation(tsph_ctx *c, double x, double y0, int inverse, double *out);
 int launch_interp_grid(tsph_ctx *c, double x0, double y0, double dh, int ncx, int ncy, double *out);

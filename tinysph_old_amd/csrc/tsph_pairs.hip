@@ -550,7 +550,13 @@ __global__ __launch_bounds__(kWave * kPoolWaves) void k_tile64(Soa d, const int 
 // --------------------------------------------------------------------------
 // mixed tile kernel
 // --------------------------------------------------------------------------
-template <int VAR, bool DYN>
+// CORR: the symplectic corrector (src/SPH_integration.h:287-309) is applied to
+// every particle of the tile as soon as its forces are known (inactive ones
+// with the a, drho they hold).  Other lanes never read a particle's FP64 state
+// in this kernel (neighbours come from the FP32 records), so the update may
+// land while the pass runs; it replaces the corrector pass that follows the
+// first pair pass of a symplectic step.
+template <int VAR, bool DYN, bool CORR>
 __global__ __launch_bounds__(kWave * kPoolWaves) void k_tile32(Soa d, Mixed mx, const int *__restrict__ cell_start,
                                                   const int *__restrict__ cellof, Consts k0,
                                                   const Consts *__restrict__ cases,
@@ -558,7 +564,8 @@ __global__ __launch_bounds__(kWave * kPoolWaves) void k_tile32(Soa d, Mixed mx, 
                                                   int rows, int *__restrict__ scal,
                                                   const int *__restrict__ tile_list, int cap,
                                                   const int *__restrict__ bound, int lds_wave,
-                                                  unsigned long long *__restrict__ counters) {
+                                                  unsigned long long *__restrict__ counters,
+                                                  double dt0, uint8_t *__restrict__ state) {
 #pragma clang fp contract(fast)
   extern __shared__ __align__(16) unsigned char smem[];
   unsigned char *wl = wave_lds(smem, lds_wave);
@@ -607,12 +614,15 @@ __global__ __launch_bounds__(kWave * kPoolWaves) void k_tile32(Soa d, Mixed mx, 
     t.base[q] += q * kGroup;
   }
   t.lq[3] += 3 * kGroup;
+  const double dtc = cases ? k.dt : dt0;  // the corrector's time step (CORR)
   if (t.lq[3] > cap) {
     const int next = tq_take<DYN>(tq, ticket);
     for (int i = t.own0 + tile_lane(); i < t.own1; i += kWave) {
-      if (!is_active<VAR>(d.type[i])) continue;
-      ++nact;
-      particle32_global<VAR>(d, mx, cs, cellof, k, cc, i, npair, ncand);
+      if (is_active<VAR>(d.type[i])) {
+        ++nact;
+        particle32_global<VAR>(d, mx, cs, cellof, k, cc, i, npair, ncand);
+      }
+      if (CORR) correct_particle(d, k, dtc, i, d.a[i], d.drho[i], state);
     }
     it = next;
     continue;
@@ -757,15 +767,33 @@ __global__ __launch_bounds__(kWave * kPoolWaves) void k_tile32(Soa d, Mixed mx, 
     // candidates: the stencil minus the particle itself (DBC: wall-wall pairs
     // are not candidates either, counted as the reference skips them)
     if (VAR != TSPH_FORCES_DBC) ncand += tot > 0 ? tot - 1 : 0;
-    // results are stored one chunk later (or after the next tile's staging)
-    if (pend_i >= 0) {
-      d.a[pend_i] = pend_a;
-      d.drho[pend_i] = pend_dr;
-    }
-    pend_i = -1;
-    if (act) {
-      finish32<VAR>(k, s, at, pend_a, pend_dr);
-      pend_i = i;
+    if (CORR) {
+      // forces, then the corrector, at once (the corrector's loads make the
+      // deferred store of the plain pass pointless)
+      if (i < t.own1) {
+        double2 a;
+        double dr;
+        if (act) {
+          finish32<VAR>(k, s, at, a, dr);
+          d.a[i] = a;
+          d.drho[i] = dr;
+        } else {
+          a = d.a[i];
+          dr = d.drho[i];
+        }
+        correct_particle(d, k, dtc, i, a, dr, state);
+      }
+    } else {
+      // results are stored one chunk later (or after the next tile's staging)
+      if (pend_i >= 0) {
+        d.a[pend_i] = pend_a;
+        d.drho[pend_i] = pend_dr;
+      }
+      pend_i = -1;
+      if (act) {
+        finish32<VAR>(k, s, at, pend_a, pend_dr);
+        pend_i = i;
+      }
     }
   }
   it = next;
@@ -868,12 +896,18 @@ namespace {
 // one pair pass: the kernel for the variant, the precision and the tile walk
 // (pooled or static)
 template <int VAR, bool DYN>
-void pair_launch(tsph_ctx *c, const TileGrid &tg, int rows, int cap) {
+void pair_launch(tsph_ctx *c, const TileGrid &tg, int rows, int cap, double corr_dt) {
   int *sc = c->scal;
-  if (c->prm.precision == TSPH_PREC_MIXED)
-    hipLaunchKernelGGL((k_tile32<VAR, DYN>), tg.grid, tg.block, tg.lds, c->stream, c->cur, c->mx,
-                       c->cell_start, c->cellof, c->k, c->cases, c->tile_base, c->ncases, rows, sc,
-                       c->tile_list, cap, c->tile_bound, tg.lds_wave, c->counters);
+  if (c->prm.precision == TSPH_PREC_MIXED && corr_dt > 0)
+    hipLaunchKernelGGL((k_tile32<VAR, DYN, true>), tg.grid, tg.block, tg.lds, c->stream, c->cur,
+                       c->mx, c->cell_start, c->cellof, c->k, c->cases, c->tile_base, c->ncases,
+                       rows, sc, c->tile_list, cap, c->tile_bound, tg.lds_wave, c->counters,
+                       corr_dt, c->seqstate);
+  else if (c->prm.precision == TSPH_PREC_MIXED)
+    hipLaunchKernelGGL((k_tile32<VAR, DYN, false>), tg.grid, tg.block, tg.lds, c->stream, c->cur,
+                       c->mx, c->cell_start, c->cellof, c->k, c->cases, c->tile_base, c->ncases,
+                       rows, sc, c->tile_list, cap, c->tile_bound, tg.lds_wave, c->counters, 0.,
+                       nullptr);
   else
     hipLaunchKernelGGL((k_tile64<VAR, DYN>), tg.grid, tg.block, tg.lds, c->stream, c->cur,
                        c->cell_start, c->cellof, c->k, c->cases, c->tile_base, c->ncases, rows, sc,
@@ -881,14 +915,14 @@ void pair_launch(tsph_ctx *c, const TileGrid &tg, int rows, int cap) {
 }
 
 template <int VAR>
-void pair_launch(tsph_ctx *c, const TileGrid &tg, int rows, int cap) {
-  if (tg.pooled) pair_launch<VAR, true>(c, tg, rows, cap);
-  else pair_launch<VAR, false>(c, tg, rows, cap);
+void pair_launch(tsph_ctx *c, const TileGrid &tg, int rows, int cap, double corr_dt) {
+  if (tg.pooled) pair_launch<VAR, true>(c, tg, rows, cap, corr_dt);
+  else pair_launch<VAR, false>(c, tg, rows, cap, corr_dt);
 }
 
 }  // namespace
 
-int launch_forces(tsph_ctx *c, int variant) {
+int launch_forces(tsph_ctx *c, int variant, double corr_dt) {
   const int rows = c->tile_rows;
   const int cap = 3 * (rows + 2) * kCellCap;
   const bool mixed = c->prm.precision == TSPH_PREC_MIXED;
@@ -898,13 +932,18 @@ int launch_forces(tsph_ctx *c, int variant) {
   }
   const TileGrid tg = tile_grid(c, mixed, forces_lds_wave(c, variant));
   if (int e = ensure_tile_list(c, tg.pooled ? tg.groups : 0)) return e;
-  return launch_timed(c, [&] {
+  const int e = launch_timed(c, [&] {
     switch (variant) {
-      case TSPH_FORCES_WCSPH: pair_launch<TSPH_FORCES_WCSPH>(c, tg, rows, cap); break;
-      case TSPH_FORCES_DBC: pair_launch<TSPH_FORCES_DBC>(c, tg, rows, cap); break;
-      default: pair_launch<TSPH_FORCES_BT>(c, tg, rows, cap);
+      case TSPH_FORCES_WCSPH: pair_launch<TSPH_FORCES_WCSPH>(c, tg, rows, cap, corr_dt); break;
+      case TSPH_FORCES_DBC: pair_launch<TSPH_FORCES_DBC>(c, tg, rows, cap, corr_dt); break;
+      default: pair_launch<TSPH_FORCES_BT>(c, tg, rows, cap, corr_dt);
     }
   });
+  if (!e && mixed && corr_dt > 0) {  // as launch_corrector leaves them
+    c->records_valid = false;
+    c->marks_valid = true;
+  }
+  return e;
 }
 
 }  // namespace tsph

@@ -98,6 +98,56 @@ __device__ __forceinline__ bool removal_decided(const uint8_t *__restrict__ stat
   return (run & 1) == 0;
 }
 
+// x + inc with Kahan's compensation (Kahan 1965): c holds the low-order part
+// lost by the previous sums, is subtracted from this increment, and receives
+// what this sum loses.  Exact as written only without contraction into FMAs
+// (the library is built with -ffp-contract=off).
+__device__ __forceinline__ double kahan_add(double x, double inc, double &c) {
+  const double y = inc - c;
+  const double t = x + y;
+  c = (t - x) - y;
+  return t;
+}
+
+// removal state of a particle at reference position seq: 0 absent, 1 kept,
+// 2 candidate (fluid outside the domain by more than eps, src/SPH_btdebug.h:17-22)
+__device__ __forceinline__ uint8_t removal_state(const Consts &k, int type, double2 r) {
+  if (type == kRemoved) return 0;
+  const bool out = type == TSPH_FLUID && ((r.y < (k.y0 - kEps)) || (r.y > k.ym + kEps) ||
+                                          (r.x < k.x0 - kEps) || (r.x > k.xm - kEps));
+  return out ? 2 : 1;
+}
+
+// Integrate_SymplecticCorrector for particle i, src/SPH_integration.h:287-309
+// (a, drho: the particle's forces), and its removal state for the
+// RemoveParticlesOutOfDomain that follows the corrector in every driver
+__device__ __forceinline__ void correct_particle(const Soa &d, const Consts &k, double dt, int i,
+                                                 double2 a, double drho, uint8_t *__restrict__ state) {
+  const double e = (-drho / d.rho[i]) * dt;
+  const double2 vo = d.v_o[i], ro = d.r_o[i];
+  const double2 v = make_double2(vo.x + a.x * dt, vo.y + a.y * dt);
+  d.v[i] = v;
+  double2 r;
+  if (!d.rc) {
+    d.rho[i] = d.rho_o[i] * ((2. - e) / (2. + e));
+    r = make_double2(ro.x + (v.x + vo.x) * dt * 0.5, ro.y + (v.y + vo.y) * dt * 0.5);
+  } else {
+    // Kahan-compensated form of the same updates: each is written as old value
+    // + increment (rho_o (2-e)/(2+e) = rho_o + rho_o (-2e/(2+e))), and the part
+    // of the increment that the rounded sum loses is carried to the next step
+    double2 c = d.rc[i];
+    double rc = d.rhoc[i];
+    const double rho_o = d.rho_o[i];
+    d.rho[i] = kahan_add(rho_o, rho_o * ((-2. * e) / (2. + e)), rc);
+    r.x = kahan_add(ro.x, (v.x + vo.x) * dt * 0.5, c.x);
+    r.y = kahan_add(ro.y, (v.y + vo.y) * dt * 0.5, c.y);
+    d.rc[i] = c;
+    d.rhoc[i] = rc;
+  }
+  d.r[i] = r;
+  state[d.seq[i]] = removal_state(k, d.type[i], r);
+}
+
 // Integrate_SymplecticPredictor's half step, src/SPH_integration.h:266-287
 // (the expressions in the reference's order; no contraction)
 __device__ __forceinline__ double2 predicted_r(double2 r, double2 v, double dt) {

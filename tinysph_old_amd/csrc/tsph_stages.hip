@@ -65,15 +65,6 @@ __global__ void k_predictor(Soa d, const int *__restrict__ scal, double dt0,
   d.v[i] = v;
 }
 
-// removal state of a particle at reference position seq: 0 absent, 1 kept,
-// 2 candidate (fluid outside the domain by more than eps, src/SPH_btdebug.h:17-22)
-__device__ __forceinline__ uint8_t removal_state(const Consts &k, int type, double2 r) {
-  if (type == kRemoved) return 0;
-  const bool out = type == TSPH_FLUID && ((r.y < (k.y0 - kEps)) || (r.y > k.ym + kEps) ||
-                                          (r.x < k.x0 - kEps) || (r.x > k.xm - kEps));
-  return out ? 2 : 1;
-}
-
 // corrector; also records the removal state for the RemoveParticlesOutOfDomain
 // that follows it in every driver
 __global__ void k_corrector(Soa d, const int *__restrict__ scal, Consts k0,
@@ -83,15 +74,7 @@ __global__ void k_corrector(Soa d, const int *__restrict__ scal, Consts k0,
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const Consts k = case_consts(k0, cases, d.cid, i);
-  const double dt = cases ? k.dt : dt0;
-  const double e = (-d.drho[i] / d.rho[i]) * dt;
-  d.rho[i] = d.rho_o[i] * ((2. - e) / (2. + e));
-  const double2 vo = d.v_o[i], a = d.a[i], ro = d.r_o[i];
-  const double2 v = make_double2(vo.x + a.x * dt, vo.y + a.y * dt);
-  d.v[i] = v;
-  const double2 r = make_double2(ro.x + (v.x + vo.x) * dt * 0.5, ro.y + (v.y + vo.y) * dt * 0.5);
-  d.r[i] = r;
-  state[d.seq[i]] = removal_state(k, d.type[i], r);
+  correct_particle(d, k, cases ? k.dt : dt0, i, d.a[i], d.drho[i], state);
 }
 
 // cell of a sampling point, computed like Particle_To_Cell
@@ -243,12 +226,14 @@ __device__ __forceinline__ void count_bad_density(const Soa &d, const int *__res
   if (bad && (threadIdx.x & 63) == 0) atomicAdd(&counters[C_BADRHO], (unsigned long long)bad);
 }
 
-__global__ __launch_bounds__(64) void k_mdbc(Soa d, const int *__restrict__ cell_start, const int *__restrict__ scal,
+// nwall: the wall count of the cell list in use (one of the two alternating slots)
+__global__ __launch_bounds__(64) void k_mdbc(Soa d, const int *__restrict__ cell_start, const int *__restrict__ nwall,
                        const int *__restrict__ wall_list, Consts k0,
-                       const Consts *__restrict__ cases, unsigned long long *__restrict__ counters) {
+                       const Consts *__restrict__ cases, unsigned long long *__restrict__ counters,
+                       const int *__restrict__ scal) {
   __shared__ __align__(16) double s_terms[64 * 12];
   const int w = blockIdx.x * kWallsPerWave + (int)(threadIdx.x / kWG);
-  const bool live = w < scal[S_NWALL];
+  const bool live = w < *nwall;
   const int i = live ? wall_list[w] : 0;
   const Consts k = live ? case_consts(k0, cases, d.cid, i) : k0;
   const double2 gn = live ? d.gnr[i] : make_double2(0., 0.);
@@ -308,12 +293,13 @@ __global__ __launch_bounds__(64) void k_mdbc(Soa d, const int *__restrict__ cell
   if (counters) count_bad_density(d, scal, wall_bad, counters);
 }
 
-__global__ __launch_bounds__(64) void k_bt_density(Soa d, const int *__restrict__ cell_start, const int *__restrict__ scal,
+__global__ __launch_bounds__(64) void k_bt_density(Soa d, const int *__restrict__ cell_start, const int *__restrict__ nwall,
                              const int *__restrict__ wall_list, Consts k0,
-                             const Consts *__restrict__ cases, unsigned long long *__restrict__ counters) {
+                             const Consts *__restrict__ cases, unsigned long long *__restrict__ counters,
+                             const int *__restrict__ scal) {
   __shared__ __align__(16) double s_terms[64 * 2];
   const int w = blockIdx.x * kWallsPerWave + (int)(threadIdx.x / kWG);
-  const bool live = w < scal[S_NWALL];
+  const bool live = w < *nwall;
   const int i = live ? wall_list[w] : 0;
   const Consts k = live ? case_consts(k0, cases, d.cid, i) : k0;
   const double m = k.m;
@@ -499,6 +485,7 @@ int launch_eos(tsph_ctx *c) {
 }
 
 int launch_predictor(tsph_ctx *c, double dt) {
+  c->olds_stale = false;  // it rewrites r_o, v_o, rho_o of every particle
   hipLaunchKernelGGL(k_predictor, dim3(grid_for(c->n_ub, 256)), dim3(256), 0, c->stream, c->cur,
                      c->scal, dt, c->cases);
   TSPH_HIP(hipGetLastError());
@@ -508,6 +495,7 @@ int launch_predictor(tsph_ctx *c, double dt) {
 }
 
 int launch_corrector(tsph_ctx *c, double dt) {
+  if (int e = ensure_olds(c)) return e;
   hipLaunchKernelGGL(k_corrector, dim3(grid_for(c->n_ub, 256)), dim3(256), 0, c->stream, c->cur,
                      c->scal, c->k, c->cases, dt, c->seqstate);
   TSPH_HIP(hipGetLastError());
@@ -516,19 +504,19 @@ int launch_corrector(tsph_ctx *c, double dt) {
   return 0;
 }
 
-int launch_mdbc(tsph_ctx *c, bool count_density) {
-  hipLaunchKernelGGL(k_mdbc, dim3(grid_for(c->n_wall, kWallsPerWave)), dim3(64), 0, c->stream, c->cur,
-                     c->cell_start, c->scal, c->wall_list, c->k, c->cases,
-                     count_density ? c->counters : nullptr);
+int launch_mdbc(tsph_ctx *c, bool count_density, hipStream_t s) {
+  hipLaunchKernelGGL(k_mdbc, dim3(grid_for(c->n_wall, kWallsPerWave)), dim3(64), 0,
+                     s ? s : c->stream, c->cur, c->cell_start, c->scal + c->nwall_idx,
+                     c->wall_list, c->k, c->cases, count_density ? c->counters : nullptr, c->scal);
   TSPH_HIP(hipGetLastError());
   c->records_valid = false;
   return 0;
 }
 
-int launch_bt_density(tsph_ctx *c, bool count_density) {
-  hipLaunchKernelGGL(k_bt_density, dim3(grid_for(c->n_wall, kWallsPerWave)), dim3(64), 0, c->stream, c->cur,
-                     c->cell_start, c->scal, c->wall_list, c->k, c->cases,
-                     count_density ? c->counters : nullptr);
+int launch_bt_density(tsph_ctx *c, bool count_density, hipStream_t s) {
+  hipLaunchKernelGGL(k_bt_density, dim3(grid_for(c->n_wall, kWallsPerWave)), dim3(64), 0,
+                     s ? s : c->stream, c->cur, c->cell_start, c->scal + c->nwall_idx,
+                     c->wall_list, c->k, c->cases, count_density ? c->counters : nullptr, c->scal);
   TSPH_HIP(hipGetLastError());
   c->records_valid = false;
   return 0;

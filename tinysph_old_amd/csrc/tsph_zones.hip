@@ -122,6 +122,10 @@ __global__ __launch_bounds__(1024) void k_bz_inlet_append(Soa d, int *__restrict
     d.v_o[ns] = vin;
     d.rho_o[ns] = 1000.;
     if (d.gnr) d.gnr[ns] = make_double2(0., 0.);
+    if (d.rc) {
+      d.rc[ns] = make_double2(0., 0.);
+      d.rhoc[ns] = 0.;
+    }
     if (d.n) d.n[ns] = make_double2(0., 0.);
     if (d.omega) {
       const double w = dp * dp;
@@ -202,6 +206,8 @@ __global__ void k_bz_outlet_apply(Soa d, const int *__restrict__ scal, Zone z,
 }  // namespace
 
 int launch_bz_inlet(tsph_ctx *c, int zone, double dt) {
+  // appended particles take slots past the last build's permutation
+  if (int e = ensure_olds(c)) return e;
   const Zone &z = c->zones[zone];
   // appended particles extend the slot range: launches cover the capacity
   c->n_ub = c->capacity;

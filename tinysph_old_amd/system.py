@@ -331,6 +331,10 @@ class ParticleSystem:
     def bz_integrate_outlet(self, zone: int, dt: float):
         L.check(self.lib.tsph_bz_integrate_outlet(self.ctx, zone, dt), "BZ_Integrate_outlet")
 
+    def set_compensated(self, on: bool = True):
+        """Kahan-compensated position and density updates (tsph_set_compensated)."""
+        L.check(self.lib.tsph_set_compensated(self.ctx, int(on)), "tsph_set_compensated")
+
     def renumber(self):
         """Relabel the live particles' reference positions 0..np-1 (tsph_renumber)."""
         L.check(self.lib.tsph_renumber(self.ctx), "tsph_renumber")
