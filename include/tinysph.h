@@ -335,6 +335,19 @@ int tsph_io_wait(tsph_ctx *ctx);
  * (i, j) at j + i*ncy.  With rho, p and v all NULL only the sizes are returned. */
 int tsph_interpolate_grid(tsph_ctx *ctx, double x0, double y0, double xm, double ym, int *ncx,
                           int *ncy, double *rho, double *p, double *v);
+/* tsph_interpolate_grid without device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * grid values are staged in one device array per GPU, allocated when the
+ * library is loaded and held until the process ends, instead of a buffer
+ * allocated per call; the caller passes the host outputs as for
+ * tsph_interpolate_grid and gets the same values.  At most
+ * TSPH_GRID_SCRATCH_POINTS points (ncx*ncy) per call (a larger grid returns
+ * TSPH_ECAPACITY, ncx and ncy untouched).  Calls that overlap on one GPU (from
+ * several threads or contexts) race on that array. */
+#define TSPH_GRID_SCRATCH_POINTS (1 << 19)
+int tsph_interpolate_grid_noalloc(tsph_ctx *ctx, double x0, double y0, double xm, double ym,
+                                  int *ncx, int *ncy, double *rho, double *p, double *v);
 /* GenerateInterpol's file: write_INTERPOLATED_mesh_to_ASCII_VTK,
  * src/SPH_grid_to_vtk.h:139-243 */
 int tsph_write_interpolated_vtk(tsph_ctx *ctx, double x0, double y0, double xm, double ym,

@@ -21,6 +21,7 @@ FORCES_WCSPH, FORCES_DBC, FORCES_BT = 0, 1, 2
 WALL_DBC, WALL_MDBC, WALL_BT = 0, 1, 2
 CASE_STILL_WATER_DBC, CASE_DAM_BREAK_MDBC, CASE_DAM_BREAK_BT, CASE_ALE_DAM_BREAK = 1, 2, 3, 4
 CASE_INFLOW_SLAB, CASE_INFLOW_CHANNEL, CASE_DAM_BREAK_MDBC_REF = 5, 6, 7
+GRID_SCRATCH_POINTS = 1 << 19  # TSPH_GRID_SCRATCH_POINTS
 INLET1, OUTLET1 = 7, 12
 SCHEME_SYMPLECTIC, SCHEME_RALE = 0, 1
 
@@ -146,6 +147,9 @@ EXPORTS = {
     "tsph_io_wait": (C.c_int, [C.c_void_p]),
     "tsph_interpolate_grid": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_double,
                                         C.POINTER(C.c_int), C.POINTER(C.c_int), _dp, _dp, _dp]),
+    "tsph_interpolate_grid_noalloc": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_double,
+                                                C.c_double, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                                _dp, _dp, _dp]),
     "tsph_write_interpolated_vtk": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_double,
                                               C.c_double, C.c_char_p]),
     "tsph_format_vtk": (C.c_int, [C.POINTER(Particles), C.c_char_p]),

@@ -305,7 +305,12 @@ int launch_mdbc(tsph_ctx *c, bool count_density = false, hipStream_t s = nullptr
 int launch_bt_density(tsph_ctx *c, bool count_density = false, hipStream_t s = nullptr);
 int launch_remove(tsph_ctx *c);
 int run_water_elevation(tsph_ctx *c, double x, double y0, int inverse, double *out);
-int launch_interp_grid(tsph_ctx *c, double x0, double y0, double dh, int ncx, int ncy, double *out);
+// soa: out holds rho[n], p[n], then v[2n] (n = ncx ncy); else 4 values per point
+int launch_interp_grid(tsph_ctx *c, double x0, double y0, double dh, int ncx, int ncy, double *out,
+                       bool soa = false);
+// device address of the allocation-free interpolation grid's scratch on the
+// current device (looked up once per device)
+int grid_scratch_ptr(double **out);
 // tsph_ale.hip
 int launch_ale_init(tsph_ctx *c);
 int launch_rale_forces(tsph_ctx *c);

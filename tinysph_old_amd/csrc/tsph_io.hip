@@ -11,11 +11,48 @@
 // writers, so the files are byte-identical for the same values.
 #include <algorithm>
 #include <fstream>
+#include <mutex>
 #include <numeric>
 
 #include "tsph_internal.h"
 
 namespace tsph {
+
+// Staging of tsph_interpolate_grid_noalloc: the grid values of one call, laid
+// out as rho[n], p[n], v[2n] (n = ncx ncy <= TSPH_GRID_SCRATCH_POINTS),
+// 16 MiB.  Each GPU holds its own copy at its own address until the process
+// ends; every call on a GPU shares that GPU's copy.  In host code `scratch` is
+// only a handle: the device address comes from hipGetSymbolAddress, once per
+// device (grid_scratch_ptr), and only that address is passed to kernels and
+// copies.
+static __device__ __align__(256) double scratch[4 * (size_t)TSPH_GRID_SCRATCH_POINTS];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch[kMaxDevices] = {};
+}  // namespace
+
+int grid_scratch_ptr(double **out) {
+  int dev = 0;
+  TSPH_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) {
+    set_error("device index beyond the scratch table");
+    return TSPH_EHIP;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      set_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+      return TSPH_EHIP;
+    }
+    g_scratch[dev] = static_cast<double *>(p);
+  }
+  *out = g_scratch[dev];
+  return 0;
+}
 
 namespace {
 
@@ -333,6 +370,38 @@ int tsph_interpolate_grid(tsph_ctx *c, double x0, double y0, double xm, double y
     if (p) p[t] = out[4 * t + 1];
     if (v) { v[2 * t] = out[4 * t + 2]; v[2 * t + 1] = out[4 * t + 3]; }
   }
+  return 0;
+}
+
+int tsph_interpolate_grid_noalloc(tsph_ctx *c, double x0, double y0, double xm, double ym, int *ncx,
+                                  int *ncy, double *rho, double *p, double *v) {
+  if (!c || !ncx || !ncy) return io_fail(TSPH_EINVAL, "null argument");
+  if (c->cases) return io_fail(TSPH_ESTATE, "not available on a batched context");
+  // the grid of tsph_interpolate_grid (src/SPH_dataInterpolation.h:31-36)
+  const double h = c->k.h, sf = 4;
+  const double dh = h / sf;
+  const int nx = (int)((xm - x0) / h) * sf;
+  const int ny = (int)((ym - y0) / h) * sf;
+  if (nx < 0 || ny < 0) return io_fail(TSPH_EINVAL, "interpolation grid too large");
+  if ((long long)nx * ny > TSPH_GRID_SCRATCH_POINTS)
+    return io_fail(TSPH_ECAPACITY, "interpolation grid of " + std::to_string((long long)nx * ny) +
+                                       " points exceeds TSPH_GRID_SCRATCH_POINTS (" +
+                                       std::to_string(TSPH_GRID_SCRATCH_POINTS) + ")");
+  *ncx = nx;
+  *ncy = ny;
+  if (!rho && !p && !v) return 0;
+  if (!c->cells_valid) return io_fail(TSPH_ESTATE, "no cell list: call tsph_particle_to_cell first");
+  const size_t cnt = (size_t)nx * ny;
+  if (cnt == 0) return 0;
+  double *dout = nullptr;
+  if (int e = grid_scratch_ptr(&dout)) return e;
+  if (int e = launch_interp_grid(c, x0, y0, dh, nx, ny, dout, true)) return e;
+  const size_t b = sizeof(double) * cnt;
+  if ((rho && hipMemcpyAsync(rho, dout, b, hipMemcpyDeviceToHost, c->stream) != hipSuccess) ||
+      (p && hipMemcpyAsync(p, dout + cnt, b, hipMemcpyDeviceToHost, c->stream) != hipSuccess) ||
+      (v && hipMemcpyAsync(v, dout + 2 * cnt, 2 * b, hipMemcpyDeviceToHost, c->stream) != hipSuccess) ||
+      hipStreamSynchronize(c->stream) != hipSuccess)
+    return io_fail(TSPH_EHIP, "tsph_interpolate_grid_noalloc: copy failed");
   return 0;
 }
 

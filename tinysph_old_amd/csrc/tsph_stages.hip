@@ -460,16 +460,25 @@ __global__ void k_interp_point(Soa d, const int *__restrict__ cell_start, Consts
 // (x0 + gx dh, y0 + gy dh), stored at POS(gx, gy) = gy + gx ncy
 // (src/SPH_defs.h:36), one lane per point
 __global__ void k_interp_grid(Soa d, const int *__restrict__ cell_start, Consts k, double x0,
-                              double y0, double dh, int ncx, int ncy, double *__restrict__ out) {
+                              double y0, double dh, int ncx, int ncy, double *__restrict__ out,
+                              bool soa) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= ncx * ncy) return;
+  const int n = ncx * ncy;
+  if (t >= n) return;
   const int gx = t / ncy, gy = t - gx * ncy;
   double v[4];
   interpolate_at(d, cell_start, k, x0 + gx * dh, y0 + gy * dh, v);
-  out[4 * t] = v[0];
-  out[4 * t + 1] = v[1];
-  out[4 * t + 2] = v[2];
-  out[4 * t + 3] = v[3];
+  if (soa) {  // rho[n], p[n], v[2n]
+    out[t] = v[0];
+    out[n + t] = v[1];
+    out[2 * n + 2 * t] = v[2];
+    out[2 * n + 2 * t + 1] = v[3];
+  } else {
+    out[4 * t] = v[0];
+    out[4 * t + 1] = v[1];
+    out[4 * t + 2] = v[2];
+    out[4 * t + 3] = v[3];
+  }
 }
 
 }  // namespace
@@ -545,9 +554,10 @@ int run_water_elevation(tsph_ctx *c, double x, double y0, int inverse, double *o
 }
 
 // out: 4 doubles per grid point, device memory
-int launch_interp_grid(tsph_ctx *c, double x0, double y0, double dh, int ncx, int ncy, double *out) {
+int launch_interp_grid(tsph_ctx *c, double x0, double y0, double dh, int ncx, int ncy, double *out,
+                       bool soa) {
   hipLaunchKernelGGL(k_interp_grid, dim3(grid_for(ncx * ncy, 64)), dim3(64), 0, c->stream, c->cur,
-                     c->cell_start, c->k, x0, y0, dh, ncx, ncy, out);
+                     c->cell_start, c->k, x0, y0, dh, ncx, ncy, out, soa);
   TSPH_HIP(hipGetLastError());
   return 0;
 }
