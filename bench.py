@@ -241,10 +241,10 @@ def leg_ensemble(steps):
     for name, prec in (("mixed", T.PREC_MIXED), ("fp64", T.PREC_FP64)):
         with T.batch_for_cases(cases, prec) as bs:
             bs.particle_to_cell()
-            rows, waves, bal, _ = T.autotune(bs, T.FORCES_WCSPH)
+            rows, waves, bal, _, ppl = T.autotune(bs, T.FORCES_WCSPH, True)
             t, cnt = timed(bs, lambda k: bs.step(T.WALL_MDBC, k), steps, 3)
             out[name] = leg_line(t, cnt, steps, n, tile_rows=rows, waves_per_cu=waves,
-                                 balanced=bal)
+                                 balanced=bal, particles_per_lane=round(ppl, 3))
     out["mixed_over_fp64"] = (out["mixed"]["pair_interactions_per_s"] /
                               out["fp64"]["pair_interactions_per_s"])
     return out

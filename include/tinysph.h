@@ -409,13 +409,16 @@ int tsph_batch_step(tsph_ctx *ctx, int wall, int nsteps);
 /* ---- autotuner ----
  * Times the pair pass `variant` on the current state for each tile walk
  * (pooled or static, see tsph_set_balanced), tile depth (cells per tile: 5,
- * 6, 7, 8, 10, 12) and persistent-wave count (waves per CU: 4, 8, 12, 16),
- * three passes each, and keeps the fastest; each wavefront owns one tile and
- * each lane one particle of it by design.  Needs a cell list; leaves a, drho
- * as one pair pass with the chosen setting leaves them.  Outputs may be
- * NULL.  No counterpart in the reference. */
+ * 6, 7, 8, 10, 12, 14, 16, 20, 24) and persistent-wave count (waves per CU:
+ * 4, 8, 12, 16), three passes each, and keeps the fastest.  The 64 lanes of a
+ * wavefront take the particles of its tile in turn, so the depth sets the
+ * particles per lane (thread): the depths above span about one to three at
+ * the configured resolutions, and *particles_per_lane returns the chosen
+ * tiling's mean (particles / (64 x non-empty tiles)).  Needs a cell list;
+ * leaves a, drho as one pair pass with the chosen setting leaves them.
+ * Outputs may be NULL.  No counterpart in the reference. */
 int tsph_autotune(tsph_ctx *ctx, int variant, int *tile_rows, int *waves_per_cu, int *balanced,
-                  double *best_ms);
+                  double *best_ms, double *particles_per_lane);
 /* Tile walk of the pair loops for the context's precision: 1 = pooled
  * (workgroups of up to 4 wavefronts, each with a chunk of the tile list of
  * about equal work whose tiles its wavefronts take from a counter in LDS; the

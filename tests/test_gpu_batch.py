@@ -167,9 +167,17 @@ def test_autotune_keeps_results(batch):
         with mk() as ps:
             ps.particle_to_cell()
             if tune:
-                rows, waves, bal, ms = T.autotune(ps, T.FORCES_WCSPH)
-                assert rows in (5, 6, 7, 8, 10, 12) and waves in (4, 8, 12, 16) and bal in (0, 1)
+                rows, waves, bal, ms, ppl = T.autotune(ps, T.FORCES_WCSPH, True)
+                assert rows in (5, 6, 7, 8, 10, 12, 14, 16, 20, 24)
+                assert waves in (4, 8, 12, 16) and bal in (0, 1)
                 assert ms > 0
+                # particles per lane of the chosen depth: a full cell holds
+                # about 8-12 particles here, so a tile of `rows` cells about
+                # 0.13-0.19 per lane and row (less where columns are partly
+                # filled)
+                print(f"tuned: {rows} rows, {waves} waves per CU, balanced {bal}, "
+                      f"{ppl:.2f} particles per lane", flush=True)
+                assert 0.02 * rows < ppl < 0.3 * rows
                 assert ps.tile_rows == rows
             else:
                 ps.compute_forces(T.FORCES_WCSPH)

@@ -1,0 +1,74 @@
+"""x^(1/7) of the FP64 pair loops' density-diffusion term (tsph_root7.h)
+against libm's pow(x, 1/7), which the reference calls
+(src/SPH_viscouse_forces.h:52).
+
+Built for the host with g++ from the same header the kernels include.  Near 1,
+where every configured case evaluates it (|x - 1| < 1e-3), the binomial series
+must give pow's double in all but a few in 10^4 draws and never be more than
+one ulp away; over the rest of the positive range (where the rounding of the
+exponent 1/7 itself moves pow's result by up to ~50 ulps, and the routine
+follows it) the Newton form stays within one ulp, and the special values
+follow pow.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "..", "tinysph_old_amd", "csrc")
+
+
+@pytest.fixture(scope="module")
+def lib(tmp_path_factory):
+    out = str(tmp_path_factory.mktemp("root7") / "libroot7.so")
+    subprocess.run(["g++", "-O2", "-ffp-contract=off", "-shared", "-fPIC", "-I", CSRC, "-o", out,
+                    os.path.join(HERE, "native", "root7_host.cpp")], check=True)
+    L = C.CDLL(out)
+    for fn in (L.root7_host, L.pow7_host):
+        fn.argtypes = [C.c_void_p, C.c_void_p, C.c_long]
+        fn.restype = None
+    return L
+
+
+def evaluate(lib, x):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    a, b = np.empty_like(x), np.empty_like(x)
+    lib.root7_host(x.ctypes.data, a.ctypes.data, x.size)
+    lib.pow7_host(x.ctypes.data, b.ctypes.data, x.size)
+    return a, b
+
+
+def ulps(a, b):
+    ia, ib = a.view(np.int64), b.view(np.int64)
+    return np.abs(ia - ib)
+
+
+@pytest.mark.parametrize("half_width,max_mismatch", [(6e-4, 1e-4), (1e-3, 2e-4), (0.0156, 2e-3)])
+def test_series_matches_pow_near_one(lib, half_width, max_mismatch):
+    rng = np.random.default_rng(7)
+    x = 1.0 + rng.uniform(-half_width, half_width, 1_000_000)
+    got, want = evaluate(lib, x)
+    d = ulps(got, want)
+    assert d.max() <= 1
+    assert (d != 0).mean() <= max_mismatch, (d != 0).mean()
+
+
+def test_newton_form_over_the_positive_range(lib):
+    rng = np.random.default_rng(8)
+    x = np.concatenate([10.0 ** rng.uniform(-300, 300, 200_000),
+                        rng.uniform(0.5, 2.0, 200_000),       # both sides of the series range
+                        np.ldexp(1.0, np.arange(-1000, 1000)),  # every power of two
+                        [5e-324, 2.2250738585072014e-308, 1.7976931348623157e308]])
+    got, want = evaluate(lib, x)
+    assert ulps(got, want).max() <= 1
+
+
+def test_special_values(lib):
+    x = np.array([0.0, -0.0, -1.0, -1e-300, np.inf, np.nan, 1.0])
+    got, want = evaluate(lib, x)
+    assert got[0] == 0.0 and got[1] == 0.0 and got[4] == np.inf and got[6] == 1.0
+    assert np.isnan(got[2]) and np.isnan(got[3]) and np.isnan(got[5])
+    assert np.array_equal(np.isnan(got), np.isnan(want))

@@ -141,7 +141,7 @@ EXPORTS = {
     "tsph_batch_download": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(Particles)]),
     "tsph_batch_step": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "tsph_autotune": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
-                                C.POINTER(C.c_int), _dp]),
+                                C.POINTER(C.c_int), _dp, _dp]),
     "tsph_set_balanced": (C.c_int, [C.c_void_p, C.c_int]),
     "tsph_write_vtk": (C.c_int, [C.c_void_p, C.c_char_p]),
     "tsph_io_wait": (C.c_int, [C.c_void_p]),

@@ -264,7 +264,7 @@ static int record_step(tsph_ctx *c, int wall, double dt, int inlet = -1, int out
   // decision (RemoveParticlesOutOfDomain) on the way.  The mixed pair pass
   // applies the corrector itself (it never reads the FP64 state it updates).
   if (c->prm.precision == TSPH_PREC_MIXED) {
-    if ((e = launch_forces(c, variant, dt))) return e;
+    if ((e = launch_forces(c, variant, true, dt))) return e;
   } else if ((e = launch_forces(c, variant)) || (e = launch_corrector(c, dt))) {
     return e;
   }
@@ -1720,13 +1720,15 @@ int tsph_set_balanced(tsph_ctx *c, int on) {
 }
 
 int tsph_autotune(tsph_ctx *c, int variant, int *tile_rows, int *waves_per_cu, int *balanced,
-                  double *best_ms) {
+                  double *best_ms, double *particles_per_lane) {
   if (!c) return fail(TSPH_EINVAL, "null argument");
   if (variant < TSPH_FORCES_WCSPH || variant > TSPH_FORCES_BT) return fail(TSPH_EINVAL, "unknown variant");
   if (!c->cells_valid) return fail(TSPH_ESTATE, "no cell list: call tsph_particle_to_cell first");
   if (variant == TSPH_FORCES_BT && !c->has_n) return fail(TSPH_ESTATE, "BT forces need wall normals (n)");
   if (c->time_pairs) return fail(TSPH_ESTATE, "pair timing is active");
-  static const int kRows[] = {5, 6, 7, 8, 10, 12};
+  // depths from about one particle per lane to about three (8-9 particles
+  // per non-empty cell at the configured resolutions)
+  static const int kRows[] = {5, 6, 7, 8, 10, 12, 14, 16, 20, 24};
   static const int kWaves[] = {4, 8, 12, 16};
   constexpr int kReps = 3;
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -1776,6 +1778,15 @@ int tsph_autotune(tsph_ctx *c, int variant, int *tile_rows, int *waves_per_cu, i
   if (waves_per_cu) *waves_per_cu = best_w;
   if (balanced) *balanced = best_b;
   if (best_ms) *best_ms = best / kReps;
+  if (particles_per_lane) {
+    // the chosen depth's particles per lane: every particle belongs to one
+    // tile of the list, whose 64 lanes take its particles in turn
+    int ntiles = 0, np = 0;
+    TSPH_HIP(hipMemcpyAsync(&ntiles, c->scal + S_NTILES, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    TSPH_HIP(hipMemcpyAsync(&np, c->scal + S_NP, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    TSPH_HIP(hipStreamSynchronize(c->stream));
+    *particles_per_lane = ntiles > 0 ? (double)np / (64.0 * ntiles) : 0.;
+  }
   return 0;
 }
 

@@ -291,9 +291,9 @@ TileGrid tile_grid(tsph_ctx *c, bool fp32, size_t lds_wave);
 // workgroups of the pooled walk that launch_forces would take now (0: static),
 // for the cell-list build to cut the tile list for
 int pooled_groups(tsph_ctx *c);
-// corr_dt > 0 (mixed path): the pass also applies the symplectic corrector
-// with that time step (batched contexts: each case's own) to every particle
-int launch_forces(tsph_ctx *c, int variant, double corr_dt = 0.);
+// corr (mixed path only): the pass also applies the symplectic corrector to
+// every particle, with time step corr_dt (batched contexts: each case's own)
+int launch_forces(tsph_ctx *c, int variant, bool corr = false, double corr_dt = 0.);
 int launch_mixed_records(tsph_ctx *c);
 // tsph_stages.hip
 int launch_eos(tsph_ctx *c);

@@ -896,9 +896,9 @@ namespace {
 // one pair pass: the kernel for the variant, the precision and the tile walk
 // (pooled or static)
 template <int VAR, bool DYN>
-void pair_launch(tsph_ctx *c, const TileGrid &tg, int rows, int cap, double corr_dt) {
+void pair_launch(tsph_ctx *c, const TileGrid &tg, int rows, int cap, bool corr, double corr_dt) {
   int *sc = c->scal;
-  if (c->prm.precision == TSPH_PREC_MIXED && corr_dt > 0)
+  if (c->prm.precision == TSPH_PREC_MIXED && corr)
     hipLaunchKernelGGL((k_tile32<VAR, DYN, true>), tg.grid, tg.block, tg.lds, c->stream, c->cur,
                        c->mx, c->cell_start, c->cellof, c->k, c->cases, c->tile_base, c->ncases,
                        rows, sc, c->tile_list, cap, c->tile_bound, tg.lds_wave, c->counters,
@@ -915,14 +915,14 @@ void pair_launch(tsph_ctx *c, const TileGrid &tg, int rows, int cap, double corr
 }
 
 template <int VAR>
-void pair_launch(tsph_ctx *c, const TileGrid &tg, int rows, int cap, double corr_dt) {
-  if (tg.pooled) pair_launch<VAR, true>(c, tg, rows, cap, corr_dt);
-  else pair_launch<VAR, false>(c, tg, rows, cap, corr_dt);
+void pair_launch(tsph_ctx *c, const TileGrid &tg, int rows, int cap, bool corr, double corr_dt) {
+  if (tg.pooled) pair_launch<VAR, true>(c, tg, rows, cap, corr, corr_dt);
+  else pair_launch<VAR, false>(c, tg, rows, cap, corr, corr_dt);
 }
 
 }  // namespace
 
-int launch_forces(tsph_ctx *c, int variant, double corr_dt) {
+int launch_forces(tsph_ctx *c, int variant, bool corr, double corr_dt) {
   const int rows = c->tile_rows;
   const int cap = 3 * (rows + 2) * kCellCap;
   const bool mixed = c->prm.precision == TSPH_PREC_MIXED;
@@ -934,12 +934,12 @@ int launch_forces(tsph_ctx *c, int variant, double corr_dt) {
   if (int e = ensure_tile_list(c, tg.pooled ? tg.groups : 0)) return e;
   const int e = launch_timed(c, [&] {
     switch (variant) {
-      case TSPH_FORCES_WCSPH: pair_launch<TSPH_FORCES_WCSPH>(c, tg, rows, cap, corr_dt); break;
-      case TSPH_FORCES_DBC: pair_launch<TSPH_FORCES_DBC>(c, tg, rows, cap, corr_dt); break;
-      default: pair_launch<TSPH_FORCES_BT>(c, tg, rows, cap, corr_dt);
+      case TSPH_FORCES_WCSPH: pair_launch<TSPH_FORCES_WCSPH>(c, tg, rows, cap, corr, corr_dt); break;
+      case TSPH_FORCES_DBC: pair_launch<TSPH_FORCES_DBC>(c, tg, rows, cap, corr, corr_dt); break;
+      default: pair_launch<TSPH_FORCES_BT>(c, tg, rows, cap, corr, corr_dt);
     }
   });
-  if (!e && mixed && corr_dt > 0) {  // as launch_corrector leaves them
+  if (!e && mixed && corr) {  // as launch_corrector leaves them
     c->records_valid = false;
     c->marks_valid = true;
   }

@@ -1,12 +1,14 @@
 // Device closures of the pair physics.  The FP64 forms keep the reference's
 // operand order and rounding points so that, compiled with -ffp-contract=off,
 // they produce the same bits as the reference for every IEEE operation
-// (only pow() comes from the device math library).  The FP32 forms are the
+// (only pow() comes from the device math library, and the hydrostatic root
+// of the diffusion term from tsph_root7.h).  The FP32 forms are the
 // mixed-precision path.
 #pragma once
 
 #include <hip/hip_runtime.h>
 #include "tsph_internal.h"
+#include "tsph_root7.h"
 
 namespace tsph {
 
@@ -75,7 +77,7 @@ __device__ __forceinline__ double diff_fourtakas64(double h, double drs, double 
                                                    double gravity) {
   const double cb = c0 * c0 * rho0 / 7.;
   const double dpH = 1. + dr_z * rho0 * gravity / cb;
-  const double drhoH = rho0 * pow(dpH, 1. / 7) - rho0;
+  const double drhoH = rho0 * root7(dpH) - rho0;  // pow(dpH, 1/7), tsph_root7.h
   const double psi = 2. * c0 * h * delta * ((nrho - arho) - drhoH) / (drs * drs + kEps * h * h);
   return psi * drdW * m / nrho;
 }

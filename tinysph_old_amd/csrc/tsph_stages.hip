@@ -159,12 +159,10 @@ __device__ __forceinline__ void wall_sums(const Soa &d, const int *__restrict__ 
         if (r2 <= r2cut) {
           const double drs = sqrt(r2);
           if (drs / k.h <= 2.0) {
-            double t[NS];
-#pragma unroll
-            for (int q = 0; q < NS; ++q) t[q] = 0.;
-            term(hh[v], drx, dry, drs, t);  // t = the candidate's terms (0 + x is x)
-#pragma unroll
-            for (int q = 0; q < NS; ++q) grp[u * NS + q] = t[q];
+            // the candidate's terms, straight into the group's LDS slots (the
+            // reference adds each to a running sum that starts at +0, which
+            // never turns -0, so the term x itself stands for 0 + x)
+            term(hh[v], drx, dry, drs, grp + u * NS);
             ok = true;
           }
         }
@@ -242,21 +240,21 @@ __global__ __launch_bounds__(64) void k_mdbc(Soa d, const int *__restrict__ cell
   // rhowsum, drhodx, drhody, a11, a12, a13, a21, a22, a23, a31, a32, a33
   double acc[12];
   wall_sums<12>(d, cell_start + k.cell_base, k, gn, live, acc, s_terms,
-                [&](double nrho, double drx, double dry, double drs, double (&a)[12]) {
+                [&](double nrho, double drx, double dry, double drs, double *a) {
                   const KW64 kw = wendland64(drs, k.h, k.awen, k.bwen);
                   const double mr = m / nrho;
-                  a[0] += m * kw.w;
-                  a[1] += drx * kw.f * m;
-                  a[2] += dry * kw.f * m;
-                  a[3] += kw.w * mr;
-                  a[4] += drx * kw.w * mr;
-                  a[5] += dry * kw.w * mr;
-                  a[6] += drx * kw.f * mr;
-                  a[7] += drx * drx * kw.f * mr;
-                  a[8] += drx * dry * kw.f * mr;
-                  a[9] += dry * kw.f * mr;
-                  a[10] += dry * drx * kw.f * mr;
-                  a[11] += dry * dry * kw.f * mr;
+                  a[0] = m * kw.w;
+                  a[1] = drx * kw.f * m;
+                  a[2] = dry * kw.f * m;
+                  a[3] = kw.w * mr;
+                  a[4] = drx * kw.w * mr;
+                  a[5] = dry * kw.w * mr;
+                  a[6] = drx * kw.f * mr;
+                  a[7] = drx * drx * kw.f * mr;
+                  a[8] = drx * dry * kw.f * mr;
+                  a[9] = dry * kw.f * mr;
+                  a[10] = dry * drx * kw.f * mr;
+                  a[11] = dry * dry * kw.f * mr;
                 });
   bool wall_bad = false;
   if (live && (threadIdx.x & (kWG - 1)) == 0) {
@@ -305,10 +303,10 @@ __global__ __launch_bounds__(64) void k_bt_density(Soa d, const int *__restrict_
   const double m = k.m;
   double acc[2];  // wsum, rhowsum (src/SPH_kernel_approx.h:1838-1953)
   wall_sums<2>(d, cell_start + k.cell_base, k, live ? d.r[i] : make_double2(0., 0.), live, acc,
-               s_terms, [&](double nrho, double, double, double drs, double (&a)[2]) {
+               s_terms, [&](double nrho, double, double, double drs, double *a) {
                  const KW64 kw = wendland64(drs, k.h, k.awen, k.bwen);
-                 a[0] += kw.w * m / nrho;
-                 a[1] += m * kw.w;
+                 a[0] = kw.w * m / nrho;
+                 a[1] = m * kw.w;
                });
   bool wall_bad = false;
   if (live && (threadIdx.x & (kWG - 1)) == 0) {

@@ -545,10 +545,12 @@ def batch_for_cases(cases: list, precision: int = L.PREC_FP64) -> BatchSystem:
     return bs
 
 
-def autotune(ps: ParticleSystem, variant: int):
+def autotune(ps: ParticleSystem, variant: int, with_particles_per_lane: bool = False):
     """tsph_autotune: (tile_rows, waves_per_cu, balanced, ms per pair pass) of the
-    fastest setting."""
-    r, w, b, ms = C.c_int(), C.c_int(), C.c_int(), C.c_double()
+    fastest setting, and its mean particles per lane when asked for."""
+    r, w, b, ms, ppl = C.c_int(), C.c_int(), C.c_int(), C.c_double(), C.c_double()
     L.check(ps.lib.tsph_autotune(ps.ctx, variant, C.byref(r), C.byref(w), C.byref(b),
-                                 C.byref(ms)), "tsph_autotune")
+                                 C.byref(ms), C.byref(ppl)), "tsph_autotune")
+    if with_particles_per_lane:
+        return r.value, w.value, b.value, ms.value, ppl.value
     return r.value, w.value, b.value, ms.value
