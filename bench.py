@@ -41,12 +41,17 @@ HBM_PEAK_GBS = 8000.0
 # algorithmic bytes (per pair: neighbour record; per active particle: own record + outputs)
 B_PAIR = {T.PREC_FP64: 50, T.PREC_MIXED: 26}
 B_PART = {T.PREC_FP64: 74, T.PREC_MIXED: 50}  # mixed keeps FP64 outputs (a 16 B + dRho 8 B)
+# the symplectic corrector fused into the first mixed pair pass of a step, per
+# particle: reads v_o, r_o (16 B each), rho_o, rho (8 B each), writes v, r
+# (16 B each), rho (8 B) and the removal state (1 B)
+B_CORR = 89
 SEED = 20241202
 KERNEL = "k_tile32<0>"  # the mixed WCSPH pair kernel (mDBC case: fluid-only interactions)
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_dam_break_mdbc_740k.md")
 
 
 SQ_PROFILE = os.path.join(ROOT, "profiles", "sq_counters_dam_break_mdbc_740k.md")
+ALE_SQ_PROFILE = os.path.join(ROOT, "profiles", "sq_counters_ale_muscl_183k.md")
 # VALU issue: one wave64 instruction per 4 cycles per SIMD, 256 CUs x 4 SIMDs, 2.4 GHz
 VALU_ISSUE_PER_S = 256 * 4 * 2.4e9 / 4
 
@@ -71,16 +76,24 @@ def measured_traffic(kernel: str):
     return profile_value(TRAFFIC_PROFILE, "traffic_bytes_per_launch", kernel)
 
 
-def valu_ceiling(kernel: str, avg_ms: float) -> dict:
-    """The VALU-issue ceiling of `kernel`: SQ_INSTS_VALU per launch (committed
-    rocprofv3 summary) at one wave instruction per 4 cycles on every SIMD, and
-    the fraction of that floor the measured launch time reaches."""
-    insts = profile_value(SQ_PROFILE, "valu_insts_per_launch", kernel)
-    if insts is None:
-        return {"valu_insts_per_launch": None, "valu_floor_ms": None, "valu_frac": None}
-    floor_ms = insts / VALU_ISSUE_PER_S * 1e3
-    return {"valu_insts_per_launch": insts, "valu_floor_ms": floor_ms,
-            "valu_frac": floor_ms / avg_ms if avg_ms > 0 else None}
+def valu_ceiling(kernel: str, avg_ms: float, path: str = SQ_PROFILE) -> dict:
+    """The VALU-issue ceiling of `kernel` from a committed rocprofv3 summary:
+    SQ_INSTS_VALU per launch at one wave instruction per 4 cycles on every
+    SIMD (valu_floor_ms, valu_frac), and SQ_ACTIVE_INST_VALU, the quad-cycles
+    those instructions occupy the SIMDs (transcendentals take four), at the
+    same rate (issue_floor_ms, issue_frac); the fractions are the floor over
+    the measured launch time."""
+    insts = profile_value(path, "valu_insts_per_launch", kernel)
+    quads = profile_value(path, "valu_quad_cycles_per_launch", kernel)
+    out = {"valu_insts_per_launch": insts, "valu_floor_ms": None, "valu_frac": None,
+           "issue_floor_ms": None, "issue_frac": None}
+    if insts is not None:
+        out["valu_floor_ms"] = insts / VALU_ISSUE_PER_S * 1e3
+        out["valu_frac"] = out["valu_floor_ms"] / avg_ms if avg_ms > 0 else None
+    if quads is not None:
+        out["issue_floor_ms"] = quads / VALU_ISSUE_PER_S * 1e3
+        out["issue_frac"] = out["issue_floor_ms"] / avg_ms if avg_ms > 0 else None
+    return out
 
 
 def copy_probe(mib: int = 512, reps: int = 3):
@@ -204,8 +217,10 @@ def ale_pass_roofline(ps, prec, reps: int = 10) -> dict:
         nbytes = pairs * bp + act * ba
         avg = ms / max(n, 1)
         gbs = nbytes / (avg * 1e-3) / 1e9
+        key = f"ale{'32' if prec == T.PREC_MIXED else '64'}:{name}"
         out[name] = {"avg_launch_ms": avg, "pairs_per_launch": pairs, "algorithmic_bytes_per_launch":
-                     nbytes, "achieved_gbs": gbs, "frac": gbs / HBM_PEAK_GBS}
+                     nbytes, "achieved_gbs": gbs, "frac": gbs / HBM_PEAK_GBS,
+                     **valu_ceiling(key, avg, ALE_SQ_PROFILE)}
     return out
 
 
@@ -315,10 +330,10 @@ def cpu_baseline(case, budget_s: float = 20.0, threads: int = 0, max_steps: int 
 
     threads = threads or host_threads()
     O = po.oracle_case(case.oracle_params(), case.arrays, omp_threads=threads)
+    # the initial cell list only, as the GPU runs it is compared with start
+    # (tsph_particle_to_cell, then steps); the pair count of one pass is taken
+    # from the GPU counters of the same case
     oracle_init(O)
-    O.call("density_to_pressure")
-    O.call("compute_forces", 0)
-    # pair count of one pass is taken from the GPU counters of the same case
     t0 = time.perf_counter()
     n = 0
     while True:
@@ -574,6 +589,8 @@ def main() -> int:
         pairs_per_launch = r["pairs_b"] / max(r["launches"], 1)
         act_per_launch = r["active_b"] / max(r["launches"], 1)
         bytes_per_launch = pairs_per_launch * B_PAIR[prec] + act_per_launch * B_PART[prec]
+        if prec == T.PREC_MIXED:  # one of the step's two launches also corrects every particle
+            bytes_per_launch += 0.5 * B_CORR * case.np
         achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
         return avg_ms, achieved, bytes_per_launch, pairs_per_launch
 

@@ -92,7 +92,7 @@ __device__ __forceinline__ int point_cell(const Consts &k, double x, double y) {
 // order, so every sum is formed in the reference order.  A candidate that is
 // not a fluid neighbour within the support adds nothing, as in the reference.
 constexpr int kWG = 16;
-constexpr int kWPre = 5;
+constexpr int kWPre = 3;
 constexpr int kWallsPerWave = 64 / kWG;
 
 template <int NS, typename Term>
@@ -172,11 +172,22 @@ __device__ __forceinline__ void wall_sums(const Soa &d, const int *__restrict__ 
       // the compiler from moving them across the hand-over
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       __builtin_amdgcn_wave_barrier();
-      for (int w = 0; w < kWG; ++w) {
-        if (!((gm >> w) & 1u)) continue;
+      // the round's terms of sum q = u + kWG x, read for all kWG candidates at
+      // once (the LDS reads in flight together), then added in candidate
+      // order; a candidate that is not a neighbour adds +0, which leaves the
+      // sum unchanged (a sum that starts at +0 and adds terms is never -0).
+      // Lanes past the last sum read sum NS - 1 and discard it.
 #pragma unroll
-        for (int x = 0; x < QPL; ++x)
-          if (u + kWG * x < NS) mine[x] += grp[w * NS + u + kWG * x];
+      for (int x = 0; x < QPL; ++x) {
+        const int q = min(u + kWG * x, NS - 1);
+#pragma unroll
+        for (int h = 0; h < kWG; h += kWG / 2) {
+          double tv[kWG / 2];
+#pragma unroll
+          for (int w = 0; w < kWG / 2; ++w) tv[w] = grp[(h + w) * NS + q];
+#pragma unroll
+          for (int w = 0; w < kWG / 2; ++w) mine[x] += ((gm >> (h + w)) & 1u) ? tv[w] : 0.;
+        }
       }
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       __builtin_amdgcn_wave_barrier();
@@ -195,18 +206,24 @@ __device__ __forceinline__ void wall_sums(const Soa &d, const int *__restrict__ 
 // The out-of-range density count of the EOS pass that follows the wall
 // densities in the symplectic step (k_eos's count; the pass itself is not
 // needed there): the wall densities just computed are counted by the lanes
-// that computed them (wall_bad), every other particle by a grid-stride sweep.
-// The sweep loads kSweep particles per thread at a time, type and density
-// together, so it costs one memory round trip per kSweep grid strides.
+// that computed them (count_walls), every other particle by a sweep that
+// runs in workgroups of its own at the end of the same launch, beside the
+// wall workgroups rather than after them.  A sweep thread loads kSweep
+// particles at a time, type and density together.
 constexpr int kSweep = 4;
 
-__device__ __forceinline__ void count_bad_density(const Soa &d, const int *__restrict__ scal,
-                                                  bool wall_bad,
-                                                  unsigned long long *__restrict__ counters) {
+__device__ __forceinline__ void count_walls(bool wall_bad, unsigned long long *__restrict__ counters) {
+  const unsigned long long w = __ballot(wall_bad);
+  if (w && (threadIdx.x & 63) == 0) atomicAdd(&counters[C_BADRHO], (unsigned long long)__popcll(w));
+}
+
+// sweep workgroup sb of nsb (64 threads each)
+__device__ __forceinline__ void sweep_density(const Soa &d, const int *__restrict__ scal, int sb,
+                                              int nsb, unsigned long long *__restrict__ counters) {
   const int n = scal[S_NP];
-  const int stride = gridDim.x * blockDim.x;
-  unsigned bad = wall_bad;
-  for (int p0 = blockIdx.x * blockDim.x + threadIdx.x; p0 < n; p0 += kSweep * stride) {
+  const int stride = nsb * 64;
+  unsigned bad = 0;
+  for (int p0 = sb * 64 + (int)threadIdx.x; p0 < n; p0 += kSweep * stride) {
     int t[kSweep];
     double rho[kSweep];
 #pragma unroll
@@ -221,20 +238,29 @@ __device__ __forceinline__ void count_bad_density(const Soa &d, const int *__res
         bad += rho[q] > 1500. || rho[q] < 0.;
   }
   for (int o = 32; o > 0; o >>= 1) bad += __shfl_xor(bad, o, 64);
-  if (bad && (threadIdx.x & 63) == 0) atomicAdd(&counters[C_BADRHO], (unsigned long long)bad);
+  if (bad && threadIdx.x == 0) atomicAdd(&counters[C_BADRHO], (unsigned long long)bad);
 }
 
+// sweep workgroups that follow wall_blocks wall workgroups for n particles
+inline int sweep_blocks(int n) { return grid_for(n, 64 * kSweep); }
+
 // nwall: the wall count of the cell list in use (one of the two alternating slots)
+// blocks past wall_blocks sweep the other particles' densities (counters set)
 __global__ __launch_bounds__(64) void k_mdbc(Soa d, const int *__restrict__ cell_start, const int *__restrict__ nwall,
                        const int *__restrict__ wall_list, Consts k0,
                        const Consts *__restrict__ cases, unsigned long long *__restrict__ counters,
-                       const int *__restrict__ scal) {
+                       const int *__restrict__ scal, int wall_blocks) {
+  if ((int)blockIdx.x >= wall_blocks) {
+    sweep_density(d, scal, blockIdx.x - wall_blocks, gridDim.x - wall_blocks, counters);
+    return;
+  }
   __shared__ __align__(16) double s_terms[64 * 12];
   const int w = blockIdx.x * kWallsPerWave + (int)(threadIdx.x / kWG);
   const bool live = w < *nwall;
   const int i = live ? wall_list[w] : 0;
   const Consts k = live ? case_consts(k0, cases, d.cid, i) : k0;
   const double2 gn = live ? d.gnr[i] : make_double2(0., 0.);
+  const double2 r = live ? d.r[i] : make_double2(0., 0.);  // loaded with the ghost node
   const double m = k.m;
   // sums of src/SPH_kernel_approx.h:1141-1342 in its order:
   // rhowsum, drhodx, drhody, a11, a12, a13, a21, a22, a23, a31, a32, a33
@@ -258,7 +284,6 @@ __global__ __launch_bounds__(64) void k_mdbc(Soa d, const int *__restrict__ cell
                 });
   bool wall_bad = false;
   if (live && (threadIdx.x & (kWG - 1)) == 0) {
-  const double2 r = d.r[i];
   const double rhowsum = acc[0], drhodx = acc[1], drhody = acc[2];
   const double a11 = acc[3], a12 = acc[4], a13 = acc[5], a21 = acc[6], a22 = acc[7], a23 = acc[8];
   const double a31 = acc[9], a32 = acc[10], a33 = acc[11];
@@ -288,13 +313,17 @@ __global__ __launch_bounds__(64) void k_mdbc(Soa d, const int *__restrict__ cell
   d.rho[i] = rho;
   wall_bad = rho > 1500. || rho < 0.;
   }
-  if (counters) count_bad_density(d, scal, wall_bad, counters);
+  if (counters) count_walls(wall_bad, counters);
 }
 
 __global__ __launch_bounds__(64) void k_bt_density(Soa d, const int *__restrict__ cell_start, const int *__restrict__ nwall,
                              const int *__restrict__ wall_list, Consts k0,
                              const Consts *__restrict__ cases, unsigned long long *__restrict__ counters,
-                             const int *__restrict__ scal) {
+                             const int *__restrict__ scal, int wall_blocks) {
+  if ((int)blockIdx.x >= wall_blocks) {
+    sweep_density(d, scal, blockIdx.x - wall_blocks, gridDim.x - wall_blocks, counters);
+    return;
+  }
   __shared__ __align__(16) double s_terms[64 * 2];
   const int w = blockIdx.x * kWallsPerWave + (int)(threadIdx.x / kWG);
   const bool live = w < *nwall;
@@ -315,7 +344,7 @@ __global__ __launch_bounds__(64) void k_bt_density(Soa d, const int *__restrict_
     d.rho[i] = rho;
     wall_bad = rho > 1500. || rho < 0.;
   }
-  if (counters) count_bad_density(d, scal, wall_bad, counters);
+  if (counters) count_walls(wall_bad, counters);
 }
 
 // Removal.  The reference erases inside an ascending loop without stepping
@@ -512,18 +541,20 @@ int launch_corrector(tsph_ctx *c, double dt) {
 }
 
 int launch_mdbc(tsph_ctx *c, bool count_density, hipStream_t s) {
-  hipLaunchKernelGGL(k_mdbc, dim3(grid_for(c->n_wall, kWallsPerWave)), dim3(64), 0,
-                     s ? s : c->stream, c->cur, c->cell_start, c->scal + c->nwall_idx,
-                     c->wall_list, c->k, c->cases, count_density ? c->counters : nullptr, c->scal);
+  const int wb = grid_for(c->n_wall, kWallsPerWave), sb = count_density ? sweep_blocks(c->n_ub) : 0;
+  hipLaunchKernelGGL(k_mdbc, dim3(wb + sb), dim3(64), 0, s ? s : c->stream, c->cur, c->cell_start,
+                     c->scal + c->nwall_idx, c->wall_list, c->k, c->cases,
+                     count_density ? c->counters : nullptr, c->scal, wb);
   TSPH_HIP(hipGetLastError());
   c->records_valid = false;
   return 0;
 }
 
 int launch_bt_density(tsph_ctx *c, bool count_density, hipStream_t s) {
-  hipLaunchKernelGGL(k_bt_density, dim3(grid_for(c->n_wall, kWallsPerWave)), dim3(64), 0,
-                     s ? s : c->stream, c->cur, c->cell_start, c->scal + c->nwall_idx,
-                     c->wall_list, c->k, c->cases, count_density ? c->counters : nullptr, c->scal);
+  const int wb = grid_for(c->n_wall, kWallsPerWave), sb = count_density ? sweep_blocks(c->n_ub) : 0;
+  hipLaunchKernelGGL(k_bt_density, dim3(wb + sb), dim3(64), 0, s ? s : c->stream, c->cur,
+                     c->cell_start, c->scal + c->nwall_idx, c->wall_list, c->k, c->cases,
+                     count_density ? c->counters : nullptr, c->scal, wb);
   TSPH_HIP(hipGetLastError());
   c->records_valid = false;
   return 0;
