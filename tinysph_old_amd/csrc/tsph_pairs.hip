@@ -499,13 +499,22 @@ __global__ __launch_bounds__(kWave * kPoolWaves) void k_tile64(Soa d, const int 
     }
     nact += act;
     const double2 ar = s_r[li], av = s_v[li], arp = s_rp[li];
-    // walk of the 9 stencil cells in reference order: cell q, LDS range [lj, lend)
+    // walk of the 9 stencil cells in reference order: cell q, LDS range [lj, lend).
+    // A cell's candidates are tested kGroup at a time (their LDS loads in
+    // flight together) and appended without branches: every slot of the group
+    // is written, the count advances by the test.  The test admits a superset
+    // of the support (r^2 <= 4h^2 (1 + 1e-12)); pair64 applies the reference's
+    // exact qq <= 2, and a candidate it drops adds nothing, as in the
+    // reference, so the list keeps every pair in the reference order.  Slots
+    // past the cell's end are masked (they lie inside the wave's LDS area).
+    const double c2 = 4.0 * k.h * k.h * (1.0 + 1e-12);
+    const bool dbc_wall = VAR == TSPH_FORCES_DBC && at == TSPH_WALL;
     int q = -1, lj = 0, lend = 0;
     bool more = act;
     Acc64 s;
     while (__any(more)) {
       int cnt = 0;
-      while (more && cnt < kList) {
+      while (more && cnt <= kList - kGroup) {
         if (lj >= lend) {
           ++q;
           if (q == 9) { more = false; break; }
@@ -519,12 +528,22 @@ __global__ __launch_bounds__(kWave * kPoolWaves) void k_tile64(Soa d, const int 
           lend = tab[e + 1] + bq;
           continue;
         }
-        const int j = lj++;
-        if (j == li) continue;
-        if (VAR == TSPH_FORCES_DBC && at == TSPH_WALL && s_t[j] == TSPH_WALL) continue;
-        ++ncand;
-        const double2 nr = s_r[j];
-        if (in_support64(ar.x - nr.x, ar.y - nr.y, k.h)) my_list[(cnt++) * kWave] = (uint16_t)j;
+        const int m = min(kGroup, lend - lj);
+        double2 pp[kGroup];
+#pragma unroll
+        for (int u = 0; u < kGroup; ++u) pp[u] = s_r[lj + u];
+#pragma unroll
+        for (int u = 0; u < kGroup; ++u) {
+          const int j = lj + u;
+          bool cand = u < m && j != li;
+          if (VAR == TSPH_FORCES_DBC) cand = cand && !(dbc_wall && s_t[j] == TSPH_WALL);
+          ncand += cand;
+          const double drx = ar.x - pp[u].x, dry = ar.y - pp[u].y;
+          const bool in = cand && (drx * drx) + (dry * dry) <= c2;
+          my_list[cnt * kWave] = (uint16_t)j;
+          cnt += in;
+        }
+        lj += m;
       }
       for (int e = 0; e < cnt; ++e) {
         const int j = my_list[e * kWave];
