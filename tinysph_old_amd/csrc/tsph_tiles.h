@@ -272,11 +272,19 @@ __device__ __forceinline__ void walk64(const P &pol, const Consts &k, const int 
           }
         }
       } else {
+        // a cell's candidates kGroup at a time, their LDS loads in flight
+        // together, appended without branches (every slot of the group is
+        // written, the count advances by the test); the exact test of
+        // in_support64, with its rounded square root only for the rare
+        // candidates within a relative 1e-12 of the boundary.  Slots past the
+        // cell's end are masked; they lie in the staged area or the next
+        // array of the wave's slots (positions come first).
+        const double c2 = 4.0 * k.h * k.h, c2lo = c2 * (1.0 - 1e-12), c2hi = c2 * (1.0 + 1e-12);
         int q = -1, lj = 0, lend = 0;
         bool more = act;
         while (__any(more)) {
           int cnt = 0;
-          while (more && cnt < kList) {
+          while (more && cnt <= kList - kGroup) {
             if (lj >= lend) {
               ++q;
               if (q == 9) { more = false; break; }
@@ -290,11 +298,23 @@ __device__ __forceinline__ void walk64(const P &pol, const Consts &k, const int 
               lend = tab[e + 1] + bq;
               continue;
             }
-            const int j = lj++;
-            if (j == li) continue;
-            ++ncand;
-            const double2 nr = pol.pos_lds(L, j);
-            if (in_support64(o.r.x - nr.x, o.r.y - nr.y, k.h)) my_list[(cnt++) * kWave] = (uint16_t)j;
+            const int m = min(kGroup, lend - lj);
+            double2 pp[kGroup];
+#pragma unroll
+            for (int u = 0; u < kGroup; ++u) pp[u] = pol.pos_lds(L, lj + u);
+#pragma unroll
+            for (int u = 0; u < kGroup; ++u) {
+              const int j = lj + u;
+              const bool cand = u < m && j != li;
+              ncand += cand;
+              const double drx = o.r.x - pp[u].x, dry = o.r.y - pp[u].y;
+              const double r2 = (drx * drx) + (dry * dry);
+              bool in = cand && r2 < c2lo;
+              if (cand && !in && r2 <= c2hi) in = sqrt(r2) / k.h <= 2.0;
+              my_list[cnt * kWave] = (uint16_t)j;
+              cnt += in;
+            }
+            lj += m;
           }
           npair += cnt;
           for (int e = 0; e < cnt; ++e) pol.pair_lds(acc, o, L, my_list[e * kWave]);
@@ -378,17 +398,32 @@ __device__ __forceinline__ void walk32(const P &pol, const Consts &k, const int 
         const int a1 = tab[t.w + rlo] + t.base[1], n1 = tab[t.w + rhi] + t.base[1] - a1;
         const int a2 = tab[2 * t.w + rlo] + t.base[2], n2 = tab[2 * t.w + rhi] + t.base[2] - a2;
         const int tot = act ? n0 + n1 + n2 : 0;
+        // candidates kGroup at a time from one column slice, their LDS loads
+        // in flight together, appended without branches; slots past the
+        // slice's range are masked (they lie in the staged area or the next
+        // array of the wave's slots, positions coming first)
+        const int n01 = n0 + n1;
         int cur = 0;
         while (__any(cur < tot)) {
           int cnt = 0;
-          while (cur < tot && cnt < kList) {
-            const int j = cur < n0 ? a0 + cur : (cur < n0 + n1 ? a1 + (cur - n0) : a2 + (cur - n0 - n1));
-            ++cur;
-            if (j == li) continue;
-            ++ncand;
-            const float2 pj = pol.pos_lds(L, j);
-            const float drx = o.p.x - pj.x, dry = o.p.y - pj.y;
-            if (drx * drx + dry * dry < r2cut) my_list[(cnt++) * kWave] = (uint16_t)j;
+          while (cur < tot && cnt <= kList - kGroup) {
+            const bool in0 = cur < n0, in01 = cur < n01;
+            const int end = in0 ? n0 : (in01 ? n01 : tot);
+            const int j0 = cur + (in0 ? a0 : (in01 ? a1 - n0 : a2 - n01));
+            const int m = min(kGroup, end - cur);
+            float2 pp[kGroup];
+#pragma unroll
+            for (int u = 0; u < kGroup; ++u) pp[u] = pol.pos_lds(L, j0 + u);
+#pragma unroll
+            for (int u = 0; u < kGroup; ++u) {
+              const int j = j0 + u;
+              const bool cand = u < m && j != li;
+              ncand += cand;
+              const float drx = o.p.x - pp[u].x, dry = o.p.y - pp[u].y;
+              my_list[cnt * kWave] = (uint16_t)j;
+              cnt += cand && drx * drx + dry * dry < r2cut;
+            }
+            cur += m;
           }
           npair += cnt;
           for (int e = 0; e < cnt; ++e) {
