@@ -757,9 +757,15 @@ template <class P, int SLOT, bool FP32>
 int launch_pass(tsph_ctx *c, const P &pol) {
   const int rows = c->tile_rows;
   // slots budgeted per tile, within 64 KiB of LDS per wavefront; a tile whose
-  // neighbourhood does not fit is gathered from global memory instead
+  // neighbourhood does not fit is gathered from global memory instead.  The
+  // ALE records are large (57 B mixed, 113 B FP64 with the MUSCL gradients),
+  // so the budget is kAleCellCap records per cell, about a full cell's count
+  // at the configured resolutions: the fullest tiles then take the global
+  // path, and the smaller LDS area lets a compute unit hold more waves, which
+  // pays more than staging every tile does
+  constexpr int kAleCellCap = 9;
   constexpr int kFixed = kTabBytes + sizeof(uint16_t) * kList * kWave;
-  const int cap = std::min(3 * (rows + 2) * kCellCap, (65536 - kPoolHdr - kFixed) / SLOT);
+  const int cap = std::min(3 * (rows + 2) * kAleCellCap, (65536 - kPoolHdr - kFixed) / SLOT);
   const TileGrid tg = tile_grid(c, FP32, kFixed + (size_t)cap * SLOT);
   if (int e = ensure_tile_list(c, tg.pooled ? tg.groups : 0)) return e;
   return launch_timed(c, [&] {
