@@ -446,12 +446,13 @@ int tsph_create(const tsph_params *prm, tsph_ctx **out) {
   set_walk_defaults(c);
   c->k = k;
   c->ncells_tot = k.ncells;
-  double *grid_scratch = nullptr;  // set-up of tsph_interpolate_grid_noalloc on this device
   if (int e = alloc_context(c)) {
     tsph_destroy(c);
     return e;
   }
-  if (int e = grid_scratch_ptr(&grid_scratch)) {
+  // set-up of tsph_interpolate_grid_noalloc: the scratch array's address on
+  // this device (the device current at creation, where the context runs)
+  if (int e = grid_scratch_ptr(&c->grid_scratch)) {
     tsph_destroy(c);
     return e;
   }

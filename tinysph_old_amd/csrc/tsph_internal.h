@@ -164,6 +164,9 @@ struct tsph_ctx {
   // follows a symplectic corrector: the next predictor rewrites them); until
   // rebuilt they are alt's values through perm (ensure_olds)
   bool olds_stale = false;
+  // device address of the allocation-free interpolation grid's static scratch
+  // array on this context's device (looked up at creation)
+  double *grid_scratch = nullptr;
   bool has_ale = false;        // ALE volume variables allocated
   uint8_t *ale_flag = nullptr; // per particle: its own ALE gamma sum is non-zero
   hipStream_t stream = nullptr;

@@ -393,8 +393,8 @@ int tsph_interpolate_grid_noalloc(tsph_ctx *c, double x0, double y0, double xm, 
   if (!c->cells_valid) return io_fail(TSPH_ESTATE, "no cell list: call tsph_particle_to_cell first");
   const size_t cnt = (size_t)nx * ny;
   if (cnt == 0) return 0;
-  double *dout = nullptr;
-  if (int e = grid_scratch_ptr(&dout)) return e;
+  double *dout = c->grid_scratch;  // this context's device's copy of the array
+  if (!dout) return io_fail(TSPH_ESTATE, "no scratch address for this context");
   if (int e = launch_interp_grid(c, x0, y0, dh, nx, ny, dout, true)) return e;
   const size_t b = sizeof(double) * cnt;
   if ((rho && hipMemcpyAsync(rho, dout, b, hipMemcpyDeviceToHost, c->stream) != hipSuccess) ||
