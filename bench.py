@@ -279,9 +279,9 @@ def leg_inflow(steps):
 
 
 def host_threads() -> int:
-    """Threads for the oracle: the host cores this process may use.  A GPU box
-    gives each GPU a share of the host and says so in OMP_NUM_THREADS; else the
-    CPU affinity mask."""
+    """Threads for the oracle: the host cores this process may use, i.e. the CPU
+    affinity mask, or OMP_NUM_THREADS where a machine that shares its host
+    between GPUs sets a smaller share there."""
     try:
         aff = len(os.sched_getaffinity(0))
     except AttributeError:

@@ -6,7 +6,8 @@ cases/damBreakLong_mDBC/SPH_simulationSymplectic.h) includes the reference's
 src/*.h headers by name and its case's constants.h / geometry.h.  A shim
 directory maps every src/*.h name to the drop-in header, the case directory
 supplies constants.h and geometry.h, and the pair-loop variant is chosen on the
-command line as INTEGRATION.md describes (the reference chooses it by #include).
+command line with -DTSPH_DROPIN_FORCES / -DTSPH_DROPIN_RALE (the reference chooses
+it by #include).
 The drivers are compiled and linked here only; they are never run, and nothing
 of the reference leaves this machine.  Skipped where /root/reference is absent."""
 import concurrent.futures as cf
