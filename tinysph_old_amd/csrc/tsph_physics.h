@@ -1,13 +1,14 @@
 // Device closures of the pair physics.  The FP64 forms keep the reference's
 // operand order and rounding points so that, compiled with -ffp-contract=off,
-// they produce the same bits as the reference for every IEEE operation
-// (only pow() comes from the device math library, and the hydrostatic root
-// of the diffusion term from tsph_root7.h).  The FP32 forms are the
-// mixed-precision path.
+// they produce the same bits as the reference for every IEEE operation; the
+// two powers of the reference (pow(rho/rho0, 7) of the EOS and pow(dpH, 1/7)
+// of the diffusion term) come from tsph_pow7.h and tsph_root7.h.  The FP32
+// forms are the mixed-precision path.
 #pragma once
 
 #include <hip/hip_runtime.h>
 #include "tsph_internal.h"
+#include "tsph_pow7.h"
 #include "tsph_root7.h"
 
 namespace tsph {
@@ -165,7 +166,7 @@ __device__ __forceinline__ void predict(double2 &r, double2 &v, double &rho, dou
 __device__ __forceinline__ double tait64(double rho, double rho0, double c0) {
   const double b = c0 * c0 * rho0 / 7.;
   const double rr = rho / rho0;
-  return b * (pow(rr, 7.) - 1);
+  return b * (pow7(rr) - 1);  // pow(rr, 7.), tsph_pow7.h
 }
 
 // ---------------------------------------------------------------------------
