@@ -120,6 +120,7 @@ def _declare(lib: C.CDLL, pfx: str) -> None:
         f("ale_compute_density_limited", None, C.c_void_p)
         f("ale_compute_acceleration_limited", None, C.c_void_p)
         f("minmod_phi", C.c_double, *([C.c_double] * 6))
+        f("muscl_states", None, _dp, C.c_int, _dp)
         f("init_ale", None, C.c_void_p)
         f("rale_forces", None, C.c_void_p)
         f("set_threads", None, C.c_void_p, C.c_int)

@@ -600,11 +600,8 @@ inline double ale_cs(double rho, double rho0, double c0) {
 // reference (alpha = 1); limited = 1 weights each reconstruction by the
 // minmod limiter of its own variable.
 struct MusclLR { double rhoL, rhoR; V2 vL, vR; };
-inline MusclLR muscl_states(const Case& s, int ai, int ni, V2 dr, int limited) {
-  const V2 ga = s.gradrho[ai], gn = s.gradrho[ni];
-  const V2 gax = s.gradvx[ai], gnx = s.gradvx[ni], gay = s.gradvy[ai], gny = s.gradvy[ni];
-  const double arho = s.rho[ai], nrho = s.rho[ni];
-  const V2 av = s.v[ai], nv = s.v[ni];
+inline MusclLR muscl_states_of(double arho, double nrho, V2 av, V2 nv, V2 ga, V2 gn, V2 gax,
+                               V2 gay, V2 gnx, V2 gny, V2 dr, int limited) {
   const V2 rij = V2{0., 0.} - dr;  // r_j - r_i
   double a1 = 1, a2 = 1, a3 = 1, a4 = 1, a5 = 1, a6 = 1;
   if (limited) {
@@ -623,6 +620,10 @@ inline MusclLR muscl_states(const Case& s, int ai, int ni, V2 dr, int limited) {
   o.vR.x = muscl_right(nv.x, gnx, dr, a5);
   o.vR.y = muscl_right(nv.y, gny, dr, a6);
   return o;
+}
+inline MusclLR muscl_states(const Case& s, int ai, int ni, V2 dr, int limited) {
+  return muscl_states_of(s.rho[ai], s.rho[ni], s.v[ai], s.v[ni], s.gradrho[ai], s.gradrho[ni],
+                         s.gradvx[ai], s.gradvy[ai], s.gradvx[ni], s.gradvy[ni], dr, limited);
 }
 
 // Compute_Density, src/ALESPH_density_and_omega_bt.h:9-281.  Active: every
@@ -1100,6 +1101,16 @@ void orc_ale_leapfrog_one(void* vc, double dt) { ale_leapfrog_one(*static_cast<C
 void orc_ale_leapfrog_two(void* vc, double dt, int step) {
   ale_leapfrog_two(*static_cast<Case*>(vc), dt, step);
 }
+// the reconstructed states of one pair (test hook): in = arho, nrho, av.x, av.y,
+// nv.x, nv.y, then the gradients ga, gn, gax, gay, gnx, gny and dr = r_i - r_j as
+// x, y pairs (20 values); out = rhoL, rhoR, vL.x, vL.y, vR.x, vR.y
+void orc_muscl_states(const double* in, int limited, double* out) {
+  auto v2 = [&](int k) { return V2{in[k], in[k + 1]}; };
+  const MusclLR o = muscl_states_of(in[0], in[1], v2(2), v2(4), v2(6), v2(8), v2(10), v2(12),
+                                    v2(14), v2(16), v2(18), limited);
+  out[0] = o.rhoL; out[1] = o.rhoR; out[2] = o.vL.x; out[3] = o.vL.y; out[4] = o.vR.x; out[5] = o.vR.y;
+}
+
 double orc_minmod_phi(double aC, double aD, double gx, double gy, double rx, double ry) {
   return minmod_phi(aC, aD, V2{gx, gy}, V2{rx, ry});
 }

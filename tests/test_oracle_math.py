@@ -320,6 +320,46 @@ def test_minmod_weight_values(L):
     assert phi(1.0, 1.0 + dA * 2, *g, *rcd) == 0.0  # r = 0
 
 
+def test_limited_reconstruction_pairs_each_variable_with_its_own_gradient(L):
+    """The six reconstructions of one pair (rho, vx, vy on both sides) each with
+    its own gradient, its own difference and its own limiter weight.  Inputs
+    are chosen so that the six minmod weights differ (1, 0.5, 0, 0.25, 1,
+    0.75; every quantity binary-exact), so a gradient, a difference or a
+    direction paired with the wrong variable or side changes the result.
+    Expected: side i (the particle) reconstructs A_i + phi/2 g_i.dr with
+    phi = minmod(2 g_i.(r_j - r_i) / (A_j - A_i) - 1), side j A_j - phi/2 g_j.dr
+    with phi = minmod(2 g_j.(r_i - r_j) / (A_i - A_j) - 1), dr = r_i - r_j."""
+    dr = (-0.25, -0.5)
+    rij = (0.25, 0.5)
+    arho, nrho = 1.0, 1.5
+    av, nv = (0.0, 0.0), (1.0, -1.0)
+    ga, gn = (2.0, 1.0), (0.5, 0.5)
+    gax, gay = (2.0, -1.0), (-0.5, -1.0)
+    gnx, gny = (2.0, 2.0), (-1.5, -1.0)
+
+    def dot(a, b):
+        return a[0] * b[0] + a[1] * b[1]
+
+    def phi(aC, aD, g, rcd):
+        return max(0.0, min(1.0, 2 * dot(g, rcd) / (aD - aC) - 1))
+
+    weights = [phi(arho, nrho, ga, rij), phi(nrho, arho, gn, dr), phi(av[0], nv[0], gax, rij),
+               phi(av[1], nv[1], gay, rij), phi(nv[0], av[0], gnx, dr), phi(nv[1], av[1], gny, dr)]
+    assert weights == [1.0, 0.5, 0.0, 0.25, 1.0, 0.75]
+    a = np.array([arho, nrho, *av, *nv, *ga, *gn, *gax, *gay, *gnx, *gny, *dr], dtype=np.float64)
+    for limited in (0, 1):
+        w = weights if limited else [1.0] * 6
+        want = [arho + w[0] * 0.5 * dot(ga, dr), nrho - w[1] * 0.5 * dot(gn, dr),
+                av[0] + w[2] * 0.5 * dot(gax, dr), av[1] + w[3] * 0.5 * dot(gay, dr),
+                nv[0] - w[4] * 0.5 * dot(gnx, dr), nv[1] - w[5] * 0.5 * dot(gny, dr)]
+        out = np.zeros(6)
+        L.orc_muscl_states(a.ctypes.data_as(po._dp), limited, out.ctypes.data_as(po._dp))
+        assert list(out) == want, (limited, list(out), want)
+    # the limited states differ from the unlimited ones in the four weighted
+    # reconstructions and agree in the two with weight 1
+    assert want[0] == arho - 0.5 and want[4] == 1.75
+
+
 def ale_blob(linear, n=20, dp=0.01):
     xs, ys = np.meshgrid(np.arange(n) * dp + 0.3, np.arange(n) * dp + 0.3, indexing="ij")
     r = np.stack([xs.ravel(), ys.ravel()], 1)
