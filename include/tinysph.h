@@ -419,13 +419,13 @@ int tsph_batch_step(tsph_ctx *ctx, int wall, int nsteps);
  * Outputs may be NULL.  No counterpart in the reference. */
 int tsph_autotune(tsph_ctx *ctx, int variant, int *tile_rows, int *waves_per_cu, int *balanced,
                   double *best_ms, double *particles_per_lane);
-/* Tile walk of the pair loops for the context's precision: 1 = pooled
- * (workgroups of up to 4 wavefronts, each with a chunk of the tile list of
- * about equal work whose tiles its wavefronts take from a counter in LDS; the
- * mixed path's default), 0 = static striding, one wavefront per workgroup (the
- * FP64 path's default).  Grids with
- * fewer than 16 tiles per persistent wave always stride.  Results do not depend
- * on it.  No counterpart in the reference. */
+/* Tile walk of the pair loops for the context's precision: 1 = pooled (the
+ * default: workgroups of up to 4 wavefronts, each with a chunk of the tile
+ * list of about equal cost, cut by the time each tile took in the previous
+ * pass, whose tiles its wavefronts take from a counter in LDS, the costliest
+ * first), 0 = static striding, one wavefront per workgroup.  Grids with fewer
+ * than 16 tiles per persistent wave, and the ALE passes, always stride.
+ * Results do not depend on it.  No counterpart in the reference. */
 int tsph_set_balanced(tsph_ctx *ctx, int balanced);
 
 /* ---- case setup (host only; no GPU needed) ----

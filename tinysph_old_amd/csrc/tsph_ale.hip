@@ -104,7 +104,7 @@ __global__ __launch_bounds__(kWave * kPoolWaves) void k_rale64(Soa d, const int 
                                                   uint8_t *__restrict__ gflag,
                                                   unsigned long long *__restrict__ counters) {
   extern __shared__ __align__(16) unsigned char smem[];
-  unsigned char *wl = wave_lds(smem, lds_wave);
+  unsigned char *wl = wave_lds<DYN>(smem, lds_wave);
   int *tab = reinterpret_cast<int *>(wl);
   double2 *s_r = reinterpret_cast<double2 *>(wl + kTabBytes);
   double2 *s_v = s_r + cap;
@@ -115,12 +115,13 @@ __global__ __launch_bounds__(kWave * kPoolWaves) void k_rale64(Soa d, const int 
 
   if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&counters[C_PASSES], 1ull);
   unsigned npair = 0, ncand = 0, nact = 0;
-  TileQueue tq = tq_begin<DYN>(scal, bound, smem);
+  TileQueue tq = tq_begin<DYN>(scal, bound, tile_list, smem);
   for (int it = tq_take<DYN>(tq, tq_ticket<DYN>(tq)); it >= 0;) {
     wave_sync();
     const int ticket = tq_ticket<DYN>(tq);  // read once this tile's loads are waited for
     Tile t;
     tile_begin(k, cell_start, tile_list[it], rows, tab, t);
+    tq_mark(tq, tile_list[it]);
     const bool staged = t.lq[3] <= cap;
     if (staged) {
       for (int l = tile_lane(); l < t.lq[3]; l += kWave) {
@@ -208,6 +209,7 @@ __global__ __launch_bounds__(kWave * kPoolWaves) void k_rale64(Soa d, const int 
     }
     it = next;
   }
+  tq_mark(tq, -1);
   wave_count(npair, ncand, nact, counters);
 }
 
@@ -279,7 +281,7 @@ __global__ __launch_bounds__(kWave * kPoolWaves) void k_rale32(Soa d, const int 
                                                   unsigned long long *__restrict__ counters) {
 #pragma clang fp contract(fast)
   extern __shared__ __align__(16) unsigned char smem[];
-  unsigned char *wl = wave_lds(smem, lds_wave);
+  unsigned char *wl = wave_lds<DYN>(smem, lds_wave);
   int *tab = reinterpret_cast<int *>(wl);
   float2 *s_pos = reinterpret_cast<float2 *>(wl + kTabBytes);
   float2 *s_vel = s_pos + cap;
@@ -295,12 +297,13 @@ __global__ __launch_bounds__(kWave * kPoolWaves) void k_rale32(Soa d, const int 
 
   if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&counters[C_PASSES], 1ull);
   unsigned npair = 0, ncand = 0, nact = 0;
-  TileQueue tq = tq_begin<DYN>(scal, bound, smem);
+  TileQueue tq = tq_begin<DYN>(scal, bound, tile_list, smem);
   for (int it = tq_take<DYN>(tq, tq_ticket<DYN>(tq)); it >= 0;) {
     wave_sync();
     const int ticket = tq_ticket<DYN>(tq);  // read once this tile's loads are waited for
     Tile t;
     tile_begin(k, cell_start, tile_list[it], rows, tab, t);
+    tq_mark(tq, tile_list[it]);
     const bool staged = t.lq[3] <= cap;
     const int mid = t.row_lo + (t.nrows >> 1);
     const double ox = k.x0 - k.of + t.cx * k.kh, oy = k.y0 - k.of + mid * k.kh;
@@ -397,6 +400,7 @@ __global__ __launch_bounds__(kWave * kPoolWaves) void k_rale32(Soa d, const int 
     }
     it = next;
   }
+  tq_mark(tq, -1);
   wave_count(npair, ncand, nact, counters);
 }
 
@@ -515,7 +519,9 @@ int launch_rale_forces(tsph_ctx *c) {
   const size_t list = sizeof(uint16_t) * kList * kWave;
   int e = 0;
   if (c->prm.precision == TSPH_PREC_MIXED) {
-    const TileGrid tg = tile_grid(c, true, kTabBytes + list + (size_t)cap * (4 * sizeof(float2) + 1));
+    // the ALE passes stride (see launch_pass, tsph_alemuscl.hip)
+    const TileGrid tg =
+        tile_grid(c, true, kTabBytes + list + (size_t)cap * (4 * sizeof(float2) + 1), false);
     if ((e = ensure_tile_list(c, tg.pooled ? tg.groups : 0))) return e;
     e = launch_timed(c, [&] {
       if (tg.pooled)
@@ -528,7 +534,8 @@ int launch_rale_forces(tsph_ctx *c) {
                            tg.lds_wave, c->ale_flag, c->counters);
     });
   } else {
-    const TileGrid tg = tile_grid(c, false, kTabBytes + list + (size_t)cap * (4 * sizeof(double2) + 1));
+    const TileGrid tg =
+        tile_grid(c, false, kTabBytes + list + (size_t)cap * (4 * sizeof(double2) + 1), false);
     if ((e = ensure_tile_list(c, tg.pooled ? tg.groups : 0))) return e;
     e = launch_timed(c, [&] {
       if (tg.pooled)

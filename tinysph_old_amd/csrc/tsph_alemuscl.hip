@@ -766,7 +766,11 @@ int launch_pass(tsph_ctx *c, const P &pol) {
   constexpr int kAleCellCap = 9;
   constexpr int kFixed = kTabBytes + sizeof(uint16_t) * kList * kWave;
   const int cap = std::min(3 * (rows + 2) * kAleCellCap, (65536 - kPoolHdr - kFixed) / SLOT);
-  const TileGrid tg = tile_grid(c, FP32, kFixed + (size_t)cap * SLOT);
+  // the static walk: a cell build serves several passes of different cost
+  // profiles, and the dam breaks these passes run on give each wave about ten
+  // tiles, too few for the pooled walk to pay for its chunk cut (183k
+  // particles, FP64: 0.56 ms per step pooled against 0.51 static)
+  const TileGrid tg = tile_grid(c, FP32, kFixed + (size_t)cap * SLOT, false);
   if (int e = ensure_tile_list(c, tg.pooled ? tg.groups : 0)) return e;
   return launch_timed(c, [&] {
     if constexpr (FP32) {

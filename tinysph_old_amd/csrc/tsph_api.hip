@@ -68,27 +68,29 @@ static void drop_graph(tsph_ctx *c) {
   }
 }
 
-// default pair-loop launch: 7-cell tiles; the mixed path's shorter tiles hide
-// more latency with 12 waves per CU in the pooled walk (740k-particle dam
-// break: 0.080 ms against 0.088 ms for the static walk at 8; the autotuner
-// picks the same), the FP64 path's larger LDS records allow
-// 8 waves per CU, where the static walk is the faster
+// default pair-loop launch: 7-cell tiles in the pooled walk (chunks cut by
+// the recorded tile costs); the mixed path's shorter records hide more
+// latency with 12 waves per CU, the FP64 path's larger LDS records allow 8
+// (740k-particle dam break, pair pass: mixed 0.070 ms pooled against 0.088
+// ms static, FP64 0.248 ms against 0.277 ms)
 static void set_walk_defaults(tsph_ctx *c) {
   const bool mixed = c->prm.precision == TSPH_PREC_MIXED;
   c->waves_per_cu = mixed ? 12 : 8;
-  c->balanced[0] = false;
-  c->balanced[1] = mixed;
+  c->balanced[0] = true;
+  c->balanced[1] = true;
 }
 
 static HostState host_state(const tsph_ctx *c) {
   return {c->cells_valid, c->tiles_valid, c->records_valid, c->marks_valid, c->olds_stale,
-          c->list_groups};
+          c->list_groups, c->list_table, c->list_cost_table};
 }
 
 static void set_host_state(tsph_ctx *c, const HostState &h) {
   c->cells_valid = h.cells_valid;
   c->tiles_valid = h.tiles_valid;
   c->list_groups = h.list_groups;
+  c->list_table = h.list_table;
+  c->list_cost_table = h.list_cost_table;
   c->records_valid = h.records_valid;
   c->marks_valid = h.marks_valid;
   c->olds_stale = h.olds_stale;
@@ -251,6 +253,11 @@ static int record_step(tsph_ctx *c, int wall, double dt, int inlet = -1, int out
     }
     if (zones && (e = launch_eos(c))) return e;
   }
+  // each build cuts the chunks of the pooled pair pass that follows it
+  const bool mixed = c->prm.precision == TSPH_PREC_MIXED;
+  const int groups = pass_groups(c, variant);
+  c->cut_groups = groups;
+  c->cut_table = mixed ? 1 : 0;
   if (zones) {
     // the buffer zones act on the predicted positions
     if ((e = launch_predictor(c, dt))) return e;
@@ -270,6 +277,8 @@ static int record_step(tsph_ctx *c, int wall, double dt, int inlet = -1, int out
   }
   // r_o, v_o, rho_o are dead after the corrector until the next predictor
   // rewrites them: this build leaves them behind
+  c->cut_groups = groups;
+  c->cut_table = 0;
   if ((e = launch_cell_list(c, true, false, 0., active, true, nullptr, true)) ||
       (e = launch_forces(c, variant)))
     return e;
@@ -364,14 +373,15 @@ static int alloc_context(tsph_ctx *c) {
   c->seq_cap = 2 * cap + kBzMax + c->ncases;
   if ((e = alloc_soa(c->cur, cap, false, false)) || (e = alloc_soa(c->alt, cap, false, false)))
     return e;
-  if ((e = dalloc(&c->scal, S_NSCAL)) ||
+  if ((e = dalloc(&c->scal, S_NSCAL + (size_t)kCostTables * (nct + 8))) ||
       (e = dalloc(&c->cell_count, nct + 1)) || (e = dalloc(&c->cell_start, nct + 2)) ||
       (e = dalloc(&c->cell_start_next, nct + 2)) ||
       (e = dalloc(&c->cellof, cap)) || (e = dalloc(&c->hrank, cap)) || (e = dalloc(&c->hcell, cap)) ||
       (e = dalloc(&c->perm, cap)) ||
       (e = dalloc(&c->seqstate, (size_t)c->seq_cap)) || (e = dalloc(&c->probe_out, 4)) ||
       (e = dalloc(&c->bz_list, 2 * kBzMax)) ||
-      (e = dalloc(&c->tile_list, nct + 8)) || (e = dalloc(&c->wall_list, cap)) ||
+      (e = dalloc(&c->tile_list, nct + 8)) || (e = dalloc(&c->list_cost, nct + 8)) ||
+      (e = dalloc(&c->wall_list, cap)) ||
       (e = dalloc(&c->scan_state, (nct + 1) / 1024 + 2)))
     return e;
   // the per-cell counters and removal states start cleared; later builds clear
@@ -403,7 +413,8 @@ static int alloc_context(tsph_ctx *c) {
   if (c->prm.precision == TSPH_PREC_MIXED) {
     if ((e = dalloc(&c->mx.rv, cap)) || (e = dalloc(&c->mx.rp, cap))) return e;
   }
-  if (hipMemsetAsync(c->scal, 0, S_NSCAL * sizeof(int), c->stream) != hipSuccess ||
+  if (hipMemsetAsync(c->scal, 0, (S_NSCAL + (size_t)kCostTables * (nct + 8)) * sizeof(int),
+                     c->stream) != hipSuccess ||
       hipMemsetAsync(c->counters, 0, counter_words(c) * sizeof(unsigned long long), c->stream) != hipSuccess ||
       hipStreamSynchronize(c->stream) != hipSuccess)
     return fail(TSPH_EHIP, "initialisation failed");
@@ -469,8 +480,8 @@ int tsph_destroy(tsph_ctx *c) {
   free_soa(c->alt);
   void *ps[] = {c->scal, c->counters, c->cell_count, c->cell_start, c->cell_start_next, c->cellof, c->hrank,
                 c->hcell, c->perm, c->scan_state, c->seqstate, c->probe_out, c->mx.rv,
-                c->mx.rp, c->mx.nrm, c->tile_list, c->wall_list, c->bz_list, c->cases,
-                c->tile_base, c->list_part, c->tile_bound};
+                c->mx.rp, c->mx.nrm, c->tile_list, c->list_cost, c->wall_list, c->bz_list,
+                c->cases, c->tile_base, c->list_part, c->tile_bound};
   for (void *p : ps)
     if (p) (void)hipFree(p);
   for (hipEvent_t ev : c->tev) (void)hipEventDestroy(ev);
@@ -1120,8 +1131,9 @@ int tsph_set_tile_rows(tsph_ctx *c, int rows) {
   if (rows != c->tile_rows) {
     drop_graph(c);
     c->tiles_valid = false;
+    c->tile_rows = rows;
+    if (int e = clear_tile_costs(c)) return e;  // costs of the old tile layout
   }
-  c->tile_rows = rows;
   return update_tile_base(c);
 }
 
@@ -1748,6 +1760,10 @@ int tsph_autotune(tsph_ctx *c, int variant, int *tile_rows, int *waves_per_cu, i
       if ((e = tsph_set_tile_rows(c, r))) break;
       for (int w : kWaves) {
         if ((e = tsph_set_waves_per_cu(c, w)) || (e = launch_forces(c, variant))) break;  // warm-up
+        // a pooled walk is timed with chunks cut by the tile costs the
+        // warm-up recorded (its own chunks were cut by estimates)
+        c->list_groups = 0;
+        if ((e = launch_forces(c, variant))) break;
         if (hipEventRecord(e0, c->stream) != hipSuccess) { e = fail(TSPH_EHIP, "event record failed"); break; }
         for (int k = 0; k < kReps && !e; ++k) e = launch_forces(c, variant);
         if (e) break;

@@ -26,6 +26,7 @@
 // pass is needed.  The scan also lists the pair loops' non-empty tiles.
 #include "tsph_internal.h"
 #include "tsph_physics.h"
+#include "tsph_tiles.h"
 
 namespace tsph {
 
@@ -151,6 +152,12 @@ struct TileScan {
   int ncases;
   int rows;
   int *list;
+  // with cost_in (one cost table: tile t at cost_in[kCostTables t]): each
+  // listed tile's cost (or own-particle estimate) at its list position in
+  // cost_list, for the chunk cut of a pooled pass
+  const int *cost_in;
+  int *cost_list;
+  CostEstimate cm;  // the estimate for a tile without a recorded cost
 };
 
 // the case whose cell range holds cell c (batched grids)
@@ -217,7 +224,7 @@ __device__ __forceinline__ void load_counts(const int *__restrict__ in, int len,
 template <int IT>
 __device__ __forceinline__ int stage_counts(int len, int block, const TileScan ts,
                                             const int (&v)[IT], int halo, int (&tl)[IT],
-                                            int *s_cnt, int *s_case) {
+                                            int (&own)[IT], int *s_cnt, int *s_case) {
   constexpr int kTile = kScanBlock * IT;
   const int b0 = block * kTile, base = b0 + threadIdx.x * IT;
 #pragma unroll
@@ -232,10 +239,11 @@ __device__ __forceinline__ int stage_counts(int len, int block, const TileScan t
   for (int j = 0; j < IT; ++j) {
     advance_grid(ts, base + j, g);
     const int2 tn = base + j < len ? tile_at(ts, base + j, g) : make_int2(-1, 0);
-    int own = 0;
-    for (int r = 0; r < tn.y; ++r) own += s_cnt[threadIdx.x * IT + j + r];
-    tl[j] = own ? tn.x : -1;
-    nt += own != 0;
+    int o = 0;
+    for (int r = 0; r < tn.y; ++r) o += s_cnt[threadIdx.x * IT + j + r];
+    tl[j] = o ? tn.x : -1;
+    own[j] = o;
+    nt += o != 0;
   }
   return nt;
 }
@@ -300,9 +308,13 @@ __global__ __launch_bounds__(kScanBlock) void k_scan(const int *__restrict__ in,
   __shared__ int s_cnt[kTile + kMaxTileRows];
   __shared__ int s_case;
   const int nb = gridDim.x, blk = blockIdx.x;
-  int v[IT], tl[IT], halo;
+  int v[IT], tl[IT], own[IT], halo;
   load_counts<IT>(in, len, blk, v, halo);
-  const int nt = stage_counts<IT>(len, blk, ts, v, halo, tl, s_cnt, &s_case);
+  const int nt = stage_counts<IT>(len, blk, ts, v, halo, tl, own, s_cnt, &s_case);
+  // the listed tiles' recorded costs, in flight during the look-back
+  int w[IT];
+#pragma unroll
+  for (int j = 0; j < IT; ++j) w[j] = ts.cost_in && tl[j] >= 0 ? ts.cost_in[kCostTables * tl[j]] : 0;
   int s = 0;
 #pragma unroll
   for (int j = 0; j < IT; ++j) s += v[j];
@@ -411,7 +423,10 @@ __global__ __launch_bounds__(kScanBlock) void k_scan(const int *__restrict__ in,
   for (int j = 0; j < IT; ++j) {
     if (base + j < len) out[base + j] = ex;
     ex += v[j];
-    if (tl[j] >= 0) ts.list[et++] = tl[j];
+    if (tl[j] >= 0) {
+      if (ts.cost_list) ts.cost_list[et] = w[j] > 0 ? w[j] : ts.cm.own * own[j] + ts.cm.tile;
+      ts.list[et++] = tl[j];
+    }
   }
   if (threadIdx.x == 0) {
     if (blk == 0) scal[nwall_idx] = 0;  // the reorder that follows counts the walls
@@ -419,52 +434,6 @@ __global__ __launch_bounds__(kScanBlock) void k_scan(const int *__restrict__ in,
       out[len] = pc + total;
       scal[S_NTILES] = pt + tiles;
     }
-  }
-}
-
-// chunk starts of a pooled walk of `groups` workgroups over the listed tiles.
-// Listed tile t's work interval is [W0, W0 + own + kTileWork0) with W0 =
-// cell_start[its first cell] + kTileWork0 t (every cell before it lies in a
-// tile before it: tiles and cells are both in (case, column, row) order) and
-// W = cell_start[ncells] + kTileWork0 ntiles; chunk g starts at the tile whose
-// interval holds ceil(g W / groups).
-__device__ __forceinline__ void cut_chunks(const int *__restrict__ cell_start, int ncells, int t,
-                                           int ntiles, int c0, int n, int groups,
-                                           int *__restrict__ bound) {
-  const long long w0 = cell_start[c0] + (long long)kTileWork0 * t;
-  const long long w1 = cell_start[c0 + n] + (long long)kTileWork0 * (t + 1);
-  const long long wt = cell_start[ncells] + (long long)kTileWork0 * ntiles;
-  const long long g0 = (w0 * groups + wt - 1) / wt, g1 = (w1 * groups + wt - 1) / wt;
-  for (long long g = g0; g < g1 && g < groups; ++g) bound[g] = t;
-}
-
-// the last chunk bound (and all of them for an empty list)
-__device__ __forceinline__ void close_chunks(int ntiles, int groups, int *__restrict__ bound) {
-  if (threadIdx.x == 0) bound[groups] = ntiles;
-  if (ntiles == 0)
-    for (int g = threadIdx.x; g < groups; g += blockDim.x) bound[g] = 0;
-}
-
-// cut_chunks for batched grids (one thread per listed tile)
-__global__ void k_tile_bounds(const int *__restrict__ cell_start, const int *__restrict__ scal,
-                              const int *__restrict__ tile_list, const Consts *__restrict__ cases,
-                              const int *__restrict__ tile_base, int ncases, int rows, int ncells,
-                              int groups, int *__restrict__ bound) {
-  const int ntiles = scal[S_NTILES];
-  if (blockIdx.x == 0) close_chunks(ntiles, groups, bound);
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < ntiles; t += gridDim.x * blockDim.x) {
-    const int tile = tile_list[t];
-    int lo = 0, hi = ncases;
-    while (hi - lo > 1) {
-      const int mid = (lo + hi) >> 1;
-      if (tile_base[mid] <= tile) lo = mid;
-      else hi = mid;
-    }
-    const Consts &k = cases[lo];
-    const int nty = (k.ncy + rows - 1) / rows, lt = tile - tile_base[lo];
-    const int cx = lt / nty, cy = (lt - cx * nty) * rows;
-    cut_chunks(cell_start, ncells, t, ntiles, k.cell_base + cx * k.ncy + cy,
-               min(rows, k.ncy - cy), groups, bound);
   }
 }
 
@@ -543,10 +512,19 @@ __device__ __forceinline__ void order_cell(const int *__restrict__ cell_start, i
   }
 }
 
-__global__ void k_order(const int *__restrict__ cell_start, int ncells,
-                        const uint32_t *__restrict__ seq, int *__restrict__ perm,
-                        int *__restrict__ cell_count) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+// (batched grids) groups > 0: the first `cuts` workgroups cut the new list
+// into the chunks of the pooled pair pass that follows (cut_chunks_block)
+__global__ __launch_bounds__(tiles::kCutBlock) void k_order(
+    const int *__restrict__ cell_start, int ncells, const uint32_t *__restrict__ seq,
+    int *__restrict__ perm, int *__restrict__ cell_count, const int *__restrict__ scal, int groups,
+    int cuts, const int *__restrict__ list_cost, int *__restrict__ bound) {
+  __shared__ int wl[tiles::kCutSeg];
+  __shared__ long long red[tiles::kCutBlock / 64];
+  if ((int)blockIdx.x < cuts) {
+    tiles::cut_chunks_block(scal, list_cost, groups, bound, blockIdx.x, wl, red);
+    return;
+  }
+  const int c = (blockIdx.x - cuts) * blockDim.x + threadIdx.x;
   if (c <= ncells) cell_count[c] = 0;
   if (c < ncells) order_cell(cell_start, c, seq, perm);
 }
@@ -554,28 +532,31 @@ __global__ void k_order(const int *__restrict__ cell_start, int ncells,
 // The same over the cells of the listed (non-empty) tiles only, for a single
 // case: every occupied cell lies in a listed tile, so the other cells need
 // neither ordering nor clearing (their counts are 0).  A bounded grid strides
-// over the (tile, row) items.
-// groups > 0: also the chunk starts of a pooled walk of that many workgroups
-// (cut_chunks).
-__global__ void k_order_tiles(const int *__restrict__ cell_start, const int *__restrict__ scal,
-                              const int *__restrict__ tile_list, int rows, int ncy, int ncells,
-                              const uint32_t *__restrict__ seq, int *__restrict__ perm,
-                              int *__restrict__ cell_count, int groups, int *__restrict__ bound) {
-  const int ntiles = scal[S_NTILES];
-  if (blockIdx.x == 0) {
-    if (threadIdx.x == 0) cell_count[ncells] = 0;  // the removed bucket
-    if (groups > 0) close_chunks(ntiles, groups, bound);
+// over the (tile, row) items.  groups > 0: the first `cuts` workgroups cut
+// the new list into the chunks of the pooled pair pass that follows, beside
+// the ordering (cut_chunks_block).
+__global__ __launch_bounds__(tiles::kCutBlock) void k_order_tiles(
+    const int *__restrict__ cell_start, const int *__restrict__ scal,
+    const int *__restrict__ tile_list, int rows, int ncy, int ncells,
+    const uint32_t *__restrict__ seq, int *__restrict__ perm, int *__restrict__ cell_count,
+    int groups, int cuts, const int *__restrict__ list_cost, int *__restrict__ bound) {
+  __shared__ int wl[tiles::kCutSeg];
+  __shared__ long long red[tiles::kCutBlock / 64];
+  if ((int)blockIdx.x < cuts) {
+    tiles::cut_chunks_block(scal, list_cost, groups, bound, blockIdx.x, wl, red);
+    return;
   }
+  const int ob = blockIdx.x - cuts, nord = gridDim.x - cuts;  // ordering workgroups
+  const int ntiles = scal[S_NTILES];
+  if (ob == 0 && threadIdx.x == 0) cell_count[ncells] = 0;  // the removed bucket
   const int items = ntiles * rows;
   const int nty = (ncy + rows - 1) / rows;
-  for (int item = blockIdx.x * blockDim.x + threadIdx.x; item < items; item += gridDim.x * blockDim.x) {
+  for (int item = ob * blockDim.x + threadIdx.x; item < items; item += nord * blockDim.x) {
     const int it = item / rows, r = item - it * rows;
     const int tile = tile_list[it];
     const int cx = tile / nty, cy = (tile - cx * nty) * rows + r;
     if (cy >= ncy) continue;
     const int c = cx * ncy + cy;
-    if (groups > 0 && r == 0)
-      cut_chunks(cell_start, ncells, it, ntiles, c, min(rows, ncy - cy), groups, bound);
     cell_count[c] = 0;
     order_cell(cell_start, c, seq, perm);
   }
@@ -711,25 +692,23 @@ __global__ void k_gather_olds(const int *__restrict__ scal, const int *__restric
 }
 
 // exclusive scan of the cell counts into cell_start, with the tile list of the
-// current tile_rows
-int scan_cells(tsph_ctx *c, hipStream_t s, int *cs, int nwall_idx) {
+// current tile_rows (and, for cost_table >= 0, the listed tiles' costs)
+int scan_cells(tsph_ctx *c, hipStream_t s, int *cs, int nwall_idx, int cost_table) {
   const int len = c->ncells_tot + 1;
-  const int groups = pooled_groups(c);
   const TileScan ts{c->k.ncy, c->k.ncells, c->k.cell_base, c->cases, c->tile_base, c->ncases,
-                    c->tile_rows, c->tile_list};
+                    c->tile_rows, c->tile_list,
+                    cost_table >= 0 ? c->scal + S_NSCAL + cost_table : nullptr,
+                    cost_table >= 0 ? c->list_cost : nullptr,
+                    cost_estimate(c->prm.precision == TSPH_PREC_MIXED)};
   if (scan_items(c) == 4)
     hipLaunchKernelGGL(k_scan<4>, dim3(scan_blocks(c)), dim3(kScanBlock), 0, s, c->cell_count,
                        len, c->scan_state, cs, c->scal, c->counters, ts, c->scan_wait, nwall_idx);
   else
     hipLaunchKernelGGL(k_scan<16>, dim3(scan_blocks(c)), dim3(kScanBlock), 0, s, c->cell_count,
                        len, c->scan_state, cs, c->scal, c->counters, ts, c->scan_wait, nwall_idx);
-  // a single grid's chunks are cut by k_order_tiles
-  if (c->cases && groups > 0)
-    hipLaunchKernelGGL(k_tile_bounds, dim3(2 * c->num_cu), dim3(256), 0, s, cs,
-                       c->scal, c->tile_list, c->cases, c->tile_base, c->ncases, c->tile_rows,
-                       c->ncells_tot, groups, c->tile_bound);
   TSPH_HIP(hipGetLastError());
-  c->list_groups = groups;
+  c->list_groups = 0;  // a pooled pass cuts the new list
+  c->list_cost_table = cost_table;
   return 0;
 }
 
@@ -768,17 +747,42 @@ int launch_cell_list(tsph_ctx *c, bool eos, bool predict, double dt, unsigned ad
   else if (decide) TSPH_HASH(false, true);
   else TSPH_HASH(false, false);
 #undef TSPH_HASH
-  if (int e = scan_cells(c, s, cs, wslot)) return e;
+  // the pooled pair pass that follows (if any) has its chunks cut by the
+  // tile ordering's last workgroup, or after the ordering for a batch, from
+  // the costs the scan lists beside the tiles
+  const int cut = c->cut_groups;
+  c->cut_groups = 0;
+  if (cut > c->bound_cap) {
+    set_error("tile list: more pooled workgroups than allocated");
+    return TSPH_ESTATE;
+  }
+  if (int e = scan_cells(c, s, cs, wslot, cut > 0 ? c->cut_table : -1)) return e;
   hipLaunchKernelGGL(k_scatter, dim3(g), dim3(nb), 0, s, c->scal, c->hcell, c->hrank,
                      cs, c->perm, c->cellof, nct, c->cur.seq, c->seqstate,
                      c->scan_state, scan_blocks(c));
-  if (c->cases)
-    hipLaunchKernelGGL(k_order, dim3(grid_for(nct + 1, nb)), dim3(nb), 0, s, cs, nct,
-                       c->cur.seq, c->perm, c->cell_count);
-  else
-    hipLaunchKernelGGL(k_order_tiles, dim3(2 * c->num_cu), dim3(nb), 0, s, cs, c->scal,
-                       c->tile_list, c->tile_rows, k.ncy, nct, c->cur.seq, c->perm, c->cell_count,
-                       c->list_groups, c->tile_bound);
+  if (c->cases) {
+    const int bs = cut > 0 ? tiles::kCutBlock : nb;
+    const int cuts = cut > 0 ? tiles::cut_segments(total_tiles(c)) : 0;
+    hipLaunchKernelGGL(k_order, dim3(cuts + grid_for(nct + 1, bs)), dim3(bs), 0, s, cs, nct,
+                       c->cur.seq, c->perm, c->cell_count, c->scal, cut, cuts, c->list_cost,
+                       c->tile_bound);
+    if (cut > 0) {
+      c->list_groups = cut;
+      c->list_table = c->cut_table;
+    }
+  } else {
+    // with a cut, one workgroup of kCutBlock threads per compute unit and the
+    // cutting one; without, two of nb (the faster ordering on its own)
+    const int nord = cut > 0 ? std::max(1, c->num_cu) : 2 * std::max(1, c->num_cu);
+    const int cuts = cut > 0 ? tiles::cut_segments(total_tiles(c)) : 0;
+    hipLaunchKernelGGL(k_order_tiles, dim3(cuts + nord), dim3(cut > 0 ? tiles::kCutBlock : nb), 0,
+                       s, cs, c->scal, c->tile_list, c->tile_rows, k.ncy, nct, c->cur.seq,
+                       c->perm, c->cell_count, cut, cuts, c->list_cost, c->tile_bound);
+    if (cut > 0) {
+      c->list_groups = cut;
+      c->list_table = c->cut_table;
+    }
+  }
   if (before_reorder) TSPH_HIP(hipStreamWaitEvent(s, before_reorder, 0));
   const Mixed mx = c->prm.precision == TSPH_PREC_MIXED ? c->mx : Mixed{};
 #define TSPH_REORDER(E, P)                                                                    \
@@ -797,7 +801,7 @@ int launch_cell_list(tsph_ctx *c, bool eos, bool predict, double dt, unsigned ad
   c->olds_stale = skip_olds && !predict;
   c->parity ^= 1;
   c->cells_valid = true;
-  c->tiles_valid = true;  // listed (and cut for the pooled walk) by the scan
+  c->tiles_valid = true;  // listed by the scan
   c->records_valid = eos && c->prm.precision == TSPH_PREC_MIXED;
   if (predict || decide) c->marks_valid = false;
   return 0;

@@ -26,11 +26,31 @@ namespace tsph {
 constexpr uint8_t kRemoved = 255;
 
 constexpr int kMaxTileRows = 24;  // upper bound of tile_rows
-constexpr int kTileWork0 = 16;    // pooled walk's cost model: per-tile work in own particles
+// Pooled walk's cost model.  Every tile pass records the time each tile took
+// its wavefront (tq_mark, tsph_tiles.h; 10 ns ticks of the device's constant
+// clock, averaged with the tile's previous record) in one of kCostTables
+// tables, at scal[S_NSCAL + kCostTables t + table] for global tile t; the
+// next pooled pass of the same table cuts its chunks by these costs
+// (cut_chunks_block).  A recorded time carries what a model of the tile's
+// work would not: what else ran on its compute unit, so that the chunks of a
+// compute unit that ran slow shrink in the next pass.  Table 1 holds the mixed
+// pair passes that apply the symplectic corrector as well, table 0 every
+// other tile pass.  A tile without a record (the first pass after a change of
+// tile layout, or a tile newly occupied) is estimated from its own particles:
+// own particles x own + tile, the least-squares fit of the measured times of
+// the 740k-particle dam break's 13,147 tiles (near-empty tiles 4.4 us mixed,
+// 3.3 us FP64; full 14.2 us and 32 us).
+struct CostEstimate {
+  int own, tile;
+};
+constexpr CostEstimate kCostMixed{19, 425}, kCostFp64{54, 300};
+constexpr int kCostTables = 2;
+inline CostEstimate cost_estimate(bool mixed) { return mixed ? kCostMixed : kCostFp64; }
 
 // device scalar slots
 // (the wall count alternates between S_NWALL and S_NWALL_B with the cell-offset
-// buffers: a build fills the other one, see cell_start_next)
+// buffers: a build fills the other one, see cell_start_next); the per-tile
+// costs follow them (kCostTables (ncells_tot + 8) slots)
 enum { S_NP = 0, S_MAXSEQ = 1, S_NTILES = 2, S_NWALL = 3, S_BZCNT = 4, S_NWALL_B = 5, S_NSCAL = 32 };
 // device counter slots (uint64)
 enum {
@@ -105,7 +125,7 @@ __device__ __forceinline__ Consts case_consts(const Consts &k0, const Consts *__
 // host-side flags that describe what the device buffers hold
 struct HostState {
   bool cells_valid, tiles_valid, records_valid, marks_valid, olds_stale;
-  int list_groups;
+  int list_groups, list_table, list_cost_table;
 };
 
 // one captured step.  A step may rebuild the cell list an odd number of times
@@ -155,7 +175,13 @@ struct tsph_ctx {
   int *tile_bound = nullptr; // pooled walk: first list entry of each workgroup's chunk
   int wave_ctr_cap = 0;      // wavefronts with count slots in `counters`
   int bound_cap = 0;         // chunks tile_bound holds
-  int list_groups = 0;       // chunks the current tile_bound was cut for (0: none)
+  int list_groups = 0;       // chunks the current tile_bound was cut for (0: none or stale)
+  int list_table = 0;        // the cost table they were cut by
+  int *list_cost = nullptr;  // costs of the listed tiles by list position (pooled cuts)
+  int list_cost_table = -1;  // the cost table list_cost was gathered from (-1: none)
+  // a pooled pair pass of `cut_groups` workgroups and cost table `cut_table`
+  // follows the next cell build (0: none): the build cuts its chunks
+  int cut_groups = 0, cut_table = 0;
   int *wall_list = nullptr;  // slots of the wall particles of the current cell list
   int n_wall = 0;            // wall particles (walls are never created or removed)
   bool records_valid = false;  // mixed-path FP32 records match the FP64 state
@@ -172,7 +198,7 @@ struct tsph_ctx {
   hipStream_t stream = nullptr;
   tsph::Soa cur, alt;
   tsph::Mixed mx;
-  int *scal = nullptr;                 // S_* slots
+  int *scal = nullptr;                 // S_* slots, then the per-tile costs
   unsigned long long *counters = nullptr;
   int *cell_count = nullptr;           // ncells+1 (last = removed bucket)
   int *cell_start = nullptr;           // ncells+2
@@ -286,14 +312,19 @@ inline unsigned forces_active_types(int variant) {
 }
 // tsph_pairs.hip
 // list of the non-empty pair-loop tiles; groups > 0: also its chunk bounds
-// for a pooled walk of that many workgroups
-int ensure_tile_list(tsph_ctx *c, int groups);
+// for a pooled walk of that many workgroups, cut by the tile costs recorded
+// in cost table `table`
+int ensure_tile_list(tsph_ctx *c, int groups, int table = 0);
 // geometry of a tile-kernel launch at the context's waves per CU (pooled when
-// walk_balanced holds for the precision)
-TileGrid tile_grid(tsph_ctx *c, bool fp32, size_t lds_wave);
-// workgroups of the pooled walk that launch_forces would take now (0: static),
-// for the cell-list build to cut the tile list for
-int pooled_groups(tsph_ctx *c);
+// may_pool and walk_balanced hold for the precision)
+TileGrid tile_grid(tsph_ctx *c, bool fp32, size_t lds_wave, bool may_pool = true);
+// clears the per-tile costs (the tile layout changed)
+int clear_tile_costs(tsph_ctx *c);
+// cuts the current tile list into `groups` chunks by cost table `table` on
+// stream s (gathering the listed tiles' costs first unless the list build did)
+int launch_cut(tsph_ctx *c, hipStream_t s, int groups, int table);
+// workgroups of a pair pass of `variant` if it takes the pooled walk, else 0
+int pass_groups(tsph_ctx *c, int variant);
 // corr (mixed path only): the pass also applies the symplectic corrector to
 // every particle, with time step corr_dt (batched contexts: each case's own)
 int launch_forces(tsph_ctx *c, int variant, bool corr = false, double corr_dt = 0.);

@@ -293,131 +293,78 @@ __device__ void particle32_global(const Soa &d, const Mixed &mx, const int *__re
 
 
 // The list of non-empty tiles in tile order, built in two passes over blocks
-// of kListBlock tiles: k_tile_count sums each block's non-empty tiles and
-// their work (own particles + kTileWork0 per tile); k_tile_emit adds up the
-// sums of the blocks before its own (the block count is small), scans its
-// block and writes the list.  For the pooled walk it also cuts the list into
-// `groups` chunks of about equal work: chunk g starts at the entry whose work
-// interval [W0, W1) holds floor(g W / groups) (W: total work).
+// of kListBlock tiles: k_tile_count counts each block's non-empty tiles;
+// k_tile_emit adds up the counts of the blocks before its own (the block
+// count is small), scans its block and writes the list.
 constexpr int kListBlock = 256;
 
-__device__ __forceinline__ int tile_own(const int *__restrict__ cell_start, const Consts &k0,
-                                        const Consts *__restrict__ cases,
-                                        const int *__restrict__ tile_base, int ncases, int rows,
-                                        int t) {
-  int lt = t, cell_base = k0.cell_base, ncy = k0.ncy;
-  if (cases) {
-    const int cse = case_of_tile(tile_base, ncases, t);
-    lt = t - tile_base[cse];
-    cell_base = cases[cse].cell_base;
-    ncy = cases[cse].ncy;
-  }
-  const int *cs = cell_start + cell_base;
-  const int nty = (ncy + rows - 1) / rows;
-  const int cx = lt / nty, cy0 = (lt - cx * nty) * rows, cy1 = min(cy0 + rows, ncy);
-  return cs[cx * ncy + cy1] - cs[cx * ncy + cy0];
-}
 
-// sums of a and b over the block (kListBlock threads)
-__device__ __forceinline__ void block_sum2(int &a, int &b, int *lds) {
-  for (int o = 32; o > 0; o >>= 1) {
-    a += __shfl_xor(a, o, 64);
-    b += __shfl_xor(b, o, 64);
-  }
-  const int wid = threadIdx.x >> 6;
-  __syncthreads();
-  if ((threadIdx.x & 63) == 0) {
-    lds[2 * wid] = a;
-    lds[2 * wid + 1] = b;
-  }
-  __syncthreads();
-  a = b = 0;
-  for (int w = 0; w < kListBlock / 64; ++w) {
-    a += lds[2 * w];
-    b += lds[2 * w + 1];
-  }
-}
 
 __global__ __launch_bounds__(kListBlock) void k_tile_count(const int *__restrict__ cell_start, Consts k0,
                                                            const Consts *__restrict__ cases,
                                                            const int *__restrict__ tile_base,
                                                            int ncases, int rows, int ntiles,
                                                            int *__restrict__ part) {
-  __shared__ int lds[2 * kListBlock / 64];
+  __shared__ int lds[kListBlock / 64];
   const int t = blockIdx.x * kListBlock + threadIdx.x;
   const int own = t < ntiles ? tile_own(cell_start, k0, cases, tile_base, ncases, rows, t) : 0;
-  int n = own > 0, w = own > 0 ? own + kTileWork0 : 0;
-  block_sum2(n, w, lds);
-  if (threadIdx.x == 0) {
-    part[2 * blockIdx.x] = n;
-    part[2 * blockIdx.x + 1] = w;
-  }
+  int n = 0;
+  block_exscan(own > 0 ? 1 : 0, n, lds);
+  if (threadIdx.x == 0) part[blockIdx.x] = n;
 }
 
 __global__ __launch_bounds__(kListBlock) void k_tile_emit(const int *__restrict__ cell_start, Consts k0,
                                                           const Consts *__restrict__ cases,
                                                           const int *__restrict__ tile_base,
                                                           int ncases, int rows, int ntiles,
-                                                          const int *__restrict__ part, int groups,
+                                                          const int *__restrict__ part,
                                                           int *__restrict__ scal,
-                                                          int *__restrict__ tile_list,
-                                                          int *__restrict__ bound) {
-  __shared__ int lds[2 * kListBlock / 64];
+                                                          int *__restrict__ tile_list) {
+  __shared__ int lds[kListBlock / 64];
   const int nb = gridDim.x;
-  // tiles and work of the blocks before this one, and of all blocks
-  int bn = 0, bw = 0, tn = 0, tw = 0;
+  // tiles of the blocks before this one, and of all blocks
+  int bn = 0, tn = 0;
   for (int b = threadIdx.x; b < nb; b += kListBlock) {
-    const int n = part[2 * b], w = part[2 * b + 1];
-    if (b < (int)blockIdx.x) {
-      bn += n;
-      bw += w;
-    }
+    const int n = part[b];
+    if (b < (int)blockIdx.x) bn += n;
     tn += n;
-    tw += w;
   }
-  block_sum2(bn, bw, lds);
-  block_sum2(tn, tw, lds);
+  int sum = 0;
+  block_exscan(bn, sum, lds);
+  bn = sum;
+  block_exscan(tn, sum, lds);
+  tn = sum;
   const int t = blockIdx.x * kListBlock + threadIdx.x;
   const int own = t < ntiles ? tile_own(cell_start, k0, cases, tile_base, ncases, rows, t) : 0;
-  const int flag = own > 0, work = flag ? own + kTileWork0 : 0;
-  // exclusive scans of flag and work over the block
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  int xn = flag, xw = work;
-  for (int o = 1; o < 64; o <<= 1) {
-    const int yn = __shfl_up(xn, o, 64), yw = __shfl_up(xw, o, 64);
-    if (lane >= o) {
-      xn += yn;
-      xw += yw;
-    }
+  const int flag = own > 0;
+  const int idx = bn + block_exscan(flag, sum, lds);
+  if (flag) tile_list[idx] = t;
+  if (blockIdx.x == 0 && threadIdx.x == 0) scal[S_NTILES] = tn;
+}
+
+// the recorded costs (cost table: tile t at cost_in[kCostTables t]) of the
+// listed tiles by list position, or their own-particle estimates
+__global__ void k_list_costs(const int *__restrict__ cell_start, Consts k0,
+                             const Consts *__restrict__ cases, const int *__restrict__ tile_base,
+                             int ncases, int rows, const int *__restrict__ scal,
+                             const int *__restrict__ cost_in, const int *__restrict__ tile_list,
+                             int *__restrict__ list_cost, CostEstimate cm) {
+  const int n = scal[S_NTILES];
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+    const int t = tile_list[e];
+    const int w = cost_in[kCostTables * t];
+    list_cost[e] = w > 0 ? w : cm.own * tile_own(cell_start, k0, cases, tile_base, ncases, rows, t) + cm.tile;
   }
-  __syncthreads();
-  if (lane == 63) {
-    lds[2 * wid] = xn;
-    lds[2 * wid + 1] = xw;
-  }
-  __syncthreads();
-  for (int w = 0; w < wid; ++w) {
-    xn += lds[2 * w];
-    xw += lds[2 * w + 1];
-  }
-  if (flag) {
-    const int idx = bn + xn - 1;
-    tile_list[idx] = t;
-    if (groups > 0) {
-      const long long w0 = bw + xw - work, w1 = w0 + work;
-      const long long g0 = (w0 * groups + tw - 1) / tw, g1 = (w1 * groups + tw - 1) / tw;
-      for (long long g = g0; g < g1 && g < groups; ++g) bound[g] = idx;
-    }
-  }
-  if (blockIdx.x == 0) {
-    if (threadIdx.x == 0) scal[S_NTILES] = tn;
-    if (groups > 0) {
-      if (tn == 0)
-        for (int g = threadIdx.x; g <= groups; g += kListBlock) bound[g] = 0;
-      else if (threadIdx.x == 0)
-        bound[groups] = tn;
-    }
-  }
+}
+
+// the chunk cut of a pooled walk as a launch of its own (cut_chunks_block,
+// one workgroup per segment)
+__global__ __launch_bounds__(kCutBlock) void k_cut_chunks(const int *__restrict__ scal,
+                                                          const int *__restrict__ list_cost,
+                                                          int groups, int *__restrict__ bound) {
+  __shared__ int wl[kCutSeg];
+  __shared__ long long red[kCutBlock / 64];
+  cut_chunks_block(scal, list_cost, groups, bound, blockIdx.x, wl, red);
 }
 
 // --------------------------------------------------------------------------
@@ -433,7 +380,7 @@ __global__ __launch_bounds__(kWave * kPoolWaves) void k_tile64(Soa d, const int 
                                                   const int *__restrict__ bound, int lds_wave,
                                                   unsigned long long *__restrict__ counters) {
   extern __shared__ __align__(16) unsigned char smem[];
-  unsigned char *wl = wave_lds(smem, lds_wave);
+  unsigned char *wl = wave_lds<DYN>(smem, lds_wave);
   int *tab = reinterpret_cast<int *>(wl);
   double2 *s_r = reinterpret_cast<double2 *>(wl + kTabBytes);
   double2 *s_v = s_r + cap;
@@ -445,7 +392,7 @@ __global__ __launch_bounds__(kWave * kPoolWaves) void k_tile64(Soa d, const int 
 
   if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&counters[C_PASSES], 1ull);
   unsigned npair = 0, ncand = 0, nact = 0;
-  TileQueue tq = tq_begin<DYN>(scal, bound, smem);
+  TileQueue tq = tq_begin<DYN>(scal, bound, tile_list, smem);
   for (int it = tq_take<DYN>(tq, tq_ticket<DYN>(tq)); it >= 0;) {
   wave_sync();  // the previous tile's LDS is no longer read
   // the next tile's ticket: its atomic is in flight with this tile's loads,
@@ -462,6 +409,7 @@ __global__ __launch_bounds__(kWave * kPoolWaves) void k_tile64(Soa d, const int 
   const int *cs = cell_start + k.cell_base;
   Tile t;
   tile_begin(k, cs, tile, rows, tab, t);
+  tq_mark(tq, tile_list[it]);
   if (t.lq[3] > cap) {
     const int next = tq_take<DYN>(tq, ticket);
     for (int i = t.own0 + tile_lane(); i < t.own1; i += kWave) {
@@ -563,6 +511,7 @@ __global__ __launch_bounds__(kWave * kPoolWaves) void k_tile64(Soa d, const int 
   }
   it = next;
   }  // tiles
+  tq_mark(tq, -1);
   wave_count(npair, ncand, nact, counters);
 }
 
@@ -587,7 +536,7 @@ __global__ __launch_bounds__(kWave * kPoolWaves) void k_tile32(Soa d, Mixed mx, 
                                                   double dt0, uint8_t *__restrict__ state) {
 #pragma clang fp contract(fast)
   extern __shared__ __align__(16) unsigned char smem[];
-  unsigned char *wl = wave_lds(smem, lds_wave);
+  unsigned char *wl = wave_lds<DYN>(smem, lds_wave);
   int *tab = reinterpret_cast<int *>(wl);
   float2 *s_pos = reinterpret_cast<float2 *>(wl + kTabBytes);  // tile frame
   float2 *s_vel = s_pos + cap;
@@ -603,7 +552,7 @@ __global__ __launch_bounds__(kWave * kPoolWaves) void k_tile32(Soa d, Mixed mx, 
   int pend_i = -1;  // particle whose results are still to be stored
   double2 pend_a = make_double2(0., 0.);
   double pend_dr = 0.;
-  TileQueue tq = tq_begin<DYN>(scal, bound, smem);
+  TileQueue tq = tq_begin<DYN>(scal, bound, tile_list, smem, CORR ? 1 : 0);
   for (int it = tq_take<DYN>(tq, tq_ticket<DYN>(tq)); it >= 0;) {
   wave_sync();  // the previous tile's LDS is no longer read
   // the next tile's ticket: its atomic is in flight with this tile's loads,
@@ -621,6 +570,7 @@ __global__ __launch_bounds__(kWave * kPoolWaves) void k_tile32(Soa d, Mixed mx, 
   const C32 cc = make_c32(k);
   Tile t;
   tile_begin(k, cs, tile, rows, tab, t);
+  tq_mark(tq, tile_list[it]);
   // LDS layout with kGroup sentinels after each column slice's records: a
   // candidate group may run past the end of its slice (the sentinels lie far
   // away and fail every distance test) and need not be masked
@@ -821,6 +771,7 @@ __global__ __launch_bounds__(kWave * kPoolWaves) void k_tile32(Soa d, Mixed mx, 
     d.a[pend_i] = pend_a;
     d.drho[pend_i] = pend_dr;
   }
+  tq_mark(tq, -1);
   wave_count(npair, ncand, nact, counters);
 }
 
@@ -843,33 +794,62 @@ int launch_mixed_records(tsph_ctx *c) {
   return 0;
 }
 
-int ensure_tile_list(tsph_ctx *c, int groups) {
-  if (c->tiles_valid && (groups == 0 || groups == c->list_groups)) return 0;
-  const Consts &k = c->k;
-  const int rows = c->tile_rows;
-  const int ntiles = total_tiles(c);
-  const int nb = grid_for(ntiles, kListBlock);
-  if (nb > c->list_part_cap || groups > c->bound_cap) {
-    set_error("tile list: more tiles or pooled workgroups than allocated");
+int launch_cut(tsph_ctx *c, hipStream_t s, int groups, int table) {
+  if (groups > c->bound_cap) {
+    set_error("tile list: more pooled workgroups than allocated");
     return TSPH_ESTATE;
   }
-  hipStream_t s = c->stream;
-  hipLaunchKernelGGL(k_tile_count, dim3(nb), dim3(kListBlock), 0, s, c->cell_start, k, c->cases,
-                     c->tile_base, c->ncases, rows, ntiles, c->list_part);
-  hipLaunchKernelGGL(k_tile_emit, dim3(nb), dim3(kListBlock), 0, s, c->cell_start, k, c->cases,
-                     c->tile_base, c->ncases, rows, ntiles, c->list_part, groups, c->scal,
-                     c->tile_list, c->tile_bound);
+  if (c->list_cost_table != table) {
+    hipLaunchKernelGGL(k_list_costs, dim3(2 * c->num_cu), dim3(256), 0, s, c->cell_start, c->k,
+                       c->cases, c->tile_base, c->ncases, c->tile_rows, c->scal,
+                       c->scal + S_NSCAL + table, c->tile_list, c->list_cost,
+                       cost_estimate(c->prm.precision == TSPH_PREC_MIXED));
+    c->list_cost_table = table;
+  }
+  hipLaunchKernelGGL(k_cut_chunks, dim3(cut_segments(total_tiles(c))), dim3(kCutBlock), 0, s,
+                     c->scal, c->list_cost, groups, c->tile_bound);
   TSPH_HIP(hipGetLastError());
-  c->tiles_valid = true;
   c->list_groups = groups;
+  c->list_table = table;
   return 0;
 }
 
-TileGrid tile_grid(tsph_ctx *c, bool fp32, size_t lds_wave) {
+int ensure_tile_list(tsph_ctx *c, int groups, int table) {
+  const Consts &k = c->k;
+  const int rows = c->tile_rows;
+  hipStream_t s = c->stream;
+  if (!c->tiles_valid) {
+    const int ntiles = total_tiles(c);
+    const int nb = grid_for(ntiles, kListBlock);
+    if (nb > c->list_part_cap) {
+      set_error("tile list: more tiles than allocated");
+      return TSPH_ESTATE;
+    }
+    hipLaunchKernelGGL(k_tile_count, dim3(nb), dim3(kListBlock), 0, s, c->cell_start, k, c->cases,
+                       c->tile_base, c->ncases, rows, ntiles, c->list_part);
+    hipLaunchKernelGGL(k_tile_emit, dim3(nb), dim3(kListBlock), 0, s, c->cell_start, k, c->cases,
+                       c->tile_base, c->ncases, rows, ntiles, c->list_part, c->scal, c->tile_list);
+    TSPH_HIP(hipGetLastError());
+    c->tiles_valid = true;
+    c->list_groups = 0;
+    c->list_cost_table = -1;
+  }
+  if (groups > 0 && (groups != c->list_groups || table != c->list_table))
+    return launch_cut(c, s, groups, table);
+  return 0;
+}
+
+int clear_tile_costs(tsph_ctx *c) {
+  TSPH_HIP(hipMemsetAsync(c->scal + S_NSCAL, 0,
+                          sizeof(int) * kCostTables * ((size_t)c->ncells_tot + 8), c->stream));
+  return 0;
+}
+
+TileGrid tile_grid(tsph_ctx *c, bool fp32, size_t lds_wave, bool may_pool) {
   TileGrid tg;
   const int waves = c->num_cu * c->waves_per_cu;
   tg.lds_wave = (int)((lds_wave + 15) / 16 * 16);
-  tg.pooled = walk_balanced(c, fp32, waves);
+  tg.pooled = may_pool && walk_balanced(c, fp32, waves);
   int pw = 1;
   if (tg.pooled) {
     // the workgroup size (up to kPoolWaves wavefronts, within one workgroup's
@@ -889,7 +869,7 @@ TileGrid tile_grid(tsph_ctx *c, bool fp32, size_t lds_wave) {
   tg.groups = 8 * ((waves + 8 * pw - 1) / (8 * pw));
   tg.grid = dim3(tg.groups);
   tg.block = dim3(kWave * pw);
-  tg.lds = kPoolHdr + (size_t)pw * tg.lds_wave;
+  tg.lds = pool_hdr(tg.pooled) + (size_t)pw * tg.lds_wave;
   return tg;
 }
 
@@ -904,9 +884,8 @@ static size_t forces_lds_wave(const tsph_ctx *c, int variant) {
   return kTabBytes + list + (size_t)cap * (3 * sizeof(double2) + 2 + (bt ? sizeof(double2) : 0));
 }
 
-int pooled_groups(tsph_ctx *c) {
-  const TileGrid tg = tile_grid(c, c->prm.precision == TSPH_PREC_MIXED,
-                                forces_lds_wave(c, TSPH_FORCES_WCSPH));
+int pass_groups(tsph_ctx *c, int variant) {
+  const TileGrid tg = tile_grid(c, c->prm.precision == TSPH_PREC_MIXED, forces_lds_wave(c, variant));
   return tg.pooled ? tg.groups : 0;
 }
 
@@ -950,7 +929,7 @@ int launch_forces(tsph_ctx *c, int variant, bool corr, double corr_dt) {
     c->records_valid = true;
   }
   const TileGrid tg = tile_grid(c, mixed, forces_lds_wave(c, variant));
-  if (int e = ensure_tile_list(c, tg.pooled ? tg.groups : 0)) return e;
+  if (int e = ensure_tile_list(c, tg.pooled ? tg.groups : 0, mixed && corr ? 1 : 0)) return e;
   const int e = launch_timed(c, [&] {
     switch (variant) {
       case TSPH_FORCES_WCSPH: pair_launch<TSPH_FORCES_WCSPH>(c, tg, rows, cap, corr, corr_dt); break;

@@ -71,15 +71,20 @@ __device__ __forceinline__ int sel3(const int (&a)[3], int q) {
 
 // Each wavefront of a tile kernel works on its own tile with its own LDS area;
 // a workgroup holds one wavefront (static walk) or up to kPoolWaves of them
-// (pooled walk), which share nothing but the pool counter.  LDS layout:
-// kPoolHdr bytes (the counter), then one area of lds_wave bytes per wavefront.
+// (pooled walk), which share nothing but the pool's counter and tile order.
+// LDS layout: pool_hdr(DYN) bytes (the pooled walk's counter, then kPoolSort
+// tile positions), then one area of lds_wave bytes per wavefront.
 constexpr int kPoolWaves = 4;
-constexpr int kPoolHdr = 16;
+constexpr int kPoolSort = 64;  // chunks of at most this many tiles go in decreasing cost
+constexpr int kPoolHdr = 16 + 4 * kPoolSort;
+constexpr int kStaticHdr = 16;
+constexpr int pool_hdr(bool pooled) { return pooled ? kPoolHdr : kStaticHdr; }
 
 __device__ __forceinline__ int tile_lane() { return threadIdx.x & (kWave - 1); }
 
+template <bool DYN>
 __device__ __forceinline__ unsigned char *wave_lds(unsigned char *smem, int lds_wave) {
-  return smem + kPoolHdr + (threadIdx.x >> 6) * lds_wave;
+  return smem + pool_hdr(DYN) + (threadIdx.x >> 6) * lds_wave;
 }
 
 // Hand-over between the lanes of one wavefront through LDS (staged records,
@@ -97,43 +102,101 @@ __device__ __forceinline__ void wave_sync() {
 //     tiles that are neighbours in the list, which share slices, are staged
 //     from the same L2;
 //   * pooled (DYN = true): the list is cut into one chunk per workgroup of
-//     about equal work (own particles plus a per-tile overhead, chunk bounds
-//     written by the list build), the chunks of XCD x again contiguous; the
-//     workgroup's wavefronts take the tiles of their chunk in order from a
-//     counter in LDS.  Waves of one compute unit thereby balance each other
-//     without any device-wide atomic.
+//     about equal cost, by the times the tiles took in the previous pass
+//     (chunk bounds written by k_cut_chunks), the chunks of XCD x again
+//     contiguous; the workgroup's wavefronts take the tiles of their chunk
+//     from a counter in LDS, the costliest first (a chunk of up to kPoolSort
+//     tiles is sorted by recorded cost when the workgroup starts), so that
+//     the tiles left for the end are the cheap ones.  Waves of one compute
+//     unit thereby balance each other, and the compute units each other
+//     through the costs, without any device-wide atomic (those serialise
+//     across the XCDs).
+// Either walk records each tile's time (tq_mark) for the next pooled cut.
 struct TileQueue {
   int ntl;   // static: tiles in the list
   int band;  // static: band of the wave
   int nwb;   // static: waves per band
   int wb;    // static: this wave's rank in its band
   int j;     // static: round of the current tile
-  int hi;    // pooled: end of the chunk
-  int *ctr;  // pooled: the chunk's next tile (LDS)
+  int lo, hi;   // pooled: the chunk
+  int *ctr;     // pooled: the chunk's next tile (LDS)
+  int *order;   // pooled, sorted chunk: list positions in decreasing cost (LDS)
+  bool sorted;  // pooled: the chunk is taken in the order of `order`
+  int *cost; // per-tile costs of the wave's table (tile t at cost[kCostTables t])
+  int cur;   // global tile being timed (-1: none)
+  int old;   // its recorded cost (lane 0)
+  unsigned long long t0;  // its start
 };
 
 __device__ __forceinline__ int band_lo(int ntl, int b) { return (int)(((long long)ntl * b) >> 3); }
 
 template <bool DYN>
-__device__ __forceinline__ TileQueue tq_begin(const int *__restrict__ scal, const int *__restrict__ bound,
-                                              unsigned char *smem) {
+__device__ __forceinline__ TileQueue tq_begin(int *__restrict__ scal, const int *__restrict__ bound,
+                                              const int *__restrict__ tile_list,
+                                              unsigned char *smem, int table = 0) {
   TileQueue tq;
   tq.ntl = 0;
   tq.band = blockIdx.x & 7;
   tq.nwb = gridDim.x >> 3;
   tq.wb = blockIdx.x >> 3;
   tq.j = -1;
-  tq.hi = 0;
+  tq.lo = tq.hi = 0;
   tq.ctr = reinterpret_cast<int *>(smem);
+  tq.order = reinterpret_cast<int *>(smem + 16);
+  tq.sorted = false;
+  tq.cost = scal + S_NSCAL + table;
+  tq.cur = -1;
+  tq.old = 0;
+  tq.t0 = 0;
   if (!DYN) {
     tq.ntl = scal[S_NTILES];
   } else {
     const int chunk = tq.band * tq.nwb + tq.wb;
+    tq.lo = bound[chunk];
     tq.hi = bound[chunk + 1];
-    if (threadIdx.x == 0) *tq.ctr = bound[chunk];
+    tq.sorted = tq.hi - tq.lo <= kPoolSort;
+    if (threadIdx.x == 0) *tq.ctr = tq.lo;
+    if (tq.sorted && threadIdx.x < kWave) {
+      // bitonic sort of (cost, position) over the first wavefront's lanes,
+      // decreasing cost, ties in list order; unused lanes sort last
+      const int l = threadIdx.x;
+      int key = -1, idx = l;
+      if (tq.lo + l < tq.hi) key = tq.cost[kCostTables * tile_list[tq.lo + l]];
+      for (int k = 2; k <= kWave; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+          const int ok = __shfl_xor(key, j, kWave), oi = __shfl_xor(idx, j, kWave);
+          const bool first = ok > key || (ok == key && oi < idx);  // the other ranks first
+          const bool keep_first = ((l & k) == 0) == ((l & j) == 0);
+          if (keep_first == first) {
+            key = ok;
+            idx = oi;
+          }
+        }
+      tq.order[l] = tq.lo + idx;
+    }
     __syncthreads();
   }
   return tq;
+}
+
+// Records the wave's current tile's cost (the time since it began, in 10 ns
+// ticks of the device's constant clock, averaged with the tile's recorded
+// cost if it has one) and begins global tile `tile` (-1: none, after the
+// last tile).
+__device__ __forceinline__ void tq_mark(TileQueue &tq, int tile) {
+  const unsigned long long now = wall_clock64();
+  if (tile_lane() == 0) {
+    if (tq.cur >= 0) {
+      const unsigned long long u = now - tq.t0;
+      int w = u < 1 ? 1 : (u > (1u << 24) ? (1 << 24) : (int)u);
+      if (tq.old > 0) w = (tq.old + w + 1) >> 1;
+      tq.cost[kCostTables * tq.cur] = w;
+    }
+    // the new tile's record, read now and used when it ends
+    tq.old = tile >= 0 ? tq.cost[kCostTables * tile] : 0;
+  }
+  tq.cur = tile;
+  tq.t0 = now;
 }
 
 // pooled walk: lane 0 takes the next tile of the chunk (read with tq_take)
@@ -152,7 +215,8 @@ __device__ __forceinline__ int tq_take(TileQueue &tq, int raw) {
     return idx < band_lo(tq.ntl, tq.band + 1) ? idx : -1;
   }
   const int v = __builtin_amdgcn_readfirstlane(raw);
-  return v < tq.hi ? v : -1;
+  if (v >= tq.hi) return -1;
+  return tq.sorted ? tq.order[v - tq.lo] : v;
 }
 
 // Loads the cell offsets of the three slices (one global load per lane, all in
@@ -197,6 +261,117 @@ __device__ __forceinline__ int case_of_tile(const int *__restrict__ tile_base, i
   return lo;
 }
 
+// own particles of global tile t (its `rows` cells of one column)
+__device__ __forceinline__ int tile_own(const int *__restrict__ cell_start, const Consts &k0,
+                                        const Consts *__restrict__ cases,
+                                        const int *__restrict__ tile_base, int ncases, int rows,
+                                        int t) {
+  int lt = t, cell_base = k0.cell_base, ncy = k0.ncy;
+  if (cases) {
+    const int cse = case_of_tile(tile_base, ncases, t);
+    lt = t - tile_base[cse];
+    cell_base = cases[cse].cell_base;
+    ncy = cases[cse].ncy;
+  }
+  const int *cs = cell_start + cell_base;
+  const int nty = (ncy + rows - 1) / rows;
+  const int cx = lt / nty, cy0 = (lt - cx * nty) * rows, cy1 = min(cy0 + rows, ncy);
+  return cs[cx * ncy + cy1] - cs[cx * ncy + cy0];
+}
+
+// exclusive scan of a over the block (blockDim.x threads, a multiple of 64);
+// total: the block's sum.  lds: one slot per wavefront.
+template <typename T>
+__device__ __forceinline__ T block_exscan(T a, T &total, T *lds) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  T x = a;
+  for (int o = 1; o < 64; o <<= 1) {
+    const T y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  __syncthreads();
+  if (lane == 63) lds[wid] = x;
+  __syncthreads();
+  T before = 0;
+  total = 0;
+  for (int w = 0; w < nw; ++w) {
+    const T v = lds[w];
+    if (w < wid) before += v;
+    total += v;
+  }
+  return before + x - a;
+}
+
+// Chunk starts of a pooled walk of `groups` workgroups over the listed tiles,
+// cut by cost: listed tile t has the cost interval [W0, W0 + w) (W0: the costs
+// of the tiles before it), and chunk g starts at the tile whose interval holds
+// g W / groups (W: the total), so that every chunk holds about W / groups.
+// The costs come by list position (list_cost, written with the list).  The
+// list is cut in segments of kCutSeg entries, one workgroup of kCutBlock
+// threads each (cut_segments: workgroups beyond the list exit at once), which
+// share nothing: each reads the whole cost list, consecutive threads on
+// consecutive entries and kCutBatch loads of each thread in flight together,
+// to form the total and the costs of the segments before its own, and keeps
+// its own segment in LDS; each thread then scans a contiguous run of it and
+// finds the chunk starts there by comparing products (g W against (W0 + w)
+// groups) rather than by divisions.
+constexpr int kCutBlock = 1024, kCutBatch = 16, kCutSeg = kCutBlock * kCutBatch;
+
+// workgroups that cut a list of at most `tiles` entries
+inline int cut_segments(int tiles) { return tiles / kCutSeg + 1; }
+
+// the cut of segment `seg` by one workgroup of kCutBlock threads; wl: kCutSeg
+// ints, red: kCutBlock / 64 long longs of LDS
+__device__ __forceinline__ void cut_chunks_block(const int *__restrict__ scal,
+                                                 const int *__restrict__ list_cost, int groups,
+                                                 int *__restrict__ bound, int seg, int *wl,
+                                                 long long *red) {
+  const int n = scal[S_NTILES];
+  const int s0 = seg * kCutSeg;
+  if (seg == 0) {
+    if (threadIdx.x == 0) bound[groups] = n;
+    if (n == 0)
+      for (int g = threadIdx.x; g < groups; g += kCutBlock) bound[g] = 0;
+  }
+  if (s0 >= n) return;
+  // the total and the costs before this segment; this segment's costs to LDS
+  long long all = 0, before = 0;
+  for (int b0 = 0; b0 < n; b0 += kCutSeg) {
+    int w[kCutBatch];
+#pragma unroll
+    for (int u = 0; u < kCutBatch; ++u) {
+      const int e = b0 + u * kCutBlock + (int)threadIdx.x;
+      w[u] = e < n ? list_cost[e] : 0;
+    }
+    long long part = 0;
+#pragma unroll
+    for (int u = 0; u < kCutBatch; ++u) part += w[u];
+    all += part;
+    if (b0 < s0) before += part;
+    if (b0 == s0) {
+#pragma unroll
+      for (int u = 0; u < kCutBatch; ++u) wl[u * kCutBlock + threadIdx.x] = w[u];
+    }
+  }
+  long long total = 0, carry = 0;
+  block_exscan(all, total, red);
+  block_exscan(before, carry, red);
+  const int m = min(kCutSeg, n - s0);
+  const int per = (m + kCutBlock - 1) / kCutBlock;
+  const int lo = min((int)threadIdx.x * per, m), hi = min(lo + per, m);
+  long long r = 0;
+  for (int e = lo; e < hi; ++e) r += wl[e];
+  long long segsum = 0;
+  long long p = carry + block_exscan(r, segsum, red);
+  // first chunk whose start g W / groups >= p
+  long long g = (p * groups + total - 1) / total;
+  for (int e = lo; e < hi; ++e) {
+    p += wl[e];
+    const long long end = p * groups;  // chunk g starts in this tile while g W < end
+    for (; g < groups && g * total < end; ++g) bound[g] = s0 + e;
+  }
+}
+
 __device__ __forceinline__ int slice_of(const Tile &t, int l) {
   return l >= t.lq[2] ? 2 : (l >= t.lq[1] ? 1 : 0);
 }
@@ -222,19 +397,20 @@ __device__ __forceinline__ void walk64(const P &pol, const Consts &k, const int 
                                        const int *__restrict__ bound, int cap, int lds_wave,
                                        unsigned long long *__restrict__ counters) {
   extern __shared__ __align__(16) unsigned char smem[];
-  unsigned char *wl = wave_lds(smem, lds_wave);
+  unsigned char *wl = wave_lds<DYN>(smem, lds_wave);
   int *tab = reinterpret_cast<int *>(wl);
   uint16_t *s_list = reinterpret_cast<uint16_t *>(wl + kTabBytes);
   const typename P::Lds L = P::lds(wl + kTabBytes + sizeof(uint16_t) * kList * kWave, cap);
   const int lane = tile_lane();
   if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&counters[C_PASSES], 1ull);
   unsigned npair = 0, ncand = 0, nact = 0;
-  TileQueue tq = tq_begin<DYN>(scal, bound, smem);
+  TileQueue tq = tq_begin<DYN>(scal, bound, tile_list, smem);
   for (int it = tq_take<DYN>(tq, tq_ticket<DYN>(tq)); it >= 0;) {
     wave_sync();
     const int ticket = tq_ticket<DYN>(tq);  // read once this tile's loads are waited for
     Tile t;
     tile_begin(k, cell_start, tile_list[it], rows, tab, t);
+    tq_mark(tq, tile_list[it]);
     const bool staged = t.lq[3] <= cap;
     if (staged)
       for (int l = lane; l < t.lq[3]; l += kWave) pol.stage(L, l, l - sel3(t.base, slice_of(t, l)));
@@ -324,6 +500,7 @@ __device__ __forceinline__ void walk64(const P &pol, const Consts &k, const int 
     }
     it = next;
   }
+  tq_mark(tq, -1);
   wave_count(npair, ncand, nact, counters);
 }
 
@@ -335,7 +512,7 @@ __device__ __forceinline__ void walk32(const P &pol, const Consts &k, const int 
                                        unsigned long long *__restrict__ counters) {
 #pragma clang fp contract(fast)
   extern __shared__ __align__(16) unsigned char smem[];
-  unsigned char *wl = wave_lds(smem, lds_wave);
+  unsigned char *wl = wave_lds<DYN>(smem, lds_wave);
   int *tab = reinterpret_cast<int *>(wl);
   uint16_t *s_list = reinterpret_cast<uint16_t *>(wl + kTabBytes);
   const typename P::Lds L = P::lds(wl + kTabBytes + sizeof(uint16_t) * kList * kWave, cap);
@@ -343,12 +520,13 @@ __device__ __forceinline__ void walk32(const P &pol, const Consts &k, const int 
   const float r2cut = 4.f * (float)k.h * (float)k.h;
   if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&counters[C_PASSES], 1ull);
   unsigned npair = 0, ncand = 0, nact = 0;
-  TileQueue tq = tq_begin<DYN>(scal, bound, smem);
+  TileQueue tq = tq_begin<DYN>(scal, bound, tile_list, smem);
   for (int it = tq_take<DYN>(tq, tq_ticket<DYN>(tq)); it >= 0;) {
     wave_sync();
     const int ticket = tq_ticket<DYN>(tq);  // read once this tile's loads are waited for
     Tile t;
     tile_begin(k, cell_start, tile_list[it], rows, tab, t);
+    tq_mark(tq, tile_list[it]);
     const bool staged = t.lq[3] <= cap;
     const int mid = t.row_lo + (t.nrows >> 1);
     const double ox = k.x0 - k.of + t.cx * k.kh, oy = k.y0 - k.of + mid * k.kh;
@@ -438,6 +616,7 @@ __device__ __forceinline__ void walk32(const P &pol, const Consts &k, const int 
     }
     it = next;
   }
+  tq_mark(tq, -1);
   wave_count(npair, ncand, nact, counters);
 }
 
