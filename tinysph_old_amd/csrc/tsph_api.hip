@@ -59,7 +59,8 @@ static void free_soa(Soa &s) {
 }
 
 static void drop_graph(tsph_ctx *c) {
-  for (Graph *g : {&c->graph[0], &c->graph[1], &c->rale_graph[0], &c->rale_graph[1],
+  for (Graph *g : {&c->graph[0], &c->graph[1], &c->graph[2], &c->graph[3], &c->rale_graph[0],
+                   &c->rale_graph[1],
                    &c->ale_graph[0], &c->ale_graph[1], &c->inflow_graph[0], &c->inflow_graph[1],
                    &c->verlet_graph[0], &c->verlet_graph[1], &c->euler_graph[0],
                    &c->euler_graph[1], &c->lf_graph[0], &c->lf_graph[1]}) {
@@ -108,13 +109,17 @@ static bool step_reads_olds(const Graph *slots, const tsph_ctx *c) {
   return slots != c->graph && slots != c->inflow_graph;
 }
 
+// group > 1: slots has 4 entries, and runs of `group` steps replay one graph
+// of that many steps (slots[2 + parity]), the rest single-step graphs
 template <typename R>
-static int replay_step(tsph_ctx *c, Graph *slots, int tag, double dt, int nsteps, R record) {
+static int replay_step(tsph_ctx *c, Graph *slots, int tag, double dt, int nsteps, R record,
+                       int group = 1) {
   if (step_reads_olds(slots, c))
     if (int e = ensure_olds(c)) return e;
-  for (int s = 0; s < nsteps; ++s) {
-    Graph &g = slots[c->parity];
-    if (!g.exec || g.wall != tag || g.dt != dt || g.n_ub != c->n_ub) {
+  for (int s = 0; s < nsteps;) {
+    const int k = group > 1 && nsteps - s >= group ? group : 1;
+    Graph &g = slots[(k > 1 ? 2 : 0) + c->parity];
+    if (!g.exec || g.wall != tag || g.dt != dt || g.n_ub != c->n_ub || g.steps != k) {
       if (g.exec) (void)hipGraphExecDestroy(g.exec);
       g = Graph{};
       const Soa cur0 = c->cur, alt0 = c->alt;
@@ -133,7 +138,8 @@ static int replay_step(tsph_ctx *c, Graph *slots, int tag, double dt, int nsteps
       };
       hipGraph_t graph = nullptr;
       TSPH_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-      const int e = record();
+      int e = 0;
+      for (int j = 0; j < k && !e; ++j) e = record();
       const hipError_t ce = hipStreamEndCapture(c->stream, &graph);
       if (e || ce != hipSuccess) {
         if (graph) (void)hipGraphDestroy(graph);
@@ -153,6 +159,7 @@ static int replay_step(tsph_ctx *c, Graph *slots, int tag, double dt, int nsteps
       g.n_ub = c->n_ub;
       g.flips = c->parity != parity0;
       g.post = host_state(c);
+      g.steps = k;
     } else {
       if (g.flips) {  // what an odd number of cell builds leaves (launch_cell_list)
         std::swap(c->cur, c->alt);
@@ -163,6 +170,7 @@ static int replay_step(tsph_ctx *c, Graph *slots, int tag, double dt, int nsteps
       set_host_state(c, g.post);
     }
     TSPH_HIP(hipGraphLaunch(g.exec, c->stream));
+    s += k;
   }
   return 0;
 }
@@ -231,26 +239,14 @@ static int record_step(tsph_ctx *c, int wall, double dt, int inlet = -1, int out
   // cases/damBreakLong_BT/SPH_simulationSymplectic.h:72-73); DBC has none.
   // Without buffer zones nothing reads the pressures (or the mixed records)
   // this EOS writes before the cell build rewrites them all, so only its
-  // out-of-range density count is taken, by the wall-density pass.
-  // Without buffer zones the wall densities run on the side stream while the
-  // first cell build sorts: the pass reads the previous cell list and writes
-  // wall densities, the sort reads positions only and writes the other
-  // cell-offset buffer and wall-count slot; the build's reorder (which reads
-  // the densities) waits for it.
-  hipEvent_t walls_done = nullptr;
+  // out-of-range density count is taken, by the wall-density pass.  The pass
+  // runs on the step's own stream: on a second stream beside the first
+  // build's hash and scan, the fork and the join in the step graph left about
+  // 25 us of idle time per step, more than the pass takes (18 us on the
+  // 740k-particle dam break).
   if (wall == TSPH_WALL_MDBC || wall == TSPH_WALL_BT) {
-    hipStream_t ws = nullptr;
-    if (!zones) {
-      TSPH_HIP(hipEventRecord(c->fork_ev, c->stream));
-      TSPH_HIP(hipStreamWaitEvent(c->side, c->fork_ev, 0));
-      ws = c->side;
-    }
-    if ((e = wall == TSPH_WALL_MDBC ? launch_mdbc(c, !zones, ws) : launch_bt_density(c, !zones, ws)))
+    if ((e = wall == TSPH_WALL_MDBC ? launch_mdbc(c, !zones) : launch_bt_density(c, !zones)))
       return e;
-    if (!zones) {
-      TSPH_HIP(hipEventRecord(c->join_ev, c->side));
-      walls_done = c->join_ev;
-    }
     if (zones && (e = launch_eos(c))) return e;
   }
   // each build cuts the chunks of the pooled pair pass that follows it
@@ -264,7 +260,7 @@ static int record_step(tsph_ctx *c, int wall, double dt, int inlet = -1, int out
     if (inlet >= 0 && (e = launch_bz_inlet(c, inlet, dt))) return e;
     if (outlet >= 0 && (e = launch_bz_outlet(c, outlet, dt))) return e;
     if ((e = launch_cell_list(c, true, false, 0., active))) return e;
-  } else if ((e = launch_cell_list(c, true, true, dt, active, false, walls_done))) {
+  } else if ((e = launch_cell_list(c, true, true, dt, active))) {
     return e;  // (the predictor is applied by the build)
   }
   // the corrector writes the removal states; the build takes the removal
@@ -279,7 +275,7 @@ static int record_step(tsph_ctx *c, int wall, double dt, int inlet = -1, int out
   // rewrites them: this build leaves them behind
   c->cut_groups = groups;
   c->cut_table = 0;
-  if ((e = launch_cell_list(c, true, false, 0., active, true, nullptr, true)) ||
+  if ((e = launch_cell_list(c, true, false, 0., active, true, true)) ||
       (e = launch_forces(c, variant)))
     return e;
   return 0;
@@ -363,11 +359,8 @@ static int make_consts(const tsph_params *prm, Consts &k) {
 // device buffers of a context holding `cap` particles over `ncells_tot` cells
 static int alloc_context(tsph_ctx *c) {
   int e = 0;
-  // both streams at the default priority
-  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-      hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
-      hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&c->join_ev, hipEventDisableTiming) != hipSuccess)
+  // the stream at the default priority
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
     return fail(TSPH_EHIP, "hipStreamCreate failed");
   const int cap = c->capacity, nct = c->ncells_tot;
   c->seq_cap = 2 * cap + kBzMax + c->ncases;
@@ -485,9 +478,6 @@ int tsph_destroy(tsph_ctx *c) {
   for (void *p : ps)
     if (p) (void)hipFree(p);
   for (hipEvent_t ev : c->tev) (void)hipEventDestroy(ev);
-  if (c->fork_ev) (void)hipEventDestroy(c->fork_ev);
-  if (c->join_ev) (void)hipEventDestroy(c->join_ev);
-  if (c->side) (void)hipStreamDestroy(c->side);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return 0;
@@ -825,7 +815,8 @@ int tsph_step_symplectic(tsph_ctx *c, int wall, double dt, int nsteps) {
       if (int e = record_step(c, wall, dt)) return e;
     return 0;
   }
-  return replay_step(c, c->graph, wall, dt, nsteps, [&] { return record_step(c, wall, dt); });
+  return replay_step(c, c->graph, wall, dt, nsteps, [&] { return record_step(c, wall, dt); },
+                     kGraphSteps);
 }
 
 int tsph_ale_enable(tsph_ctx *c) {
@@ -1719,7 +1710,8 @@ int tsph_batch_step(tsph_ctx *c, int wall, int nsteps) {
       if (int e = record_step(c, wall, 0.)) return e;
     return 0;
   }
-  return replay_step(c, c->graph, 400 + wall, 0., nsteps, [&] { return record_step(c, wall, 0.); });
+  return replay_step(c, c->graph, 400 + wall, 0., nsteps, [&] { return record_step(c, wall, 0.); },
+                     kGraphSteps);
 }
 
 // ---- autotuner -----------------------------------------------------------------

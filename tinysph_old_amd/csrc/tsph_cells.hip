@@ -724,7 +724,7 @@ int ensure_olds(tsph_ctx *c) {
 }
 
 int launch_cell_list(tsph_ctx *c, bool eos, bool predict, double dt, unsigned ad_skip,
-                     bool decide, hipEvent_t before_reorder, bool skip_olds) {
+                     bool decide, bool skip_olds) {
   // a build that moves r_o, v_o, rho_o needs them current (the predictor's
   // build rewrites them instead)
   if (!predict && !skip_olds)
@@ -783,7 +783,6 @@ int launch_cell_list(tsph_ctx *c, bool eos, bool predict, double dt, unsigned ad
       c->list_table = c->cut_table;
     }
   }
-  if (before_reorder) TSPH_HIP(hipStreamWaitEvent(s, before_reorder, 0));
   const Mixed mx = c->prm.precision == TSPH_PREC_MIXED ? c->mx : Mixed{};
 #define TSPH_REORDER(E, P)                                                                    \
   hipLaunchKernelGGL((k_reorder<E, P>), dim3(g), dim3(nb), 0, s, cs, nct, c->perm,                 \

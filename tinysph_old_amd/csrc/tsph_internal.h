@@ -139,7 +139,11 @@ struct Graph {
   int n_ub = -1;
   bool flips = false;  // the step swaps cur and alt an odd number of times
   HostState post{};    // host flags after the step
+  int steps = 1;       // steps captured in the graph
 };
+// steps captured in one graph of a run of whole symplectic steps: each graph
+// launch costs about 8 us of idle time between the steps it separates
+constexpr int kGraphSteps = 8;
 
 }  // namespace tsph
 
@@ -208,8 +212,6 @@ struct tsph_ctx {
   // list can run while the next build sorts (record_step).
   int *cell_start_next = nullptr;      // ncells+2
   int nwall_idx = 3;                   // S_NWALL or S_NWALL_B: the current wall count
-  hipStream_t side = nullptr;          // second stream of a step graph's parallel branch
-  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
   int *cellof = nullptr;               // cell of each slot (sorted order)
   int *hrank = nullptr;                // rank inside the cell from the hash pass
   int *hcell = nullptr;                // cell from the hash pass (pre-sort order)
@@ -226,7 +228,9 @@ struct tsph_ctx {
   int tev_used = 0;
   float last_pair_ms = 0.f;
   int parity = 0;             // flips whenever cur and alt are swapped
-  tsph::Graph graph[2];       // symplectic step, by parity at the step's start
+  // symplectic step, by parity at the step's start: one step ([0], [1]) or
+  // kGraphSteps steps ([2], [3])
+  tsph::Graph graph[4];
   tsph::Graph rale_graph[2];  // Riemann-ALE step, by parity at the step's start
   tsph::Graph ale_graph[2];   // ALE-SPH MUSCL step, by parity at the step's start
   tsph::Graph inflow_graph[2];  // symplectic step with buffer zones
@@ -295,13 +299,10 @@ struct TileGrid {
 // decide: RemoveParticlesOutOfDomain's decision is taken by the build, in
 // place of a launch_remove immediately before (the removal states must be
 // current: a corrector precedes)
-// before_reorder: an event the reorder waits for (work on another stream
-// that reads the previous cell list and the particle state the reorder rewrites)
 // skip_olds: r_o, v_o, rho_o are not moved (olds_stale afterwards), for a build
 // after which they are dead until the next predictor rewrites them
 int launch_cell_list(tsph_ctx *c, bool eos = false, bool predict = false, double dt = 0.,
-                     unsigned ad_skip = 0, bool decide = false, hipEvent_t before_reorder = nullptr,
-                     bool skip_olds = false);
+                     unsigned ad_skip = 0, bool decide = false, bool skip_olds = false);
 // r_o, v_o, rho_o of the current generation, gathered from the previous one
 // through the last build's permutation if that build left them behind; every
 // launch that reads or moves them calls this first
@@ -335,8 +336,8 @@ int launch_predictor(tsph_ctx *c, double dt);
 int launch_corrector(tsph_ctx *c, double dt);
 // count_density: also take the out-of-range density count of an EOS pass
 // right after (in place of that pass, where nothing reads its pressures)
-int launch_mdbc(tsph_ctx *c, bool count_density = false, hipStream_t s = nullptr);
-int launch_bt_density(tsph_ctx *c, bool count_density = false, hipStream_t s = nullptr);
+int launch_mdbc(tsph_ctx *c, bool count_density = false);
+int launch_bt_density(tsph_ctx *c, bool count_density = false);
 int launch_remove(tsph_ctx *c);
 int run_water_elevation(tsph_ctx *c, double x, double y0, int inverse, double *out);
 // soa: out holds rho[n], p[n], then v[2n] (n = ncx ncy); else 4 values per point

@@ -540,9 +540,9 @@ int launch_corrector(tsph_ctx *c, double dt) {
   return 0;
 }
 
-int launch_mdbc(tsph_ctx *c, bool count_density, hipStream_t s) {
+int launch_mdbc(tsph_ctx *c, bool count_density) {
   const int wb = grid_for(c->n_wall, kWallsPerWave), sb = count_density ? sweep_blocks(c->n_ub) : 0;
-  hipLaunchKernelGGL(k_mdbc, dim3(wb + sb), dim3(64), 0, s ? s : c->stream, c->cur, c->cell_start,
+  hipLaunchKernelGGL(k_mdbc, dim3(wb + sb), dim3(64), 0, c->stream, c->cur, c->cell_start,
                      c->scal + c->nwall_idx, c->wall_list, c->k, c->cases,
                      count_density ? c->counters : nullptr, c->scal, wb);
   TSPH_HIP(hipGetLastError());
@@ -550,9 +550,9 @@ int launch_mdbc(tsph_ctx *c, bool count_density, hipStream_t s) {
   return 0;
 }
 
-int launch_bt_density(tsph_ctx *c, bool count_density, hipStream_t s) {
+int launch_bt_density(tsph_ctx *c, bool count_density) {
   const int wb = grid_for(c->n_wall, kWallsPerWave), sb = count_density ? sweep_blocks(c->n_ub) : 0;
-  hipLaunchKernelGGL(k_bt_density, dim3(wb + sb), dim3(64), 0, s ? s : c->stream, c->cur,
+  hipLaunchKernelGGL(k_bt_density, dim3(wb + sb), dim3(64), 0, c->stream, c->cur,
                      c->cell_start, c->scal + c->nwall_idx, c->wall_list, c->k, c->cases,
                      count_density ? c->counters : nullptr, c->scal, wb);
   TSPH_HIP(hipGetLastError());
