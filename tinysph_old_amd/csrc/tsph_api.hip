@@ -239,15 +239,14 @@ static int record_step(tsph_ctx *c, int wall, double dt, int inlet = -1, int out
   // cases/damBreakLong_BT/SPH_simulationSymplectic.h:72-73); DBC has none.
   // Without buffer zones nothing reads the pressures (or the mixed records)
   // this EOS writes before the cell build rewrites them all, so only its
-  // out-of-range density count is taken, by the wall-density pass.  The pass
-  // runs on the step's own stream: on a second stream beside the first
-  // build's hash and scan, the fork and the join in the step graph left about
-  // 25 us of idle time per step, more than the pass takes (18 us on the
-  // 740k-particle dam break).
-  if (wall == TSPH_WALL_MDBC || wall == TSPH_WALL_BT) {
-    if ((e = wall == TSPH_WALL_MDBC ? launch_mdbc(c, !zones) : launch_bt_density(c, !zones)))
-      return e;
-    if (zones && (e = launch_eos(c))) return e;
+  // out-of-range density count is taken, by the wall-density pass.  Without
+  // buffer zones the pass runs in workgroups of the first build's hash launch,
+  // beside the hash (on a second stream the fork and the join in the step
+  // graph left about 25 us of idle time per step, more than the pass takes).
+  const bool walls_in_build = !zones && (wall == TSPH_WALL_MDBC || wall == TSPH_WALL_BT);
+  if (zones && (wall == TSPH_WALL_MDBC || wall == TSPH_WALL_BT)) {
+    if ((e = wall == TSPH_WALL_MDBC ? launch_mdbc(c) : launch_bt_density(c))) return e;
+    if ((e = launch_eos(c))) return e;
   }
   // each build cuts the chunks of the pooled pair pass that follows it
   const bool mixed = c->prm.precision == TSPH_PREC_MIXED;
@@ -260,7 +259,8 @@ static int record_step(tsph_ctx *c, int wall, double dt, int inlet = -1, int out
     if (inlet >= 0 && (e = launch_bz_inlet(c, inlet, dt))) return e;
     if (outlet >= 0 && (e = launch_bz_outlet(c, outlet, dt))) return e;
     if ((e = launch_cell_list(c, true, false, 0., active))) return e;
-  } else if ((e = launch_cell_list(c, true, true, dt, active))) {
+  } else if ((e = launch_cell_list(c, true, true, dt, active, false, false,
+                                   walls_in_build ? wall : 0, true))) {
     return e;  // (the predictor is applied by the build)
   }
   // the corrector writes the removal states; the build takes the removal

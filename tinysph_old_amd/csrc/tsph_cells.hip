@@ -27,6 +27,7 @@
 #include "tsph_internal.h"
 #include "tsph_physics.h"
 #include "tsph_tiles.h"
+#include "tsph_walls.h"
 
 namespace tsph {
 
@@ -54,16 +55,42 @@ __device__ __forceinline__ unsigned long long lanemask_lt() {
 // from the removal states of the corrector that precedes the build.  The
 // reference positions of removed particles are marked absent by k_scatter, so
 // no decision in this pass reads a state that this pass changes.
-template <bool PRED, bool DECIDE>
+// The wall-density pass that precedes a build in the symplectic step (WALLS:
+// TSPH_WALL_MDBC or TSPH_WALL_BT; 0: none) runs in the first wb.walls +
+// wb.sweep workgroups of the hash launch (64 threads each), beside the hash
+// rather than before it: it reads the previous cell list and writes wall
+// densities, the hash reads positions and writes the new cell counts.
+struct WallPass {
+  Soa d;
+  const int *cell_start, *nwall, *wall_list;
+  int walls, sweep;  // wall and sweep workgroups
+  unsigned long long *counters;  // the density count (or nullptr)
+};
+
+template <bool PRED, bool DECIDE, int WALLS>
 __global__ void k_hash(const double2 *__restrict__ r, const double2 *__restrict__ v,
                        uint8_t *__restrict__ type,
                        const uint32_t *__restrict__ seq, const int *__restrict__ scal, Consts k0,
                        const Consts *__restrict__ cases, const uint8_t *__restrict__ cid, double dt0,
                        int ncells_tot, int *__restrict__ cell_count, int *__restrict__ hcell,
                        int *__restrict__ hrank, uint8_t *__restrict__ seqstate,
-                       unsigned long long *__restrict__ counters) {
+                       unsigned long long *__restrict__ counters, WallPass wp) {
+  int hb = blockIdx.x;  // hash workgroup
+  if (WALLS) {
+    const int b = blockIdx.x;
+    if (b < wp.walls + wp.sweep) {
+      if (WALLS == TSPH_WALL_MDBC)
+        walls::mdbc_block(b, wp.walls, wp.sweep, wp.d, wp.cell_start, wp.nwall, wp.wall_list, k0,
+                          cases, wp.counters, scal);
+      else
+        walls::bt_block(b, wp.walls, wp.sweep, wp.d, wp.cell_start, wp.nwall, wp.wall_list, k0,
+                        cases, wp.counters, scal);
+      return;
+    }
+    hb = b - wp.walls - wp.sweep;
+  }
   const int n = scal[S_NP];
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = hb * blockDim.x + threadIdx.x;
   const bool live = i < n;
   int c = -1;
   if (live) {
@@ -724,7 +751,7 @@ int ensure_olds(tsph_ctx *c) {
 }
 
 int launch_cell_list(tsph_ctx *c, bool eos, bool predict, double dt, unsigned ad_skip,
-                     bool decide, bool skip_olds) {
+                     bool decide, bool skip_olds, int walls, bool count_density) {
   // a build that moves r_o, v_o, rho_o needs them current (the predictor's
   // build rewrites them instead)
   if (!predict && !skip_olds)
@@ -738,14 +765,34 @@ int launch_cell_list(tsph_ctx *c, bool eos, bool predict, double dt, unsigned ad
   // cell list does not use (swapped in at the end)
   int *cs = c->cell_start_next;
   const int wslot = c->nwall_idx == S_NWALL ? S_NWALL_B : S_NWALL;
-#define TSPH_HASH(P, D)                                                                        \
-  hipLaunchKernelGGL((k_hash<P, D>), dim3(g), dim3(nb), 0, s, c->cur.r, c->cur.v, c->cur.type,  \
-                     c->cur.seq, c->scal, k, c->cases, c->cur.cid, dt, nct, c->cell_count,      \
-                     c->hcell, c->hrank, c->seqstate, c->counters)
-  if (predict && decide) TSPH_HASH(true, true);
-  else if (predict) TSPH_HASH(true, false);
-  else if (decide) TSPH_HASH(false, true);
-  else TSPH_HASH(false, false);
+  WallPass wp{c->cur, c->cell_start, c->scal + c->nwall_idx, c->wall_list, 0, 0,
+               count_density ? c->counters : nullptr};
+  int hg = g, hbs = nb;
+  if (walls) {
+    wp.walls = walls::wall_blocks(c);
+    wp.sweep = count_density ? walls::sweep_blocks(c->n_ub) : 0;
+    hbs = 64;
+    hg = wp.walls + wp.sweep + grid_for(c->n_ub, hbs);
+    c->records_valid = false;
+  }
+#define TSPH_HASH(P, D, W)                                                                     \
+  hipLaunchKernelGGL((k_hash<P, D, W>), dim3(hg), dim3(hbs), 0, s, c->cur.r, c->cur.v,            \
+                     c->cur.type, c->cur.seq, c->scal, k, c->cases, c->cur.cid, dt, nct,          \
+                     c->cell_count, c->hcell, c->hrank, c->seqstate, c->counters, wp)
+#define TSPH_HASH_W(P, D)                                                                      \
+  if (walls == TSPH_WALL_MDBC) TSPH_HASH(P, D, TSPH_WALL_MDBC);                               \
+  else if (walls == TSPH_WALL_BT) TSPH_HASH(P, D, TSPH_WALL_BT);                              \
+  else TSPH_HASH(P, D, 0)
+  if (predict && decide) {
+    TSPH_HASH_W(true, true);
+  } else if (predict) {
+    TSPH_HASH_W(true, false);
+  } else if (decide) {
+    TSPH_HASH_W(false, true);
+  } else {
+    TSPH_HASH_W(false, false);
+  }
+#undef TSPH_HASH_W
 #undef TSPH_HASH
   // the pooled pair pass that follows (if any) has its chunks cut by the
   // tile ordering's last workgroup, or after the ordering for a batch, from

@@ -301,8 +301,13 @@ struct TileGrid {
 // current: a corrector precedes)
 // skip_olds: r_o, v_o, rho_o are not moved (olds_stale afterwards), for a build
 // after which they are dead until the next predictor rewrites them
+// walls (TSPH_WALL_MDBC or TSPH_WALL_BT; 0: none): the wall densities of that
+// treatment are computed first, from the previous cell list, in the same
+// launch as the build's hash (as launch_mdbc / launch_bt_density with
+// count_density)
 int launch_cell_list(tsph_ctx *c, bool eos = false, bool predict = false, double dt = 0.,
-                     unsigned ad_skip = 0, bool decide = false, bool skip_olds = false);
+                     unsigned ad_skip = 0, bool decide = false, bool skip_olds = false,
+                     int walls = 0, bool count_density = false);
 // r_o, v_o, rho_o of the current generation, gathered from the previous one
 // through the last build's permutation if that build left them behind; every
 // launch that reads or moves them calls this first
