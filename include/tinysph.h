@@ -427,6 +427,14 @@ int tsph_autotune(tsph_ctx *ctx, int variant, int *tile_rows, int *waves_per_cu,
  * than 16 tiles per persistent wave, and the ALE passes, always stride.
  * Results do not depend on it.  No counterpart in the reference. */
 int tsph_set_balanced(tsph_ctx *ctx, int balanced);
+/* Per-cell order of the cell-list build (Particle_To_Cell's push_back in
+ * ascending particle index, src/SPH_find_pairs.h:45-92): 1 = ranked by the
+ * pass that moves the particles (each particle counts those of its cell with
+ * a lower index: one launch fewer), 2 = a separate ordering pass over the
+ * cells (fewer bytes moved), 0 = by size (the default: 1 up to 2,000,000
+ * particles, 2 above).  Results do not depend on it.  No counterpart in the
+ * reference. */
+int tsph_set_cell_order(tsph_ctx *ctx, int mode);
 
 /* ---- case setup (host only; no GPU needed) ----
  * Presets built with the reference's geometry routines (src/SPH_create.h,

@@ -86,6 +86,23 @@ def test_batch_fp64_matches_oracle():
         assert rel_err(g["p"], O.get(po.F_P)) <= 1e-6
 
 
+@pytest.mark.parametrize("precision", [T.PREC_FP64, T.PREC_MIXED], ids=["fp64", "mixed"])
+def test_batch_cell_order_modes_agree(precision):
+    """A batch ordered by a pass of its own (the default above 2M particles)
+    and one ranked by the reorder give identical states."""
+    cases = T.build_ensemble(3, SEED + 3, 2000, 6000)
+    outs = []
+    for mode in (1, 2):
+        print(f"[case] batch of 3, cell order {mode}, precision {precision}", flush=True)
+        with T.batch_for_cases(cases, precision) as bs:
+            bs.set_cell_order(mode)
+            bs.particle_to_cell()
+            bs.step(T.WALL_MDBC, 6)
+            outs.append([bs.download_case(q, FIELDS) for q in range(len(cases))])
+    for q in range(len(cases)):
+        assert_same(outs[1][q], outs[0][q], f"case {q}")
+
+
 def test_batch_graph_replay_equals_eager_launches():
     cases = T.build_ensemble(4, SEED + 2, 2000, 8000)
     out = []

@@ -143,6 +143,7 @@ EXPORTS = {
     "tsph_autotune": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                 C.POINTER(C.c_int), _dp, _dp]),
     "tsph_set_balanced": (C.c_int, [C.c_void_p, C.c_int]),
+    "tsph_set_cell_order": (C.c_int, [C.c_void_p, C.c_int]),
     "tsph_write_vtk": (C.c_int, [C.c_void_p, C.c_char_p]),
     "tsph_io_wait": (C.c_int, [C.c_void_p]),
     "tsph_interpolate_grid": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_double,

@@ -370,12 +370,15 @@ static int alloc_context(tsph_ctx *c) {
       (e = dalloc(&c->cell_count, nct + 1)) || (e = dalloc(&c->cell_start, nct + 2)) ||
       (e = dalloc(&c->cell_start_next, nct + 2)) ||
       (e = dalloc(&c->cellof, cap)) || (e = dalloc(&c->hrank, cap)) || (e = dalloc(&c->hcell, cap)) ||
-      (e = dalloc(&c->perm, cap)) ||
+      (e = dalloc(&c->perm, cap)) || (e = dalloc(&c->arrival, cap)) ||
+      (e = dalloc(&c->arrival_seq, cap)) || (e = dalloc(&c->arrival_span, cap)) ||
       (e = dalloc(&c->seqstate, (size_t)c->seq_cap)) || (e = dalloc(&c->probe_out, 4)) ||
       (e = dalloc(&c->bz_list, 2 * kBzMax)) ||
       (e = dalloc(&c->tile_list, nct + 8)) || (e = dalloc(&c->list_cost, nct + 8)) ||
       (e = dalloc(&c->wall_list, cap)) ||
-      (e = dalloc(&c->scan_state, (nct + 1) / 1024 + 2)))
+      (e = dalloc(&c->scan_state, (nct + 1) / 1024 + 2)) ||
+      (e = dalloc(&c->scan_cost, (nct + 1) / 1024 + 2)) ||
+      (e = dalloc(&c->scan_tile0, (nct + 1) / 1024 + 2)))
     return e;
   // the per-cell counters and removal states start cleared; later builds clear
   // them as they consume them (the single-pass scan's words likewise)
@@ -472,7 +475,7 @@ int tsph_destroy(tsph_ctx *c) {
   free_soa(c->cur);
   free_soa(c->alt);
   void *ps[] = {c->scal, c->counters, c->cell_count, c->cell_start, c->cell_start_next, c->cellof, c->hrank,
-                c->hcell, c->perm, c->scan_state, c->seqstate, c->probe_out, c->mx.rv,
+                c->hcell, c->perm, c->arrival, c->arrival_seq, c->arrival_span, c->scan_state, c->scan_cost, c->scan_tile0, c->seqstate, c->probe_out, c->mx.rv,
                 c->mx.rp, c->mx.nrm, c->tile_list, c->list_cost, c->wall_list, c->bz_list,
                 c->cases, c->tile_base, c->list_part, c->tile_bound};
   for (void *p : ps)
@@ -1721,6 +1724,14 @@ int tsph_set_balanced(tsph_ctx *c, int on) {
   bool &b = c->balanced[c->prm.precision == TSPH_PREC_MIXED ? 1 : 0];
   if (b != (on != 0)) drop_graph(c);
   b = on != 0;
+  return 0;
+}
+
+int tsph_set_cell_order(tsph_ctx *c, int mode) {
+  if (!c) return fail(TSPH_EINVAL, "null argument");
+  if (mode < 0 || mode > 2) return fail(TSPH_EINVAL, "cell order must be 0, 1 or 2");
+  if (c->cell_order != mode) drop_graph(c);
+  c->cell_order = mode;
   return 0;
 }
 

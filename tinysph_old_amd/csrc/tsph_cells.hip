@@ -9,21 +9,22 @@
 //                 bucket `ncells`
 //   2. scan       exclusive scan of the per-cell counts -> cell_start (CSR);
 //                 also lists the non-empty pair-loop tiles
-//   3. k_scatter  sorted position = cell_start[cell] + rank
-//   4. k_order    per cell, order its slots by `seq` (reference particle
-//                 order), which makes the result identical to the reference's
-//                 serial push_back in ascending particle index and independent
-//                 of the atomic arrival order; clears the counts
-//   5. k_reorder  gather the SoA fields into (cell, seq) order; also lists the
-//                 wall particles for the wall-density gathers
+//   3. k_scatter  arrival position = cell_start[cell] + rank, with the seq
+//                 of the particle placed there
+//   4. k_reorder  each arrival position ranks its seq among those of its cell
+//                 (reference particle order), which makes the result identical
+//                 to the reference's serial push_back in ascending particle
+//                 index and independent of the atomic arrival order, and moves
+//                 the SoA fields to the ranked slot; also lists the wall
+//                 particles for the wall-density gathers and clears the counts
 //
-// Steps 1-4 are a one-digit counting sort on the cell key (the key range is
+// Steps 1-3 are a one-digit counting sort on the cell key (the key range is
 // the cell count, so a single digit covers it and its scan is the cell-start
-// table the pair loops need anyway); step 4 supplies the stability.
-// All passes read the particle count from device memory, so the sequence can
-// be captured in a HIP graph.  The per-cell counters are cleared by the
-// per-cell pass that follows their scan (k_order), so no separate clearing
-// pass is needed.  The scan also lists the pair loops' non-empty tiles.
+// table the pair loops need anyway); the rank in step 4 supplies the
+// stability.  All passes read the particle count from device memory, so the
+// sequence can be captured in a HIP graph.  The per-cell counters are cleared
+// by the reorder that follows their scan, so no separate clearing pass is
+// needed.  The scan also lists the pair loops' non-empty tiles.
 #include "tsph_internal.h"
 #include "tsph_physics.h"
 #include "tsph_tiles.h"
@@ -170,7 +171,7 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int *lds, int &total)
 // first cell.  A block stages its counts in LDS with the counts of the next
 // kMaxTileRows cells after it, marks the tiles whose first cell is in the
 // block (non-empty: one of their cells holds a particle), and lists them in
-// tile order.  The counts are cleared later, by k_order, so the scan reads
+// tile order.  The counts are cleared later, by the reorder, so the scan reads
 // them intact (the fallback of a stalled block included).
 struct TileScan {
   int ncy, ncells, cell_base;  // the single case (cases == nullptr)
@@ -185,6 +186,9 @@ struct TileScan {
   const int *cost_in;
   int *cost_list;
   CostEstimate cm;  // the estimate for a tile without a recorded cost
+  // with cost_list: per scan block, the listed tiles' summed cost and the
+  // list position of the first (tiles::cut_chunks_scan)
+  int *blk_cost, *blk_tile0;
 };
 
 // the case whose cell range holds cell c (batched grids)
@@ -446,13 +450,26 @@ __global__ __launch_bounds__(kScanBlock) void k_scan(const int *__restrict__ in,
   ex += pc;
   et += pt;
   const int base = blk * kTile + threadIdx.x * IT;
+  int csum = 0;
 #pragma unroll
   for (int j = 0; j < IT; ++j) {
     if (base + j < len) out[base + j] = ex;
     ex += v[j];
     if (tl[j] >= 0) {
-      if (ts.cost_list) ts.cost_list[et] = w[j] > 0 ? w[j] : ts.cm.own * own[j] + ts.cm.tile;
+      if (ts.cost_list) {
+        const int wc = w[j] > 0 ? w[j] : ts.cm.own * own[j] + ts.cm.tile;
+        ts.cost_list[et] = wc;
+        csum += wc;
+      }
       ts.list[et++] = tl[j];
+    }
+  }
+  if (ts.cost_list) {
+    int unused0 = 0, unused1 = 0;
+    block_reduce3(csum, unused0, unused1, lds);
+    if (threadIdx.x == 0) {
+      ts.blk_cost[blk] = csum;
+      ts.blk_tile0[blk] = pt;
     }
   }
   if (threadIdx.x == 0) {
@@ -464,22 +481,58 @@ __global__ __launch_bounds__(kScanBlock) void k_scan(const int *__restrict__ in,
   }
 }
 
-// also marks the reference positions of removed particles (the trailing
+// Arrival position of every particle: cell_start[cell] + its rank from the
+// hash.  RANK (the reorder ranks the cells' particles by seq): with the seq of
+// the particle placed there and the span of its cell (arrival rank | cell
+// count << 16; all ones for a cell of more than 0xffff particles), so that the
+// reorder needs no cell offsets; otherwise k_order / k_order_tiles order
+// `arrival` (then perm) in place.  Also marks the reference positions of removed particles (the trailing
 // bucket) absent in the removal states, and clears k_scan's words for the
-// next build
-__global__ void k_scatter(const int *__restrict__ scal, const int *__restrict__ hcell,
-                          const int *__restrict__ hrank, const int *__restrict__ cell_start,
-                          int *__restrict__ perm, int *__restrict__ cellof, int ncells_tot,
-                          const uint32_t *__restrict__ seq, uint8_t *__restrict__ seqstate,
-                          unsigned long long *__restrict__ scan_state, int scan_blocks) {
+// next build.  CUT: the first `cuts` workgroups (one wavefront per scan block)
+// cut the new tile list into the chunks of the pooled pair pass that follows
+// from the scan's per-block sums (cut_chunks_scan), beside the placement.
+constexpr int kScatterBlock = 256;
+// particles up to which the reorder ranks the cells' particles itself (on the
+// 740k-particle dam break that saves 6 us of a 0.28 ms step; on 4M and 7.8M
+// particles the 12 more bytes per particle it moves cost 2-3% of the step)
+constexpr int kRankMax = 2000000;
+// ordering workgroups (of 256 threads) per compute unit, single case
+constexpr int kOrderWaves = 4;
+template <bool CUT, bool RANK>
+__global__ __launch_bounds__(kScatterBlock) void k_scatter(
+    const int *__restrict__ scal, const int *__restrict__ hcell, const int *__restrict__ hrank,
+    const int *__restrict__ cell_start, int *__restrict__ arrival,
+    uint32_t *__restrict__ arrival_seq, uint32_t *__restrict__ arrival_span,
+    int *__restrict__ cellof, int ncells_tot,
+    const uint32_t *__restrict__ seq, uint8_t *__restrict__ seqstate,
+    unsigned long long *__restrict__ scan_state, int scan_blocks, int groups, int cuts,
+    const int *__restrict__ list_cost, const int *__restrict__ blk_cost,
+    const int *__restrict__ blk_tile0, int *__restrict__ bound) {
+  int blk = blockIdx.x, nblk = gridDim.x;
+  if constexpr (CUT) {
+    __shared__ long long red[kScatterBlock / 64], pfx[kScatterBlock / 64];
+    if (blk < cuts) {
+      tiles::cut_chunks_scan(scal, list_cost, blk_cost, blk_tile0, scan_blocks, groups, bound,
+                             blk, red, pfx);
+      return;
+    }
+    blk -= cuts;
+    nblk -= cuts;
+  }
   const int n = scal[S_NP];
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  for (int b = i; b < scan_blocks; b += gridDim.x * blockDim.x) scan_state[b] = 0;
+  const int i = blk * blockDim.x + threadIdx.x;
+  for (int b = i; b < scan_blocks; b += nblk * blockDim.x) scan_state[b] = 0;
   if (i >= n) return;
   const int c = hcell[i];
   if (c == ncells_tot) seqstate[seq[i]] = 0;
-  const int pos = cell_start[c] + hrank[i];
-  perm[pos] = i;
+  const int s0 = cell_start[c], rank = hrank[i];
+  const int pos = s0 + rank;
+  arrival[pos] = i;
+  if (RANK) {
+    const int cnt = cell_start[c + 1] - s0;
+    arrival_seq[pos] = seq[i];
+    arrival_span[pos] = cnt <= 0xffff ? (uint32_t)rank | (uint32_t)cnt << 16 : 0xffffffffu;
+  }
   cellof[pos] = c;
 }
 
@@ -539,19 +592,11 @@ __device__ __forceinline__ void order_cell(const int *__restrict__ cell_start, i
   }
 }
 
-// (batched grids) groups > 0: the first `cuts` workgroups cut the new list
-// into the chunks of the pooled pair pass that follows (cut_chunks_block)
-__global__ __launch_bounds__(tiles::kCutBlock) void k_order(
-    const int *__restrict__ cell_start, int ncells, const uint32_t *__restrict__ seq,
-    int *__restrict__ perm, int *__restrict__ cell_count, const int *__restrict__ scal, int groups,
-    int cuts, const int *__restrict__ list_cost, int *__restrict__ bound) {
-  __shared__ int wl[tiles::kCutSeg];
-  __shared__ long long red[tiles::kCutBlock / 64];
-  if ((int)blockIdx.x < cuts) {
-    tiles::cut_chunks_block(scal, list_cost, groups, bound, blockIdx.x, wl, red);
-    return;
-  }
-  const int c = (blockIdx.x - cuts) * blockDim.x + threadIdx.x;
+// per cell of a batched grid: its slots ordered by seq, its count cleared
+__global__ void k_order(const int *__restrict__ cell_start, int ncells,
+                        const uint32_t *__restrict__ seq, int *__restrict__ perm,
+                        int *__restrict__ cell_count) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c <= ncells) cell_count[c] = 0;
   if (c < ncells) order_cell(cell_start, c, seq, perm);
 }
@@ -559,26 +604,17 @@ __global__ __launch_bounds__(tiles::kCutBlock) void k_order(
 // The same over the cells of the listed (non-empty) tiles only, for a single
 // case: every occupied cell lies in a listed tile, so the other cells need
 // neither ordering nor clearing (their counts are 0).  A bounded grid strides
-// over the (tile, row) items.  groups > 0: the first `cuts` workgroups cut
-// the new list into the chunks of the pooled pair pass that follows, beside
-// the ordering (cut_chunks_block).
-__global__ __launch_bounds__(tiles::kCutBlock) void k_order_tiles(
-    const int *__restrict__ cell_start, const int *__restrict__ scal,
-    const int *__restrict__ tile_list, int rows, int ncy, int ncells,
-    const uint32_t *__restrict__ seq, int *__restrict__ perm, int *__restrict__ cell_count,
-    int groups, int cuts, const int *__restrict__ list_cost, int *__restrict__ bound) {
-  __shared__ int wl[tiles::kCutSeg];
-  __shared__ long long red[tiles::kCutBlock / 64];
-  if ((int)blockIdx.x < cuts) {
-    tiles::cut_chunks_block(scal, list_cost, groups, bound, blockIdx.x, wl, red);
-    return;
-  }
-  const int ob = blockIdx.x - cuts, nord = gridDim.x - cuts;  // ordering workgroups
+// over the (tile, row) items.
+__global__ void k_order_tiles(const int *__restrict__ cell_start, const int *__restrict__ scal,
+                              const int *__restrict__ tile_list, int rows, int ncy, int ncells,
+                              const uint32_t *__restrict__ seq, int *__restrict__ perm,
+                              int *__restrict__ cell_count) {
   const int ntiles = scal[S_NTILES];
-  if (ob == 0 && threadIdx.x == 0) cell_count[ncells] = 0;  // the removed bucket
+  if (blockIdx.x == 0 && threadIdx.x == 0) cell_count[ncells] = 0;  // the removed bucket
   const int items = ntiles * rows;
   const int nty = (ncy + rows - 1) / rows;
-  for (int item = ob * blockDim.x + threadIdx.x; item < items; item += nord * blockDim.x) {
+  for (int item = blockIdx.x * blockDim.x + threadIdx.x; item < items;
+       item += gridDim.x * blockDim.x) {
     const int it = item / rows, r = item - it * rows;
     const int tile = tile_list[it];
     const int cx = tile / nty, cy = (tile - cx * nty) * rows + r;
@@ -589,9 +625,47 @@ __global__ __launch_bounds__(tiles::kCutBlock) void k_order_tiles(
   }
 }
 
-// Gathers every field into sorted order (the state after the build is the
-// reference's state, permuted), lists the wall particles, and sets the new
-// particle count (removed particles sit after n_new and are dropped).
+// slot of arrival position x in (cell, seq) order: the cell's first slot plus
+// the number of the cell's particles that precede x's in seq (ties, which
+// distinct seqs never give, broken by arrival position so that every arrival
+// position of the cell gets a slot of its own).  The cell's keys are read
+// kRankBatch at a time, all of them in flight together (one round for all
+// but the fullest cells).
+constexpr int kRankBatch = 16;
+__device__ __forceinline__ int ranked_slot(const int *__restrict__ cell_start,
+                                           const int *__restrict__ cellof,
+                                           const uint32_t *__restrict__ span,
+                                           const uint32_t *__restrict__ key, int x) {
+  const uint32_t sp = span[x];
+  int s, e;
+  if (sp != 0xffffffffu) {
+    s = x - (int)(sp & 0xffffu);
+    e = s + (int)(sp >> 16);
+  } else {
+    const int c = cellof[x];
+    s = cell_start[c];
+    e = cell_start[c + 1];
+  }
+  const uint32_t mine = key[x];
+  int slot = s;
+  for (int a0 = s; a0 < e; a0 += kRankBatch) {
+    uint32_t kv[kRankBatch];
+#pragma unroll
+    for (int u = 0; u < kRankBatch; ++u) kv[u] = a0 + u < e ? key[a0 + u] : 0u;
+#pragma unroll
+    for (int u = 0; u < kRankBatch; ++u) {
+      const int a = a0 + u;
+      slot += a < e && (kv[u] < mine || (kv[u] == mine && a < x));
+    }
+  }
+  return slot;
+}
+
+// Moves every field from its arrival position to its ranked slot (the state
+// after the build is the reference's state, permuted; perm records the
+// permutation), lists the wall particles, clears the cell counts for the next
+// build and sets the new particle count (removed particles sit after n_new and
+// are dropped).
 // With EOS the pressure is recomputed as Density_to_pressure does
 // (src/SPH_pressure.h:65-89) and the FP32 records of the mixed path are
 // written, for the step sequences in which that stage follows the cell build.
@@ -600,20 +674,32 @@ __global__ __launch_bounds__(tiles::kCutBlock) void k_order_tiles(
 // immediately precedes the cell build.  ad_skip (a type mask): a and drho of
 // those types are not moved, for the step sequences in which a pair pass that
 // overwrites them for exactly those types follows the build.
-template <bool EOS, bool PRED>
-__global__ void k_reorder(const int *__restrict__ cell_start, int ncells, const int *__restrict__ perm,
+template <bool EOS, bool PRED, bool RANK>
+__global__ void k_reorder(const int *__restrict__ cell_start, int ncells,
+                          const int *__restrict__ arrival, const uint32_t *__restrict__ arrival_seq,
+                          const uint32_t *__restrict__ arrival_span, int *__restrict__ perm,
+                          int *__restrict__ cell_count,
                           Soa src, Soa dst, int *__restrict__ scal, int *__restrict__ wall_list,
                           const int *__restrict__ cellof, Consts k0,
                           const Consts *__restrict__ cases, Mixed mx, double dt0,
                           unsigned ad_skip, unsigned long long *__restrict__ counters,
                           int nwall_idx, bool move_olds) {
   const int n_new = cell_start[ncells];
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (blockIdx.x == 0 && threadIdx.x == 0) scal[S_NP] = n_new;
-  int wall = 0;
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;  // arrival position
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    scal[S_NP] = n_new;
+    if (RANK) cell_count[ncells] = 0;  // the removed bucket
+  }
+  int wall = 0, k = x;
   unsigned bad = 0;
-  if (k < n_new) {
-    const int j = perm[k];
+  if (x < n_new) {
+    const int j = arrival[x];
+    const int cell = RANK || (EOS && mx.rv) ? cellof[x] : 0;
+    if (RANK) {
+      k = ranked_slot(cell_start, cellof, arrival_span, arrival_seq, x);
+      cell_count[cell] = 0;
+      perm[k] = j;
+    }
     const uint8_t t = src.type[j];
     wall = t == TSPH_WALL;
     double2 r = src.r[j], v = src.v[j];
@@ -664,7 +750,7 @@ __global__ void k_reorder(const int *__restrict__ cell_start, int ncells, const 
       dst.n[k] = nv;
     }
     if (EOS && mx.rv)
-      write_mixed_record(mx, kc, cellof[k] - kc.cell_base, k, r, v, rho, p, src.n != nullptr, nv);
+      write_mixed_record(mx, kc, cell - kc.cell_base, k, r, v, rho, p, src.n != nullptr, nv);
     if (src.omega) {
       dst.omega[k] = src.omega[j];
       dst.omegarho[k] = src.omegarho[j];
@@ -726,7 +812,8 @@ int scan_cells(tsph_ctx *c, hipStream_t s, int *cs, int nwall_idx, int cost_tabl
                     c->tile_rows, c->tile_list,
                     cost_table >= 0 ? c->scal + S_NSCAL + cost_table : nullptr,
                     cost_table >= 0 ? c->list_cost : nullptr,
-                    cost_estimate(c->prm.precision == TSPH_PREC_MIXED)};
+                    cost_estimate(c->prm.precision == TSPH_PREC_MIXED), c->scan_cost,
+                    c->scan_tile0};
   if (scan_items(c) == 4)
     hipLaunchKernelGGL(k_scan<4>, dim3(scan_blocks(c)), dim3(kScanBlock), 0, s, c->cell_count,
                        len, c->scan_state, cs, c->scal, c->counters, ts, c->scan_wait, nwall_idx);
@@ -794,9 +881,9 @@ int launch_cell_list(tsph_ctx *c, bool eos, bool predict, double dt, unsigned ad
   }
 #undef TSPH_HASH_W
 #undef TSPH_HASH
-  // the pooled pair pass that follows (if any) has its chunks cut by the
-  // tile ordering's last workgroup, or after the ordering for a batch, from
-  // the costs the scan lists beside the tiles
+  // the pooled pair pass that follows (if any) has its chunks cut in the
+  // first workgroups of the scatter, from the costs the scan lists beside the
+  // tiles and its per-block sums of them
   const int cut = c->cut_groups;
   c->cut_groups = 0;
   if (cut > c->bound_cap) {
@@ -804,41 +891,56 @@ int launch_cell_list(tsph_ctx *c, bool eos, bool predict, double dt, unsigned ad
     return TSPH_ESTATE;
   }
   if (int e = scan_cells(c, s, cs, wslot, cut > 0 ? c->cut_table : -1)) return e;
-  hipLaunchKernelGGL(k_scatter, dim3(g), dim3(nb), 0, s, c->scal, c->hcell, c->hrank,
-                     cs, c->perm, c->cellof, nct, c->cur.seq, c->seqstate,
-                     c->scan_state, scan_blocks(c));
-  if (c->cases) {
-    const int bs = cut > 0 ? tiles::kCutBlock : nb;
-    const int cuts = cut > 0 ? tiles::cut_segments(total_tiles(c)) : 0;
-    hipLaunchKernelGGL(k_order, dim3(cuts + grid_for(nct + 1, bs)), dim3(bs), 0, s, cs, nct,
-                       c->cur.seq, c->perm, c->cell_count, c->scal, cut, cuts, c->list_cost,
-                       c->tile_bound);
-    if (cut > 0) {
-      c->list_groups = cut;
-      c->list_table = c->cut_table;
-    }
+  const int sg = grid_for(c->n_ub, kScatterBlock);
+  // the per-cell order: ranked by the reorder (one launch fewer), or by its
+  // own pass (fewer bytes moved); by size, up to kRankMax particles ranked
+  const bool rank = c->cell_order == 1 || (c->cell_order == 0 && c->n_ub <= kRankMax);
+  int *arrival = rank ? c->arrival : c->perm;
+#define TSPH_SCATTER(CUT, RANK, GRID, CUTS, GROUPS)                                            \
+  hipLaunchKernelGGL((k_scatter<CUT, RANK>), dim3(GRID), dim3(kScatterBlock), 0, s, c->scal,       \
+                     c->hcell, c->hrank, cs, arrival, c->arrival_seq, c->arrival_span, c->cellof,   \
+                     nct, c->cur.seq, c->seqstate, c->scan_state, scan_blocks(c), GROUPS, CUTS,     \
+                     c->list_cost, c->scan_cost, c->scan_tile0, c->tile_bound)
+  if (cut > 0) {
+    // one cutting wavefront per scan block, in the first workgroups
+    const int cuts = (scan_blocks(c) + kScatterBlock / 64 - 1) / (kScatterBlock / 64);
+    if (rank) TSPH_SCATTER(true, true, cuts + sg, cuts, cut);
+    else TSPH_SCATTER(true, false, cuts + sg, cuts, cut);
+    c->list_groups = cut;
+    c->list_table = c->cut_table;
   } else {
-    // with a cut, one workgroup of kCutBlock threads per compute unit and the
-    // cutting one; without, two of nb (the faster ordering on its own)
-    const int nord = cut > 0 ? std::max(1, c->num_cu) : 2 * std::max(1, c->num_cu);
-    const int cuts = cut > 0 ? tiles::cut_segments(total_tiles(c)) : 0;
-    hipLaunchKernelGGL(k_order_tiles, dim3(cuts + nord), dim3(cut > 0 ? tiles::kCutBlock : nb), 0,
-                       s, cs, c->scal, c->tile_list, c->tile_rows, k.ncy, nct, c->cur.seq,
-                       c->perm, c->cell_count, cut, cuts, c->list_cost, c->tile_bound);
-    if (cut > 0) {
-      c->list_groups = cut;
-      c->list_table = c->cut_table;
-    }
+    if (rank) TSPH_SCATTER(false, true, sg, 0, 0);
+    else TSPH_SCATTER(false, false, sg, 0, 0);
+  }
+#undef TSPH_SCATTER
+  if (!rank) {
+    if (c->cases)
+      hipLaunchKernelGGL(k_order, dim3(grid_for(nct + 1, nb)), dim3(nb), 0, s, cs, nct, c->cur.seq,
+                         c->perm, c->cell_count);
+    else
+      hipLaunchKernelGGL(k_order_tiles, dim3(kOrderWaves * std::max(1, c->num_cu)), dim3(nb), 0, s,
+                         cs, c->scal, c->tile_list, c->tile_rows, k.ncy, nct, c->cur.seq, c->perm,
+                         c->cell_count);
   }
   const Mixed mx = c->prm.precision == TSPH_PREC_MIXED ? c->mx : Mixed{};
-#define TSPH_REORDER(E, P)                                                                    \
-  hipLaunchKernelGGL((k_reorder<E, P>), dim3(g), dim3(nb), 0, s, cs, nct, c->perm,                 \
-                     c->cur, c->alt, c->scal, c->wall_list, c->cellof, k, c->cases, mx, dt,       \
-                     ad_skip, c->counters, wslot, !skip_olds)
-  if (eos && predict) TSPH_REORDER(true, true);
-  else if (eos) TSPH_REORDER(true, false);
-  else if (predict) TSPH_REORDER(false, true);
-  else TSPH_REORDER(false, false);
+#define TSPH_REORDER(E, P, R)                                                                 \
+  hipLaunchKernelGGL((k_reorder<E, P, R>), dim3(g), dim3(nb), 0, s, cs, nct, arrival,              \
+                     c->arrival_seq, c->arrival_span, c->perm, c->cell_count, c->cur, c->alt,     \
+                     c->scal, c->wall_list, c->cellof, k, c->cases, mx, dt, ad_skip, c->counters, \
+                     wslot, !skip_olds)
+#define TSPH_REORDER_R(E, P)                                                                  \
+  if (rank) TSPH_REORDER(E, P, true);                                                         \
+  else TSPH_REORDER(E, P, false)
+  if (eos && predict) {
+    TSPH_REORDER_R(true, true);
+  } else if (eos) {
+    TSPH_REORDER_R(true, false);
+  } else if (predict) {
+    TSPH_REORDER_R(false, true);
+  } else {
+    TSPH_REORDER_R(false, false);
+  }
+#undef TSPH_REORDER_R
 #undef TSPH_REORDER
   TSPH_HIP(hipGetLastError());
   std::swap(c->cur, c->alt);

@@ -172,6 +172,7 @@ struct tsph_ctx {
   // chunks per workgroup, tiles taken from an LDS counter by the workgroup's
   // waves) or the plain static walk; defaults set at creation
   bool balanced[2] = {false, false};
+  int cell_order = 0;  // tsph_set_cell_order: 0 by size, 1 ranked, 2 separate pass
   int *tile_list = nullptr;  // non-empty tiles of the current cell list, in tile order
   bool tiles_valid = false;  // tile_list matches the cell list and tile_rows
   int *list_part = nullptr;  // per list-build block: tiles, work
@@ -216,9 +217,16 @@ struct tsph_ctx {
   int *hrank = nullptr;                // rank inside the cell from the hash pass
   int *hcell = nullptr;                // cell from the hash pass (pre-sort order)
   int *perm = nullptr;                 // sorted position -> old slot
+  // the scatter's placement (cell, then arrival order) and the seq of each
+  // placed particle, which the reorder ranks within the cell
+  int *arrival = nullptr;              // arrival position -> old slot
+  uint32_t *arrival_seq = nullptr;     // arrival position -> seq
+  uint32_t *arrival_span = nullptr;    // arrival rank | cell count << 16
   // cell-list scan: one published word per block (zero: not yet published),
   // and the polls a block makes for an earlier block's word (-1: none)
   unsigned long long *scan_state = nullptr;
+  // per scan block: the listed tiles' summed cost and first list position
+  int *scan_cost = nullptr, *scan_tile0 = nullptr;
   int scan_wait = 1 << 20;
   uint8_t *seqstate = nullptr;         // removal: per-seq state
   double *probe_out = nullptr;         // 4 doubles for probe results

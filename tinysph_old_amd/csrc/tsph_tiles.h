@@ -372,6 +372,61 @@ __device__ __forceinline__ void cut_chunks_block(const int *__restrict__ scal,
   }
 }
 
+// The same cut right after the cell-list scan that listed the tiles, from its
+// per-block sums: the listed tiles of scan block b sit at list positions
+// [tile0[b], tile0[b + 1]) (the last block's end: the list length) and cost
+// bsum[b] together.  Workgroup wg (blockDim.x threads, one wavefront per scan
+// block, nb blocks) forms the costs before its blocks and the total from the
+// nb block sums; each wavefront then reads its block's tile costs 64 at a
+// time, scans them across the lanes, and writes the starts of the chunks that
+// begin at its tiles.  red, pfx: blockDim.x / 64 long longs of LDS each.
+__device__ __forceinline__ void cut_chunks_scan(const int *__restrict__ scal,
+                                                const int *__restrict__ list_cost,
+                                                const int *__restrict__ bsum,
+                                                const int *__restrict__ tile0, int nb, int groups,
+                                                int *__restrict__ bound, int wg, long long *red,
+                                                long long *pfx) {
+  const int n = scal[S_NTILES];
+  const int nw = blockDim.x >> 6, wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (wg == 0) {
+    if (threadIdx.x == 0) bound[groups] = n;
+    if (n == 0)
+      for (int g = threadIdx.x; g < groups; g += blockDim.x) bound[g] = 0;
+  }
+  if (n == 0 || wg * nw >= nb) return;
+  long long carry = 0;
+  for (int r0 = 0; r0 < nb; r0 += blockDim.x) {
+    const int q = r0 + threadIdx.x;
+    long long sum = 0;
+    const long long ex = block_exscan<long long>(q < nb ? bsum[q] : 0, sum, red) + carry;
+    const int qw = q - wg * nw;
+    if (qw >= 0 && qw < nw) pfx[qw] = ex;
+    carry += sum;
+  }
+  __syncthreads();
+  const long long total = carry;
+  const int b = wg * nw + wid;
+  if (b >= nb || total <= 0) return;
+  long long p = pfx[wid];
+  const int t1 = b + 1 < nb ? tile0[b + 1] : n;
+  for (int e0 = tile0[b]; e0 < t1; e0 += 64) {
+    const int e = e0 + lane;
+    const long long w = e < t1 ? list_cost[e] : 0;
+    long long x = w;
+    for (int o = 1; o < 64; o <<= 1) {
+      const long long y = __shfl_up(x, o, 64);
+      if (lane >= o) x += y;
+    }
+    if (e < t1) {
+      const long long w0 = p + x - w;  // the costs before tile e
+      long long g = (w0 * groups + total - 1) / total;
+      const long long end = (w0 + w) * groups;
+      for (; g < groups && g * total < end; ++g) bound[g] = e;
+    }
+    p += __shfl(x, 63, 64);
+  }
+}
+
 __device__ __forceinline__ int slice_of(const Tile &t, int l) {
   return l >= t.lq[2] ? 2 : (l >= t.lq[1] ? 1 : 0);
 }

@@ -449,6 +449,10 @@ class ParticleSystem:
     def set_balanced(self, on: bool):
         L.check(self.lib.tsph_set_balanced(self.ctx, int(bool(on))), "tsph_set_balanced")
 
+    def set_cell_order(self, mode: int):
+        """0: by size (default), 1: ranked by the reorder, 2: separate ordering pass."""
+        L.check(self.lib.tsph_set_cell_order(self.ctx, mode), "tsph_set_cell_order")
+
     def set_waves_per_cu(self, waves: int):
         L.check(self.lib.tsph_set_waves_per_cu(self.ctx, waves), "tsph_set_waves_per_cu")
 
