@@ -206,8 +206,8 @@ def stage_step(ps, wall, dt):
 @pytest.mark.parametrize("name", list(CASES))
 @pytest.mark.parametrize("precision", [T.PREC_FP64, T.PREC_MIXED], ids=["fp64", "mixed"])
 def test_graph_replay_equals_eager(name, precision):
-    """The captured step (wall densities on a second stream beside the first
-    cell build's sort; on the mixed path the corrector inside the first pair
+    """The captured step (wall densities in workgroups of the first cell
+    build's hash launch; on the mixed path the corrector inside the first pair
     pass) equals the stages launched one by one, bit for bit."""
     case = make(name)
     with T.system_for_case(case, precision) as a, T.system_for_case(case, precision) as b:
