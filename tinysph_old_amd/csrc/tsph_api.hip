@@ -243,10 +243,16 @@ static int record_step(tsph_ctx *c, int wall, double dt, int inlet = -1, int out
   // buffer zones the pass runs in workgroups of the first build's hash launch,
   // beside the hash (on a second stream the fork and the join in the step
   // graph left about 25 us of idle time per step, more than the pass takes).
+  // With buffer zones the pass runs in a launch of its own, before the
+  // predictor (the zone passes act on predicted positions, so the build cannot
+  // apply the predictor); its EOS is still only the out-of-range density
+  // count, taken by the pass's sweep: the zone passes read no pressure, and the
+  // build's EOS rewrites every pressure that EOS would write (inlet and outlet
+  // particles keep theirs either way).
   const bool walls_in_build = !zones && (wall == TSPH_WALL_MDBC || wall == TSPH_WALL_BT);
   if (zones && (wall == TSPH_WALL_MDBC || wall == TSPH_WALL_BT)) {
-    if ((e = wall == TSPH_WALL_MDBC ? launch_mdbc(c) : launch_bt_density(c))) return e;
-    if ((e = launch_eos(c))) return e;
+    if ((e = wall == TSPH_WALL_MDBC ? launch_mdbc(c, true) : launch_bt_density(c, true)))
+      return e;
   }
   // each build cuts the chunks of the pooled pair pass that follows it
   const bool mixed = c->prm.precision == TSPH_PREC_MIXED;
