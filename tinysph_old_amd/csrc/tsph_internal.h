@@ -31,7 +31,8 @@ constexpr int kMaxTileRows = 24;  // upper bound of tile_rows
 // clock, averaged with the tile's previous record) in one of kCostTables
 // tables, at scal[S_NSCAL + kCostTables t + table] for global tile t; the
 // next pooled pass of the same table cuts its chunks by these costs
-// (cut_chunks_block).  A recorded time carries what a model of the tile's
+// (cut_chunks_scan in a cell-list build, cut_chunks_block for a list reused
+// without one).  A recorded time carries what a model of the tile's
 // work would not: what else ran on its compute unit, so that the chunks of a
 // compute unit that ran slow shrink in the next pass.  Table 1 holds the mixed
 // pair passes that apply the symplectic corrector as well, table 0 every

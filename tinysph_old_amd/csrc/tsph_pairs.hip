@@ -357,8 +357,8 @@ __global__ void k_list_costs(const int *__restrict__ cell_start, Consts k0,
   }
 }
 
-// the chunk cut of a pooled walk as a launch of its own (cut_chunks_block,
-// one workgroup per segment)
+// the chunk cut of a pooled walk as a launch of its own, for a tile list that
+// its cell-list build did not cut (cut_chunks_block, one workgroup per segment)
 __global__ __launch_bounds__(kCutBlock) void k_cut_chunks(const int *__restrict__ scal,
                                                           const int *__restrict__ list_cost,
                                                           int groups, int *__restrict__ bound) {
